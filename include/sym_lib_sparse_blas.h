@@ -1,0 +1,62 @@
+/*
+ * sym_lib_sparse_blas.h -- the reference's sparse triangular solve entry
+ * points, re-exported by libsympiler_amd.so with the same C++ signatures
+ * (namespace sym_lib, C++ linkage, same mangled names), so code written
+ * against include/sympiler/sparse_blas_lib.h relinks unchanged.
+ *
+ * Semantics follow the reference: blocking, stateless, host arrays owned by
+ * the caller, x read as the right-hand side and overwritten with the
+ * solution.  Underneath, each call copies the arrays to the GPU, runs the
+ * HIP kernels through the C ABI of sympiler_amd.h and copies x back.  The
+ * solve and SpMV results are bit-identical to the reference's serial
+ * routines (one lane per row, no FMA contraction).  Where the reference
+ * returns void, a failure (bad structure, zero pivot, HIP error) throws
+ * std::runtime_error with the library's message.
+ */
+#ifndef SYM_LIB_SPARSE_BLAS_H
+#define SYM_LIB_SPARSE_BLAS_H
+#ifdef __cplusplus
+namespace sym_lib {
+
+/* sparse_blas_lib.h:18-20 */
+void spmv_csr(int n, const int *Ap, const int *Ai, const double *Ax,
+              const double *x, double *y);
+/* sparse_blas_lib.h:32-33 */
+void spmv_csr(int n, const int *Ap, const int *Ai, const double *Ax,
+              const double *x, const double *y, double *z, double a, double b);
+
+/* sparse_blas_lib.h:126-127 */
+void sptrsv_csr(int n, int *Lp, int *Li, double *Lx, double *x);
+/* sparse_blas_lib.h:145-147; the level arrays are checked for size and
+ * validity; the GPU runs its own level schedule (results are identical for
+ * every valid schedule). */
+void sptrsv_csr_levelset(int n, const int *Lp, const int *Li, const double *Lx,
+                         double *x, int levels, const int *levelPtr,
+                         const int *levelSet);
+/* sparse_blas_lib.h:150-152; executes the given coarsened schedule. */
+void sptrsv_csr_lbc(int n, int *Lp, int *Li, double *Lx, double *x,
+                    int level_no, int *level_ptr, int *par_ptr,
+                    int *partition);
+/* sparse_blas_lib.h:169-170 (CSC of L, diagonal first) */
+void sptrsv_csc(int n, int *Lp, int *Li, double *Lx, double *x);
+/* sparse_blas_lib.h:174-175 */
+void sptrsv_csc_levelset(int n, int *Lp, int *Li, double *Lx, double *x,
+                         int levels, const int *levelptr, const int *levelset);
+/* sparse_blas_lib.h:177-178 */
+void sptrsv_csc_lbc(int n, int *Lp, int *Li, double *Lx, double *x,
+                    int level_no, int *level_ptr, int *par_ptr,
+                    int *partition);
+/* sparse_blas_lib.h:323-330: L^T x = b from the CSC of L; returns 0 on
+ * null input, 1 otherwise. */
+int ltsolve(int n, int *Lp, int *Li, double *Lx, double *x);
+
+/* The level-set inspector of the un-vendored aggregation submodule, with the
+ * signature its call sites use (Catch_tests/sptrsv_tests.cpp:59,
+ * demo/sptrsv_demo_utils.h:44-45): input the CSC of L, outputs new[]'d for
+ * the caller to delete[], returns the number of levels. */
+int build_levelSet_CSC(int n, const int *Lp, const int *Li, int *&level_ptr,
+                       int *&level_set);
+
+}  // namespace sym_lib
+#endif
+#endif

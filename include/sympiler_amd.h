@@ -1,0 +1,190 @@
+/*
+ * sympiler_amd.h -- C ABI of the MI355X (gfx950) sparse lower-triangular
+ * solve library libsympiler_amd.so.
+ *
+ * Everything here is `extern "C"`, plain pointers and sizes.  Matrix and
+ * vector arguments named d_* are DEVICE pointers (hipMalloc'd or torch CUDA
+ * tensors); h_* are host pointers.  Streams are hipStream_t passed as void*.
+ *
+ * Matrix layout is the reference's (include/sympiler/sparse_blas_lib.h):
+ *   CSR lower triangle L: Lp[n+1], Li[nnz], Lx[nnz]; column indices
+ *   ascending, diagonal LAST in each row (sparse_blas/sptrsv.cpp:20,23);
+ *   int32 indices, FP64 (or FP32) values.
+ *
+ * The reference's own blocking, stateless C++ entry points
+ * (namespace sym_lib, C++ linkage) are re-exported by the same library on
+ * top of this ABI; their declarations are in sym_lib_sparse_blas.h.
+ *
+ * Errors: every call returns a symamd_status; symamd_last_error() returns a
+ * thread-local message for the last failing call.  A handle is bound to one
+ * stream and is not thread-safe; separate handles are independent.
+ */
+#ifndef SYMPILER_AMD_H
+#define SYMPILER_AMD_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  SYMAMD_OK = 0,
+  SYMAMD_ERR_INVALID_ARG = 1,   /* null pointer, negative size, bad enum   */
+  SYMAMD_ERR_STRUCTURE = 2,     /* not lower / diagonal not last / unsorted */
+  SYMAMD_ERR_ZERO_PIVOT = 3,    /* a zero diagonal entry                    */
+  SYMAMD_ERR_HIP = 4,           /* a HIP runtime call failed                */
+  SYMAMD_ERR_ALLOC = 5,         /* device or host allocation failed         */
+  SYMAMD_ERR_TIMEOUT = 6,       /* a sync-free wait exceeded its watchdog   */
+  SYMAMD_ERR_NOT_SUPPORTED = 7
+} symamd_status;
+
+typedef enum { SYMAMD_F64 = 0, SYMAMD_F32 = 1 } symamd_dtype;
+
+/* Dense right-hand-side layouts: element (i, c) of an n-by-k block lives at
+ *   COL_MAJOR: B[i + c*ld]   (ld >= n; the reference's multi-RHS convention,
+ *                             parsy Triangular_BCSC.cpp:114)
+ *   ROW_MAJOR: B[i*ld + c]   (ld >= k; right-hand sides interleaved)     */
+typedef enum { SYMAMD_COL_MAJOR = 0, SYMAMD_ROW_MAJOR = 1 } symamd_layout;
+
+/* Executors for the forward (and adjoint) solve. */
+typedef enum {
+  SYMAMD_EXEC_AUTO = 0,      /* tuned choice if symamd_tune ran, else heuristic */
+  SYMAMD_EXEC_LEVELSET = 1,  /* one kernel launch per level (plain level set)   */
+  SYMAMD_EXEC_SYNCFREE = 2,  /* one persistent launch, per-row ready flags      */
+  SYMAMD_EXEC_LEVELSET_GRAPH = 3 /* the per-level launches replayed as a hipGraph */
+} symamd_executor;
+
+typedef struct symamd_handle_s *symamd_handle;
+typedef struct symamd_plan_s *symamd_plan;
+typedef struct symamd_graph_s *symamd_graph;
+
+/* ---------------------------------------------------------------- handle */
+int symamd_create(symamd_handle *h, void *stream);
+int symamd_destroy(symamd_handle h);
+int symamd_set_stream(symamd_handle h, void *stream);
+/* Sync-free watchdog: a wait longer than `ms` milliseconds aborts the solve
+ * with SYMAMD_ERR_TIMEOUT (default 20000). */
+int symamd_set_timeout_ms(symamd_handle h, double ms);
+const char *symamd_last_error(void);
+/* Library version string. */
+const char *symamd_version(void);
+
+/* ------------------------------------------------------------- inspector */
+/* Level sets of the dependency DAG of a device CSR lower triangle, computed
+ * on the GPU.  level(i) = 0 if row i has no off-diagonal, else
+ * 1 + max level(j) over L[i,j] != 0; d_level_set lists rows level by level,
+ * ascending row id inside a level.  d_level_ptr needs n+1 entries.
+ * Replaces the un-vendored build_levelSet_CSC (call sites
+ * Catch_tests/sptrsv_tests.cpp:59, demo/sptrsv_demo_utils.h:44-45), whose
+ * input is the CSC of L; this entry point takes the CSR of the same L. */
+int symamd_levelset(symamd_handle h, int n, const int *d_Lp, const int *d_Li,
+                    int *d_level_ptr, int *d_level_set, int *levels);
+
+/* Coarsened 3-array schedule (level_ptr / par_ptr / partition), the format
+ * sym_lib::sptrsv_csr_lbc consumes (sparse_blas/sptrsv.cpp:57-75): `span`
+ * consecutive levels form one coarse level; inside it the rows are split
+ * into partitions closed under dependencies (every dependency of a row in a
+ * partition is in an earlier coarse level or earlier in the same
+ * partition), merged greedily up to `max_part_cost` (cost = row nnz).
+ * Output arrays are device arrays of size n+1 (level_ptr, par_ptr) and n
+ * (partition).  Replaces the un-vendored get_coarse_levelSet_DAG_CSC_tree
+ * (call site demo/sptrsv_demo_utils.h:83-88). */
+int symamd_coarsen(symamd_handle h, int n, const int *d_Lp, const int *d_Li,
+                   int span, int max_part_cost, int *d_level_ptr,
+                   int *d_par_ptr, int *d_partition, int *level_no,
+                   int *part_no);
+
+/* Execute a caller-given coarsened schedule on a device CSR of L (FP64), in
+ * place on d_x, with the semantics of sym_lib::sptrsv_csr_lbc
+ * (sparse_blas/sptrsv.cpp:57-75): coarse levels in order, partitions of a
+ * coarse level in parallel, rows of a partition in listed order.
+ * h_level_ptr is a HOST array (level_no+1); d_par_ptr and d_partition are
+ * device arrays. */
+int symamd_solve_lbc(symamd_handle h, int n, const int *d_Lp, const int *d_Li,
+                     const double *d_Lx, double *d_x, int level_no,
+                     const int *h_level_ptr, const int *d_par_ptr,
+                     const int *d_partition);
+
+/* ------------------------------------------------------------------ plan */
+/* Analyse L (device CSR, diagonal last) once: level sets, a level-ordered
+ * copy of L, work units and ready flags.  The plan owns its device memory;
+ * d_Lp/d_Li/d_Lx may be freed after the call. */
+int symamd_analyze(symamd_handle h, int n, const int *d_Lp, const int *d_Li,
+                   const void *d_Lx, symamd_dtype dtype, symamd_plan *plan);
+/* Analyse an uneven batch of independent systems given as one concatenated
+ * CSR: system s owns rows h_row_off[s] .. h_row_off[s+1]-1; d_Li holds
+ * column ids LOCAL to each system.  The plan solves all of them in one
+ * launch (block-diagonal union). */
+int symamd_analyze_batched(symamd_handle h, int nsys, const int *h_row_off,
+                           const int *d_Lp, const int *d_Li, const void *d_Lx,
+                           symamd_dtype dtype, symamd_plan *plan);
+/* Refresh the plan's copy of the values (same pattern). */
+int symamd_plan_update_values(symamd_handle h, symamd_plan plan,
+                              const void *d_Lx);
+/* n, nnz, number of levels, max rows in one level. */
+int symamd_plan_info(symamd_plan plan, int *n, int64_t *nnz, int *levels,
+                     int *max_level_rows);
+/* Copy the plan's level sets to host arrays (level_ptr: levels+1, level_set: n). */
+int symamd_plan_levels(symamd_plan plan, int *h_level_ptr, int *h_level_set);
+int symamd_plan_destroy(symamd_plan plan);
+
+/* ----------------------------------------------------------------- solve */
+/* Solve L X = B in place for k right-hand sides (X overwrites B), on the
+ * handle's stream, asynchronously.  dtype of B is the plan's dtype. */
+int symamd_solve(symamd_handle h, symamd_plan plan, void *d_B, int k,
+                 int64_t ld, symamd_layout layout, symamd_executor exec);
+/* Synchronise the handle's stream and report a sync-free watchdog expiry
+ * recorded by any earlier solve on this plan (SYMAMD_ERR_TIMEOUT). */
+int symamd_plan_check(symamd_handle h, symamd_plan plan);
+
+/* Capture one solve (fixed B pointer, k, layout, executor) as a hipGraph
+ * and replay it with symamd_graph_launch on the handle's stream. */
+int symamd_graph_create(symamd_handle h, symamd_plan plan, void *d_B, int k,
+                        int64_t ld, symamd_layout layout,
+                        symamd_executor exec, symamd_graph *g);
+int symamd_graph_launch(symamd_handle h, symamd_graph g);
+int symamd_graph_destroy(symamd_graph g);
+
+/* --------------------------------------------------------------- adjoint */
+/* For x = L^{-1} b and an upstream gradient gx = dloss/dx:
+ *   gb = L^{-T} gx                      (transposed solve, reverse levels;
+ *                                        the reference's ltsolve,
+ *                                        sparse_blas/sptrsv_lt.cpp:11-20)
+ *   gLx[p] = -(gb[i] * x[j])            for every stored entry p = (i, j) of
+ *                                        the caller's CSR of L (same order
+ *                                        as the Lx given to analyze).
+ * d_gb may alias d_gx.  d_gLx may be NULL (gb only). */
+int symamd_adjoint(symamd_handle h, symamd_plan plan, const void *d_x,
+                   const void *d_gx, void *d_gb, void *d_gLx,
+                   symamd_executor exec);
+
+/* ------------------------------------------------------------------ SpMV */
+/* y = A x (sparse_blas/spmv.cpp:12-21). FP64. */
+int symamd_spmv_csr(symamd_handle h, int n, const int *d_Ap, const int *d_Ai,
+                    const double *d_Ax, const double *d_x, double *d_y);
+/* z = a * (z + A x) + b * y (sparse_blas/spmv.cpp:23-34). FP64. */
+int symamd_spmv_csr_axpby(symamd_handle h, int n, const int *d_Ap,
+                          const int *d_Ai, const double *d_Ax,
+                          const double *d_x, const double *d_y, double *d_z,
+                          double a, double b);
+/* Relative residual ||L X - B||_2 / ||B||_2 of every column c < k, written to
+ * h_rel[c], using the plan's copy of L (blocking). */
+int symamd_residual(symamd_handle h, symamd_plan plan, const void *d_X,
+                    const void *d_B, int k, int64_t ld, symamd_layout layout,
+                    double *h_rel);
+
+/* ----------------------------------------------------------------- tuner */
+/* Time the candidate schedules for k right-hand sides in `layout` on
+ * scratch data and remember the fastest in the plan (and in a process-wide
+ * cache keyed by the matrix fingerprint); SYMAMD_EXEC_AUTO then uses it.
+ * Writes a human-readable summary (may be NULL). */
+int symamd_tune(symamd_handle h, symamd_plan plan, int k,
+                symamd_layout layout, char *summary, int summary_len);
+
+/* Low-level schedule override used by the tuner and by tests: lanes per row
+ * (1..64, power of two) for the given executor; 0 restores the heuristic. */
+int symamd_plan_set_lanes(symamd_plan plan, int lanes_per_row);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,318 @@
+"""The oracle -- TEST INFRASTRUCTURE ONLY.
+
+CPU restatement (oracle/oracle.c -> liboracle.so) of the reference's
+sparse triangular solve path, plus, when built in the development
+container, the reference's own sources compiled by oracle/Makefile into
+oracle/_ref/libsymref.so.  Only tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py may import this package, and only as the
+checker or the timed CPU baseline; the product (sympiler_amd/) never does.
+
+Functions and the reference lines they follow:
+  sptrsv_csr           sparse_blas/sptrsv.cpp:17-25
+  sptrsv_csr_levelset  sparse_blas/sptrsv.cpp:40-54
+  sptrsv_csr_lbc       sparse_blas/sptrsv.cpp:57-75
+  sptrsv_csc           sparse_blas/sptrsv.cpp:129-137
+  ltsolve              sparse_blas/sptrsv_lt.cpp:11-20
+  spmv_csr / _axpby    sparse_blas/spmv.cpp:12-21 / :23-34
+  spic0_csc_left       sparse_blas/spic0.cpp:11-46
+  levelset_csr/_csc    build_levelSet_CSC (aggregation, absent); definition
+                       of parsy/common/TreeUtils.cpp:89-141
+  sptrsm_csr           k columns of sptrsv_csr
+  adjoint              ltsolve + grad over L's pattern (closed form)
+  check_schedule       validity of a level_ptr/par_ptr/partition schedule
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import c_double, c_int, c_int64, c_void_p
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_PATH = os.path.join(_HERE, "liboracle.so")
+REF_PATH = os.path.join(_HERE, "_ref", "libsymref.so")
+
+_o = None
+_r = None
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(c_void_p)
+
+
+def olib():
+    global _o
+    if _o is None:
+        if not os.path.exists(ORACLE_PATH):
+            raise RuntimeError("oracle/liboracle.so missing: run `make -C oracle`")
+        L = ctypes.CDLL(ORACLE_PATH)
+        vp, i, d, i64 = c_void_p, c_int, c_double, c_int64
+        sigs = {
+            "orc_sptrsv_csr": ([i, vp, vp, vp, vp], None),
+            "orc_sptrsv_csr_levelset": ([i, vp, vp, vp, vp, i, vp, vp, i], None),
+            "orc_sptrsv_csr_lbc": ([i, vp, vp, vp, vp, i, vp, vp, vp], None),
+            "orc_sptrsv_csc": ([i, vp, vp, vp, vp], None),
+            "orc_ltsolve": ([i, vp, vp, vp, vp], c_int),
+            "orc_sptrsm_csr": ([i, vp, vp, vp, vp, i, i64, i64], None),
+            "orc_spmv_csr": ([i, vp, vp, vp, vp, vp], None),
+            "orc_spmv_csr_axpby": ([i, vp, vp, vp, vp, vp, vp, d, d], None),
+            "orc_spic0_csc_left": ([i, vp, vp, vp, vp, vp, vp], c_int),
+            "orc_levelset_csr": ([i, vp, vp, vp, vp, vp], c_int),
+            "orc_levelset_csc": ([i, vp, vp, vp, vp], c_int),
+            "orc_check_schedule": ([i, vp, vp, i, vp, vp, vp], c_int),
+            "orc_adjoint": ([i, vp, vp, vp, vp, vp, vp, vp, vp, vp], None),
+            "orc_csr_to_csc": ([i, vp, vp, vp, vp, vp, vp, vp], None),
+        }
+        for name, (args, res) in sigs.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _o = L
+    return _o
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_PATH)
+
+
+def rlib():
+    """The reference's own compiled functions (oracle/_ref), or raise."""
+    global _r
+    if _r is None:
+        if not ref_available():
+            raise RuntimeError("oracle/_ref/libsymref.so not built (needs /root/reference)")
+        L = ctypes.CDLL(REF_PATH)
+        vp, i, d = c_void_p, c_int, c_double
+        sigs = {
+            "ref_sptrsv_csr": ([i, vp, vp, vp, vp], None),
+            "ref_sptrsv_csr_levelset": ([i, vp, vp, vp, vp, i, vp, vp], None),
+            "ref_sptrsv_csr_lbc": ([i, vp, vp, vp, vp, i, vp, vp, vp], None),
+            "ref_sptrsv_csc": ([i, vp, vp, vp, vp], None),
+            "ref_ltsolve": ([i, vp, vp, vp, vp], c_int),
+            "ref_spmv_csr": ([i, vp, vp, vp, vp, vp], None),
+            "ref_spmv_csr_axpby": ([i, vp, vp, vp, vp, vp, vp, d, d], None),
+            "ref_spic0_csc_left": ([i, vp, vp, vp, vp, vp, vp], c_int),
+            "ref_spic0_csr": ([i, vp, vp, vp], None),
+            "ref_getLevelSet": ([i, vp, vp, vp], c_int),
+            "ref_set_threads": ([i], None),
+        }
+        for name, (args, res) in sigs.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _r = L
+    return _r
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+# ------------------------------------------------------------- restatement
+def sptrsv_csr(Lp, Li, Lx, b):
+    x = _f64(b).copy()
+    olib().orc_sptrsv_csr(len(Lp) - 1, _p(_i32(Lp)), _p(_i32(Li)), _p(_f64(Lx)), _p(x))
+    return x
+
+
+def sptrsv_csr_levelset(Lp, Li, Lx, b, level_ptr, level_set, nthreads=0):
+    x = _f64(b).copy()
+    olib().orc_sptrsv_csr_levelset(len(Lp) - 1, _p(_i32(Lp)), _p(_i32(Li)),
+                                   _p(_f64(Lx)), _p(x), len(level_ptr) - 1,
+                                   _p(_i32(level_ptr)), _p(_i32(level_set)), nthreads)
+    return x
+
+
+def sptrsv_csr_lbc(Lp, Li, Lx, b, level_ptr, par_ptr, partition):
+    x = _f64(b).copy()
+    olib().orc_sptrsv_csr_lbc(len(Lp) - 1, _p(_i32(Lp)), _p(_i32(Li)), _p(_f64(Lx)),
+                              _p(x), len(level_ptr) - 1, _p(_i32(level_ptr)),
+                              _p(_i32(par_ptr)), _p(_i32(partition)))
+    return x
+
+
+def sptrsv_csc(Cp, Ci, Cx, b):
+    x = _f64(b).copy()
+    olib().orc_sptrsv_csc(len(Cp) - 1, _p(_i32(Cp)), _p(_i32(Ci)), _p(_f64(Cx)), _p(x))
+    return x
+
+
+def ltsolve(Cp, Ci, Cx, b):
+    x = _f64(b).copy()
+    olib().orc_ltsolve(len(Cp) - 1, _p(_i32(Cp)), _p(_i32(Ci)), _p(_f64(Cx)), _p(x))
+    return x
+
+
+def sptrsm_csr(Lp, Li, Lx, B):
+    """B: (k, n) array = column-major n x k block; returns X of the same shape."""
+    X = _f64(B).copy()
+    k, n = X.shape
+    olib().orc_sptrsm_csr(n, _p(_i32(Lp)), _p(_i32(Li)), _p(_f64(Lx)), _p(X), k, 1, n)
+    return X
+
+
+def spmv_csr(Ap, Ai, Ax, x):
+    y = np.empty(len(Ap) - 1, np.float64)
+    olib().orc_spmv_csr(len(Ap) - 1, _p(_i32(Ap)), _p(_i32(Ai)), _p(_f64(Ax)),
+                        _p(_f64(x)), _p(y))
+    return y
+
+
+def spmv_csr_axpby(Ap, Ai, Ax, x, y, z, a, b):
+    z = _f64(z).copy()
+    olib().orc_spmv_csr_axpby(len(Ap) - 1, _p(_i32(Ap)), _p(_i32(Ai)), _p(_f64(Ax)),
+                              _p(_f64(x)), _p(_f64(y)), _p(z), a, b)
+    return z
+
+
+def spic0_csc_left(Ap, Ai, Ax, prunePtr, pruneSet):
+    Lx = np.zeros(len(Ax), np.float64)
+    ok = olib().orc_spic0_csc_left(len(Ap) - 1, _p(_i32(Ap)), _p(_i32(Ai)), _p(_f64(Ax)),
+                                   _p(_i32(prunePtr)), _p(_i32(pruneSet)), _p(Lx))
+    return Lx, bool(ok)
+
+
+def levelset_csr(Lp, Li):
+    n = len(Lp) - 1
+    lp = np.zeros(n + 1, np.int32)
+    ls = np.zeros(max(n, 1), np.int32)
+    lev = np.zeros(max(n, 1), np.int32)
+    nl = olib().orc_levelset_csr(n, _p(_i32(Lp)), _p(_i32(Li)), _p(lp), _p(ls), _p(lev))
+    return lp[: nl + 1], ls[:n], lev[:n]
+
+
+def levelset_csc(Cp, Ci):
+    n = len(Cp) - 1
+    lp = np.zeros(n + 1, np.int32)
+    ls = np.zeros(max(n, 1), np.int32)
+    nl = olib().orc_levelset_csc(n, _p(_i32(Cp)), _p(_i32(Ci)), _p(lp), _p(ls))
+    return lp[: nl + 1], ls[:n]
+
+
+def check_schedule(Lp, Li, level_ptr, par_ptr, partition) -> int:
+    return olib().orc_check_schedule(len(Lp) - 1, _p(_i32(Lp)), _p(_i32(Li)),
+                                     len(level_ptr) - 1, _p(_i32(level_ptr)),
+                                     _p(_i32(par_ptr)), _p(_i32(partition)))
+
+
+def csr_to_csc(Ap, Ai, Ax):
+    n = len(Ap) - 1
+    nnz = int(Ap[-1])
+    Cp = np.zeros(n + 1, np.int32)
+    Ci = np.zeros(max(nnz, 1), np.int32)
+    Cx = np.zeros(max(nnz, 1), np.float64)
+    mp = np.zeros(max(nnz, 1), np.int32)
+    olib().orc_csr_to_csc(n, _p(_i32(Ap)), _p(_i32(Ai)), _p(_f64(Ax)), _p(Cp), _p(Ci),
+                          _p(Cx), _p(mp))
+    return Cp, Ci[:nnz], Cx[:nnz], mp[:nnz]
+
+
+def adjoint(Lp, Li, Lx, x, gx):
+    """(gb, gL): gb = L^-T gx (ltsolve on the CSC of L), gL[p] = -gb[i] x[j]."""
+    n = len(Lp) - 1
+    Cp, Ci, Cx, _ = csr_to_csc(Lp, Li, Lx)
+    gb = np.empty(n, np.float64)
+    gL = np.empty(len(Lx), np.float64)
+    olib().orc_adjoint(n, _p(_i32(Lp)), _p(_i32(Li)), _p(Cp), _p(Ci), _p(Cx),
+                       _p(_f64(x)), _p(_f64(gx)), _p(gb), _p(gL))
+    return gb, gL
+
+
+def prune_set(Lp, Li):
+    """Row pattern of the strict lower part, per column c: the s < c with
+    L[c, s] != 0, ascending (the prune set spic0_csc_left walks)."""
+    n = len(Lp) - 1
+    ptr = np.zeros(n + 1, np.int32)
+    lens = np.diff(Lp) - 1
+    ptr[1:] = np.cumsum(lens)
+    keep = np.ones(len(Li), bool)
+    keep[np.asarray(Lp[1:]) - 1] = False
+    return ptr, _i32(np.asarray(Li)[keep])
+
+
+# ------------------------------------------------------------ reference .so
+class ref:
+    """Thin ctypes calls onto the reference's own compiled functions."""
+
+    @staticmethod
+    def sptrsv_csr(Lp, Li, Lx, b):
+        x = _f64(b).copy()
+        rlib().ref_sptrsv_csr(len(Lp) - 1, _p(_i32(Lp).copy()), _p(_i32(Li).copy()),
+                              _p(_f64(Lx).copy()), _p(x))
+        return x
+
+    @staticmethod
+    def sptrsv_csr_levelset(Lp, Li, Lx, b, level_ptr, level_set):
+        x = _f64(b).copy()
+        rlib().ref_sptrsv_csr_levelset(len(Lp) - 1, _p(_i32(Lp)), _p(_i32(Li)),
+                                       _p(_f64(Lx)), _p(x), len(level_ptr) - 1,
+                                       _p(_i32(level_ptr)), _p(_i32(level_set)))
+        return x
+
+    @staticmethod
+    def sptrsv_csr_lbc(Lp, Li, Lx, b, level_ptr, par_ptr, partition):
+        x = _f64(b).copy()
+        rlib().ref_sptrsv_csr_lbc(len(Lp) - 1, _p(_i32(Lp).copy()), _p(_i32(Li).copy()),
+                                  _p(_f64(Lx).copy()), _p(x), len(level_ptr) - 1,
+                                  _p(_i32(level_ptr).copy()), _p(_i32(par_ptr).copy()),
+                                  _p(_i32(partition).copy()))
+        return x
+
+    @staticmethod
+    def sptrsv_csc(Cp, Ci, Cx, b):
+        x = _f64(b).copy()
+        rlib().ref_sptrsv_csc(len(Cp) - 1, _p(_i32(Cp).copy()), _p(_i32(Ci).copy()),
+                              _p(_f64(Cx).copy()), _p(x))
+        return x
+
+    @staticmethod
+    def ltsolve(Cp, Ci, Cx, b):
+        x = _f64(b).copy()
+        rlib().ref_ltsolve(len(Cp) - 1, _p(_i32(Cp).copy()), _p(_i32(Ci).copy()),
+                           _p(_f64(Cx).copy()), _p(x))
+        return x
+
+    @staticmethod
+    def spmv_csr(Ap, Ai, Ax, x):
+        y = np.empty(len(Ap) - 1, np.float64)
+        rlib().ref_spmv_csr(len(Ap) - 1, _p(_i32(Ap)), _p(_i32(Ai)), _p(_f64(Ax)),
+                            _p(_f64(x)), _p(y))
+        return y
+
+    @staticmethod
+    def spmv_csr_axpby(Ap, Ai, Ax, x, y, z, a, b):
+        z = _f64(z).copy()
+        rlib().ref_spmv_csr_axpby(len(Ap) - 1, _p(_i32(Ap)), _p(_i32(Ai)), _p(_f64(Ax)),
+                                  _p(_f64(x)), _p(_f64(y)), _p(z), a, b)
+        return z
+
+    @staticmethod
+    def spic0_csc_left(Ap, Ai, Ax, prunePtr, pruneSet):
+        Lx = np.zeros(len(Ax), np.float64)
+        ok = rlib().ref_spic0_csc_left(len(Ap) - 1, _p(_i32(Ap)), _p(_i32(Ai)),
+                                       _p(_f64(Ax)), _p(_i32(prunePtr)),
+                                       _p(_i32(pruneSet)), _p(Lx))
+        return Lx, bool(ok)
+
+    @staticmethod
+    def spic0_csr(Ap, Ai, Ax):
+        Lx = _f64(Ax).copy()
+        rlib().ref_spic0_csr(len(Ap) - 1, _p(Lx), _p(_i32(Ap).copy()), _p(_i32(Ai).copy()))
+        return Lx
+
+    @staticmethod
+    def set_threads(n):
+        rlib().ref_set_threads(int(n))
+
+    @staticmethod
+    def getLevelSet(parent):
+        n = len(parent)
+        lp = np.zeros(n + 1, np.int32)
+        ls = np.zeros(max(n, 1), np.int32)
+        nl = rlib().ref_getLevelSet(n, _p(_i32(parent)), _p(lp), _p(ls))
+        return lp[: nl + 1], ls[:n]

@@ -1,0 +1,332 @@
+/*
+ * oracle.c -- CPU restatement of Sympiler's sparse triangular solve path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP library in
+ * sympiler_amd/.  Only tests/, __graft_entry__.smoke() and the cpu_baseline
+ * leg of bench.py load it, and only to compare against or to time the CPU
+ * path.  The product never links, loads or calls it.
+ *
+ * Every routine follows the operation order of the reference function it
+ * cites, so that, built with -ffp-contract=off (no FMA contraction, as the
+ * reference's own x86-64 build without -mfma), it reproduces the reference
+ * bit for bit.  That claim is checked by tests/test_oracle.py against
+ * oracle/_ref (the reference sources compiled by oracle/Makefile) and against
+ * the committed golden vectors in tests/golden/.
+ *
+ * Layout conventions (reference include/sympiler/sparse_blas_lib.h):
+ *   CSR lower triangle: diagonal LAST in each row   (sptrsv.cpp:20,23)
+ *   CSC lower triangle: diagonal FIRST in each column (sptrsv.cpp:132-133)
+ *   int32 indices, FP64 values, x in/out.
+ */
+#include <stdlib.h>
+#include <string.h>
+#include <math.h>
+#include <stdint.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ---------------------------------------------------------------- solves */
+
+/* sparse_blas/sptrsv.cpp:17-25 (sym_lib::sptrsv_csr): serial forward
+ * substitution, j ascending, multiply then subtract, divide by the
+ * last-in-row diagonal. */
+void orc_sptrsv_csr(int n, const int *Lp, const int *Li, const double *Lx,
+                    double *x) {
+  for (int i = 0; i < n; i++) {
+    for (int j = Lp[i]; j < Lp[i + 1] - 1; j++) x[i] -= Lx[j] * x[Li[j]];
+    x[i] /= Lx[Lp[i + 1] - 1];
+  }
+}
+
+/* sparse_blas/sptrsv.cpp:40-54 (sym_lib::sptrsv_csr_levelset): the same row
+ * arithmetic, rows of one level in parallel, one fork/join per level. */
+void orc_sptrsv_csr_levelset(int n, const int *Lp, const int *Li,
+                             const double *Lx, double *x, int levels,
+                             const int *levelPtr, const int *levelSet,
+                             int nthreads) {
+  (void)n;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+  (void)nthreads;
+#endif
+  for (int l = 0; l < levels; l++) {
+#pragma omp parallel for default(shared) schedule(static)
+    for (int k = levelPtr[l]; k < levelPtr[l + 1]; ++k) {
+      int i = levelSet[k];
+      for (int j = Lp[i]; j < Lp[i + 1] - 1; j++) x[i] -= Lx[j] * x[Li[j]];
+      x[i] /= Lx[Lp[i + 1] - 1];
+    }
+  }
+}
+
+/* sparse_blas/sptrsv.cpp:57-75 (sym_lib::sptrsv_csr_lbc): coarse levels in
+ * order, partitions of a coarse level in parallel, rows of a partition in
+ * their listed order. */
+void orc_sptrsv_csr_lbc(int n, const int *Lp, const int *Li, const double *Lx,
+                        double *x, int level_no, const int *level_ptr,
+                        const int *par_ptr, const int *partition) {
+  (void)n;
+  for (int i1 = 0; i1 < level_no; ++i1) {
+#pragma omp parallel for schedule(static)
+    for (int j1 = level_ptr[i1]; j1 < level_ptr[i1 + 1]; ++j1) {
+      for (int k1 = par_ptr[j1]; k1 < par_ptr[j1 + 1]; ++k1) {
+        int i = partition[k1];
+        for (int j = Lp[i]; j < Lp[i + 1] - 1; j++) x[i] -= Lx[j] * x[Li[j]];
+        x[i] /= Lx[Lp[i + 1] - 1];
+      }
+    }
+  }
+}
+
+/* sparse_blas/sptrsv.cpp:129-137 (sym_lib::sptrsv_csc): column-push forward
+ * substitution on the CSC of L, diagonal first. */
+void orc_sptrsv_csc(int n, const int *Lp, const int *Li, const double *Lx,
+                    double *x) {
+  for (int i = 0; i < n; i++) {
+    x[i] /= Lx[Lp[i]];
+    for (int j = Lp[i] + 1; j < Lp[i + 1]; j++) x[Li[j]] -= Lx[j] * x[i];
+  }
+}
+
+/* sparse_blas/sptrsv_lt.cpp:11-20 (sym_lib::ltsolve): L^T x = b from the CSC
+ * of L, columns j = n-1 .. 0, entries below the diagonal ascending, then the
+ * divide by the first-in-column diagonal.  Returns 0 on null input. */
+int orc_ltsolve(int n, const int *Lp, const int *Li, const double *Lx,
+                double *x) {
+  if (!Lp || !Li || !x) return 0;
+  for (int j = n - 1; j >= 0; j--) {
+    for (int p = Lp[j] + 1; p < Lp[j + 1]; p++) x[j] -= Lx[p] * x[Li[p]];
+    x[j] /= Lx[Lp[j]];
+  }
+  return 1;
+}
+
+/* Multi-RHS forward solve: the reference has no CSR SpTRSM, so the oracle
+ * for k right-hand sides is k independent sptrsv_csr solves, one per column
+ * of X (sptrsv.cpp:17-25 applied column by column).  X element (i, c) lives
+ * at X[i * rs + c * cs] (column-major: rs = 1, cs = ld; row-major: rs = ld,
+ * cs = 1). */
+void orc_sptrsm_csr(int n, const int *Lp, const int *Li, const double *Lx,
+                    double *X, int k, int64_t rs, int64_t cs) {
+#pragma omp parallel for schedule(static)
+  for (int c = 0; c < k; c++) {
+    double *x = X + (int64_t)c * cs;
+    for (int i = 0; i < n; i++) {
+      double v = x[(int64_t)i * rs];
+      for (int j = Lp[i]; j < Lp[i + 1] - 1; j++)
+        v -= Lx[j] * x[(int64_t)Li[j] * rs];
+      v /= Lx[Lp[i + 1] - 1];
+      x[(int64_t)i * rs] = v;
+    }
+  }
+}
+
+/* ---------------------------------------------------------------- SpMV */
+
+/* sparse_blas/spmv.cpp:12-21 (sym_lib::spmv_csr, 6 args): y = A x with y
+ * zeroed first, entries ascending. */
+void orc_spmv_csr(int n, const int *Ap, const int *Ai, const double *Ax,
+                  const double *x, double *y) {
+  for (int i = 0; i < n; i++) {
+    y[i] = 0;
+    for (int j = Ap[i]; j < Ap[i + 1]; j++) y[i] += Ax[j] * x[Ai[j]];
+  }
+}
+
+/* sparse_blas/spmv.cpp:23-34 (sym_lib::spmv_csr, 9 args):
+ * z[i] = a * (z[i] + sum_j A[i,j] x[j]) + b * y[i].  z is accumulated into,
+ * so callers pre-zero it for z = a A x + b y. */
+void orc_spmv_csr_axpby(int n, const int *Ap, const int *Ai, const double *Ax,
+                        const double *x, const double *y, double *z, double a,
+                        double b) {
+  for (int i = 0; i < n; i++) {
+    for (int j = Ap[i]; j < Ap[i + 1]; j++) z[i] += Ax[j] * x[Ai[j]];
+    z[i] = a * z[i] + b * y[i];
+  }
+}
+
+/* ---------------------------------------------------------------- IC(0) */
+
+/* sparse_blas/spic0.cpp:11-46 (sym_lib::spic0_csc_left): left-looking IC(0)
+ * on the CSC lower triangle of A (diagonal first).  prunePtr/pruneSet is the
+ * row pattern of the strict lower part (for column c: the columns s < c with
+ * L[c,s] != 0, ascending).  Returns 0 on a non-positive pivot. */
+int orc_spic0_csc_left(int n, const int *Ap, const int *Ai, const double *Ax,
+                       const int *prunePtr, const int *pruneSet, double *Lx) {
+  double *f = (double *)calloc((size_t)n, sizeof(double));
+  for (int col = 0; col < n; ++col) {
+    for (int p = Ap[col]; p < Ap[col + 1]; ++p) f[Ai[p]] = Ax[p];
+    for (int q = prunePtr[col]; q < prunePtr[col + 1]; ++q) {
+      int s = pruneSet[q];
+      double t = 0;
+      int found = 0;
+      for (int l = Ap[s]; l < Ap[s + 1]; ++l) {
+        if (Ai[l] == col && !found) {
+          t = Lx[l];
+          found = 1;
+        }
+        if (found) f[Ai[l]] -= Lx[l] * t;
+      }
+    }
+    if (f[col] <= 0) {
+      free(f);
+      return 0;
+    }
+    double d = sqrt(f[col]);
+    f[col] = 0;
+    Lx[Ap[col]] = d;
+    for (int p = Ap[col] + 1; p < Ap[col + 1]; ++p) {
+      Lx[p] = f[Ai[p]] / d;
+      f[Ai[p]] = 0;
+    }
+  }
+  free(f);
+  return 1;
+}
+
+/* --------------------------------------------------------- level sets */
+
+/* Level-set inspector on the CSR of L (the reference's build_levelSet_CSC
+ * lives in the un-vendored aggregation submodule; its call sites are
+ * Catch_tests/sptrsv_tests.cpp:59 and demo/sptrsv_demo_utils.h:44).  The
+ * definition restated here is the one the in-tree analog
+ * parsy/common/TreeUtils.cpp:89-141 (getLevelSet) implements: level(i) = 0
+ * when row i has no off-diagonal entry, else 1 + max level(j) over
+ * L[i,j] != 0; levelSet lists rows by level, ascending row id inside a
+ * level (getLevelSet scans i ascending each round).  Returns the number of
+ * levels.  levelPtr has room for n+1 entries. */
+int orc_levelset_csr(int n, const int *Lp, const int *Li, int *levelPtr,
+                     int *levelSet, int *level_of) {
+  int *lev = level_of ? level_of : (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+  int nlev = 0;
+  for (int i = 0; i < n; i++) {
+    int m = -1;
+    for (int j = Lp[i]; j < Lp[i + 1] - 1; j++)
+      if (lev[Li[j]] > m) m = lev[Li[j]];
+    lev[i] = m + 1;
+    if (lev[i] + 1 > nlev) nlev = lev[i] + 1;
+  }
+  memset(levelPtr, 0, sizeof(int) * (size_t)(nlev + 1));
+  for (int i = 0; i < n; i++) levelPtr[lev[i] + 1]++;
+  for (int l = 0; l < nlev; l++) levelPtr[l + 1] += levelPtr[l];
+  int *fill = (int *)malloc(sizeof(int) * (size_t)(nlev + 1));
+  memcpy(fill, levelPtr, sizeof(int) * (size_t)(nlev + 1));
+  for (int i = 0; i < n; i++) levelSet[fill[lev[i]]++] = i;
+  free(fill);
+  if (!level_of) free(lev);
+  return nlev;
+}
+
+/* The same levels computed from the CSC of L (diagonal first), which is the
+ * input the reference inspector takes (sptrsv_demo_utils.h:44 passes the CSC
+ * of L).  Push form: level(r) >= level(c) + 1 for every L[r,c] != 0. */
+int orc_levelset_csc(int n, const int *Lp, const int *Li, int *levelPtr,
+                     int *levelSet) {
+  int *lev = (int *)calloc((size_t)(n > 0 ? n : 1), sizeof(int));
+  int nlev = n > 0 ? 1 : 0;
+  for (int c = 0; c < n; c++) {
+    for (int p = Lp[c] + 1; p < Lp[c + 1]; p++) {
+      int r = Li[p];
+      if (lev[c] + 1 > lev[r]) lev[r] = lev[c] + 1;
+    }
+    if (lev[c] + 1 > nlev) nlev = lev[c] + 1;
+  }
+  memset(levelPtr, 0, sizeof(int) * (size_t)(nlev + 1));
+  for (int i = 0; i < n; i++) levelPtr[lev[i] + 1]++;
+  for (int l = 0; l < nlev; l++) levelPtr[l + 1] += levelPtr[l];
+  int *fill = (int *)malloc(sizeof(int) * (size_t)(nlev + 1));
+  memcpy(fill, levelPtr, sizeof(int) * (size_t)(nlev + 1));
+  for (int i = 0; i < n; i++) levelSet[fill[lev[i]]++] = i;
+  free(fill);
+  free(lev);
+  return nlev;
+}
+
+/* Validity of a 3-array coarsened schedule (level_ptr / par_ptr / partition,
+ * the format sptrsv.cpp:57-75 consumes).  Returns 0 when valid, else a
+ * positive code:
+ *   1 shape (pointers not monotone or out of range)
+ *   2 some row missing or listed twice
+ *   3 a dependency j -> i with coarse(j) > coarse(i)
+ *   4 a dependency inside one coarse level across two partitions
+ *   5 a dependency inside one partition listed after its consumer */
+int orc_check_schedule(int n, const int *Lp, const int *Li, int level_no,
+                       const int *level_ptr, const int *par_ptr,
+                       const int *partition) {
+  if (level_no < 0 || level_ptr[0] != 0) return 1;
+  for (int l = 0; l < level_no; l++)
+    if (level_ptr[l + 1] < level_ptr[l]) return 1;
+  int nparts = level_ptr[level_no];
+  if (par_ptr[0] != 0) return 1;
+  for (int p = 0; p < nparts; p++)
+    if (par_ptr[p + 1] < par_ptr[p]) return 1;
+  if (par_ptr[nparts] != n) return 1;
+  int *coarse = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+  int *part = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+  int *pos = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+  for (int i = 0; i < n; i++) coarse[i] = -1;
+  int rc = 0;
+  for (int l = 0; l < level_no && !rc; l++)
+    for (int p = level_ptr[l]; p < level_ptr[l + 1] && !rc; p++)
+      for (int k = par_ptr[p]; k < par_ptr[p + 1]; k++) {
+        int i = partition[k];
+        if (i < 0 || i >= n || coarse[i] != -1) {
+          rc = 2;
+          break;
+        }
+        coarse[i] = l;
+        part[i] = p;
+        pos[i] = k;
+      }
+  for (int i = 0; i < n && !rc; i++)
+    for (int j = Lp[i]; j < Lp[i + 1] - 1 && !rc; j++) {
+      int d = Li[j];
+      if (coarse[d] > coarse[i]) rc = 3;
+      else if (coarse[d] == coarse[i] && part[d] != part[i]) rc = 4;
+      else if (coarse[d] == coarse[i] && pos[d] > pos[i]) rc = 5;
+    }
+  free(coarse);
+  free(part);
+  free(pos);
+  return rc;
+}
+
+/* ------------------------------------------------------------- adjoint */
+
+/* Adjoint of x = L^{-1} b for a loss with gradient gx = dloss/dx.  The
+ * reference has no adjoint; this restates the closed form
+ *   gb = L^{-T} gx           (ltsolve, sptrsv_lt.cpp:11-20, on the CSC of L)
+ *   gL[p] = -(gb[i] * x[j])  for every stored entry p = (i, j) of the CSR of L
+ * (from d x = -L^{-1} dL x).  Lcsc_* is the CSC of the same L, diagonal
+ * first, rows ascending in each column.  gb may alias gx. */
+void orc_adjoint(int n, const int *Lp, const int *Li, const int *Cp,
+                 const int *Ci, const double *Cx, const double *x,
+                 const double *gx, double *gb, double *gL) {
+  if (gb != gx) memcpy(gb, gx, sizeof(double) * (size_t)n);
+  orc_ltsolve(n, Cp, Ci, Cx, gb);
+  for (int i = 0; i < n; i++)
+    for (int p = Lp[i]; p < Lp[i + 1]; p++) gL[p] = -(gb[i] * x[Li[p]]);
+}
+
+/* CSR -> CSC transpose (stable: entries of a column in ascending row order,
+ * which puts the diagonal of a lower triangle first).  map[p] gives the CSC
+ * position of CSR entry p (may be NULL). */
+void orc_csr_to_csc(int n, const int *Ap, const int *Ai, const double *Ax,
+                    int *Cp, int *Ci, double *Cx, int *map) {
+  memset(Cp, 0, sizeof(int) * (size_t)(n + 1));
+  for (int i = 0; i < n; i++)
+    for (int p = Ap[i]; p < Ap[i + 1]; p++) Cp[Ai[p] + 1]++;
+  for (int c = 0; c < n; c++) Cp[c + 1] += Cp[c];
+  int *fill = (int *)malloc(sizeof(int) * (size_t)(n + 1));
+  memcpy(fill, Cp, sizeof(int) * (size_t)(n + 1));
+  for (int i = 0; i < n; i++)
+    for (int p = Ap[i]; p < Ap[i + 1]; p++) {
+      int q = fill[Ai[p]]++;
+      Ci[q] = i;
+      if (Cx) Cx[q] = Ax[p];
+      if (map) map[p] = q;
+    }
+  free(fill);
+}

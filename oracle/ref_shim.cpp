@@ -1,0 +1,87 @@
+// ref_shim.cpp -- C entry points onto the reference's own compiled
+// functions, so tests can load oracle/_ref/libsymref.so with ctypes.
+//
+// TEST INFRASTRUCTURE ONLY: built by oracle/Makefile (target `ref`) together
+// with the reference sources where they lie; used by tests/ to pin the C
+// restatement in oracle/oracle.c and, optionally, as the cpu_baseline of
+// bench.py.  Nothing in the product loads it.
+//
+// The prototypes below restate the reference declarations in
+// include/sympiler/sparse_blas_lib.h (cited per line); that header cannot be
+// included as-is because it opens a header of the un-vendored aggregation
+// submodule.
+#include <cstddef>
+#include <cstdint>
+#include <omp.h>
+
+namespace sym_lib {
+void sptrsv_csr(int n, int *Lp, int *Li, double *Lx, double *x);  // :126-127
+void sptrsv_csr_levelset(int n, const int *Lp, const int *Li,
+                         const double *Lx, double *x, int levels,
+                         const int *levelPtr, const int *levelSet);  // :145-147
+void sptrsv_csr_lbc(int n, int *Lp, int *Li, double *Lx, double *x,
+                    int level_no, int *level_ptr, int *par_ptr,
+                    int *partition);  // :150-152
+void sptrsv_csc(int n, int *Lp, int *Li, double *Lx, double *x);  // :169-170
+int ltsolve(int n, int *Lp, int *Li, double *Lx, double *x);      // :330
+void spmv_csr(int n, const int *Ap, const int *Ai, const double *Ax,
+              const double *x, double *y);  // :18-20
+void spmv_csr(int n, const int *Ap, const int *Ai, const double *Ax,
+              const double *x, const double *y, double *z, double a,
+              double b);  // :32-33
+bool spic0_csc_left(int n, const int *Ap, const int *Ai, const double *Ax,
+                    const int *prunePtr, const int *pruneSet,
+                    double *Lx);  // :256-259
+void spic0_csr(int n, double *val, int *rowPtr, int *rowIdx);  // :242
+namespace parsy {
+// include/sympiler/parsy/common/TreeUtils.h (impl TreeUtils.cpp:89-141)
+int getLevelSet(size_t n, const int *inTree, int *levelPtr, int *levelSet);
+}  // namespace parsy
+}  // namespace sym_lib
+
+extern "C" {
+// Thread count of the reference's OpenMP regions (it reads the caller's
+// omp_set_num_threads, sptrsv_demo.cpp:62).
+void ref_set_threads(int n) { omp_set_num_threads(n); }
+
+void ref_sptrsv_csr(int n, int *Lp, int *Li, double *Lx, double *x) {
+  sym_lib::sptrsv_csr(n, Lp, Li, Lx, x);
+}
+void ref_sptrsv_csr_levelset(int n, const int *Lp, const int *Li,
+                             const double *Lx, double *x, int levels,
+                             const int *levelPtr, const int *levelSet) {
+  sym_lib::sptrsv_csr_levelset(n, Lp, Li, Lx, x, levels, levelPtr, levelSet);
+}
+void ref_sptrsv_csr_lbc(int n, int *Lp, int *Li, double *Lx, double *x,
+                        int level_no, int *level_ptr, int *par_ptr,
+                        int *partition) {
+  sym_lib::sptrsv_csr_lbc(n, Lp, Li, Lx, x, level_no, level_ptr, par_ptr,
+                          partition);
+}
+void ref_sptrsv_csc(int n, int *Lp, int *Li, double *Lx, double *x) {
+  sym_lib::sptrsv_csc(n, Lp, Li, Lx, x);
+}
+int ref_ltsolve(int n, int *Lp, int *Li, double *Lx, double *x) {
+  return sym_lib::ltsolve(n, Lp, Li, Lx, x);
+}
+void ref_spmv_csr(int n, const int *Ap, const int *Ai, const double *Ax,
+                  const double *x, double *y) {
+  sym_lib::spmv_csr(n, Ap, Ai, Ax, x, y);
+}
+void ref_spmv_csr_axpby(int n, const int *Ap, const int *Ai, const double *Ax,
+                        const double *x, const double *y, double *z, double a,
+                        double b) {
+  sym_lib::spmv_csr(n, Ap, Ai, Ax, x, y, z, a, b);
+}
+int ref_spic0_csc_left(int n, const int *Ap, const int *Ai, const double *Ax,
+                       const int *prunePtr, const int *pruneSet, double *Lx) {
+  return sym_lib::spic0_csc_left(n, Ap, Ai, Ax, prunePtr, pruneSet, Lx) ? 1
+                                                                        : 0;
+}
+void ref_spic0_csr(int n, double *val, int *rowPtr, int *rowIdx) {
+  sym_lib::spic0_csr(n, val, rowPtr, rowIdx);
+}
+int ref_getLevelSet(int n, const int *inTree, int *levelPtr, int *levelSet) {
+  return sym_lib::parsy::getLevelSet((size_t)n, inTree, levelPtr, levelSet);
+}
+}

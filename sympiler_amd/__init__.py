@@ -1,0 +1,359 @@
+"""sympiler_amd -- MI355X (gfx950) sparse lower-triangular solve.
+
+Host-side Python mirror of the C ABI in include/sympiler_amd.h, loaded with
+ctypes from the in-tree libsympiler_amd.so.  Device arrays are torch CUDA
+tensors (ROCm); their data pointers and torch's current HIP stream are what
+cross the boundary.  There is no CPU fallback: if the HIP library is missing
+every entry point raises.
+
+Reference mapping (reference include/sympiler/sparse_blas_lib.h):
+  Plan.solve(k=1)        sym_lib::sptrsv_csr / sptrsv_csr_levelset
+                         (sparse_blas/sptrsv.cpp:17-25, :40-54)
+  Plan.solve(k>1)        k independent sptrsv_csr solves, one per column
+  Plan.adjoint           sym_lib::ltsolve (sparse_blas/sptrsv_lt.cpp:11-20)
+                         + grad over L's values
+  levelset()             build_levelSet_CSC (aggregation submodule, absent)
+  coarsen()              get_coarse_levelSet_DAG_CSC_tree (absent)
+  spmv_csr()             sym_lib::spmv_csr (sparse_blas/spmv.cpp:12-34)
+  solve_lbc()            sym_lib::sptrsv_csr_lbc (sparse_blas/sptrsv.cpp:57-75)
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import (POINTER, byref, c_char_p, c_double, c_int, c_int64,
+                    c_void_p)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsympiler_amd.so")
+GEN_PATH = os.path.join(_HERE, "libsymamd_gen.so")
+
+F64, F32 = 0, 1
+COL_MAJOR, ROW_MAJOR = 0, 1
+EXEC = {"auto": 0, "levelset": 1, "syncfree": 2, "graph": 3}
+STATUS = {0: "OK", 1: "INVALID_ARG", 2: "STRUCTURE", 3: "ZERO_PIVOT", 4: "HIP",
+          5: "ALLOC", 6: "TIMEOUT", 7: "NOT_SUPPORTED"}
+
+
+class SymAmdError(RuntimeError):
+    """A C-ABI call returned a non-zero status."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """The loaded HIP library; raises if it was never built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `make -C sympiler_amd/csrc` "
+                "(or __graft_entry__.build()); there is no CPU fallback")
+        L = ctypes.CDLL(LIB_PATH, mode=ctypes.RTLD_GLOBAL)
+        vp, i32, i64, dbl = c_void_p, c_int, c_int64, c_double
+        sig = {
+            "symamd_create": [POINTER(vp), vp],
+            "symamd_destroy": [vp],
+            "symamd_set_stream": [vp, vp],
+            "symamd_set_timeout_ms": [vp, dbl],
+            "symamd_levelset": [vp, i32, vp, vp, vp, vp, POINTER(i32)],
+            "symamd_coarsen": [vp, i32, vp, vp, i32, i32, vp, vp, vp,
+                               POINTER(i32), POINTER(i32)],
+            "symamd_solve_lbc": [vp, i32, vp, vp, vp, vp, i32, vp, vp, vp],
+            "symamd_analyze": [vp, i32, vp, vp, vp, i32, POINTER(vp)],
+            "symamd_analyze_batched": [vp, i32, vp, vp, vp, vp, i32, POINTER(vp)],
+            "symamd_plan_update_values": [vp, vp, vp],
+            "symamd_plan_info": [vp, POINTER(i32), POINTER(i64), POINTER(i32),
+                                 POINTER(i32)],
+            "symamd_plan_levels": [vp, vp, vp],
+            "symamd_plan_destroy": [vp],
+            "symamd_plan_set_lanes": [vp, i32],
+            "symamd_solve": [vp, vp, vp, i32, i64, i32, i32],
+            "symamd_plan_check": [vp, vp],
+            "symamd_graph_create": [vp, vp, vp, i32, i64, i32, i32, POINTER(vp)],
+            "symamd_graph_launch": [vp, vp],
+            "symamd_graph_destroy": [vp],
+            "symamd_adjoint": [vp, vp, vp, vp, vp, vp, i32],
+            "symamd_spmv_csr": [vp, i32, vp, vp, vp, vp, vp],
+            "symamd_spmv_csr_axpby": [vp, i32, vp, vp, vp, vp, vp, vp, dbl, dbl],
+            "symamd_residual": [vp, vp, vp, vp, i32, i64, i32, vp],
+            "symamd_tune": [vp, vp, i32, i32, c_char_p, i32],
+        }
+        for name, args in sig.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = c_int
+        L.symamd_last_error.restype = c_char_p
+        L.symamd_last_error.argtypes = []
+        L.symamd_version.restype = c_char_p
+        L.symamd_version.argtypes = []
+        _lib = L
+    return _lib
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise SymAmdError(rc, lib().symamd_last_error().decode())
+
+
+def _ptr(t) -> c_void_p:
+    return c_void_p(0 if t is None else t.data_ptr())
+
+
+def _require_cuda(*tensors):
+    for t in tensors:
+        if t is not None and not t.is_cuda:
+            raise ValueError("sympiler_amd expects device (cuda) tensors")
+        if t is not None and not t.is_contiguous():
+            raise ValueError("sympiler_amd expects contiguous tensors")
+
+
+def _dtype_code(dt) -> int:
+    import torch
+    if dt == torch.float64:
+        return F64
+    if dt == torch.float32:
+        return F32
+    raise ValueError(f"unsupported dtype {dt}")
+
+
+class Handle:
+    """A library handle bound to a HIP stream (torch's current stream when
+    `stream` is None)."""
+
+    def __init__(self, stream=None, timeout_ms: float | None = None):
+        import torch
+        if stream is None:
+            stream = torch.cuda.current_stream()
+        self._h = c_void_p()
+        _check(lib().symamd_create(byref(self._h), c_void_p(stream.cuda_stream)))
+        self.stream = stream
+        if timeout_ms is not None:
+            _check(lib().symamd_set_timeout_ms(self._h, float(timeout_ms)))
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.symamd_destroy(self._h)
+            self._h = None
+
+
+_default_handle = None
+
+
+def default_handle() -> Handle:
+    global _default_handle
+    if _default_handle is None:
+        _default_handle = Handle()
+    return _default_handle
+
+
+class Plan:
+    """Analysed lower-triangular L (CSR, diagonal last), resident on the GPU.
+
+    Lp, Li: int32 cuda tensors; Lx: float64 or float32 cuda tensor.
+    If `row_offsets` is given, the rows form independent systems with column
+    ids local to each system (symamd_analyze_batched).
+    """
+
+    def __init__(self, Lp, Li, Lx, handle: Handle | None = None,
+                 row_offsets=None):
+        import torch
+        _require_cuda(Lp, Li, Lx)
+        if Lp.dtype != torch.int32 or Li.dtype != torch.int32:
+            raise ValueError("Lp and Li must be int32")
+        self.h = handle or default_handle()
+        self.n = Lp.numel() - 1
+        self.dtype = Lx.dtype
+        self._p = c_void_p()
+        if row_offsets is None:
+            _check(lib().symamd_analyze(self.h._h, self.n, _ptr(Lp), _ptr(Li),
+                                        _ptr(Lx), _dtype_code(Lx.dtype),
+                                        byref(self._p)))
+            self.nsys = 1
+        else:
+            off = (c_int * len(row_offsets))(*[int(v) for v in row_offsets])
+            self.nsys = len(row_offsets) - 1
+            _check(lib().symamd_analyze_batched(self.h._h, self.nsys, off,
+                                                _ptr(Lp), _ptr(Li), _ptr(Lx),
+                                                _dtype_code(Lx.dtype),
+                                                byref(self._p)))
+        n, nnz, lev, mx = c_int(), c_int64(), c_int(), c_int()
+        _check(lib().symamd_plan_info(self._p, byref(n), byref(nnz), byref(lev),
+                                      byref(mx)))
+        self.nnz, self.levels, self.max_level_rows = nnz.value, lev.value, mx.value
+
+    def __del__(self):
+        if getattr(self, "_p", None) and _lib is not None:
+            _lib.symamd_plan_destroy(self._p)
+            self._p = None
+
+    # ----------------------------------------------------------- inspection
+    def level_sets(self):
+        """(level_ptr, level_set) as numpy int32 arrays."""
+        import numpy as np
+        lp = np.zeros(self.levels + 1, dtype=np.int32)
+        ls = np.zeros(max(self.n, 1), dtype=np.int32)
+        _check(lib().symamd_plan_levels(self._p, lp.ctypes.data_as(c_void_p),
+                                        ls.ctypes.data_as(c_void_p)))
+        return lp, ls[: self.n]
+
+    def set_lanes(self, lanes: int):
+        _check(lib().symamd_plan_set_lanes(self._p, int(lanes)))
+
+    def update_values(self, Lx):
+        _require_cuda(Lx)
+        if Lx.dtype != self.dtype:
+            raise ValueError("value dtype differs from the plan's")
+        _check(lib().symamd_plan_update_values(self.h._h, self._p, _ptr(Lx)))
+
+    # ---------------------------------------------------------------- solve
+    @staticmethod
+    def _layout(B, layout):
+        if B.dim() == 1:
+            return 1, B.numel(), COL_MAJOR
+        if layout in ("col", COL_MAJOR):
+            # torch (k, n) contiguous == column-major n x k with ld = n
+            return B.shape[0], B.shape[1], COL_MAJOR
+        if layout in ("row", ROW_MAJOR):
+            return B.shape[1], B.shape[1], ROW_MAJOR
+        raise ValueError(f"unknown layout {layout}")
+
+    def solve(self, B, layout="col", executor="auto"):
+        """Solve L X = B in place.  B: (n,) vector; or (k, n) for
+        column-major right-hand sides (layout='col'); or (n, k) for
+        row-major (layout='row')."""
+        _require_cuda(B)
+        if B.dtype != self.dtype:
+            raise ValueError("right-hand side dtype differs from the plan's")
+        k, ld, lay = self._layout(B, layout)
+        _check(lib().symamd_solve(self.h._h, self._p, _ptr(B), k, ld, lay,
+                                  EXEC[executor]))
+        return B
+
+    def check(self):
+        """Synchronise and raise if a sync-free watchdog expired."""
+        _check(lib().symamd_plan_check(self.h._h, self._p))
+
+    def graph(self, B, layout="col", executor="auto") -> "Graph":
+        _require_cuda(B)
+        k, ld, lay = self._layout(B, layout)
+        g = c_void_p()
+        _check(lib().symamd_graph_create(self.h._h, self._p, _ptr(B), k, ld,
+                                         lay, EXEC[executor], byref(g)))
+        return Graph(self, g, B)
+
+    def adjoint(self, x, gx, gb=None, gLx=None, executor="auto"):
+        """gb = L^-T gx and gLx[p] = -gb[i] * x[j] over L's pattern."""
+        import torch
+        _require_cuda(x, gx, gb, gLx)
+        if gb is None:
+            gb = torch.empty_like(gx)
+        _check(lib().symamd_adjoint(self.h._h, self._p, _ptr(x), _ptr(gx),
+                                    _ptr(gb), _ptr(gLx), EXEC[executor]))
+        return gb, gLx
+
+    def residual(self, X, B, layout="col"):
+        """Relative residual ||L X - B|| / ||B|| per column (list of floats)."""
+        import numpy as np
+        _require_cuda(X, B)
+        k, ld, lay = self._layout(X, layout)
+        out = np.zeros(k, dtype=np.float64)
+        _check(lib().symamd_residual(self.h._h, self._p, _ptr(X), _ptr(B), k,
+                                     ld, lay, out.ctypes.data_as(c_void_p)))
+        return out
+
+    def tune(self, k=1, layout="col") -> str:
+        buf = ctypes.create_string_buffer(4096)
+        lay = COL_MAJOR if layout in ("col", COL_MAJOR) else ROW_MAJOR
+        _check(lib().symamd_tune(self.h._h, self._p, int(k), lay, buf, 4096))
+        return buf.value.decode()
+
+
+class Graph:
+    """A captured solve (hipGraph) replayed on the handle's stream."""
+
+    def __init__(self, plan: Plan, g, B):
+        self.plan, self._g, self._B = plan, g, B
+
+    def launch(self):
+        _check(lib().symamd_graph_launch(self.plan.h._h, self._g))
+
+    def __del__(self):
+        if getattr(self, "_g", None) and _lib is not None:
+            _lib.symamd_graph_destroy(self._g)
+            self._g = None
+
+
+# ------------------------------------------------------------ free functions
+def levelset(Lp, Li, handle: Handle | None = None):
+    """GPU level sets of a device CSR lower triangle -> (level_ptr, level_set)
+    int32 cuda tensors."""
+    import torch
+    _require_cuda(Lp, Li)
+    h = handle or default_handle()
+    n = Lp.numel() - 1
+    lp = torch.empty(n + 1, dtype=torch.int32, device=Lp.device)
+    ls = torch.empty(max(n, 1), dtype=torch.int32, device=Lp.device)
+    lev = c_int()
+    _check(lib().symamd_levelset(h._h, n, _ptr(Lp), _ptr(Li), _ptr(lp), _ptr(ls),
+                                 byref(lev)))
+    return lp[: lev.value + 1], ls[:n]
+
+
+def coarsen(Lp, Li, span=4, max_part_cost=4000, handle: Handle | None = None):
+    """Coarsened (level_ptr, par_ptr, partition) schedule, int32 cuda tensors."""
+    import torch
+    _require_cuda(Lp, Li)
+    h = handle or default_handle()
+    n = Lp.numel() - 1
+    lp = torch.empty(n + 1, dtype=torch.int32, device=Lp.device)
+    pp = torch.empty(n + 1, dtype=torch.int32, device=Lp.device)
+    part = torch.empty(max(n, 1), dtype=torch.int32, device=Lp.device)
+    lno, pno = c_int(), c_int()
+    _check(lib().symamd_coarsen(h._h, n, _ptr(Lp), _ptr(Li), int(span),
+                                int(max_part_cost), _ptr(lp), _ptr(pp),
+                                _ptr(part), byref(lno), byref(pno)))
+    return lp[: lno.value + 1], pp[: pno.value + 1], part[:n]
+
+
+def solve_lbc(Lp, Li, Lx, x, level_ptr, par_ptr, partition,
+              handle: Handle | None = None):
+    """Execute a caller schedule in place on x (FP64)."""
+    import numpy as np
+    _require_cuda(Lp, Li, Lx, x, par_ptr, partition)
+    h = handle or default_handle()
+    lp_host = np.ascontiguousarray(level_ptr.cpu().numpy() if hasattr(level_ptr, "cpu")
+                                   else level_ptr, dtype=np.int32)
+    _check(lib().symamd_solve_lbc(h._h, Lp.numel() - 1, _ptr(Lp), _ptr(Li),
+                                  _ptr(Lx), _ptr(x), len(lp_host) - 1,
+                                  lp_host.ctypes.data_as(c_void_p), _ptr(par_ptr),
+                                  _ptr(partition)))
+    return x
+
+
+def spmv_csr(Ap, Ai, Ax, x, y=None, z=None, a=None, b=None,
+             handle: Handle | None = None):
+    """y = A x, or (with y, z, a, b) z = a * (z + A x) + b * y; FP64."""
+    import torch
+    h = handle or default_handle()
+    n = Ap.numel() - 1
+    if z is None:
+        out = torch.empty(n, dtype=torch.float64, device=Ap.device)
+        _require_cuda(Ap, Ai, Ax, x)
+        _check(lib().symamd_spmv_csr(h._h, n, _ptr(Ap), _ptr(Ai), _ptr(Ax),
+                                     _ptr(x), _ptr(out)))
+        return out
+    _require_cuda(Ap, Ai, Ax, x, y, z)
+    _check(lib().symamd_spmv_csr_axpby(h._h, n, _ptr(Ap), _ptr(Ai), _ptr(Ax),
+                                       _ptr(x), _ptr(y), _ptr(z), float(a),
+                                       float(b)))
+    return z
+
+
+def version() -> str:
+    return lib().symamd_version().decode()

@@ -1,0 +1,725 @@
+// api.hip -- the extern "C" ABI of libsympiler_amd.so: handles, plans,
+// solve/adjoint/SpMV dispatch, captured graphs, the schedule tuner and the
+// coarsened-schedule inspector.
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstring>
+#include <numeric>
+
+#include "common.h"
+
+namespace symamd {
+
+namespace {
+thread_local std::string g_last_error;
+std::map<uint64_t, std::map<int64_t, Choice>> g_tune_cache;  // fingerprint ->
+
+uint64_t fnv1a(uint64_t h, const void *data, size_t len) {
+  const unsigned char *b = (const unsigned char *)data;
+  for (size_t i = 0; i < len; ++i) {
+    h ^= b[i];
+    h *= 0x100000001b3ull;
+  }
+  return h;
+}
+
+// Element strides of an n-by-k block in `layout` with leading dimension ld.
+int strides(int n, int k, int64_t ld, int layout, int64_t *rs, int64_t *cs) {
+  if (layout == SYMAMD_COL_MAJOR) {
+    if (k > 1 && ld < n) return fail(SYMAMD_ERR_INVALID_ARG, "column-major ld %lld < n %d", (long long)ld, n);
+    *rs = 1;
+    *cs = ld;
+  } else if (layout == SYMAMD_ROW_MAJOR) {
+    if (n > 1 && ld < k) return fail(SYMAMD_ERR_INVALID_ARG, "row-major ld %lld < k %d", (long long)ld, k);
+    *rs = ld;
+    *cs = 1;
+  } else {
+    return fail(SYMAMD_ERR_INVALID_ARG, "unknown layout %d", layout);
+  }
+  return SYMAMD_OK;
+}
+
+void free_plan_arrays(symamd_plan p) {
+  void *ptrs[] = {p->d_lvl_ptr, p->d_perm, p->d_rp,   p->d_ci,   p->d_val,
+                  p->d_fmap,    p->d_orp,  p->d_trp,  p->d_tci,  p->d_tval,
+                  p->d_tmap,    p->d_flags, p->d_state};
+  for (void *q : ptrs)
+    if (q) (void)hipFree(q);
+  for (auto &kv : p->units)
+    if (kv.second.d_ub) (void)hipFree(kv.second.d_ub);
+  for (auto &kv : p->gcache) {
+    if (kv.second.second) (void)hipGraphExecDestroy(kv.second.second);
+    if (kv.second.first) (void)hipGraphDestroy(kv.second.first);
+  }
+}
+
+// Default schedule when nothing was tuned.
+Choice heuristic(symamd_plan p, int k) {
+  Choice c;
+  c.exec = SYMAMD_EXEC_SYNCFREE;
+  if (k == 1) {
+    c.lanes = p->avg_row_nnz < 8.0 ? 1 : std::min(64, pow2_floor((int)(p->avg_row_nnz / 2)));
+  } else {
+    c.lanes = 1;
+  }
+  return c;
+}
+
+Choice choose(symamd_plan p, int k, int layout, int exec) {
+  Choice c;
+  auto it = p->tuned.find(tune_key(k, layout));
+  if (exec == SYMAMD_EXEC_AUTO) {
+    if (it != p->tuned.end()) c = it->second;
+    else c = heuristic(p, k);
+  } else {
+    c = heuristic(p, k);
+    c.exec = exec;
+    if (it != p->tuned.end() && it->second.exec == exec) c.lanes = it->second.lanes;
+  }
+  if (p->lanes_override > 0) c.lanes = p->lanes_override;
+  return c;
+}
+
+int check_handle(symamd_handle h) {
+  if (!h) return fail(SYMAMD_ERR_INVALID_ARG, "null handle");
+  return SYMAMD_OK;
+}
+
+int check_plan(symamd_plan p) {
+  if (!p) return fail(SYMAMD_ERR_INVALID_ARG, "null plan");
+  return SYMAMD_OK;
+}
+
+int analyze_common(symamd_handle h, int n, const int *d_Lp, const int *d_Li,
+                   const void *d_Lx, symamd_dtype dtype, symamd_plan *out) {
+  if (dtype != SYMAMD_F64 && dtype != SYMAMD_F32)
+    return fail(SYMAMD_ERR_INVALID_ARG, "unknown dtype %d", (int)dtype);
+  auto *p = new symamd_plan_s();
+  p->n = n;
+  p->dtype = dtype;
+  p->device = h->device;
+  int rc = SYMAMD_OK;
+  auto cleanup = [&](int code) {
+    free_plan_arrays(p);
+    delete p;
+    return code;
+  };
+  int nnz_i = 0;
+  int lp0 = 0;
+  if (hipMemcpy(&nnz_i, d_Lp + n, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(&lp0, d_Lp, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+    return cleanup(fail(SYMAMD_ERR_HIP, "cannot read Lp from the device"));
+  if (lp0 != 0 || nnz_i < n) return cleanup(fail(SYMAMD_ERR_STRUCTURE, "Lp[0] must be 0 and Lp[n] >= n"));
+  p->nnz = nnz_i;
+  const size_t vs = dtype == SYMAMD_F64 ? 8 : 4;
+  const size_t nn = (size_t)(n > 0 ? n : 1);
+  const size_t mm = (size_t)(p->nnz > 0 ? p->nnz : 1);
+  if (hipMalloc(&p->d_state, sizeof(int) * ST_WORDS) != hipSuccess ||
+      hipMalloc(&p->d_perm, sizeof(int) * nn) != hipSuccess ||
+      hipMalloc(&p->d_lvl_ptr, sizeof(int) * (nn + 1)) != hipSuccess ||
+      hipMalloc(&p->d_rp, sizeof(int) * (nn + 1)) != hipSuccess ||
+      hipMalloc(&p->d_orp, sizeof(int) * (nn + 1)) != hipSuccess ||
+      hipMalloc(&p->d_ci, sizeof(int) * mm) != hipSuccess ||
+      hipMalloc(&p->d_fmap, sizeof(int) * mm) != hipSuccess ||
+      hipMalloc(&p->d_val, vs * mm) != hipSuccess ||
+      hipMalloc(&p->d_flags, sizeof(int) * nn) != hipSuccess)
+    return cleanup(fail(SYMAMD_ERR_ALLOC, "device allocation for the plan failed (n=%d nnz=%lld)", n, (long long)p->nnz));
+  if ((rc = check_structure(h, n, d_Lp, d_Li, p->d_state)) != SYMAMD_OK) return cleanup(rc);
+  if ((rc = compute_levels(h, n, d_Lp, d_Li, p->d_state, p->d_lvl_ptr, p->d_perm,
+                           &p->levels, &p->h_lvl_ptr)) != SYMAMD_OK)
+    return cleanup(rc);
+  if ((rc = build_plan_arrays(h, p, d_Lp, d_Li, d_Lx)) != SYMAMD_OK) return cleanup(rc);
+  if ((rc = gather_values(h, p, d_Lx)) != SYMAMD_OK) return cleanup(rc);
+  if (hipMemsetAsync(p->d_flags, 0, sizeof(int) * nn, h->stream) != hipSuccess ||
+      hipMemsetAsync(p->d_state, 0, sizeof(int) * ST_WORDS, h->stream) != hipSuccess ||
+      hipStreamSynchronize(h->stream) != hipSuccess)
+    return cleanup(fail(SYMAMD_ERR_HIP, "plan state initialisation failed"));
+  for (int l = 0; l < p->levels; ++l)
+    p->max_level_rows = std::max(p->max_level_rows, p->h_lvl_ptr[l + 1] - p->h_lvl_ptr[l]);
+  p->avg_row_nnz = n > 0 ? (double)p->nnz / n : 1.0;
+  uint64_t fp = 0xcbf29ce484222325ull;
+  fp = fnv1a(fp, &n, sizeof(n));
+  fp = fnv1a(fp, &p->nnz, sizeof(p->nnz));
+  fp = fnv1a(fp, &dtype, sizeof(dtype));
+  fp = fnv1a(fp, p->h_lvl_ptr.data(), sizeof(int) * p->h_lvl_ptr.size());
+  p->fingerprint = fp;
+  auto it = g_tune_cache.find(fp);
+  if (it != g_tune_cache.end()) p->tuned = it->second;
+  *out = p;
+  return SYMAMD_OK;
+}
+
+// Adds each row's system offset to its local column ids.
+__global__ void k_globalise(int n, int nsys, const int *__restrict__ off,
+                            const int *__restrict__ Lp,
+                            const int *__restrict__ Li, int *Gi) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int lo = 0, hi = nsys - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= i) lo = mid; else hi = mid - 1;
+  }
+  const int base = off[lo];
+  for (int j = Lp[i]; j < Lp[i + 1]; ++j) Gi[j] = Li[j] + base;
+}
+
+__global__ void k_fill(int64_t m, double v, void *dst, int is64) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= m) return;
+  if (is64) ((double *)dst)[t] = v;
+  else ((float *)dst)[t] = (float)v;
+}
+
+int solve_impl(symamd_handle h, symamd_plan p, void *d_B, int k, int64_t ld,
+               int layout, int exec, bool graph_ok);
+
+}  // namespace
+
+void set_error(const char *fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+}
+
+int fail(int code, const char *fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+// --------------------------------------------------------------- coarsen
+// Host implementation of the coarsened schedule.  Levels come from the GPU
+// inspector; `span` consecutive levels form one coarse level; inside it the
+// rows are grouped into the connected components of the dependency edges
+// that stay inside the coarse level (union-find), so no dependency crosses
+// two partitions of one coarse level; components are then packed in order
+// of their first row into partitions of cost (row nnz) at most max_cost.
+// Rows inside a partition are listed by (level, row id), which respects
+// every dependency inside it.
+int coarsen(symamd_handle h, int n, const int *d_Lp, const int *d_Li, int span,
+            int max_cost, int *d_level_ptr, int *d_par_ptr, int *d_partition,
+            int *level_no, int *part_no) {
+  if (span < 1) span = 1;
+  std::vector<int> Lp(n + 1), Li, lp, ls(n > 0 ? n : 1);
+  SYMAMD_HIP(hipMemcpy(Lp.data(), d_Lp, sizeof(int) * (n + 1), hipMemcpyDeviceToHost));
+  Li.resize(Lp[n] > 0 ? Lp[n] : 1);
+  SYMAMD_HIP(hipMemcpy(Li.data(), d_Li, sizeof(int) * Lp[n], hipMemcpyDeviceToHost));
+  int *d_state = nullptr, *d_lp = nullptr, *d_ls = nullptr;
+  SYMAMD_HIP(hipMalloc(&d_state, sizeof(int) * ST_WORDS));
+  SYMAMD_HIP(hipMalloc(&d_lp, sizeof(int) * (n + 1)));
+  SYMAMD_HIP(hipMalloc(&d_ls, sizeof(int) * (n > 0 ? n : 1)));
+  int levels = 0;
+  int rc = check_structure(h, n, d_Lp, d_Li, d_state);
+  if (rc == SYMAMD_OK)
+    rc = compute_levels(h, n, d_Lp, d_Li, d_state, d_lp, d_ls, &levels, &lp);
+  if (rc == SYMAMD_OK && n > 0)
+    if (hipMemcpy(ls.data(), d_ls, sizeof(int) * n, hipMemcpyDeviceToHost) != hipSuccess)
+      rc = fail(SYMAMD_ERR_HIP, "copy of level set failed");
+  (void)hipFree(d_state);
+  (void)hipFree(d_lp);
+  (void)hipFree(d_ls);
+  if (rc != SYMAMD_OK) return rc;
+  std::vector<int> lev(n > 0 ? n : 1), parent(n > 0 ? n : 1);
+  for (int l = 0; l < levels; ++l)
+    for (int q = lp[l]; q < lp[l + 1]; ++q) lev[ls[q]] = l;
+  auto find = [&](int x) {
+    while (parent[x] != x) {
+      parent[x] = parent[parent[x]];
+      x = parent[x];
+    }
+    return x;
+  };
+  for (int i = 0; i < n; ++i) parent[i] = i;
+  for (int i = 0; i < n; ++i)
+    for (int j = Lp[i]; j < Lp[i + 1] - 1; ++j) {
+      int d = Li[j];
+      if (lev[d] / span == lev[i] / span) {
+        int a = find(i), b = find(d);
+        if (a != b) parent[std::max(a, b)] = std::min(a, b);
+      }
+    }
+  const int ncoarse = (levels + span - 1) / span;
+  std::vector<int> out_lp(1, 0), out_pp(1, 0), out_part;
+  out_part.reserve(n);
+  std::vector<int> comp_slot(n > 0 ? n : 1, -1);
+  for (int cl = 0; cl < ncoarse; ++cl) {
+    const int q0 = lp[cl * span], q1 = lp[std::min(levels, (cl + 1) * span)];
+    // components in order of first appearance (level order)
+    std::vector<int> roots;
+    std::vector<std::vector<int>> members;
+    std::vector<long long> cost;
+    for (int q = q0; q < q1; ++q) {
+      int i = ls[q], r = find(i);
+      if (comp_slot[r] < 0) {
+        comp_slot[r] = (int)roots.size();
+        roots.push_back(r);
+        members.emplace_back();
+        cost.push_back(0);
+      }
+      int s = comp_slot[r];
+      members[s].push_back(i);  // ls is (level, row) ordered already
+      cost[s] += Lp[i + 1] - Lp[i];
+    }
+    int parts_here = 0;
+    long long acc = 0;
+    std::vector<int> cur;
+    auto flush = [&]() {
+      if (cur.empty()) return;
+      // merged components keep (level, row id) order
+      std::stable_sort(cur.begin(), cur.end(), [&](int a, int b) {
+        return lev[a] != lev[b] ? lev[a] < lev[b] : a < b;
+      });
+      out_part.insert(out_part.end(), cur.begin(), cur.end());
+      out_pp.push_back((int)out_part.size());
+      ++parts_here;
+      cur.clear();
+      acc = 0;
+    };
+    for (size_t s = 0; s < roots.size(); ++s) {
+      if (!cur.empty() && max_cost > 0 && acc + cost[s] > max_cost) flush();
+      cur.insert(cur.end(), members[s].begin(), members[s].end());
+      acc += cost[s];
+    }
+    flush();
+    for (int r : roots) comp_slot[r] = -1;
+    out_lp.push_back(out_lp.back() + parts_here);
+  }
+  *level_no = ncoarse;
+  *part_no = out_lp.back();
+  SYMAMD_HIP(hipMemcpy(d_level_ptr, out_lp.data(), sizeof(int) * out_lp.size(), hipMemcpyHostToDevice));
+  SYMAMD_HIP(hipMemcpy(d_par_ptr, out_pp.data(), sizeof(int) * out_pp.size(), hipMemcpyHostToDevice));
+  if (n > 0)
+    SYMAMD_HIP(hipMemcpy(d_partition, out_part.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+  return SYMAMD_OK;
+}
+
+namespace {
+
+std::string graph_key(void *d_B, int k, int64_t rs, int64_t cs, const Choice &c) {
+  char buf[160];
+  snprintf(buf, sizeof buf, "%p/%d/%lld/%lld/%d/%d", d_B, k, (long long)rs,
+           (long long)cs, c.exec, c.lanes);
+  return buf;
+}
+
+// Record the per-level launches of one solve into a graph on a private
+// capture stream.  Units for the sync-free path are prepared beforehand so
+// nothing allocates while capturing.
+int capture_solve(symamd_handle h, symamd_plan p, void *d_B, int k, int64_t rs,
+                  int64_t cs, const Choice &c, hipGraph_t *g, hipGraphExec_t *ge) {
+  if (c.exec == SYMAMD_EXEC_SYNCFREE) {
+    UnitSet *us = nullptr;
+    SYMAMD_TRY(plan_units(h, p, forward_rows_per_unit(k, c.lanes), &us));
+  }
+  hipStream_t cap = nullptr;
+  SYMAMD_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+  hipStream_t saved = h->stream;
+  h->stream = cap;
+  int rc = SYMAMD_OK;
+  if (hipStreamBeginCapture(cap, hipStreamCaptureModeRelaxed) != hipSuccess) {
+    rc = fail(SYMAMD_ERR_HIP, "hipStreamBeginCapture failed");
+  } else {
+    rc = launch_forward(h, p, d_B, k, rs, cs,
+                        c.exec == SYMAMD_EXEC_LEVELSET_GRAPH ? SYMAMD_EXEC_LEVELSET : c.exec,
+                        c.lanes);
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(cap, &graph);
+    if (rc == SYMAMD_OK && e != hipSuccess)
+      rc = fail(SYMAMD_ERR_HIP, "hipStreamEndCapture failed: %s", hipGetErrorString(e));
+    if (rc == SYMAMD_OK) {
+      e = hipGraphInstantiate(ge, graph, nullptr, nullptr, 0);
+      if (e != hipSuccess) rc = fail(SYMAMD_ERR_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+      *g = graph;
+    } else if (graph) {
+      (void)hipGraphDestroy(graph);
+    }
+  }
+  h->stream = saved;
+  (void)hipStreamDestroy(cap);
+  return rc;
+}
+
+int solve_impl(symamd_handle h, symamd_plan p, void *d_B, int k, int64_t ld,
+               int layout, int exec, bool graph_ok) {
+  int64_t rs = 0, cs = 0;
+  SYMAMD_TRY(strides(p->n, k, ld, layout, &rs, &cs));
+  if (k == 0 || p->n == 0) return SYMAMD_OK;
+  Choice c = choose(p, k, layout, exec);
+  if (c.exec == SYMAMD_EXEC_LEVELSET_GRAPH && graph_ok) {
+    std::string key = graph_key(d_B, k, rs, cs, c);
+    auto it = p->gcache.find(key);
+    if (it == p->gcache.end()) {
+      hipGraph_t g = nullptr;
+      hipGraphExec_t ge = nullptr;
+      SYMAMD_TRY(capture_solve(h, p, d_B, k, rs, cs, c, &g, &ge));
+      it = p->gcache.emplace(key, std::make_pair(g, ge)).first;
+    }
+    SYMAMD_HIP(hipGraphLaunch(it->second.second, h->stream));
+    return SYMAMD_OK;
+  }
+  int e = c.exec == SYMAMD_EXEC_LEVELSET_GRAPH ? SYMAMD_EXEC_LEVELSET : c.exec;
+  return launch_forward(h, p, d_B, k, rs, cs, e, c.lanes);
+}
+
+}  // namespace
+}  // namespace symamd
+
+using namespace symamd;
+
+extern "C" {
+
+const char *symamd_last_error(void) { return g_last_error.c_str(); }
+
+const char *symamd_version(void) { return "sympiler_amd 0.1 (gfx950)"; }
+
+int symamd_create(symamd_handle *h, void *stream) {
+  if (!h) return fail(SYMAMD_ERR_INVALID_ARG, "null handle pointer");
+  auto *x = new symamd_handle_s();
+  x->stream = (hipStream_t)stream;
+  if (hipGetDevice(&x->device) != hipSuccess) {
+    delete x;
+    return fail(SYMAMD_ERR_HIP, "no HIP device");
+  }
+  if (hipDeviceGetAttribute(&x->num_cus, hipDeviceAttributeMultiprocessorCount,
+                            x->device) != hipSuccess ||
+      x->num_cus < 1)
+    x->num_cus = 256;
+  *h = x;
+  return SYMAMD_OK;
+}
+
+int symamd_destroy(symamd_handle h) {
+  delete h;
+  return SYMAMD_OK;
+}
+
+int symamd_set_stream(symamd_handle h, void *stream) {
+  SYMAMD_TRY(check_handle(h));
+  h->stream = (hipStream_t)stream;
+  return SYMAMD_OK;
+}
+
+int symamd_set_timeout_ms(symamd_handle h, double ms) {
+  SYMAMD_TRY(check_handle(h));
+  if (!(ms > 0)) return fail(SYMAMD_ERR_INVALID_ARG, "timeout must be positive");
+  h->timeout_ms = ms;
+  return SYMAMD_OK;
+}
+
+int symamd_levelset(symamd_handle h, int n, const int *d_Lp, const int *d_Li,
+                    int *d_level_ptr, int *d_level_set, int *levels) {
+  SYMAMD_TRY(check_handle(h));
+  if (n < 0 || !levels || (n > 0 && (!d_Lp || !d_Li || !d_level_ptr || !d_level_set)))
+    return fail(SYMAMD_ERR_INVALID_ARG, "symamd_levelset: bad arguments");
+  int *d_state = nullptr;
+  SYMAMD_HIP(hipMalloc(&d_state, sizeof(int) * ST_WORDS));
+  int rc = check_structure(h, n, d_Lp, d_Li, d_state);
+  if (rc == SYMAMD_OK)
+    rc = compute_levels(h, n, d_Lp, d_Li, d_state, d_level_ptr, d_level_set,
+                        levels, nullptr);
+  (void)hipFree(d_state);
+  return rc;
+}
+
+int symamd_coarsen(symamd_handle h, int n, const int *d_Lp, const int *d_Li,
+                   int span, int max_part_cost, int *d_level_ptr,
+                   int *d_par_ptr, int *d_partition, int *level_no,
+                   int *part_no) {
+  SYMAMD_TRY(check_handle(h));
+  if (n < 0 || !level_no || !part_no || !d_Lp || !d_level_ptr || !d_par_ptr ||
+      (n > 0 && (!d_Li || !d_partition)))
+    return fail(SYMAMD_ERR_INVALID_ARG, "symamd_coarsen: bad arguments");
+  return coarsen(h, n, d_Lp, d_Li, span, max_part_cost, d_level_ptr, d_par_ptr,
+                 d_partition, level_no, part_no);
+}
+
+int symamd_analyze(symamd_handle h, int n, const int *d_Lp, const int *d_Li,
+                   const void *d_Lx, symamd_dtype dtype, symamd_plan *plan) {
+  SYMAMD_TRY(check_handle(h));
+  if (n < 0 || !plan || !d_Lp || (n > 0 && (!d_Li || !d_Lx)))
+    return fail(SYMAMD_ERR_INVALID_ARG, "symamd_analyze: bad arguments");
+  return analyze_common(h, n, d_Lp, d_Li, d_Lx, dtype, plan);
+}
+
+int symamd_analyze_batched(symamd_handle h, int nsys, const int *h_row_off,
+                           const int *d_Lp, const int *d_Li, const void *d_Lx,
+                           symamd_dtype dtype, symamd_plan *plan) {
+  SYMAMD_TRY(check_handle(h));
+  if (nsys < 1 || !h_row_off || !plan || !d_Lp)
+    return fail(SYMAMD_ERR_INVALID_ARG, "symamd_analyze_batched: bad arguments");
+  for (int s = 0; s < nsys; ++s)
+    if (h_row_off[s + 1] < h_row_off[s])
+      return fail(SYMAMD_ERR_INVALID_ARG, "row offsets must be non-decreasing");
+  const int n = h_row_off[nsys];
+  int nnz = 0;
+  SYMAMD_HIP(hipMemcpy(&nnz, d_Lp + n, sizeof(int), hipMemcpyDeviceToHost));
+  int *d_off = nullptr, *d_Gi = nullptr;
+  SYMAMD_HIP(hipMalloc(&d_off, sizeof(int) * (nsys + 1)));
+  SYMAMD_HIP(hipMalloc(&d_Gi, sizeof(int) * (nnz > 0 ? nnz : 1)));
+  SYMAMD_HIP(hipMemcpy(d_off, h_row_off, sizeof(int) * (nsys + 1), hipMemcpyHostToDevice));
+  if (n > 0) {
+    k_globalise<<<(n + 255) / 256, 256, 0, h->stream>>>(n, nsys, d_off, d_Lp, d_Li, d_Gi);
+    SYMAMD_HIP(hipGetLastError());
+  }
+  int rc = analyze_common(h, n, d_Lp, d_Gi, d_Lx, dtype, plan);
+  (void)hipStreamSynchronize(h->stream);
+  (void)hipFree(d_off);
+  (void)hipFree(d_Gi);
+  if (rc == SYMAMD_OK) {
+    (*plan)->nsys = nsys;
+    (*plan)->h_row_off.assign(h_row_off, h_row_off + nsys + 1);
+  }
+  return rc;
+}
+
+int symamd_plan_update_values(symamd_handle h, symamd_plan plan, const void *d_Lx) {
+  SYMAMD_TRY(check_handle(h));
+  SYMAMD_TRY(check_plan(plan));
+  if (plan->nnz > 0 && !d_Lx) return fail(SYMAMD_ERR_INVALID_ARG, "null values");
+  return gather_values(h, plan, d_Lx);
+}
+
+int symamd_plan_info(symamd_plan plan, int *n, int64_t *nnz, int *levels,
+                     int *max_level_rows) {
+  SYMAMD_TRY(check_plan(plan));
+  if (n) *n = plan->n;
+  if (nnz) *nnz = plan->nnz;
+  if (levels) *levels = plan->levels;
+  if (max_level_rows) *max_level_rows = plan->max_level_rows;
+  return SYMAMD_OK;
+}
+
+int symamd_plan_levels(symamd_plan plan, int *h_level_ptr, int *h_level_set) {
+  SYMAMD_TRY(check_plan(plan));
+  if (h_level_ptr)
+    std::copy(plan->h_lvl_ptr.begin(), plan->h_lvl_ptr.end(), h_level_ptr);
+  if (h_level_set && plan->n > 0)
+    SYMAMD_HIP(hipMemcpy(h_level_set, plan->d_perm, sizeof(int) * plan->n,
+                         hipMemcpyDeviceToHost));
+  return SYMAMD_OK;
+}
+
+int symamd_plan_destroy(symamd_plan plan) {
+  if (!plan) return SYMAMD_OK;
+  (void)hipDeviceSynchronize();
+  free_plan_arrays(plan);
+  delete plan;
+  return SYMAMD_OK;
+}
+
+int symamd_plan_set_lanes(symamd_plan plan, int lanes_per_row) {
+  SYMAMD_TRY(check_plan(plan));
+  if (lanes_per_row < 0 || lanes_per_row > 64 ||
+      (lanes_per_row & (lanes_per_row - 1)) != 0)
+    return fail(SYMAMD_ERR_INVALID_ARG, "lanes per row must be 0 or a power of two <= 64");
+  plan->lanes_override = lanes_per_row;
+  return SYMAMD_OK;
+}
+
+int symamd_solve(symamd_handle h, symamd_plan plan, void *d_B, int k, int64_t ld,
+                 symamd_layout layout, symamd_executor exec) {
+  SYMAMD_TRY(check_handle(h));
+  SYMAMD_TRY(check_plan(plan));
+  if (k < 0 || (k > 0 && plan->n > 0 && !d_B))
+    return fail(SYMAMD_ERR_INVALID_ARG, "symamd_solve: bad arguments");
+  if (exec < SYMAMD_EXEC_AUTO || exec > SYMAMD_EXEC_LEVELSET_GRAPH)
+    return fail(SYMAMD_ERR_INVALID_ARG, "unknown executor %d", (int)exec);
+  return solve_impl(h, plan, d_B, k, ld, layout, exec, true);
+}
+
+int symamd_plan_check(symamd_handle h, symamd_plan plan) {
+  SYMAMD_TRY(check_handle(h));
+  SYMAMD_TRY(check_plan(plan));
+  int st[ST_WORDS];
+  SYMAMD_HIP(hipStreamSynchronize(h->stream));
+  SYMAMD_HIP(hipMemcpy(st, plan->d_state, sizeof(st), hipMemcpyDeviceToHost));
+  if (st[ST_ERROR]) return fail(SYMAMD_ERR_TIMEOUT, "a sync-free wait exceeded the watchdog");
+  return SYMAMD_OK;
+}
+
+int symamd_graph_create(symamd_handle h, symamd_plan plan, void *d_B, int k,
+                        int64_t ld, symamd_layout layout, symamd_executor exec,
+                        symamd_graph *g) {
+  SYMAMD_TRY(check_handle(h));
+  SYMAMD_TRY(check_plan(plan));
+  if (!g || k < 1 || !d_B) return fail(SYMAMD_ERR_INVALID_ARG, "symamd_graph_create: bad arguments");
+  int64_t rs = 0, cs = 0;
+  SYMAMD_TRY(strides(plan->n, k, ld, layout, &rs, &cs));
+  Choice c = choose(plan, k, layout, exec);
+  auto *x = new symamd_graph_s();
+  x->plan = plan;
+  int rc = capture_solve(h, plan, d_B, k, rs, cs, c, &x->graph, &x->exec);
+  if (rc != SYMAMD_OK) {
+    delete x;
+    return rc;
+  }
+  *g = x;
+  return SYMAMD_OK;
+}
+
+int symamd_graph_launch(symamd_handle h, symamd_graph g) {
+  SYMAMD_TRY(check_handle(h));
+  if (!g) return fail(SYMAMD_ERR_INVALID_ARG, "null graph");
+  SYMAMD_HIP(hipGraphLaunch(g->exec, h->stream));
+  return SYMAMD_OK;
+}
+
+int symamd_graph_destroy(symamd_graph g) {
+  if (!g) return SYMAMD_OK;
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  delete g;
+  return SYMAMD_OK;
+}
+
+int symamd_adjoint(symamd_handle h, symamd_plan plan, const void *d_x,
+                   const void *d_gx, void *d_gb, void *d_gLx,
+                   symamd_executor exec) {
+  SYMAMD_TRY(check_handle(h));
+  SYMAMD_TRY(check_plan(plan));
+  if (plan->n > 0 && (!d_x || !d_gx || !d_gb))
+    return fail(SYMAMD_ERR_INVALID_ARG, "symamd_adjoint: bad arguments");
+  Choice c = choose(plan, 1, SYMAMD_COL_MAJOR, exec);
+  int e = c.exec == SYMAMD_EXEC_LEVELSET_GRAPH ? SYMAMD_EXEC_LEVELSET : c.exec;
+  return launch_adjoint(h, plan, d_x, d_gx, d_gb, d_gLx, e, c.lanes);
+}
+
+int symamd_spmv_csr(symamd_handle h, int n, const int *d_Ap, const int *d_Ai,
+                    const double *d_Ax, const double *d_x, double *d_y) {
+  SYMAMD_TRY(check_handle(h));
+  if (n < 0 || (n > 0 && (!d_Ap || !d_Ai || !d_Ax || !d_x || !d_y)))
+    return fail(SYMAMD_ERR_INVALID_ARG, "symamd_spmv_csr: bad arguments");
+  return launch_spmv(h, n, d_Ap, d_Ai, d_Ax, d_x, nullptr, d_y, 0, 0, 0);
+}
+
+int symamd_spmv_csr_axpby(symamd_handle h, int n, const int *d_Ap,
+                          const int *d_Ai, const double *d_Ax,
+                          const double *d_x, const double *d_y, double *d_z,
+                          double a, double b) {
+  SYMAMD_TRY(check_handle(h));
+  if (n < 0 || (n > 0 && (!d_Ap || !d_Ai || !d_Ax || !d_x || !d_y || !d_z)))
+    return fail(SYMAMD_ERR_INVALID_ARG, "symamd_spmv_csr_axpby: bad arguments");
+  return launch_spmv(h, n, d_Ap, d_Ai, d_Ax, d_x, d_y, d_z, a, b, 1);
+}
+
+int symamd_residual(symamd_handle h, symamd_plan plan, const void *d_X,
+                    const void *d_B, int k, int64_t ld, symamd_layout layout,
+                    double *h_rel) {
+  SYMAMD_TRY(check_handle(h));
+  SYMAMD_TRY(check_plan(plan));
+  if (k < 0 || !h_rel || (k > 0 && plan->n > 0 && (!d_X || !d_B)))
+    return fail(SYMAMD_ERR_INVALID_ARG, "symamd_residual: bad arguments");
+  int64_t rs = 0, cs = 0;
+  SYMAMD_TRY(strides(plan->n, k, ld, layout, &rs, &cs));
+  return plan_residual(h, plan, d_X, d_B, k, rs, cs, h_rel);
+}
+
+int symamd_tune(symamd_handle h, symamd_plan plan, int k, symamd_layout layout,
+                char *summary, int summary_len) {
+  SYMAMD_TRY(check_handle(h));
+  SYMAMD_TRY(check_plan(plan));
+  if (k < 1) return fail(SYMAMD_ERR_INVALID_ARG, "k must be >= 1");
+  if (plan->n == 0) return SYMAMD_OK;
+  const int64_t ld = layout == SYMAMD_COL_MAJOR ? plan->n : k;
+  const size_t vs = plan->dtype == SYMAMD_F64 ? 8 : 4;
+  const int64_t m = (int64_t)plan->n * k;
+  void *d_B = nullptr;
+  SYMAMD_HIP(hipMalloc(&d_B, vs * m));
+  k_fill<<<(unsigned)((m + 255) / 256), 256, 0, h->stream>>>(m, 1.0, d_B, vs == 8);
+  hipEvent_t e0, e1;
+  SYMAMD_HIP(hipEventCreate(&e0));
+  SYMAMD_HIP(hipEventCreate(&e1));
+  const int kt = pow2_ceil(k < 64 ? k : 64);
+  const int gmax_rows = std::max(1, pow2_ceil((int)std::ceil(plan->avg_row_nnz)) * 2);
+  std::string log;
+  Choice best;
+  float best_ms = 1e30f;
+  const int saved_override = plan->lanes_override;
+  plan->lanes_override = 0;
+  const int execs[] = {SYMAMD_EXEC_SYNCFREE, SYMAMD_EXEC_LEVELSET_GRAPH};
+  int rc = SYMAMD_OK;
+  for (int ex : execs) {
+    for (int g = 1; g * kt <= 64 && g <= std::min(64, gmax_rows); g *= 2) {
+      Choice c;
+      c.exec = ex;
+      c.lanes = g;
+      plan->tuned[tune_key(k, layout)] = c;
+      rc = solve_impl(h, plan, d_B, k, ld, layout, SYMAMD_EXEC_AUTO, true);  // warm / capture
+      if (rc != SYMAMD_OK) break;
+      float ms = 1e30f;
+      for (int rep = 0; rep < 3 && rc == SYMAMD_OK; ++rep) {
+        (void)hipEventRecord(e0, h->stream);
+        rc = solve_impl(h, plan, d_B, k, ld, layout, SYMAMD_EXEC_AUTO, true);
+        (void)hipEventRecord(e1, h->stream);
+        (void)hipEventSynchronize(e1);
+        float t = 0;
+        (void)hipEventElapsedTime(&t, e0, e1);
+        ms = std::min(ms, t);
+      }
+      if (rc != SYMAMD_OK) break;
+      char line[128];
+      snprintf(line, sizeof line, "exec=%d lanes=%d ms=%.4f\n", ex, g, ms);
+      log += line;
+      if (ms < best_ms * 0.97f) {  // prefer earlier (fewer lanes, sync-free) on ties
+        best_ms = ms;
+        best = c;
+      }
+    }
+    if (rc != SYMAMD_OK) break;
+  }
+  plan->lanes_override = saved_override;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipStreamSynchronize(h->stream);
+  // drop the graphs captured on the scratch block
+  char prefix[32];
+  snprintf(prefix, sizeof prefix, "%p/", d_B);
+  for (auto it = plan->gcache.begin(); it != plan->gcache.end();) {
+    if (it->first.compare(0, strlen(prefix), prefix) == 0) {
+      (void)hipGraphExecDestroy(it->second.second);
+      (void)hipGraphDestroy(it->second.first);
+      it = plan->gcache.erase(it);
+    } else {
+      ++it;
+    }
+  }
+  (void)hipFree(d_B);
+  if (rc != SYMAMD_OK) {
+    plan->tuned.erase(tune_key(k, layout));
+    return rc;
+  }
+  plan->tuned[tune_key(k, layout)] = best;
+  g_tune_cache[plan->fingerprint][tune_key(k, layout)] = best;
+  char line[128];
+  snprintf(line, sizeof line, "best exec=%d lanes=%d ms=%.4f\n", best.exec, best.lanes, best_ms);
+  log += line;
+  if (summary && summary_len > 0) {
+    strncpy(summary, log.c_str(), summary_len - 1);
+    summary[summary_len - 1] = 0;
+  }
+  SYMAMD_TRY(symamd_plan_check(h, plan));
+  return SYMAMD_OK;
+}
+
+int symamd_solve_lbc(symamd_handle h, int n, const int *d_Lp, const int *d_Li,
+                     const double *d_Lx, double *d_x, int level_no,
+                     const int *h_level_ptr, const int *d_par_ptr,
+                     const int *d_partition) {
+  SYMAMD_TRY(check_handle(h));
+  if (n < 0 || level_no < 0 || (level_no > 0 && (!h_level_ptr || !d_par_ptr || !d_partition)) ||
+      (n > 0 && (!d_Lp || !d_Li || !d_Lx || !d_x)))
+    return fail(SYMAMD_ERR_INVALID_ARG, "symamd_solve_lbc: bad arguments");
+  return launch_lbc(h, n, d_Lp, d_Li, d_Lx, d_x, level_no, h_level_ptr,
+                    d_par_ptr, d_partition);
+}
+
+}  // extern "C"

@@ -1,0 +1,211 @@
+// common.h -- internal definitions shared by the HIP translation units of
+// libsympiler_amd.so: status plumbing, device-coherent memory helpers, the
+// plan structure and kernel-launch declarations.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/sympiler_amd.h"
+
+namespace symamd {
+
+// ------------------------------------------------------------ error state
+void set_error(const char *fmt, ...);
+int fail(int code, const char *fmt, ...);
+
+#define SYMAMD_HIP(call)                                                   \
+  do {                                                                     \
+    hipError_t e_ = (call);                                                \
+    if (e_ != hipSuccess)                                                  \
+      return ::symamd::fail(SYMAMD_ERR_HIP, "%s failed: %s (%s:%d)", #call, \
+                            hipGetErrorString(e_), __FILE__, __LINE__);    \
+  } while (0)
+
+#define SYMAMD_TRY(expr)         \
+  do {                           \
+    int s_ = (expr);             \
+    if (s_ != SYMAMD_OK) return s_; \
+  } while (0)
+
+// ------------------------------------------------- device memory helpers
+//
+// Cross-workgroup communication inside one kernel (ready flags and the
+// values they guard) uses agent-scope relaxed atomics: on gfx950 these are
+// loads/stores with device scope that bypass the non-coherent per-XCD
+// caches, so a value written by a wave on one XCD is seen by a wave on
+// another without cache maintenance.  The producer orders "values, then
+// flag" with a wait for its outstanding stores (publish_fence).
+
+template <typename T>
+__device__ __forceinline__ T ld_dev(const T *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ void st_dev(T *p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// All stores this wave issued have completed (and, being device-scope, are
+// visible device-wide) before any later memory instruction issues.
+__device__ __forceinline__ void publish_fence() {
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+}
+
+// Non-temporal streaming load for read-once matrix data.
+template <typename T>
+__device__ __forceinline__ T ld_stream(const T *p) {
+  return __builtin_nontemporal_load(p);
+}
+
+// 100 MHz constant clock, used by the sync-free watchdog.
+__device__ __forceinline__ uint64_t wall_ticks() {
+  return __builtin_amdgcn_s_memrealtime();
+}
+
+// Shared state of one plan's persistent kernels (device memory, int32):
+//   [0] epoch   -- launches completed; flags equal to epoch+1 are "ready"
+//   [1] ticket  -- next work unit to hand out
+//   [2] done    -- waves that finished this launch
+//   [3] error   -- sticky watchdog flag
+//   [4] nlev    -- scratch for the inspector
+enum { ST_EPOCH = 0, ST_TICKET = 1, ST_DONE = 2, ST_ERROR = 3, ST_SCRATCH = 4,
+       ST_WORDS = 8 };
+
+// Called by every wave at the end of a persistent launch: the last wave
+// resets the ticket and done counters and advances the epoch, so the next
+// launch (stream-ordered) starts clean without any host-side memset.  This
+// keeps the kernels replayable from a captured hipGraph.
+__device__ __forceinline__ void persistent_finish(int *state, int total_waves) {
+  int lane = threadIdx.x & 63;
+  if (lane == 0) {
+    int old = atomicAdd(&state[ST_DONE], 1);
+    if (old == total_waves - 1) {
+      state[ST_TICKET] = 0;
+      state[ST_DONE] = 0;
+      __hip_atomic_store(&state[ST_EPOCH], state[ST_EPOCH] + 1,
+                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
+// ------------------------------------------------------------------ plan
+
+struct UnitSet {
+  int rows = 0;          // rows per unit (cap)
+  int count = 0;         // number of units
+  int *d_ub = nullptr;   // unit u covers positions [ub[u], ub[u+1])
+};
+
+struct Choice {
+  int exec = SYMAMD_EXEC_SYNCFREE;
+  int lanes = 1;  // lanes cooperating on one row's nonzeros
+};
+
+}  // namespace symamd
+
+struct symamd_handle_s {
+  hipStream_t stream = nullptr;
+  int device = 0;
+  int num_cus = 0;
+  double timeout_ms = 20000.0;
+  uint64_t timeout_ticks() const { return (uint64_t)(timeout_ms * 1.0e5); }
+};
+
+struct symamd_plan_s {
+  int n = 0;
+  int64_t nnz = 0;
+  int levels = 0;
+  int max_level_rows = 0;
+  symamd_dtype dtype = SYMAMD_F64;
+  int device = 0;
+  std::vector<int> h_lvl_ptr;          // levels+1
+  int *d_lvl_ptr = nullptr;            // levels+1
+  int *d_perm = nullptr;               // position -> row (level set)
+  int *d_rp = nullptr;                 // level-ordered row pointers, n+1
+  int *d_ci = nullptr;                 // level-ordered column ids (row ids), nnz
+  void *d_val = nullptr;               // level-ordered values, nnz
+  int *d_fmap = nullptr;               // caller CSR position of each entry
+  int *d_orp = nullptr;                // caller CSR row pointers, n+1
+  // transposed copy (CSC of L, level order), built on first adjoint
+  int *d_trp = nullptr;
+  int *d_tci = nullptr;
+  void *d_tval = nullptr;
+  int *d_tmap = nullptr;
+  // persistent-kernel state
+  int *d_flags = nullptr;              // n ready tags
+  int *d_state = nullptr;              // ST_WORDS ints
+  std::map<int, symamd::UnitSet> units;  // rows-per-unit -> units
+  // schedule choices
+  int lanes_override = 0;
+  std::map<int64_t, symamd::Choice> tuned;  // key(k, layout) -> choice
+  uint64_t fingerprint = 0;
+  double avg_row_nnz = 1.0;
+  int max_row_nnz = 1;
+  // batch bookkeeping
+  int nsys = 1;
+  std::vector<int> h_row_off;
+  // captured per-level launch sequences, keyed by (B, k, strides, lanes)
+  std::map<std::string, std::pair<hipGraph_t, hipGraphExec_t>> gcache;
+};
+
+struct symamd_graph_s {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  symamd_plan plan = nullptr;
+};
+
+namespace symamd {
+
+// Ensures the unit partition with `rows` rows per unit exists.
+int plan_units(symamd_handle h, symamd_plan p, int rows, UnitSet **out);
+// Ensures the transposed (CSC) copy exists.
+int plan_transpose(symamd_handle h, symamd_plan p);
+
+// Kernel launchers (executors.hip / adjoint.hip / spmv.hip / inspector.hip)
+int launch_forward(symamd_handle h, symamd_plan p, void *d_B, int k,
+                   int64_t rs, int64_t cs, int exec, int lanes);
+int launch_adjoint(symamd_handle h, symamd_plan p, const void *d_x,
+                   const void *d_gx, void *d_gb, void *d_gL, int exec,
+                   int lanes);
+int compute_levels(symamd_handle h, int n, const int *d_Lp, const int *d_Li,
+                   int *d_state, int *d_level_ptr, int *d_level_set,
+                   int *levels, std::vector<int> *h_level_ptr);
+int build_plan_arrays(symamd_handle h, symamd_plan p, const int *d_Lp,
+                      const int *d_Li, const void *d_Lx);
+int gather_values(symamd_handle h, symamd_plan p, const void *d_Lx);
+int check_structure(symamd_handle h, int n, const int *d_Lp, const int *d_Li,
+                    int *d_state);
+int launch_spmv(symamd_handle h, int n, const int *d_Ap, const int *d_Ai,
+                const double *d_Ax, const double *d_x, const double *d_y,
+                double *d_z, double a, double b, int mode);
+int plan_residual(symamd_handle h, symamd_plan p, const void *d_X,
+                  const void *d_B, int k, int64_t rs, int64_t cs,
+                  double *h_rel);
+int forward_rows_per_unit(int k, int lanes);
+int coarsen(symamd_handle h, int n, const int *d_Lp, const int *d_Li,
+            int span, int max_cost, int *d_level_ptr, int *d_par_ptr,
+            int *d_partition, int *level_no, int *part_no);
+int launch_lbc(symamd_handle h, int n, const int *d_Lp, const int *d_Li,
+               const double *d_Lx, double *d_x, int level_no,
+               const int *h_level_ptr, const int *d_par_ptr,
+               const int *d_partition);
+
+inline int64_t tune_key(int k, int layout) { return ((int64_t)k << 2) | layout; }
+
+inline int pow2_floor(int v) {
+  int r = 1;
+  while (r * 2 <= v) r *= 2;
+  return r;
+}
+inline int pow2_ceil(int v) {
+  int r = 1;
+  while (r < v) r *= 2;
+  return r;
+}
+
+}  // namespace symamd

@@ -1,0 +1,247 @@
+// dropin.cpp -- sym_lib:: entry points with the reference's signatures
+// (include/sympiler/sparse_blas_lib.h), implemented on the C ABI.
+//
+// Each call is blocking and stateless, as in the reference: host arrays are
+// copied to the device, the plan is built, the HIP kernels run with one lane
+// per row (bit-identical to the serial reference), and x is copied back.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/sym_lib_sparse_blas.h"
+#include "../../include/sympiler_amd.h"
+
+namespace {
+
+void check(int rc, const char *what) {
+  if (rc != SYMAMD_OK)
+    throw std::runtime_error(std::string("sympiler_amd ") + what + ": " +
+                             symamd_last_error());
+}
+
+void hcheck(hipError_t e, const char *what) {
+  if (e != hipSuccess)
+    throw std::runtime_error(std::string("sympiler_amd ") + what + ": " +
+                             hipGetErrorString(e));
+}
+
+symamd_handle handle() {
+  thread_local symamd_handle h = nullptr;
+  if (!h) check(symamd_create(&h, nullptr), "create");
+  return h;
+}
+
+template <typename T>
+struct DevBuf {
+  T *p = nullptr;
+  DevBuf() = default;
+  explicit DevBuf(size_t count, const T *src = nullptr) {
+    hcheck(hipMalloc(&p, sizeof(T) * (count ? count : 1)), "hipMalloc");
+    if (src && count) hcheck(hipMemcpy(p, src, sizeof(T) * count, hipMemcpyHostToDevice), "upload");
+  }
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  void download(T *dst, size_t count) const {
+    if (count) hcheck(hipMemcpy(dst, p, sizeof(T) * count, hipMemcpyDeviceToHost), "download");
+  }
+};
+
+struct Plan {
+  symamd_plan p = nullptr;
+  ~Plan() {
+    if (p) symamd_plan_destroy(p);
+  }
+};
+
+// CSC (diagonal first) -> CSR of the same lower triangle (diagonal last).
+void csc_to_csr(int n, const int *Cp, const int *Ci, const double *Cx,
+                std::vector<int> &Rp, std::vector<int> &Ri,
+                std::vector<double> &Rx) {
+  const int nnz = Cp[n];
+  Rp.assign(n + 1, 0);
+  Ri.resize(nnz);
+  Rx.resize(nnz);
+  for (int p = 0; p < nnz; ++p) Rp[Ci[p] + 1]++;
+  for (int i = 0; i < n; ++i) Rp[i + 1] += Rp[i];
+  std::vector<int> fill(Rp.begin(), Rp.end() - 1);
+  for (int c = 0; c < n; ++c)
+    for (int p = Cp[c]; p < Cp[c + 1]; ++p) {
+      int q = fill[Ci[p]]++;
+      Ri[q] = c;
+      Rx[q] = Cx[p];
+    }
+}
+
+// Solve L x = b in place on the GPU from a host CSR (diagonal last).
+void solve_csr_host(int n, const int *Lp, const int *Li, const double *Lx,
+                    double *x) {
+  if (n <= 0) return;
+  symamd_handle h = handle();
+  const int nnz = Lp[n];
+  DevBuf<int> dLp(n + 1, Lp), dLi(nnz, Li);
+  DevBuf<double> dLx(nnz, Lx), dx(n, x);
+  Plan pl;
+  check(symamd_analyze(h, n, dLp.p, dLi.p, dLx.p, SYMAMD_F64, &pl.p), "analyze");
+  check(symamd_plan_set_lanes(pl.p, 1), "set_lanes");
+  check(symamd_solve(h, pl.p, dx.p, 1, n, SYMAMD_COL_MAJOR, SYMAMD_EXEC_SYNCFREE), "solve");
+  check(symamd_plan_check(h, pl.p), "solve");
+  dx.download(x, n);
+}
+
+// Checks a caller level set: every row once, dependencies in earlier levels.
+void check_levels(int n, const int *Rp, const int *Ri, int levels,
+                  const int *levelPtr, const int *levelSet) {
+  if (levels < 0 || !levelPtr || (n > 0 && !levelSet) || levelPtr[0] != 0 ||
+      levelPtr[levels] != n)
+    throw std::runtime_error("sympiler_amd: level set does not cover the n rows");
+  std::vector<int> lev(n, -1);
+  for (int l = 0; l < levels; ++l)
+    for (int k = levelPtr[l]; k < levelPtr[l + 1]; ++k) {
+      int i = levelSet[k];
+      if (i < 0 || i >= n || lev[i] != -1)
+        throw std::runtime_error("sympiler_amd: level set lists a row twice or out of range");
+      lev[i] = l;
+    }
+  for (int i = 0; i < n; ++i)
+    for (int j = Rp[i]; j < Rp[i + 1] - 1; ++j)
+      if (lev[Ri[j]] >= lev[i])
+        throw std::runtime_error("sympiler_amd: level set violates a dependency");
+}
+
+}  // namespace
+
+namespace sym_lib {
+
+void spmv_csr(int n, const int *Ap, const int *Ai, const double *Ax,
+              const double *x, double *y) {
+  if (n <= 0) return;
+  const int nnz = Ap[n];
+  int ncol = n;
+  for (int p = 0; p < nnz; ++p) ncol = Ai[p] + 1 > ncol ? Ai[p] + 1 : ncol;
+  DevBuf<int> dAp(n + 1, Ap), dAi(nnz, Ai);
+  DevBuf<double> dAx(nnz, Ax), dx(ncol, x), dy(n);
+  symamd_handle h = handle();
+  check(symamd_spmv_csr(h, n, dAp.p, dAi.p, dAx.p, dx.p, dy.p), "spmv_csr");
+  hcheck(hipDeviceSynchronize(), "spmv_csr");
+  dy.download(y, n);
+}
+
+void spmv_csr(int n, const int *Ap, const int *Ai, const double *Ax,
+              const double *x, const double *y, double *z, double a, double b) {
+  if (n <= 0) return;
+  const int nnz = Ap[n];
+  int ncol = n;
+  for (int p = 0; p < nnz; ++p) ncol = Ai[p] + 1 > ncol ? Ai[p] + 1 : ncol;
+  DevBuf<int> dAp(n + 1, Ap), dAi(nnz, Ai);
+  DevBuf<double> dAx(nnz, Ax), dx(ncol, x), dy(n, y), dz(n, z);
+  symamd_handle h = handle();
+  check(symamd_spmv_csr_axpby(h, n, dAp.p, dAi.p, dAx.p, dx.p, dy.p, dz.p, a, b),
+        "spmv_csr");
+  hcheck(hipDeviceSynchronize(), "spmv_csr");
+  dz.download(z, n);
+}
+
+void sptrsv_csr(int n, int *Lp, int *Li, double *Lx, double *x) {
+  solve_csr_host(n, Lp, Li, Lx, x);
+}
+
+void sptrsv_csr_levelset(int n, const int *Lp, const int *Li, const double *Lx,
+                         double *x, int levels, const int *levelPtr,
+                         const int *levelSet) {
+  check_levels(n, Lp, Li, levels, levelPtr, levelSet);
+  solve_csr_host(n, Lp, Li, Lx, x);
+}
+
+void sptrsv_csr_lbc(int n, int *Lp, int *Li, double *Lx, double *x,
+                    int level_no, int *level_ptr, int *par_ptr,
+                    int *partition) {
+  if (n <= 0) return;
+  symamd_handle h = handle();
+  const int nnz = Lp[n];
+  const int nparts = level_no > 0 ? level_ptr[level_no] : 0;
+  DevBuf<int> dLp(n + 1, Lp), dLi(nnz, Li), dpp(nparts + 1, par_ptr),
+      dpart(n, partition);
+  DevBuf<double> dLx(nnz, Lx), dx(n, x);
+  check(symamd_solve_lbc(h, n, dLp.p, dLi.p, dLx.p, dx.p, level_no, level_ptr,
+                         dpp.p, dpart.p),
+        "sptrsv_csr_lbc");
+  hcheck(hipDeviceSynchronize(), "sptrsv_csr_lbc");
+  dx.download(x, n);
+}
+
+void sptrsv_csc(int n, int *Lp, int *Li, double *Lx, double *x) {
+  if (n <= 0) return;
+  std::vector<int> Rp, Ri;
+  std::vector<double> Rx;
+  csc_to_csr(n, Lp, Li, Lx, Rp, Ri, Rx);
+  solve_csr_host(n, Rp.data(), Ri.data(), Rx.data(), x);
+}
+
+void sptrsv_csc_levelset(int n, int *Lp, int *Li, double *Lx, double *x,
+                         int levels, const int *levelptr, const int *levelset) {
+  if (n <= 0) return;
+  std::vector<int> Rp, Ri;
+  std::vector<double> Rx;
+  csc_to_csr(n, Lp, Li, Lx, Rp, Ri, Rx);
+  check_levels(n, Rp.data(), Ri.data(), levels, levelptr, levelset);
+  solve_csr_host(n, Rp.data(), Ri.data(), Rx.data(), x);
+}
+
+void sptrsv_csc_lbc(int n, int *Lp, int *Li, double *Lx, double *x,
+                    int level_no, int *level_ptr, int *par_ptr,
+                    int *partition) {
+  if (n <= 0) return;
+  std::vector<int> Rp, Ri;
+  std::vector<double> Rx;
+  csc_to_csr(n, Lp, Li, Lx, Rp, Ri, Rx);
+  sptrsv_csr_lbc(n, Rp.data(), Ri.data(), Rx.data(), x, level_no, level_ptr,
+                 par_ptr, partition);
+}
+
+int ltsolve(int n, int *Lp, int *Li, double *Lx, double *x) {
+  if (!Lp || !Li || !x) return 0;
+  if (n <= 0) return 1;
+  std::vector<int> Rp, Ri;
+  std::vector<double> Rx;
+  csc_to_csr(n, Lp, Li, Lx, Rp, Ri, Rx);
+  symamd_handle h = handle();
+  const int nnz = Rp[n];
+  DevBuf<int> dLp(n + 1, Rp.data()), dLi(nnz, Ri.data());
+  DevBuf<double> dLx(nnz, Rx.data()), dx(n, x), dzero(n);
+  Plan pl;
+  check(symamd_analyze(h, n, dLp.p, dLi.p, dLx.p, SYMAMD_F64, &pl.p), "analyze");
+  check(symamd_plan_set_lanes(pl.p, 1), "set_lanes");
+  // gb = L^-T gx with gx = b, in place; x is not used when gLx is NULL
+  check(symamd_adjoint(h, pl.p, dzero.p, dx.p, dx.p, nullptr, SYMAMD_EXEC_SYNCFREE),
+        "ltsolve");
+  check(symamd_plan_check(h, pl.p), "ltsolve");
+  dx.download(x, n);
+  return 1;
+}
+
+int build_levelSet_CSC(int n, const int *Lp, const int *Li, int *&level_ptr,
+                       int *&level_set) {
+  level_ptr = new int[n + 1];
+  level_set = new int[n > 0 ? n : 1];
+  level_ptr[0] = 0;
+  if (n <= 0) return 0;
+  std::vector<double> dummy(Lp[n], 1.0);
+  std::vector<int> Rp, Ri;
+  std::vector<double> Rx;
+  csc_to_csr(n, Lp, Li, dummy.data(), Rp, Ri, Rx);
+  symamd_handle h = handle();
+  DevBuf<int> dLp(n + 1, Rp.data()), dLi(Rp[n], Ri.data()), dlp(n + 1), dls(n);
+  int levels = 0;
+  check(symamd_levelset(h, n, dLp.p, dLi.p, dlp.p, dls.p, &levels), "build_levelSet_CSC");
+  dlp.download(level_ptr, levels + 1);
+  dls.download(level_set, n);
+  return levels;
+}
+
+}  // namespace sym_lib

@@ -1,0 +1,504 @@
+// inspector.hip -- GPU inspector for the sparse lower-triangular solve:
+// structure validation, dependency levels, level sets, the level-ordered
+// copy of L that the executors stream, and the transposed copy the adjoint
+// streams.
+//
+// Reference: the level-set inspector build_levelSet_CSC lives in the
+// un-vendored aggregation submodule (call sites Catch_tests/
+// sptrsv_tests.cpp:59, demo/sptrsv_demo_utils.h:44); the definition
+// implemented here is the one the in-tree analog
+// sparse_blas/parsy/common/TreeUtils.cpp:89-141 (getLevelSet) computes:
+// ASAP levels, rows ascending inside a level.
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+#include <rocprim/iterator/counting_iterator.hpp>
+
+#include "common.h"
+
+namespace symamd {
+
+namespace {
+
+// ------------------------------------------------------ structure check
+// Bit 0: a row with no entries; bit 1: last entry of a row is not its
+// diagonal; bit 2: columns not strictly ascending or out of range.
+__global__ void k_check_structure(int n, const int *__restrict__ Lp,
+                                  const int *__restrict__ Li, int *flag) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int b = Lp[i], e = Lp[i + 1];
+  int bad = 0;
+  if (e <= b) {
+    bad |= 1;
+  } else {
+    if (Li[e - 1] != i) bad |= 2;
+    int prev = -1;
+    for (int j = b; j < e; ++j) {
+      int c = Li[j];
+      if (c <= prev || c < 0 || c > i) { bad |= 4; break; }
+      prev = c;
+    }
+  }
+  if (bad) atomicOr(flag, bad);
+}
+
+// ------------------------------------------------------------- levels
+// Sync-free longest path over the DAG of L in natural row order.  Waves
+// take chunks of 64 consecutive rows from a ticket counter, so every row a
+// wave waits on is held by a wave that is already running.  lev[] starts at
+// -1 and is its own ready flag: lev[j] >= 0 means row j is done.  A lane
+// whose dependency is not ready yet keeps its place and retries on the next
+// pass of the wave-wide loop, so lanes of one wave never block each other
+// (row i may depend on row i-1 in the same chunk).
+__global__ void k_levels(int n, const int *__restrict__ Lp,
+                         const int *__restrict__ Li, int *lev, int *state,
+                         uint64_t timeout) {
+  const int lane = threadIdx.x & 63;
+  int local_max = -1;
+  for (;;) {
+    int chunk = 0;
+    if (lane == 0) chunk = atomicAdd(&state[ST_TICKET], 1);
+    chunk = __shfl(chunk, 0);
+    const int64_t row0 = (int64_t)chunk * 64;
+    if (row0 >= n) break;
+    const int i = (int)row0 + lane;
+    bool done = i >= n;
+    int j = done ? 0 : Lp[i];
+    const int je = done ? 0 : Lp[i + 1] - 1;
+    int m = -1;
+    uint64_t t0 = 0;
+    int spins = 0;
+    for (;;) {
+      if (!done) {
+        while (j < je) {
+          int v = ld_dev(&lev[Li[j]]);
+          if (v < 0) break;
+          m = v > m ? v : m;
+          ++j;
+        }
+        if (j >= je) {
+          st_dev(&lev[i], m + 1);
+          local_max = m + 1 > local_max ? m + 1 : local_max;
+          done = true;
+        }
+      }
+      if (__all(done)) break;
+      if ((++spins & 255) == 0) {
+        uint64_t now = wall_ticks();
+        if (t0 == 0) t0 = now;
+        else if (now - t0 > timeout) {
+          // give up: publish something so dependants drain, report error
+          if (!done) {
+            atomicOr(&state[ST_ERROR], 1);
+            st_dev(&lev[i], m + 1);
+            done = true;
+          }
+        }
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+  }
+  // wave max -> global max
+  for (int off = 32; off > 0; off >>= 1) {
+    int o = __shfl_xor(local_max, off);
+    local_max = o > local_max ? o : local_max;
+  }
+  if (lane == 0) atomicMax(&state[ST_SCRATCH], local_max);
+}
+
+// level_ptr from level-sorted keys: position p starts level key[p] when it
+// differs from key[p-1] (levels are contiguous 0..nlev-1).
+__global__ void k_level_ptr(int n, int nlev, const int *__restrict__ skey,
+                            int *level_ptr) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  int l = skey[p];
+  int prev = p == 0 ? -1 : skey[p - 1];
+  for (int t = prev + 1; t <= l; ++t) level_ptr[t] = p;
+  if (p == n - 1) level_ptr[nlev] = n;
+}
+
+// ----------------------------------------------------- plan arrays
+__global__ void k_perm_rowlen(int n, const int *__restrict__ perm,
+                              const int *__restrict__ Lp, int *len) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) {
+    int i = perm[p];
+    len[p] = Lp[i + 1] - Lp[i];
+  }
+  if (p == n) len[p] = 0;
+}
+
+// One wave per 64 consecutive level-ordered rows; each lane copies one
+// row's entries (rows are short, 1-8 entries for most workloads).
+template <typename T>
+__global__ void k_perm_copy(int n, const int *__restrict__ perm,
+                            const int *__restrict__ Lp,
+                            const int *__restrict__ Li,
+                            const T *__restrict__ Lx,
+                            const int *__restrict__ rp, int *ci, T *val,
+                            int *fmap) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  int i = perm[p];
+  int src = Lp[i], len = Lp[i + 1] - src, dst = rp[p];
+  for (int t = 0; t < len; ++t) {
+    ci[dst + t] = Li[src + t];
+    if (Lx) val[dst + t] = Lx[src + t];
+    fmap[dst + t] = src + t;
+  }
+}
+
+template <typename T>
+__global__ void k_gather(int64_t m, const int *__restrict__ map,
+                         const T *__restrict__ src, T *dst) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < m) dst[t] = src[map[t]];
+}
+
+template <typename T>
+__global__ void k_diag_zero(int n, const int *__restrict__ rp,
+                            const T *__restrict__ val, int *flag) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n && val[rp[p + 1] - 1] == T(0)) atomicOr(flag, 8);
+}
+
+// Transpose keys: entry t of the level-ordered copy is (row perm[p],
+// column ci[t]); key = column * 2^32 + row orders it column-major with rows
+// ascending (diagonal first in each column of a lower triangle).
+__global__ void k_transpose_keys(int n, const int *__restrict__ perm,
+                                 const int *__restrict__ rp,
+                                 const int *__restrict__ ci, uint64_t *key,
+                                 int *idx) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  uint64_t row = (uint32_t)perm[p];
+  for (int t = rp[p]; t < rp[p + 1]; ++t) {
+    key[t] = ((uint64_t)(uint32_t)ci[t] << 32) | row;
+    idx[t] = t;
+  }
+}
+
+// Column pointers of the natural-order CSC from column-sorted keys.
+__global__ void k_col_ptr(int64_t nnz, int n, const uint64_t *__restrict__ key,
+                          int *cp) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nnz) return;
+  int c = (int)(key[t] >> 32);
+  int prev = t == 0 ? -1 : (int)(key[t - 1] >> 32);
+  for (int q = prev + 1; q <= c; ++q) cp[q] = (int)t;
+  if (t == nnz - 1)
+    for (int q = c + 1; q <= n; ++q) cp[q] = (int)nnz;
+}
+
+// Level-ordered transposed copy: position q holds column perm[q] of L.
+template <typename T>
+__global__ void k_transpose_copy(int n, const int *__restrict__ perm,
+                                 const int *__restrict__ cp,
+                                 const uint64_t *__restrict__ skey,
+                                 const int *__restrict__ sidx,
+                                 const int *__restrict__ trp,
+                                 const int *__restrict__ fmap,
+                                 const T *__restrict__ val, int *tci, T *tval,
+                                 int *tmap) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  int c = perm[q];
+  int src = cp[c], len = cp[c + 1] - src, dst = trp[q];
+  for (int t = 0; t < len; ++t) {
+    uint64_t k = skey[src + t];
+    int e = sidx[src + t];
+    tci[dst + t] = (int)(k & 0xffffffffu);
+    tval[dst + t] = val[e];
+    tmap[dst + t] = fmap[e];
+  }
+}
+
+__global__ void k_collen(int n, const int *__restrict__ perm,
+                         const int *__restrict__ cp, int *len) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) {
+    int c = perm[q];
+    len[q] = cp[c + 1] - cp[c];
+  }
+  if (q == n) len[q] = 0;
+}
+
+// Work units: level l splits into ceil(size/rows) units of at most `rows`
+// consecutive positions.  upl = units per level, scanned into uoff.
+__global__ void k_units_per_level(int nlev, const int *__restrict__ lp,
+                                  int rows, int *upl) {
+  int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l < nlev) upl[l] = (lp[l + 1] - lp[l] + rows - 1) / rows;
+  if (l == nlev) upl[l] = 0;
+}
+
+__global__ void k_unit_bounds(int nlev, int U, int n, const int *__restrict__ lp,
+                              const int *__restrict__ uoff, int rows, int *ub) {
+  int u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u > U) return;
+  if (u == U) { ub[U] = n; return; }
+  // level containing unit u: last l with uoff[l] <= u
+  int lo = 0, hi = nlev - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (uoff[mid] <= u) lo = mid; else hi = mid - 1;
+  }
+  ub[u] = lp[lo] + (u - uoff[lo]) * rows;
+}
+
+inline int blocks_for(int64_t m, int bs = 256) {
+  return (int)((m + bs - 1) / bs);
+}
+
+// exclusive scan of len[0..m] (m+1 entries, len[m] = 0) into out[0..m]
+int scan_exclusive(symamd_handle h, const int *len, int *out, int64_t m1) {
+  size_t bytes = 0;
+  SYMAMD_HIP(rocprim::exclusive_scan(nullptr, bytes, len, out, 0, (size_t)m1,
+                                     rocprim::plus<int>(), h->stream));
+  void *tmp = nullptr;
+  SYMAMD_HIP(hipMallocAsync(&tmp, bytes ? bytes : 4, h->stream));
+  hipError_t e = rocprim::exclusive_scan(tmp, bytes, len, out, 0, (size_t)m1,
+                                         rocprim::plus<int>(), h->stream);
+  SYMAMD_HIP(hipFreeAsync(tmp, h->stream));
+  SYMAMD_HIP(e);
+  return SYMAMD_OK;
+}
+
+}  // namespace
+
+int check_structure(symamd_handle h, int n, const int *d_Lp, const int *d_Li,
+                    int *d_state) {
+  if (n == 0) return SYMAMD_OK;
+  SYMAMD_HIP(hipMemsetAsync(d_state + ST_SCRATCH, 0, sizeof(int), h->stream));
+  k_check_structure<<<blocks_for(n), 256, 0, h->stream>>>(n, d_Lp, d_Li,
+                                                          d_state + ST_SCRATCH);
+  SYMAMD_HIP(hipGetLastError());
+  int flag = 0;
+  SYMAMD_HIP(hipMemcpyAsync(&flag, d_state + ST_SCRATCH, sizeof(int),
+                            hipMemcpyDeviceToHost, h->stream));
+  SYMAMD_HIP(hipStreamSynchronize(h->stream));
+  if (flag & 1) return fail(SYMAMD_ERR_STRUCTURE, "a row has no entries (no diagonal)");
+  if (flag & 2) return fail(SYMAMD_ERR_STRUCTURE, "a row's last entry is not its diagonal (CSR lower triangle needs the diagonal last)");
+  if (flag & 4) return fail(SYMAMD_ERR_STRUCTURE, "column indices not strictly ascending, negative, or above the diagonal");
+  return SYMAMD_OK;
+}
+
+int compute_levels(symamd_handle h, int n, const int *d_Lp, const int *d_Li,
+                   int *d_state, int *d_level_ptr, int *d_level_set,
+                   int *levels, std::vector<int> *h_level_ptr) {
+  *levels = 0;
+  if (n == 0) {
+    int zero = 0;
+    SYMAMD_HIP(hipMemcpyAsync(d_level_ptr, &zero, sizeof(int),
+                              hipMemcpyHostToDevice, h->stream));
+    SYMAMD_HIP(hipStreamSynchronize(h->stream));
+    if (h_level_ptr) h_level_ptr->assign(1, 0);
+    return SYMAMD_OK;
+  }
+  int *d_lev = nullptr, *d_skey = nullptr;
+  SYMAMD_HIP(hipMallocAsync(&d_lev, sizeof(int) * n, h->stream));
+  SYMAMD_HIP(hipMallocAsync(&d_skey, sizeof(int) * n, h->stream));
+  SYMAMD_HIP(hipMemsetAsync(d_lev, 0xff, sizeof(int) * n, h->stream));
+  SYMAMD_HIP(hipMemsetAsync(d_state, 0, sizeof(int) * ST_WORDS, h->stream));
+  int waves_needed = (n + 63) / 64;
+  int grid = (waves_needed + 3) / 4;
+  int cap = h->num_cus * 8;
+  if (grid > cap) grid = cap;
+  k_levels<<<grid, 256, 0, h->stream>>>(n, d_Lp, d_Li, d_lev, d_state,
+                                        h->timeout_ticks());
+  SYMAMD_HIP(hipGetLastError());
+  int st[ST_WORDS];
+  SYMAMD_HIP(hipMemcpyAsync(st, d_state, sizeof(st), hipMemcpyDeviceToHost,
+                            h->stream));
+  SYMAMD_HIP(hipStreamSynchronize(h->stream));
+  if (st[ST_ERROR]) {
+    (void)hipFreeAsync(d_lev, h->stream);
+    (void)hipFreeAsync(d_skey, h->stream);
+    return fail(SYMAMD_ERR_TIMEOUT, "level inspector watchdog expired");
+  }
+  const int nlev = st[ST_SCRATCH] + 1;
+  unsigned end_bit = 1;
+  while ((1u << end_bit) <= (unsigned)(nlev - 1) && end_bit < 31) ++end_bit;
+  size_t bytes = 0;
+  rocprim::counting_iterator<int> rows(0);
+  SYMAMD_HIP(rocprim::radix_sort_pairs(nullptr, bytes, (unsigned *)d_lev,
+                                       (unsigned *)d_skey, rows, d_level_set,
+                                       (size_t)n, 0, end_bit, h->stream));
+  void *tmp = nullptr;
+  SYMAMD_HIP(hipMallocAsync(&tmp, bytes ? bytes : 4, h->stream));
+  SYMAMD_HIP(rocprim::radix_sort_pairs(tmp, bytes, (unsigned *)d_lev,
+                                       (unsigned *)d_skey, rows, d_level_set,
+                                       (size_t)n, 0, end_bit, h->stream));
+  k_level_ptr<<<blocks_for(n), 256, 0, h->stream>>>(n, nlev, d_skey,
+                                                    d_level_ptr);
+  SYMAMD_HIP(hipGetLastError());
+  SYMAMD_HIP(hipFreeAsync(tmp, h->stream));
+  SYMAMD_HIP(hipFreeAsync(d_lev, h->stream));
+  SYMAMD_HIP(hipFreeAsync(d_skey, h->stream));
+  if (h_level_ptr) {
+    h_level_ptr->resize(nlev + 1);
+    SYMAMD_HIP(hipMemcpyAsync(h_level_ptr->data(), d_level_ptr,
+                              sizeof(int) * (nlev + 1), hipMemcpyDeviceToHost,
+                              h->stream));
+  }
+  SYMAMD_HIP(hipStreamSynchronize(h->stream));
+  *levels = nlev;
+  return SYMAMD_OK;
+}
+
+int build_plan_arrays(symamd_handle h, symamd_plan p, const int *d_Lp,
+                      const int *d_Li, const void *d_Lx) {
+  const int n = p->n;
+  int *d_len = nullptr;
+  SYMAMD_HIP(hipMallocAsync(&d_len, sizeof(int) * (n + 1), h->stream));
+  k_perm_rowlen<<<blocks_for(n + 1), 256, 0, h->stream>>>(n, p->d_perm, d_Lp,
+                                                          d_len);
+  SYMAMD_HIP(hipGetLastError());
+  SYMAMD_TRY(scan_exclusive(h, d_len, p->d_rp, n + 1));
+  SYMAMD_HIP(hipFreeAsync(d_len, h->stream));
+  SYMAMD_HIP(hipMemcpyAsync(p->d_orp, d_Lp, sizeof(int) * (n + 1),
+                            hipMemcpyDeviceToDevice, h->stream));
+  if (n > 0) {
+    if (p->dtype == SYMAMD_F64)
+      k_perm_copy<double><<<blocks_for(n), 256, 0, h->stream>>>(
+          n, p->d_perm, d_Lp, d_Li, (const double *)d_Lx, p->d_rp, p->d_ci,
+          (double *)p->d_val, p->d_fmap);
+    else
+      k_perm_copy<float><<<blocks_for(n), 256, 0, h->stream>>>(
+          n, p->d_perm, d_Lp, d_Li, (const float *)d_Lx, p->d_rp, p->d_ci,
+          (float *)p->d_val, p->d_fmap);
+    SYMAMD_HIP(hipGetLastError());
+  }
+  return SYMAMD_OK;
+}
+
+int gather_values(symamd_handle h, symamd_plan p, const void *d_Lx) {
+  const int64_t m = p->nnz;
+  if (m == 0) return SYMAMD_OK;
+  if (p->dtype == SYMAMD_F64) {
+    k_gather<double><<<blocks_for(m), 256, 0, h->stream>>>(
+        m, p->d_fmap, (const double *)d_Lx, (double *)p->d_val);
+    if (p->d_tval)
+      k_gather<double><<<blocks_for(m), 256, 0, h->stream>>>(
+          m, p->d_tmap, (const double *)d_Lx, (double *)p->d_tval);
+  } else {
+    k_gather<float><<<blocks_for(m), 256, 0, h->stream>>>(
+        m, p->d_fmap, (const float *)d_Lx, (float *)p->d_val);
+    if (p->d_tval)
+      k_gather<float><<<blocks_for(m), 256, 0, h->stream>>>(
+          m, p->d_tmap, (const float *)d_Lx, (float *)p->d_tval);
+  }
+  SYMAMD_HIP(hipGetLastError());
+  // zero pivots
+  SYMAMD_HIP(hipMemsetAsync(p->d_state + ST_SCRATCH, 0, sizeof(int), h->stream));
+  if (p->dtype == SYMAMD_F64)
+    k_diag_zero<double><<<blocks_for(p->n), 256, 0, h->stream>>>(
+        p->n, p->d_rp, (const double *)p->d_val, p->d_state + ST_SCRATCH);
+  else
+    k_diag_zero<float><<<blocks_for(p->n), 256, 0, h->stream>>>(
+        p->n, p->d_rp, (const float *)p->d_val, p->d_state + ST_SCRATCH);
+  SYMAMD_HIP(hipGetLastError());
+  int flag = 0;
+  SYMAMD_HIP(hipMemcpyAsync(&flag, p->d_state + ST_SCRATCH, sizeof(int),
+                            hipMemcpyDeviceToHost, h->stream));
+  SYMAMD_HIP(hipStreamSynchronize(h->stream));
+  if (flag & 8) return fail(SYMAMD_ERR_ZERO_PIVOT, "L has a zero diagonal entry");
+  return SYMAMD_OK;
+}
+
+int plan_units(symamd_handle h, symamd_plan p, int rows, UnitSet **out) {
+  auto it = p->units.find(rows);
+  if (it != p->units.end()) {
+    *out = &it->second;
+    return SYMAMD_OK;
+  }
+  UnitSet us;
+  us.rows = rows;
+  const int nlev = p->levels;
+  // count on the host from the level pointers (levels+1 ints, already there)
+  int64_t U = 0;
+  for (int l = 0; l < nlev; ++l)
+    U += (p->h_lvl_ptr[l + 1] - p->h_lvl_ptr[l] + rows - 1) / rows;
+  us.count = (int)U;
+  SYMAMD_HIP(hipMalloc(&us.d_ub, sizeof(int) * (U + 1)));
+  if (nlev > 0) {
+    int *d_upl = nullptr, *d_uoff = nullptr;
+    SYMAMD_HIP(hipMallocAsync(&d_upl, sizeof(int) * (nlev + 1), h->stream));
+    SYMAMD_HIP(hipMallocAsync(&d_uoff, sizeof(int) * (nlev + 1), h->stream));
+    k_units_per_level<<<blocks_for(nlev + 1), 256, 0, h->stream>>>(
+        nlev, p->d_lvl_ptr, rows, d_upl);
+    SYMAMD_TRY(scan_exclusive(h, d_upl, d_uoff, nlev + 1));
+    k_unit_bounds<<<blocks_for(U + 1), 256, 0, h->stream>>>(
+        nlev, (int)U, p->n, p->d_lvl_ptr, d_uoff, rows, us.d_ub);
+    SYMAMD_HIP(hipGetLastError());
+    SYMAMD_HIP(hipFreeAsync(d_upl, h->stream));
+    SYMAMD_HIP(hipFreeAsync(d_uoff, h->stream));
+  } else {
+    int zero = 0;
+    SYMAMD_HIP(hipMemcpyAsync(us.d_ub, &zero, sizeof(int),
+                              hipMemcpyHostToDevice, h->stream));
+  }
+  SYMAMD_HIP(hipStreamSynchronize(h->stream));
+  auto res = p->units.emplace(rows, us);
+  *out = &res.first->second;
+  return SYMAMD_OK;
+}
+
+int plan_transpose(symamd_handle h, symamd_plan p) {
+  if (p->d_trp) return SYMAMD_OK;
+  const int n = p->n;
+  const int64_t nnz = p->nnz;
+  const size_t vs = p->dtype == SYMAMD_F64 ? 8 : 4;
+  SYMAMD_HIP(hipMalloc(&p->d_trp, sizeof(int) * (n + 1)));
+  SYMAMD_HIP(hipMalloc(&p->d_tci, sizeof(int) * (nnz ? nnz : 1)));
+  SYMAMD_HIP(hipMalloc(&p->d_tval, vs * (nnz ? nnz : 1)));
+  SYMAMD_HIP(hipMalloc(&p->d_tmap, sizeof(int) * (nnz ? nnz : 1)));
+  if (n == 0) {
+    SYMAMD_HIP(hipMemsetAsync(p->d_trp, 0, sizeof(int), h->stream));
+    SYMAMD_HIP(hipStreamSynchronize(h->stream));
+    return SYMAMD_OK;
+  }
+  uint64_t *d_key = nullptr, *d_skey = nullptr;
+  int *d_idx = nullptr, *d_sidx = nullptr, *d_cp = nullptr, *d_len = nullptr;
+  SYMAMD_HIP(hipMallocAsync(&d_key, 8 * nnz, h->stream));
+  SYMAMD_HIP(hipMallocAsync(&d_skey, 8 * nnz, h->stream));
+  SYMAMD_HIP(hipMallocAsync(&d_idx, 4 * nnz, h->stream));
+  SYMAMD_HIP(hipMallocAsync(&d_sidx, 4 * nnz, h->stream));
+  SYMAMD_HIP(hipMallocAsync(&d_cp, 4 * (n + 1), h->stream));
+  SYMAMD_HIP(hipMallocAsync(&d_len, 4 * (n + 1), h->stream));
+  k_transpose_keys<<<blocks_for(n), 256, 0, h->stream>>>(n, p->d_perm, p->d_rp,
+                                                         p->d_ci, d_key, d_idx);
+  SYMAMD_HIP(hipGetLastError());
+  size_t bytes = 0;
+  SYMAMD_HIP(rocprim::radix_sort_pairs(nullptr, bytes, d_key, d_skey, d_idx,
+                                       d_sidx, (size_t)nnz, 0, 64, h->stream));
+  void *tmp = nullptr;
+  SYMAMD_HIP(hipMallocAsync(&tmp, bytes ? bytes : 4, h->stream));
+  SYMAMD_HIP(rocprim::radix_sort_pairs(tmp, bytes, d_key, d_skey, d_idx,
+                                       d_sidx, (size_t)nnz, 0, 64, h->stream));
+  SYMAMD_HIP(hipFreeAsync(tmp, h->stream));
+  k_col_ptr<<<blocks_for(nnz), 256, 0, h->stream>>>(nnz, n, d_skey, d_cp);
+  k_collen<<<blocks_for(n + 1), 256, 0, h->stream>>>(n, p->d_perm, d_cp, d_len);
+  SYMAMD_HIP(hipGetLastError());
+  SYMAMD_TRY(scan_exclusive(h, d_len, p->d_trp, n + 1));
+  if (p->dtype == SYMAMD_F64)
+    k_transpose_copy<double><<<blocks_for(n), 256, 0, h->stream>>>(
+        n, p->d_perm, d_cp, d_skey, d_sidx, p->d_trp, p->d_fmap,
+        (const double *)p->d_val, p->d_tci, (double *)p->d_tval, p->d_tmap);
+  else
+    k_transpose_copy<float><<<blocks_for(n), 256, 0, h->stream>>>(
+        n, p->d_perm, d_cp, d_skey, d_sidx, p->d_trp, p->d_fmap,
+        (const float *)p->d_val, p->d_tci, (float *)p->d_tval, p->d_tmap);
+  SYMAMD_HIP(hipGetLastError());
+  SYMAMD_HIP(hipFreeAsync(d_key, h->stream));
+  SYMAMD_HIP(hipFreeAsync(d_skey, h->stream));
+  SYMAMD_HIP(hipFreeAsync(d_idx, h->stream));
+  SYMAMD_HIP(hipFreeAsync(d_sidx, h->stream));
+  SYMAMD_HIP(hipFreeAsync(d_cp, h->stream));
+  SYMAMD_HIP(hipFreeAsync(d_len, h->stream));
+  SYMAMD_HIP(hipStreamSynchronize(h->stream));
+  return SYMAMD_OK;
+}
+
+}  // namespace symamd

@@ -1,0 +1,91 @@
+"""CPU checks of the boundary: the libraries load, export every function the
+public headers declare, and the Python mirror refuses to run without them.
+No compute call is made (there may be no GPU)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+INC = os.path.join(ROOT, "include")
+LIB = os.path.join(ROOT, "sympiler_amd", "libsympiler_amd.so")
+GEN = os.path.join(ROOT, "sympiler_amd", "libsymamd_gen.so")
+
+
+def declared_c_functions(header):
+    text = open(os.path.join(INC, header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(sym(?:amd|gen)_[a-z0-9_]+)\s*\(", text)))
+
+
+def exported(path):
+    out = subprocess.check_output(["nm", "-D", "--defined-only", path], text=True)
+    return {line.split()[-1] for line in out.splitlines() if " T " in line}
+
+
+def test_library_loads():
+    ctypes.CDLL(LIB)
+    ctypes.CDLL(GEN)
+
+
+def test_every_declared_entry_point_is_exported():
+    names = declared_c_functions("sympiler_amd.h")
+    assert len(names) >= 25
+    missing = [n for n in names if n not in exported(LIB)]
+    assert not missing, missing
+    gnames = declared_c_functions("sympiler_amd_gen.h")
+    assert not [n for n in gnames if n not in exported(GEN)]
+
+
+def test_sym_lib_dropins_exported_with_reference_mangling():
+    # mangled names of the reference's declarations in
+    # include/sympiler/sparse_blas_lib.h (namespace sym_lib, C++ linkage)
+    want = {
+        "_ZN7sym_lib10sptrsv_csrEiPiS0_PdS1_",
+        "_ZN7sym_lib19sptrsv_csr_levelsetEiPKiS1_PKdPdiS1_S1_",
+        "_ZN7sym_lib14sptrsv_csr_lbcEiPiS0_PdS1_iS0_S0_S0_",
+        "_ZN7sym_lib10sptrsv_cscEiPiS0_PdS1_",
+        "_ZN7sym_lib19sptrsv_csc_levelsetEiPiS0_PdS1_iPKiS3_",
+        "_ZN7sym_lib14sptrsv_csc_lbcEiPiS0_PdS1_iS0_S0_S0_",
+        "_ZN7sym_lib7ltsolveEiPiS0_PdS1_",
+        "_ZN7sym_lib8spmv_csrEiPKiS1_PKdS3_Pd",
+        "_ZN7sym_lib8spmv_csrEiPKiS1_PKdS3_S3_Pddd",
+        "_ZN7sym_lib18build_levelSet_CSCEiPKiS1_RPiS3_",
+    }
+    missing = want - exported(LIB)
+    assert not missing, missing
+
+
+def test_mangled_names_match_the_reference_objects():
+    # the same functions in the reference's own build (when present)
+    import oracle
+    if not oracle.ref_available():
+        pytest.skip("oracle/_ref not built")
+    ref = exported(oracle.REF_PATH)
+    ours = exported(LIB)
+    for name in ("_ZN7sym_lib10sptrsv_csrEiPiS0_PdS1_",
+                 "_ZN7sym_lib19sptrsv_csr_levelsetEiPKiS1_PKdPdiS1_S1_",
+                 "_ZN7sym_lib14sptrsv_csr_lbcEiPiS0_PdS1_iS0_S0_S0_",
+                 "_ZN7sym_lib7ltsolveEiPiS0_PdS1_",
+                 "_ZN7sym_lib8spmv_csrEiPKiS1_PKdS3_Pd"):
+        assert name in ref and name in ours, name
+
+
+def test_python_mirror_fails_loudly_without_library(monkeypatch, tmp_path):
+    import sympiler_amd as sa
+    monkeypatch.setattr(sa, "_lib", None)
+    monkeypatch.setattr(sa, "LIB_PATH", str(tmp_path / "missing.so"))
+    with pytest.raises(RuntimeError, match="no CPU fallback"):
+        sa.lib()
+
+
+def test_product_never_imports_the_oracle():
+    pkg = os.path.join(ROOT, "sympiler_amd")
+    for dirpath, _, files in os.walk(pkg):
+        for f in files:
+            if f.endswith((".py", ".hip", ".cpp", ".h")):
+                text = open(os.path.join(dirpath, f)).read()
+                assert "import oracle" not in text and "from oracle" not in text, f
+                assert "liboracle" not in text and "libsymref" not in text, f

@@ -1,0 +1,217 @@
+"""GPU parity tests: the HIP path through the C ABI against the oracle.
+
+The oracle is the CPU restatement in oracle/ (itself pinned bit for bit to
+the reference's compiled sources and golden vectors by tests/test_oracle.py).
+With one lane per (row, right-hand side) the GPU repeats the reference's
+operation order without FMA contraction, so results must be BIT-IDENTICAL;
+with several lanes per row the row sum is re-associated and the bar is a
+relative error <= 1e-12 (FP64).
+"""
+import numpy as np
+import pytest
+
+import oracle
+import sympiler_amd as sa
+from sympiler_amd import workloads as W
+from tests.helpers import (KA_B, RTOL64, bitwise_equal, etree_lower,
+                           known_answer_csr, rel_err, to_dev)
+
+pytestmark = pytest.mark.gpu
+
+# (n, density) of the reference's "parallel vs serial" test,
+# Catch_tests/sptrsv_tests.cpp:47-51
+REF_RANDOM = [(10, 0.2), (100, 0.03), (500, 0.004), (1000, 0.0005), (10000, 0.0001)]
+
+
+def _plan(Lp, Li, Lx, dev, dtype=None):
+    import torch
+    dLp, dLi, dLx = to_dev(Lp, Li, Lx, device=dev)
+    if dtype is not None:
+        dLx = dLx.to(dtype)
+    return sa.Plan(dLp, dLi, dLx)
+
+
+def _solve(plan, b, dev, executor, lanes):
+    plan.set_lanes(lanes)
+    x = to_dev(b, device=dev)
+    plan.solve(x, executor=executor)
+    plan.check()
+    return x.cpu().numpy()
+
+
+# ---------------------------------------------------------------- levels
+@pytest.mark.parametrize("case", ["c1", "lap3d16", "random", "chain", "empty_rows"])
+def test_levelset_matches_oracle(dev, case):
+    if case == "c1":
+        w = W.config1()
+        Lp, Li = w.Lp, w.Li
+    elif case == "lap3d16":
+        Lp, Li, _ = W.lap3d_tril(16, 16, 16)
+    elif case == "random":
+        Lp, Li, _ = W.random_lower(3000, 0.002, 0, seed=11)
+    elif case == "chain":
+        Lp, Li, _ = W.lap2d_tril(1, 5000)  # bidiagonal: 5000 levels
+    else:
+        Lp, Li, _ = W.random_lower(2000, 0.0, 0, seed=1)  # diagonal only
+    lp_o, ls_o, _ = oracle.levelset_csr(Lp, Li)
+    dLp, dLi = to_dev(Lp, Li, device=dev)
+    lp, ls = sa.levelset(dLp, dLi)
+    assert np.array_equal(lp.cpu().numpy(), lp_o)
+    assert np.array_equal(ls.cpu().numpy(), ls_o)
+
+
+# ---------------------------------------------------------------- solves
+@pytest.mark.parametrize("executor", ["syncfree", "levelset", "graph"])
+def test_c1_bitwise(dev, executor):
+    w = W.config1()
+    x_ref = oracle.sptrsv_csr(w.Lp, w.Li, w.Lx, w.b)
+    plan = _plan(w.Lp, w.Li, w.Lx, dev)
+    assert plan.levels == 199
+    x = _solve(plan, w.b, dev, executor, 1)
+    assert bitwise_equal(x, x_ref)
+    r = plan.residual(to_dev(x, device=dev), to_dev(w.b, device=dev))
+    assert r[0] < 1e-14
+
+
+@pytest.mark.parametrize("lanes", [2, 4, 8, 64])
+def test_c1_multilane(dev, lanes):
+    w = W.config1()
+    x_ref = oracle.sptrsv_csr(w.Lp, w.Li, w.Lx, w.b)
+    plan = _plan(w.Lp, w.Li, w.Lx, dev)
+    for ex in ("syncfree", "levelset"):
+        x = _solve(plan, w.b, dev, ex, lanes)
+        assert rel_err(x, x_ref) <= RTOL64
+
+
+@pytest.mark.parametrize("n,density", REF_RANDOM)
+def test_reference_random_cases(dev, n, density):
+    # the reference test solves with x = 1 as right-hand side
+    Lp, Li, Lx = W.random_lower(n, density, 0, seed=n)
+    b = np.ones(n)
+    x_ref = oracle.sptrsv_csr(Lp, Li, Lx, b)
+    plan = _plan(Lp, Li, Lx, dev)
+    for ex in ("syncfree", "levelset"):
+        assert bitwise_equal(_solve(plan, b, dev, ex, 1), x_ref)
+
+
+def test_known_answer(dev):
+    Rp, Ri, Rx = known_answer_csr()
+    plan = _plan(Rp, Ri, Rx, dev)
+    x = _solve(plan, KA_B, dev, "syncfree", 1)
+    assert np.all(x == 1.0)
+
+
+@pytest.mark.parametrize("n,density,min_off", [(4000, 0.003, 1), (20000, 0.0005, 3)])
+def test_random_irregular(dev, n, density, min_off):
+    Lp, Li, Lx = W.random_lower(n, density, min_off, seed=7)
+    b = W.uniform(5, 0, n)
+    x_ref = oracle.sptrsv_csr(Lp, Li, Lx, b)
+    plan = _plan(Lp, Li, Lx, dev)
+    assert bitwise_equal(_solve(plan, b, dev, "syncfree", 1), x_ref)
+    for lanes in (4, 16):
+        assert rel_err(_solve(plan, b, dev, "syncfree", lanes), x_ref) <= RTOL64
+
+
+def test_c2_full_size(dev):
+    w = W.config2()
+    x_ref = oracle.sptrsv_csr(w.Lp, w.Li, w.Lx, w.b)
+    plan = _plan(w.Lp, w.Li, w.Lx, dev)
+    assert plan.levels == 3 * 128 - 2
+    x = _solve(plan, w.b, dev, "syncfree", 1)
+    assert bitwise_equal(x, x_ref)
+    assert rel_err(x, w.x_true) < 1e-10
+    r = plan.residual(to_dev(x, device=dev), to_dev(w.b, device=dev))
+    assert r[0] < 1e-14
+
+
+def test_c3_full_size(dev):
+    w = W.config3(k=1)
+    b = w.b[0]
+    x_ref = oracle.sptrsv_csr(w.Lp, w.Li, w.Lx, b)
+    plan = _plan(w.Lp, w.Li, w.Lx, dev)
+    assert bitwise_equal(_solve(plan, b, dev, "syncfree", 1), x_ref)
+    assert rel_err(_solve(plan, b, dev, "syncfree", 8), x_ref) <= RTOL64
+
+
+# ---------------------------------------------------------------- SpTRSM
+@pytest.mark.parametrize("k", [1, 3, 8, 32, 33, 128])
+@pytest.mark.parametrize("layout", ["col", "row"])
+def test_sptrsm(dev, k, layout):
+    n = 3000
+    Lp, Li, Lx = W.random_lower(n, 0.002, 1, seed=3)
+    B = W.uniform(9, 0, n * k).reshape(k, n)  # column-major n x k
+    X_ref = oracle.sptrsm_csr(Lp, Li, Lx, B)
+    plan = _plan(Lp, Li, Lx, dev)
+    for ex in ("syncfree", "levelset"):
+        plan.set_lanes(1)
+        dB = to_dev(B if layout == "col" else np.ascontiguousarray(B.T), device=dev)
+        plan.solve(dB, layout=layout, executor=ex)
+        plan.check()
+        X = dB.cpu().numpy()
+        if layout == "row":
+            X = X.T
+        assert bitwise_equal(X, X_ref), (ex, k, layout)
+
+
+# ---------------------------------------------------------------- batched
+def test_batched_small(dev):
+    w = W.config4(count=24, lo=8, hi=40, seed=3)
+    offs = w.row_offsets
+    plan = sa.Plan(*to_dev(w.Lp, w.Li, w.Lx, device=dev), row_offsets=offs)
+    x = _solve(plan, w.b, dev, "syncfree", 1)
+    for s in range(len(offs) - 1):
+        r0, r1 = offs[s], offs[s + 1]
+        p0, p1 = w.Lp[r0], w.Lp[r1]
+        xs = oracle.sptrsv_csr(w.Lp[r0:r1 + 1] - p0, w.Li[p0:p1], w.Lx[p0:p1], w.b[r0:r1])
+        assert bitwise_equal(x[r0:r1], xs)
+
+
+# ---------------------------------------------------------------- adjoint
+@pytest.mark.parametrize("executor", ["syncfree", "levelset"])
+def test_adjoint_bitwise(dev, executor):
+    Lp, Li, Lx = W.ic0_lap3d(16)
+    n = len(Lp) - 1
+    b = W.uniform(4, 0, n)
+    x = oracle.sptrsv_csr(Lp, Li, Lx, b)
+    gx = x.copy()  # loss = 0.5 ||x||^2
+    gb_ref, gL_ref = oracle.adjoint(Lp, Li, Lx, x, gx)
+    plan = _plan(Lp, Li, Lx, dev)
+    plan.set_lanes(1)
+    import torch
+    dx, dgx = to_dev(x, gx, device=dev)
+    gL = torch.empty(len(Lx), dtype=torch.float64, device=dev)
+    gb, _ = plan.adjoint(dx, dgx, gLx=gL, executor=executor)
+    plan.check()
+    assert bitwise_equal(gb.cpu().numpy(), gb_ref)
+    assert bitwise_equal(gL.cpu().numpy(), gL_ref)
+
+
+# ------------------------------------------------------------------ SpMV
+def test_spmv_bitwise(dev):
+    Lp, Li, Lx = W.random_lower(5000, 0.003, 1, seed=2)
+    x = W.uniform(1, 0, 5000)
+    y_ref = oracle.spmv_csr(Lp, Li, Lx, x)
+    dLp, dLi, dLx, dx = to_dev(Lp, Li, Lx, x, device=dev)
+    y = sa.spmv_csr(dLp, dLi, dLx, dx)
+    assert bitwise_equal(y.cpu().numpy(), y_ref)
+    yv = W.uniform(2, 0, 5000)
+    z0 = W.uniform(3, 0, 5000)
+    z_ref = oracle.spmv_csr_axpby(Lp, Li, Lx, x, yv, z0, 0.5, -2.0)
+    dz = to_dev(z0, device=dev)
+    sa.spmv_csr(dLp, dLi, dLx, dx, to_dev(yv, device=dev), dz, 0.5, -2.0)
+    assert bitwise_equal(dz.cpu().numpy(), z_ref)
+
+
+# ------------------------------------------------------------------ errors
+def test_structure_errors(dev):
+    Lp, Li, Lx = W.lap2d_tril(4, 4)
+    bad = Li.copy()
+    bad[Lp[5] - 1] = 3  # row 4: last entry is not the diagonal
+    with pytest.raises(sa.SymAmdError) as e:
+        _plan(Lp, bad, Lx, dev)
+    assert e.value.code == 2
+    zero = Lx.copy()
+    zero[Lp[3] - 1] = 0.0
+    with pytest.raises(sa.SymAmdError) as e:
+        _plan(Lp, Li, zero, dev)
+    assert e.value.code == 3

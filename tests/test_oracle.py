@@ -1,0 +1,366 @@
+"""Pins the oracle (oracle/oracle.c) before it is trusted as the checker.
+
+Each restated function is checked against at least one of: the reference's
+own compiled function (oracle/_ref, bit for bit), a golden vector written
+from it (tests/golden/), a known answer from the reference's tests, or a
+closed form / brute force chosen so that a dropped term, a wrong sign or
+index, or a transposed operand fails.
+"""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from oracle import ref
+from sympiler_amd import workloads as W
+from tests.helpers import (KA_B, KA_CSC_I, KA_CSC_P, bitwise_equal,
+                           etree_lower, known_answer_csr)
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+needs_ref = pytest.mark.skipif(not oracle.ref_available(),
+                               reason="oracle/_ref not built (reference sources absent)")
+
+
+def digests():
+    with open(os.path.join(GOLD, "digests.json")) as f:
+        return json.load(f)
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a, np.float64).tobytes()).hexdigest()
+
+
+# --------------------------------------------------------- known answers
+def test_known_answer_csc_and_csr():
+    # Catch_tests/sptrsv_tests.cpp:17-44: x must be exactly 1
+    x = oracle.sptrsv_csc(KA_CSC_P, KA_CSC_I, np.ones(26), KA_B)
+    assert np.all(x == 1.0)
+    Rp, Ri, Rx = known_answer_csr()
+    assert np.all(oracle.sptrsv_csr(Rp, Ri, Rx, KA_B) == 1.0)
+
+
+def test_spmv_known_answers():
+    # Catch_tests/spmv_tests.cpp:23-35
+    Ap = np.array([0, 2, 3, 4, 6], np.int32)
+    Ai = np.array([0, 1, 3, 3, 1, 3], np.int32)
+    Ax = np.array([1.0, 2.0, 1.0, 1.0, 1.0, 1.0])
+    assert np.array_equal(oracle.spmv_csr(Ap, Ai, Ax, np.ones(4)), [3, 1, 1, 2])
+    # Catch_tests/spmv_tests.cpp:53-66
+    Ap = np.array([0, 5, 8, 12, 16, 19, 24, 29, 34, 39, 44], np.int32)
+    Ai = np.array([0, 1, 5, 7, 8, 0, 1, 7, 2, 6, 7, 9, 3, 4, 5, 8, 3, 4, 6, 0, 3,
+                   5, 6, 8, 2, 4, 5, 6, 9, 0, 1, 2, 7, 9, 0, 3, 5, 8, 9, 2, 6, 7,
+                   8, 9], np.int32)
+    y = oracle.spmv_csr(Ap, Ai, np.ones(44), np.ones(10))
+    assert np.array_equal(y, [5, 3, 4, 4, 3, 5, 5, 5, 5, 5])
+
+
+def test_spmv_laplacian_closed_form():
+    # tril of the 2D 5-point Laplacian times 1: 4 minus the number of lower
+    # neighbours (x > 0) + (y > 0)
+    nx, ny = 13, 9
+    Lp, Li, Lx = W.lap2d_tril(nx, ny)
+    y = oracle.spmv_csr(Lp, Li, Lx, np.ones(nx * ny))
+    xs, ys = np.meshgrid(np.arange(nx), np.arange(ny))
+    want = 4.0 - (xs > 0).astype(float).ravel() - (ys > 0).astype(float).ravel()
+    assert np.array_equal(y, want)
+    z0 = np.arange(nx * ny, dtype=float)
+    yv = np.full(nx * ny, 2.0)
+    z = oracle.spmv_csr_axpby(Lp, Li, Lx, np.ones(nx * ny), yv, z0, 3.0, -0.5)
+    assert np.array_equal(z, 3.0 * (z0 + want) - 1.0)
+
+
+def test_solve_closed_form_recovers_x_true():
+    w = W.config1()
+    x = oracle.sptrsv_csr(w.Lp, w.Li, w.Lx, w.b)
+    assert np.max(np.abs(x - w.x_true)) < 1e-13
+
+
+# ------------------------------------------------------- golden vectors
+def test_c1_golden_vector():
+    w = W.config1()
+    x = oracle.sptrsv_csr(w.Lp, w.Li, w.Lx, w.b)
+    gold = np.load(os.path.join(GOLD, "c1_x.npy"))
+    assert bitwise_equal(x, gold)
+
+
+def test_c2_c5_golden_digests():
+    d = digests()
+    for name, g in (("c5", 64), ("c2", 128)):
+        Lp, Li, Lx = W.ic0_lap3d(g, 0.01)
+        assert sha(Lx) == d[f"{name}_ic0_Lx"]["sha256"], name
+        if name == "c2":
+            x_true = W.uniform(1, 0, len(Lp) - 1)
+            b = oracle.spmv_csr(Lp, Li, Lx, x_true)
+            assert sha(b) == d["c2_b"]["sha256"]
+            assert sha(oracle.sptrsv_csr(Lp, Li, Lx, b)) == d["c2_x"]["sha256"]
+        else:
+            b = W.uniform(4, 0, len(Lp) - 1)
+            x = oracle.sptrsv_csr(Lp, Li, Lx, b)
+            assert sha(x) == d["c5_x"]["sha256"]
+            Cp, Ci, Cx, _ = oracle.csr_to_csc(Lp, Li, Lx)
+            assert sha(oracle.ltsolve(Cp, Ci, Cx, x)) == d["c5_ltsolve_x"]["sha256"]
+
+
+def test_c3_c4_golden_digests():
+    d = digests()
+    w3 = W.config3(k=2)
+    assert w3.nnz == d["c3_nnz"]
+    X = oracle.sptrsm_csr(w3.Lp, w3.Li, w3.Lx, w3.b)
+    for c in range(2):
+        assert sha(X[c]) == d[f"c3_x_col{c}"]["sha256"]
+    w4 = W.config4()
+    assert w4.n == d["c4_n"]
+    # the batch as one block-diagonal matrix (global column ids)
+    x = oracle.sptrsv_csr(w4.Lp, w4.meta["global_Li"], w4.Lx, w4.b)
+    assert sha(x) == d["c4_x"]["sha256"]
+
+
+# --------------------------------------------------- reference bit-parity
+@needs_ref
+@pytest.mark.parametrize("seed,n,density", [(1, 10, 0.2), (2, 100, 0.03), (3, 500, 0.004),
+                                            (4, 1000, 0.0005), (5, 10000, 0.0001),
+                                            (6, 3000, 0.003)])
+def test_solves_match_reference(seed, n, density):
+    Lp, Li, Lx = W.random_lower(n, density, 1, seed=seed)
+    b = W.uniform(seed, 0, n)
+    x = oracle.sptrsv_csr(Lp, Li, Lx, b)
+    assert bitwise_equal(x, ref.sptrsv_csr(Lp, Li, Lx, b))
+    lp, ls, _ = oracle.levelset_csr(Lp, Li)
+    assert bitwise_equal(oracle.sptrsv_csr_levelset(Lp, Li, Lx, b, lp, ls, 4), x)
+    assert bitwise_equal(ref.sptrsv_csr_levelset(Lp, Li, Lx, b, lp, ls), x)
+    Cp, Ci, Cx, _ = oracle.csr_to_csc(Lp, Li, Lx)
+    assert bitwise_equal(oracle.sptrsv_csc(Cp, Ci, Cx, b), ref.sptrsv_csc(Cp, Ci, Cx, b))
+    # the column-push CSC solve accumulates in the same order as the CSR pull
+    assert bitwise_equal(oracle.sptrsv_csc(Cp, Ci, Cx, b), x)
+    assert bitwise_equal(oracle.ltsolve(Cp, Ci, Cx, b), ref.ltsolve(Cp, Ci, Cx, b))
+    assert bitwise_equal(oracle.spmv_csr(Lp, Li, Lx, b), ref.spmv_csr(Lp, Li, Lx, b))
+    z0 = W.uniform(seed + 100, 0, n)
+    assert bitwise_equal(oracle.spmv_csr_axpby(Lp, Li, Lx, b, z0, z0 * 0.5, 0.7, -1.3),
+                         ref.spmv_csr_axpby(Lp, Li, Lx, b, z0, z0 * 0.5, 0.7, -1.3))
+
+
+@needs_ref
+def test_lbc_matches_reference():
+    Lp, Li, Lx = W.random_lower(2000, 0.002, 1, seed=9)
+    b = W.uniform(9, 0, 2000)
+    lp, ls, lev = oracle.levelset_csr(Lp, Li)
+    # a coarsened schedule: 3 levels per coarse level, one partition each
+    nl = len(lp) - 1
+    cl = list(range(0, nl, 3)) + [nl]
+    level_ptr = np.arange(len(cl), dtype=np.int32)
+    par_ptr = np.array([lp[c] for c in cl], np.int32)
+    assert oracle.check_schedule(Lp, Li, level_ptr, par_ptr, ls) == 0
+    x = oracle.sptrsv_csr_lbc(Lp, Li, Lx, b, level_ptr, par_ptr, ls)
+    assert bitwise_equal(x, ref.sptrsv_csr_lbc(Lp, Li, Lx, b, level_ptr, par_ptr, ls))
+    assert bitwise_equal(x, oracle.sptrsv_csr(Lp, Li, Lx, b))
+
+
+@needs_ref
+@pytest.mark.parametrize("g", [3, 6, 12])
+def test_ic0_matches_reference(g):
+    Lp, Li, A = W.lap3d_tril(g, g, g, 6.01, -1.0)
+    Cp, Ci, Cx, mp = oracle.csr_to_csc(Lp, Li, A)
+    pp, ps = oracle.prune_set(Lp, Li)
+    mine, ok = oracle.spic0_csc_left(Cp, Ci, Cx, pp, ps)
+    theirs, ok2 = ref.spic0_csc_left(Cp, Ci, Cx, pp, ps)
+    assert ok and ok2 and bitwise_equal(mine, theirs)
+
+
+# ------------------------------------------------------ IC(0) textbook
+def dense_ic0(n, Lp, Li, A):
+    """Dense textbook IC(0) restricted to the pattern: s ascending."""
+    pat = np.zeros((n, n), bool)
+    M = np.zeros((n, n))
+    for i in range(n):
+        for p in range(Lp[i], Lp[i + 1]):
+            pat[i, Li[p]] = True
+            M[i, Li[p]] = A[p]
+    L = np.zeros((n, n))
+    for c in range(n):
+        d = M[c, c]
+        for s in range(c):
+            d -= L[c, s] * L[c, s]
+        L[c, c] = np.sqrt(d)
+        for r in range(c + 1, n):
+            if not pat[r, c]:
+                continue
+            f = M[r, c]
+            for s in range(c):
+                f -= L[r, s] * L[c, s]
+            L[r, c] = f / L[c, c]
+    return np.array([L[i, Li[p]] for i in range(n) for p in range(Lp[i], Lp[i + 1])])
+
+
+@pytest.mark.parametrize("shape", [(4, 4, 1), (3, 3, 3), (5, 4, 3)])
+def test_ic0_textbook(shape):
+    nx, ny, nz = shape
+    Lp, Li, A = W.lap3d_tril(nx, ny, nz, 6.01, -1.0)
+    n = len(Lp) - 1
+    dense = dense_ic0(n, Lp, Li, A)
+    Cp, Ci, Cx, mp = oracle.csr_to_csc(Lp, Li, A)
+    pp, ps = oracle.prune_set(Lp, Li)
+    lc, ok = oracle.spic0_csc_left(Cp, Ci, Cx, pp, ps)
+    assert ok and bitwise_equal(lc[mp], dense)
+    # a non-SPD input is reported
+    bad = Cx.copy()
+    bad[Cp[:-1]] = 0.5
+    assert not oracle.spic0_csc_left(Cp, Ci, bad, pp, ps)[1]
+
+
+def test_reference_spic0_csr_is_not_textbook():
+    # the reference's spic0_csr (spic0.cpp:106-127) skips Schur updates; the
+    # recorded factor of the 4x4-grid Laplacian shows L[1,1] = 2 where
+    # textbook IC(0) gives sqrt(3.75)
+    got = np.load(os.path.join(GOLD, "spic0_csr_4x4.npy"))
+    Lp, Li, A = W.lap2d_tril(4, 4, 4.0, -1.0)
+    Cp, Ci, Cx, mp = oracle.csr_to_csc(Lp, Li, A)
+    pp, ps = oracle.prune_set(Lp, Li)
+    tb, ok = oracle.spic0_csc_left(Cp, Ci, Cx, pp, ps)
+    assert got[Cp[1]] == 2.0 and tb[Cp[1]] == np.sqrt(3.75)
+    assert np.max(np.abs(got - tb)) > 0.05
+
+
+# --------------------------------------------------------- level sets
+def brute_levels(Lp, Li):
+    """Longest path into each node by memoised recursion over dependencies
+    (a different algorithm from the oracle's single forward pass)."""
+    n = len(Lp) - 1
+    memo = {}
+
+    def depth(i):
+        if i in memo:
+            return memo[i]
+        deps = [Li[p] for p in range(Lp[i], Lp[i + 1] - 1)]
+        memo[i] = 0 if not deps else 1 + max(depth(j) for j in deps)
+        return memo[i]
+
+    return np.array([depth(i) for i in range(n)])
+
+
+def lp_ls_from_levels(lev):
+    nl = int(lev.max()) + 1 if len(lev) else 0
+    lp = np.zeros(nl + 1, np.int32)
+    for l in lev:
+        lp[l + 1] += 1
+    lp = np.cumsum(lp).astype(np.int32)
+    ls = np.argsort(lev, kind="stable").astype(np.int32)
+    return lp, ls
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_levels_brute_force(seed):
+    Lp, Li, _ = W.random_lower(60, 0.08, 0, seed=seed)
+    lev = brute_levels(Lp, Li)
+    lp, ls, lev_o = oracle.levelset_csr(Lp, Li)
+    assert np.array_equal(lev_o, lev)
+    blp, bls = lp_ls_from_levels(lev)
+    assert np.array_equal(lp, blp) and np.array_equal(ls, bls)
+    Cp, Ci, _, _ = oracle.csr_to_csc(Lp, Li, np.ones(len(Li)))
+    clp, cls = oracle.levelset_csc(Cp, Ci)
+    assert np.array_equal(clp, lp) and np.array_equal(cls, ls)
+
+
+def test_levels_grid_closed_forms():
+    nx, ny = 17, 11
+    Lp, Li, _ = W.lap2d_tril(nx, ny)
+    lp, ls, lev = oracle.levelset_csr(Lp, Li)
+    xs, ys = np.meshgrid(np.arange(nx), np.arange(ny))
+    assert np.array_equal(lev, (xs + ys).ravel())
+    assert len(lp) - 1 == nx + ny - 1
+    g = 9
+    Lp, Li, _ = W.lap3d_tril(g, g, g)
+    lp, ls, lev = oracle.levelset_csr(Lp, Li)
+    z, y, x = np.meshgrid(np.arange(g), np.arange(g), np.arange(g), indexing="ij")
+    assert np.array_equal(lev, (x + y + z).ravel())
+    assert len(lp) - 1 == 3 * g - 2
+
+
+def test_levels_match_reference_tree_fixtures():
+    # parsy getLevelSet (TreeUtils.cpp:89-141) on forests, from the reference
+    f = np.load(os.path.join(GOLD, "tree_levels.npz"))
+    t = 0
+    while f"parent{t}" in f:
+        Lp, Li, _ = etree_lower(f[f"parent{t}"])
+        lp, ls, _ = oracle.levelset_csr(Lp, Li)
+        assert np.array_equal(lp, f[f"level_ptr{t}"]), t
+        assert np.array_equal(ls, f[f"level_set{t}"]), t
+        t += 1
+    assert t == 4
+
+
+# ----------------------------------------------------------- schedules
+def test_check_schedule_rejects_planted_errors():
+    Lp, Li, _ = W.lap2d_tril(6, 5)
+    lp, ls, _ = oracle.levelset_csr(Lp, Li)
+    nl = len(lp) - 1
+    level_ptr = np.arange(nl + 1, dtype=np.int32)  # one partition per level
+    assert oracle.check_schedule(Lp, Li, level_ptr, lp, ls) == 0
+    dup = ls.copy()
+    dup[1] = dup[0]
+    assert oracle.check_schedule(Lp, Li, level_ptr, lp, dup) == 2
+    rev = ls[::-1].copy()  # every dependency points forward
+    assert oracle.check_schedule(Lp, Li, level_ptr, lp, rev) == 3
+    # merge levels 1 and 2 but split their rows into two partitions
+    lp2 = np.array([0, 1, lp[2], lp[3]] + list(lp[4:]), np.int32)
+    lvl2 = np.array([0, 1, 3] + list(range(4, len(lp2))), np.int32)
+    assert oracle.check_schedule(Lp, Li, lvl2, lp2, ls) == 4
+    # one partition, rows in reverse level order inside it
+    one = np.array([0, 1], np.int32)
+    assert oracle.check_schedule(Lp, Li, one, np.array([0, len(ls)], np.int32), rev) == 5
+    assert oracle.check_schedule(Lp, Li, one, np.array([0, len(ls)], np.int32), ls) == 0
+
+
+# -------------------------------------------------------------- adjoint
+def test_adjoint_against_dense_gradient():
+    # IC(0) of the 16^3 Laplacian (n = 4096): gradient of 0.5 ||L^-1 b||^2
+    Lp, Li, Lx = W.ic0_lap3d(16)
+    n = len(Lp) - 1
+    b = W.uniform(4, 0, n)
+    x = oracle.sptrsv_csr(Lp, Li, Lx, b)
+    gb, gL = oracle.adjoint(Lp, Li, Lx, x, x)
+    D = np.zeros((n, n))
+    rows = np.repeat(np.arange(n), np.diff(Lp))
+    D[rows, Li] = Lx
+    xd = np.linalg.solve(D, b)
+    gbd = np.linalg.solve(D.T, xd)
+    assert np.max(np.abs(gb - gbd)) / np.max(np.abs(gbd)) < 1e-12
+    gLd = -np.outer(gbd, xd)[rows, Li]
+    assert np.max(np.abs(gL - gLd)) / np.max(np.abs(gLd)) < 1e-12
+
+
+def test_adjoint_finite_differences():
+    Lp, Li, Lx = W.random_lower(300, 0.02, 1, seed=4)
+    n = len(Lp) - 1
+    b = W.uniform(8, 0, n)
+
+    def loss(Lx_, b_):
+        x = oracle.sptrsv_csr(Lp, Li, Lx_, b_)
+        return 0.5 * float(x @ x)
+
+    x = oracle.sptrsv_csr(Lp, Li, Lx, b)
+    gb, gL = oracle.adjoint(Lp, Li, Lx, x, x)
+    rng = np.random.default_rng(0)
+    h = 1e-6
+    for p in rng.choice(len(Lx), 24, replace=False):
+        e = np.zeros_like(Lx)
+        e[p] = h
+        fd = (loss(Lx + e, b) - loss(Lx - e, b)) / (2 * h)
+        assert abs(fd - gL[p]) <= 1e-6 * max(1.0, abs(gL[p])), p
+    for i in rng.choice(n, 12, replace=False):
+        e = np.zeros(n)
+        e[i] = h
+        fd = (loss(Lx, b + e) - loss(Lx, b - e)) / (2 * h)
+        assert abs(fd - gb[i]) <= 1e-6 * max(1.0, abs(gb[i])), i
+
+
+def test_csr_csc_round_trip():
+    Lp, Li, Lx = W.random_lower(500, 0.01, 1, seed=1)
+    Cp, Ci, Cx, mp = oracle.csr_to_csc(Lp, Li, Lx)
+    Rp, Ri, Rx, mp2 = oracle.csr_to_csc(Cp, Ci, Cx)
+    assert np.array_equal(Rp, Lp) and np.array_equal(Ri, Li) and np.array_equal(Rx, Lx)
+    assert np.array_equal(Cx[mp], Lx)
+    # diagonal first in every column of the CSC
+    assert np.array_equal(Ci[Cp[:-1]], np.arange(500))
