@@ -49,7 +49,7 @@ typedef enum { SYMAMD_COL_MAJOR = 0, SYMAMD_ROW_MAJOR = 1 } symamd_layout;
 typedef enum {
   SYMAMD_EXEC_AUTO = 0,      /* tuned choice if symamd_tune ran, else heuristic */
   SYMAMD_EXEC_LEVELSET = 1,  /* one kernel launch per level (plain level set)   */
-  SYMAMD_EXEC_SYNCFREE = 2,  /* one persistent launch, per-row ready flags      */
+  SYMAMD_EXEC_SYNCFREE = 2,  /* one persistent launch, per-unit ready flags     */
   SYMAMD_EXEC_LEVELSET_GRAPH = 3 /* the per-level launches replayed as a hipGraph */
 } symamd_executor;
 
@@ -64,6 +64,12 @@ int symamd_set_stream(symamd_handle h, void *stream);
 /* Sync-free watchdog: a wait longer than `ms` milliseconds aborts the solve
  * with SYMAMD_ERR_TIMEOUT (default 20000). */
 int symamd_set_timeout_ms(symamd_handle h, double ms);
+/* Handle options.
+ *   SYMAMD_OPT_TICKETS: 1 = persistent kernels take work units from a device
+ *     ticket counter instead of a co-resident round-robin deal (the mode
+ *     used automatically inside graph capture); 0 = automatic (default). */
+enum { SYMAMD_OPT_TICKETS = 1 };
+int symamd_set_option(symamd_handle h, int option, int64_t value);
 const char *symamd_last_error(void);
 /* Library version string. */
 const char *symamd_version(void);

@@ -61,6 +61,7 @@ def lib() -> ctypes.CDLL:
             "symamd_destroy": [vp],
             "symamd_set_stream": [vp, vp],
             "symamd_set_timeout_ms": [vp, dbl],
+            "symamd_set_option": [vp, i32, i64],
             "symamd_levelset": [vp, i32, vp, vp, vp, vp, POINTER(i32)],
             "symamd_coarsen": [vp, i32, vp, vp, i32, i32, vp, vp, vp,
                                POINTER(i32), POINTER(i32)],
@@ -135,6 +136,11 @@ class Handle:
         self.stream = stream
         if timeout_ms is not None:
             _check(lib().symamd_set_timeout_ms(self._h, float(timeout_ms)))
+
+    def set_tickets(self, on: bool):
+        """Persistent kernels take units from a ticket counter (True) or are
+        dealt round-robin from a cooperative launch (False, default)."""
+        _check(lib().symamd_set_option(self._h, 1, int(bool(on))))
 
     def __del__(self):
         if getattr(self, "_h", None) and _lib is not None:

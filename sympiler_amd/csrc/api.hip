@@ -43,11 +43,15 @@ int strides(int n, int k, int64_t ld, int layout, int64_t *rs, int64_t *cs) {
 void free_plan_arrays(symamd_plan p) {
   void *ptrs[] = {p->d_lvl_ptr, p->d_perm, p->d_rp,   p->d_ci,   p->d_val,
                   p->d_fmap,    p->d_orp,  p->d_trp,  p->d_tci,  p->d_tval,
-                  p->d_tmap,    p->d_flags, p->d_state};
+                  p->d_tmap,    p->d_flags, p->d_state, p->d_ipos};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
-  for (auto &kv : p->units)
-    if (kv.second.d_ub) (void)hipFree(kv.second.d_ub);
+  for (auto &kv : p->units) {
+    UnitSet &u = kv.second;
+    void *up[] = {u.d_ub, u.d_dptr, u.d_dlist, u.d_rdptr, u.d_rdlist, u.d_uflags};
+    for (void *q : up)
+      if (q) (void)hipFree(q);
+  }
   for (auto &kv : p->gcache) {
     if (kv.second.second) (void)hipGraphExecDestroy(kv.second.second);
     if (kv.second.first) (void)hipGraphDestroy(kv.second.first);
@@ -318,7 +322,7 @@ int capture_solve(symamd_handle h, symamd_plan p, void *d_B, int k, int64_t rs,
                   int64_t cs, const Choice &c, hipGraph_t *g, hipGraphExec_t *ge) {
   if (c.exec == SYMAMD_EXEC_SYNCFREE) {
     UnitSet *us = nullptr;
-    SYMAMD_TRY(plan_units(h, p, forward_rows_per_unit(k, c.lanes), &us));
+    SYMAMD_TRY(plan_units(h, p, forward_rows_per_unit(k, c.lanes), false, &us));
   }
   hipStream_t cap = nullptr;
   SYMAMD_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
@@ -406,6 +410,17 @@ int symamd_set_stream(symamd_handle h, void *stream) {
   SYMAMD_TRY(check_handle(h));
   h->stream = (hipStream_t)stream;
   return SYMAMD_OK;
+}
+
+int symamd_set_option(symamd_handle h, int option, int64_t value) {
+  SYMAMD_TRY(check_handle(h));
+  switch (option) {
+    case SYMAMD_OPT_TICKETS:
+      h->force_tickets = value != 0;
+      return SYMAMD_OK;
+    default:
+      return fail(SYMAMD_ERR_INVALID_ARG, "unknown option %d", option);
+  }
 }
 
 int symamd_set_timeout_ms(symamd_handle h, double ms) {

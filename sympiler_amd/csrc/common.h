@@ -95,10 +95,19 @@ __device__ __forceinline__ void persistent_finish(int *state, int total_waves) {
 
 // ------------------------------------------------------------------ plan
 
+// Work units of R consecutive level-ordered rows inside one level, with the
+// units their rows depend on (forward) and the units that depend on them
+// (reverse, for the transposed solve).
 struct UnitSet {
-  int rows = 0;          // rows per unit (cap)
-  int count = 0;         // number of units
-  int *d_ub = nullptr;   // unit u covers positions [ub[u], ub[u+1])
+  int rows = 0;             // rows per unit (cap)
+  int count = 0;            // number of units
+  int *d_ub = nullptr;      // unit u covers positions [ub[u], ub[u+1])
+  int *d_dptr = nullptr;    // forward dependency units of u:
+  int *d_dlist = nullptr;   //   dlist[dptr[u] .. dptr[u+1])
+  int *d_rdptr = nullptr;   // reverse dependency units (built on demand)
+  int *d_rdlist = nullptr;
+  int *d_uflags = nullptr;  // ready tags, one per unit
+  int64_t deps = 0, rdeps = 0;
 };
 
 struct Choice {
@@ -113,6 +122,7 @@ struct symamd_handle_s {
   int device = 0;
   int num_cus = 0;
   double timeout_ms = 20000.0;
+  int force_tickets = 0;  // persistent kernels take units from a ticket counter
   uint64_t timeout_ticks() const { return (uint64_t)(timeout_ms * 1.0e5); }
 };
 
@@ -130,6 +140,7 @@ struct symamd_plan_s {
   int *d_ci = nullptr;                 // level-ordered column ids (row ids), nnz
   void *d_val = nullptr;               // level-ordered values, nnz
   int *d_fmap = nullptr;               // caller CSR position of each entry
+  int *d_ipos = nullptr;               // row -> position
   int *d_orp = nullptr;                // caller CSR row pointers, n+1
   // transposed copy (CSC of L, level order), built on first adjoint
   int *d_trp = nullptr;
@@ -161,8 +172,10 @@ struct symamd_graph_s {
 
 namespace symamd {
 
-// Ensures the unit partition with `rows` rows per unit exists.
-int plan_units(symamd_handle h, symamd_plan p, int rows, UnitSet **out);
+// Ensures the unit partition with `rows` rows per unit exists, with its
+// forward dependency lists (and the reverse ones when `reverse`).
+int plan_units(symamd_handle h, symamd_plan p, int rows, bool reverse,
+               UnitSet **out);
 // Ensures the transposed (CSC) copy exists.
 int plan_transpose(symamd_handle h, symamd_plan p);
 

@@ -407,41 +407,193 @@ int gather_values(symamd_handle h, symamd_plan p, const void *d_Lx) {
   return SYMAMD_OK;
 }
 
-int plan_units(symamd_handle h, symamd_plan p, int rows, UnitSet **out) {
-  auto it = p->units.find(rows);
-  if (it != p->units.end()) {
-    *out = &it->second;
-    return SYMAMD_OK;
+namespace {
+
+// unit of every position (thread per unit; units hold at most a few hundred
+// rows)
+__global__ void k_unit_of_pos(int U, const int *__restrict__ ub, int *uop) {
+  int u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= U) return;
+  for (int q = ub[u]; q < ub[u + 1]; ++q) uop[q] = u;
+}
+
+__global__ void k_inverse_perm(int n, const int *__restrict__ perm, int *ipos) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) ipos[perm[q]] = q;
+}
+
+// Dependency entries of a unit: the off-diagonal entries of its rows.  In
+// the forward copy the diagonal is the last entry of a row, in the
+// transposed copy the first; either way a row contributes len-1 entries.
+__global__ void k_unit_dep_count(int U, const int *__restrict__ ub,
+                                 const int *__restrict__ rp, int *cnt) {
+  int u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u > U) return;
+  if (u == U) { cnt[U] = 0; return; }
+  int c = 0;
+  for (int q = ub[u]; q < ub[u + 1]; ++q) c += rp[q + 1] - rp[q] - 1;
+  cnt[u] = c;
+}
+
+__global__ void k_unit_dep_keys(int U, const int *__restrict__ ub,
+                                const int *__restrict__ rp,
+                                const int *__restrict__ ci, int skip_first,
+                                const int *__restrict__ ipos,
+                                const int *__restrict__ uop,
+                                const int *__restrict__ off, uint64_t *keys) {
+  int u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= U) return;
+  int o = off[u];
+  for (int q = ub[u]; q < ub[u + 1]; ++q) {
+    const int b = rp[q] + (skip_first ? 1 : 0);
+    const int e = rp[q + 1] - (skip_first ? 0 : 1);
+    for (int j = b; j < e; ++j)
+      keys[o++] = ((uint64_t)(uint32_t)u << 32) | (uint32_t)uop[ipos[ci[j]]];
   }
-  UnitSet us;
-  us.rows = rows;
-  const int nlev = p->levels;
-  // count on the host from the level pointers (levels+1 ints, already there)
-  int64_t U = 0;
-  for (int l = 0; l < nlev; ++l)
-    U += (p->h_lvl_ptr[l + 1] - p->h_lvl_ptr[l] + rows - 1) / rows;
-  us.count = (int)U;
-  SYMAMD_HIP(hipMalloc(&us.d_ub, sizeof(int) * (U + 1)));
-  if (nlev > 0) {
-    int *d_upl = nullptr, *d_uoff = nullptr;
-    SYMAMD_HIP(hipMallocAsync(&d_upl, sizeof(int) * (nlev + 1), h->stream));
-    SYMAMD_HIP(hipMallocAsync(&d_uoff, sizeof(int) * (nlev + 1), h->stream));
-    k_units_per_level<<<blocks_for(nlev + 1), 256, 0, h->stream>>>(
-        nlev, p->d_lvl_ptr, rows, d_upl);
-    SYMAMD_TRY(scan_exclusive(h, d_upl, d_uoff, nlev + 1));
-    k_unit_bounds<<<blocks_for(U + 1), 256, 0, h->stream>>>(
-        nlev, (int)U, p->n, p->d_lvl_ptr, d_uoff, rows, us.d_ub);
-    SYMAMD_HIP(hipGetLastError());
-    SYMAMD_HIP(hipFreeAsync(d_upl, h->stream));
-    SYMAMD_HIP(hipFreeAsync(d_uoff, h->stream));
-  } else {
-    int zero = 0;
-    SYMAMD_HIP(hipMemcpyAsync(us.d_ub, &zero, sizeof(int),
-                              hipMemcpyHostToDevice, h->stream));
-  }
+}
+
+__global__ void k_mark_unique(int64_t m, const uint64_t *__restrict__ keys,
+                              int *flag) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < m) flag[t] = (t == 0 || keys[t] != keys[t - 1]) ? 1 : 0;
+  if (t == m) flag[m] = 0;
+}
+
+__global__ void k_compact_deps(int64_t m, const uint64_t *__restrict__ keys,
+                               const int *__restrict__ flag,
+                               const int *__restrict__ pos, int *dlist,
+                               int *dcount) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= m || !flag[t]) return;
+  dlist[pos[t]] = (int)(keys[t] & 0xffffffffu);
+  atomicAdd(&dcount[(int)(keys[t] >> 32)], 1);
+}
+
+// Deduplicated dependency-unit lists of every unit, from the row lists
+// (rp, ci) of the level-ordered copy (forward) or of the transposed copy
+// (reverse, skip_first = 1).
+int build_unit_deps(symamd_handle h, symamd_plan p, const UnitSet &us,
+                    const int *rp, const int *ci, int skip_first, int *uop,
+                    int **d_dptr, int **d_dlist, int64_t *total) {
+  const int U = us.count;
+  int *d_cnt = nullptr, *d_off = nullptr;
+  SYMAMD_HIP(hipMalloc(d_dptr, sizeof(int) * (U + 1)));
+  SYMAMD_HIP(hipMallocAsync(&d_cnt, sizeof(int) * (U + 1), h->stream));
+  SYMAMD_HIP(hipMallocAsync(&d_off, sizeof(int) * (U + 1), h->stream));
+  k_unit_dep_count<<<blocks_for(U + 1), 256, 0, h->stream>>>(U, us.d_ub, rp, d_cnt);
+  SYMAMD_HIP(hipGetLastError());
+  SYMAMD_TRY(scan_exclusive(h, d_cnt, d_off, U + 1));
+  int m = 0;
+  SYMAMD_HIP(hipMemcpyAsync(&m, d_off + U, sizeof(int), hipMemcpyDeviceToHost, h->stream));
   SYMAMD_HIP(hipStreamSynchronize(h->stream));
-  auto res = p->units.emplace(rows, us);
-  *out = &res.first->second;
+  uint64_t *d_keys = nullptr, *d_skeys = nullptr;
+  int *d_flag = nullptr, *d_pos = nullptr;
+  const int64_t mm = m > 0 ? m : 1;
+  SYMAMD_HIP(hipMallocAsync(&d_keys, 8 * mm, h->stream));
+  SYMAMD_HIP(hipMallocAsync(&d_skeys, 8 * mm, h->stream));
+  SYMAMD_HIP(hipMallocAsync(&d_flag, 4 * (mm + 1), h->stream));
+  SYMAMD_HIP(hipMallocAsync(&d_pos, 4 * (mm + 1), h->stream));
+  k_unit_dep_keys<<<blocks_for(U), 256, 0, h->stream>>>(U, us.d_ub, rp, ci, skip_first,
+                                                        p->d_ipos, uop, d_off, d_keys);
+  SYMAMD_HIP(hipGetLastError());
+  unsigned ubits = 1;
+  while ((1u << ubits) < (unsigned)U && ubits < 31) ++ubits;
+  if (m > 0) {
+    size_t bytes = 0;
+    SYMAMD_HIP(rocprim::radix_sort_keys(nullptr, bytes, d_keys, d_skeys, (size_t)m, 0,
+                                        32 + ubits, h->stream));
+    void *tmp = nullptr;
+    SYMAMD_HIP(hipMallocAsync(&tmp, bytes ? bytes : 4, h->stream));
+    SYMAMD_HIP(rocprim::radix_sort_keys(tmp, bytes, d_keys, d_skeys, (size_t)m, 0,
+                                        32 + ubits, h->stream));
+    SYMAMD_HIP(hipFreeAsync(tmp, h->stream));
+  }
+  k_mark_unique<<<blocks_for(m + 1), 256, 0, h->stream>>>(m, d_skeys, d_flag);
+  SYMAMD_HIP(hipGetLastError());
+  SYMAMD_TRY(scan_exclusive(h, d_flag, d_pos, m + 1));
+  int uniq = 0;
+  SYMAMD_HIP(hipMemcpyAsync(&uniq, d_pos + m, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  SYMAMD_HIP(hipStreamSynchronize(h->stream));
+  SYMAMD_HIP(hipMalloc(d_dlist, sizeof(int) * (uniq > 0 ? uniq : 1)));
+  SYMAMD_HIP(hipMemsetAsync(d_cnt, 0, sizeof(int) * (U + 1), h->stream));
+  k_compact_deps<<<blocks_for(m), 256, 0, h->stream>>>(m, d_skeys, d_flag, d_pos,
+                                                        *d_dlist, d_cnt);
+  SYMAMD_HIP(hipGetLastError());
+  SYMAMD_TRY(scan_exclusive(h, d_cnt, *d_dptr, U + 1));
+  SYMAMD_HIP(hipFreeAsync(d_keys, h->stream));
+  SYMAMD_HIP(hipFreeAsync(d_skeys, h->stream));
+  SYMAMD_HIP(hipFreeAsync(d_flag, h->stream));
+  SYMAMD_HIP(hipFreeAsync(d_pos, h->stream));
+  SYMAMD_HIP(hipFreeAsync(d_cnt, h->stream));
+  SYMAMD_HIP(hipFreeAsync(d_off, h->stream));
+  SYMAMD_HIP(hipStreamSynchronize(h->stream));
+  *total = uniq;
+  return SYMAMD_OK;
+}
+
+}  // namespace
+
+int plan_units(symamd_handle h, symamd_plan p, int rows, bool reverse,
+               UnitSet **out) {
+  auto it = p->units.find(rows);
+  if (it == p->units.end()) {
+    UnitSet us;
+    us.rows = rows;
+    const int nlev = p->levels;
+    // count on the host from the level pointers (levels+1 ints, already there)
+    int64_t U = 0;
+    for (int l = 0; l < nlev; ++l)
+      U += (p->h_lvl_ptr[l + 1] - p->h_lvl_ptr[l] + rows - 1) / rows;
+    us.count = (int)U;
+    SYMAMD_HIP(hipMalloc(&us.d_ub, sizeof(int) * (U + 1)));
+    SYMAMD_HIP(hipMalloc(&us.d_uflags, sizeof(int) * (U > 0 ? U : 1)));
+    SYMAMD_HIP(hipMemsetAsync(us.d_uflags, 0, sizeof(int) * (U > 0 ? U : 1), h->stream));
+    if (nlev > 0) {
+      int *d_upl = nullptr, *d_uoff = nullptr;
+      SYMAMD_HIP(hipMallocAsync(&d_upl, sizeof(int) * (nlev + 1), h->stream));
+      SYMAMD_HIP(hipMallocAsync(&d_uoff, sizeof(int) * (nlev + 1), h->stream));
+      k_units_per_level<<<blocks_for(nlev + 1), 256, 0, h->stream>>>(
+          nlev, p->d_lvl_ptr, rows, d_upl);
+      SYMAMD_TRY(scan_exclusive(h, d_upl, d_uoff, nlev + 1));
+      k_unit_bounds<<<blocks_for(U + 1), 256, 0, h->stream>>>(
+          nlev, (int)U, p->n, p->d_lvl_ptr, d_uoff, rows, us.d_ub);
+      SYMAMD_HIP(hipGetLastError());
+      SYMAMD_HIP(hipFreeAsync(d_upl, h->stream));
+      SYMAMD_HIP(hipFreeAsync(d_uoff, h->stream));
+    } else {
+      int zero = 0;
+      SYMAMD_HIP(hipMemcpyAsync(us.d_ub, &zero, sizeof(int),
+                                hipMemcpyHostToDevice, h->stream));
+    }
+    it = p->units.emplace(rows, us).first;
+  }
+  UnitSet &us = it->second;
+  const bool need_fwd = !us.d_dptr, need_rev = reverse && !us.d_rdptr;
+  if (need_fwd || need_rev) {
+    if (!p->d_ipos) {
+      SYMAMD_HIP(hipMalloc(&p->d_ipos, sizeof(int) * (p->n > 0 ? p->n : 1)));
+      k_inverse_perm<<<blocks_for(p->n), 256, 0, h->stream>>>(p->n, p->d_perm, p->d_ipos);
+      SYMAMD_HIP(hipGetLastError());
+    }
+    int *d_uop = nullptr;
+    SYMAMD_HIP(hipMallocAsync(&d_uop, sizeof(int) * (p->n > 0 ? p->n : 1), h->stream));
+    k_unit_of_pos<<<blocks_for(us.count), 256, 0, h->stream>>>(us.count, us.d_ub, d_uop);
+    SYMAMD_HIP(hipGetLastError());
+    int rc = SYMAMD_OK;
+    if (need_fwd)
+      rc = build_unit_deps(h, p, us, p->d_rp, p->d_ci, 0, d_uop, &us.d_dptr,
+                           &us.d_dlist, &us.deps);
+    if (rc == SYMAMD_OK && need_rev) {
+      rc = plan_transpose(h, p);
+      if (rc == SYMAMD_OK)
+        rc = build_unit_deps(h, p, us, p->d_trp, p->d_tci, 1, d_uop, &us.d_rdptr,
+                             &us.d_rdlist, &us.rdeps);
+    }
+    (void)hipFreeAsync(d_uop, h->stream);
+    (void)hipStreamSynchronize(h->stream);
+    if (rc != SYMAMD_OK) return rc;
+  }
+  *out = &us;
   return SYMAMD_OK;
 }
 

@@ -112,13 +112,17 @@ def test_random_irregular(dev, n, density, min_off):
         assert rel_err(_solve(plan, b, dev, "syncfree", lanes), x_ref) <= RTOL64
 
 
-def test_c2_full_size(dev):
+@pytest.mark.parametrize("deal", ["round_robin", "tickets"])
+def test_c2_full_size(dev, deal):
     w = W.config2()
     x_ref = oracle.sptrsv_csr(w.Lp, w.Li, w.Lx, w.b)
-    plan = _plan(w.Lp, w.Li, w.Lx, dev)
+    h = sa.Handle()
+    h.set_tickets(deal == "tickets")
+    plan = sa.Plan(*to_dev(w.Lp, w.Li, w.Lx, device=dev), handle=h)
     assert plan.levels == 3 * 128 - 2
-    x = _solve(plan, w.b, dev, "syncfree", 1)
-    assert bitwise_equal(x, x_ref)
+    for _ in range(2):  # a second solve reuses the unit flags (next epoch)
+        x = _solve(plan, w.b, dev, "syncfree", 1)
+        assert bitwise_equal(x, x_ref)
     assert rel_err(x, w.x_true) < 1e-10
     r = plan.residual(to_dev(x, device=dev), to_dev(w.b, device=dev))
     assert r[0] < 1e-14
