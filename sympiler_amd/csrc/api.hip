@@ -43,12 +43,12 @@ int strides(int n, int k, int64_t ld, int layout, int64_t *rs, int64_t *cs) {
 void free_plan_arrays(symamd_plan p) {
   void *ptrs[] = {p->d_lvl_ptr, p->d_perm, p->d_rp,   p->d_ci,   p->d_val,
                   p->d_fmap,    p->d_orp,  p->d_trp,  p->d_tci,  p->d_tval,
-                  p->d_tmap,    p->d_flags, p->d_state, p->d_ipos};
+                  p->d_tmap,    p->d_state, p->d_ipos};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   for (auto &kv : p->units) {
     UnitSet &u = kv.second;
-    void *up[] = {u.d_ub, u.d_dptr, u.d_dlist, u.d_rdptr, u.d_rdlist, u.d_uflags};
+    void *up[] = {u.d_ub, u.d_dptr, u.d_dlist, u.d_uflags};
     for (void *q : up)
       if (q) (void)hipFree(q);
   }
@@ -126,8 +126,7 @@ int analyze_common(symamd_handle h, int n, const int *d_Lp, const int *d_Li,
       hipMalloc(&p->d_orp, sizeof(int) * (nn + 1)) != hipSuccess ||
       hipMalloc(&p->d_ci, sizeof(int) * mm) != hipSuccess ||
       hipMalloc(&p->d_fmap, sizeof(int) * mm) != hipSuccess ||
-      hipMalloc(&p->d_val, vs * mm) != hipSuccess ||
-      hipMalloc(&p->d_flags, sizeof(int) * nn) != hipSuccess)
+      hipMalloc(&p->d_val, vs * mm) != hipSuccess)
     return cleanup(fail(SYMAMD_ERR_ALLOC, "device allocation for the plan failed (n=%d nnz=%lld)", n, (long long)p->nnz));
   if ((rc = check_structure(h, n, d_Lp, d_Li, p->d_state)) != SYMAMD_OK) return cleanup(rc);
   if ((rc = compute_levels(h, n, d_Lp, d_Li, p->d_state, p->d_lvl_ptr, p->d_perm,
@@ -135,8 +134,7 @@ int analyze_common(symamd_handle h, int n, const int *d_Lp, const int *d_Li,
     return cleanup(rc);
   if ((rc = build_plan_arrays(h, p, d_Lp, d_Li, d_Lx)) != SYMAMD_OK) return cleanup(rc);
   if ((rc = gather_values(h, p, d_Lx)) != SYMAMD_OK) return cleanup(rc);
-  if (hipMemsetAsync(p->d_flags, 0, sizeof(int) * nn, h->stream) != hipSuccess ||
-      hipMemsetAsync(p->d_state, 0, sizeof(int) * ST_WORDS, h->stream) != hipSuccess ||
+  if (hipMemsetAsync(p->d_state, 0, sizeof(int) * ST_WORDS, h->stream) != hipSuccess ||
       hipStreamSynchronize(h->stream) != hipSuccess)
     return cleanup(fail(SYMAMD_ERR_HIP, "plan state initialisation failed"));
   for (int l = 0; l < p->levels; ++l)
@@ -320,7 +318,8 @@ std::string graph_key(void *d_B, int k, int64_t rs, int64_t cs, const Choice &c)
 // nothing allocates while capturing.
 int capture_solve(symamd_handle h, symamd_plan p, void *d_B, int k, int64_t rs,
                   int64_t cs, const Choice &c, hipGraph_t *g, hipGraphExec_t *ge) {
-  if (c.exec == SYMAMD_EXEC_SYNCFREE) {
+  {
+    // both executors run on the plan's work units; build them outside capture
     UnitSet *us = nullptr;
     SYMAMD_TRY(plan_units(h, p, forward_rows_per_unit(k, c.lanes), false, &us));
   }

@@ -95,19 +95,24 @@ __device__ __forceinline__ void persistent_finish(int *state, int total_waves) {
 
 // ------------------------------------------------------------------ plan
 
-// Work units of R consecutive level-ordered rows inside one level, with the
-// units their rows depend on (forward) and the units that depend on them
-// (reverse, for the transposed solve).
+// Entries of one work unit staged in a wave's LDS slab (a single row with
+// more entries forms a unit of its own and is read from global memory).
+constexpr int UNIT_EMAX = 512;
+
+// Work units: runs of at most `rows` consecutive level-ordered rows inside
+// one level (and at most UNIT_EMAX entries), with the units their rows
+// depend on.  Units of the forward solve depend on units of earlier levels;
+// units of the transposed solve (cut by the transposed copy) on units of
+// later levels.
 struct UnitSet {
   int rows = 0;             // rows per unit (cap)
   int count = 0;            // number of units
+  std::vector<int> h_uoff;  // first unit of each level (levels+1)
   int *d_ub = nullptr;      // unit u covers positions [ub[u], ub[u+1])
-  int *d_dptr = nullptr;    // forward dependency units of u:
+  int *d_dptr = nullptr;    // dependency units of u:
   int *d_dlist = nullptr;   //   dlist[dptr[u] .. dptr[u+1])
-  int *d_rdptr = nullptr;   // reverse dependency units (built on demand)
-  int *d_rdlist = nullptr;
   int *d_uflags = nullptr;  // ready tags, one per unit
-  int64_t deps = 0, rdeps = 0;
+  int64_t deps = 0;
 };
 
 struct Choice {
@@ -148,7 +153,6 @@ struct symamd_plan_s {
   void *d_tval = nullptr;
   int *d_tmap = nullptr;
   // persistent-kernel state
-  int *d_flags = nullptr;              // n ready tags
   int *d_state = nullptr;              // ST_WORDS ints
   std::map<int, symamd::UnitSet> units;  // rows-per-unit -> units
   // schedule choices
@@ -172,9 +176,9 @@ struct symamd_graph_s {
 
 namespace symamd {
 
-// Ensures the unit partition with `rows` rows per unit exists, with its
-// forward dependency lists (and the reverse ones when `reverse`).
-int plan_units(symamd_handle h, symamd_plan p, int rows, bool reverse,
+// Ensures the unit partition with `rows` rows per unit exists with its
+// dependency lists, for the forward solve or (adjoint) the transposed one.
+int plan_units(symamd_handle h, symamd_plan p, int rows, bool adjoint,
                UnitSet **out);
 // Ensures the transposed (CSC) copy exists.
 int plan_transpose(symamd_handle h, symamd_plan p);

@@ -4,85 +4,93 @@
 // Row arithmetic (reference sparse_blas/sptrsv.cpp:17-25, sptrsv_csr):
 //   x[i] -= L[i,j] * x[j] for the stored j < i in ascending order,
 //   then x[i] /= L[i,i]   (diagonal last in the row).
-// With one lane per (row, right-hand side) the lane repeats exactly that
-// sequence; this file is compiled with -ffp-contract=off, so the result is
-// bit-identical to the reference's serial solve.  With G > 1 lanes per row
-// the off-diagonal sum is split over the G lanes and reduced with
-// cross-lane shuffles, then x = (b - sum) / L[i,i].
-//
-// Transposed solve (reference sparse_blas/sptrsv_lt.cpp:11-20, ltsolve):
+// Transposed solve (reference sparse_blas/sptrsv_lt.cpp:11-20, ltsolve),
+// on the transposed copy (rows of L^T, diagonal first):
 //   gb[i] -= L[r,i] * gb[r] for the stored r > i ascending, then
 //   gb[i] /= L[i,i]; rows visited in reverse level order.
+// With one lane per (row, right-hand side) the lane repeats exactly that
+// sequence; this file is compiled with -ffp-contract=off, so the result is
+// bit-identical to the reference's serial routines.  With G > 1 lanes per
+// row the off-diagonal sum is split over the G lanes and reduced with
+// cross-lane shuffles, then x = (b - sum) / L[i,i].
+//
+// Work unit = a run of rows of one level (at most 64 rows and UNIT_EMAX
+// entries, cut by the inspector).  One wave solves one unit:
+//   1. stage the unit's row pointers, row ids and entries (contiguous in
+//      the level-ordered copy) in the wave's LDS slab, and load the
+//      right-hand sides of its rows into registers;
+//   2. (sync-free only) wait until every unit its rows depend on carries
+//      the launch's ready tag -- one coherent load per dependency unit;
+//   3. load the dependency values (coherent loads, NR rows per lane in
+//      flight), finish the rows, store them with device-scope stores;
+//   4. (sync-free only) wait for those stores and publish the unit's tag.
+// Only steps 3-4 lie on the dependency chain.
 //
 // Lane layout inside a 64-wide wavefront: lane = (row, split, column) with
 // the column index fastest, so with row-major right-hand sides the KT lanes
-// of one row read and write KT consecutive values of that row.  Each lane
-// carries NR rows at once and issues their loads together, so a wave keeps
-// NR * (64 / (G*KT)) rows in flight.
+// of one row read and write KT consecutive values of that row.
 //
 // Executors
-//   LEVELSET  one launch per level; plain loads and stores (kernel
-//             boundaries order the levels).
-//   SYNCFREE  one persistent launch over work units (runs of R rows inside
-//             one level, in level order).  The inspector records, for each
-//             unit, the units holding its rows' dependencies; a wave waits
-//             on those units' ready flags (one coherent load per dependency
-//             unit, with back-off), solves its rows and publishes its own
-//             unit flag.  Units are dealt to waves round-robin from a
-//             cooperative launch (all waves co-resident), or taken from a
-//             ticket counter when the launch is captured in a graph; either
-//             way every unit a wave waits on is held by a running wave that
-//             reaches it first, so the grid cannot deadlock.  Waits are
-//             bounded by a watchdog on the 100 MHz clock that records an
-//             error and lets the wave proceed, so every wave exits.
+//   LEVELSET  one launch per level, one wave per unit; plain loads and
+//             stores (kernel boundaries order the levels).
+//   SYNCFREE  one persistent launch.  Units are dealt to waves round-robin
+//             from a cooperative launch (all waves co-resident), or taken
+//             from a ticket counter when the launch is captured in a graph;
+//             either way every unit a wave waits on is held by a running
+//             wave that reaches it first, so the grid cannot deadlock.
+//             Waits are bounded by a watchdog on the 100 MHz clock that
+//             records an error and lets the wave proceed, so every wave
+//             exits.
 #include "common.h"
 
 namespace symamd {
 
 namespace {
 
+constexpr int UNIT_ROWS_MAX = 64;
+
 template <typename T>
-struct FwdArgs {
-  int n;
+struct Slab {
+  int ci[UNIT_EMAX];
+  T val[UNIT_EMAX];
+  int rp[UNIT_ROWS_MAX + 1];
+  int row[UNIT_ROWS_MAX];
+};
+
+// Arguments shared by the forward and the transposed solve.
+template <typename T>
+struct SolveArgs {
   int k;
   int64_t rs, cs;            // element (i, c) at X[i*rs + c*cs]
-  const int *rp;             // level-ordered row pointers
-  const int *ci;             // column ids (original numbering)
-  const T *val;              // values
+  const int *rp;             // level-ordered row pointers of the copy solved
+  const int *ci;             // its column ids (original row numbering)
+  const T *val;
   const int *perm;           // position -> row
-  T *X;
+  const T *B;                // right-hand sides (== X for the in-place forward)
+  T *X;                      // solution
+  // transposed solve only: grad over L's pattern from the forward copy
+  const int *frp, *fci;      // level-ordered forward copy
+  const int *orp;            // caller CSR row pointers
+  const T *x;                // forward solution
+  T *gL;
   int *state;
   const int *ub;             // unit bounds (positions)
   const int *dptr, *dlist;   // dependency units of each unit
   int *uflags;               // unit ready tags
   int U;
   int ticketed;              // 1: units from the ticket counter
+  int reverse;               // 1: units visited from last to first
   uint64_t timeout;
   int total_waves;
 };
 
-template <typename T>
-struct AdjArgs {
-  int n;
-  const int *trp, *tci;      // level-ordered transposed copy
-  const T *tval;
-  const int *rp, *ci;        // level-ordered forward copy (for gL)
-  const int *orp;            // caller CSR row pointers
-  const int *perm;
-  const T *x, *gx;
-  T *gb, *gL;
-  int *state;
-  const int *ub;
-  const int *dptr, *dlist;   // reverse dependency units
-  int *uflags;
-  int U;
-  int ticketed;
-  uint64_t timeout;
-  int total_waves;
-};
+__device__ __forceinline__ void wave_lds_sync() {
+  asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+  __builtin_amdgcn_wave_barrier();
+}
 
 // Wait until every dependency unit of unit u carries `target`.  One lane per
-// dependency unit, one coherent load each, sleeping between rounds.
+// dependency unit, one coherent load each.
 __device__ __forceinline__ void wait_units(const int *__restrict__ dptr,
                                            const int *__restrict__ dlist,
                                            int u, const int *uflags,
@@ -97,12 +105,11 @@ __device__ __forceinline__ void wait_units(const int *__restrict__ dptr,
     if (pending) pending = ld_dev(f) != target;
     if (!__any(pending)) continue;
     const uint64_t t0 = wall_ticks();
-    int nap = 1, rounds = 0;
+    int rounds = 0;
     while (__any(pending)) {
-      for (int z = 0; z < nap; ++z) __builtin_amdgcn_s_sleep(1);
-      if (nap < 4) nap <<= 1;
+      __builtin_amdgcn_s_sleep(1);
       if (pending) pending = ld_dev(f) != target;
-      if ((++rounds & 31) == 0 && wall_ticks() - t0 > timeout) {
+      if ((++rounds & 63) == 0 && wall_ticks() - t0 > timeout) {
         if (pending) atomicOr(&state[ST_ERROR], 1);
         pending = false;
       }
@@ -129,68 +136,62 @@ __device__ __forceinline__ void st_x(T *p, T v) {
   else *p = v;
 }
 
-// Solve the rows at positions [q0, q1) (at most NR * 64/(G*KT) of them)
-// for all k right-hand sides with one wave.
-template <typename T, int G, int KT, int NR, bool SYNC>
-__device__ __forceinline__ void fwd_rows(const FwdArgs<T> &a, int q0, int q1) {
-  constexpr int LPR = G * KT;
-  constexpr int RPW = 64 / LPR;
+// Dependency values loaded per lane and row before any is consumed: a row
+// with at most DEP_CHUNK (strided) entries needs one memory round trip.
+constexpr int DEP_CHUNK = 4;
+
+// Finish NR rows per lane for all k right-hand sides: for each column tile
+// load the dependency values in chunks (all loads of a chunk in flight at
+// once), accumulate them in ascending entry order, divide by the diagonal
+// and store.  cis/vals index the row entries jb..je (stride G).
+template <typename T, int G, int KT, int NR, bool SYNC, bool ADJ>
+__device__ __forceinline__ void finish_rows(const SolveArgs<T> &a,
+                                            const int *cis, const T *vals,
+                                            const int (&i)[NR], const int (&jb)[NR],
+                                            const int (&je)[NR], const T (&diag)[NR],
+                                            T (&bv)[NR], const bool (&act)[NR],
+                                            int iters, const int (&qpos)[NR]) {
   const int lane = threadIdx.x & 63;
-  const int rr = lane / LPR;
   const int s = (lane / KT) % G;
   const int c = lane % KT;
-  int i[NR], jb[NR], je[NR];
-  T diag[NR];
-  bool act[NR];
-  int iters = 0;
-#pragma unroll
-  for (int r = 0; r < NR; ++r) {
-    const int q = q0 + rr + r * RPW;
-    act[r] = q < q1;
-    i[r] = 0;
-    jb[r] = 0;
-    je[r] = 0;
-    diag[r] = T(1);
-    if (act[r]) {
-      i[r] = a.perm[q];
-      jb[r] = a.rp[q] + s;
-      je[r] = a.rp[q + 1] - 1;
-      diag[r] = a.val[je[r]];
-      const int it = (je[r] - jb[r] + G - 1) / G;
-      iters = it > iters ? it : iters;
-    }
-  }
   for (int c0 = 0; c0 < a.k; c0 += KT) {
     const int col = c0 + c;
     const bool colok = col < a.k;
     const int64_t coff = (int64_t)col * a.cs;
-    T bv[NR], acc[NR];
-#pragma unroll
-    for (int r = 0; r < NR; ++r) {
-      bv[r] = (act[r] && colok) ? a.X[(int64_t)i[r] * a.rs + coff] : T(0);
-      acc[r] = G == 1 ? bv[r] : T(0);
-    }
-    for (int t = 0; t < iters; ++t) {
-      int d[NR];
-      T v[NR], xd[NR];
-#pragma unroll
-      for (int r = 0; r < NR; ++r) {
-        const int j = jb[r] + t * G;
-        const bool live = j < je[r];
-        d[r] = live ? a.ci[j] : -1;
-        v[r] = live ? a.val[j] : T(0);
-      }
+    if (c0 > 0) {
 #pragma unroll
       for (int r = 0; r < NR; ++r)
-        xd[r] = (d[r] >= 0 && colok) ? ld_x<SYNC>(&a.X[(int64_t)d[r] * a.rs + coff])
-                                     : T(0);
+        bv[r] = (act[r] && colok) ? a.B[(int64_t)i[r] * a.rs + coff] : T(0);
+    }
+    T acc[NR];
 #pragma unroll
-      for (int r = 0; r < NR; ++r) {
-        if (d[r] >= 0) {
-          if (G == 1) acc[r] -= v[r] * xd[r];
-          else acc[r] += v[r] * xd[r];
+    for (int r = 0; r < NR; ++r) acc[r] = G == 1 ? bv[r] : T(0);
+    // Branch-free chunk: dead slots load the lane's own right-hand side
+    // address (always valid) and are discarded, so the compiler keeps all
+    // DEP_CHUNK * NR loads in flight instead of waiting at every branch.
+    const int64_t coff_safe = colok ? coff : 0;
+    for (int t0 = 0; t0 < iters; t0 += DEP_CHUNK) {
+      T xd[NR][DEP_CHUNK];
+#pragma unroll
+      for (int r = 0; r < NR; ++r)
+#pragma unroll
+        for (int t = 0; t < DEP_CHUNK; ++t) {
+          const int j = jb[r] + (t0 + t) * G;
+          const bool live = j < je[r];
+          const int d = cis[live ? j : 0];
+          xd[r][t] = ld_x<SYNC>(&a.X[(int64_t)(live ? d : i[r]) * a.rs + coff_safe]);
         }
-      }
+#pragma unroll
+      for (int r = 0; r < NR; ++r)
+#pragma unroll
+        for (int t = 0; t < DEP_CHUNK; ++t) {
+          const int j = jb[r] + (t0 + t) * G;
+          const bool live = j < je[r];
+          const T v = vals[live ? j : 0];
+          const T prod = v * xd[r][t];
+          if (G == 1) acc[r] = live ? acc[r] - prod : acc[r];
+          else acc[r] = live ? acc[r] + prod : acc[r];
+        }
     }
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
@@ -202,7 +203,106 @@ __device__ __forceinline__ void fwd_rows(const FwdArgs<T> &a, int q0, int q1) {
         xi = (bv[r] - sum) / diag[r];
       }
       if (act[r] && colok && s == 0) st_x<SYNC>(&a.X[(int64_t)i[r] * a.rs + coff], xi);
+      if constexpr (ADJ) {
+        // grad over row i of L: gL[p] = -(gb[i] * x[j]) at the caller's
+        // CSR positions of that row
+        if (a.gL && act[r]) {
+          const int q = qpos[r];
+          const int fb = a.frp[q], len = a.frp[q + 1] - fb;
+          const int ob = a.orp[i[r]];
+          for (int t = s; t < len; t += G) a.gL[ob + t] = -(xi * a.x[a.fci[fb + t]]);
+        }
+      }
     }
+  }
+}
+
+// Rows of one staged unit: set up the lane's NR rows from the slab's row
+// pointers and the entries in `cis`/`vals` (the LDS slab, or global memory
+// for a unit too large for it), load their right-hand sides, wait for the
+// dependency units, finish the rows.  Called with pointers of a single
+// address space so the compiler emits LDS or global loads, never FLAT ones
+// (FLAT loads count against both wait counters and would serialize the
+// dependency loads).
+template <typename T, int G, int KT, int NR, bool SYNC, bool ADJ>
+__device__ __forceinline__ void unit_rows(const SolveArgs<T> &a, int u, int q0,
+                                          int nrows, const Slab<T> &sl,
+                                          const int *cis, const T *vals,
+                                          int target) {
+  constexpr int LPR = G * KT;
+  constexpr int RPW = 64 / LPR;
+  const int lane = threadIdx.x & 63;
+  const int rr = lane / LPR;
+  const int s = (lane / KT) % G;
+  const int c = lane % KT;
+  int i[NR], jb[NR], je[NR];
+  T diag[NR], bv[NR];
+  bool act[NR];
+  int iters = 0;
+  const int col0 = c;
+  const bool colok0 = col0 < a.k;
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    const int lr = rr + r * RPW;
+    act[r] = lr < nrows;
+    i[r] = 0;
+    jb[r] = 0;
+    je[r] = 0;
+    diag[r] = T(1);
+    bv[r] = T(0);
+    if (act[r]) {
+      i[r] = sl.row[lr];
+      const int b0 = sl.rp[lr], b1 = sl.rp[lr + 1];
+      if (ADJ) {
+        jb[r] = b0 + 1 + s;
+        je[r] = b1;
+        diag[r] = vals[b0];
+      } else {
+        jb[r] = b0 + s;
+        je[r] = b1 - 1;
+        diag[r] = vals[b1 - 1];
+      }
+      const int it = (je[r] - jb[r] + G - 1) / G;
+      iters = it > iters ? it : iters;
+      if (colok0) bv[r] = a.B[(int64_t)i[r] * a.rs + (int64_t)col0 * a.cs];
+    }
+  }
+  // 2. dependencies
+  if constexpr (SYNC) wait_units(a.dptr, a.dlist, u, a.uflags, target, a.state, a.timeout);
+  // 3. solve
+  int qpos[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) qpos[r] = q0 + rr + r * RPW;
+  finish_rows<T, G, KT, NR, SYNC, ADJ>(a, cis, vals, i, jb, je, diag, bv, act, iters,
+                                       qpos);
+}
+
+// Solve unit u with one wave.  ADJ selects the transposed solve (diagonal
+// first in each row of the copy, grad over L's pattern afterwards).
+template <typename T, int G, int KT, int NR, bool SYNC, bool ADJ>
+__device__ __forceinline__ void solve_unit(const SolveArgs<T> &a, int u,
+                                           Slab<T> &sl, int target) {
+  const int lane = threadIdx.x & 63;
+  const int q0 = a.ub[u], q1 = a.ub[u + 1];
+  const int nrows = q1 - q0;
+  const int e0 = a.rp[q0];
+  const int ne = a.rp[q1] - e0;
+  // 1. stage the unit
+  for (int t = lane; t <= nrows; t += 64) {
+    sl.rp[t] = a.rp[q0 + t] - e0;
+    if (t < nrows) sl.row[t] = a.perm[q0 + t];
+  }
+  if (ne <= UNIT_EMAX) {
+    for (int t = lane; t < ne; t += 64) {
+      sl.ci[t] = a.ci[e0 + t];
+      sl.val[t] = a.val[e0 + t];
+    }
+    wave_lds_sync();
+    unit_rows<T, G, KT, NR, SYNC, ADJ>(a, u, q0, nrows, sl, sl.ci, sl.val, target);
+  } else {
+    wave_lds_sync();
+    unit_rows<T, G, KT, NR, SYNC, ADJ>(a, u, q0, nrows, sl, a.ci + e0, a.val + e0,
+                                       target);
   }
 }
 
@@ -219,107 +319,10 @@ __device__ __forceinline__ int next_unit(int *state, int ticketed, int &cursor,
   return u;
 }
 
-template <typename T, int G, int KT, int NR>
-__global__ __launch_bounds__(256) void k_fwd_syncfree(FwdArgs<T> a) {
-  const int lane = threadIdx.x & 63;
-  const int target = ld_dev(&a.state[ST_EPOCH]) + 1;
-  int cursor = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int stride = a.total_waves;
-  for (;;) {
-    const int u = next_unit(a.state, a.ticketed, cursor, stride);
-    if (u >= a.U) break;
-    wait_units(a.dptr, a.dlist, u, a.uflags, target, a.state, a.timeout);
-    fwd_rows<T, G, KT, NR, true>(a, a.ub[u], a.ub[u + 1]);
-    publish_fence();
-    if (lane == 0) st_dev(&a.uflags[u], target);
-  }
-  persistent_finish(a.state, a.total_waves);
-}
-
-template <typename T, int G, int KT, int NR>
-__global__ __launch_bounds__(256) void k_fwd_level(FwdArgs<T> a, int q0, int q1) {
-  constexpr int R = NR * 64 / (G * KT);
-  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int qs = q0 + wave * R;
-  if (qs >= q1) return;  // uniform across the wave
-  fwd_rows<T, G, KT, NR, false>(a, qs, qs + R < q1 ? qs + R : q1);
-}
-
-// --------------------------------------------------------------- adjoint
-// Rows at positions [q0, q1) of the transposed solve, G lanes per row, NR
-// rows per lane; then grad over L's pattern for the same rows.
-template <typename T, int G, int NR, bool SYNC>
-__device__ __forceinline__ void adj_rows(const AdjArgs<T> &a, int q0, int q1) {
-  constexpr int RPW = 64 / G;
-  const int lane = threadIdx.x & 63;
-  const int rr = lane / G;
-  const int s = lane % G;
-  int i[NR], tb[NR], te[NR];
-  T diag[NR], g[NR], acc[NR];
-  bool act[NR];
-  int iters = 0;
-#pragma unroll
-  for (int r = 0; r < NR; ++r) {
-    const int q = q0 + rr + r * RPW;
-    act[r] = q < q1;
-    i[r] = 0;
-    tb[r] = 0;
-    te[r] = 0;
-    diag[r] = T(1);
-    g[r] = T(0);
-    if (act[r]) {
-      i[r] = a.perm[q];
-      const int b0 = a.trp[q];
-      tb[r] = b0 + 1 + s;
-      te[r] = a.trp[q + 1];
-      diag[r] = a.tval[b0];
-      g[r] = a.gx[i[r]];
-      const int it = (te[r] - tb[r] + G - 1) / G;
-      iters = it > iters ? it : iters;
-    }
-    acc[r] = G == 1 ? g[r] : T(0);
-  }
-  for (int t = 0; t < iters; ++t) {
-    int d[NR];
-    T v[NR], gr[NR];
-#pragma unroll
-    for (int r = 0; r < NR; ++r) {
-      const int j = tb[r] + t * G;
-      const bool live = j < te[r];
-      d[r] = live ? a.tci[j] : -1;
-      v[r] = live ? a.tval[j] : T(0);
-    }
-#pragma unroll
-    for (int r = 0; r < NR; ++r) gr[r] = d[r] >= 0 ? ld_x<SYNC>(&a.gb[d[r]]) : T(0);
-#pragma unroll
-    for (int r = 0; r < NR; ++r) {
-      if (d[r] >= 0) {
-        if (G == 1) acc[r] -= v[r] * gr[r];
-        else acc[r] += v[r] * gr[r];
-      }
-    }
-  }
-#pragma unroll
-  for (int r = 0; r < NR; ++r) {
-    T gbi;
-    if (G == 1) {
-      gbi = acc[r] / diag[r];
-    } else {
-      const T sum = lane_sum<T, G, 1>(acc[r]);
-      gbi = (g[r] - sum) / diag[r];
-    }
-    if (act[r] && s == 0) st_x<SYNC>(&a.gb[i[r]], gbi);
-    if (a.gL && act[r]) {
-      const int q = q0 + rr + r * RPW;
-      const int fb = a.rp[q], len = a.rp[q + 1] - fb;
-      const int ob = a.orp[i[r]];
-      for (int t = s; t < len; t += G) a.gL[ob + t] = -(gbi * a.x[a.ci[fb + t]]);
-    }
-  }
-}
-
-template <typename T, int G, int NR>
-__global__ __launch_bounds__(256) void k_adj_syncfree(AdjArgs<T> a) {
+template <typename T, int G, int KT, int NR, bool ADJ>
+__global__ __launch_bounds__(256) void k_syncfree(SolveArgs<T> a) {
+  __shared__ Slab<T> slabs[4];
+  Slab<T> &sl = slabs[threadIdx.x >> 6];
   const int lane = threadIdx.x & 63;
   const int target = ld_dev(&a.state[ST_EPOCH]) + 1;
   int cursor = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -327,22 +330,47 @@ __global__ __launch_bounds__(256) void k_adj_syncfree(AdjArgs<T> a) {
   for (;;) {
     const int t = next_unit(a.state, a.ticketed, cursor, stride);
     if (t >= a.U) break;
-    const int u = a.U - 1 - t;  // reverse level order
-    wait_units(a.dptr, a.dlist, u, a.uflags, target, a.state, a.timeout);
-    adj_rows<T, G, NR, true>(a, a.ub[u], a.ub[u + 1]);
+    const int u = a.reverse ? a.U - 1 - t : t;
+    solve_unit<T, G, KT, NR, true, ADJ>(a, u, sl, target);
     publish_fence();
     if (lane == 0) st_dev(&a.uflags[u], target);
   }
   persistent_finish(a.state, a.total_waves);
 }
 
-template <typename T, int G, int NR>
-__global__ __launch_bounds__(256) void k_adj_level(AdjArgs<T> a, int q0, int q1) {
-  constexpr int R = NR * 64 / G;
-  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int qs = q0 + wave * R;
-  if (qs >= q1) return;
-  adj_rows<T, G, NR, false>(a, qs, qs + R < q1 ? qs + R : q1);
+// One launch per level: each group of G*KT lanes solves one row of
+// positions [q0, q1), reading the row straight from the level-ordered copy
+// (no staging: the kernel boundary already separates it from its
+// dependencies).
+template <typename T, int G, int KT, bool ADJ>
+__global__ __launch_bounds__(256) void k_level(SolveArgs<T> a, int q0, int q1) {
+  constexpr int LPR = G * KT;
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int q = q0 + gid / LPR;
+  if (q - (gid % 64) / LPR >= q1) return;  // whole wave past the level
+  const int s = ((threadIdx.x & 63) / KT) % G;
+  const int c = threadIdx.x % KT;
+  int i[1] = {0}, jb[1] = {0}, je[1] = {0}, qpos[1] = {q};
+  T diag[1] = {T(1)}, bv[1] = {T(0)};
+  bool act[1] = {q < q1};
+  int iters = 0;
+  if (act[0]) {
+    i[0] = a.perm[q];
+    const int b0 = a.rp[q], b1 = a.rp[q + 1];
+    if (ADJ) {
+      jb[0] = b0 + 1 + s;
+      je[0] = b1;
+      diag[0] = a.val[b0];
+    } else {
+      jb[0] = b0 + s;
+      je[0] = b1 - 1;
+      diag[0] = a.val[b1 - 1];
+    }
+    iters = (je[0] - jb[0] + G - 1) / G;
+    if (c < a.k) bv[0] = a.B[(int64_t)i[0] * a.rs + (int64_t)c * a.cs];
+  }
+  finish_rows<T, G, KT, 1, false, ADJ>(a, a.ci, a.val, i, jb, je, diag, bv, act, iters,
+                                       qpos);
 }
 
 // ------------------------------------------------- caller schedules
@@ -388,8 +416,8 @@ bool capturing(hipStream_t s) {
 
 // Launch a persistent kernel: cooperatively (round-robin units) when
 // possible, else as a plain launch taking units from the ticket counter.
-template <typename Args>
-int launch_persistent(symamd_handle h, const void *kfn, Args &a) {
+template <typename T>
+int launch_persistent(symamd_handle h, const void *kfn, SolveArgs<T> &a) {
   const int grid = resident_grid(h, kfn, a.U);
   a.total_waves = grid * 4;
   if (!capturing(h->stream) && !h->force_tickets) {
@@ -406,109 +434,113 @@ int launch_persistent(symamd_handle h, const void *kfn, Args &a) {
   return SYMAMD_OK;
 }
 
-template <typename T, int G, int KT, int NR>
-int run_fwd(symamd_handle h, symamd_plan p, FwdArgs<T> a, int exec) {
+template <typename T, int G, int KT, int NR, bool ADJ>
+int run(symamd_handle h, symamd_plan p, SolveArgs<T> a, int exec) {
   constexpr int R = NR * 64 / (G * KT);
-  if (exec == SYMAMD_EXEC_SYNCFREE) {
-    UnitSet *us = nullptr;
-    SYMAMD_TRY(plan_units(h, p, R, false, &us));
-    a.ub = us->d_ub;
-    a.U = us->count;
-    a.dptr = us->d_dptr;
-    a.dlist = us->d_dlist;
-    a.uflags = us->d_uflags;
-    return launch_persistent(h, (const void *)&k_fwd_syncfree<T, G, KT, NR>, a);
-  }
-  for (int l = 0; l < p->levels; ++l) {
+  static_assert(R <= UNIT_ROWS_MAX, "unit rows exceed the LDS slab");
+  UnitSet *us = nullptr;
+  SYMAMD_TRY(plan_units(h, p, R, ADJ, &us));
+  a.ub = us->d_ub;
+  a.U = us->count;
+  a.dptr = us->d_dptr;
+  a.dlist = us->d_dlist;
+  a.uflags = us->d_uflags;
+  a.reverse = ADJ ? 1 : 0;
+  if (exec == SYMAMD_EXEC_SYNCFREE)
+    return launch_persistent(h, (const void *)&k_syncfree<T, G, KT, NR, ADJ>, a);
+  constexpr int LPR = G * KT;
+  for (int l0 = 0; l0 < p->levels; ++l0) {
+    const int l = ADJ ? p->levels - 1 - l0 : l0;
     const int q0 = p->h_lvl_ptr[l], q1 = p->h_lvl_ptr[l + 1];
-    const int64_t waves = (q1 - q0 + R - 1) / R;
-    const int grid = (int)((waves + 3) / 4);
-    k_fwd_level<T, G, KT, NR><<<grid, 256, 0, h->stream>>>(a, q0, q1);
+    const int64_t lanes = (int64_t)(q1 - q0) * LPR;
+    const int grid = (int)((lanes + 255) / 256);
+    k_level<T, G, KT, ADJ><<<grid, 256, 0, h->stream>>>(a, q0, q1);
   }
   SYMAMD_HIP(hipGetLastError());
   return SYMAMD_OK;
 }
 
 // Rows per lane: enough rows in flight per wave to hide memory latency.
-template <int LPR>
-constexpr int rows_per_lane() {
-  return LPR >= 16 ? 8 : (LPR >= 4 ? 4 : (LPR >= 2 ? 2 : 1));
+constexpr int rows_per_lane(int lpr) {
+  return lpr >= 4 ? 4 : (lpr >= 2 ? 2 : 1);
 }
 
-template <typename T, int KT>
-int fwd_pick_g(symamd_handle h, symamd_plan p, FwdArgs<T> a, int exec, int g) {
-#define SYMAMD_FWD_G(GV)                                                       \
+template <typename T, int KT, bool ADJ>
+int pick_g(symamd_handle h, symamd_plan p, SolveArgs<T> a, int exec, int g) {
+#define SYMAMD_G(GV)                                                           \
   case GV:                                                                     \
     if constexpr (GV * KT <= 64)                                               \
-      return run_fwd<T, GV, KT, rows_per_lane<GV * KT>()>(h, p, a, exec);      \
+      return run<T, GV, KT, rows_per_lane(GV * KT), ADJ>(h, p, a, exec);       \
     break;
   switch (g) {
-    SYMAMD_FWD_G(1)
-    SYMAMD_FWD_G(2)
-    SYMAMD_FWD_G(4)
-    SYMAMD_FWD_G(8)
-    SYMAMD_FWD_G(16)
-    SYMAMD_FWD_G(32)
-    SYMAMD_FWD_G(64)
+    SYMAMD_G(1)
+    SYMAMD_G(2)
+    SYMAMD_G(4)
+    SYMAMD_G(8)
+    SYMAMD_G(16)
+    SYMAMD_G(32)
+    SYMAMD_G(64)
     default: break;
   }
-#undef SYMAMD_FWD_G
+#undef SYMAMD_G
   return fail(SYMAMD_ERR_INVALID_ARG, "lanes per row %d not supported with a %d-column tile", g, KT);
 }
 
 template <typename T>
-int fwd_dispatch(symamd_handle h, symamd_plan p, FwdArgs<T> a, int exec,
+int fwd_dispatch(symamd_handle h, symamd_plan p, SolveArgs<T> a, int exec,
                  int g, int kt) {
   switch (kt) {
-    case 1: return fwd_pick_g<T, 1>(h, p, a, exec, g);
-    case 2: return fwd_pick_g<T, 2>(h, p, a, exec, g);
-    case 4: return fwd_pick_g<T, 4>(h, p, a, exec, g);
-    case 8: return fwd_pick_g<T, 8>(h, p, a, exec, g);
-    case 16: return fwd_pick_g<T, 16>(h, p, a, exec, g);
-    case 32: return fwd_pick_g<T, 32>(h, p, a, exec, g);
-    case 64: return fwd_pick_g<T, 64>(h, p, a, exec, g);
+    case 1: return pick_g<T, 1, false>(h, p, a, exec, g);
+    case 2: return pick_g<T, 2, false>(h, p, a, exec, g);
+    case 4: return pick_g<T, 4, false>(h, p, a, exec, g);
+    case 8: return pick_g<T, 8, false>(h, p, a, exec, g);
+    case 16: return pick_g<T, 16, false>(h, p, a, exec, g);
+    case 32: return pick_g<T, 32, false>(h, p, a, exec, g);
+    case 64: return pick_g<T, 64, false>(h, p, a, exec, g);
     default: break;
   }
   return fail(SYMAMD_ERR_INVALID_ARG, "column tile %d not supported", kt);
 }
 
-template <typename T, int G>
-int run_adj(symamd_handle h, symamd_plan p, AdjArgs<T> a, int exec) {
-  constexpr int NR = 1;
-  constexpr int R = NR * 64 / G;
-  if (exec == SYMAMD_EXEC_SYNCFREE) {
-    UnitSet *us = nullptr;
-    SYMAMD_TRY(plan_units(h, p, R, true, &us));
-    a.ub = us->d_ub;
-    a.U = us->count;
-    a.dptr = us->d_rdptr;
-    a.dlist = us->d_rdlist;
-    a.uflags = us->d_uflags;
-    return launch_persistent(h, (const void *)&k_adj_syncfree<T, G, NR>, a);
-  }
-  for (int l = p->levels - 1; l >= 0; --l) {
-    const int q0 = p->h_lvl_ptr[l], q1 = p->h_lvl_ptr[l + 1];
-    const int64_t waves = (q1 - q0 + R - 1) / R;
-    const int grid = (int)((waves + 3) / 4);
-    k_adj_level<T, G, NR><<<grid, 256, 0, h->stream>>>(a, q0, q1);
-  }
-  SYMAMD_HIP(hipGetLastError());
-  return SYMAMD_OK;
+template <typename T>
+SolveArgs<T> forward_args(symamd_handle h, symamd_plan p, void *d_B, int k,
+                          int64_t rs, int64_t cs) {
+  SolveArgs<T> a{};
+  a.k = k;
+  a.rs = rs;
+  a.cs = cs;
+  a.rp = p->d_rp;
+  a.ci = p->d_ci;
+  a.val = (const T *)p->d_val;
+  a.perm = p->d_perm;
+  a.B = (const T *)d_B;
+  a.X = (T *)d_B;
+  a.state = p->d_state;
+  a.timeout = h->timeout_ticks();
+  return a;
 }
 
 template <typename T>
-int adj_dispatch(symamd_handle h, symamd_plan p, AdjArgs<T> a, int exec, int g) {
-  switch (g) {
-    case 1: return run_adj<T, 1>(h, p, a, exec);
-    case 2: return run_adj<T, 2>(h, p, a, exec);
-    case 4: return run_adj<T, 4>(h, p, a, exec);
-    case 8: return run_adj<T, 8>(h, p, a, exec);
-    case 16: return run_adj<T, 16>(h, p, a, exec);
-    case 32: return run_adj<T, 32>(h, p, a, exec);
-    case 64: return run_adj<T, 64>(h, p, a, exec);
-    default: break;
-  }
-  return fail(SYMAMD_ERR_INVALID_ARG, "lanes per row %d not supported", g);
+SolveArgs<T> adjoint_args(symamd_handle h, symamd_plan p, const void *d_x,
+                          const void *d_gx, void *d_gb, void *d_gL) {
+  SolveArgs<T> a{};
+  a.k = 1;
+  a.rs = 1;
+  a.cs = 0;
+  a.rp = p->d_trp;
+  a.ci = p->d_tci;
+  a.val = (const T *)p->d_tval;
+  a.perm = p->d_perm;
+  a.B = (const T *)d_gx;
+  a.X = (T *)d_gb;
+  a.frp = p->d_rp;
+  a.fci = p->d_ci;
+  a.orp = p->d_orp;
+  a.x = (const T *)d_x;
+  a.gL = (T *)d_gL;
+  a.state = p->d_state;
+  a.timeout = h->timeout_ticks();
+  return a;
 }
 
 }  // namespace
@@ -517,8 +549,7 @@ int forward_rows_per_unit(int k, int lanes) {
   int kt = pow2_ceil(k < 64 ? (k > 0 ? k : 1) : 64);
   if (lanes * kt > 64) lanes = 64 / kt;
   const int lpr = lanes * kt;
-  const int nr = lpr >= 16 ? 8 : (lpr >= 4 ? 4 : (lpr >= 2 ? 2 : 1));
-  return nr * 64 / lpr;
+  return rows_per_lane(lpr) * 64 / lpr;
 }
 
 int launch_lbc(symamd_handle h, int n, const int *d_Lp, const int *d_Li,
@@ -539,22 +570,13 @@ int launch_lbc(symamd_handle h, int n, const int *d_Lp, const int *d_Li,
 int launch_forward(symamd_handle h, symamd_plan p, void *d_B, int k, int64_t rs,
                    int64_t cs, int exec, int lanes) {
   if (p->n == 0 || k == 0) return SYMAMD_OK;
-  int kt = pow2_ceil(k < 64 ? k : 64);
+  const int kt = pow2_ceil(k < 64 ? k : 64);
   if (lanes * kt > 64) lanes = 64 / kt;
-  if (p->dtype == SYMAMD_F64) {
-    FwdArgs<double> a{};
-    a.n = p->n; a.k = k; a.rs = rs; a.cs = cs;
-    a.rp = p->d_rp; a.ci = p->d_ci; a.val = (const double *)p->d_val;
-    a.perm = p->d_perm; a.X = (double *)d_B;
-    a.state = p->d_state; a.timeout = h->timeout_ticks();
-    return fwd_dispatch<double>(h, p, a, exec, lanes, kt);
-  }
-  FwdArgs<float> a{};
-  a.n = p->n; a.k = k; a.rs = rs; a.cs = cs;
-  a.rp = p->d_rp; a.ci = p->d_ci; a.val = (const float *)p->d_val;
-  a.perm = p->d_perm; a.X = (float *)d_B;
-  a.state = p->d_state; a.timeout = h->timeout_ticks();
-  return fwd_dispatch<float>(h, p, a, exec, lanes, kt);
+  if (p->dtype == SYMAMD_F64)
+    return fwd_dispatch<double>(h, p, forward_args<double>(h, p, d_B, k, rs, cs), exec,
+                                lanes, kt);
+  return fwd_dispatch<float>(h, p, forward_args<float>(h, p, d_B, k, rs, cs), exec,
+                             lanes, kt);
 }
 
 int launch_adjoint(symamd_handle h, symamd_plan p, const void *d_x,
@@ -562,22 +584,11 @@ int launch_adjoint(symamd_handle h, symamd_plan p, const void *d_x,
                    int lanes) {
   if (p->n == 0) return SYMAMD_OK;
   SYMAMD_TRY(plan_transpose(h, p));
-  if (p->dtype == SYMAMD_F64) {
-    AdjArgs<double> a{};
-    a.n = p->n; a.trp = p->d_trp; a.tci = p->d_tci;
-    a.tval = (const double *)p->d_tval; a.rp = p->d_rp; a.ci = p->d_ci;
-    a.orp = p->d_orp; a.perm = p->d_perm; a.x = (const double *)d_x;
-    a.gx = (const double *)d_gx; a.gb = (double *)d_gb; a.gL = (double *)d_gL;
-    a.state = p->d_state; a.timeout = h->timeout_ticks();
-    return adj_dispatch<double>(h, p, a, exec, lanes);
-  }
-  AdjArgs<float> a{};
-  a.n = p->n; a.trp = p->d_trp; a.tci = p->d_tci;
-  a.tval = (const float *)p->d_tval; a.rp = p->d_rp; a.ci = p->d_ci;
-  a.orp = p->d_orp; a.perm = p->d_perm; a.x = (const float *)d_x;
-  a.gx = (const float *)d_gx; a.gb = (float *)d_gb; a.gL = (float *)d_gL;
-  a.state = p->d_state; a.timeout = h->timeout_ticks();
-  return adj_dispatch<float>(h, p, a, exec, lanes);
+  if (p->dtype == SYMAMD_F64)
+    return pick_g<double, 1, true>(h, p, adjoint_args<double>(h, p, d_x, d_gx, d_gb, d_gL),
+                                   exec, lanes);
+  return pick_g<float, 1, true>(h, p, adjoint_args<float>(h, p, d_x, d_gx, d_gb, d_gL),
+                                exec, lanes);
 }
 
 }  // namespace symamd

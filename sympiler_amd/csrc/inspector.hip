@@ -226,25 +226,46 @@ __global__ void k_collen(int n, const int *__restrict__ perm,
 
 // Work units: level l splits into ceil(size/rows) units of at most `rows`
 // consecutive positions.  upl = units per level, scanned into uoff.
+// Work units: each level is cut, in order, into runs of at most `rows`
+// rows holding at most `emax` entries of the row copy `rp` (a single row
+// longer than emax forms a unit of its own).  One thread per level scans
+// its rows twice: first to count the units (upl), then, after the scan of
+// the counts into uoff, to write the unit starts (ub).
+__device__ __forceinline__ int unit_scan(const int *__restrict__ rp, int q0,
+                                         int q1, int rows, int emax,
+                                         int *ub_out) {
+  int cnt = 0, in_rows = 0, ents = 0;
+  for (int q = q0; q < q1; ++q) {
+    const int len = rp[q + 1] - rp[q];
+    if (in_rows == rows || (in_rows > 0 && ents + len > emax)) {
+      in_rows = 0;
+      ents = 0;
+    }
+    if (in_rows == 0) {
+      if (ub_out) ub_out[cnt] = q;
+      ++cnt;
+    }
+    ++in_rows;
+    ents += len;
+  }
+  return cnt;
+}
+
 __global__ void k_units_per_level(int nlev, const int *__restrict__ lp,
-                                  int rows, int *upl) {
+                                  const int *__restrict__ rp, int rows,
+                                  int emax, int *upl) {
   int l = blockIdx.x * blockDim.x + threadIdx.x;
-  if (l < nlev) upl[l] = (lp[l + 1] - lp[l] + rows - 1) / rows;
+  if (l < nlev) upl[l] = unit_scan(rp, lp[l], lp[l + 1], rows, emax, nullptr);
   if (l == nlev) upl[l] = 0;
 }
 
 __global__ void k_unit_bounds(int nlev, int U, int n, const int *__restrict__ lp,
-                              const int *__restrict__ uoff, int rows, int *ub) {
-  int u = blockIdx.x * blockDim.x + threadIdx.x;
-  if (u > U) return;
-  if (u == U) { ub[U] = n; return; }
-  // level containing unit u: last l with uoff[l] <= u
-  int lo = 0, hi = nlev - 1;
-  while (lo < hi) {
-    int mid = (lo + hi + 1) >> 1;
-    if (uoff[mid] <= u) lo = mid; else hi = mid - 1;
-  }
-  ub[u] = lp[lo] + (u - uoff[lo]) * rows;
+                              const int *__restrict__ rp,
+                              const int *__restrict__ uoff, int rows, int emax,
+                              int *ub) {
+  int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l < nlev) unit_scan(rp, lp[l], lp[l + 1], rows, emax, ub + uoff[l]);
+  if (l == nlev) ub[U] = n;
 }
 
 inline int blocks_for(int64_t m, int bs = 256) {
@@ -533,67 +554,69 @@ int build_unit_deps(symamd_handle h, symamd_plan p, const UnitSet &us,
 
 }  // namespace
 
-int plan_units(symamd_handle h, symamd_plan p, int rows, bool reverse,
+int plan_units(symamd_handle h, symamd_plan p, int rows, bool adjoint,
                UnitSet **out) {
-  auto it = p->units.find(rows);
-  if (it == p->units.end()) {
-    UnitSet us;
-    us.rows = rows;
-    const int nlev = p->levels;
-    // count on the host from the level pointers (levels+1 ints, already there)
-    int64_t U = 0;
-    for (int l = 0; l < nlev; ++l)
-      U += (p->h_lvl_ptr[l + 1] - p->h_lvl_ptr[l] + rows - 1) / rows;
-    us.count = (int)U;
-    SYMAMD_HIP(hipMalloc(&us.d_ub, sizeof(int) * (U + 1)));
-    SYMAMD_HIP(hipMalloc(&us.d_uflags, sizeof(int) * (U > 0 ? U : 1)));
-    SYMAMD_HIP(hipMemsetAsync(us.d_uflags, 0, sizeof(int) * (U > 0 ? U : 1), h->stream));
-    if (nlev > 0) {
-      int *d_upl = nullptr, *d_uoff = nullptr;
-      SYMAMD_HIP(hipMallocAsync(&d_upl, sizeof(int) * (nlev + 1), h->stream));
-      SYMAMD_HIP(hipMallocAsync(&d_uoff, sizeof(int) * (nlev + 1), h->stream));
-      k_units_per_level<<<blocks_for(nlev + 1), 256, 0, h->stream>>>(
-          nlev, p->d_lvl_ptr, rows, d_upl);
-      SYMAMD_TRY(scan_exclusive(h, d_upl, d_uoff, nlev + 1));
-      k_unit_bounds<<<blocks_for(U + 1), 256, 0, h->stream>>>(
-          nlev, (int)U, p->n, p->d_lvl_ptr, d_uoff, rows, us.d_ub);
-      SYMAMD_HIP(hipGetLastError());
-      SYMAMD_HIP(hipFreeAsync(d_upl, h->stream));
-      SYMAMD_HIP(hipFreeAsync(d_uoff, h->stream));
-    } else {
-      int zero = 0;
-      SYMAMD_HIP(hipMemcpyAsync(us.d_ub, &zero, sizeof(int),
-                                hipMemcpyHostToDevice, h->stream));
-    }
-    it = p->units.emplace(rows, us).first;
+  const int key = rows * 2 + (adjoint ? 1 : 0);
+  auto it = p->units.find(key);
+  if (it != p->units.end()) {
+    *out = &it->second;
+    return SYMAMD_OK;
   }
-  UnitSet &us = it->second;
-  const bool need_fwd = !us.d_dptr, need_rev = reverse && !us.d_rdptr;
-  if (need_fwd || need_rev) {
-    if (!p->d_ipos) {
-      SYMAMD_HIP(hipMalloc(&p->d_ipos, sizeof(int) * (p->n > 0 ? p->n : 1)));
-      k_inverse_perm<<<blocks_for(p->n), 256, 0, h->stream>>>(p->n, p->d_perm, p->d_ipos);
-      SYMAMD_HIP(hipGetLastError());
-    }
-    int *d_uop = nullptr;
-    SYMAMD_HIP(hipMallocAsync(&d_uop, sizeof(int) * (p->n > 0 ? p->n : 1), h->stream));
-    k_unit_of_pos<<<blocks_for(us.count), 256, 0, h->stream>>>(us.count, us.d_ub, d_uop);
+  if (adjoint) SYMAMD_TRY(plan_transpose(h, p));
+  // units of the forward solve are cut by the entries of the forward copy,
+  // those of the transposed solve by the entries of the transposed copy
+  const int *rp = adjoint ? p->d_trp : p->d_rp;
+  const int *ci = adjoint ? p->d_tci : p->d_ci;
+  UnitSet us;
+  us.rows = rows;
+  const int nlev = p->levels;
+  us.h_uoff.assign(nlev + 1, 0);
+  if (nlev > 0) {
+    int *d_upl = nullptr, *d_uoff = nullptr;
+    SYMAMD_HIP(hipMallocAsync(&d_upl, sizeof(int) * (nlev + 1), h->stream));
+    SYMAMD_HIP(hipMallocAsync(&d_uoff, sizeof(int) * (nlev + 1), h->stream));
+    k_units_per_level<<<blocks_for(nlev + 1), 256, 0, h->stream>>>(
+        nlev, p->d_lvl_ptr, rp, rows, UNIT_EMAX, d_upl);
     SYMAMD_HIP(hipGetLastError());
-    int rc = SYMAMD_OK;
-    if (need_fwd)
-      rc = build_unit_deps(h, p, us, p->d_rp, p->d_ci, 0, d_uop, &us.d_dptr,
-                           &us.d_dlist, &us.deps);
-    if (rc == SYMAMD_OK && need_rev) {
-      rc = plan_transpose(h, p);
-      if (rc == SYMAMD_OK)
-        rc = build_unit_deps(h, p, us, p->d_trp, p->d_tci, 1, d_uop, &us.d_rdptr,
-                             &us.d_rdlist, &us.rdeps);
-    }
-    (void)hipFreeAsync(d_uop, h->stream);
-    (void)hipStreamSynchronize(h->stream);
-    if (rc != SYMAMD_OK) return rc;
+    SYMAMD_TRY(scan_exclusive(h, d_upl, d_uoff, nlev + 1));
+    SYMAMD_HIP(hipMemcpyAsync(us.h_uoff.data(), d_uoff, sizeof(int) * (nlev + 1),
+                              hipMemcpyDeviceToHost, h->stream));
+    SYMAMD_HIP(hipStreamSynchronize(h->stream));
+    const int U = us.h_uoff[nlev];
+    us.count = U;
+    SYMAMD_HIP(hipMalloc(&us.d_ub, sizeof(int) * (U + 1)));
+    k_unit_bounds<<<blocks_for(nlev + 1), 256, 0, h->stream>>>(
+        nlev, U, p->n, p->d_lvl_ptr, rp, d_uoff, rows, UNIT_EMAX, us.d_ub);
+    SYMAMD_HIP(hipGetLastError());
+    SYMAMD_HIP(hipFreeAsync(d_upl, h->stream));
+    SYMAMD_HIP(hipFreeAsync(d_uoff, h->stream));
+  } else {
+    us.count = 0;
+    SYMAMD_HIP(hipMalloc(&us.d_ub, sizeof(int)));
+    SYMAMD_HIP(hipMemsetAsync(us.d_ub, 0, sizeof(int), h->stream));
   }
-  *out = &us;
+  const int U = us.count;
+  SYMAMD_HIP(hipMalloc(&us.d_uflags, sizeof(int) * (U > 0 ? U : 1)));
+  SYMAMD_HIP(hipMemsetAsync(us.d_uflags, 0, sizeof(int) * (U > 0 ? U : 1), h->stream));
+  if (!p->d_ipos) {
+    SYMAMD_HIP(hipMalloc(&p->d_ipos, sizeof(int) * (p->n > 0 ? p->n : 1)));
+    k_inverse_perm<<<blocks_for(p->n), 256, 0, h->stream>>>(p->n, p->d_perm, p->d_ipos);
+    SYMAMD_HIP(hipGetLastError());
+  }
+  int *d_uop = nullptr;
+  SYMAMD_HIP(hipMallocAsync(&d_uop, sizeof(int) * (p->n > 0 ? p->n : 1), h->stream));
+  k_unit_of_pos<<<blocks_for(U), 256, 0, h->stream>>>(U, us.d_ub, d_uop);
+  SYMAMD_HIP(hipGetLastError());
+  int rc = build_unit_deps(h, p, us, rp, ci, adjoint ? 1 : 0, d_uop, &us.d_dptr,
+                           &us.d_dlist, &us.deps);
+  (void)hipFreeAsync(d_uop, h->stream);
+  (void)hipStreamSynchronize(h->stream);
+  if (rc != SYMAMD_OK) {
+    (void)hipFree(us.d_ub);
+    (void)hipFree(us.d_uflags);
+    return rc;
+  }
+  *out = &p->units.emplace(key, us).first->second;
   return SYMAMD_OK;
 }
 
