@@ -67,8 +67,16 @@ int symamd_set_timeout_ms(symamd_handle h, double ms);
 /* Handle options.
  *   SYMAMD_OPT_TICKETS: 1 = persistent kernels take work units from a device
  *     ticket counter instead of a co-resident round-robin deal (the mode
- *     used automatically inside graph capture); 0 = automatic (default). */
-enum { SYMAMD_OPT_TICKETS = 1 };
+ *     used automatically inside graph capture); 0 = automatic (default).
+ *   SYMAMD_OPT_TRACE_PTR / SYMAMD_OPT_TRACE_CAP: device pointer (as an
+ *     integer; 0 disables) and capacity in uint64 entries of a trace
+ *     buffer.  Sync-free launches then record, per work unit u, three
+ *     100 MHz clock stamps: [3u] unit taken, [3u+1] dependencies ready,
+ *     [3u+2] unit published (units beyond the capacity are not traced). */
+/*   SYMAMD_OPT_MAX_WAVES: cap on the wavefronts of a persistent launch
+ *     (0 = as many as fit the GPU, the default). */
+enum { SYMAMD_OPT_TICKETS = 1, SYMAMD_OPT_TRACE_PTR = 2, SYMAMD_OPT_TRACE_CAP = 3,
+       SYMAMD_OPT_MAX_WAVES = 4 };
 int symamd_set_option(symamd_handle h, int option, int64_t value);
 const char *symamd_last_error(void);
 /* Library version string. */

@@ -142,6 +142,23 @@ class Handle:
         dealt round-robin from a cooperative launch (False, default)."""
         _check(lib().symamd_set_option(self._h, 1, int(bool(on))))
 
+    def set_max_waves(self, waves: int):
+        """Cap the wavefronts of persistent launches (0: fill the GPU)."""
+        _check(lib().symamd_set_option(self._h, 4, int(waves)))
+
+    def set_trace(self, buf):
+        """Record per-unit clock stamps of sync-free launches into `buf`
+        (a uint64/int64 cuda tensor of 3 entries per unit), or stop (None)."""
+        if buf is None:
+            _check(lib().symamd_set_option(self._h, 2, 0))
+            _check(lib().symamd_set_option(self._h, 3, 0))
+            return
+        _require_cuda(buf)
+        if buf.element_size() != 8:
+            raise ValueError("trace buffer needs 64-bit entries")
+        _check(lib().symamd_set_option(self._h, 2, buf.data_ptr()))
+        _check(lib().symamd_set_option(self._h, 3, buf.numel()))
+
     def __del__(self):
         if getattr(self, "_h", None) and _lib is not None:
             _lib.symamd_destroy(self._h)

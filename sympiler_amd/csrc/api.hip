@@ -417,6 +417,18 @@ int symamd_set_option(symamd_handle h, int option, int64_t value) {
     case SYMAMD_OPT_TICKETS:
       h->force_tickets = value != 0;
       return SYMAMD_OK;
+    case SYMAMD_OPT_TRACE_PTR:
+      h->trace = (uint64_t *)(intptr_t)value;
+      return SYMAMD_OK;
+    case SYMAMD_OPT_MAX_WAVES:
+      if (value < 0 || value > (1 << 24))
+        return fail(SYMAMD_ERR_INVALID_ARG, "max waves out of range");
+      h->max_waves = (int)value;
+      return SYMAMD_OK;
+    case SYMAMD_OPT_TRACE_CAP:
+      if (value < 0) return fail(SYMAMD_ERR_INVALID_ARG, "trace capacity must be >= 0");
+      h->trace_cap = value;
+      return SYMAMD_OK;
     default:
       return fail(SYMAMD_ERR_INVALID_ARG, "unknown option %d", option);
   }

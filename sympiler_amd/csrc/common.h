@@ -128,6 +128,9 @@ struct symamd_handle_s {
   int num_cus = 0;
   double timeout_ms = 20000.0;
   int force_tickets = 0;  // persistent kernels take units from a ticket counter
+  uint64_t *trace = nullptr;  // per-unit timestamps of sync-free launches
+  int64_t trace_cap = 0;
+  int max_waves = 0;  // cap on the waves of a persistent launch (0: occupancy)
   uint64_t timeout_ticks() const { return (uint64_t)(timeout_ms * 1.0e5); }
 };
 

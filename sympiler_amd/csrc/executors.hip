@@ -82,7 +82,17 @@ struct SolveArgs {
   int reverse;               // 1: units visited from last to first
   uint64_t timeout;
   int total_waves;
+  // optional per-unit trace (100 MHz clock): [3u] wave took the unit,
+  // [3u+1] its dependencies were ready, [3u+2] it was published
+  uint64_t *trace;
+  int64_t trace_cap;
 };
+
+__device__ __forceinline__ void trace_mark(uint64_t *trace, int64_t cap,
+                                           int u, int slot) {
+  if (trace && (threadIdx.x & 63) == 0 && 3 * (int64_t)u + 2 < cap)
+    trace[3 * (int64_t)u + slot] = wall_ticks();
+}
 
 __device__ __forceinline__ void wave_lds_sync() {
   asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
@@ -268,7 +278,10 @@ __device__ __forceinline__ void unit_rows(const SolveArgs<T> &a, int u, int q0,
     }
   }
   // 2. dependencies
-  if constexpr (SYNC) wait_units(a.dptr, a.dlist, u, a.uflags, target, a.state, a.timeout);
+  if constexpr (SYNC) {
+    wait_units(a.dptr, a.dlist, u, a.uflags, target, a.state, a.timeout);
+    trace_mark(a.trace, a.trace_cap, u, 1);
+  }
   // 3. solve
   int qpos[NR];
 #pragma unroll
@@ -331,9 +344,11 @@ __global__ __launch_bounds__(256) void k_syncfree(SolveArgs<T> a) {
     const int t = next_unit(a.state, a.ticketed, cursor, stride);
     if (t >= a.U) break;
     const int u = a.reverse ? a.U - 1 - t : t;
+    trace_mark(a.trace, a.trace_cap, u, 0);
     solve_unit<T, G, KT, NR, true, ADJ>(a, u, sl, target);
     publish_fence();
     if (lane == 0) st_dev(&a.uflags[u], target);
+    trace_mark(a.trace, a.trace_cap, u, 2);
   }
   persistent_finish(a.state, a.total_waves);
 }
@@ -403,6 +418,7 @@ int resident_grid(symamd_handle h, const void *kernel, int units) {
       per_cu < 1)
     per_cu = 1;
   int64_t cap = (int64_t)per_cu * h->num_cus;
+  if (h->max_waves > 0 && (h->max_waves + 3) / 4 < cap) cap = (h->max_waves + 3) / 4;
   int64_t want = (units + 3) / 4;
   if (want < 1) want = 1;
   return (int)(want < cap ? want : cap);
@@ -517,6 +533,8 @@ SolveArgs<T> forward_args(symamd_handle h, symamd_plan p, void *d_B, int k,
   a.X = (T *)d_B;
   a.state = p->d_state;
   a.timeout = h->timeout_ticks();
+  a.trace = h->trace;
+  a.trace_cap = h->trace_cap;
   return a;
 }
 
@@ -540,6 +558,8 @@ SolveArgs<T> adjoint_args(symamd_handle h, symamd_plan p, const void *d_x,
   a.gL = (T *)d_gL;
   a.state = p->d_state;
   a.timeout = h->timeout_ticks();
+  a.trace = h->trace;
+  a.trace_cap = h->trace_cap;
   return a;
 }
 
