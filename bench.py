@@ -47,7 +47,105 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--tune", action="store_true", help="run the schedule tuner first")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the other workloads (Zipf SpTRSM, batch, adjoint 64^3)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a device copy of 1 GiB (x5) to scale HBM counters")
     return ap.parse_args()
+
+
+def calibrate_copy():
+    """Five device-to-device copies of 1 GiB: 1 GiB read + 1 GiB written per
+    copy kernel.  Used to scale FETCH_SIZE / WRITE_SIZE counter readings."""
+    import torch
+    a = torch.empty(1 << 27, dtype=torch.float64, device="cuda:0").uniform_()
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration": "copy", "bytes_read_per_copy": 1 << 30,
+                      "bytes_written_per_copy": 1 << 30, "copies": 5}))
+
+
+def other_configs(torch, sa, W, dev, stream, plain_k1):
+    """The other workloads, a few timed repetitions each (seconds of GPU
+    time): Zipf-irregular SpTRSM (n = 1e6, k = 1/8/32/128, column-major), the
+    512-system uneven batch replayed from a hipGraph, and the 64^3 IC(0)
+    adjoint (forward vs forward+backward, FP64 and FP32)."""
+    out = {}
+    # Zipf(1.8) irregular lower triangle, B column-major (k, n)
+    w3 = W.config3(k=1)
+    plan3 = sa.Plan(*(torch.from_numpy(a).to(dev) for a in (w3.Lp, w3.Li, w3.Lx)))
+    zipf = {"n": w3.n, "nnz": w3.nnz, "levels": plan3.levels}
+    for k in (1, 8, 32, 128):
+        B0 = torch.from_numpy(W.uniform(2, 0, w3.n * k).reshape(k, w3.n)).to(dev)
+        X = torch.empty_like(B0)
+        tune = plan3.tune(k=k, layout="col").strip().splitlines()[-1]
+        wl, el = timed_steps(lambda: plan3.solve(X, layout="col"), lambda: X.copy_(B0),
+                             5, 1, stream, torch)
+        plan3.check()
+        res = float(plan3.residual(X, B0, layout="col").max())
+        ms = float(np.mean(el))
+        zipf[f"k{k}"] = {"kernel_ms": round(ms, 4), "rhs_solves_per_s": round(k / (ms * 1e-3), 1),
+                         "gbs": round(W.algorithmic_bytes_forward(w3.n, w3.nnz, k) / (ms * 1e-3) / 1e9, 1),
+                         "max_residual": res, "tuned": tune}
+        del B0, X
+    out["zipf_sptrsm_col_major"] = zipf
+    log(f"zipf: {zipf}")
+    del plan3
+
+    # 512 uneven systems, one captured launch replayed
+    w4 = W.config4()
+    plan4 = sa.Plan(*(torch.from_numpy(a).to(dev) for a in (w4.Lp, w4.Li, w4.Lx)),
+                    row_offsets=w4.row_offsets)
+    b4 = torch.from_numpy(w4.b).to(dev)
+    X4 = b4.clone()
+    tune = plan4.tune(k=1).strip().splitlines()[-1]
+    g = plan4.graph(X4)
+    wg, eg = timed_steps(g.launch, lambda: X4.copy_(b4), 10, 2, stream, torch)
+    plan4.check()
+    err = float(np.max(np.abs(X4.cpu().numpy() - w4.x_true)))
+    ms = float(np.mean(eg))
+    out["batch512"] = {"n_total": w4.n, "nnz_total": w4.nnz, "levels": plan4.levels,
+                       "graph_replay_ms": round(ms, 4),
+                       "system_solves_per_s": round(512 / (ms * 1e-3), 1),
+                       "rows_per_s": round(w4.n / (ms * 1e-3), 1),
+                       "gbs": round(W.algorithmic_bytes_forward(w4.n, w4.nnz, 1) / (ms * 1e-3) / 1e9, 1),
+                       "max_abs_err_vs_x_true": err, "tuned": tune}
+    log(f"batch512: {out['batch512']}")
+    del g, plan4
+
+    # adjoint through autograd on the 64^3 IC(0) factor, FP64 and FP32
+    from sympiler_amd.autograd import TriangularSolve
+    Lp5, Li5, Lx5 = W.ic0_lap3d(64, 0.01)
+    n5 = len(Lp5) - 1
+    b5 = torch.from_numpy(W.uniform(4, 0, n5)).to(dev)
+    adj = {"n": n5, "nnz": int(Lp5[-1])}
+    for name, dt in (("fp64", torch.float64), ("fp32", torch.float32)):
+        dLx = torch.from_numpy(Lx5).to(dev).to(dt)
+        plan5 = sa.Plan(torch.from_numpy(Lp5).to(dev), torch.from_numpy(Li5).to(dev), dLx)
+        bb = b5.to(dt)
+        Lx_t = dLx.clone().requires_grad_(True)
+        b_t = bb.clone().requires_grad_(True)
+
+        def fwd():
+            with torch.no_grad():
+                TriangularSolve.apply(dLx, bb, plan5)
+
+        def fwdbwd():
+            x = TriangularSolve.apply(Lx_t, b_t, plan5)
+            (0.5 * (x * x).sum()).backward()
+
+        wf, _ = timed_steps(fwd, lambda: None, 10, 2, stream, torch)
+        wb, _ = timed_steps(fwdbwd, lambda: None, 10, 2, stream, torch)
+        plan5.check()
+        adj[name] = {"forward_ms": round(1e3 * float(np.mean(wf)), 4),
+                     "forward_backward_ms": round(1e3 * float(np.mean(wb)), 4),
+                     "ratio": round(float(np.mean(wb)) / float(np.mean(wf)), 2)}
+        del plan5
+    out["adjoint_64cubed_autograd"] = adj
+    log(f"adjoint 64^3: {adj}")
+    return out
 
 
 def log(msg):
@@ -79,6 +177,9 @@ def timed_steps(fn, reset, steps, warmup, stream, torch):
 
 def main():
     args = parse()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; the multi-GPU "
                           "sharding is a design note (shard right-hand sides or "
@@ -199,6 +300,10 @@ def main():
             log(f"col-major k={k}: {detail['col_major_k']}")
             del Bc, Xc
 
+    if not args.no_detail and not args.no_configs:
+        detail["configs"] = other_configs(torch, sa, W, dev, stream,
+                                          detail["single_rhs"]["levelset_plain"])
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(Lp, Li, Lx, Bcols, X, args, k)
@@ -222,13 +327,31 @@ def main():
                    "layout": args.layout, "executor": args.executor},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_fwd_syncfree (one launch per solve)",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(f"ic0_lap3d_{g}^3", k, args.layout),
+                     "traffic_source": "profiles/hbm_traffic.txt (rocprofv3 --pmc "
+                                       "FETCH_SIZE, WRITE_SIZE; copy-calibrated)",
+                     "kernel": "k_syncfree (one persistent launch per solve)",
                      "algorithmic_bytes_per_launch": int(bytes_alg)},
         "cpu_baseline": cpu,
         "detail": detail,
     }
     print(json.dumps(out), flush=True)
+
+
+def measured_traffic(workload, k, layout):
+    """HBM bytes per launch of the headline kernel as measured by the
+    committed rocprofv3 counter passes (profiles/hbm_traffic.txt), or None
+    when this workload was not profiled."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 4 and not line.startswith("#") and f[0] == workload \
+                and f[1] == str(k) and f[2] == layout:
+            return int(f[3])
+    return None
 
 
 def cpu_baseline(Lp, Li, Lx, Bcols, Xgpu, args, k):
