@@ -67,7 +67,29 @@ def calibrate_copy():
                       "bytes_written_per_copy": 1 << 30, "copies": 5}))
 
 
-def other_configs(torch, sa, W, dev, stream, plain_k1):
+def latency_floor(torch, sa, W, dev, stream, levels, ms_k, ms_1):
+    """Per-level synchronisation cost of the sync-free executor: a chain
+    (bidiagonal) system with one row per level has no parallelism and almost
+    no bytes, so its solve time / levels is the cost of one dependent step.
+    T_floor = levels * t_sync bounds any solve with that many levels."""
+    out = {}
+    Lp, Li, Lx = W.lap2d_tril(1, levels)  # bidiagonal: `levels` levels
+    plan = sa.Plan(*(torch.from_numpy(a).to(dev) for a in (Lp, Li, Lx)))
+    for k in (1, 32):
+        B0 = torch.from_numpy(W.uniform(3, 0, levels * k).reshape(levels, k)).to(dev)
+        X = torch.empty_like(B0)
+        _, ev = timed_steps(lambda: plan.solve(X, layout="row", executor="syncfree"),
+                            lambda: X.copy_(B0), 10, 2, stream, torch)
+        plan.check()
+        t = float(np.mean(ev))
+        out[f"k{k}"] = {"chain_ms": round(t, 4), "t_sync_us": round(1e3 * t / levels, 3),
+                        "floor_ms": round(t, 4)}
+    out["headline_ms_over_floor"] = round(ms_k / out["k32"]["floor_ms"], 2)
+    out["single_rhs_ms_over_floor"] = round(ms_1 / out["k1"]["floor_ms"], 2)
+    return out
+
+
+def other_configs(torch, sa, W, dev, stream):
     """The other workloads, a few timed repetitions each (seconds of GPU
     time): Zipf-irregular SpTRSM (n = 1e6, k = 1/8/32/128, column-major), the
     512-system uneven batch replayed from a hipGraph, and the 64^3 IC(0)
@@ -268,9 +290,16 @@ def main():
             log(f"k=1 {name}: {singles[name]}")
         plan.set_lanes(0)
         detail["single_rhs"] = singles
-        plain = singles["levelset_plain"]["rhs_solves_per_s"]
+        # the plain level-set baseline: one kernel per level, launched eagerly
+        # or replayed from a graph, whichever is faster on this box
+        plain = max(singles["levelset_plain"]["rhs_solves_per_s"],
+                    singles["levelset_graph"]["rhs_solves_per_s"])
+        detail["plain_levelset_rhs_solves_per_s"] = plain
         detail["headline_vs_plain_levelset"] = round(value / plain, 2)
         detail["headline_frac_hbm_peak"] = round(achieved / HBM_PEAK_GBS, 4)
+        detail["latency_floor"] = latency_floor(torch, sa, W, dev, stream, plan.levels,
+                                                ms_kernel, singles["syncfree"]["kernel_ms"])
+        log(f"latency floor: {detail['latency_floor']}")
 
         # fused adjoint: gb = L^-T gx, grad over L's pattern; loss 0.5||x||^2
         xs = x1.clone()
@@ -301,8 +330,7 @@ def main():
             del Bc, Xc
 
     if not args.no_detail and not args.no_configs:
-        detail["configs"] = other_configs(torch, sa, W, dev, stream,
-                                          detail["single_rhs"]["levelset_plain"])
+        detail["configs"] = other_configs(torch, sa, W, dev, stream)
 
     cpu = None
     if not args.no_cpu:

@@ -74,9 +74,20 @@ int symamd_set_timeout_ms(symamd_handle h, double ms);
  *     100 MHz clock stamps: [3u] unit taken, [3u+1] dependencies ready,
  *     [3u+2] unit published (units beyond the capacity are not traced). */
 /*   SYMAMD_OPT_MAX_WAVES: cap on the wavefronts of a persistent launch
- *     (0 = as many as fit the GPU, the default). */
+ *     (0 = as many as fit the GPU, the default).
+ *   SYMAMD_OPT_COLMAJOR_DIRECT: 1 = solve column-major blocks of k > 1
+ *     right-hand sides in place (one cache line per row and column); 0 =
+ *     (default) transpose them into a level-ordered row-major workspace owned
+ *     by the plan, solve there and transpose back. */
+/*   SYMAMD_OPT_MARKS: 1 (default) = sync-free solves of a single
+ *     right-hand side mark each row's right-hand side with a signalling-NaN
+ *     pattern after reading it and publish a unit's ready tag without waiting
+ *     for its value stores; readers re-load any value still marked.  0 = the
+ *     tag is published after the stores are acknowledged.  The solve
+ *     overwrites B in place either way; intermediate contents of B are
+ *     unspecified while a solve runs. */
 enum { SYMAMD_OPT_TICKETS = 1, SYMAMD_OPT_TRACE_PTR = 2, SYMAMD_OPT_TRACE_CAP = 3,
-       SYMAMD_OPT_MAX_WAVES = 4 };
+       SYMAMD_OPT_MAX_WAVES = 4, SYMAMD_OPT_COLMAJOR_DIRECT = 5, SYMAMD_OPT_MARKS = 6 };
 int symamd_set_option(symamd_handle h, int option, int64_t value);
 const char *symamd_last_error(void);
 /* Library version string. */

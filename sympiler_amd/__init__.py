@@ -142,6 +142,17 @@ class Handle:
         dealt round-robin from a cooperative launch (False, default)."""
         _check(lib().symamd_set_option(self._h, 1, int(bool(on))))
 
+    def set_colmajor_direct(self, on: bool):
+        """Solve column-major multi-RHS blocks in place (True) instead of
+        through the plan's level-ordered workspace (False, default)."""
+        _check(lib().symamd_set_option(self._h, 5, int(bool(on))))
+
+    def set_marks(self, on: bool):
+        """Marked sync-free protocol (True, default): ready tags are
+        published without waiting for the value stores; readers re-load
+        values still carrying the mark."""
+        _check(lib().symamd_set_option(self._h, 6, int(bool(on))))
+
     def set_max_waves(self, waves: int):
         """Cap the wavefronts of persistent launches (0: fill the GPU)."""
         _check(lib().symamd_set_option(self._h, 4, int(waves)))

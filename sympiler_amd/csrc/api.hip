@@ -43,7 +43,7 @@ int strides(int n, int k, int64_t ld, int layout, int64_t *rs, int64_t *cs) {
 void free_plan_arrays(symamd_plan p) {
   void *ptrs[] = {p->d_lvl_ptr, p->d_perm, p->d_rp,   p->d_ci,   p->d_val,
                   p->d_fmap,    p->d_orp,  p->d_trp,  p->d_tci,  p->d_tval,
-                  p->d_tmap,    p->d_state, p->d_ipos};
+                  p->d_tmap,    p->d_state, p->d_ipos, p->d_cpos, p->d_ws};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   for (auto &kv : p->units) {
@@ -316,13 +316,33 @@ std::string graph_key(void *d_B, int k, int64_t rs, int64_t cs, const Choice &c)
 // Record the per-level launches of one solve into a graph on a private
 // capture stream.  Units for the sync-free path are prepared beforehand so
 // nothing allocates while capturing.
-int capture_solve(symamd_handle h, symamd_plan p, void *d_B, int k, int64_t rs,
-                  int64_t cs, const Choice &c, hipGraph_t *g, hipGraphExec_t *ge) {
-  {
-    // both executors run on the plan's work units; build them outside capture
-    UnitSet *us = nullptr;
-    SYMAMD_TRY(plan_units(h, p, forward_rows_per_unit(k, c.lanes), false, &us));
-  }
+// Column-major blocks of several right-hand sides are solved through the
+// level-ordered workspace (coalesced); everything else in place.
+bool through_workspace(symamd_handle h, int k, int layout) {
+  return layout == SYMAMD_COL_MAJOR && k > 1 && !h->colmajor_direct;
+}
+
+// Everything a solve allocates or builds, done before any launch so that a
+// solve can be captured into a graph.
+int prepare_solve(symamd_handle h, symamd_plan p, int k, int layout, const Choice &c) {
+  UnitSet *us = nullptr;
+  SYMAMD_TRY(plan_units(h, p, forward_rows_per_unit(k, c.lanes), false, &us));
+  if (through_workspace(h, k, layout)) SYMAMD_TRY(prepare_workspace(h, p, k));
+  return SYMAMD_OK;
+}
+
+int run_solve(symamd_handle h, symamd_plan p, void *d_B, int k, int64_t ld,
+              int64_t rs, int64_t cs, int layout, int exec, int lanes) {
+  if (exec == SYMAMD_EXEC_LEVELSET_GRAPH) exec = SYMAMD_EXEC_LEVELSET;
+  if (through_workspace(h, k, layout))
+    return launch_forward_ws(h, p, d_B, k, ld, exec, lanes);
+  return launch_forward(h, p, d_B, k, rs, cs, exec, lanes);
+}
+
+int capture_solve(symamd_handle h, symamd_plan p, void *d_B, int k, int64_t ld,
+                  int64_t rs, int64_t cs, int layout, const Choice &c, hipGraph_t *g,
+                  hipGraphExec_t *ge) {
+  SYMAMD_TRY(prepare_solve(h, p, k, layout, c));
   hipStream_t cap = nullptr;
   SYMAMD_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
   hipStream_t saved = h->stream;
@@ -331,9 +351,7 @@ int capture_solve(symamd_handle h, symamd_plan p, void *d_B, int k, int64_t rs,
   if (hipStreamBeginCapture(cap, hipStreamCaptureModeRelaxed) != hipSuccess) {
     rc = fail(SYMAMD_ERR_HIP, "hipStreamBeginCapture failed");
   } else {
-    rc = launch_forward(h, p, d_B, k, rs, cs,
-                        c.exec == SYMAMD_EXEC_LEVELSET_GRAPH ? SYMAMD_EXEC_LEVELSET : c.exec,
-                        c.lanes);
+    rc = run_solve(h, p, d_B, k, ld, rs, cs, layout, c.exec, c.lanes);
     hipGraph_t graph = nullptr;
     hipError_t e = hipStreamEndCapture(cap, &graph);
     if (rc == SYMAMD_OK && e != hipSuccess)
@@ -358,19 +376,19 @@ int solve_impl(symamd_handle h, symamd_plan p, void *d_B, int k, int64_t ld,
   if (k == 0 || p->n == 0) return SYMAMD_OK;
   Choice c = choose(p, k, layout, exec);
   if (c.exec == SYMAMD_EXEC_LEVELSET_GRAPH && graph_ok) {
-    std::string key = graph_key(d_B, k, rs, cs, c);
+    std::string key = graph_key(d_B, k, rs, cs, c) + (through_workspace(h, k, layout) ? "/ws" : "");
     auto it = p->gcache.find(key);
     if (it == p->gcache.end()) {
       hipGraph_t g = nullptr;
       hipGraphExec_t ge = nullptr;
-      SYMAMD_TRY(capture_solve(h, p, d_B, k, rs, cs, c, &g, &ge));
+      SYMAMD_TRY(capture_solve(h, p, d_B, k, ld, rs, cs, layout, c, &g, &ge));
       it = p->gcache.emplace(key, std::make_pair(g, ge)).first;
     }
     SYMAMD_HIP(hipGraphLaunch(it->second.second, h->stream));
     return SYMAMD_OK;
   }
-  int e = c.exec == SYMAMD_EXEC_LEVELSET_GRAPH ? SYMAMD_EXEC_LEVELSET : c.exec;
-  return launch_forward(h, p, d_B, k, rs, cs, e, c.lanes);
+  SYMAMD_TRY(prepare_solve(h, p, k, layout, c));
+  return run_solve(h, p, d_B, k, ld, rs, cs, layout, c.exec, c.lanes);
 }
 
 }  // namespace
@@ -419,6 +437,12 @@ int symamd_set_option(symamd_handle h, int option, int64_t value) {
       return SYMAMD_OK;
     case SYMAMD_OPT_TRACE_PTR:
       h->trace = (uint64_t *)(intptr_t)value;
+      return SYMAMD_OK;
+    case SYMAMD_OPT_MARKS:
+      h->unmarked = value == 0;
+      return SYMAMD_OK;
+    case SYMAMD_OPT_COLMAJOR_DIRECT:
+      h->colmajor_direct = value != 0;
       return SYMAMD_OK;
     case SYMAMD_OPT_MAX_WAVES:
       if (value < 0 || value > (1 << 24))
@@ -583,7 +607,7 @@ int symamd_graph_create(symamd_handle h, symamd_plan plan, void *d_B, int k,
   Choice c = choose(plan, k, layout, exec);
   auto *x = new symamd_graph_s();
   x->plan = plan;
-  int rc = capture_solve(h, plan, d_B, k, rs, cs, c, &x->graph, &x->exec);
+  int rc = capture_solve(h, plan, d_B, k, ld, rs, cs, layout, c, &x->graph, &x->exec);
   if (rc != SYMAMD_OK) {
     delete x;
     return rc;

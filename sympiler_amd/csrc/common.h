@@ -131,6 +131,8 @@ struct symamd_handle_s {
   uint64_t *trace = nullptr;  // per-unit timestamps of sync-free launches
   int64_t trace_cap = 0;
   int max_waves = 0;  // cap on the waves of a persistent launch (0: occupancy)
+  int colmajor_direct = 0;  // solve column-major blocks in place, no workspace
+  int unmarked = 0;         // sync-free tags wait for the value stores
   uint64_t timeout_ticks() const { return (uint64_t)(timeout_ms * 1.0e5); }
 };
 
@@ -149,6 +151,9 @@ struct symamd_plan_s {
   void *d_val = nullptr;               // level-ordered values, nnz
   int *d_fmap = nullptr;               // caller CSR position of each entry
   int *d_ipos = nullptr;               // row -> position
+  int *d_cpos = nullptr;               // column ids of the copy, in positions
+  void *d_ws = nullptr;                // level-ordered row-major workspace
+  size_t ws_bytes = 0;
   int *d_orp = nullptr;                // caller CSR row pointers, n+1
   // transposed copy (CSC of L, level order), built on first adjoint
   int *d_trp = nullptr;
@@ -207,6 +212,10 @@ int plan_residual(symamd_handle h, symamd_plan p, const void *d_X,
                   const void *d_B, int k, int64_t rs, int64_t cs,
                   double *h_rel);
 int forward_rows_per_unit(int k, int lanes);
+// Column-major multi-RHS solves through a level-ordered workspace.
+int prepare_workspace(symamd_handle h, symamd_plan p, int k);
+int launch_forward_ws(symamd_handle h, symamd_plan p, void *d_B, int k, int64_t ld,
+                      int exec, int lanes);
 int coarsen(symamd_handle h, int n, const int *d_Lp, const int *d_Li,
             int span, int max_cost, int *d_level_ptr, int *d_par_ptr,
             int *d_partition, int *level_no, int *part_no);

@@ -65,7 +65,7 @@ struct SolveArgs {
   const int *rp;             // level-ordered row pointers of the copy solved
   const int *ci;             // its column ids (original row numbering)
   const T *val;
-  const int *perm;           // position -> row
+  const int *perm;           // position -> row (null: rows are positions)
   const T *B;                // right-hand sides (== X for the in-place forward)
   T *X;                      // solution
   // transposed solve only: grad over L's pattern from the forward copy
@@ -99,8 +99,10 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_wave_barrier();
 }
 
-// Wait until every dependency unit of unit u carries `target`.  One lane per
-// dependency unit, one coherent load each.
+// Wait until every dependency unit of unit u carries `target`.  One lane
+// per dependency unit, one coherent load each.  (Batching several tags per
+// lane into one round trip measured slower on the 128^3 IC(0) factor: the
+// extra address setup sits on the critical path of every unit.)
 __device__ __forceinline__ void wait_units(const int *__restrict__ dptr,
                                            const int *__restrict__ dlist,
                                            int u, const int *uflags,
@@ -150,17 +152,73 @@ __device__ __forceinline__ void st_x(T *p, T v) {
 // with at most DEP_CHUNK (strided) entries needs one memory round trip.
 constexpr int DEP_CHUNK = 4;
 
+// The mark: a signalling-NaN bit pattern that no arithmetic produces
+// (operations return quiet NaNs).  In the marked sync-free protocol a unit
+// overwrites its rows' right-hand sides with it right after reading them,
+// so a solved value can be told apart from one not yet stored.
+__device__ __forceinline__ bool is_mark(double v) {
+  return __double_as_longlong(v) == (long long)0x7FF4D0D0CAFEF00Dull;
+}
+__device__ __forceinline__ bool is_mark(float v) {
+  return __float_as_uint(v) == 0x7FA0CAFEu;
+}
+template <typename T>
+__device__ __forceinline__ T mark_value();
+template <>
+__device__ __forceinline__ double mark_value<double>() {
+  return __longlong_as_double((long long)0x7FF4D0D0CAFEF00Dull);
+}
+template <>
+__device__ __forceinline__ float mark_value<float>() {
+  return __uint_as_float(0x7FA0CAFEu);
+}
+
+// Rows a lane finished in the transposed solve, kept for the grad-L pass
+// that runs after the unit is published (off the dependency chain).
+template <typename T, int NR>
+struct Solved {
+  T x[NR];
+  int i[NR];
+  int q[NR];
+  bool act[NR];
+};
+
+// grad over row i of L: gL[p] = -(gb[i] * x[j]) at the caller's CSR
+// positions of that row (read from the level-ordered forward copy).
+template <typename T, int G, int NR>
+__device__ __forceinline__ void grad_rows(const SolveArgs<T> &a,
+                                          const Solved<T, NR> &sv) {
+  if (!a.gL) return;
+  const int s = (threadIdx.x & 63) % G;
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    if (!sv.act[r]) continue;
+    const int q = sv.q[r];
+    const int fb = a.frp[q], len = a.frp[q + 1] - fb;
+    const int ob = a.orp[sv.i[r]];
+    for (int t = s; t < len; t += G) a.gL[ob + t] = -(sv.x[r] * a.x[a.fci[fb + t]]);
+  }
+}
+
 // Finish NR rows per lane for all k right-hand sides: for each column tile
 // load the dependency values in chunks (all loads of a chunk in flight at
 // once), accumulate them in ascending entry order, divide by the diagonal
-// and store.  cis/vals index the row entries jb..je (stride G).
-template <typename T, int G, int KT, int NR, bool SYNC, bool ADJ>
+// and store.  cis/vals index the row entries jb..je (stride G).  For the
+// transposed solve the finished rows are also returned in `sv`.
+// SY: SY_LEVEL (kernel boundaries order the rows), SY_FLAGS (dependency
+// values are ready once their unit's tag is), SY_MARKS (marked protocol:
+// a value still reading as the mark is loaded again).
+constexpr int SY_LEVEL = 0, SY_FLAGS = 1, SY_MARKS = 2;
+
+template <typename T, int G, int KT, int NR, int SY, bool ADJ>
 __device__ __forceinline__ void finish_rows(const SolveArgs<T> &a,
                                             const int *cis, const T *vals,
                                             const int (&i)[NR], const int (&jb)[NR],
                                             const int (&je)[NR], const T (&diag)[NR],
                                             T (&bv)[NR], const bool (&act)[NR],
-                                            int iters, const int (&qpos)[NR]) {
+                                            int iters, const int (&qpos)[NR],
+                                            Solved<T, NR> &sv) {
+  constexpr bool SYNC = SY != SY_LEVEL;
   const int lane = threadIdx.x & 63;
   const int s = (lane / KT) % G;
   const int c = lane % KT;
@@ -191,6 +249,34 @@ __device__ __forceinline__ void finish_rows(const SolveArgs<T> &a,
           const int d = cis[live ? j : 0];
           xd[r][t] = ld_x<SYNC>(&a.X[(int64_t)(live ? d : i[r]) * a.rs + coff_safe]);
         }
+      if constexpr (SY == SY_MARKS) {
+        // A dependency still reading as the mark was published by its
+        // unit's tag before its own store landed: load it again.
+        const uint64_t t0w = wall_ticks();
+        int rounds = 0;
+        for (;;) {
+          bool again = false;
+#pragma unroll
+          for (int r = 0; r < NR; ++r)
+#pragma unroll
+            for (int t = 0; t < DEP_CHUNK; ++t)
+              again |= (jb[r] + (t0 + t) * G < je[r]) && colok && is_mark(xd[r][t]);
+          if (!__any(again)) break;
+          __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+          for (int r = 0; r < NR; ++r)
+#pragma unroll
+            for (int t = 0; t < DEP_CHUNK; ++t) {
+              const int j = jb[r] + (t0 + t) * G;
+              if (j < je[r] && colok && is_mark(xd[r][t]))
+                xd[r][t] = ld_dev(&a.X[(int64_t)cis[j] * a.rs + coff]);
+            }
+          if ((++rounds & 63) == 0 && wall_ticks() - t0w > a.timeout) {
+            atomicOr(&a.state[ST_ERROR], 2);
+            break;
+          }
+        }
+      }
 #pragma unroll
       for (int r = 0; r < NR; ++r)
 #pragma unroll
@@ -214,14 +300,10 @@ __device__ __forceinline__ void finish_rows(const SolveArgs<T> &a,
       }
       if (act[r] && colok && s == 0) st_x<SYNC>(&a.X[(int64_t)i[r] * a.rs + coff], xi);
       if constexpr (ADJ) {
-        // grad over row i of L: gL[p] = -(gb[i] * x[j]) at the caller's
-        // CSR positions of that row
-        if (a.gL && act[r]) {
-          const int q = qpos[r];
-          const int fb = a.frp[q], len = a.frp[q + 1] - fb;
-          const int ob = a.orp[i[r]];
-          for (int t = s; t < len; t += G) a.gL[ob + t] = -(xi * a.x[a.fci[fb + t]]);
-        }
+        sv.x[r] = xi;
+        sv.i[r] = i[r];
+        sv.q[r] = qpos[r];
+        sv.act[r] = act[r];
       }
     }
   }
@@ -234,11 +316,11 @@ __device__ __forceinline__ void finish_rows(const SolveArgs<T> &a,
 // address space so the compiler emits LDS or global loads, never FLAT ones
 // (FLAT loads count against both wait counters and would serialize the
 // dependency loads).
-template <typename T, int G, int KT, int NR, bool SYNC, bool ADJ>
+template <typename T, int G, int KT, int NR, int SY, bool ADJ>
 __device__ __forceinline__ void unit_rows(const SolveArgs<T> &a, int u, int q0,
                                           int nrows, const Slab<T> &sl,
                                           const int *cis, const T *vals,
-                                          int target) {
+                                          int target, Solved<T, NR> &sv) {
   constexpr int LPR = G * KT;
   constexpr int RPW = 64 / LPR;
   const int lane = threadIdx.x & 63;
@@ -278,7 +360,16 @@ __device__ __forceinline__ void unit_rows(const SolveArgs<T> &a, int u, int q0,
     }
   }
   // 2. dependencies
-  if constexpr (SYNC) {
+  if constexpr (SY == SY_MARKS) {
+    // mark the rows (after reading their right-hand sides) and make the
+    // marks visible before this unit can publish its tag
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+      if (act[r] && colok0)
+        st_dev(&a.X[(int64_t)i[r] * a.rs + (int64_t)col0 * a.cs], mark_value<T>());
+    publish_fence();
+  }
+  if constexpr (SY != SY_LEVEL) {
     wait_units(a.dptr, a.dlist, u, a.uflags, target, a.state, a.timeout);
     trace_mark(a.trace, a.trace_cap, u, 1);
   }
@@ -286,15 +377,17 @@ __device__ __forceinline__ void unit_rows(const SolveArgs<T> &a, int u, int q0,
   int qpos[NR];
 #pragma unroll
   for (int r = 0; r < NR; ++r) qpos[r] = q0 + rr + r * RPW;
-  finish_rows<T, G, KT, NR, SYNC, ADJ>(a, cis, vals, i, jb, je, diag, bv, act, iters,
-                                       qpos);
+  finish_rows<T, G, KT, NR, SY, ADJ>(a, cis, vals, i, jb, je, diag, bv, act, iters,
+                                     qpos, sv);
 }
 
 // Solve unit u with one wave.  ADJ selects the transposed solve (diagonal
-// first in each row of the copy, grad over L's pattern afterwards).
-template <typename T, int G, int KT, int NR, bool SYNC, bool ADJ>
+// first in each row of the copy); its finished rows come back in `sv` for
+// the grad-L pass.
+template <typename T, int G, int KT, int NR, int SY, bool ADJ>
 __device__ __forceinline__ void solve_unit(const SolveArgs<T> &a, int u,
-                                           Slab<T> &sl, int target) {
+                                           Slab<T> &sl, int target,
+                                           Solved<T, NR> &sv) {
   const int lane = threadIdx.x & 63;
   const int q0 = a.ub[u], q1 = a.ub[u + 1];
   const int nrows = q1 - q0;
@@ -303,7 +396,7 @@ __device__ __forceinline__ void solve_unit(const SolveArgs<T> &a, int u,
   // 1. stage the unit
   for (int t = lane; t <= nrows; t += 64) {
     sl.rp[t] = a.rp[q0 + t] - e0;
-    if (t < nrows) sl.row[t] = a.perm[q0 + t];
+    if (t < nrows) sl.row[t] = a.perm ? a.perm[q0 + t] : q0 + t;
   }
   if (ne <= UNIT_EMAX) {
     for (int t = lane; t < ne; t += 64) {
@@ -311,11 +404,11 @@ __device__ __forceinline__ void solve_unit(const SolveArgs<T> &a, int u,
       sl.val[t] = a.val[e0 + t];
     }
     wave_lds_sync();
-    unit_rows<T, G, KT, NR, SYNC, ADJ>(a, u, q0, nrows, sl, sl.ci, sl.val, target);
+    unit_rows<T, G, KT, NR, SY, ADJ>(a, u, q0, nrows, sl, sl.ci, sl.val, target, sv);
   } else {
     wave_lds_sync();
-    unit_rows<T, G, KT, NR, SYNC, ADJ>(a, u, q0, nrows, sl, a.ci + e0, a.val + e0,
-                                       target);
+    unit_rows<T, G, KT, NR, SY, ADJ>(a, u, q0, nrows, sl, a.ci + e0, a.val + e0,
+                                     target, sv);
   }
 }
 
@@ -332,7 +425,7 @@ __device__ __forceinline__ int next_unit(int *state, int ticketed, int &cursor,
   return u;
 }
 
-template <typename T, int G, int KT, int NR, bool ADJ>
+template <typename T, int G, int KT, int NR, bool ADJ, int SY>
 __global__ __launch_bounds__(256) void k_syncfree(SolveArgs<T> a) {
   __shared__ Slab<T> slabs[4];
   Slab<T> &sl = slabs[threadIdx.x >> 6];
@@ -345,10 +438,14 @@ __global__ __launch_bounds__(256) void k_syncfree(SolveArgs<T> a) {
     if (t >= a.U) break;
     const int u = a.reverse ? a.U - 1 - t : t;
     trace_mark(a.trace, a.trace_cap, u, 0);
-    solve_unit<T, G, KT, NR, true, ADJ>(a, u, sl, target);
-    publish_fence();
+    Solved<T, NR> sv;
+    solve_unit<T, G, KT, NR, SY, ADJ>(a, u, sl, target, sv);
+    // marked protocol: the tag is only a hint (consumers re-check each value
+    // against the mark), so it need not wait for the value stores
+    if constexpr (SY != SY_MARKS) publish_fence();
     if (lane == 0) st_dev(&a.uflags[u], target);
     trace_mark(a.trace, a.trace_cap, u, 2);
+    if constexpr (ADJ) grad_rows<T, G, NR>(a, sv);
   }
   persistent_finish(a.state, a.total_waves);
 }
@@ -370,7 +467,7 @@ __global__ __launch_bounds__(256) void k_level(SolveArgs<T> a, int q0, int q1) {
   bool act[1] = {q < q1};
   int iters = 0;
   if (act[0]) {
-    i[0] = a.perm[q];
+    i[0] = a.perm ? a.perm[q] : q;
     const int b0 = a.rp[q], b1 = a.rp[q + 1];
     if (ADJ) {
       jb[0] = b0 + 1 + s;
@@ -384,8 +481,10 @@ __global__ __launch_bounds__(256) void k_level(SolveArgs<T> a, int q0, int q1) {
     iters = (je[0] - jb[0] + G - 1) / G;
     if (c < a.k) bv[0] = a.B[(int64_t)i[0] * a.rs + (int64_t)c * a.cs];
   }
-  finish_rows<T, G, KT, 1, false, ADJ>(a, a.ci, a.val, i, jb, je, diag, bv, act, iters,
-                                       qpos);
+  Solved<T, 1> sv;
+  finish_rows<T, G, KT, 1, SY_LEVEL, ADJ>(a, a.ci, a.val, i, jb, je, diag, bv, act,
+                                          iters, qpos, sv);
+  if constexpr (ADJ) grad_rows<T, G, 1>(a, sv);
 }
 
 // ------------------------------------------------- caller schedules
@@ -462,8 +561,17 @@ int run(symamd_handle h, symamd_plan p, SolveArgs<T> a, int exec) {
   a.dlist = us->d_dlist;
   a.uflags = us->d_uflags;
   a.reverse = ADJ ? 1 : 0;
-  if (exec == SYMAMD_EXEC_SYNCFREE)
-    return launch_persistent(h, (const void *)&k_syncfree<T, G, KT, NR, ADJ>, a);
+  if (exec == SYMAMD_EXEC_SYNCFREE) {
+    // the marked protocol is built for single right-hand sides only; with
+    // 32 columns its mark writes and re-loads cost more than the store
+    // acknowledgement they save
+    if constexpr (KT == 1)
+      if (a.k == 1 && !h->unmarked)
+        return launch_persistent(
+            h, (const void *)&k_syncfree<T, G, KT, NR, ADJ, SY_MARKS>, a);
+    return launch_persistent(h, (const void *)&k_syncfree<T, G, KT, NR, ADJ, SY_FLAGS>,
+                             a);
+  }
   constexpr int LPR = G * KT;
   for (int l0 = 0; l0 < p->levels; ++l0) {
     const int l = ADJ ? p->levels - 1 - l0 : l0;
@@ -563,7 +671,110 @@ SolveArgs<T> adjoint_args(symamd_handle h, symamd_plan p, const void *d_x,
   return a;
 }
 
+// Column-major right-hand sides <-> level-ordered row-major workspace,
+// W[ipos[i]*k + c] = B[i + c*ld], through LDS tiles of 64 rows x 32
+// columns: the column-major side moves 64 consecutive rows of one column
+// per access, the workspace side 32 consecutive columns of one row.
+template <typename T, bool TO_WS>
+__global__ __launch_bounds__(256) void k_ws_transpose(int n, int k, int64_t ld, T *B,
+                                                      const int *__restrict__ ipos,
+                                                      T *W) {
+  __shared__ T tile[64][33];
+  const int i0 = blockIdx.x * 64;
+  const int c0 = blockIdx.y * 32;
+  const int tr = threadIdx.x % 64, tc = threadIdx.x / 64;  // column-major side
+  const int wc = threadIdx.x % 32, wr = threadIdx.x / 32;  // workspace side
+  if (TO_WS) {
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      const int c = tc + 4 * m, i = i0 + tr;
+      if (i < n && c0 + c < k) tile[tr][c] = B[i + (int64_t)(c0 + c) * ld];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      const int r = wr + 8 * m, i = i0 + r;
+      if (i < n && c0 + wc < k) W[(int64_t)ipos[i] * k + c0 + wc] = tile[r][wc];
+    }
+  } else {
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      const int r = wr + 8 * m, i = i0 + r;
+      if (i < n && c0 + wc < k) tile[r][wc] = W[(int64_t)ipos[i] * k + c0 + wc];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      const int c = tc + 4 * m, i = i0 + tr;
+      if (i < n && c0 + c < k) B[i + (int64_t)(c0 + c) * ld] = tile[tr][c];
+    }
+  }
+}
+
+__global__ void k_cols_to_positions(int64_t nnz, const int *__restrict__ ci,
+                                    const int *__restrict__ ipos, int *cpos) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < nnz) cpos[t] = ipos[ci[t]];
+}
+
 }  // namespace
+
+int prepare_workspace(symamd_handle h, symamd_plan p, int k) {
+  if (!p->d_ipos) {
+    UnitSet *us = nullptr;  // building any unit set builds the inverse map
+    SYMAMD_TRY(plan_units(h, p, 64, false, &us));
+  }
+  if (!p->d_cpos) {
+    SYMAMD_HIP(hipMalloc(&p->d_cpos, sizeof(int) * (p->nnz > 0 ? p->nnz : 1)));
+    if (p->nnz > 0) {
+      k_cols_to_positions<<<(unsigned)((p->nnz + 255) / 256), 256, 0, h->stream>>>(
+          p->nnz, p->d_ci, p->d_ipos, p->d_cpos);
+      SYMAMD_HIP(hipGetLastError());
+    }
+  }
+  const size_t need = (size_t)p->n * k * (p->dtype == SYMAMD_F64 ? 8 : 4);
+  if (need > p->ws_bytes) {
+    if (p->d_ws) SYMAMD_HIP(hipFree(p->d_ws));
+    p->d_ws = nullptr;
+    p->ws_bytes = 0;
+    SYMAMD_HIP(hipMalloc(&p->d_ws, need));
+    p->ws_bytes = need;
+  }
+  SYMAMD_HIP(hipStreamSynchronize(h->stream));
+  return SYMAMD_OK;
+}
+
+// Column-major k > 1: transpose into the level-ordered workspace, solve
+// there with rows = positions (coalesced right-hand sides, dependency ids
+// in position numbering), transpose back.  prepare_workspace must have run.
+int launch_forward_ws(symamd_handle h, symamd_plan p, void *d_B, int k, int64_t ld,
+                      int exec, int lanes) {
+  if (p->n == 0 || k == 0) return SYMAMD_OK;
+  const int kt = pow2_ceil(k < 64 ? k : 64);
+  if (lanes * kt > 64) lanes = 64 / kt;
+  dim3 grid((p->n + 63) / 64, (k + 31) / 32);
+  if (p->dtype == SYMAMD_F64) {
+    k_ws_transpose<double, true><<<grid, 256, 0, h->stream>>>(p->n, k, ld, (double *)d_B,
+                                                              p->d_ipos, (double *)p->d_ws);
+    SolveArgs<double> a = forward_args<double>(h, p, p->d_ws, k, k, 1);
+    a.ci = p->d_cpos;
+    a.perm = nullptr;
+    SYMAMD_TRY(fwd_dispatch<double>(h, p, a, exec, lanes, kt));
+    k_ws_transpose<double, false><<<grid, 256, 0, h->stream>>>(p->n, k, ld, (double *)d_B,
+                                                               p->d_ipos, (double *)p->d_ws);
+  } else {
+    k_ws_transpose<float, true><<<grid, 256, 0, h->stream>>>(p->n, k, ld, (float *)d_B,
+                                                             p->d_ipos, (float *)p->d_ws);
+    SolveArgs<float> a = forward_args<float>(h, p, p->d_ws, k, k, 1);
+    a.ci = p->d_cpos;
+    a.perm = nullptr;
+    SYMAMD_TRY(fwd_dispatch<float>(h, p, a, exec, lanes, kt));
+    k_ws_transpose<float, false><<<grid, 256, 0, h->stream>>>(p->n, k, ld, (float *)d_B,
+                                                              p->d_ipos, (float *)p->d_ws);
+  }
+  SYMAMD_HIP(hipGetLastError());
+  return SYMAMD_OK;
+}
 
 int forward_rows_per_unit(int k, int lanes) {
   int kt = pow2_ceil(k < 64 ? (k > 0 ? k : 1) : 64);

@@ -112,12 +112,14 @@ def test_random_irregular(dev, n, density, min_off):
         assert rel_err(_solve(plan, b, dev, "syncfree", lanes), x_ref) <= RTOL64
 
 
-@pytest.mark.parametrize("deal", ["round_robin", "tickets"])
-def test_c2_full_size(dev, deal):
+@pytest.mark.parametrize("deal,marks", [("round_robin", True), ("tickets", True),
+                                        ("round_robin", False)])
+def test_c2_full_size(dev, deal, marks):
     w = W.config2()
     x_ref = oracle.sptrsv_csr(w.Lp, w.Li, w.Lx, w.b)
     h = sa.Handle()
     h.set_tickets(deal == "tickets")
+    h.set_marks(marks)
     plan = sa.Plan(*to_dev(w.Lp, w.Li, w.Lx, device=dev), handle=h)
     assert plan.levels == 3 * 128 - 2
     for _ in range(2):  # a second solve reuses the unit flags (next epoch)
@@ -145,16 +147,20 @@ def test_sptrsm(dev, k, layout):
     Lp, Li, Lx = W.random_lower(n, 0.002, 1, seed=3)
     B = W.uniform(9, 0, n * k).reshape(k, n)  # column-major n x k
     X_ref = oracle.sptrsm_csr(Lp, Li, Lx, B)
-    plan = _plan(Lp, Li, Lx, dev)
-    for ex in ("syncfree", "levelset"):
-        plan.set_lanes(1)
-        dB = to_dev(B if layout == "col" else np.ascontiguousarray(B.T), device=dev)
-        plan.solve(dB, layout=layout, executor=ex)
-        plan.check()
-        X = dB.cpu().numpy()
-        if layout == "row":
-            X = X.T
-        assert bitwise_equal(X, X_ref), (ex, k, layout)
+    h = sa.Handle()
+    plan = sa.Plan(*to_dev(Lp, Li, Lx, device=dev), handle=h)
+    # column-major: through the level-ordered workspace and in place
+    for direct in ((False, True) if layout == "col" else (False,)):
+        h.set_colmajor_direct(direct)
+        for ex in ("syncfree", "levelset", "graph"):
+            plan.set_lanes(1)
+            dB = to_dev(B if layout == "col" else np.ascontiguousarray(B.T), device=dev)
+            plan.solve(dB, layout=layout, executor=ex)
+            plan.check()
+            X = dB.cpu().numpy()
+            if layout == "row":
+                X = X.T
+            assert bitwise_equal(X, X_ref), (ex, k, layout, direct)
 
 
 # ---------------------------------------------------------------- batched
