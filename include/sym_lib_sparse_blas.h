@@ -37,6 +37,24 @@ void sptrsv_csr_levelset(int n, const int *Lp, const int *Li, const double *Lx,
 void sptrsv_csr_lbc(int n, int *Lp, int *Li, double *Lx, double *x,
                     int level_no, int *level_ptr, int *par_ptr,
                     int *partition);
+/* sparse_blas_lib.h:156-161; b is the right-hand side, x receives the
+ * solution; tmp (threads*n doubles in the reference) is accepted and not
+ * written -- its contents after the reference call depend on its thread
+ * schedule. */
+void sptrsv_csr_lbc_redundant(int n, int *Lp, int *Li, double *Lx, double *x,
+                              double *b, int level_no, int *level_ptr,
+                              int *par_ptr, int *partition, double *tmp);
+/* sparse_blas_lib.h:137-138: unit-lower solve with the lower part of a
+ * general CSR matrix (entries [Lp[i], diags[i]) of each row); like the
+ * reference it does not divide. */
+void sptrsv_csr_ilu(int n, const int *Lp, const int *Li, const double *Lx,
+                    const int *diags, double *x);
+/* sparse_blas_lib.h:139-142: the same rows on a caller schedule; like the
+ * reference this variant divides by Lx[diags[i]]. */
+void sptrsv_csr_ilu_lbc(int n, const int *Lp, const int *Li, const double *Lx,
+                        const int *diags, double *x, const int level_no,
+                        const int *level_ptr, const int *par_ptr,
+                        const int *partition);
 /* sparse_blas_lib.h:169-170 (CSC of L, diagonal first) */
 void sptrsv_csc(int n, int *Lp, int *Li, double *Lx, double *x);
 /* sparse_blas_lib.h:174-175 */

@@ -129,6 +129,32 @@ int symamd_solve_lbc(symamd_handle h, int n, const int *d_Lp, const int *d_Li,
                      const int *h_level_ptr, const int *d_par_ptr,
                      const int *d_partition);
 
+/* The same schedule with a separate right-hand side d_b, with the
+ * arithmetic of sym_lib::sptrsv_csr_lbc_redundant (decl
+ * sparse_blas_lib.h:156-161, impl sparse_blas/sptrsv.cpp:78-106): each row
+ * accumulates its products from zero and adds b[i] last,
+ * x[i] = ((0 - sum_j Lx[j] x[Li[j]]) + b[i]) / diag.  The reference's
+ * threads*n scratch array is not needed. */
+int symamd_solve_lbc_redundant(symamd_handle h, int n, const int *d_Lp,
+                               const int *d_Li, const double *d_Lx, double *d_x,
+                               const double *d_b, int level_no,
+                               const int *h_level_ptr, const int *d_par_ptr,
+                               const int *d_partition);
+
+/* Extract the lower factor stored inside a general CSR matrix A (the L of
+ * an ILU(0) factor kept in A's pattern) as a CSR lower triangle with the
+ * diagonal last: row i = A's entries [Ap[i], diags[i]) followed by column i
+ * with value Ax[diags[i]] (unit_diag == 0) or 1.0 (unit_diag != 0).  Solving
+ * with it reproduces, bit for bit, sym_lib::sptrsv_csr_ilu (unit_diag = 1:
+ * that routine does not divide, sparse_blas/sptrsv.cpp:28-38) and
+ * sym_lib::sptrsv_csr_ilu_lbc (unit_diag = 0: it divides by Ax[diags[i]],
+ * sparse_blas/sptrsv.cpp:109-126); decls sparse_blas_lib.h:137-142.
+ * d_Lp needs n+1 entries, d_Li/d_Lx at least Ap[n]; *nnz gets Lp[n].
+ * SYMAMD_ERR_INVALID_ARG when some diags[i] lies outside row i. */
+int symamd_ilu_lower(symamd_handle h, int n, const int *d_Ap, const int *d_Ai,
+                     const double *d_Ax, const int *d_diags, int unit_diag,
+                     int *d_Lp, int *d_Li, double *d_Lx, int *nnz);
+
 /* ------------------------------------------------------------------ plan */
 /* Analyse L (device CSR, diagonal last) once: level sets, a level-ordered
  * copy of L, work units and ready flags.  The plan owns its device memory;

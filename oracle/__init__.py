@@ -11,6 +11,9 @@ Functions and the reference lines they follow:
   sptrsv_csr           sparse_blas/sptrsv.cpp:17-25
   sptrsv_csr_levelset  sparse_blas/sptrsv.cpp:40-54
   sptrsv_csr_lbc       sparse_blas/sptrsv.cpp:57-75
+  sptrsv_csr_lbc_redundant  sparse_blas/sptrsv.cpp:78-106
+  sptrsv_csr_ilu       sparse_blas/sptrsv.cpp:28-38 (unit lower, no divide)
+  sptrsv_csr_ilu_lbc   sparse_blas/sptrsv.cpp:109-126 (divides)
   sptrsv_csc           sparse_blas/sptrsv.cpp:129-137
   ltsolve              sparse_blas/sptrsv_lt.cpp:11-20
   spmv_csr / _axpby    sparse_blas/spmv.cpp:12-21 / :23-34
@@ -52,6 +55,9 @@ def olib():
             "orc_sptrsv_csr": ([i, vp, vp, vp, vp], None),
             "orc_sptrsv_csr_levelset": ([i, vp, vp, vp, vp, i, vp, vp, i], None),
             "orc_sptrsv_csr_lbc": ([i, vp, vp, vp, vp, i, vp, vp, vp], None),
+            "orc_sptrsv_csr_lbc_redundant": ([i, vp, vp, vp, vp, vp, i, vp, vp, vp], None),
+            "orc_sptrsv_csr_ilu": ([i, vp, vp, vp, vp, vp], None),
+            "orc_sptrsv_csr_ilu_lbc": ([i, vp, vp, vp, vp, vp, i, vp, vp, vp], None),
             "orc_sptrsv_csc": ([i, vp, vp, vp, vp], None),
             "orc_ltsolve": ([i, vp, vp, vp, vp], c_int),
             "orc_sptrsm_csr": ([i, vp, vp, vp, vp, i, i64, i64], None),
@@ -88,6 +94,11 @@ def rlib():
             "ref_sptrsv_csr": ([i, vp, vp, vp, vp], None),
             "ref_sptrsv_csr_levelset": ([i, vp, vp, vp, vp, i, vp, vp], None),
             "ref_sptrsv_csr_lbc": ([i, vp, vp, vp, vp, i, vp, vp, vp], None),
+            "ref_sptrsv_csr_lbc_redundant": ([i, vp, vp, vp, vp, vp, i, vp, vp, vp, vp],
+                                             None),
+            "ref_max_threads": ([], c_int),
+            "ref_sptrsv_csr_ilu": ([i, vp, vp, vp, vp, vp], None),
+            "ref_sptrsv_csr_ilu_lbc": ([i, vp, vp, vp, vp, vp, i, vp, vp, vp], None),
             "ref_sptrsv_csc": ([i, vp, vp, vp, vp], None),
             "ref_ltsolve": ([i, vp, vp, vp, vp], c_int),
             "ref_spmv_csr": ([i, vp, vp, vp, vp, vp], None),
@@ -133,6 +144,31 @@ def sptrsv_csr_lbc(Lp, Li, Lx, b, level_ptr, par_ptr, partition):
     olib().orc_sptrsv_csr_lbc(len(Lp) - 1, _p(_i32(Lp)), _p(_i32(Li)), _p(_f64(Lx)),
                               _p(x), len(level_ptr) - 1, _p(_i32(level_ptr)),
                               _p(_i32(par_ptr)), _p(_i32(partition)))
+    return x
+
+
+def sptrsv_csr_lbc_redundant(Lp, Li, Lx, x0, b, level_ptr, par_ptr, partition):
+    x = _f64(x0).copy()
+    olib().orc_sptrsv_csr_lbc_redundant(len(Lp) - 1, _p(_i32(Lp)), _p(_i32(Li)),
+                                        _p(_f64(Lx)), _p(x), _p(_f64(b)),
+                                        len(level_ptr) - 1, _p(_i32(level_ptr)),
+                                        _p(_i32(par_ptr)), _p(_i32(partition)))
+    return x
+
+
+def sptrsv_csr_ilu(Ap, Ai, Ax, diags, b):
+    x = _f64(b).copy()
+    olib().orc_sptrsv_csr_ilu(len(Ap) - 1, _p(_i32(Ap)), _p(_i32(Ai)), _p(_f64(Ax)),
+                              _p(_i32(diags)), _p(x))
+    return x
+
+
+def sptrsv_csr_ilu_lbc(Ap, Ai, Ax, diags, b, level_ptr, par_ptr, partition):
+    x = _f64(b).copy()
+    olib().orc_sptrsv_csr_ilu_lbc(len(Ap) - 1, _p(_i32(Ap)), _p(_i32(Ai)), _p(_f64(Ax)),
+                                  _p(_i32(diags)), _p(x), len(level_ptr) - 1,
+                                  _p(_i32(level_ptr)), _p(_i32(par_ptr)),
+                                  _p(_i32(partition)))
     return x
 
 
@@ -261,6 +297,33 @@ class ref:
                                   _p(_f64(Lx).copy()), _p(x), len(level_ptr) - 1,
                                   _p(_i32(level_ptr).copy()), _p(_i32(par_ptr).copy()),
                                   _p(_i32(partition).copy()))
+        return x
+
+    @staticmethod
+    def sptrsv_csr_lbc_redundant(Lp, Li, Lx, x0, b, level_ptr, par_ptr, partition):
+        x = _f64(x0).copy()
+        n = len(Lp) - 1
+        tmp = np.zeros(max(1, rlib().ref_max_threads() * n))
+        rlib().ref_sptrsv_csr_lbc_redundant(
+            n, _p(_i32(Lp).copy()), _p(_i32(Li).copy()), _p(_f64(Lx).copy()), _p(x),
+            _p(_f64(b).copy()), len(level_ptr) - 1, _p(_i32(level_ptr).copy()),
+            _p(_i32(par_ptr).copy()), _p(_i32(partition).copy()), _p(tmp))
+        return x
+
+    @staticmethod
+    def sptrsv_csr_ilu(Ap, Ai, Ax, diags, b):
+        x = _f64(b).copy()
+        rlib().ref_sptrsv_csr_ilu(len(Ap) - 1, _p(_i32(Ap)), _p(_i32(Ai)), _p(_f64(Ax)),
+                                  _p(_i32(diags)), _p(x))
+        return x
+
+    @staticmethod
+    def sptrsv_csr_ilu_lbc(Ap, Ai, Ax, diags, b, level_ptr, par_ptr, partition):
+        x = _f64(b).copy()
+        rlib().ref_sptrsv_csr_ilu_lbc(len(Ap) - 1, _p(_i32(Ap)), _p(_i32(Ai)),
+                                      _p(_f64(Ax)), _p(_i32(diags)), _p(x),
+                                      len(level_ptr) - 1, _p(_i32(level_ptr)),
+                                      _p(_i32(par_ptr)), _p(_i32(partition)))
         return x
 
     @staticmethod

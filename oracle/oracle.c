@@ -80,6 +80,63 @@ void orc_sptrsv_csr_lbc(int n, const int *Lp, const int *Li, const double *Lx,
   }
 }
 
+/* sparse_blas/sptrsv.cpp:78-106 (sym_lib::sptrsv_csr_lbc_redundant): the
+ * LBC schedule with a separate right-hand side b; each row accumulates its
+ * products into a zeroed temporary and adds b[i] last:
+ * x[i] = ((0 - sum_j Lx[j]*x[Li[j]]) + b[i]) / diag.  The reference's
+ * per-thread tmp array is scratch whose final contents depend on its
+ * thread schedule; this restatement keeps the temporary in a register. */
+void orc_sptrsv_csr_lbc_redundant(int n, const int *Lp, const int *Li,
+                                  const double *Lx, double *x, const double *b,
+                                  int level_no, const int *level_ptr,
+                                  const int *par_ptr, const int *partition) {
+  (void)n;
+  for (int i1 = 0; i1 < level_no; ++i1) {
+#pragma omp parallel for schedule(static)
+    for (int j1 = level_ptr[i1]; j1 < level_ptr[i1 + 1]; ++j1) {
+      for (int k1 = par_ptr[j1]; k1 < par_ptr[j1 + 1]; ++k1) {
+        int i = partition[k1];
+        double t = 0.0;
+        for (int j = Lp[i]; j < Lp[i + 1] - 1; j++) t -= Lx[j] * x[Li[j]];
+        t = (t + b[i]) / Lx[Lp[i + 1] - 1];
+        x[i] = t;
+      }
+    }
+  }
+}
+
+/* sparse_blas/sptrsv.cpp:28-38 (sym_lib::sptrsv_csr_ilu): unit-lower solve
+ * with the L of a general CSR matrix A (the ILU(0) factor stored in A's
+ * pattern): row i uses A's entries [Ap[i], diags[i]) and does NOT divide
+ * (the division is commented out in the reference). */
+void orc_sptrsv_csr_ilu(int n, const int *Ap, const int *Ai, const double *Ax,
+                        const int *diags, double *x) {
+  for (int i = 0; i < n; i++) {
+    int dia = diags[i];
+    for (int j = Ap[i]; j < dia; j++) x[i] -= Ax[j] * x[Ai[j]];
+  }
+}
+
+/* sparse_blas/sptrsv.cpp:109-126 (sym_lib::sptrsv_csr_ilu_lbc): the same
+ * rows on an LBC schedule; this variant DOES divide by Ax[diags[i]]. */
+void orc_sptrsv_csr_ilu_lbc(int n, const int *Ap, const int *Ai,
+                            const double *Ax, const int *diags, double *x,
+                            int level_no, const int *level_ptr,
+                            const int *par_ptr, const int *partition) {
+  (void)n;
+  for (int i1 = 0; i1 < level_no; ++i1) {
+#pragma omp parallel for schedule(static)
+    for (int j1 = level_ptr[i1]; j1 < level_ptr[i1 + 1]; ++j1) {
+      for (int k1 = par_ptr[j1]; k1 < par_ptr[j1 + 1]; ++k1) {
+        int i = partition[k1];
+        int dia = diags[i];
+        for (int j = Ap[i]; j < dia; j++) x[i] -= Ax[j] * x[Ai[j]];
+        x[i] /= Ax[dia];
+      }
+    }
+  }
+}
+
 /* sparse_blas/sptrsv.cpp:129-137 (sym_lib::sptrsv_csc): column-push forward
  * substitution on the CSC of L, diagonal first. */
 void orc_sptrsv_csc(int n, const int *Lp, const int *Li, const double *Lx,

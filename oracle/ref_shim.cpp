@@ -22,6 +22,16 @@ void sptrsv_csr_levelset(int n, const int *Lp, const int *Li,
 void sptrsv_csr_lbc(int n, int *Lp, int *Li, double *Lx, double *x,
                     int level_no, int *level_ptr, int *par_ptr,
                     int *partition);  // :150-152
+void sptrsv_csr_lbc_redundant(int n, int *Lp, int *Li, double *Lx, double *x,
+                              double *b, int level_no, int *level_ptr,
+                              int *par_ptr, int *partition,
+                              double *tmp);  // :156-161
+void sptrsv_csr_ilu(int n, const int *Lp, const int *Li, const double *Lx,
+                    const int *diags, double *x);  // :137-138
+void sptrsv_csr_ilu_lbc(int n, const int *Lp, const int *Li, const double *Lx,
+                        const int *diags, double *x, const int level_no,
+                        const int *level_ptr, const int *par_ptr,
+                        const int *partition);  // :139-142
 void sptrsv_csc(int n, int *Lp, int *Li, double *Lx, double *x);  // :169-170
 int ltsolve(int n, int *Lp, int *Li, double *Lx, double *x);      // :330
 void spmv_csr(int n, const int *Ap, const int *Ai, const double *Ax,
@@ -57,6 +67,26 @@ void ref_sptrsv_csr_lbc(int n, int *Lp, int *Li, double *Lx, double *x,
                         int *partition) {
   sym_lib::sptrsv_csr_lbc(n, Lp, Li, Lx, x, level_no, level_ptr, par_ptr,
                           partition);
+}
+// tmp: omp_get_max_threads() * n doubles, as the reference requires
+void ref_sptrsv_csr_lbc_redundant(int n, int *Lp, int *Li, double *Lx,
+                                  double *x, double *b, int level_no,
+                                  int *level_ptr, int *par_ptr, int *partition,
+                                  double *tmp) {
+  sym_lib::sptrsv_csr_lbc_redundant(n, Lp, Li, Lx, x, b, level_no, level_ptr,
+                                    par_ptr, partition, tmp);
+}
+int ref_max_threads() { return omp_get_max_threads(); }
+void ref_sptrsv_csr_ilu(int n, const int *Lp, const int *Li, const double *Lx,
+                        const int *diags, double *x) {
+  sym_lib::sptrsv_csr_ilu(n, Lp, Li, Lx, diags, x);
+}
+void ref_sptrsv_csr_ilu_lbc(int n, const int *Lp, const int *Li,
+                            const double *Lx, const int *diags, double *x,
+                            int level_no, const int *level_ptr,
+                            const int *par_ptr, const int *partition) {
+  sym_lib::sptrsv_csr_ilu_lbc(n, Lp, Li, Lx, diags, x, level_no, level_ptr,
+                              par_ptr, partition);
 }
 void ref_sptrsv_csc(int n, int *Lp, int *Li, double *Lx, double *x) {
   sym_lib::sptrsv_csc(n, Lp, Li, Lx, x);

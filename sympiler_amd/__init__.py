@@ -16,6 +16,9 @@ Reference mapping (reference include/sympiler/sparse_blas_lib.h):
   coarsen()              get_coarse_levelSet_DAG_CSC_tree (absent)
   spmv_csr()             sym_lib::spmv_csr (sparse_blas/spmv.cpp:12-34)
   solve_lbc()            sym_lib::sptrsv_csr_lbc (sparse_blas/sptrsv.cpp:57-75)
+  solve_lbc_redundant()  sym_lib::sptrsv_csr_lbc_redundant (sptrsv.cpp:78-106)
+  ilu_lower()            the lower factor read by sym_lib::sptrsv_csr_ilu /
+                         sptrsv_csr_ilu_lbc (sptrsv.cpp:28-38, :109-126)
 """
 from __future__ import annotations
 
@@ -66,6 +69,9 @@ def lib() -> ctypes.CDLL:
             "symamd_coarsen": [vp, i32, vp, vp, i32, i32, vp, vp, vp,
                                POINTER(i32), POINTER(i32)],
             "symamd_solve_lbc": [vp, i32, vp, vp, vp, vp, i32, vp, vp, vp],
+            "symamd_solve_lbc_redundant": [vp, i32, vp, vp, vp, vp, vp, i32, vp, vp, vp],
+            "symamd_ilu_lower": [vp, i32, vp, vp, vp, vp, i32, vp, vp, vp,
+                                 POINTER(i32)],
             "symamd_analyze": [vp, i32, vp, vp, vp, i32, POINTER(vp)],
             "symamd_analyze_batched": [vp, i32, vp, vp, vp, vp, i32, POINTER(vp)],
             "symamd_plan_update_values": [vp, vp, vp],
@@ -368,6 +374,43 @@ def solve_lbc(Lp, Li, Lx, x, level_ptr, par_ptr, partition,
                                   lp_host.ctypes.data_as(c_void_p), _ptr(par_ptr),
                                   _ptr(partition)))
     return x
+
+
+def solve_lbc_redundant(Lp, Li, Lx, x, b, level_ptr, par_ptr, partition,
+                        handle: Handle | None = None):
+    """Execute a caller schedule with right-hand side b, writing x, with the
+    arithmetic of sptrsv_csr_lbc_redundant (products from zero, b added
+    last); FP64."""
+    import numpy as np
+    _require_cuda(Lp, Li, Lx, x, b, par_ptr, partition)
+    h = handle or default_handle()
+    lp_host = np.ascontiguousarray(level_ptr.cpu().numpy() if hasattr(level_ptr, "cpu")
+                                   else level_ptr, dtype=np.int32)
+    _check(lib().symamd_solve_lbc_redundant(h._h, Lp.numel() - 1, _ptr(Lp), _ptr(Li),
+                                            _ptr(Lx), _ptr(x), _ptr(b), len(lp_host) - 1,
+                                            lp_host.ctypes.data_as(c_void_p),
+                                            _ptr(par_ptr), _ptr(partition)))
+    return x
+
+
+def ilu_lower(Ap, Ai, Ax, diags, unit_diag: bool, handle: Handle | None = None):
+    """The lower factor inside a general CSR matrix A with diagonal positions
+    `diags`, as a CSR lower triangle (diagonal last): (Lp, Li, Lx).  With
+    unit_diag the diagonal is 1.0 (sptrsv_csr_ilu does not divide), else
+    A's value at diags[i] (sptrsv_csr_ilu_lbc divides by it)."""
+    import torch
+    _require_cuda(Ap, Ai, Ax, diags)
+    h = handle or default_handle()
+    n = Ap.numel() - 1
+    cap = max(int(Ap[-1].item()) if n > 0 else 0, 1)
+    Lp = torch.empty(n + 1, dtype=torch.int32, device=Ap.device)
+    Li = torch.empty(cap, dtype=torch.int32, device=Ap.device)
+    Lx = torch.empty(cap, dtype=torch.float64, device=Ap.device)
+    nnz = c_int()
+    _check(lib().symamd_ilu_lower(h._h, n, _ptr(Ap), _ptr(Ai), _ptr(Ax), _ptr(diags),
+                                  1 if unit_diag else 0, _ptr(Lp), _ptr(Li), _ptr(Lx),
+                                  byref(nnz)))
+    return Lp, Li[: nnz.value], Lx[: nnz.value]
 
 
 def spmv_csr(Ap, Ai, Ax, x, y=None, z=None, a=None, b=None,

@@ -768,8 +768,32 @@ int symamd_solve_lbc(symamd_handle h, int n, const int *d_Lp, const int *d_Li,
   if (n < 0 || level_no < 0 || (level_no > 0 && (!h_level_ptr || !d_par_ptr || !d_partition)) ||
       (n > 0 && (!d_Lp || !d_Li || !d_Lx || !d_x)))
     return fail(SYMAMD_ERR_INVALID_ARG, "symamd_solve_lbc: bad arguments");
-  return launch_lbc(h, n, d_Lp, d_Li, d_Lx, d_x, level_no, h_level_ptr,
+  return launch_lbc(h, n, d_Lp, d_Li, d_Lx, d_x, nullptr, level_no, h_level_ptr,
                     d_par_ptr, d_partition);
+}
+
+int symamd_solve_lbc_redundant(symamd_handle h, int n, const int *d_Lp,
+                               const int *d_Li, const double *d_Lx, double *d_x,
+                               const double *d_b, int level_no,
+                               const int *h_level_ptr, const int *d_par_ptr,
+                               const int *d_partition) {
+  SYMAMD_TRY(check_handle(h));
+  if (n < 0 || level_no < 0 || (level_no > 0 && (!h_level_ptr || !d_par_ptr || !d_partition)) ||
+      (n > 0 && (!d_Lp || !d_Li || !d_Lx || !d_x || !d_b)))
+    return fail(SYMAMD_ERR_INVALID_ARG, "symamd_solve_lbc_redundant: bad arguments");
+  return launch_lbc(h, n, d_Lp, d_Li, d_Lx, d_x, d_b, level_no, h_level_ptr,
+                    d_par_ptr, d_partition);
+}
+
+int symamd_ilu_lower(symamd_handle h, int n, const int *d_Ap, const int *d_Ai,
+                     const double *d_Ax, const int *d_diags, int unit_diag,
+                     int *d_Lp, int *d_Li, double *d_Lx, int *nnz) {
+  SYMAMD_TRY(check_handle(h));
+  if (n < 0 || !nnz || !d_Lp || (n > 0 && (!d_Ap || !d_Ai || !d_Ax || !d_diags ||
+                                          !d_Li || !d_Lx)))
+    return fail(SYMAMD_ERR_INVALID_ARG, "symamd_ilu_lower: bad arguments");
+  return ilu_lower(h, n, d_Ap, d_Ai, d_Ax, d_diags, unit_diag ? 1 : 0, d_Lp, d_Li, d_Lx,
+                   nnz);
 }
 
 }  // extern "C"

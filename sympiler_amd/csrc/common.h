@@ -219,10 +219,15 @@ int launch_forward_ws(symamd_handle h, symamd_plan p, void *d_B, int k, int64_t 
 int coarsen(symamd_handle h, int n, const int *d_Lp, const int *d_Li,
             int span, int max_cost, int *d_level_ptr, int *d_par_ptr,
             int *d_partition, int *level_no, int *part_no);
+// d_b == nullptr: sptrsv_csr_lbc arithmetic in place on d_x; else the
+// sptrsv_csr_lbc_redundant arithmetic with right-hand side d_b.
 int launch_lbc(symamd_handle h, int n, const int *d_Lp, const int *d_Li,
-               const double *d_Lx, double *d_x, int level_no,
+               const double *d_Lx, double *d_x, const double *d_b, int level_no,
                const int *h_level_ptr, const int *d_par_ptr,
                const int *d_partition);
+int ilu_lower(symamd_handle h, int n, const int *d_Ap, const int *d_Ai,
+              const double *d_Ax, const int *d_diags, int unit, int *d_Lp,
+              int *d_Li, double *d_Lx, int *nnz_out);
 
 inline int64_t tune_key(int k, int layout) { return ((int64_t)k << 2) | layout; }
 

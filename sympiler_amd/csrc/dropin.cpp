@@ -175,6 +175,66 @@ void sptrsv_csr_lbc(int n, int *Lp, int *Li, double *Lx, double *x,
   dx.download(x, n);
 }
 
+void sptrsv_csr_lbc_redundant(int n, int *Lp, int *Li, double *Lx, double *x,
+                              double *b, int level_no, int *level_ptr,
+                              int *par_ptr, int *partition, double *tmp) {
+  (void)tmp;
+  if (n <= 0) return;
+  symamd_handle h = handle();
+  const int nnz = Lp[n];
+  const int nparts = level_no > 0 ? level_ptr[level_no] : 0;
+  DevBuf<int> dLp(n + 1, Lp), dLi(nnz, Li), dpp(nparts + 1, par_ptr),
+      dpart(n, partition);
+  DevBuf<double> dLx(nnz, Lx), dx(n, x), db(n, b);
+  check(symamd_solve_lbc_redundant(h, n, dLp.p, dLi.p, dLx.p, dx.p, db.p, level_no,
+                                   level_ptr, dpp.p, dpart.p),
+        "sptrsv_csr_lbc_redundant");
+  hcheck(hipDeviceSynchronize(), "sptrsv_csr_lbc_redundant");
+  dx.download(x, n);
+}
+
+void sptrsv_csr_ilu(int n, const int *Lp, const int *Li, const double *Lx,
+                    const int *diags, double *x) {
+  if (n <= 0) return;
+  symamd_handle h = handle();
+  const int nnz = Lp[n];
+  DevBuf<int> dAp(n + 1, Lp), dAi(nnz, Li), ddiag(n, diags), dLp(n + 1), dLi(nnz);
+  DevBuf<double> dAx(nnz, Lx), dLx(nnz), dx(n, x);
+  int lnnz = 0;
+  check(symamd_ilu_lower(h, n, dAp.p, dAi.p, dAx.p, ddiag.p, 1, dLp.p, dLi.p, dLx.p,
+                         &lnnz),
+        "sptrsv_csr_ilu");
+  Plan pl;
+  check(symamd_analyze(h, n, dLp.p, dLi.p, dLx.p, SYMAMD_F64, &pl.p), "sptrsv_csr_ilu");
+  check(symamd_plan_set_lanes(pl.p, 1), "set_lanes");
+  check(symamd_solve(h, pl.p, dx.p, 1, n, SYMAMD_COL_MAJOR, SYMAMD_EXEC_SYNCFREE),
+        "sptrsv_csr_ilu");
+  check(symamd_plan_check(h, pl.p), "sptrsv_csr_ilu");
+  dx.download(x, n);
+}
+
+void sptrsv_csr_ilu_lbc(int n, const int *Lp, const int *Li, const double *Lx,
+                        const int *diags, double *x, const int level_no,
+                        const int *level_ptr, const int *par_ptr,
+                        const int *partition) {
+  if (n <= 0) return;
+  symamd_handle h = handle();
+  const int nnz = Lp[n];
+  const int nparts = level_no > 0 ? level_ptr[level_no] : 0;
+  DevBuf<int> dAp(n + 1, Lp), dAi(nnz, Li), ddiag(n, diags), dLp(n + 1), dLi(nnz),
+      dpp(nparts + 1, par_ptr), dpart(n, partition);
+  DevBuf<double> dAx(nnz, Lx), dLx(nnz), dx(n, x);
+  int lnnz = 0;
+  check(symamd_ilu_lower(h, n, dAp.p, dAi.p, dAx.p, ddiag.p, 0, dLp.p, dLi.p, dLx.p,
+                         &lnnz),
+        "sptrsv_csr_ilu_lbc");
+  check(symamd_solve_lbc(h, n, dLp.p, dLi.p, dLx.p, dx.p, level_no, level_ptr, dpp.p,
+                         dpart.p),
+        "sptrsv_csr_ilu_lbc");
+  hcheck(hipDeviceSynchronize(), "sptrsv_csr_ilu_lbc");
+  dx.download(x, n);
+}
+
 void sptrsv_csc(int n, int *Lp, int *Li, double *Lx, double *x) {
   if (n <= 0) return;
   std::vector<int> Rp, Ri;

@@ -507,6 +507,26 @@ __global__ __launch_bounds__(256) void k_lbc_level(
   }
 }
 
+// The same schedule with a separate right-hand side, with the arithmetic of
+// sparse_blas/sptrsv.cpp:78-106 (sptrsv_csr_lbc_redundant): the products
+// are accumulated from zero and b[i] is added last,
+// x[i] = ((0 - sum_j Lx[j] x[Li[j]]) + b[i]) / diag.  The reference's
+// per-thread scratch row is a register here.
+__global__ __launch_bounds__(256) void k_lbc_redundant_level(
+    const int *__restrict__ Lp, const int *__restrict__ Li,
+    const double *__restrict__ Lx, double *x, const double *__restrict__ b,
+    const int *__restrict__ par_ptr, const int *__restrict__ partition, int p0, int p1) {
+  const int part = p0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (part >= p1) return;
+  for (int k1 = par_ptr[part]; k1 < par_ptr[part + 1]; ++k1) {
+    const int i = partition[k1];
+    double t = 0.0;
+    const int je = Lp[i + 1] - 1;
+    for (int j = Lp[i]; j < je; ++j) t -= Lx[j] * x[Li[j]];
+    x[i] = (t + b[i]) / Lx[je];
+  }
+}
+
 // ------------------------------------------------------------ dispatch
 
 // Co-resident grid of 256-thread blocks for a persistent kernel.
@@ -784,15 +804,19 @@ int forward_rows_per_unit(int k, int lanes) {
 }
 
 int launch_lbc(symamd_handle h, int n, const int *d_Lp, const int *d_Li,
-               const double *d_Lx, double *d_x, int level_no,
+               const double *d_Lx, double *d_x, const double *d_b, int level_no,
                const int *h_level_ptr, const int *d_par_ptr,
                const int *d_partition) {
   (void)n;
   for (int l = 0; l < level_no; ++l) {
     const int p0 = h_level_ptr[l], p1 = h_level_ptr[l + 1];
     if (p1 <= p0) continue;
-    k_lbc_level<<<(p1 - p0 + 255) / 256, 256, 0, h->stream>>>(
-        d_Lp, d_Li, d_Lx, d_x, d_par_ptr, d_partition, p0, p1);
+    if (d_b)
+      k_lbc_redundant_level<<<(p1 - p0 + 255) / 256, 256, 0, h->stream>>>(
+          d_Lp, d_Li, d_Lx, d_x, d_b, d_par_ptr, d_partition, p0, p1);
+    else
+      k_lbc_level<<<(p1 - p0 + 255) / 256, 256, 0, h->stream>>>(
+          d_Lp, d_Li, d_Lx, d_x, d_par_ptr, d_partition, p0, p1);
   }
   SYMAMD_HIP(hipGetLastError());
   return SYMAMD_OK;

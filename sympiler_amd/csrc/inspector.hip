@@ -286,7 +286,82 @@ int scan_exclusive(symamd_handle h, const int *len, int *out, int64_t m1) {
   return SYMAMD_OK;
 }
 
+// Lower factor stored inside a general CSR matrix A with diagonal positions
+// diags[i] (the L of ILU(0), as sparse_blas/sptrsv.cpp:28-38 and :109-126
+// read it: entries [Ap[i], diags[i]) and the value at diags[i]).
+// len[i] = row length of the extracted factor; bad |= 1 when diags[i] lies
+// outside row i.
+__global__ void k_ilu_rowlen(int n, const int *__restrict__ Ap,
+                             const int *__restrict__ diags, int *len, int *bad) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > n) return;
+  if (i == n) {
+    len[n] = 0;
+    return;
+  }
+  const int d = diags[i];
+  const bool ok = d >= Ap[i] && d < Ap[i + 1];
+  if (!ok) atomicOr(bad, 1);
+  len[i] = ok ? d - Ap[i] + 1 : 1;
+}
+
+__global__ void k_ilu_copy(int n, const int *__restrict__ Ap, const int *__restrict__ Ai,
+                           const double *__restrict__ Ax, const int *__restrict__ diags,
+                           int unit, const int *__restrict__ Lp, int *Li, double *Lx) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int a0 = Ap[i], d = diags[i], o = Lp[i];
+  for (int j = a0; j < d; ++j) {
+    Li[o + j - a0] = Ai[j];
+    Lx[o + j - a0] = Ax[j];
+  }
+  Li[o + d - a0] = i;
+  Lx[o + d - a0] = unit ? 1.0 : Ax[d];
+}
+
 }  // namespace
+
+int ilu_lower(symamd_handle h, int n, const int *d_Ap, const int *d_Ai,
+              const double *d_Ax, const int *d_diags, int unit, int *d_Lp,
+              int *d_Li, double *d_Lx, int *nnz_out) {
+  if (n == 0) {
+    SYMAMD_HIP(hipMemsetAsync(d_Lp, 0, sizeof(int), h->stream));
+    SYMAMD_HIP(hipStreamSynchronize(h->stream));
+    *nnz_out = 0;
+    return SYMAMD_OK;
+  }
+  int *len = nullptr;  // n+1 row lengths, then the error flag
+  SYMAMD_HIP(hipMallocAsync(&len, sizeof(int) * (n + 2), h->stream));
+  int *bad = len + n + 1;
+  int rc = SYMAMD_OK;
+  int hb[2] = {0, 0};
+  if (hipMemsetAsync(bad, 0, sizeof(int), h->stream) != hipSuccess) {
+    rc = fail(SYMAMD_ERR_HIP, "hipMemsetAsync failed");
+  } else {
+    k_ilu_rowlen<<<blocks_for(n + 1), 256, 0, h->stream>>>(n, d_Ap, d_diags, len, bad);
+    rc = scan_exclusive(h, len, d_Lp, n + 1);
+  }
+  if (rc == SYMAMD_OK &&
+      (hipMemcpyAsync(&hb[0], bad, sizeof(int), hipMemcpyDeviceToHost, h->stream) !=
+           hipSuccess ||
+       hipMemcpyAsync(&hb[1], d_Lp + n, sizeof(int), hipMemcpyDeviceToHost, h->stream) !=
+           hipSuccess ||
+       hipStreamSynchronize(h->stream) != hipSuccess))
+    rc = fail(SYMAMD_ERR_HIP, "reading the factor size failed");
+  if (rc == SYMAMD_OK && hb[0])
+    rc = fail(SYMAMD_ERR_INVALID_ARG, "diags[i] does not lie inside row i");
+  if (rc == SYMAMD_OK) {
+    k_ilu_copy<<<blocks_for(n), 256, 0, h->stream>>>(n, d_Ap, d_Ai, d_Ax, d_diags, unit,
+                                                     d_Lp, d_Li, d_Lx);
+    if (hipGetLastError() != hipSuccess) rc = fail(SYMAMD_ERR_HIP, "k_ilu_copy launch failed");
+  }
+  (void)hipFreeAsync(len, h->stream);
+  if (rc == SYMAMD_OK) {
+    SYMAMD_HIP(hipStreamSynchronize(h->stream));
+    *nnz_out = hb[1];
+  }
+  return rc;
+}
 
 int check_structure(symamd_handle h, int n, const int *d_Lp, const int *d_Li,
                     int *d_state) {

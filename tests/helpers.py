@@ -61,3 +61,55 @@ def known_answer_csr():
     # CSC of L is the CSR of L^T; transposing it gives the CSR of L
     Rp, Ri, Rx, _ = oracle.csr_to_csc(KA_CSC_P, KA_CSC_I, Cx)
     return Rp, Ri, Rx
+
+
+def general_with_diags(Lp, Li, Lx, upper_per_row, seed):
+    """A general CSR matrix A whose lower triangle (diagonal included) is the
+    given L, plus up to `upper_per_row` random strictly-upper entries per row
+    (columns ascending), and the position of each row's diagonal -- the
+    layout sptrsv_csr_ilu / sptrsv_csr_ilu_lbc take
+    (sparse_blas/sptrsv.cpp:28-38, :109-126)."""
+    rng = np.random.default_rng(seed)
+    n = len(Lp) - 1
+    Ap = np.zeros(n + 1, np.int32)
+    Ai, Ax, diags = [], [], np.zeros(n, np.int32)
+    for i in range(n):
+        lo, hi = Lp[i], Lp[i + 1]
+        Ai.extend(Li[lo:hi].tolist())
+        Ax.extend(Lx[lo:hi].tolist())
+        diags[i] = len(Ai) - 1
+        m = min(upper_per_row, n - 1 - i)
+        if m > 0:
+            cols = np.sort(rng.choice(np.arange(i + 1, n), size=rng.integers(0, m + 1),
+                                      replace=False))
+            Ai.extend(cols.tolist())
+            Ax.extend(rng.uniform(-1.0, 1.0, len(cols)).tolist())
+        Ap[i + 1] = len(Ai)
+    return Ap, np.asarray(Ai, np.int32), np.asarray(Ax, np.float64), diags
+
+
+def ilu_lower_host(Ap, Ai, Ax, diags, unit):
+    """Row i of the lower factor inside A: entries [Ap[i], diags[i]) then the
+    diagonal (value 1.0 if `unit`, else Ax[diags[i]])."""
+    n = len(Ap) - 1
+    Lp = np.zeros(n + 1, np.int32)
+    Li, Lx = [], []
+    for i in range(n):
+        Li.extend(Ai[Ap[i]:diags[i]].tolist() + [i])
+        Lx.extend(Ax[Ap[i]:diags[i]].tolist() + [1.0 if unit else Ax[diags[i]]])
+        Lp[i + 1] = len(Li)
+    return Lp, np.asarray(Li, np.int32), np.asarray(Lx, np.float64)
+
+
+def lbc_schedules(lp, ls):
+    """Two valid 3-array schedules (level_ptr, par_ptr, partition) built from
+    a level set: every row its own partition, one coarse level per level;
+    and three levels per coarse level, one partition each (rows in level
+    order)."""
+    nl = len(lp) - 1
+    per_row = (np.asarray(lp, np.int32), np.arange(len(ls) + 1, dtype=np.int32),
+               np.asarray(ls, np.int32))
+    cl = list(range(0, nl, 3)) + [nl]
+    coarse = (np.arange(len(cl), dtype=np.int32),
+              np.array([lp[c] for c in cl], np.int32), np.asarray(ls, np.int32))
+    return {"per_row": per_row, "coarse3": coarse}

@@ -17,7 +17,8 @@ import oracle
 from oracle import ref
 from sympiler_amd import workloads as W
 from tests.helpers import (KA_B, KA_CSC_I, KA_CSC_P, bitwise_equal,
-                           etree_lower, known_answer_csr)
+                           etree_lower, general_with_diags, ilu_lower_host,
+                           known_answer_csr, lbc_schedules)
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 needs_ref = pytest.mark.skipif(not oracle.ref_available(),
@@ -156,6 +157,73 @@ def test_lbc_matches_reference():
     x = oracle.sptrsv_csr_lbc(Lp, Li, Lx, b, level_ptr, par_ptr, ls)
     assert bitwise_equal(x, ref.sptrsv_csr_lbc(Lp, Li, Lx, b, level_ptr, par_ptr, ls))
     assert bitwise_equal(x, oracle.sptrsv_csr(Lp, Li, Lx, b))
+
+
+@needs_ref
+@pytest.mark.parametrize("sched", ["per_row", "coarse3"])
+def test_lbc_redundant_matches_reference(sched):
+    Lp, Li, Lx = W.random_lower(3000, 0.002, 1, seed=12)
+    n = len(Lp) - 1
+    b = W.uniform(12, 0, n)
+    x0 = W.uniform(13, 0, n)  # not read for the result: every row is rewritten
+    lp, ls, _ = oracle.levelset_csr(Lp, Li)
+    s = lbc_schedules(lp, ls)[sched]
+    x = oracle.sptrsv_csr_lbc_redundant(Lp, Li, Lx, x0, b, *s)
+    assert bitwise_equal(x, ref.sptrsv_csr_lbc_redundant(Lp, Li, Lx, x0, b, *s))
+    # b is added after the products, so x differs from sptrsv_csr by rounding
+    # only; a dropped b or a sign error would not
+    x_ser = oracle.sptrsv_csr(Lp, Li, Lx, b)
+    assert np.max(np.abs(x - x_ser)) <= 1e-13 * np.max(np.abs(x_ser))
+    assert not bitwise_equal(x, x_ser)
+
+
+def test_lbc_redundant_exact_on_known_answer():
+    # all-ones L and b = L*ones: every partial sum is an exact integer, so
+    # the different association still gives x == 1 exactly
+    Rp, Ri, Rx = known_answer_csr()
+    lp, ls, _ = oracle.levelset_csr(Rp, Ri)
+    for s in lbc_schedules(lp, ls).values():
+        x = oracle.sptrsv_csr_lbc_redundant(Rp, Ri, Rx, np.zeros(10), KA_B, *s)
+        assert np.all(x == 1.0)
+
+
+@needs_ref
+@pytest.mark.parametrize("seed", [1, 2])
+def test_ilu_matches_reference(seed):
+    L = W.random_lower(2500, 0.003, 1, seed=20 + seed)
+    Ap, Ai, Ax, diags = general_with_diags(*L, upper_per_row=4, seed=seed)
+    n = len(Ap) - 1
+    b = W.uniform(30 + seed, 0, n)
+    x = oracle.sptrsv_csr_ilu(Ap, Ai, Ax, diags, b)
+    assert bitwise_equal(x, ref.sptrsv_csr_ilu(Ap, Ai, Ax, diags, b))
+    # the serial variant does not divide: it is the unit-lower solve of the
+    # lower part; the upper entries are never read
+    Up, Ui, Ux = ilu_lower_host(Ap, Ai, Ax, diags, unit=True)
+    assert bitwise_equal(x, oracle.sptrsv_csr(Up, Ui, Ux, b))
+    lp, ls, _ = oracle.levelset_csr(Up, Ui)
+    Dp, Di, Dx = ilu_lower_host(Ap, Ai, Ax, diags, unit=False)
+    for s in lbc_schedules(lp, ls).values():
+        xl = oracle.sptrsv_csr_ilu_lbc(Ap, Ai, Ax, diags, b, *s)
+        assert bitwise_equal(xl, ref.sptrsv_csr_ilu_lbc(Ap, Ai, Ax, diags, b, *s))
+        # the LBC variant divides by the diagonal: the solve of the lower part
+        assert bitwise_equal(xl, oracle.sptrsv_csr(Dp, Di, Dx, b))
+
+
+def test_ilu_closed_form():
+    # b = L_unit * x_true recovers x_true; the strictly upper part of A and
+    # its diagonal values must not enter the unit solve
+    L = W.random_lower(1500, 0.004, 1, seed=5)
+    Ap, Ai, Ax, diags = general_with_diags(*L, upper_per_row=6, seed=9)
+    Up, Ui, Ux = ilu_lower_host(Ap, Ai, Ax, diags, unit=True)
+    xt = W.uniform(7, 0, len(Ap) - 1)
+    b = oracle.spmv_csr(Up, Ui, Ux, xt)
+    x = oracle.sptrsv_csr_ilu(Ap, Ai, Ax, diags, b)
+    assert np.max(np.abs(x - xt)) < 1e-10
+    Ax2 = Ax.copy()
+    Ax2[diags] *= 3.0
+    for i in range(len(Ap) - 1):
+        Ax2[diags[i] + 1:Ap[i + 1]] = 99.0
+    assert bitwise_equal(oracle.sptrsv_csr_ilu(Ap, Ai, Ax2, diags, b), x)
 
 
 @needs_ref
