@@ -86,8 +86,23 @@ int symamd_set_timeout_ms(symamd_handle h, double ms);
  *     tag is published after the stores are acknowledged.  The solve
  *     overwrites B in place either way; intermediate contents of B are
  *     unspecified while a solve runs. */
+/*   SYMAMD_OPT_REGIONS (read by symamd_analyze*): 1 = cluster the rows into
+ *     compact regions of the dependency graph (heavy-edge matching, at most
+ *     SYMAMD_OPT_REGION_ROWS rows each) when at least half of L's
+ *     off-diagonal entries fall inside a region; the units of each region
+ *     are then solved level after level by one wavefront of the sync-free
+ *     executor (the owner schedule), which waits only on dependencies owned
+ *     by other wavefronts.  0 = never (default: on MI355X the owner
+ *     schedule serializes each unit's staging and loads behind the previous
+ *     one and measured slower than the round-robin deal); 2 = always (no
+ *     threshold).  Results are identical either way.
+ *   SYMAMD_OPT_REGION_ROWS: region size cap (default 512).
+ *   SYMAMD_OPT_OWNERS (read at solve time): 1 (default) = plans with
+ *     regions run the owner schedule; 0 = they deal units like other plans
+ *     (for comparison; results are identical). */
 enum { SYMAMD_OPT_TICKETS = 1, SYMAMD_OPT_TRACE_PTR = 2, SYMAMD_OPT_TRACE_CAP = 3,
-       SYMAMD_OPT_MAX_WAVES = 4, SYMAMD_OPT_COLMAJOR_DIRECT = 5, SYMAMD_OPT_MARKS = 6 };
+       SYMAMD_OPT_MAX_WAVES = 4, SYMAMD_OPT_COLMAJOR_DIRECT = 5, SYMAMD_OPT_MARKS = 6,
+       SYMAMD_OPT_REGIONS = 7, SYMAMD_OPT_REGION_ROWS = 8, SYMAMD_OPT_OWNERS = 9 };
 int symamd_set_option(symamd_handle h, int option, int64_t value);
 const char *symamd_last_error(void);
 /* Library version string. */
@@ -176,6 +191,10 @@ int symamd_plan_info(symamd_plan plan, int *n, int64_t *nnz, int *levels,
                      int *max_level_rows);
 /* Copy the plan's level sets to host arrays (level_ptr: levels+1, level_set: n). */
 int symamd_plan_levels(symamd_plan plan, int *h_level_ptr, int *h_level_set);
+/* Regions of the plan (0 when it has no region order) and the fraction of
+ * L's off-diagonal entries whose two rows share a region (computed whenever
+ * the clustering ran). */
+int symamd_plan_regions(symamd_plan plan, int *count, double *inside);
 int symamd_plan_destroy(symamd_plan plan);
 
 /* ----------------------------------------------------------------- solve */

@@ -78,6 +78,7 @@ def lib() -> ctypes.CDLL:
             "symamd_plan_info": [vp, POINTER(i32), POINTER(i64), POINTER(i32),
                                  POINTER(i32)],
             "symamd_plan_levels": [vp, vp, vp],
+            "symamd_plan_regions": [vp, POINTER(i32), POINTER(dbl)],
             "symamd_plan_destroy": [vp],
             "symamd_plan_set_lanes": [vp, i32],
             "symamd_solve": [vp, vp, vp, i32, i64, i32, i32],
@@ -163,6 +164,19 @@ class Handle:
         """Cap the wavefronts of persistent launches (0: fill the GPU)."""
         _check(lib().symamd_set_option(self._h, 4, int(waves)))
 
+    def set_regions(self, mode: int, max_rows: int | None = None):
+        """Region order for plans analysed with this handle: 0 off (default),
+        1 when at least half of L's entries fall inside regions, 2 always;
+        `max_rows` caps the region size (default 512)."""
+        _check(lib().symamd_set_option(self._h, 7, int(mode)))
+        if max_rows is not None:
+            _check(lib().symamd_set_option(self._h, 8, int(max_rows)))
+
+    def set_owners(self, on: bool):
+        """Run plans with regions on the owner schedule (True, default) or
+        deal their units like other plans (False)."""
+        _check(lib().symamd_set_option(self._h, 9, int(bool(on))))
+
     def set_trace(self, buf):
         """Record per-unit clock stamps of sync-free launches into `buf`
         (a uint64/int64 cuda tensor of 3 entries per unit), or stop (None)."""
@@ -241,6 +255,13 @@ class Plan:
         _check(lib().symamd_plan_levels(self._p, lp.ctypes.data_as(c_void_p),
                                         ls.ctypes.data_as(c_void_p)))
         return lp, ls[: self.n]
+
+    def regions(self):
+        """(number of regions, fraction of L's entries inside a region);
+        0 regions when the plan has no region order."""
+        cnt, inside = c_int(), c_double()
+        _check(lib().symamd_plan_regions(self._p, byref(cnt), byref(inside)))
+        return cnt.value, inside.value
 
     def set_lanes(self, lanes: int):
         _check(lib().symamd_plan_set_lanes(self._p, int(lanes)))

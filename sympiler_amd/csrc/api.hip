@@ -43,12 +43,14 @@ int strides(int n, int k, int64_t ld, int layout, int64_t *rs, int64_t *cs) {
 void free_plan_arrays(symamd_plan p) {
   void *ptrs[] = {p->d_lvl_ptr, p->d_perm, p->d_rp,   p->d_ci,   p->d_val,
                   p->d_fmap,    p->d_orp,  p->d_trp,  p->d_tci,  p->d_tval,
-                  p->d_tmap,    p->d_state, p->d_ipos, p->d_cpos, p->d_ws};
+                  p->d_tmap,    p->d_state, p->d_ipos, p->d_cpos, p->d_ws,
+                  p->d_lvl_set, p->d_preg};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   for (auto &kv : p->units) {
     UnitSet &u = kv.second;
-    void *up[] = {u.d_ub, u.d_dptr, u.d_dlist, u.d_uflags};
+    void *up[] = {u.d_ub,   u.d_dptr,  u.d_dlist,  u.d_uflags,
+                  u.d_wptr, u.d_wlist, u.d_xdptr, u.d_xdlist};
     for (void *q : up)
       if (q) (void)hipFree(q);
   }
@@ -132,6 +134,7 @@ int analyze_common(symamd_handle h, int n, const int *d_Lp, const int *d_Li,
   if ((rc = compute_levels(h, n, d_Lp, d_Li, p->d_state, p->d_lvl_ptr, p->d_perm,
                            &p->levels, &p->h_lvl_ptr)) != SYMAMD_OK)
     return cleanup(rc);
+  if ((rc = order_by_regions(h, p, d_Lp, d_Li)) != SYMAMD_OK) return cleanup(rc);
   if ((rc = build_plan_arrays(h, p, d_Lp, d_Li, d_Lx)) != SYMAMD_OK) return cleanup(rc);
   if ((rc = gather_values(h, p, d_Lx)) != SYMAMD_OK) return cleanup(rc);
   if (hipMemsetAsync(p->d_state, 0, sizeof(int) * ST_WORDS, h->stream) != hipSuccess ||
@@ -444,6 +447,18 @@ int symamd_set_option(symamd_handle h, int option, int64_t value) {
     case SYMAMD_OPT_COLMAJOR_DIRECT:
       h->colmajor_direct = value != 0;
       return SYMAMD_OK;
+    case SYMAMD_OPT_REGIONS:
+      if (value < 0 || value > 2) return fail(SYMAMD_ERR_INVALID_ARG, "regions: 0, 1 or 2");
+      h->regions = (int)value;
+      return SYMAMD_OK;
+    case SYMAMD_OPT_OWNERS:
+      h->owners = value != 0;
+      return SYMAMD_OK;
+    case SYMAMD_OPT_REGION_ROWS:
+      if (value < 2 || value > (1 << 20))
+        return fail(SYMAMD_ERR_INVALID_ARG, "region rows out of range");
+      h->region_rows = (int)value;
+      return SYMAMD_OK;
     case SYMAMD_OPT_MAX_WAVES:
       if (value < 0 || value > (1 << 24))
         return fail(SYMAMD_ERR_INVALID_ARG, "max waves out of range");
@@ -553,8 +568,15 @@ int symamd_plan_levels(symamd_plan plan, int *h_level_ptr, int *h_level_set) {
   if (h_level_ptr)
     std::copy(plan->h_lvl_ptr.begin(), plan->h_lvl_ptr.end(), h_level_ptr);
   if (h_level_set && plan->n > 0)
-    SYMAMD_HIP(hipMemcpy(h_level_set, plan->d_perm, sizeof(int) * plan->n,
-                         hipMemcpyDeviceToHost));
+    SYMAMD_HIP(hipMemcpy(h_level_set, plan->d_lvl_set ? plan->d_lvl_set : plan->d_perm,
+                         sizeof(int) * plan->n, hipMemcpyDeviceToHost));
+  return SYMAMD_OK;
+}
+
+int symamd_plan_regions(symamd_plan plan, int *count, double *inside) {
+  SYMAMD_TRY(check_plan(plan));
+  if (count) *count = plan->d_preg ? plan->nregions : 0;
+  if (inside) *inside = plan->region_intra;
   return SYMAMD_OK;
 }
 

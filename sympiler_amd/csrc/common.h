@@ -113,6 +113,16 @@ struct UnitSet {
   int *d_dlist = nullptr;   //   dlist[dptr[u] .. dptr[u+1])
   int *d_uflags = nullptr;  // ready tags, one per unit
   int64_t deps = 0;
+  // owner schedule (plans with regions): unit u belongs to wave
+  // owner(region of u) of `nw` co-resident waves; wave w solves
+  // wlist[wptr[w] .. wptr[w+1]) in that order and waits only on the
+  // dependencies owned by other waves, xdlist[xdptr[u] .. xdptr[u+1])
+  int nw = 0;
+  int *d_wptr = nullptr;
+  int *d_wlist = nullptr;
+  int *d_xdptr = nullptr;
+  int *d_xdlist = nullptr;
+  int64_t xdeps = 0;
 };
 
 struct Choice {
@@ -133,6 +143,9 @@ struct symamd_handle_s {
   int max_waves = 0;  // cap on the waves of a persistent launch (0: occupancy)
   int colmajor_direct = 0;  // solve column-major blocks in place, no workspace
   int unmarked = 0;         // sync-free tags wait for the value stores
+  int regions = 0;          // 0 off, 1 when enough edges are inside, 2 always
+  int region_rows = 512;    // region size cap
+  int owners = 1;           // solve plans with regions on the owner schedule
   uint64_t timeout_ticks() const { return (uint64_t)(timeout_ms * 1.0e5); }
 };
 
@@ -145,7 +158,14 @@ struct symamd_plan_s {
   int device = 0;
   std::vector<int> h_lvl_ptr;          // levels+1
   int *d_lvl_ptr = nullptr;            // levels+1
-  int *d_perm = nullptr;               // position -> row (level set)
+  int *d_perm = nullptr;               // position -> row (level order)
+  // with regions: positions of a level are ordered by (region rank, row);
+  // d_lvl_set keeps the canonical level set (ascending rows in a level) and
+  // d_preg the region rank of each position.  Both null without regions.
+  int *d_lvl_set = nullptr;
+  int *d_preg = nullptr;
+  int nregions = 0;
+  double region_intra = 0.0;           // fraction of entries inside regions
   int *d_rp = nullptr;                 // level-ordered row pointers, n+1
   int *d_ci = nullptr;                 // level-ordered column ids (row ids), nnz
   void *d_val = nullptr;               // level-ordered values, nnz
@@ -225,6 +245,12 @@ int launch_lbc(symamd_handle h, int n, const int *d_Lp, const int *d_Li,
                const double *d_Lx, double *d_x, const double *d_b, int level_no,
                const int *h_level_ptr, const int *d_par_ptr,
                const int *d_partition);
+// Regions (regions.cpp): clustering of the rows, the region-ordered level
+// permutation of a plan, and the owner schedule of a unit set for nw waves.
+int cluster_regions(int n, const int *Lp, const int *Li, int max_rows,
+                    std::vector<int> &region);
+int order_by_regions(symamd_handle h, symamd_plan p, const int *d_Lp, const int *d_Li);
+int plan_owners(symamd_handle h, symamd_plan p, UnitSet *us, int nw);
 int ilu_lower(symamd_handle h, int n, const int *d_Ap, const int *d_Ai,
               const double *d_Ax, const int *d_diags, int unit, int *d_Lp,
               int *d_Li, double *d_Lx, int *nnz_out);

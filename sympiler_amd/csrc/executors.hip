@@ -77,6 +77,10 @@ struct SolveArgs {
   const int *ub;             // unit bounds (positions)
   const int *dptr, *dlist;   // dependency units of each unit
   int *uflags;               // unit ready tags
+  // owner schedule (null: units dealt round-robin or by ticket): wave w
+  // solves wlist[wptr[w] .. wptr[w+1]); dptr/dlist then hold only the
+  // dependencies owned by other waves
+  const int *wptr, *wlist;
   int U;
   int ticketed;              // 1: units from the ticket counter
   int reverse;               // 1: units visited from last to first
@@ -431,12 +435,28 @@ __global__ __launch_bounds__(256) void k_syncfree(SolveArgs<T> a) {
   Slab<T> &sl = slabs[threadIdx.x >> 6];
   const int lane = threadIdx.x & 63;
   const int target = ld_dev(&a.state[ST_EPOCH]) + 1;
-  int cursor = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  int cursor = wave;
   const int stride = a.total_waves;
+  // owner schedule: this wave's own list, in order (reverse for the
+  // transposed solve)
+  int w0 = 0, w1 = 0;
+  if (a.wlist && wave < a.total_waves) {
+    w0 = a.wptr[wave];
+    w1 = a.wptr[wave + 1];
+  }
+  cursor = a.wlist ? w0 : cursor;
   for (;;) {
-    const int t = next_unit(a.state, a.ticketed, cursor, stride);
-    if (t >= a.U) break;
-    const int u = a.reverse ? a.U - 1 - t : t;
+    int u;
+    if (a.wlist) {
+      if (cursor >= w1) break;
+      const int t = cursor++;
+      u = a.wlist[a.reverse ? w1 - 1 - (t - w0) : t];
+    } else {
+      const int t = next_unit(a.state, a.ticketed, cursor, stride);
+      if (t >= a.U) break;
+      u = a.reverse ? a.U - 1 - t : t;
+    }
     trace_mark(a.trace, a.trace_cap, u, 0);
     Solved<T, NR> sv;
     solve_unit<T, G, KT, NR, SY, ADJ>(a, u, sl, target, sv);
@@ -549,13 +569,35 @@ bool capturing(hipStream_t s) {
   return st != hipStreamCaptureStatusNone;
 }
 
-// Launch a persistent kernel: cooperatively (round-robin units) when
-// possible, else as a plain launch taking units from the ticket counter.
+// Launch a persistent kernel.  Plans with a region order use the owner
+// schedule on every co-resident wave (cooperative launch); otherwise units
+// are dealt round-robin (cooperative) or, inside graph capture or when
+// forced, taken from the ticket counter with the full dependency lists.
 template <typename T>
-int launch_persistent(symamd_handle h, const void *kfn, SolveArgs<T> &a) {
+int launch_persistent(symamd_handle h, symamd_plan p, UnitSet *us, const void *kfn,
+                      SolveArgs<T> &a) {
+  a.wptr = nullptr;
+  a.wlist = nullptr;
+  const bool coop = !capturing(h->stream) && !h->force_tickets;
+  if (coop && p->d_preg && h->owners) {
+    const int grid = resident_grid(h, kfn, 1 << 30);
+    SolveArgs<T> b = a;
+    b.total_waves = grid * 4;
+    b.ticketed = 0;
+    SYMAMD_TRY(plan_owners(h, p, us, b.total_waves));
+    b.wptr = us->d_wptr;
+    b.wlist = us->d_wlist;
+    b.dptr = us->d_xdptr;
+    b.dlist = us->d_xdlist;
+    void *args[] = {&b};
+    if (hipLaunchCooperativeKernel(kfn, dim3(grid), dim3(256), args, 0, h->stream) ==
+        hipSuccess)
+      return SYMAMD_OK;
+    (void)hipGetLastError();
+  }
   const int grid = resident_grid(h, kfn, a.U);
   a.total_waves = grid * 4;
-  if (!capturing(h->stream) && !h->force_tickets) {
+  if (coop) {
     a.ticketed = 0;
     void *args[] = {&a};
     hipError_t e = hipLaunchCooperativeKernel(kfn, dim3(grid), dim3(256), args, 0,
@@ -588,9 +630,9 @@ int run(symamd_handle h, symamd_plan p, SolveArgs<T> a, int exec) {
     if constexpr (KT == 1)
       if (a.k == 1 && !h->unmarked)
         return launch_persistent(
-            h, (const void *)&k_syncfree<T, G, KT, NR, ADJ, SY_MARKS>, a);
-    return launch_persistent(h, (const void *)&k_syncfree<T, G, KT, NR, ADJ, SY_FLAGS>,
-                             a);
+            h, p, us, (const void *)&k_syncfree<T, G, KT, NR, ADJ, SY_MARKS>, a);
+    return launch_persistent(h, p, us,
+                             (const void *)&k_syncfree<T, G, KT, NR, ADJ, SY_FLAGS>, a);
   }
   constexpr int LPR = G * KT;
   for (int l0 = 0; l0 < p->levels; ++l0) {

@@ -231,13 +231,17 @@ __global__ void k_collen(int n, const int *__restrict__ perm,
 // longer than emax forms a unit of its own).  One thread per level scans
 // its rows twice: first to count the units (upl), then, after the scan of
 // the counts into uoff, to write the unit starts (ub).
-__device__ __forceinline__ int unit_scan(const int *__restrict__ rp, int q0,
+// With a region order (preg != null) a unit also ends where the region of
+// the positions changes, so every unit lies inside one region.
+__device__ __forceinline__ int unit_scan(const int *__restrict__ rp,
+                                         const int *__restrict__ preg, int q0,
                                          int q1, int rows, int emax,
                                          int *ub_out) {
   int cnt = 0, in_rows = 0, ents = 0;
   for (int q = q0; q < q1; ++q) {
     const int len = rp[q + 1] - rp[q];
-    if (in_rows == rows || (in_rows > 0 && ents + len > emax)) {
+    if (in_rows == rows || (in_rows > 0 && ents + len > emax) ||
+        (in_rows > 0 && preg && preg[q] != preg[q - 1])) {
       in_rows = 0;
       ents = 0;
     }
@@ -252,19 +256,21 @@ __device__ __forceinline__ int unit_scan(const int *__restrict__ rp, int q0,
 }
 
 __global__ void k_units_per_level(int nlev, const int *__restrict__ lp,
-                                  const int *__restrict__ rp, int rows,
+                                  const int *__restrict__ rp,
+                                  const int *__restrict__ preg, int rows,
                                   int emax, int *upl) {
   int l = blockIdx.x * blockDim.x + threadIdx.x;
-  if (l < nlev) upl[l] = unit_scan(rp, lp[l], lp[l + 1], rows, emax, nullptr);
+  if (l < nlev) upl[l] = unit_scan(rp, preg, lp[l], lp[l + 1], rows, emax, nullptr);
   if (l == nlev) upl[l] = 0;
 }
 
 __global__ void k_unit_bounds(int nlev, int U, int n, const int *__restrict__ lp,
                               const int *__restrict__ rp,
+                              const int *__restrict__ preg,
                               const int *__restrict__ uoff, int rows, int emax,
                               int *ub) {
   int l = blockIdx.x * blockDim.x + threadIdx.x;
-  if (l < nlev) unit_scan(rp, lp[l], lp[l + 1], rows, emax, ub + uoff[l]);
+  if (l < nlev) unit_scan(rp, preg, lp[l], lp[l + 1], rows, emax, ub + uoff[l]);
   if (l == nlev) ub[U] = n;
 }
 
@@ -651,7 +657,7 @@ int plan_units(symamd_handle h, symamd_plan p, int rows, bool adjoint,
     SYMAMD_HIP(hipMallocAsync(&d_upl, sizeof(int) * (nlev + 1), h->stream));
     SYMAMD_HIP(hipMallocAsync(&d_uoff, sizeof(int) * (nlev + 1), h->stream));
     k_units_per_level<<<blocks_for(nlev + 1), 256, 0, h->stream>>>(
-        nlev, p->d_lvl_ptr, rp, rows, UNIT_EMAX, d_upl);
+        nlev, p->d_lvl_ptr, rp, p->d_preg, rows, UNIT_EMAX, d_upl);
     SYMAMD_HIP(hipGetLastError());
     SYMAMD_TRY(scan_exclusive(h, d_upl, d_uoff, nlev + 1));
     SYMAMD_HIP(hipMemcpyAsync(us.h_uoff.data(), d_uoff, sizeof(int) * (nlev + 1),
@@ -661,7 +667,7 @@ int plan_units(symamd_handle h, symamd_plan p, int rows, bool adjoint,
     us.count = U;
     SYMAMD_HIP(hipMalloc(&us.d_ub, sizeof(int) * (U + 1)));
     k_unit_bounds<<<blocks_for(nlev + 1), 256, 0, h->stream>>>(
-        nlev, U, p->n, p->d_lvl_ptr, rp, d_uoff, rows, UNIT_EMAX, us.d_ub);
+        nlev, U, p->n, p->d_lvl_ptr, rp, p->d_preg, d_uoff, rows, UNIT_EMAX, us.d_ub);
     SYMAMD_HIP(hipGetLastError());
     SYMAMD_HIP(hipFreeAsync(d_upl, h->stream));
     SYMAMD_HIP(hipFreeAsync(d_uoff, h->stream));

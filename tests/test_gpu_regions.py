@@ -1,0 +1,136 @@
+"""Region order and owner schedule of the sync-free executor: solves stay
+bit-identical to the oracle whichever way units are ordered and dealt
+(owner schedule, round-robin, tickets, graph replay), and the plan's level
+sets stay the canonical ones (ascending rows inside a level)."""
+import numpy as np
+import pytest
+
+import oracle
+import sympiler_amd as sa
+from sympiler_amd import workloads as W
+from tests.helpers import RTOL64, bitwise_equal, rel_err, to_dev
+
+pytestmark = pytest.mark.gpu
+
+
+def _plan(Lp, Li, Lx, dev, mode=1, rows=None, handle=None):
+    h = handle or sa.Handle()
+    h.set_regions(mode, rows)
+    return sa.Plan(*to_dev(Lp, Li, Lx, device=dev), handle=h), h
+
+
+def test_grid_regions_are_cubes(dev):
+    # 32^3 Laplacian pattern, regions of at most 512 rows: 8x8x8 blocks
+    Lp, Li, Lx = W.ic0_lap3d(32, 0.01)
+    plan, _ = _plan(Lp, Li, Lx, dev)
+    count, inside = plan.regions()
+    assert count == (32 // 8) ** 3
+    # per axis 31 * 32^2 edges, of which the 3 * 32^2 at x (or y, z) = 8, 16,
+    # 24 cross a region boundary
+    assert abs(inside - (1.0 - 3.0 / 31.0)) < 1e-12
+
+
+def test_level_sets_stay_canonical(dev):
+    Lp, Li, Lx = W.ic0_lap3d(16, 0.01)
+    plan, _ = _plan(Lp, Li, Lx, dev, mode=2, rows=64)
+    assert plan.regions()[0] > 0
+    lp, ls = plan.level_sets()
+    lp_o, ls_o, _ = oracle.levelset_csr(Lp, Li)
+    assert np.array_equal(lp, lp_o) and np.array_equal(ls, ls_o)
+
+
+@pytest.mark.parametrize("owners", [True, False])
+def test_c2_single_rhs_owner_schedule(dev, owners):
+    w = W.config2()
+    x_ref = oracle.sptrsv_csr(w.Lp, w.Li, w.Lx, w.b)
+    plan, h = _plan(w.Lp, w.Li, w.Lx, dev)
+    assert plan.regions()[0] == 4096
+    h.set_owners(owners)
+    for _ in range(2):  # the second solve runs on the next epoch of the tags
+        x = to_dev(w.b, device=dev)
+        plan.solve(x, executor="syncfree")
+        plan.check()
+        assert bitwise_equal(x.cpu().numpy(), x_ref)
+
+
+def test_c2_multi_rhs_and_adjoint(dev):
+    Lp, Li, Lx = W.ic0_lap3d(64, 0.01)
+    n = len(Lp) - 1
+    plan, h = _plan(Lp, Li, Lx, dev)
+    assert plan.regions()[0] == 512
+    k = 32
+    B = W.uniform(5, 0, n * k).reshape(k, n)
+    X_ref = oracle.sptrsm_csr(Lp, Li, Lx, B)
+    for layout in ("row", "col"):
+        dB = to_dev(np.ascontiguousarray(B.T) if layout == "row" else B, device=dev)
+        plan.solve(dB, layout=layout, executor="syncfree")
+        plan.check()
+        X = dB.cpu().numpy()
+        assert bitwise_equal(X.T if layout == "row" else X, X_ref), layout
+    import torch
+    b = W.uniform(6, 0, n)
+    x = oracle.sptrsv_csr(Lp, Li, Lx, b)
+    gb_ref, gL_ref = oracle.adjoint(Lp, Li, Lx, x, x)
+    plan.set_lanes(1)
+    dx = to_dev(x, device=dev)
+    gL = torch.empty(len(Lx), dtype=torch.float64, device=dev)
+    gb, _ = plan.adjoint(dx, dx.clone(), gLx=gL, executor="syncfree")
+    plan.check()
+    assert bitwise_equal(gb.cpu().numpy(), gb_ref)
+    assert bitwise_equal(gL.cpu().numpy(), gL_ref)
+
+
+@pytest.mark.parametrize("deal", ["owners", "round_robin", "tickets"])
+def test_forced_regions_on_irregular_matrix(dev, deal):
+    # a random lower triangle has few edges inside regions: forced regions
+    # give many cross-wave dependencies, the hardest case for the schedule
+    Lp, Li, Lx = W.random_lower(20000, 0.0005, 1, seed=31)
+    n = len(Lp) - 1
+    plan, h = _plan(Lp, Li, Lx, dev, mode=2, rows=64)
+    assert plan.regions()[0] > 0
+    h.set_owners(deal == "owners")
+    h.set_tickets(deal == "tickets")
+    b = W.uniform(31, 0, n)
+    x_ref = oracle.sptrsv_csr(Lp, Li, Lx, b)
+    for lanes in (1, 4):
+        plan.set_lanes(lanes)
+        x = to_dev(b, device=dev)
+        plan.solve(x, executor="syncfree")
+        plan.check()
+        if lanes == 1:
+            assert bitwise_equal(x.cpu().numpy(), x_ref)
+        else:
+            assert rel_err(x.cpu().numpy(), x_ref) <= RTOL64
+    plan.set_lanes(0)
+    B = W.uniform(32, 0, n * 8).reshape(8, n)
+    dB = to_dev(np.ascontiguousarray(B.T), device=dev)
+    plan.solve(dB, layout="row", executor="syncfree")
+    plan.check()
+    assert bitwise_equal(dB.cpu().numpy().T, oracle.sptrsm_csr(Lp, Li, Lx, B))
+
+
+def test_batched_grids_with_regions_graph_replay(dev):
+    w = W.config4(count=16, lo=32, hi=96, seed=5)
+    h = sa.Handle()
+    h.set_regions(1, 256)
+    plan = sa.Plan(*to_dev(w.Lp, w.Li, w.Lx, device=dev), row_offsets=w.row_offsets,
+                   handle=h)
+    assert plan.regions()[0] > 0
+    b = to_dev(w.b, device=dev)
+    offs = w.row_offsets
+    ref = np.concatenate([
+        oracle.sptrsv_csr(w.Lp[offs[s]:offs[s + 1] + 1] - w.Lp[offs[s]],
+                          w.Li[w.Lp[offs[s]]:w.Lp[offs[s + 1]]],
+                          w.Lx[w.Lp[offs[s]]:w.Lp[offs[s + 1]]], w.b[offs[s]:offs[s + 1]])
+        for s in range(len(offs) - 1)])
+    x = b.clone()
+    plan.solve(x, executor="syncfree")
+    plan.check()
+    assert bitwise_equal(x.cpu().numpy(), ref)
+    X = b.clone()
+    g = plan.graph(X)
+    for _ in range(3):  # replays inside capture use the ticket deal
+        X.copy_(b)
+        g.launch()
+        plan.check()
+        assert bitwise_equal(X.cpu().numpy(), ref)
