@@ -276,7 +276,8 @@ def main():
         singles = {}
         for name, ex, lanes in (("levelset_plain", "levelset", 1),
                                 ("levelset_graph", "graph", 1),
-                                ("syncfree", "syncfree", 1)):
+                                ("syncfree", "syncfree", 1),
+                                ("regions", "regions", 1)):
             plan.set_lanes(lanes)
             w1, e1 = timed_steps(lambda: plan.solve(x1, executor=ex), reset1, 10, 2,
                                  stream, torch)
@@ -316,6 +317,18 @@ def main():
                              "gbs": round(ab / (np.mean(ea) * 1e-3) / 1e9, 1),
                              "vs_plain_levelset": round(1.0 / float(np.mean(wa)) / plain, 2)}
         log(f"adjoint: {detail['adjoint']}")
+
+        # the headline block on the region executor
+        wr, er = timed_steps(lambda: plan.solve(X, layout=args.layout, executor="regions"),
+                             reset, 10, 2, stream, torch)
+        plan.check()
+        res_r = plan.residual(X, B0, layout=args.layout)
+        detail["regions_k"] = {"ms": round(1e3 * float(np.mean(wr)), 4),
+                               "kernel_ms": round(float(np.mean(er)), 4),
+                               "rhs_solves_per_s": round(k / float(np.mean(wr)), 1),
+                               "gbs": round(bytes_alg / (np.mean(er) * 1e-3) / 1e9, 1),
+                               "max_residual": float(res_r.max())}
+        log(f"regions k={k}: {detail['regions_k']}")
 
         if args.layout == "row":
             Bc = torch.from_numpy(Bcols).to(dev)

@@ -50,7 +50,11 @@ typedef enum {
   SYMAMD_EXEC_AUTO = 0,      /* tuned choice if symamd_tune ran, else heuristic */
   SYMAMD_EXEC_LEVELSET = 1,  /* one kernel launch per level (plain level set)   */
   SYMAMD_EXEC_SYNCFREE = 2,  /* one persistent launch, per-unit ready flags     */
-  SYMAMD_EXEC_LEVELSET_GRAPH = 3 /* the per-level launches replayed as a hipGraph */
+  SYMAMD_EXEC_LEVELSET_GRAPH = 3, /* the per-level launches replayed as a hipGraph */
+  SYMAMD_EXEC_REGIONS = 4    /* one persistent launch over compact regions of rows,
+                                each solved by one wavefront in LDS (forward
+                                solves, 1 <= k <= 64; other cases, and matrices
+                                whose region graph has a cycle, run SYNCFREE) */
 } symamd_executor;
 
 typedef struct symamd_handle_s *symamd_handle;

@@ -33,7 +33,7 @@ GEN_PATH = os.path.join(_HERE, "libsymamd_gen.so")
 
 F64, F32 = 0, 1
 COL_MAJOR, ROW_MAJOR = 0, 1
-EXEC = {"auto": 0, "levelset": 1, "syncfree": 2, "graph": 3}
+EXEC = {"auto": 0, "levelset": 1, "syncfree": 2, "graph": 3, "regions": 4}
 STATUS = {0: "OK", 1: "INVALID_ARG", 2: "STRUCTURE", 3: "ZERO_PIVOT", 4: "HIP",
           5: "ALLOC", 6: "TIMEOUT", 7: "NOT_SUPPORTED"}
 

@@ -54,6 +54,7 @@ void free_plan_arrays(symamd_plan p) {
     for (void *q : up)
       if (q) (void)hipFree(q);
   }
+  for (auto &kv : p->rplans) free_region_plan(kv.second);
   for (auto &kv : p->gcache) {
     if (kv.second.second) (void)hipGraphExecDestroy(kv.second.second);
     if (kv.second.first) (void)hipGraphDestroy(kv.second.first);
@@ -328,6 +329,7 @@ bool through_workspace(symamd_handle h, int k, int layout) {
 // Everything a solve allocates or builds, done before any launch so that a
 // solve can be captured into a graph.
 int prepare_solve(symamd_handle h, symamd_plan p, int k, int layout, const Choice &c) {
+  if (c.exec == SYMAMD_EXEC_REGIONS && region_usable(h, p, k)) return SYMAMD_OK;
   UnitSet *us = nullptr;
   SYMAMD_TRY(plan_units(h, p, forward_rows_per_unit(k, c.lanes), false, &us));
   if (through_workspace(h, k, layout)) SYMAMD_TRY(prepare_workspace(h, p, k));
@@ -337,6 +339,12 @@ int prepare_solve(symamd_handle h, symamd_plan p, int k, int layout, const Choic
 int run_solve(symamd_handle h, symamd_plan p, void *d_B, int k, int64_t ld,
               int64_t rs, int64_t cs, int layout, int exec, int lanes) {
   if (exec == SYMAMD_EXEC_LEVELSET_GRAPH) exec = SYMAMD_EXEC_LEVELSET;
+  if (exec == SYMAMD_EXEC_REGIONS) {
+    // the region schedule reads every layout in place (built beforehand by
+    // prepare_solve, so nothing is built here while capturing)
+    if (region_usable(h, p, k)) return launch_regions(h, p, d_B, k, rs, cs);
+    exec = SYMAMD_EXEC_SYNCFREE;
+  }
   if (through_workspace(h, k, layout))
     return launch_forward_ws(h, p, d_B, k, ld, exec, lanes);
   return launch_forward(h, p, d_B, k, rs, cs, exec, lanes);
@@ -603,7 +611,7 @@ int symamd_solve(symamd_handle h, symamd_plan plan, void *d_B, int k, int64_t ld
   SYMAMD_TRY(check_plan(plan));
   if (k < 0 || (k > 0 && plan->n > 0 && !d_B))
     return fail(SYMAMD_ERR_INVALID_ARG, "symamd_solve: bad arguments");
-  if (exec < SYMAMD_EXEC_AUTO || exec > SYMAMD_EXEC_LEVELSET_GRAPH)
+  if (exec < SYMAMD_EXEC_AUTO || exec > SYMAMD_EXEC_REGIONS)
     return fail(SYMAMD_ERR_INVALID_ARG, "unknown executor %d", (int)exec);
   return solve_impl(h, plan, d_B, k, ld, layout, exec, true);
 }
@@ -660,8 +668,11 @@ int symamd_adjoint(symamd_handle h, symamd_plan plan, const void *d_x,
   SYMAMD_TRY(check_plan(plan));
   if (plan->n > 0 && (!d_x || !d_gx || !d_gb))
     return fail(SYMAMD_ERR_INVALID_ARG, "symamd_adjoint: bad arguments");
+  if (exec < SYMAMD_EXEC_AUTO || exec > SYMAMD_EXEC_REGIONS)
+    return fail(SYMAMD_ERR_INVALID_ARG, "unknown executor %d", (int)exec);
   Choice c = choose(plan, 1, SYMAMD_COL_MAJOR, exec);
   int e = c.exec == SYMAMD_EXEC_LEVELSET_GRAPH ? SYMAMD_EXEC_LEVELSET : c.exec;
+  if (e == SYMAMD_EXEC_REGIONS) e = SYMAMD_EXEC_SYNCFREE;  // forward solves only
   return launch_adjoint(h, plan, d_x, d_gx, d_gb, d_gLx, e, c.lanes);
 }
 

@@ -125,6 +125,48 @@ struct UnitSet {
   int64_t xdeps = 0;
 };
 
+// Region schedule: each region (a compact cluster of rows, at most `cap`
+// rows) is solved by one wavefront in LDS, its internal levels in order;
+// regions wait for the regions they read from (a DAG, ranked
+// topologically).  All arrays are region-major in rank order.
+struct RegionPlan {
+  int cap = 0;        // region size cap the clustering used
+  int R = 0;          // regions (0: no region schedule for this cap)
+  int smax = 0;       // max rows in a region
+  int hmax = 0;       // max halo rows (rows of other regions read)
+  int emax = 0;       // max off-diagonal entries in a region
+  int slmax = 0;      // max internal levels (slices) of a region
+  double inside = 0;  // fraction of off-diagonal entries inside regions
+  int *d_rptr = nullptr;    // R+1: rows of region r = rows[rptr[r] .. rptr[r+1])
+  int *d_rows = nullptr;    // n: row ids, each region's in internal level order
+  int *d_sptr = nullptr;    // R+1: slices of region r = slices[sptr[r] .. sptr[r+1]]
+  int *d_slices = nullptr;  // local row offsets of slice starts (+ end), per region
+  int *d_hptr = nullptr;    // R+1
+  int *d_halo = nullptr;    // halo row ids
+  int *d_dptr = nullptr;    // R+1: regions region r waits for
+  int *d_dlist = nullptr;
+  int *d_eptr = nullptr;    // n+1 over the region-major rows: their entries
+  int *d_ecode = nullptr;   // LDS slot of the entry's column: local row, or
+                            // smax + halo index
+  void *d_eval = nullptr;   // entry values (gathered from the plan's copy)
+  int *d_esrc = nullptr;    // position of each entry in the level-ordered copy
+  void *d_diag = nullptr;   // diagonal of each region-major row
+  int *d_dsrc = nullptr;    // its position in the level-ordered copy
+  int *d_rflags = nullptr;  // ready tags, one per region
+  int64_t entries = 0;
+};
+
+// Host copy of a region schedule (same arrays as RegionPlan, before upload).
+struct HostRegions {
+  int R = 0, smax = 0, hmax = 0, emax = 0, slmax = 0;
+  double inside = 0;
+  std::vector<int> rptr, rows, sptr, slices, hptr, halo, dptr, dlist, eptr, ecode, esrc,
+      dsrc;
+};
+bool build_region_schedule(int n, int64_t nnz, int levels, const int *lvl_ptr,
+                           const int *perm, const int *crp, const int *cci, int cap,
+                           HostRegions &out);
+
 struct Choice {
   int exec = SYMAMD_EXEC_SYNCFREE;
   int lanes = 1;  // lanes cooperating on one row's nonzeros
@@ -183,6 +225,7 @@ struct symamd_plan_s {
   // persistent-kernel state
   int *d_state = nullptr;              // ST_WORDS ints
   std::map<int, symamd::UnitSet> units;  // rows-per-unit -> units
+  std::map<int, symamd::RegionPlan> rplans;  // region cap -> region schedule
   // schedule choices
   int lanes_override = 0;
   std::map<int64_t, symamd::Choice> tuned;  // key(k, layout) -> choice
@@ -251,6 +294,22 @@ int cluster_regions(int n, const int *Lp, const int *Li, int max_rows,
                     std::vector<int> &region);
 int order_by_regions(symamd_handle h, symamd_plan p, const int *d_Lp, const int *d_Li);
 int plan_owners(symamd_handle h, symamd_plan p, UnitSet *us, int nw);
+// Region schedule of the plan for region cap `cap` (built on first use;
+// rp->R == 0 when the region graph has a cycle or there is nothing to gain).
+int plan_regions(symamd_handle h, symamd_plan p, int cap, RegionPlan **out);
+void free_region_plan(RegionPlan &rp);
+// Re-gathers the region schedules' values from the plan's level-ordered copy.
+int gather_region_values(symamd_handle h, symamd_plan p);
+// Region-schedule forward solve (region_exec.hip); SYMAMD_ERR_NOT_SUPPORTED
+// when the plan has no usable region schedule for k right-hand sides.
+int launch_regions(symamd_handle h, symamd_plan p, void *d_B, int k, int64_t rs,
+                   int64_t cs);
+// Region size cap used for k right-hand sides (the LDS of one wave holds
+// the region's and its halo's k values per row).
+int region_cap(int k);
+// True when a region schedule for k right-hand sides exists (building it on
+// first use) and its buffers fit one wave's LDS.
+bool region_usable(symamd_handle h, symamd_plan p, int k);
 int ilu_lower(symamd_handle h, int n, const int *d_Ap, const int *d_Ai,
               const double *d_Ax, const int *d_diags, int unit, int *d_Lp,
               int *d_Li, double *d_Lx, int *nnz_out);

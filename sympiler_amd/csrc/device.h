@@ -1,0 +1,63 @@
+// device.h -- wave-level helpers shared by the persistent solve kernels
+// (executors.hip: work units; region_exec.hip: regions).
+#pragma once
+
+#include "common.h"
+
+namespace symamd {
+
+// LDS writes of the wave are complete and visible to all its lanes.
+__device__ __forceinline__ void wave_lds_sync() {
+  asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Wait until every dependency u' of item u (dlist[dptr[u] .. dptr[u+1]))
+// carries `target` in flags[u'].  One lane per dependency, one coherent load
+// each; later rounds re-load only the pending ones after a short sleep.  A
+// wait longer than `timeout` ticks of the 100 MHz clock records an error in
+// state[ST_ERROR] and gives up, so every wave exits.  (Batching several tags
+// per lane into one round trip measured slower on the 128^3 IC(0) factor:
+// the extra address setup sits on the critical path of every unit.)
+__device__ __forceinline__ void wait_units(const int *__restrict__ dptr,
+                                           const int *__restrict__ dlist,
+                                           int u, const int *flags,
+                                           int target, int *state,
+                                           uint64_t timeout) {
+  const int lane = threadIdx.x & 63;
+  const int d0 = dptr[u], d1 = dptr[u + 1];
+  for (int base = d0; base < d1; base += 64) {
+    const int t = base + lane;
+    bool pending = t < d1;
+    const int *f = pending ? &flags[dlist[t]] : flags;
+    if (pending) pending = ld_dev(f) != target;
+    if (!__any(pending)) continue;
+    const uint64_t t0 = wall_ticks();
+    int rounds = 0;
+    while (__any(pending)) {
+      __builtin_amdgcn_s_sleep(1);
+      if (pending) pending = ld_dev(f) != target;
+      if ((++rounds & 63) == 0 && wall_ticks() - t0 > timeout) {
+        if (pending) atomicOr(&state[ST_ERROR], 1);
+        pending = false;
+      }
+    }
+  }
+  asm volatile("" ::: "memory");
+}
+
+// Next item of this wave: round-robin over a co-resident grid (cursor
+// advances by the number of waves) or the next ticket.
+__device__ __forceinline__ int next_unit(int *state, int ticketed, int &cursor,
+                                         int stride) {
+  if (ticketed) {
+    int u = 0;
+    if ((threadIdx.x & 63) == 0) u = atomicAdd(&state[ST_TICKET], 1);
+    return __shfl(u, 0);
+  }
+  const int u = cursor;
+  cursor += stride;
+  return u;
+}
+
+}  // namespace symamd

@@ -41,7 +41,7 @@
 //             Waits are bounded by a watchdog on the 100 MHz clock that
 //             records an error and lets the wave proceed, so every wave
 //             exits.
-#include "common.h"
+#include "device.h"
 
 namespace symamd {
 
@@ -96,42 +96,6 @@ __device__ __forceinline__ void trace_mark(uint64_t *trace, int64_t cap,
                                            int u, int slot) {
   if (trace && (threadIdx.x & 63) == 0 && 3 * (int64_t)u + 2 < cap)
     trace[3 * (int64_t)u + slot] = wall_ticks();
-}
-
-__device__ __forceinline__ void wave_lds_sync() {
-  asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-  __builtin_amdgcn_wave_barrier();
-}
-
-// Wait until every dependency unit of unit u carries `target`.  One lane
-// per dependency unit, one coherent load each.  (Batching several tags per
-// lane into one round trip measured slower on the 128^3 IC(0) factor: the
-// extra address setup sits on the critical path of every unit.)
-__device__ __forceinline__ void wait_units(const int *__restrict__ dptr,
-                                           const int *__restrict__ dlist,
-                                           int u, const int *uflags,
-                                           int target, int *state,
-                                           uint64_t timeout) {
-  const int lane = threadIdx.x & 63;
-  const int d0 = dptr[u], d1 = dptr[u + 1];
-  for (int base = d0; base < d1; base += 64) {
-    const int t = base + lane;
-    bool pending = t < d1;
-    const int *f = pending ? &uflags[dlist[t]] : uflags;
-    if (pending) pending = ld_dev(f) != target;
-    if (!__any(pending)) continue;
-    const uint64_t t0 = wall_ticks();
-    int rounds = 0;
-    while (__any(pending)) {
-      __builtin_amdgcn_s_sleep(1);
-      if (pending) pending = ld_dev(f) != target;
-      if ((++rounds & 63) == 0 && wall_ticks() - t0 > timeout) {
-        if (pending) atomicOr(&state[ST_ERROR], 1);
-        pending = false;
-      }
-    }
-  }
-  asm volatile("" ::: "memory");
 }
 
 template <typename T, int G, int KT>
@@ -414,19 +378,6 @@ __device__ __forceinline__ void solve_unit(const SolveArgs<T> &a, int u,
     unit_rows<T, G, KT, NR, SY, ADJ>(a, u, q0, nrows, sl, a.ci + e0, a.val + e0,
                                      target, sv);
   }
-}
-
-// Next unit of this wave: round-robin (co-resident grid) or ticket.
-__device__ __forceinline__ int next_unit(int *state, int ticketed, int &cursor,
-                                         int stride) {
-  if (ticketed) {
-    int u = 0;
-    if ((threadIdx.x & 63) == 0) u = atomicAdd(&state[ST_TICKET], 1);
-    return __shfl(u, 0);
-  }
-  const int u = cursor;
-  cursor += stride;
-  return u;
 }
 
 template <typename T, int G, int KT, int NR, bool ADJ, int SY>

@@ -492,6 +492,7 @@ int gather_values(symamd_handle h, symamd_plan p, const void *d_Lx) {
           m, p->d_tmap, (const float *)d_Lx, (float *)p->d_tval);
   }
   SYMAMD_HIP(hipGetLastError());
+  SYMAMD_TRY(gather_region_values(h, p));
   // zero pivots
   SYMAMD_HIP(hipMemsetAsync(p->d_state + ST_SCRATCH, 0, sizeof(int), h->stream));
   if (p->dtype == SYMAMD_F64)

@@ -269,4 +269,391 @@ int plan_owners(symamd_handle h, symamd_plan p, UnitSet *us, int nw) {
   return SYMAMD_OK;
 }
 
+void free_region_plan(RegionPlan &rp) {
+  void *ptrs[] = {rp.d_rptr, rp.d_rows,  rp.d_sptr, rp.d_slices, rp.d_hptr,
+                  rp.d_halo, rp.d_dptr,  rp.d_dlist, rp.d_eptr,  rp.d_ecode,
+                  rp.d_eval, rp.d_esrc,  rp.d_diag, rp.d_dsrc,   rp.d_rflags};
+  for (void *q : ptrs)
+    if (q) (void)hipFree(q);
+  rp = RegionPlan{};
+}
+
+namespace {
+
+template <typename V>
+int upload(V **dst, const std::vector<V> &src) {
+  const size_t count = src.empty() ? 1 : src.size();
+  SYMAMD_HIP(hipMalloc(dst, sizeof(V) * count));
+  if (!src.empty())
+    SYMAMD_HIP(hipMemcpy(*dst, src.data(), sizeof(V) * src.size(), hipMemcpyHostToDevice));
+  return SYMAMD_OK;
+}
+
+}  // namespace
+
+// Region schedule for region cap `cap`, from the plan's level-ordered copy:
+//  - regions: heavy-edge clusters of at most `cap` rows;
+//  - region graph: r waits for every region holding a column of one of its
+//    rows; ranked by longest path (then first level, then id), so every
+//    region waits only on regions of lower rank.  A cycle (possible for
+//    irregular clusters) leaves the schedule empty (R = 0);
+//  - inside a region the rows are ordered by (level, row) and cut into
+//    slices of equal level; every off-diagonal entry is coded with the LDS
+//    slot of its column: its local row, or smax + its index in the region's
+//    halo (the sorted distinct rows of other regions it reads).
+namespace {
+
+// Region graph: regions region r reads from (deduplicated, in row order).
+void region_graph(int n, const int *Lp, const int *Li, const std::vector<int> &reg, int R,
+                  std::vector<int> &gptr, std::vector<int> &glist) {
+  std::vector<int> rows(n);
+  std::iota(rows.begin(), rows.end(), 0);
+  const std::vector<int> by_reg = counting_sort(rows, R, [&](int i) { return reg[i]; });
+  std::vector<int> start(R + 1, 0), mark(R, -1);
+  for (int i = 0; i < n; ++i) start[reg[i] + 1]++;
+  for (int r = 0; r < R; ++r) start[r + 1] += start[r];
+  gptr.assign(R + 1, 0);
+  glist.clear();
+  for (int r = 0; r < R; ++r) {
+    for (int t = start[r]; t < start[r + 1]; ++t) {
+      const int i = by_reg[t];
+      for (int j = Lp[i]; j < Lp[i + 1] - 1; ++j) {
+        const int d = reg[Li[j]];
+        if (d != r && mark[d] != r) {
+          mark[d] = r;
+          glist.push_back(d);
+        }
+      }
+    }
+    gptr[r + 1] = (int)glist.size();
+  }
+}
+
+// Strongly connected components of the region graph (iterative Tarjan);
+// returns the number of components, comp[r] = component of region r.
+int region_sccs(int R, const std::vector<int> &gptr, const std::vector<int> &glist,
+                std::vector<int> &comp) {
+  std::vector<int> index(R, -1), low(R, 0), stack, cs_node, cs_edge;
+  std::vector<char> on(R, 0);
+  comp.assign(R, -1);
+  int next = 0, ncomp = 0;
+  for (int s = 0; s < R; ++s) {
+    if (index[s] >= 0) continue;
+    cs_node.push_back(s);
+    cs_edge.push_back(gptr[s]);
+    index[s] = low[s] = next++;
+    stack.push_back(s);
+    on[s] = 1;
+    while (!cs_node.empty()) {
+      const int v = cs_node.back();
+      int &e = cs_edge.back();
+      if (e < gptr[v + 1]) {
+        const int w = glist[e++];
+        if (index[w] < 0) {
+          index[w] = low[w] = next++;
+          stack.push_back(w);
+          on[w] = 1;
+          cs_node.push_back(w);
+          cs_edge.push_back(gptr[w]);
+        } else if (on[w]) {
+          low[v] = std::min(low[v], index[w]);
+        }
+        continue;
+      }
+      if (low[v] == index[v]) {
+        int w;
+        do {
+          w = stack.back();
+          stack.pop_back();
+          on[w] = 0;
+          comp[w] = ncomp;
+        } while (w != v);
+        ++ncomp;
+      }
+      cs_node.pop_back();
+      cs_edge.pop_back();
+      if (!cs_node.empty()) {
+        const int u = cs_node.back();
+        low[u] = std::min(low[u], low[v]);
+      }
+    }
+  }
+  return ncomp;
+}
+
+// Makes the region graph acyclic: the regions of every strongly connected
+// component are merged; a merged set larger than `cap` is cut into level
+// bands (whole levels while they fit, a single level larger than `cap` in
+// chunks).  Rows depend only on lower levels, so the bands of one component
+// depend only on earlier bands, and edges between different components all
+// point one way: the refined graph is acyclic.  Returns the region count.
+int make_acyclic(int n, const int *Lp, const int *Li, const std::vector<int> &lev,
+                 int cap, std::vector<int> &reg, int R) {
+  std::vector<int> gptr, glist, comp;
+  region_graph(n, Lp, Li, reg, R, gptr, glist);
+  const int C = region_sccs(R, gptr, glist, comp);
+  if (C == R) return R;
+  std::vector<int> csize(C, 0);
+  for (int r = 0; r < R; ++r) csize[comp[r]]++;  // regions per component
+  std::vector<int> rows(n);
+  std::iota(rows.begin(), rows.end(), 0);
+  int maxlev = 0;
+  for (int i = 0; i < n; ++i) maxlev = std::max(maxlev, lev[i]);
+  // rows of each component in (level, row) order
+  const std::vector<int> by_lev = counting_sort(rows, maxlev + 1, [&](int i) { return lev[i]; });
+  const std::vector<int> by_comp =
+      counting_sort(by_lev, C, [&](int i) { return comp[reg[i]]; });
+  std::vector<int> newreg(n);
+  std::vector<int> single(R, -1);  // new id of a region kept as it is
+  int m = 0;
+  size_t t = 0;
+  while (t < by_comp.size()) {
+    const int c = comp[reg[by_comp[t]]];
+    size_t e = t;
+    while (e < by_comp.size() && comp[reg[by_comp[e]]] == c) ++e;
+    if (csize[c] == 1) {
+      const int r = reg[by_comp[t]];
+      if (single[r] < 0) single[r] = m++;
+      for (size_t k = t; k < e; ++k) newreg[by_comp[k]] = single[r];
+    } else {
+      int filled = 0;
+      size_t k = t;
+      while (k < e) {
+        size_t lend = k;  // rows of this level
+        while (lend < e && lev[by_comp[lend]] == lev[by_comp[k]]) ++lend;
+        const int cnt = (int)(lend - k);
+        if (filled > 0 && filled + cnt > cap) {
+          ++m;
+          filled = 0;
+        }
+        for (size_t q = k; q < lend; ++q) {
+          if (filled == cap) {
+            ++m;
+            filled = 0;
+          }
+          newreg[by_comp[q]] = m;
+          ++filled;
+        }
+        k = lend;
+      }
+      ++m;
+    }
+    t = e;
+  }
+  reg.swap(newreg);
+  return m;
+}
+
+}  // namespace
+
+// Host part, from the level-ordered copy (perm, crp, cci: position -> row,
+// row pointers and column ids of the copy; lvl_ptr: levels + 1 bounds).
+// Returns false when the region graph has a cycle (not expected after
+// make_acyclic; kept as a guard).
+bool build_region_schedule(int n, int64_t nnz, int levels, const int *lvl_ptr,
+                           const int *perm, const int *crp_, const int *cci_, int cap,
+                           HostRegions &o) {
+  const std::vector<int> crp(crp_, crp_ + n + 1), cci(cci_, cci_ + nnz);
+  std::vector<int> ipos(n), lev(n), Lp(n + 1, 0), Li(nnz);
+  for (int q = 0; q < n; ++q) ipos[perm[q]] = q;
+  for (int l = 0; l < levels; ++l)
+    for (int q = lvl_ptr[l]; q < lvl_ptr[l + 1]; ++q) lev[perm[q]] = l;
+  for (int i = 0; i < n; ++i) Lp[i + 1] = Lp[i] + (crp[ipos[i] + 1] - crp[ipos[i]]);
+  for (int i = 0; i < n; ++i)
+    std::copy(cci.begin() + crp[ipos[i]], cci.begin() + crp[ipos[i] + 1], Li.begin() + Lp[i]);
+  std::vector<int> reg;
+  int R = cluster_regions(n, Lp.data(), Li.data(), cap, reg);
+  R = make_acyclic(n, Lp.data(), Li.data(), lev, cap, reg, R);
+  int64_t inside = 0, offd = 0;
+  for (int i = 0; i < n; ++i)
+    for (int j = Lp[i]; j < Lp[i + 1] - 1; ++j) {
+      ++offd;
+      inside += reg[Li[j]] == reg[i];
+    }
+  o.inside = offd ? (double)inside / (double)offd : 0.0;
+  // region graph (deduplicated) and its longest-path depth
+  std::vector<int> rows_nat(n);
+  std::iota(rows_nat.begin(), rows_nat.end(), 0);
+  const std::vector<int> by_reg = counting_sort(rows_nat, R, [&](int i) { return reg[i]; });
+  std::vector<int> rstart(R + 1, 0);
+  for (int i = 0; i < n; ++i) rstart[reg[i] + 1]++;
+  for (int r = 0; r < R; ++r) rstart[r + 1] += rstart[r];
+  std::vector<int> gptr(R + 1, 0), glist, mark(R, -1);
+  for (int r = 0; r < R; ++r) {
+    for (int t = rstart[r]; t < rstart[r + 1]; ++t) {
+      const int i = by_reg[t];
+      for (int j = Lp[i]; j < Lp[i + 1] - 1; ++j) {
+        const int d = reg[Li[j]];
+        if (d != r && mark[d] != r) {
+          mark[d] = r;
+          glist.push_back(d);
+        }
+      }
+    }
+    gptr[r + 1] = (int)glist.size();
+  }
+  std::vector<int> indeg(R, 0), succ_ptr(R + 1, 0), succ(glist.size());
+  for (int r = 0; r < R; ++r) indeg[r] = gptr[r + 1] - gptr[r];
+  for (int d : glist) succ_ptr[d + 1]++;
+  for (int r = 0; r < R; ++r) succ_ptr[r + 1] += succ_ptr[r];
+  {
+    std::vector<int> fill(succ_ptr.begin(), succ_ptr.end() - 1);
+    for (int r = 0; r < R; ++r)
+      for (int e = gptr[r]; e < gptr[r + 1]; ++e) succ[fill[glist[e]]++] = r;
+  }
+  std::vector<int> depth(R, 0), queue;
+  queue.reserve(R);
+  for (int r = 0; r < R; ++r)
+    if (indeg[r] == 0) queue.push_back(r);
+  for (size_t qh = 0; qh < queue.size(); ++qh) {
+    const int r = queue[qh];
+    for (int e = succ_ptr[r]; e < succ_ptr[r + 1]; ++e) {
+      const int s = succ[e];
+      depth[s] = std::max(depth[s], depth[r] + 1);
+      if (--indeg[s] == 0) queue.push_back(s);
+    }
+  }
+  if ((int)queue.size() != R) return false;  // cyclic region graph
+  std::vector<int> first(R, levels);
+  for (int i = 0; i < n; ++i) first[reg[i]] = std::min(first[reg[i]], lev[i]);
+  std::vector<int> ids(R);
+  std::iota(ids.begin(), ids.end(), 0);
+  int maxdepth = 0;
+  for (int r = 0; r < R; ++r) maxdepth = std::max(maxdepth, depth[r]);
+  const std::vector<int> o1 = counting_sort(ids, levels, [&](int r) { return first[r]; });
+  const std::vector<int> order = counting_sort(o1, maxdepth + 1, [&](int r) { return depth[r]; });
+  std::vector<int> rank(R);
+  for (int k = 0; k < R; ++k) rank[order[k]] = k;
+  // rows in (rank, level, row) order
+  const std::vector<int> by_lev = counting_sort(rows_nat, levels, [&](int i) { return lev[i]; });
+  const std::vector<int> rows = counting_sort(by_lev, R, [&](int i) { return rank[reg[i]]; });
+  std::vector<int> rptr(R + 1, 0), lidx(n);
+  for (int i = 0; i < n; ++i) rptr[rank[reg[i]] + 1]++;
+  for (int r = 0; r < R; ++r) rptr[r + 1] += rptr[r];
+  int smax = 0;
+  for (int r = 0; r < R; ++r) smax = std::max(smax, rptr[r + 1] - rptr[r]);
+  for (int r = 0; r < R; ++r)
+    for (int t = rptr[r]; t < rptr[r + 1]; ++t) lidx[rows[t]] = t - rptr[r];
+  // slices, halos, entries, dependency lists (in ranks)
+  std::vector<int> sptr(R + 1, 0), slices, hptr(R + 1, 0), halo, dptr(R + 1, 0), dlist;
+  std::vector<int> eptr(n + 1, 0), ecode, esrc, dsrc(n);
+  std::vector<int> hslot(n, -1), hstamp(n, -1);
+  ecode.reserve(offd);
+  esrc.reserve(offd);
+  int hmax = 0, emax = 0, slmax = 0;
+  for (int r = 0; r < R; ++r) {
+    const int g = order[r];  // cluster id of rank r
+    const int a = rptr[r], b = rptr[r + 1];
+    for (int t = a; t < b; ++t)
+      if (t == a || lev[rows[t]] != lev[rows[t - 1]]) slices.push_back(t - a);
+    slices.push_back(b - a);
+    sptr[r + 1] = (int)slices.size();
+    slmax = std::max(slmax, sptr[r + 1] - sptr[r] - 1);
+    const size_t h0 = halo.size();
+    for (int t = a; t < b; ++t) {
+      const int i = rows[t];
+      for (int j = Lp[i]; j < Lp[i + 1] - 1; ++j) {
+        const int c = Li[j];
+        if (reg[c] != g && hstamp[c] != r) {
+          hstamp[c] = r;
+          halo.push_back(c);
+        }
+      }
+    }
+    std::sort(halo.begin() + h0, halo.end());
+    for (size_t k = h0; k < halo.size(); ++k) hslot[halo[k]] = (int)(k - h0);
+    hptr[r + 1] = (int)halo.size();
+    hmax = std::max(hmax, hptr[r + 1] - hptr[r]);
+    const int e0 = (int)ecode.size();
+    for (int t = a; t < b; ++t) {
+      const int i = rows[t];
+      const int q = ipos[i];
+      for (int j = Lp[i]; j < Lp[i + 1] - 1; ++j) {
+        const int c = Li[j];
+        ecode.push_back(reg[c] == g ? lidx[c] : smax + hslot[c]);
+        esrc.push_back(crp[q] + (j - Lp[i]));
+      }
+      eptr[t + 1] = (int)ecode.size();
+      dsrc[t] = crp[q + 1] - 1;
+    }
+    emax = std::max(emax, (int)ecode.size() - e0);
+    for (int e = gptr[g]; e < gptr[g + 1]; ++e) dlist.push_back(rank[glist[e]]);
+    dptr[r + 1] = (int)dlist.size();
+  }
+  o.R = R;
+  o.smax = smax;
+  o.hmax = hmax;
+  o.emax = emax;
+  o.slmax = slmax;
+  o.rptr.swap(rptr);
+  o.rows = rows;
+  o.sptr.swap(sptr);
+  o.slices.swap(slices);
+  o.hptr.swap(hptr);
+  o.halo.swap(halo);
+  o.dptr.swap(dptr);
+  o.dlist.swap(dlist);
+  o.eptr.swap(eptr);
+  o.ecode.swap(ecode);
+  o.esrc.swap(esrc);
+  o.dsrc.swap(dsrc);
+  return true;
+}
+
+// Device part: the schedule for cap `cap` (built once per plan and cap).
+int plan_regions(symamd_handle h, symamd_plan p, int cap, RegionPlan **out) {
+  auto it = p->rplans.find(cap);
+  if (it != p->rplans.end()) {
+    *out = &it->second;
+    return SYMAMD_OK;
+  }
+  RegionPlan rp;
+  rp.cap = cap;
+  const int n = p->n;
+  HostRegions o;
+  if (n > 0 && p->levels >= 2) {
+    std::vector<int> perm(n), crp(n + 1), cci(p->nnz);
+    SYMAMD_HIP(hipStreamSynchronize(h->stream));
+    SYMAMD_HIP(hipMemcpy(perm.data(), p->d_perm, sizeof(int) * n, hipMemcpyDeviceToHost));
+    SYMAMD_HIP(hipMemcpy(crp.data(), p->d_rp, sizeof(int) * (n + 1), hipMemcpyDeviceToHost));
+    SYMAMD_HIP(hipMemcpy(cci.data(), p->d_ci, sizeof(int) * p->nnz, hipMemcpyDeviceToHost));
+    if (!build_region_schedule(n, p->nnz, p->levels, p->h_lvl_ptr.data(), perm.data(),
+                               crp.data(), cci.data(), cap, o))
+      o.R = 0;
+  }
+  rp.inside = o.inside;
+  if (o.R == 0) {
+    *out = &p->rplans.emplace(cap, rp).first->second;
+    return SYMAMD_OK;
+  }
+  rp.R = o.R;
+  rp.smax = o.smax;
+  rp.hmax = o.hmax;
+  rp.emax = o.emax;
+  rp.slmax = o.slmax;
+  rp.entries = (int64_t)o.ecode.size();
+  const size_t vs = p->dtype == SYMAMD_F64 ? 8 : 4;
+  int rc = SYMAMD_OK;
+  if ((rc = upload(&rp.d_rptr, o.rptr)) || (rc = upload(&rp.d_rows, o.rows)) ||
+      (rc = upload(&rp.d_sptr, o.sptr)) || (rc = upload(&rp.d_slices, o.slices)) ||
+      (rc = upload(&rp.d_hptr, o.hptr)) || (rc = upload(&rp.d_halo, o.halo)) ||
+      (rc = upload(&rp.d_dptr, o.dptr)) || (rc = upload(&rp.d_dlist, o.dlist)) ||
+      (rc = upload(&rp.d_eptr, o.eptr)) || (rc = upload(&rp.d_ecode, o.ecode)) ||
+      (rc = upload(&rp.d_esrc, o.esrc)) || (rc = upload(&rp.d_dsrc, o.dsrc))) {
+    free_region_plan(rp);
+    return rc;
+  }
+  if (hipMalloc(&rp.d_eval, vs * (o.ecode.empty() ? 1 : o.ecode.size())) != hipSuccess ||
+      hipMalloc(&rp.d_diag, vs * n) != hipSuccess ||
+      hipMalloc(&rp.d_rflags, sizeof(int) * rp.R) != hipSuccess ||
+      hipMemset(rp.d_rflags, 0, sizeof(int) * rp.R) != hipSuccess) {
+    free_region_plan(rp);
+    return fail(SYMAMD_ERR_ALLOC, "region schedule arrays");
+  }
+  RegionPlan *stored = &p->rplans.emplace(cap, rp).first->second;
+  SYMAMD_TRY(gather_region_values(h, p));
+  *out = stored;
+  return SYMAMD_OK;
+}
+
 }  // namespace symamd

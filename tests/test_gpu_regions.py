@@ -109,6 +109,107 @@ def test_forced_regions_on_irregular_matrix(dev, deal):
     assert bitwise_equal(dB.cpu().numpy().T, oracle.sptrsm_csr(Lp, Li, Lx, B))
 
 
+# ------------------------------------------------- region executor (LDS)
+_RH = []
+
+
+def _rh():
+    """Handle for the region-executor tests: a 5 s watchdog, so a wait that
+    never ends fails the test quickly instead of after the 20 s default."""
+    if not _RH:
+        _RH.append(sa.Handle(timeout_ms=5000.0))
+    return _RH[0]
+
+
+def _solve_regions(plan, B, dev, layout="row"):
+    dB = to_dev(B, device=dev)
+    plan.solve(dB, layout=layout, executor="regions")
+    plan.check()
+    return dB.cpu().numpy()
+
+
+def test_region_executor_c2(dev):
+    w = W.config2()
+    plan = sa.Plan(*to_dev(w.Lp, w.Li, w.Lx, device=dev), handle=_rh())
+    x_ref = oracle.sptrsv_csr(w.Lp, w.Li, w.Lx, w.b)
+    for _ in range(2):  # second solve: next epoch of the region tags
+        assert bitwise_equal(_solve_regions(plan, w.b, dev), x_ref)
+    n = len(w.Lp) - 1
+    k = 32
+    B = W.uniform(40, 0, n * k).reshape(k, n)
+    X_ref = oracle.sptrsm_csr(w.Lp, w.Li, w.Lx, B)
+    X = _solve_regions(plan, np.ascontiguousarray(B.T), dev, "row")
+    assert bitwise_equal(X.T, X_ref)
+
+
+@pytest.mark.parametrize("k,layout", [(1, "row"), (3, "row"), (8, "col"), (64, "row")])
+def test_region_executor_grids(dev, k, layout):
+    for Lp, Li, Lx in (W.ic0_lap3d(24, 0.01), W.lap2d_tril(150, 97)):
+        n = len(Lp) - 1
+        plan = sa.Plan(*to_dev(Lp, Li, Lx, device=dev), handle=_rh())
+        B = W.uniform(41 + k, 0, n * k).reshape(k, n)
+        X_ref = oracle.sptrsm_csr(Lp, Li, Lx, B)
+        X = _solve_regions(plan, np.ascontiguousarray(B.T) if layout == "row" else B, dev,
+                           layout)
+        assert bitwise_equal(X.T if layout == "row" else X, X_ref)
+
+
+def test_region_executor_irregular_falls_back(dev):
+    # few entries inside regions and large halos: the region schedule does
+    # not fit one wave's LDS and the solve runs on the sync-free executor
+    Lp, Li, Lx = W.random_lower(30000, 0.0008, 1, seed=44)
+    n = len(Lp) - 1
+    plan = sa.Plan(*to_dev(Lp, Li, Lx, device=dev), handle=_rh())
+    plan.set_lanes(1)  # the fallback then repeats the reference order exactly
+    b = W.uniform(44, 0, n)
+    assert bitwise_equal(_solve_regions(plan, b, dev), oracle.sptrsv_csr(Lp, Li, Lx, b))
+
+
+def test_region_executor_fp32_and_value_update(dev):
+    import torch
+    Lp, Li, Lx = W.ic0_lap3d(20, 0.01)
+    n = len(Lp) - 1
+    dLp, dLi, dLx = to_dev(Lp, Li, Lx, device=dev)
+    b = W.uniform(45, 0, n)
+    # FP32: the region kernel and the sync-free kernel (one lane per row)
+    # perform the same float operations in the same order
+    p32 = sa.Plan(dLp, dLi, dLx.to(torch.float32), handle=_rh())
+    p32.set_lanes(1)
+    xr = to_dev(b, device=dev).to(torch.float32)
+    p32.solve(xr, executor="regions")
+    xs = to_dev(b, device=dev).to(torch.float32)
+    p32.solve(xs, executor="syncfree")
+    p32.check()
+    assert torch.equal(xr.view(torch.int32), xs.view(torch.int32))
+    # new values on the same pattern reach the region schedule
+    plan = sa.Plan(dLp, dLi, dLx, handle=_rh())
+    _solve_regions(plan, b, dev)
+    Lx2 = Lx * 1.5
+    plan.update_values(to_dev(Lx2, device=dev))
+    assert bitwise_equal(_solve_regions(plan, b, dev), oracle.sptrsv_csr(Lp, Li, Lx2, b))
+
+
+def test_region_executor_batched_and_captured(dev):
+    w = W.config4(count=20, lo=32, hi=120, seed=6)
+    plan = sa.Plan(*to_dev(w.Lp, w.Li, w.Lx, device=dev), row_offsets=w.row_offsets,
+                   handle=_rh())
+    offs = w.row_offsets
+    ref = np.concatenate([
+        oracle.sptrsv_csr(w.Lp[offs[s]:offs[s + 1] + 1] - w.Lp[offs[s]],
+                          w.Li[w.Lp[offs[s]]:w.Lp[offs[s + 1]]],
+                          w.Lx[w.Lp[offs[s]]:w.Lp[offs[s + 1]]], w.b[offs[s]:offs[s + 1]])
+        for s in range(len(offs) - 1)])
+    assert bitwise_equal(_solve_regions(plan, w.b, dev), ref)
+    b = to_dev(w.b, device=dev)
+    X = b.clone()
+    g = plan.graph(X, executor="regions")
+    for _ in range(3):  # captured launches take regions by ticket
+        X.copy_(b)
+        g.launch()
+        plan.check()
+        assert bitwise_equal(X.cpu().numpy(), ref)
+
+
 def test_batched_grids_with_regions_graph_replay(dev):
     w = W.config4(count=16, lo=32, hi=96, seed=5)
     h = sa.Handle()
