@@ -32,6 +32,8 @@ sys.path.insert(0, ROOT)
 METRIC = ("SpTRSV RHS-solves/s + HBM GB/s, batched/adjoint vs plain GPU, "
           "rel err≤1e-12")
 HBM_PEAK_GBS = 8000.0
+# the kernel each executor launches once per solve
+KERNELS = {"syncfree": "k_syncfree", "regions": "k_regions"}
 
 
 def parse():
@@ -232,6 +234,13 @@ def main():
         f"max level rows={plan.max_level_rows}")
     if args.tune:
         log("tune k=%d: %s" % (k, plan.tune(k, args.layout).replace("\n", "; ")))
+    # the executor the automatic choice takes (building the region schedule
+    # when it is a candidate: timed apart from the analysis)
+    t0 = time.time()
+    chosen, _ = plan.choice(k, args.layout)
+    schedule_s = time.time() - t0
+    executor = chosen if args.executor == "auto" else args.executor
+    log(f"executor: {executor} (choice made in {schedule_s*1e3:.0f} ms)")
 
     if args.layout == "row":
         B0 = torch.from_numpy(np.ascontiguousarray(Bcols.T)).to(dev)  # (n, k)
@@ -260,6 +269,7 @@ def main():
         f"{value:.0f} RHS-solves/s, {achieved:.0f} GB/s, max residual {res.max():.2e}")
 
     detail = {"analyze_ms": round(analyze_s * 1e3, 2),
+              "schedule_ms": round(schedule_s * 1e3, 2),
               "headline_kernel_ms": round(ms_kernel, 4),
               "headline_gbs": round(achieved, 1),
               "headline_rel_err_vs_x_true": err_true,
@@ -365,14 +375,15 @@ def main():
                  f"B = L X_true, X_true ~ U[-1,1] (splitmix64 seed 1)"),
         "config": {"workload": f"ic0_lap3d_{g}^3 SpTRSM k={k} {args.layout}-major",
                    "n": n, "nnz": nnz, "levels": plan.levels, "k": k,
-                   "layout": args.layout, "executor": args.executor},
+                   "layout": args.layout, "executor": executor},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(f"ic0_lap3d_{g}^3", k, args.layout),
+                     "traffic": measured_traffic(f"ic0_lap3d_{g}^3", k, args.layout,
+                                                 KERNELS.get(executor, "")),
                      "traffic_source": "profiles/hbm_traffic.txt (rocprofv3 --pmc "
                                        "FETCH_SIZE, WRITE_SIZE; copy-calibrated)",
-                     "kernel": "k_syncfree (one persistent launch per solve)",
+                     "kernel": KERNELS.get(executor, executor) + " (one launch per solve)",
                      "algorithmic_bytes_per_launch": int(bytes_alg)},
         "cpu_baseline": cpu,
         "detail": detail,
@@ -380,18 +391,18 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def measured_traffic(workload, k, layout):
+def measured_traffic(workload, k, layout, kernel):
     """HBM bytes per launch of the headline kernel as measured by the
-    committed rocprofv3 counter passes (profiles/hbm_traffic.txt), or None
-    when this workload was not profiled."""
+    committed rocprofv3 counter passes (profiles/hbm_traffic.txt: workload,
+    k, layout, kernel, bytes), or None when it was not profiled."""
     path = os.path.join(ROOT, "profiles", "hbm_traffic.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
         f = line.split()
-        if len(f) == 4 and not line.startswith("#") and f[0] == workload \
-                and f[1] == str(k) and f[2] == layout:
-            return int(f[3])
+        if len(f) == 5 and not line.startswith("#") and f[0] == workload \
+                and f[1] == str(k) and f[2] == layout and f[3] == kernel:
+            return int(f[4])
     return None
 
 

@@ -103,10 +103,15 @@ int symamd_set_timeout_ms(symamd_handle h, double ms);
  *   SYMAMD_OPT_REGION_ROWS: region size cap (default 512).
  *   SYMAMD_OPT_OWNERS (read at solve time): 1 (default) = plans with
  *     regions run the owner schedule; 0 = they deal units like other plans
- *     (for comparison; results are identical). */
+ *     (for comparison; results are identical).
+ *   SYMAMD_OPT_REGION_CAP (read at solve time): region size cap of the
+ *     SYMAMD_EXEC_REGIONS schedule (0, the default, picks it from k: 512
+ *     rows for k <= 4, 256 for k >= 8); each cap's schedule is built once
+ *     per plan. */
 enum { SYMAMD_OPT_TICKETS = 1, SYMAMD_OPT_TRACE_PTR = 2, SYMAMD_OPT_TRACE_CAP = 3,
        SYMAMD_OPT_MAX_WAVES = 4, SYMAMD_OPT_COLMAJOR_DIRECT = 5, SYMAMD_OPT_MARKS = 6,
-       SYMAMD_OPT_REGIONS = 7, SYMAMD_OPT_REGION_ROWS = 8, SYMAMD_OPT_OWNERS = 9 };
+       SYMAMD_OPT_REGIONS = 7, SYMAMD_OPT_REGION_ROWS = 8, SYMAMD_OPT_OWNERS = 9,
+       SYMAMD_OPT_REGION_CAP = 10 };
 int symamd_set_option(symamd_handle h, int option, int64_t value);
 const char *symamd_last_error(void);
 /* Library version string. */
@@ -199,6 +204,14 @@ int symamd_plan_levels(symamd_plan plan, int *h_level_ptr, int *h_level_set);
  * L's off-diagonal entries whose two rows share a region (computed whenever
  * the clustering ran). */
 int symamd_plan_regions(symamd_plan plan, int *count, double *inside);
+/* The executor (and lanes per row) SYMAMD_EXEC_AUTO takes for a forward
+ * solve of k right-hand sides in `layout`: the tuned choice if symamd_tune
+ * ran, else SYMAMD_EXEC_REGIONS when the plan has a usable region schedule
+ * with at least 60 % of L's off-diagonal entries inside regions (k = 1 or a
+ * row-major block), else SYMAMD_EXEC_SYNCFREE.  May build the region
+ * schedule. */
+int symamd_choice(symamd_handle h, symamd_plan plan, int k, symamd_layout layout,
+                  int *exec, int *lanes);
 int symamd_plan_destroy(symamd_plan plan);
 
 /* ----------------------------------------------------------------- solve */

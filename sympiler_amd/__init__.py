@@ -79,6 +79,7 @@ def lib() -> ctypes.CDLL:
                                  POINTER(i32)],
             "symamd_plan_levels": [vp, vp, vp],
             "symamd_plan_regions": [vp, POINTER(i32), POINTER(dbl)],
+            "symamd_choice": [vp, vp, i32, i32, POINTER(i32), POINTER(i32)],
             "symamd_plan_destroy": [vp],
             "symamd_plan_set_lanes": [vp, i32],
             "symamd_solve": [vp, vp, vp, i32, i64, i32, i32],
@@ -172,6 +173,10 @@ class Handle:
         if max_rows is not None:
             _check(lib().symamd_set_option(self._h, 8, int(max_rows)))
 
+    def set_region_cap(self, rows: int):
+        """Region size cap of the region executor (0: chosen from k)."""
+        _check(lib().symamd_set_option(self._h, 10, int(rows)))
+
     def set_owners(self, on: bool):
         """Run plans with regions on the owner schedule (True, default) or
         deal their units like other plans (False)."""
@@ -255,6 +260,14 @@ class Plan:
         _check(lib().symamd_plan_levels(self._p, lp.ctypes.data_as(c_void_p),
                                         ls.ctypes.data_as(c_void_p)))
         return lp, ls[: self.n]
+
+    def choice(self, k=1, layout="col"):
+        """(executor name, lanes per row) the automatic choice takes for a
+        forward solve of k right-hand sides in `layout`."""
+        ex, lanes = c_int(), c_int()
+        lay = ROW_MAJOR if layout == "row" else COL_MAJOR
+        _check(lib().symamd_choice(self.h._h, self._p, int(k), lay, byref(ex), byref(lanes)))
+        return {v: n for n, v in EXEC.items()}[ex.value], lanes.value
 
     def regions(self):
         """(number of regions, fraction of L's entries inside a region);

@@ -73,12 +73,17 @@ Choice heuristic(symamd_plan p, int k) {
   return c;
 }
 
-Choice choose(symamd_plan p, int k, int layout, int exec) {
+// forward: a forward solve (the region executor is a candidate)
+Choice choose(symamd_handle h, symamd_plan p, int k, int layout, int exec, bool forward) {
   Choice c;
   auto it = p->tuned.find(tune_key(k, layout));
   if (exec == SYMAMD_EXEC_AUTO) {
-    if (it != p->tuned.end()) c = it->second;
-    else c = heuristic(p, k);
+    if (it != p->tuned.end()) {
+      c = it->second;
+    } else {
+      c = heuristic(p, k);
+      if (forward && region_preferred(h, p, k, layout)) c.exec = SYMAMD_EXEC_REGIONS;
+    }
   } else {
     c = heuristic(p, k);
     c.exec = exec;
@@ -385,7 +390,7 @@ int solve_impl(symamd_handle h, symamd_plan p, void *d_B, int k, int64_t ld,
   int64_t rs = 0, cs = 0;
   SYMAMD_TRY(strides(p->n, k, ld, layout, &rs, &cs));
   if (k == 0 || p->n == 0) return SYMAMD_OK;
-  Choice c = choose(p, k, layout, exec);
+  Choice c = choose(h, p, k, layout, exec, true);
   if (c.exec == SYMAMD_EXEC_LEVELSET_GRAPH && graph_ok) {
     std::string key = graph_key(d_B, k, rs, cs, c) + (through_workspace(h, k, layout) ? "/ws" : "");
     auto it = p->gcache.find(key);
@@ -461,6 +466,11 @@ int symamd_set_option(symamd_handle h, int option, int64_t value) {
       return SYMAMD_OK;
     case SYMAMD_OPT_OWNERS:
       h->owners = value != 0;
+      return SYMAMD_OK;
+    case SYMAMD_OPT_REGION_CAP:
+      if (value != 0 && (value < 2 || value > 4096))
+        return fail(SYMAMD_ERR_INVALID_ARG, "region cap: 0 or 2..4096");
+      h->region_cap = (int)value;
       return SYMAMD_OK;
     case SYMAMD_OPT_REGION_ROWS:
       if (value < 2 || value > (1 << 20))
@@ -581,6 +591,17 @@ int symamd_plan_levels(symamd_plan plan, int *h_level_ptr, int *h_level_set) {
   return SYMAMD_OK;
 }
 
+int symamd_choice(symamd_handle h, symamd_plan plan, int k, symamd_layout layout,
+                  int *exec, int *lanes) {
+  SYMAMD_TRY(check_handle(h));
+  SYMAMD_TRY(check_plan(plan));
+  if (k < 1) return fail(SYMAMD_ERR_INVALID_ARG, "k must be >= 1");
+  Choice c = choose(h, plan, k, layout, SYMAMD_EXEC_AUTO, true);
+  if (exec) *exec = c.exec;
+  if (lanes) *lanes = c.lanes;
+  return SYMAMD_OK;
+}
+
 int symamd_plan_regions(symamd_plan plan, int *count, double *inside) {
   SYMAMD_TRY(check_plan(plan));
   if (count) *count = plan->d_preg ? plan->nregions : 0;
@@ -634,7 +655,7 @@ int symamd_graph_create(symamd_handle h, symamd_plan plan, void *d_B, int k,
   if (!g || k < 1 || !d_B) return fail(SYMAMD_ERR_INVALID_ARG, "symamd_graph_create: bad arguments");
   int64_t rs = 0, cs = 0;
   SYMAMD_TRY(strides(plan->n, k, ld, layout, &rs, &cs));
-  Choice c = choose(plan, k, layout, exec);
+  Choice c = choose(h, plan, k, layout, exec, true);
   auto *x = new symamd_graph_s();
   x->plan = plan;
   int rc = capture_solve(h, plan, d_B, k, ld, rs, cs, layout, c, &x->graph, &x->exec);
@@ -670,7 +691,7 @@ int symamd_adjoint(symamd_handle h, symamd_plan plan, const void *d_x,
     return fail(SYMAMD_ERR_INVALID_ARG, "symamd_adjoint: bad arguments");
   if (exec < SYMAMD_EXEC_AUTO || exec > SYMAMD_EXEC_REGIONS)
     return fail(SYMAMD_ERR_INVALID_ARG, "unknown executor %d", (int)exec);
-  Choice c = choose(plan, 1, SYMAMD_COL_MAJOR, exec);
+  Choice c = choose(h, plan, 1, SYMAMD_COL_MAJOR, exec, false);
   int e = c.exec == SYMAMD_EXEC_LEVELSET_GRAPH ? SYMAMD_EXEC_LEVELSET : c.exec;
   if (e == SYMAMD_EXEC_REGIONS) e = SYMAMD_EXEC_SYNCFREE;  // forward solves only
   return launch_adjoint(h, plan, d_x, d_gx, d_gb, d_gLx, e, c.lanes);
@@ -728,10 +749,13 @@ int symamd_tune(symamd_handle h, symamd_plan plan, int k, symamd_layout layout,
   float best_ms = 1e30f;
   const int saved_override = plan->lanes_override;
   plan->lanes_override = 0;
-  const int execs[] = {SYMAMD_EXEC_SYNCFREE, SYMAMD_EXEC_LEVELSET_GRAPH};
+  const int execs[] = {SYMAMD_EXEC_SYNCFREE, SYMAMD_EXEC_LEVELSET_GRAPH, SYMAMD_EXEC_REGIONS};
   int rc = SYMAMD_OK;
   for (int ex : execs) {
-    for (int g = 1; g * kt <= 64 && g <= std::min(64, gmax_rows); g *= 2) {
+    // the region executor has no lanes-per-row choice and needs a schedule
+    if (ex == SYMAMD_EXEC_REGIONS && !region_usable(h, plan, k)) continue;
+    const int gmax = ex == SYMAMD_EXEC_REGIONS ? 1 : std::min(64, gmax_rows);
+    for (int g = 1; g * kt <= 64 && g <= gmax; g *= 2) {
       Choice c;
       c.exec = ex;
       c.lanes = g;

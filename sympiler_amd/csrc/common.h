@@ -130,6 +130,7 @@ struct UnitSet {
 // regions wait for the regions they read from (a DAG, ranked
 // topologically).  All arrays are region-major in rank order.
 struct RegionPlan {
+  static constexpr int MAX_GROUPS = 8;  // column groups per region (tags)
   int cap = 0;        // region size cap the clustering used
   int R = 0;          // regions (0: no region schedule for this cap)
   int smax = 0;       // max rows in a region
@@ -152,7 +153,7 @@ struct RegionPlan {
   int *d_esrc = nullptr;    // position of each entry in the level-ordered copy
   void *d_diag = nullptr;   // diagonal of each region-major row
   int *d_dsrc = nullptr;    // its position in the level-ordered copy
-  int *d_rflags = nullptr;  // ready tags, one per region
+  int *d_rflags = nullptr;  // task ready tags, MAX_GROUPS per region
   int64_t entries = 0;
 };
 
@@ -188,6 +189,7 @@ struct symamd_handle_s {
   int regions = 0;          // 0 off, 1 when enough edges are inside, 2 always
   int region_rows = 512;    // region size cap
   int owners = 1;           // solve plans with regions on the owner schedule
+  int region_cap = 0;       // region executor: region size cap (0: by k)
   uint64_t timeout_ticks() const { return (uint64_t)(timeout_ms * 1.0e5); }
 };
 
@@ -305,11 +307,16 @@ int gather_region_values(symamd_handle h, symamd_plan p);
 int launch_regions(symamd_handle h, symamd_plan p, void *d_B, int k, int64_t rs,
                    int64_t cs);
 // Region size cap used for k right-hand sides (the LDS of one wave holds
-// the region's and its halo's k values per row).
-int region_cap(int k);
+// the region's and its halo's values of one column group per row), unless
+// the handle sets one.
+int region_cap(symamd_handle h, int k);
 // True when a region schedule for k right-hand sides exists (building it on
 // first use) and its buffers fit one wave's LDS.
 bool region_usable(symamd_handle h, symamd_plan p, int k);
+// Whether the automatic choice takes the region executor: a usable region
+// schedule with at least 60 % of L's off-diagonal entries inside regions,
+// for one right-hand side or a row-major block.
+bool region_preferred(symamd_handle h, symamd_plan p, int k, int layout);
 int ilu_lower(symamd_handle h, int n, const int *d_Ap, const int *d_Ai,
               const double *d_Ax, const int *d_diags, int unit, int *d_Lp,
               int *d_Li, double *d_Lx, int *nnz_out);

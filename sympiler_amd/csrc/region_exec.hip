@@ -1,31 +1,41 @@
 // region_exec.hip -- region-schedule forward solve for gfx950.
 //
-// One wavefront solves one region (a compact cluster of at most `cap` rows,
-// built by regions.cpp) entirely in LDS:
+// Work = tasks.  A task is one region (a compact cluster of at most `cap`
+// rows, built by regions.cpp) times one group of at most KG_MAX right-hand
+// side columns; column groups are independent solves, so a task waits only
+// on the same column group of the regions it reads.  One wavefront solves
+// one task entirely in LDS:
 //   1. stage the region's entries (LDS slot code + value), diagonals, row
-//      ids, slice bounds and right-hand sides -- none of it depends on other
-//      regions, so it overlaps the wait;
-//   2. wait until every region it reads from carries the launch's tag;
+//      ids, slice bounds and the group's right-hand sides -- none of it
+//      depends on other tasks, so it overlaps the wait;
+//   2. wait until the tasks it reads from carry the launch's tag;
 //   3. load the halo (the values of those regions' rows it reads) into LDS;
 //   4. solve the internal levels (slices) one after another; the lanes of a
-//      slice take (row, column) pairs, column fastest, and read every
-//      dependency from LDS; slices are ordered by the wave's own LDS
-//      ordering, no cross-wave hand-off;
-//   5. store the rows, wait for the stores, publish the region's tag.
+//      slice take (row, column) pairs, column fastest, every dependency
+//      read from LDS; slices are ordered by the wave's own LDS ordering,
+//      with no cross-wave hand-off;
+//   5. store the rows, wait for the stores, publish the task's tag.
+// Every global -> LDS copy keeps STAGE_U loads per lane in flight, so a
+// task pays a few memory round trips instead of one per 64 values.
+//
 // Row arithmetic is the reference's (sparse_blas/sptrsv.cpp:17-25): the
 // right-hand side minus each product in the row's stored order, then the
 // division by the diagonal, one lane per (row, column), compiled without FMA
 // contraction -- bit-identical to sptrsv_csr.
 //
-// Regions are ranked topologically and taken by ticket, so the lowest
-// unfinished region is always held by a running wave with all its inputs
-// published: the grid cannot deadlock, whether or not all its blocks are
-// resident, and the watchdog of wait_units bounds every wait.
+// Tasks are ordered by region rank (topological) then column group and
+// taken by ticket, so the lowest unfinished task is always held by a
+// running wave with all its inputs published: the grid cannot deadlock,
+// whether or not all its blocks are resident (these blocks carry up to
+// 64 KiB of LDS each), and the watchdog of wait_units bounds every wait.
 #include "device.h"
 
 namespace symamd {
 
 namespace {
+
+constexpr int KG_MAX = 8;   // right-hand side columns per task
+constexpr int STAGE_U = 8;  // loads per lane in flight while staging
 
 template <typename T>
 struct RegionArgs {
@@ -35,10 +45,9 @@ struct RegionArgs {
   T *X;
   const int *rptr, *rows, *sptr, *slices, *hptr, *halo, *dptr, *dlist, *eptr, *ecode;
   const T *eval, *diag;
-  int *rflags;
+  int *tflags;     // task tags: region rank * groups + group
   int *state;
-  int R, smax, hmax, emax, slmax;
-  int ticketed;
+  int R, groups, smax, hmax, emax, slmax;
   int total_waves;
   uint64_t timeout;
 };
@@ -48,10 +57,65 @@ constexpr int log2c() {
   return KT <= 1 ? 0 : 1 + log2c<KT / 2>();
 }
 
-// LDS bytes of one wave's region buffers.
+// Columns per task for k right-hand sides.
+int task_cols(int k) { return k >= KG_MAX ? KG_MAX : pow2_ceil(k < 1 ? 1 : k); }
+
+// LDS bytes of one wave's task buffers.
 size_t region_lds_bytes(const RegionPlan &rp, int kt, size_t vs) {
   return ((size_t)(rp.smax + rp.hmax) * kt + rp.emax + rp.smax) * vs +
-         ((size_t)rp.emax + (rp.smax + 1) + (rp.slmax + 1) + rp.smax) * sizeof(int);
+         ((size_t)rp.emax + (rp.smax + 1) + (rp.slmax + 1) + rp.smax + rp.hmax) *
+             sizeof(int);
+}
+
+// dst[t] = src[t] for t < count (LDS <- global), STAGE_U loads in flight.
+template <typename V>
+__device__ __forceinline__ void stage(V *dst, const V *__restrict__ src, int count) {
+  const int lane = threadIdx.x;
+  for (int t0 = lane; t0 < count; t0 += 64 * STAGE_U) {
+    V v[STAGE_U];
+#pragma unroll
+    for (int u = 0; u < STAGE_U; ++u) {
+      const int t = t0 + 64 * u;
+      v[u] = t < count ? src[t] : V(0);
+    }
+#pragma unroll
+    for (int u = 0; u < STAGE_U; ++u) {
+      const int t = t0 + 64 * u;
+      if (t < count) dst[t] = v[u];
+    }
+  }
+}
+
+// xs[l*KT + c] = M[rows[l]*rs + (c0 + c)*cs] for the nr rows and the kc
+// columns of the group (LDS <- global), STAGE_U loads in flight; coherent
+// loads when COH (values other waves published in this launch).  `rows`
+// lies in LDS: a row id read from global memory would make every value load
+// wait for it (and, through the in-order load counter, for all loads
+// before it).
+template <typename T, int KT, bool COH>
+__device__ __forceinline__ void stage_rows(T *xs, const T *M, const int *rows, int nr,
+                                           int kc, int c0, int64_t rs, int64_t cs) {
+  constexpr int KS = log2c<KT>();
+  const int lane = threadIdx.x;
+  const int count = nr * KT;
+  for (int t0 = lane; t0 < count; t0 += 64 * STAGE_U) {
+    T v[STAGE_U];
+#pragma unroll
+    for (int u = 0; u < STAGE_U; ++u) {
+      const int t = t0 + 64 * u;
+      const int c = t & (KT - 1);
+      v[u] = T(0);
+      if (t < count && c < kc) {
+        const T *p = &M[(int64_t)rows[t >> KS] * rs + (int64_t)(c0 + c) * cs];
+        v[u] = COH ? ld_dev(p) : *p;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < STAGE_U; ++u) {
+      const int t = t0 + 64 * u;
+      if (t < count) xs[t] = v[u];
+    }
+  }
 }
 
 template <typename T, int KT>
@@ -65,48 +129,44 @@ __global__ __launch_bounds__(64) void k_regions(RegionArgs<T> a) {
   int *lep = lcode + a.emax;                         // smax + 1 entry bounds
   int *lsl = lep + a.smax + 1;                       // slmax + 1 slice bounds
   int *lrow = lsl + a.slmax + 1;                     // smax row ids
+  int *lhalo = lrow + a.smax;                        // hmax halo row ids
   const int lane = threadIdx.x;
   const int target = ld_dev(&a.state[ST_EPOCH]) + 1;
-  int cursor = blockIdx.x;
+  const int ntasks = a.R * a.groups;
+  int cursor = 0;
   for (;;) {
-    const int r = next_unit(a.state, a.ticketed, cursor, a.total_waves);
-    if (r >= a.R) break;
+    const int task = next_unit(a.state, 1, cursor, 0);
+    if (task >= ntasks) break;
+    const int r = task / a.groups, g = task - r * a.groups;
+    const int c0 = g * KT;
+    const int kc = a.k - c0 < KT ? a.k - c0 : KT;
     const int base = a.rptr[r], nr = a.rptr[r + 1] - base;
     const int e0 = a.eptr[base], ne = a.eptr[base + nr] - e0;
     const int s0 = a.sptr[r], ns = a.sptr[r + 1] - s0 - 1;
     const int h0 = a.hptr[r], nh = a.hptr[r + 1] - h0;
     // 1. stage
+    stage(lcode, a.ecode + e0, ne);
+    stage(lval, a.eval + e0, ne);
+    stage(lrow, a.rows + base, nr);
+    stage(ldiag, a.diag + base, nr);
+    stage(lsl, a.slices + s0, ns + 1);
+    stage(lhalo, a.halo + h0, nh);
     for (int t = lane; t <= nr; t += 64) lep[t] = a.eptr[base + t] - e0;
-    for (int t = lane; t < nr; t += 64) {
-      lrow[t] = a.rows[base + t];
-      ldiag[t] = a.diag[base + t];
-    }
-    for (int t = lane; t < ne; t += 64) {
-      lcode[t] = a.ecode[e0 + t];
-      lval[t] = a.eval[e0 + t];
-    }
-    for (int t = lane; t <= ns; t += 64) lsl[t] = a.slices[s0 + t];
     wave_lds_sync();
-    for (int t = lane; t < nr * KT; t += 64) {
-      const int l = t >> KS, c = t & (KT - 1);
-      if (c < a.k) xs[t] = a.B[(int64_t)lrow[l] * a.rs + (int64_t)c * a.cs];
-    }
-    // 2. inputs published
-    wait_units(a.dptr, a.dlist, r, a.rflags, target, a.state, a.timeout);
+    stage_rows<T, KT, false>(xs, a.B, lrow, nr, kc, c0, a.rs, a.cs);
+    // 2. inputs published (same column group of the regions read)
+    wait_units(a.dptr, a.dlist, r, a.tflags, target, a.state, a.timeout, a.groups, g);
     // 3. halo
-    for (int t = lane; t < nh * KT; t += 64) {
-      const int hh = t >> KS, c = t & (KT - 1);
-      if (c < a.k)
-        xs[(a.smax + hh) * KT + c] =
-            ld_dev(&a.X[(int64_t)a.halo[h0 + hh] * a.rs + (int64_t)c * a.cs]);
-    }
+    stage_rows<T, KT, true>(xs + (size_t)a.smax * KT, a.X, lhalo, nh, kc, c0, a.rs, a.cs);
     wave_lds_sync();
-    // 4. internal levels
+    // 4. internal levels (one (row, column) pair per lane at a time: solving
+    // several pairs per lane together measured slower -- the per-pair bounds
+    // and guards cost more than the extra LDS parallelism gives)
     for (int s = 0; s < ns; ++s) {
       const int q1 = lsl[s + 1] * KT;
       for (int t = lsl[s] * KT + lane; t < q1; t += 64) {
         const int l = t >> KS, c = t & (KT - 1);
-        if (c < a.k) {
+        if (c < kc) {
           T acc = xs[t];
           for (int e = lep[l]; e < lep[l + 1]; ++e) acc -= lval[e] * xs[lcode[e] * KT + c];
           xs[t] = acc / ldiag[l];
@@ -116,11 +176,12 @@ __global__ __launch_bounds__(64) void k_regions(RegionArgs<T> a) {
     }
     // 5. store, publish
     for (int t = lane; t < nr * KT; t += 64) {
-      const int l = t >> KS, c = t & (KT - 1);
-      if (c < a.k) st_dev(&a.X[(int64_t)lrow[l] * a.rs + (int64_t)c * a.cs], xs[t]);
+      const int c = t & (KT - 1);
+      if (c < kc)
+        st_dev(&a.X[(int64_t)lrow[t >> KS] * a.rs + (int64_t)(c0 + c) * a.cs], xs[t]);
     }
     publish_fence();
-    if (lane == 0) st_dev(&a.rflags[r], target);
+    if (lane == 0) st_dev(&a.tflags[task], target);
   }
   persistent_finish(a.state, a.total_waves);
 }
@@ -144,11 +205,8 @@ int gather_one(symamd_handle h, symamd_plan p, RegionPlan &rp) {
   return SYMAMD_OK;
 }
 
-// Regions are always taken by ticket: a wave holds a region only while it
-// runs, so the deal stays deadlock-free even when fewer blocks are resident
-// than the occupancy estimate promises (these blocks carry up to 64 KiB of
-// LDS each, and the estimate can sit exactly at the CU's LDS capacity).
-// The grid keeps one block per CU of margin below that estimate.
+// Tasks are taken by ticket; the grid keeps one block per CU of margin
+// below the occupancy estimate.
 template <typename T, int KT>
 int region_launch(symamd_handle h, symamd_plan p, const RegionPlan &rp, void *d_B,
                   int k, int64_t rs, int64_t cs, size_t lds) {
@@ -158,9 +216,11 @@ int region_launch(symamd_handle h, symamd_plan p, const RegionPlan &rp, void *d_
       per_cu < 1)
     per_cu = 1;
   if (per_cu > 1) --per_cu;
+  const int groups = (k + KT - 1) / KT;
+  const int64_t ntasks = (int64_t)rp.R * groups;
   int64_t grid = (int64_t)per_cu * h->num_cus;
   if (h->max_waves > 0 && h->max_waves < grid) grid = h->max_waves;
-  if (grid > rp.R) grid = rp.R;
+  if (grid > ntasks) grid = ntasks;
   if (grid < 1) grid = 1;
   RegionArgs<T> a{};
   a.k = k;
@@ -180,16 +240,16 @@ int region_launch(symamd_handle h, symamd_plan p, const RegionPlan &rp, void *d_
   a.ecode = rp.d_ecode;
   a.eval = (const T *)rp.d_eval;
   a.diag = (const T *)rp.d_diag;
-  a.rflags = rp.d_rflags;
+  a.tflags = rp.d_rflags;
   a.state = p->d_state;
   a.R = rp.R;
+  a.groups = groups;
   a.smax = rp.smax;
   a.hmax = rp.hmax;
   a.emax = rp.emax;
   a.slmax = rp.slmax;
   a.total_waves = (int)grid;
   a.timeout = h->timeout_ticks();
-  a.ticketed = 1;
   void *args[] = {&a};
   SYMAMD_HIP(hipLaunchKernel(kfn, dim3((unsigned)grid), dim3(64), args, lds, h->stream));
   return SYMAMD_OK;
@@ -202,19 +262,28 @@ int region_dispatch(symamd_handle h, symamd_plan p, const RegionPlan &rp, void *
     case 1: return region_launch<T, 1>(h, p, rp, d_B, k, rs, cs, lds);
     case 2: return region_launch<T, 2>(h, p, rp, d_B, k, rs, cs, lds);
     case 4: return region_launch<T, 4>(h, p, rp, d_B, k, rs, cs, lds);
-    case 8: return region_launch<T, 8>(h, p, rp, d_B, k, rs, cs, lds);
-    case 16: return region_launch<T, 16>(h, p, rp, d_B, k, rs, cs, lds);
-    case 32: return region_launch<T, 32>(h, p, rp, d_B, k, rs, cs, lds);
-    default: return region_launch<T, 64>(h, p, rp, d_B, k, rs, cs, lds);
+    default: return region_launch<T, 8>(h, p, rp, d_B, k, rs, cs, lds);
   }
 }
 
 }  // namespace
 
-int region_cap(int k) {
-  const int kt = pow2_ceil(k < 1 ? 1 : k);
-  const int cap = pow2_floor(2048 / kt);
+// Default cap: 512 rows for one or two columns per task, 128 for eight
+// (measured on the 128^3 IC(0) factor: k=1 is fastest at 512 -- 1024 and 256
+// are slower -- and k=32 at 128, where one wave's LDS still allows seven
+// waves per CU).
+int region_cap(symamd_handle h, int k) {
+  if (h->region_cap > 0) return h->region_cap;
+  const int cap = pow2_floor(1024 / task_cols(k));
   return cap > 512 ? 512 : (cap < 16 ? 16 : cap);
+}
+
+bool region_preferred(symamd_handle h, symamd_plan p, int k, int layout) {
+  if (k > 1 && layout == SYMAMD_COL_MAJOR) return false;
+  if (!region_usable(h, p, k)) return false;
+  RegionPlan *rp = nullptr;
+  if (plan_regions(h, p, region_cap(h, k), &rp) != SYMAMD_OK) return false;
+  return rp->inside >= 0.6;
 }
 
 int gather_region_values(symamd_handle h, symamd_plan p) {
@@ -230,24 +299,22 @@ int gather_region_values(symamd_handle h, symamd_plan p) {
 }
 
 bool region_usable(symamd_handle h, symamd_plan p, int k) {
-  if (k < 1 || k > 64) return false;
+  if (k < 1 || k > KG_MAX * RegionPlan::MAX_GROUPS) return false;
   RegionPlan *rp = nullptr;
-  if (plan_regions(h, p, region_cap(k), &rp) != SYMAMD_OK || rp->R == 0) return false;
+  if (plan_regions(h, p, region_cap(h, k), &rp) != SYMAMD_OK || rp->R == 0) return false;
   const size_t vs = p->dtype == SYMAMD_F64 ? 8 : 4;
-  return region_lds_bytes(*rp, pow2_ceil(k), vs) <= 64 * 1024;
+  return region_lds_bytes(*rp, task_cols(k), vs) <= 64 * 1024;
 }
 
 int launch_regions(symamd_handle h, symamd_plan p, void *d_B, int k, int64_t rs,
                    int64_t cs) {
-  if (k < 1 || k > 64) return fail(SYMAMD_ERR_NOT_SUPPORTED, "region schedule: 1 <= k <= 64");
+  if (!region_usable(h, p, k))
+    return fail(SYMAMD_ERR_NOT_SUPPORTED, "no region schedule for this matrix and k");
   RegionPlan *rp = nullptr;
-  SYMAMD_TRY(plan_regions(h, p, region_cap(k), &rp));
-  if (rp->R == 0) return fail(SYMAMD_ERR_NOT_SUPPORTED, "no region schedule for this matrix");
-  const int kt = pow2_ceil(k);
+  SYMAMD_TRY(plan_regions(h, p, region_cap(h, k), &rp));
+  const int kt = task_cols(k);
   const size_t vs = p->dtype == SYMAMD_F64 ? 8 : 4;
   const size_t lds = region_lds_bytes(*rp, kt, vs);
-  if (lds > 64 * 1024)
-    return fail(SYMAMD_ERR_NOT_SUPPORTED, "region buffers exceed 64 KiB of LDS");
   if (p->dtype == SYMAMD_F64)
     return region_dispatch<double>(h, p, *rp, d_B, k, rs, cs, kt, lds);
   return region_dispatch<float>(h, p, *rp, d_B, k, rs, cs, kt, lds);

@@ -645,8 +645,8 @@ int plan_regions(symamd_handle h, symamd_plan p, int cap, RegionPlan **out) {
   }
   if (hipMalloc(&rp.d_eval, vs * (o.ecode.empty() ? 1 : o.ecode.size())) != hipSuccess ||
       hipMalloc(&rp.d_diag, vs * n) != hipSuccess ||
-      hipMalloc(&rp.d_rflags, sizeof(int) * rp.R) != hipSuccess ||
-      hipMemset(rp.d_rflags, 0, sizeof(int) * rp.R) != hipSuccess) {
+      hipMalloc(&rp.d_rflags, sizeof(int) * rp.R * RegionPlan::MAX_GROUPS) != hipSuccess ||
+      hipMemset(rp.d_rflags, 0, sizeof(int) * rp.R * RegionPlan::MAX_GROUPS) != hipSuccess) {
     free_region_plan(rp);
     return fail(SYMAMD_ERR_ALLOC, "region schedule arrays");
   }
