@@ -34,8 +34,9 @@ namespace symamd {
 
 namespace {
 
-constexpr int KG_MAX = 8;   // right-hand side columns per task
-constexpr int STAGE_U = 8;  // loads per lane in flight while staging
+constexpr int KG_MAX = 8;       // right-hand side columns per task
+constexpr int STAGE_U = 8;      // loads per lane in flight while staging
+constexpr int ENTRY_CHUNK = 4;  // row entries whose LDS reads are issued together
 
 template <typename T>
 struct RegionArgs {
@@ -67,20 +68,31 @@ size_t region_lds_bytes(const RegionPlan &rp, int kt, size_t vs) {
              sizeof(int);
 }
 
-// dst[t] = src[t] for t < count (LDS <- global), STAGE_U loads in flight.
-template <typename V>
+// LDS writes of the block are visible to all its threads (one wave: the
+// wave's own ordering suffices).
+template <int NT>
+__device__ __forceinline__ void block_sync() {
+  if constexpr (NT == 64)
+    wave_lds_sync();
+  else
+    __syncthreads();
+}
+
+// dst[t] = src[t] for t < count (LDS <- global), STAGE_U loads per thread
+// in flight, NT threads.
+template <int NT, typename V>
 __device__ __forceinline__ void stage(V *dst, const V *__restrict__ src, int count) {
   const int lane = threadIdx.x;
-  for (int t0 = lane; t0 < count; t0 += 64 * STAGE_U) {
+  for (int t0 = lane; t0 < count; t0 += NT * STAGE_U) {
     V v[STAGE_U];
 #pragma unroll
     for (int u = 0; u < STAGE_U; ++u) {
-      const int t = t0 + 64 * u;
+      const int t = t0 + NT * u;
       v[u] = t < count ? src[t] : V(0);
     }
 #pragma unroll
     for (int u = 0; u < STAGE_U; ++u) {
-      const int t = t0 + 64 * u;
+      const int t = t0 + NT * u;
       if (t < count) dst[t] = v[u];
     }
   }
@@ -92,17 +104,17 @@ __device__ __forceinline__ void stage(V *dst, const V *__restrict__ src, int cou
 // lies in LDS: a row id read from global memory would make every value load
 // wait for it (and, through the in-order load counter, for all loads
 // before it).
-template <typename T, int KT, bool COH>
+template <int NT, typename T, int KT, bool COH>
 __device__ __forceinline__ void stage_rows(T *xs, const T *M, const int *rows, int nr,
                                            int kc, int c0, int64_t rs, int64_t cs) {
   constexpr int KS = log2c<KT>();
   const int lane = threadIdx.x;
   const int count = nr * KT;
-  for (int t0 = lane; t0 < count; t0 += 64 * STAGE_U) {
+  for (int t0 = lane; t0 < count; t0 += NT * STAGE_U) {
     T v[STAGE_U];
 #pragma unroll
     for (int u = 0; u < STAGE_U; ++u) {
-      const int t = t0 + 64 * u;
+      const int t = t0 + NT * u;
       const int c = t & (KT - 1);
       v[u] = T(0);
       if (t < count && c < kc) {
@@ -112,14 +124,15 @@ __device__ __forceinline__ void stage_rows(T *xs, const T *M, const int *rows, i
     }
 #pragma unroll
     for (int u = 0; u < STAGE_U; ++u) {
-      const int t = t0 + 64 * u;
+      const int t = t0 + NT * u;
       if (t < count) xs[t] = v[u];
     }
   }
 }
 
-template <typename T, int KT>
-__global__ __launch_bounds__(64) void k_regions(RegionArgs<T> a) {
+// NT threads (one or four waves) per task.
+template <typename T, int KT, int NT>
+__global__ __launch_bounds__(NT) void k_regions(RegionArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int KS = log2c<KT>();
   T *xs = reinterpret_cast<T *>(smem);               // (smax + hmax) * KT values
@@ -130,12 +143,21 @@ __global__ __launch_bounds__(64) void k_regions(RegionArgs<T> a) {
   int *lsl = lep + a.smax + 1;                       // slmax + 1 slice bounds
   int *lrow = lsl + a.slmax + 1;                     // smax row ids
   int *lhalo = lrow + a.smax;                        // hmax halo row ids
+  __shared__ int s_task;
   const int lane = threadIdx.x;
   const int target = ld_dev(&a.state[ST_EPOCH]) + 1;
   const int ntasks = a.R * a.groups;
   int cursor = 0;
   for (;;) {
-    const int task = next_unit(a.state, 1, cursor, 0);
+    int task;
+    if constexpr (NT == 64) {
+      task = next_unit(a.state, 1, cursor, 0);
+    } else {
+      // the previous task's last block_sync ordered every read of s_task
+      if (lane == 0) s_task = atomicAdd(&a.state[ST_TICKET], 1);
+      __syncthreads();
+      task = s_task;
+    }
     if (task >= ntasks) break;
     const int r = task / a.groups, g = task - r * a.groups;
     const int c0 = g * KT;
@@ -145,42 +167,66 @@ __global__ __launch_bounds__(64) void k_regions(RegionArgs<T> a) {
     const int s0 = a.sptr[r], ns = a.sptr[r + 1] - s0 - 1;
     const int h0 = a.hptr[r], nh = a.hptr[r + 1] - h0;
     // 1. stage
-    stage(lcode, a.ecode + e0, ne);
-    stage(lval, a.eval + e0, ne);
-    stage(lrow, a.rows + base, nr);
-    stage(ldiag, a.diag + base, nr);
-    stage(lsl, a.slices + s0, ns + 1);
-    stage(lhalo, a.halo + h0, nh);
-    for (int t = lane; t <= nr; t += 64) lep[t] = a.eptr[base + t] - e0;
-    wave_lds_sync();
-    stage_rows<T, KT, false>(xs, a.B, lrow, nr, kc, c0, a.rs, a.cs);
-    // 2. inputs published (same column group of the regions read)
-    wait_units(a.dptr, a.dlist, r, a.tflags, target, a.state, a.timeout, a.groups, g);
+    stage<NT>(lcode, a.ecode + e0, ne);
+    stage<NT>(lval, a.eval + e0, ne);
+    stage<NT>(lrow, a.rows + base, nr);
+    stage<NT>(ldiag, a.diag + base, nr);
+    stage<NT>(lsl, a.slices + s0, ns + 1);
+    stage<NT>(lhalo, a.halo + h0, nh);
+    for (int t = lane; t <= nr; t += NT) lep[t] = a.eptr[base + t] - e0;
+    block_sync<NT>();
+    stage_rows<NT, T, KT, false>(xs, a.B, lrow, nr, kc, c0, a.rs, a.cs);
+    // 2. inputs published (same column group of the regions read); one wave
+    // polls, the others wait at the barrier
+    if (lane < 64)
+      wait_units(a.dptr, a.dlist, r, a.tflags, target, a.state, a.timeout, a.groups, g);
+    if constexpr (NT > 64) __syncthreads();
     // 3. halo
-    stage_rows<T, KT, true>(xs + (size_t)a.smax * KT, a.X, lhalo, nh, kc, c0, a.rs, a.cs);
-    wave_lds_sync();
-    // 4. internal levels (one (row, column) pair per lane at a time: solving
-    // several pairs per lane together measured slower -- the per-pair bounds
-    // and guards cost more than the extra LDS parallelism gives)
+    stage_rows<NT, T, KT, true>(xs + (size_t)a.smax * KT, a.X, lhalo, nh, kc, c0, a.rs,
+                                a.cs);
+    block_sync<NT>();
+    // 4. internal levels (one (row, column) pair per thread at a time: solving
+    // several pairs per thread together measured slower -- the per-pair
+    // bounds and guards cost more than the extra LDS parallelism gives)
     for (int s = 0; s < ns; ++s) {
       const int q1 = lsl[s + 1] * KT;
-      for (int t = lsl[s] * KT + lane; t < q1; t += 64) {
+      for (int t = lsl[s] * KT + lane; t < q1; t += NT) {
         const int l = t >> KS, c = t & (KT - 1);
         if (c < kc) {
           T acc = xs[t];
-          for (int e = lep[l]; e < lep[l + 1]; ++e) acc -= lval[e] * xs[lcode[e] * KT + c];
+          const int eb = lep[l], ee = lep[l + 1];
+          // entries in chunks: all slot codes and values of a chunk, then
+          // all its dependency values, then the products in entry order
+          // (dead slots re-read the row's first entry, always valid)
+          for (int e0 = eb; e0 < ee; e0 += ENTRY_CHUNK) {
+            int cd[ENTRY_CHUNK];
+            T vv[ENTRY_CHUNK], xv[ENTRY_CHUNK];
+#pragma unroll
+            for (int u = 0; u < ENTRY_CHUNK; ++u) {
+              const int e = e0 + u < ee ? e0 + u : eb;
+              cd[u] = lcode[e];
+              vv[u] = lval[e];
+            }
+#pragma unroll
+            for (int u = 0; u < ENTRY_CHUNK; ++u) xv[u] = xs[cd[u] * KT + c];
+#pragma unroll
+            for (int u = 0; u < ENTRY_CHUNK; ++u)
+              if (e0 + u < ee) acc -= vv[u] * xv[u];
+          }
           xs[t] = acc / ldiag[l];
         }
       }
-      wave_lds_sync();
+      block_sync<NT>();
     }
-    // 5. store, publish
-    for (int t = lane; t < nr * KT; t += 64) {
+    // 5. store; every wave waits for its own stores, then one thread
+    // publishes (the barrier also ends every read of this task's LDS)
+    for (int t = lane; t < nr * KT; t += NT) {
       const int c = t & (KT - 1);
       if (c < kc)
         st_dev(&a.X[(int64_t)lrow[t >> KS] * a.rs + (int64_t)(c0 + c) * a.cs], xs[t]);
     }
     publish_fence();
+    if constexpr (NT > 64) __syncthreads();
     if (lane == 0) st_dev(&a.tflags[task], target);
   }
   persistent_finish(a.state, a.total_waves);
@@ -206,13 +252,16 @@ int gather_one(symamd_handle h, symamd_plan p, RegionPlan &rp) {
 }
 
 // Tasks are taken by ticket; the grid keeps one block per CU of margin
-// below the occupancy estimate.
+// below the occupancy estimate.  One wave per task: a four-wave block per
+// task (NT = 256) did not complete on the 32-column 128^3 solve and is not
+// used until the cause is known.
 template <typename T, int KT>
 int region_launch(symamd_handle h, symamd_plan p, const RegionPlan &rp, void *d_B,
                   int k, int64_t rs, int64_t cs, size_t lds) {
-  const void *kfn = (const void *)&k_regions<T, KT>;
+  constexpr int NT = 64;
+  const void *kfn = (const void *)&k_regions<T, KT, NT>;
   int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, 64, lds) != hipSuccess ||
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, NT, lds) != hipSuccess ||
       per_cu < 1)
     per_cu = 1;
   if (per_cu > 1) --per_cu;
@@ -248,10 +297,10 @@ int region_launch(symamd_handle h, symamd_plan p, const RegionPlan &rp, void *d_
   a.hmax = rp.hmax;
   a.emax = rp.emax;
   a.slmax = rp.slmax;
-  a.total_waves = (int)grid;
+  a.total_waves = (int)grid * (NT / 64);
   a.timeout = h->timeout_ticks();
   void *args[] = {&a};
-  SYMAMD_HIP(hipLaunchKernel(kfn, dim3((unsigned)grid), dim3(64), args, lds, h->stream));
+  SYMAMD_HIP(hipLaunchKernel(kfn, dim3((unsigned)grid), dim3(NT), args, lds, h->stream));
   return SYMAMD_OK;
 }
 
