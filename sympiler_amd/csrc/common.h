@@ -131,11 +131,12 @@ struct UnitSet {
 // topologically).  All arrays are region-major in rank order.
 struct RegionPlan {
   static constexpr int MAX_GROUPS = 8;  // column groups per region (tags)
+  static constexpr int PACK = 4;        // entries per row kept in the packed arrays
   int cap = 0;        // region size cap the clustering used
   int R = 0;          // regions (0: no region schedule for this cap)
   int smax = 0;       // max rows in a region
   int hmax = 0;       // max halo rows (rows of other regions read)
-  int emax = 0;       // max off-diagonal entries in a region
+  int emax = 0;       // max overflow entries (beyond PACK per row) in a region
   int slmax = 0;      // max internal levels (slices) of a region
   double inside = 0;  // fraction of off-diagonal entries inside regions
   int *d_rptr = nullptr;    // R+1: rows of region r = rows[rptr[r] .. rptr[r+1])
@@ -146,11 +147,18 @@ struct RegionPlan {
   int *d_halo = nullptr;    // halo row ids
   int *d_dptr = nullptr;    // R+1: regions region r waits for
   int *d_dlist = nullptr;
-  int *d_eptr = nullptr;    // n+1 over the region-major rows: their entries
-  int *d_ecode = nullptr;   // LDS slot of the entry's column: local row, or
-                            // smax + halo index
-  void *d_eval = nullptr;   // entry values (gathered from the plan's copy)
-  int *d_esrc = nullptr;    // position of each entry in the level-ordered copy
+  // entries of every region-major row: the first PACK packed per row --
+  // count of all its off-diagonal entries, LDS slot codes (the column's local
+  // row, or smax + halo index; padding slots hold 0) and values -- and the
+  // rest, in order, in the overflow list
+  int *d_pcnt = nullptr;    // n
+  int *d_pcode = nullptr;   // PACK * n
+  void *d_pval = nullptr;   // PACK * n values (gathered from the plan's copy)
+  int *d_psrc = nullptr;    // PACK * n positions in the level-ordered copy
+  int *d_eptr = nullptr;    // n+1 over the region-major rows: overflow entries
+  int *d_ecode = nullptr;   // their LDS slot codes
+  void *d_eval = nullptr;   // their values (gathered from the plan's copy)
+  int *d_esrc = nullptr;    // their positions in the level-ordered copy
   void *d_diag = nullptr;   // diagonal of each region-major row
   int *d_dsrc = nullptr;    // its position in the level-ordered copy
   int *d_rflags = nullptr;  // task ready tags, MAX_GROUPS per region
@@ -162,7 +170,7 @@ struct HostRegions {
   int R = 0, smax = 0, hmax = 0, emax = 0, slmax = 0;
   double inside = 0;
   std::vector<int> rptr, rows, sptr, slices, hptr, halo, dptr, dlist, eptr, ecode, esrc,
-      dsrc;
+      dsrc, pcnt, pcode, psrc;
 };
 bool build_region_schedule(int n, int64_t nnz, int levels, const int *lvl_ptr,
                            const int *perm, const int *crp, const int *cci, int cap,

@@ -6,6 +6,14 @@
 
 namespace symamd {
 
+// Optional per-item trace (100 MHz clock): trace[3u + slot] for item u
+// (work unit or region task) when it fits the buffer; lane 0 writes.
+__device__ __forceinline__ void trace_mark(uint64_t *trace, int64_t cap,
+                                           int u, int slot) {
+  if (trace && (threadIdx.x & 63) == 0 && 3 * (int64_t)u + 2 < cap)
+    trace[3 * (int64_t)u + slot] = wall_ticks();
+}
+
 // LDS writes of the wave are complete and visible to all its lanes.
 __device__ __forceinline__ void wave_lds_sync() {
   asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");

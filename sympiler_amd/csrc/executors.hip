@@ -92,12 +92,6 @@ struct SolveArgs {
   int64_t trace_cap;
 };
 
-__device__ __forceinline__ void trace_mark(uint64_t *trace, int64_t cap,
-                                           int u, int slot) {
-  if (trace && (threadIdx.x & 63) == 0 && 3 * (int64_t)u + 2 < cap)
-    trace[3 * (int64_t)u + slot] = wall_ticks();
-}
-
 template <typename T, int G, int KT>
 __device__ __forceinline__ T lane_sum(T v) {
 #pragma unroll

@@ -34,9 +34,9 @@ namespace symamd {
 
 namespace {
 
-constexpr int KG_MAX = 8;       // right-hand side columns per task
-constexpr int STAGE_U = 8;      // loads per lane in flight while staging
-constexpr int ENTRY_CHUNK = 4;  // row entries whose LDS reads are issued together
+constexpr int KG_MAX = 8;   // right-hand side columns per task
+constexpr int STAGE_U = 8;  // loads per lane in flight while staging
+constexpr int PK = RegionPlan::PACK;
 
 template <typename T>
 struct RegionArgs {
@@ -45,12 +45,17 @@ struct RegionArgs {
   const T *B;      // right-hand sides (the same buffer as X: in place)
   T *X;
   const int *rptr, *rows, *sptr, *slices, *hptr, *halo, *dptr, *dlist, *eptr, *ecode;
-  const T *eval, *diag;
+  const int *pcnt, *pcode;
+  const T *pval, *eval, *diag;
   int *tflags;     // task tags: region rank * groups + group
   int *state;
   int R, groups, smax, hmax, emax, slmax;
   int total_waves;
   uint64_t timeout;
+  // optional per-task trace (100 MHz clock): [3t] task taken, [3t+1] its
+  // inputs published, [3t+2] the task published
+  uint64_t *trace;
+  int64_t trace_cap;
 };
 
 template <int KT>
@@ -61,11 +66,36 @@ constexpr int log2c() {
 // Columns per task for k right-hand sides.
 int task_cols(int k) { return k >= KG_MAX ? KG_MAX : pow2_ceil(k < 1 ? 1 : k); }
 
-// LDS bytes of one wave's task buffers.
+// Byte offsets of one task's LDS buffers (each 16-byte aligned), the same on
+// host (launch size) and device (carving): values first, then ints.
+struct RegionLds {
+  size_t xs, pval, lval, ldiag, pcode, pcnt, lcode, lep, lsl, lrow, lhalo, total;
+};
+
+__host__ __device__ inline size_t lds_align(size_t v) { return (v + 15) & ~size_t(15); }
+
+__host__ __device__ inline RegionLds region_lds(int smax, int hmax, int emax, int slmax,
+                                                int kt, size_t vs) {
+  RegionLds L;
+  size_t o = 0;
+  L.xs = o;    o = lds_align(o + (size_t)(smax + hmax) * kt * vs);  // values, halo
+  L.pval = o;  o = lds_align(o + (size_t)PK * smax * vs);            // packed values
+  L.lval = o;  o = lds_align(o + (size_t)emax * vs);                 // overflow values
+  L.ldiag = o; o = lds_align(o + (size_t)smax * vs);
+  L.pcode = o; o = lds_align(o + (size_t)PK * smax * 4);             // packed slots
+  L.pcnt = o;  o = lds_align(o + (size_t)smax * 4);
+  L.lcode = o; o = lds_align(o + (size_t)emax * 4);                  // overflow slots
+  L.lep = o;   o = lds_align(o + (size_t)(smax + 1) * 4);
+  L.lsl = o;   o = lds_align(o + (size_t)(slmax + 1) * 4);
+  L.lrow = o;  o = lds_align(o + (size_t)smax * 4);
+  L.lhalo = o; o = lds_align(o + (size_t)hmax * 4);
+  L.total = o;
+  return L;
+}
+
+// LDS bytes of one task's buffers.
 size_t region_lds_bytes(const RegionPlan &rp, int kt, size_t vs) {
-  return ((size_t)(rp.smax + rp.hmax) * kt + rp.emax + rp.smax) * vs +
-         ((size_t)rp.emax + (rp.smax + 1) + (rp.slmax + 1) + rp.smax + rp.hmax) *
-             sizeof(int);
+  return region_lds(rp.smax, rp.hmax, rp.emax, rp.slmax, kt, vs).total;
 }
 
 // LDS writes of the block are visible to all its threads (one wave: the
@@ -135,14 +165,21 @@ template <typename T, int KT, int NT>
 __global__ __launch_bounds__(NT) void k_regions(RegionArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int KS = log2c<KT>();
-  T *xs = reinterpret_cast<T *>(smem);               // (smax + hmax) * KT values
-  T *lval = xs + (size_t)(a.smax + a.hmax) * KT;     // emax entry values
-  T *ldiag = lval + a.emax;                          // smax diagonals
-  int *lcode = reinterpret_cast<int *>(ldiag + a.smax);  // emax LDS slots
-  int *lep = lcode + a.emax;                         // smax + 1 entry bounds
-  int *lsl = lep + a.smax + 1;                       // slmax + 1 slice bounds
-  int *lrow = lsl + a.slmax + 1;                     // smax row ids
-  int *lhalo = lrow + a.smax;                        // hmax halo row ids
+  // two values per thread in flight for tasks of 8 columns (their slices
+  // hold up to 2 (row, column) pairs per thread), one for narrower tasks
+  constexpr int VPL = KT >= 8 ? 2 : 1;
+  const RegionLds L = region_lds(a.smax, a.hmax, a.emax, a.slmax, KT, sizeof(T));
+  T *xs = reinterpret_cast<T *>(smem + L.xs);        // (smax + hmax) * KT values
+  T *pval = reinterpret_cast<T *>(smem + L.pval);    // PK per row: packed values
+  T *lval = reinterpret_cast<T *>(smem + L.lval);    // emax overflow values
+  T *ldiag = reinterpret_cast<T *>(smem + L.ldiag);  // smax diagonals
+  int *pcode = reinterpret_cast<int *>(smem + L.pcode);  // PK per row: LDS slots
+  int *pcnt = reinterpret_cast<int *>(smem + L.pcnt);    // entries per row
+  int *lcode = reinterpret_cast<int *>(smem + L.lcode);  // emax overflow slots
+  int *lep = reinterpret_cast<int *>(smem + L.lep);      // smax + 1 overflow bounds
+  int *lsl = reinterpret_cast<int *>(smem + L.lsl);      // slmax + 1 slice bounds
+  int *lrow = reinterpret_cast<int *>(smem + L.lrow);    // smax row ids
+  int *lhalo = reinterpret_cast<int *>(smem + L.lhalo);  // hmax halo row ids
   __shared__ int s_task;
   const int lane = threadIdx.x;
   const int target = ld_dev(&a.state[ST_EPOCH]) + 1;
@@ -159,6 +196,7 @@ __global__ __launch_bounds__(NT) void k_regions(RegionArgs<T> a) {
       task = s_task;
     }
     if (task >= ntasks) break;
+    trace_mark(a.trace, a.trace_cap, task, 0);
     const int r = task / a.groups, g = task - r * a.groups;
     const int c0 = g * KT;
     const int kc = a.k - c0 < KT ? a.k - c0 : KT;
@@ -167,6 +205,9 @@ __global__ __launch_bounds__(NT) void k_regions(RegionArgs<T> a) {
     const int s0 = a.sptr[r], ns = a.sptr[r + 1] - s0 - 1;
     const int h0 = a.hptr[r], nh = a.hptr[r + 1] - h0;
     // 1. stage
+    stage<NT>(pcode, a.pcode + (size_t)PK * base, PK * nr);
+    stage<NT>(pval, a.pval + (size_t)PK * base, PK * nr);
+    stage<NT>(pcnt, a.pcnt + base, nr);
     stage<NT>(lcode, a.ecode + e0, ne);
     stage<NT>(lval, a.eval + e0, ne);
     stage<NT>(lrow, a.rows + base, nr);
@@ -181,39 +222,50 @@ __global__ __launch_bounds__(NT) void k_regions(RegionArgs<T> a) {
     if (lane < 64)
       wait_units(a.dptr, a.dlist, r, a.tflags, target, a.state, a.timeout, a.groups, g);
     if constexpr (NT > 64) __syncthreads();
+    trace_mark(a.trace, a.trace_cap, task, 1);
     // 3. halo
     stage_rows<NT, T, KT, true>(xs + (size_t)a.smax * KT, a.X, lhalo, nh, kc, c0, a.rs,
                                 a.cs);
     block_sync<NT>();
-    // 4. internal levels (one (row, column) pair per thread at a time: solving
-    // several pairs per thread together measured slower -- the per-pair
-    // bounds and guards cost more than the extra LDS parallelism gives)
+    // 4. internal levels.  Per (row, column) pair: the row's packed slot
+    // codes and values (addresses known from the row alone, one round of LDS
+    // reads), then the dependency values, then the products in entry order
+    // (padding slots read slot 0 and are not accumulated), then the row's
+    // overflow entries in order, then the division.  VPL pairs per thread
+    // are in flight together.
     for (int s = 0; s < ns; ++s) {
       const int q1 = lsl[s + 1] * KT;
-      for (int t = lsl[s] * KT + lane; t < q1; t += NT) {
-        const int l = t >> KS, c = t & (KT - 1);
-        if (c < kc) {
-          T acc = xs[t];
-          const int eb = lep[l], ee = lep[l + 1];
-          // entries in chunks: all slot codes and values of a chunk, then
-          // all its dependency values, then the products in entry order
-          // (dead slots re-read the row's first entry, always valid)
-          for (int e0 = eb; e0 < ee; e0 += ENTRY_CHUNK) {
-            int cd[ENTRY_CHUNK];
-            T vv[ENTRY_CHUNK], xv[ENTRY_CHUNK];
+      for (int t0 = lsl[s] * KT + lane; t0 < q1; t0 += NT * VPL) {
+        int ll[VPL], cc[VPL], cnt[VPL], cd[VPL][PK];
+        T acc[VPL], pv[VPL][PK], xv[VPL][PK];
+        bool ok[VPL];
 #pragma unroll
-            for (int u = 0; u < ENTRY_CHUNK; ++u) {
-              const int e = e0 + u < ee ? e0 + u : eb;
-              cd[u] = lcode[e];
-              vv[u] = lval[e];
-            }
+        for (int v = 0; v < VPL; ++v) {
+          const int t = t0 + NT * v;
+          cc[v] = t & (KT - 1);
+          ok[v] = t < q1 && cc[v] < kc;
+          ll[v] = ok[v] ? t >> KS : 0;
+          cnt[v] = pcnt[ll[v]];
 #pragma unroll
-            for (int u = 0; u < ENTRY_CHUNK; ++u) xv[u] = xs[cd[u] * KT + c];
-#pragma unroll
-            for (int u = 0; u < ENTRY_CHUNK; ++u)
-              if (e0 + u < ee) acc -= vv[u] * xv[u];
+          for (int u = 0; u < PK; ++u) {
+            cd[v][u] = pcode[PK * ll[v] + u];
+            pv[v][u] = pval[PK * ll[v] + u];
           }
-          xs[t] = acc / ldiag[l];
+          acc[v] = ok[v] ? xs[t] : T(0);
+        }
+#pragma unroll
+        for (int v = 0; v < VPL; ++v)
+#pragma unroll
+          for (int u = 0; u < PK; ++u) xv[v][u] = xs[cd[v][u] * KT + cc[v]];
+#pragma unroll
+        for (int v = 0; v < VPL; ++v) {
+#pragma unroll
+          for (int u = 0; u < PK; ++u)
+            if (u < cnt[v]) acc[v] -= pv[v][u] * xv[v][u];
+          if (cnt[v] > PK)
+            for (int e = lep[ll[v]]; e < lep[ll[v] + 1]; ++e)
+              acc[v] -= lval[e] * xs[lcode[e] * KT + cc[v]];
+          if (ok[v]) xs[t0 + NT * v] = acc[v] / ldiag[ll[v]];
         }
       }
       block_sync<NT>();
@@ -228,6 +280,7 @@ __global__ __launch_bounds__(NT) void k_regions(RegionArgs<T> a) {
     publish_fence();
     if constexpr (NT > 64) __syncthreads();
     if (lane == 0) st_dev(&a.tflags[task], target);
+    trace_mark(a.trace, a.trace_cap, task, 2);
   }
   persistent_finish(a.state, a.total_waves);
 }
@@ -245,16 +298,19 @@ int gather_one(symamd_handle h, symamd_plan p, RegionPlan &rp) {
   if (rp.entries > 0)
     k_region_gather<T><<<(unsigned)((rp.entries + 255) / 256), 256, 0, h->stream>>>(
         rp.entries, rp.d_esrc, val, (T *)rp.d_eval);
+  const int64_t packed = (int64_t)PK * p->n;
+  k_region_gather<T><<<(unsigned)((packed + 255) / 256), 256, 0, h->stream>>>(
+      packed, rp.d_psrc, val, (T *)rp.d_pval);
   k_region_gather<T><<<(unsigned)((p->n + 255) / 256), 256, 0, h->stream>>>(
       p->n, rp.d_dsrc, val, (T *)rp.d_diag);
   SYMAMD_HIP(hipGetLastError());
   return SYMAMD_OK;
 }
 
-// Tasks are taken by ticket; the grid keeps one block per CU of margin
-// below the occupancy estimate.  One wave per task: a four-wave block per
-// task (NT = 256) did not complete on the 32-column 128^3 solve and is not
-// used until the cause is known.
+// Tasks are taken by ticket, so the grid may be as large as the occupancy
+// estimate (a block that is not resident yet simply takes later tickets).
+// One wave per task: a four-wave block per task (NT = 256) did not complete
+// on the 32-column 128^3 solve and is not used until the cause is known.
 template <typename T, int KT>
 int region_launch(symamd_handle h, symamd_plan p, const RegionPlan &rp, void *d_B,
                   int k, int64_t rs, int64_t cs, size_t lds) {
@@ -264,7 +320,6 @@ int region_launch(symamd_handle h, symamd_plan p, const RegionPlan &rp, void *d_
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, NT, lds) != hipSuccess ||
       per_cu < 1)
     per_cu = 1;
-  if (per_cu > 1) --per_cu;
   const int groups = (k + KT - 1) / KT;
   const int64_t ntasks = (int64_t)rp.R * groups;
   int64_t grid = (int64_t)per_cu * h->num_cus;
@@ -287,6 +342,9 @@ int region_launch(symamd_handle h, symamd_plan p, const RegionPlan &rp, void *d_
   a.dlist = rp.d_dlist;
   a.eptr = rp.d_eptr;
   a.ecode = rp.d_ecode;
+  a.pcnt = rp.d_pcnt;
+  a.pcode = rp.d_pcode;
+  a.pval = (const T *)rp.d_pval;
   a.eval = (const T *)rp.d_eval;
   a.diag = (const T *)rp.d_diag;
   a.tflags = rp.d_rflags;
@@ -299,6 +357,8 @@ int region_launch(symamd_handle h, symamd_plan p, const RegionPlan &rp, void *d_
   a.slmax = rp.slmax;
   a.total_waves = (int)grid * (NT / 64);
   a.timeout = h->timeout_ticks();
+  a.trace = h->trace;
+  a.trace_cap = h->trace_cap;
   void *args[] = {&a};
   SYMAMD_HIP(hipLaunchKernel(kfn, dim3((unsigned)grid), dim3(NT), args, lds, h->stream));
   return SYMAMD_OK;

@@ -270,9 +270,10 @@ int plan_owners(symamd_handle h, symamd_plan p, UnitSet *us, int nw) {
 }
 
 void free_region_plan(RegionPlan &rp) {
-  void *ptrs[] = {rp.d_rptr, rp.d_rows,  rp.d_sptr, rp.d_slices, rp.d_hptr,
-                  rp.d_halo, rp.d_dptr,  rp.d_dlist, rp.d_eptr,  rp.d_ecode,
-                  rp.d_eval, rp.d_esrc,  rp.d_diag, rp.d_dsrc,   rp.d_rflags};
+  void *ptrs[] = {rp.d_rptr, rp.d_rows, rp.d_sptr,  rp.d_slices, rp.d_hptr,
+                  rp.d_halo, rp.d_dptr, rp.d_dlist, rp.d_eptr,   rp.d_ecode,
+                  rp.d_eval, rp.d_esrc, rp.d_diag,  rp.d_dsrc,   rp.d_rflags,
+                  rp.d_pcnt, rp.d_pcode, rp.d_pval, rp.d_psrc};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   rp = RegionPlan{};
@@ -536,10 +537,13 @@ bool build_region_schedule(int n, int64_t nnz, int levels, const int *lvl_ptr,
     for (int t = rptr[r]; t < rptr[r + 1]; ++t) lidx[rows[t]] = t - rptr[r];
   // slices, halos, entries, dependency lists (in ranks)
   std::vector<int> sptr(R + 1, 0), slices, hptr(R + 1, 0), halo, dptr(R + 1, 0), dlist;
+  // entries: the first RegionPlan::PACK of every row packed per row (pcnt,
+  // pcode, psrc; padding slots read LDS slot 0 and the row's diagonal, and
+  // are never accumulated), the rest in the overflow list (eptr, ecode, esrc)
+  constexpr int PK = RegionPlan::PACK;
   std::vector<int> eptr(n + 1, 0), ecode, esrc, dsrc(n);
+  std::vector<int> pcnt(n), pcode((size_t)PK * n, 0), psrc((size_t)PK * n, 0);
   std::vector<int> hslot(n, -1), hstamp(n, -1);
-  ecode.reserve(offd);
-  esrc.reserve(offd);
   int hmax = 0, emax = 0, slmax = 0;
   for (int r = 0; r < R; ++r) {
     const int g = order[r];  // cluster id of rank r
@@ -568,13 +572,23 @@ bool build_region_schedule(int n, int64_t nnz, int levels, const int *lvl_ptr,
     for (int t = a; t < b; ++t) {
       const int i = rows[t];
       const int q = ipos[i];
+      dsrc[t] = crp[q + 1] - 1;
+      pcnt[t] = Lp[i + 1] - 1 - Lp[i];
+      for (int u = 0; u < PK; ++u) psrc[(size_t)PK * t + u] = dsrc[t];
       for (int j = Lp[i]; j < Lp[i + 1] - 1; ++j) {
         const int c = Li[j];
-        ecode.push_back(reg[c] == g ? lidx[c] : smax + hslot[c]);
-        esrc.push_back(crp[q] + (j - Lp[i]));
+        const int u = j - Lp[i];
+        const int code = reg[c] == g ? lidx[c] : smax + hslot[c];
+        const int src = crp[q] + u;
+        if (u < PK) {
+          pcode[(size_t)PK * t + u] = code;
+          psrc[(size_t)PK * t + u] = src;
+        } else {
+          ecode.push_back(code);
+          esrc.push_back(src);
+        }
       }
       eptr[t + 1] = (int)ecode.size();
-      dsrc[t] = crp[q + 1] - 1;
     }
     emax = std::max(emax, (int)ecode.size() - e0);
     for (int e = gptr[g]; e < gptr[g + 1]; ++e) dlist.push_back(rank[glist[e]]);
@@ -597,6 +611,9 @@ bool build_region_schedule(int n, int64_t nnz, int levels, const int *lvl_ptr,
   o.ecode.swap(ecode);
   o.esrc.swap(esrc);
   o.dsrc.swap(dsrc);
+  o.pcnt.swap(pcnt);
+  o.pcode.swap(pcode);
+  o.psrc.swap(psrc);
   return true;
 }
 
@@ -639,11 +656,14 @@ int plan_regions(symamd_handle h, symamd_plan p, int cap, RegionPlan **out) {
       (rc = upload(&rp.d_hptr, o.hptr)) || (rc = upload(&rp.d_halo, o.halo)) ||
       (rc = upload(&rp.d_dptr, o.dptr)) || (rc = upload(&rp.d_dlist, o.dlist)) ||
       (rc = upload(&rp.d_eptr, o.eptr)) || (rc = upload(&rp.d_ecode, o.ecode)) ||
-      (rc = upload(&rp.d_esrc, o.esrc)) || (rc = upload(&rp.d_dsrc, o.dsrc))) {
+      (rc = upload(&rp.d_esrc, o.esrc)) || (rc = upload(&rp.d_dsrc, o.dsrc)) ||
+      (rc = upload(&rp.d_pcnt, o.pcnt)) || (rc = upload(&rp.d_pcode, o.pcode)) ||
+      (rc = upload(&rp.d_psrc, o.psrc))) {
     free_region_plan(rp);
     return rc;
   }
   if (hipMalloc(&rp.d_eval, vs * (o.ecode.empty() ? 1 : o.ecode.size())) != hipSuccess ||
+      hipMalloc(&rp.d_pval, vs * RegionPlan::PACK * n) != hipSuccess ||
       hipMalloc(&rp.d_diag, vs * n) != hipSuccess ||
       hipMalloc(&rp.d_rflags, sizeof(int) * rp.R * RegionPlan::MAX_GROUPS) != hipSuccess ||
       hipMemset(rp.d_rflags, 0, sizeof(int) * rp.R * RegionPlan::MAX_GROUPS) != hipSuccess) {
