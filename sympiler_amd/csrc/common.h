@@ -4,6 +4,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <climits>
 #include <cstdint>
 #include <cstdio>
 #include <map>
@@ -139,33 +140,68 @@ struct RegionPlan {
   int emax = 0;       // max overflow entries (beyond PACK per row) in a region
   int slmax = 0;      // max internal levels (slices) of a region
   double inside = 0;  // fraction of off-diagonal entries inside regions
-  int *d_rptr = nullptr;    // R+1: rows of region r = rows[rptr[r] .. rptr[r+1])
-  int *d_rows = nullptr;    // n: row ids, each region's in internal level order
-  int *d_sptr = nullptr;    // R+1: slices of region r = slices[sptr[r] .. sptr[r+1]]
-  int *d_slices = nullptr;  // local row offsets of slice starts (+ end), per region
-  int *d_hptr = nullptr;    // R+1
-  int *d_halo = nullptr;    // halo row ids
-  int *d_dptr = nullptr;    // R+1: regions region r waits for
-  int *d_dlist = nullptr;
-  // entries of every region-major row: the first PACK packed per row --
-  // count of all its off-diagonal entries, LDS slot codes (the column's local
-  // row, or smax + halo index; padding slots hold 0) and values -- and the
-  // rest, in order, in the overflow list
-  int *d_pcnt = nullptr;    // n
-  int *d_pcode = nullptr;   // PACK * n
-  void *d_pval = nullptr;   // PACK * n values (gathered from the plan's copy)
-  int *d_psrc = nullptr;    // PACK * n positions in the level-ordered copy
-  int *d_eptr = nullptr;    // n+1 over the region-major rows: overflow entries
-  int *d_ecode = nullptr;   // their LDS slot codes
-  void *d_eval = nullptr;   // their values (gathered from the plan's copy)
-  int *d_esrc = nullptr;    // their positions in the level-ordered copy
-  void *d_diag = nullptr;   // diagonal of each region-major row
-  int *d_dsrc = nullptr;    // its position in the level-ordered copy
+  // One record per region (byte layout rec_layout(), also its LDS image),
+  // described by rhead[8r ..] = {rows, halo rows, overflow entries, slices,
+  // dependencies, first 16-byte unit of the record in `rec`, its units, 0}.
+  void *d_rec = nullptr;
+  int *d_rhead = nullptr;
+  int recmax = 0;  // largest record, bytes
+  // matrix values of the records: rec (as values) [vdst[t]] = value at
+  // position vsrc[t] of the plan's level-ordered copy
+  int *d_vsrc = nullptr;
+  int *d_vdst = nullptr;
+  int64_t nvals = 0;
   int *d_rflags = nullptr;  // task ready tags, MAX_GROUPS per region
-  int64_t entries = 0;
 };
 
-// Host copy of a region schedule (same arrays as RegionPlan, before upload).
+// Sections of a region record, in bytes from its start, each 16-byte
+// aligned: the packed values (PACK per row; padding slots hold +0.0), the
+// diagonals, the overflow values (entries beyond PACK, row
+// by row in order), the packed LDS slot codes (uint16), the
+// per-row count of off-diagonal entries (uint8, capped at PACK + 1), the
+// row ids, the halo row ids, the slice bounds (ns + 1 local row offsets),
+// the ranks of the regions it waits for, the overflow bounds per row (nr + 1
+// ints, only when ne > 0) and the overflow slot codes (uint16).  A slot
+// code is the column's local row in the region, or smax + its index in the
+// halo; padding slots name slot smax + hmax, which holds zeros, so a
+// padding product is +0.0 * +0.0 and subtracting it leaves any value
+// unchanged, bit for bit.
+struct RecLayout {
+  int pval, diag, lval, code, cnt, rows, halo, slices, deps, lep, lcode, bytes;
+};
+__host__ __device__ inline RecLayout rec_layout(int nr, int nh, int ne, int ns, int nd,
+                                                int vs) {
+  constexpr int PK = RegionPlan::PACK;
+  RecLayout L;
+  int o = 0;
+  auto put = [&o](int &field, int bytes) {
+    field = o;
+    o = (o + bytes + 15) & ~15;
+  };
+  put(L.pval, PK * nr * vs);
+  put(L.diag, nr * vs);
+  put(L.lval, ne * vs);
+  put(L.code, PK * nr * 2);
+  put(L.cnt, nr);
+  put(L.rows, nr * 4);
+  put(L.halo, nh * 4);
+  put(L.slices, (ns + 1) * 4);
+  put(L.deps, nd * 4);
+  put(L.lep, ne > 0 ? (nr + 1) * 4 : 0);
+  put(L.lcode, ne * 2);
+  L.bytes = o;
+  return L;
+}
+// Slot codes are uint16: a region with its halo and the zero slot may use
+// at most this many LDS slots.
+constexpr int REGION_SLOTS_MAX = 65535;
+
+// Host copy of a region schedule, before packing into records.  Rows are
+// region-major in rank order; entries of every row: the first PACK packed
+// per row (pcnt = all its off-diagonal entries, pcode/psrc; padding slots
+// hold code smax + hmax and the diagonal's position), the rest in order in the
+// overflow list (eptr over the region-major rows, ecode, esrc); psrc, esrc
+// and dsrc are positions in the level-ordered copy of the matrix.
 struct HostRegions {
   int R = 0, smax = 0, hmax = 0, emax = 0, slmax = 0;
   double inside = 0;
@@ -175,6 +211,15 @@ struct HostRegions {
 bool build_region_schedule(int n, int64_t nnz, int levels, const int *lvl_ptr,
                            const int *perm, const int *crp, const int *cci, int cap,
                            HostRegions &out);
+// Records of a schedule for values of vs bytes: the static bytes (values
+// zero), headers (8 ints per region, as RegionPlan::d_rhead) and the value
+// gather list.  False when the records would exceed the 32-bit offsets.
+struct HostRecords {
+  std::vector<uint8_t> bytes;
+  std::vector<int> rhead, vsrc, vdst;
+  int recmax = 0;
+};
+bool pack_region_records(const HostRegions &o, int vs, HostRecords &out);
 
 struct Choice {
   int exec = SYMAMD_EXEC_SYNCFREE;

@@ -20,26 +20,25 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_wave_barrier();
 }
 
-// Wait until every dependency u' of item u (dlist[dptr[u] .. dptr[u+1]))
-// carries `target` in flags[u' * mul + add].  One lane per dependency, one
+// Wait until every item u' of list[0 .. count) (global memory or LDS)
+// carries `target` in flags[u' * mul + add].  One lane per item, one
 // coherent load each; later rounds re-load only the pending ones after a
 // short sleep.  A wait longer than `timeout` ticks of the 100 MHz clock
 // records an error in state[ST_ERROR] and gives up, so every wave exits.
 // (Batching several tags per lane into one round trip measured slower on the
 // 128^3 IC(0) factor: the extra address setup sits on the critical path of
 // every unit.)
-__device__ __forceinline__ void wait_units(const int *__restrict__ dptr,
-                                           const int *__restrict__ dlist,
-                                           int u, const int *flags,
-                                           int target, int *state,
-                                           uint64_t timeout, int mul = 1,
-                                           int add = 0) {
+// Items outside [0, limit) are read as item 0.
+__device__ __forceinline__ void wait_list(const int *list, int count, const int *flags,
+                                          int target, int *state, uint64_t timeout,
+                                          int mul = 1, int add = 0, int limit = INT_MAX) {
   const int lane = threadIdx.x & 63;
-  const int d0 = dptr[u], d1 = dptr[u + 1];
-  for (int base = d0; base < d1; base += 64) {
+  for (int base = 0; base < count; base += 64) {
     const int t = base + lane;
-    bool pending = t < d1;
-    const int *f = pending ? &flags[dlist[t] * mul + add] : flags;
+    bool pending = t < count;
+    int item = pending ? list[t] : 0;
+    item = (unsigned)item < (unsigned)limit ? item : 0;
+    const int *f = pending ? &flags[item * mul + add] : flags;
     if (pending) pending = ld_dev(f) != target;
     if (!__any(pending)) continue;
     const uint64_t t0 = wall_ticks();
@@ -54,6 +53,15 @@ __device__ __forceinline__ void wait_units(const int *__restrict__ dptr,
     }
   }
   asm volatile("" ::: "memory");
+}
+
+// wait_list over the dependencies of item u, dlist[dptr[u] .. dptr[u+1]).
+__device__ __forceinline__ void wait_units(const int *__restrict__ dptr,
+                                           const int *__restrict__ dlist, int u,
+                                           const int *flags, int target, int *state,
+                                           uint64_t timeout) {
+  const int d0 = dptr[u];
+  wait_list(dlist + d0, dptr[u + 1] - d0, flags, target, state, timeout);
 }
 
 // Next item of this wave: round-robin over a co-resident grid (cursor

@@ -5,9 +5,12 @@
 // side columns; column groups are independent solves, so a task waits only
 // on the same column group of the regions it reads.  One wavefront solves
 // one task entirely in LDS:
-//   1. stage the region's entries (LDS slot code + value), diagonals, row
-//      ids, slice bounds and the group's right-hand sides -- none of it
-//      depends on other tasks, so it overlaps the wait;
+//   1. copy the region's record -- packed entries (LDS slot codes and
+//      values), diagonals, row ids, halo row ids, slice bounds and the
+//      regions it waits for, one contiguous block laid out as its LDS image
+//      -- straight into LDS (direct global -> LDS loads, all in flight at
+//      once), then the group's right-hand sides of its rows the same way;
+//      none of it depends on other tasks, so it overlaps the wait;
 //   2. wait until the tasks it reads from carry the launch's tag;
 //   3. load the halo (the values of those regions' rows it reads) into LDS;
 //   4. solve the internal levels (slices) one after another; the lanes of a
@@ -15,8 +18,9 @@
 //      read from LDS; slices are ordered by the wave's own LDS ordering,
 //      with no cross-wave hand-off;
 //   5. store the rows, wait for the stores, publish the task's tag.
-// Every global -> LDS copy keeps STAGE_U loads per lane in flight, so a
-// task pays a few memory round trips instead of one per 64 values.
+// The next task's ticket and header are fetched while this one runs, so a
+// task pays about four memory round trips (record, halo, stores, and the
+// wait itself).
 //
 // Row arithmetic is the reference's (sparse_blas/sptrsv.cpp:17-25): the
 // right-hand side minus each product in the row's stored order, then the
@@ -25,9 +29,11 @@
 //
 // Tasks are ordered by region rank (topological) then column group and
 // taken by ticket, so the lowest unfinished task is always held by a
-// running wave with all its inputs published: the grid cannot deadlock,
-// whether or not all its blocks are resident (these blocks carry up to
-// 64 KiB of LDS each), and the watchdog of wait_units bounds every wait.
+// running wave with all its inputs published (a wave holds at most one
+// further ticket, always above the one it is solving): the grid cannot
+// deadlock, whether or not all its blocks are resident (these blocks carry
+// up to 64 KiB of LDS each), and the watchdog of wait_list bounds every
+// wait.
 #include "device.h"
 
 namespace symamd {
@@ -35,8 +41,16 @@ namespace symamd {
 namespace {
 
 constexpr int KG_MAX = 8;   // right-hand side columns per task
-constexpr int STAGE_U = 8;  // loads per lane in flight while staging
+constexpr int STAGE_U = 8;  // loads per lane in flight while staging (strided path)
 constexpr int PK = RegionPlan::PACK;
+static_assert(PK == 4, "the slice loop unpacks four uint16 slot codes per row");
+// cache policy bits of a direct global -> LDS load: 16 = sc1, the bits of an
+// agent-scope atomic load (values other waves published in this launch)
+constexpr int AUX_PLAIN = 0;
+constexpr int AUX_COH = 16;
+
+typedef __attribute__((address_space(1))) void gmem_void;
+typedef __attribute__((address_space(3))) void lds_void;
 
 template <typename T>
 struct RegionArgs {
@@ -44,12 +58,13 @@ struct RegionArgs {
   int64_t rs, cs;  // element (i, c) at X[i*rs + c*cs]
   const T *B;      // right-hand sides (the same buffer as X: in place)
   T *X;
-  const int *rptr, *rows, *sptr, *slices, *hptr, *halo, *dptr, *dlist, *eptr, *ecode;
-  const int *pcnt, *pcode;
-  const T *pval, *eval, *diag;
-  int *tflags;     // task tags: region rank * groups + group
+  int n;           // rows: row ids read from LDS are clamped below n
+  const uint8_t *rec;  // region records
+  const int *head;     // 8 ints per region (RegionPlan::d_rhead)
+  int *tflags;         // task tags: region rank * groups + group
   int *state;
-  int R, groups, smax, hmax, emax, slmax;
+  int R, groups, smax, hmax;
+  int xsb;  // LDS bytes of the value slots; the record image follows
   int total_waves;
   uint64_t timeout;
   // optional per-task trace (100 MHz clock): [3t] task taken, [3t+1] its
@@ -66,258 +81,346 @@ constexpr int log2c() {
 // Columns per task for k right-hand sides.
 int task_cols(int k) { return k >= KG_MAX ? KG_MAX : pow2_ceil(k < 1 ? 1 : k); }
 
-// Byte offsets of one task's LDS buffers (each 16-byte aligned), the same on
-// host (launch size) and device (carving): values first, then ints.
-struct RegionLds {
-  size_t xs, pval, lval, ldiag, pcode, pcnt, lcode, lep, lsl, lrow, lhalo, total;
-};
-
-__host__ __device__ inline size_t lds_align(size_t v) { return (v + 15) & ~size_t(15); }
-
-__host__ __device__ inline RegionLds region_lds(int smax, int hmax, int emax, int slmax,
-                                                int kt, size_t vs) {
-  RegionLds L;
-  size_t o = 0;
-  L.xs = o;    o = lds_align(o + (size_t)(smax + hmax) * kt * vs);  // values, halo
-  L.pval = o;  o = lds_align(o + (size_t)PK * smax * vs);            // packed values
-  L.lval = o;  o = lds_align(o + (size_t)emax * vs);                 // overflow values
-  L.ldiag = o; o = lds_align(o + (size_t)smax * vs);
-  L.pcode = o; o = lds_align(o + (size_t)PK * smax * 4);             // packed slots
-  L.pcnt = o;  o = lds_align(o + (size_t)smax * 4);
-  L.lcode = o; o = lds_align(o + (size_t)emax * 4);                  // overflow slots
-  L.lep = o;   o = lds_align(o + (size_t)(smax + 1) * 4);
-  L.lsl = o;   o = lds_align(o + (size_t)(slmax + 1) * 4);
-  L.lrow = o;  o = lds_align(o + (size_t)smax * 4);
-  L.lhalo = o; o = lds_align(o + (size_t)hmax * 4);
-  L.total = o;
-  return L;
+// LDS of one task: (smax + hmax + 1) * KT value slots (the region's rows,
+// its halo, the zero slot), then the record image.
+int xs_bytes(const RegionPlan &rp, int kt, int vs) {
+  return (int)((((int64_t)rp.smax + rp.hmax + 1) * kt * vs + 15) & ~int64_t(15));
+}
+size_t region_lds_bytes(const RegionPlan &rp, int kt, int vs) {
+  return (size_t)xs_bytes(rp, kt, vs) + (size_t)rp.recmax;
 }
 
-// LDS bytes of one task's buffers.
-size_t region_lds_bytes(const RegionPlan &rp, int kt, size_t vs) {
-  return region_lds(rp.smax, rp.hmax, rp.emax, rp.slmax, kt, vs).total;
-}
+__device__ __forceinline__ void vm_wait() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
-// LDS writes of the block are visible to all its threads (one wave: the
-// wave's own ordering suffices).
-template <int NT>
-__device__ __forceinline__ void block_sync() {
-  if constexpr (NT == 64)
-    wave_lds_sync();
-  else
-    __syncthreads();
-}
-
-// dst[t] = src[t] for t < count (LDS <- global), STAGE_U loads per thread
-// in flight, NT threads.
-template <int NT, typename V>
-__device__ __forceinline__ void stage(V *dst, const V *__restrict__ src, int count) {
+// LDS dst[0 .. bytes) = global src[0 .. bytes) (both 16-byte aligned, bytes
+// a multiple of 16): 16 bytes per lane per instruction, 1 KiB per
+// instruction, every instruction in flight together; vm_wait() completes
+// them.
+__device__ __forceinline__ void dma_bytes(uint8_t *dst, const uint8_t *src, int bytes) {
   const int lane = threadIdx.x;
-  for (int t0 = lane; t0 < count; t0 += NT * STAGE_U) {
-    V v[STAGE_U];
+  for (int o = 0; o < bytes; o += 1024)
+    if (o + 16 * lane < bytes)
+      __builtin_amdgcn_global_load_lds((gmem_void *)(src + o + 16 * lane),
+                                       (lds_void *)(dst + o), 16, 0, 0);
+}
+
+// xs[l*KT + c] = M[rows[l]*rs + c0 + c] for l < nr, c < kc (unit column
+// stride): each row's KT slots are KT*sizeof(T) contiguous LDS bytes, moved
+// ES bytes per lane by direct global -> LDS loads, 64/LPR rows per
+// instruction.  The launch picks ES so that kc*sizeof(T), rs*sizeof(T) and
+// the buffer are ES-aligned.  `rows` lies in LDS (ids clamped below n).
+template <typename T, int KT, int ES, int AUX>
+__device__ __forceinline__ void dma_rows(T *xs, const T *M, const int *rows, int nr, int kc,
+                                         int c0, int64_t rs, int n) {
+  constexpr int RB = KT * (int)sizeof(T);
+  static_assert((ES == 16 || ES == 4) && RB % ES == 0, "row bytes: a multiple of the load size");
+  constexpr int LPR = RB / ES;   // lanes per row
+  constexpr int RPI = 64 / LPR;  // rows per instruction
+  constexpr int UB = 8;          // instructions per batch: row ids read first
+  const int lane = threadIdx.x;
+  const int lr = lane / LPR, part = lane - lr * LPR;
+  const bool live = (part + 1) * ES <= kc * (int)sizeof(T);
+  for (int b0 = 0; b0 < nr; b0 += RPI * UB) {
+    int rid[UB];
 #pragma unroll
-    for (int u = 0; u < STAGE_U; ++u) {
-      const int t = t0 + NT * u;
-      v[u] = t < count ? src[t] : V(0);
+    for (int u = 0; u < UB; ++u) {
+      const int l = b0 + RPI * u + lr;
+      const int id = live && l < nr ? rows[l] : -1;
+      rid[u] = id < n ? id : 0;
     }
 #pragma unroll
-    for (int u = 0; u < STAGE_U; ++u) {
-      const int t = t0 + NT * u;
-      if (t < count) dst[t] = v[u];
+    for (int u = 0; u < UB; ++u) {
+      if (rid[u] < 0) continue;
+      const uint8_t *src =
+          reinterpret_cast<const uint8_t *>(M + (int64_t)rid[u] * rs + c0) + part * ES;
+      lds_void *dst = (lds_void *)(xs + (size_t)(b0 + RPI * u) * KT);
+      // the builtin takes its size as a literal
+      if constexpr (ES == 16)
+        __builtin_amdgcn_global_load_lds((gmem_void *)src, dst, 16, 0, AUX);
+      else
+        __builtin_amdgcn_global_load_lds((gmem_void *)src, dst, 4, 0, AUX);
     }
   }
 }
 
-// xs[l*KT + c] = M[rows[l]*rs + (c0 + c)*cs] for the nr rows and the kc
-// columns of the group (LDS <- global), STAGE_U loads in flight; coherent
-// loads when COH (values other waves published in this launch).  `rows`
-// lies in LDS: a row id read from global memory would make every value load
-// wait for it (and, through the in-order load counter, for all loads
-// before it).
-template <int NT, typename T, int KT, bool COH>
+// Strided columns (column-major blocks): xs[l*KT + c] = M[rows[l]*rs +
+// (c0 + c)*cs] through registers, STAGE_U loads in flight; coherent loads
+// when COH.
+template <typename T, int KT, bool COH>
 __device__ __forceinline__ void stage_rows(T *xs, const T *M, const int *rows, int nr,
-                                           int kc, int c0, int64_t rs, int64_t cs) {
+                                           int kc, int c0, int64_t rs, int64_t cs, int n) {
   constexpr int KS = log2c<KT>();
   const int lane = threadIdx.x;
   const int count = nr * KT;
-  for (int t0 = lane; t0 < count; t0 += NT * STAGE_U) {
+  for (int t0 = lane; t0 < count; t0 += 64 * STAGE_U) {
     T v[STAGE_U];
 #pragma unroll
     for (int u = 0; u < STAGE_U; ++u) {
-      const int t = t0 + NT * u;
+      const int t = t0 + 64 * u;
       const int c = t & (KT - 1);
       v[u] = T(0);
       if (t < count && c < kc) {
-        const T *p = &M[(int64_t)rows[t >> KS] * rs + (int64_t)(c0 + c) * cs];
+        const int id = rows[t >> KS];
+        const T *p = &M[(int64_t)((unsigned)id < (unsigned)n ? id : 0) * rs +
+                        (int64_t)(c0 + c) * cs];
         v[u] = COH ? ld_dev(p) : *p;
       }
     }
 #pragma unroll
     for (int u = 0; u < STAGE_U; ++u) {
-      const int t = t0 + NT * u;
+      const int t = t0 + 64 * u;
       if (t < count) xs[t] = v[u];
     }
   }
 }
 
-// NT threads (one or four waves) per task.
-template <typename T, int KT, int NT>
-__global__ __launch_bounds__(NT) void k_regions(RegionArgs<T> a) {
-  extern __shared__ __align__(16) unsigned char smem[];
+template <typename T, int KT, int ES, bool COH>
+__device__ __forceinline__ void load_rows(T *xs, const T *M, const int *rows, int nr, int kc,
+                                          int c0, int64_t rs, int64_t cs, int n) {
+  if constexpr (ES > 0)
+    dma_rows<T, KT, ES, COH ? AUX_COH : AUX_PLAIN>(xs, M, rows, nr, kc, c0, rs, n);
+  else
+    stage_rows<T, KT, COH>(xs, M, rows, nr, kc, c0, rs, cs, n);
+}
+
+// Operands of a lane's VPL (row, column) pairs that do not depend on the
+// slice being solved: position, entry count, slot codes and values, the
+// diagonal and the right-hand side (a pair's slot is written only when its
+// own slice is solved).  A pair outside the slice reads row 0 and stores
+// nothing.
+template <typename T, int VPL>
+struct Pairs {
+  int t[VPL], cc[VPL], cnt[VPL], ll[VPL];
+  uint2 cw[VPL];  // four uint16 slot codes, unpacked where they are used
+  bool ok[VPL];
+  T pv[VPL][PK], dg[VPL], acc[VPL];
+};
+
+template <typename T, int KT, int VPL>
+__device__ __forceinline__ void load_pairs(Pairs<T, VPL> &m, int t0, int q1, int kc,
+                                           const T *xs, const uint8_t *cnt8,
+                                           const uint16_t *code, const T *pval,
+                                           const T *ldiag) {
   constexpr int KS = log2c<KT>();
-  // two values per thread in flight for tasks of 8 columns (their slices
-  // hold up to 2 (row, column) pairs per thread), one for narrower tasks
+#pragma unroll
+  for (int v = 0; v < VPL; ++v) {
+    const int t = t0 + 64 * v;
+    m.t[v] = t;
+    m.cc[v] = t & (KT - 1);
+    m.ok[v] = t < q1 && m.cc[v] < kc;
+    const int ll = m.ok[v] ? t >> KS : 0;
+    m.ll[v] = ll;
+    m.cnt[v] = cnt8[ll];
+    m.cw[v] = *reinterpret_cast<const uint2 *>(code + PK * ll);
+#pragma unroll
+    for (int u = 0; u < PK; ++u) m.pv[v][u] = pval[PK * ll + u];
+    m.dg[v] = ldiag[ll];
+    m.acc[v] = xs[m.ok[v] ? t : 0];
+  }
+}
+
+// The row arithmetic of the pairs: the four packed products in entry order
+// (padding products are +0.0 * +0.0 from the zero slot, which leave the
+// value unchanged, so no test is needed), then the row's overflow entries in
+// order (bounds clamped to [0, ne]), then the division; the result goes to
+// the pair's slot.
+template <typename T, int KT, int VPL>
+__device__ __forceinline__ void solve_pairs(Pairs<T, VPL> &m, T *xs, const int *lep,
+                                            const T *lval, const uint16_t *lcode, int ne) {
+  T xv[VPL][PK];
+#pragma unroll
+  for (int v = 0; v < VPL; ++v) {
+    const int cd[PK] = {(int)(m.cw[v].x & 0xffffu), (int)(m.cw[v].x >> 16),
+                        (int)(m.cw[v].y & 0xffffu), (int)(m.cw[v].y >> 16)};
+#pragma unroll
+    for (int u = 0; u < PK; ++u) xv[v][u] = xs[cd[u] * KT + m.cc[v]];
+  }
+#pragma unroll
+  for (int v = 0; v < VPL; ++v) {
+#pragma unroll
+    for (int u = 0; u < PK; ++u) m.acc[v] -= m.pv[v][u] * xv[v][u];
+    if (m.cnt[v] > PK)
+      for (int e = max(lep[m.ll[v]], 0), e1 = min(lep[m.ll[v] + 1], ne); e < e1; ++e)
+        m.acc[v] -= lval[e] * xs[lcode[e] * KT + m.cc[v]];
+    if (m.ok[v]) xs[m.t[v]] = m.acc[v] / m.dg[v];
+  }
+}
+
+// Keeps the compiler from moving LDS accesses across this point (the
+// hardware keeps one wave's LDS operations in order).
+__device__ __forceinline__ void lds_order() {
+  __builtin_amdgcn_wave_barrier();
+  asm volatile("" ::: "memory");
+}
+
+// The internal levels of a task, slice after slice.  Each lane solves VPL
+// (row, column) pairs at a time; the first pairs of the next slice are
+// loaded while this slice is solved, so only the dependency values sit on
+// the chain.  The wave's LDS operations complete in issue order, so the
+// next slice's reads see this slice's writes without a wait.  Slice bounds:
+// lane j holds lsl[j]; with SHORT (ns <= 62) every bound is read from that
+// register, otherwise those beyond lane 63 from LDS.  Bounds are clamped to
+// [0, nr].
+template <typename T, int KT, int VPL, bool SHORT>
+__device__ __forceinline__ void solve_slices(T *xs, const uint8_t *rb, const RecLayout &L,
+                                             int nr, int ns, int ne, int kc) {
+  const int lane = threadIdx.x;
+  const T *pval = reinterpret_cast<const T *>(rb + L.pval);
+  const T *ldiag = reinterpret_cast<const T *>(rb + L.diag);
+  const T *lval = reinterpret_cast<const T *>(rb + L.lval);
+  const uint16_t *code = reinterpret_cast<const uint16_t *>(rb + L.code);
+  const uint8_t *cnt8 = rb + L.cnt;
+  const int *lsl = reinterpret_cast<const int *>(rb + L.slices);
+  const int *lep = reinterpret_cast<const int *>(rb + L.lep);
+  const uint16_t *lcode = reinterpret_cast<const uint16_t *>(rb + L.lcode);
+  const int myb = lane <= ns ? lsl[lane] : 0;
+  auto bound = [&](int s) {
+    int v;
+    if constexpr (SHORT)
+      v = __builtin_amdgcn_readlane(myb, s);
+    else
+      v = s < 64 ? __builtin_amdgcn_readlane(myb, s) : lsl[s];
+    return min(max(v, 0), nr);
+  };
+  Pairs<T, VPL> cur, nxt;
+  int b0 = bound(0), b1 = ns > 0 ? bound(1) : 0;
+  if (ns > 0) load_pairs<T, KT, VPL>(cur, b0 * KT + lane, b1 * KT, kc, xs, cnt8, code, pval, ldiag);
+  for (int s = 0; s < ns; ++s) {
+    // after the last slice b2 == b1: the loads are harmless and unused
+    const int b2 = s + 1 < ns ? bound(s + 2) : b1;
+    load_pairs<T, KT, VPL>(nxt, b1 * KT + lane, b2 * KT, kc, xs, cnt8, code, pval, ldiag);
+    solve_pairs<T, KT, VPL>(cur, xs, lep, lval, lcode, ne);
+    for (int t0 = b0 * KT + lane + 64 * VPL; t0 < b1 * KT; t0 += 64 * VPL) {
+      Pairs<T, VPL> more;
+      load_pairs<T, KT, VPL>(more, t0, b1 * KT, kc, xs, cnt8, code, pval, ldiag);
+      solve_pairs<T, KT, VPL>(more, xs, lep, lval, lcode, ne);
+    }
+    lds_order();
+    cur = nxt;
+    b0 = b1;
+    b1 = b2;
+  }
+}
+
+__device__ __forceinline__ void load_head(const int *head, int r, int4 &h0, int4 &h1) {
+  const int4 *p = reinterpret_cast<const int4 *>(head) + 2 * (int64_t)r;
+  h0 = p[0];
+  h1 = p[1];
+}
+
+// One wave per task.  ES: bytes per direct global -> LDS load of the row
+// values (16 or 4), or 0 for strided columns.
+template <typename T, int KT, int ES>
+__global__ __launch_bounds__(64) void k_regions(RegionArgs<T> a) {
+  extern __shared__ __align__(16) uint8_t smem[];
+  constexpr int KS = log2c<KT>();
+  constexpr int VS = (int)sizeof(T);
+  // two values per lane in flight for tasks of 8 columns (their slices hold
+  // up to 2 (row, column) pairs per lane), one for narrower tasks
   constexpr int VPL = KT >= 8 ? 2 : 1;
-  const RegionLds L = region_lds(a.smax, a.hmax, a.emax, a.slmax, KT, sizeof(T));
-  T *xs = reinterpret_cast<T *>(smem + L.xs);        // (smax + hmax) * KT values
-  T *pval = reinterpret_cast<T *>(smem + L.pval);    // PK per row: packed values
-  T *lval = reinterpret_cast<T *>(smem + L.lval);    // emax overflow values
-  T *ldiag = reinterpret_cast<T *>(smem + L.ldiag);  // smax diagonals
-  int *pcode = reinterpret_cast<int *>(smem + L.pcode);  // PK per row: LDS slots
-  int *pcnt = reinterpret_cast<int *>(smem + L.pcnt);    // entries per row
-  int *lcode = reinterpret_cast<int *>(smem + L.lcode);  // emax overflow slots
-  int *lep = reinterpret_cast<int *>(smem + L.lep);      // smax + 1 overflow bounds
-  int *lsl = reinterpret_cast<int *>(smem + L.lsl);      // slmax + 1 slice bounds
-  int *lrow = reinterpret_cast<int *>(smem + L.lrow);    // smax row ids
-  int *lhalo = reinterpret_cast<int *>(smem + L.lhalo);  // hmax halo row ids
-  __shared__ int s_task;
+  T *xs = reinterpret_cast<T *>(smem);  // (smax + hmax) * KT values
+  uint8_t *rb = smem + a.xsb;           // record image
   const int lane = threadIdx.x;
   const int target = ld_dev(&a.state[ST_EPOCH]) + 1;
   const int ntasks = a.R * a.groups;
-  int cursor = 0;
-  for (;;) {
-    int task;
-    if constexpr (NT == 64) {
-      task = next_unit(a.state, 1, cursor, 0);
-    } else {
-      // the previous task's last block_sync ordered every read of s_task
-      if (lane == 0) s_task = atomicAdd(&a.state[ST_TICKET], 1);
-      __syncthreads();
-      task = s_task;
-    }
-    if (task >= ntasks) break;
+  // the zero slot, never written afterwards (rows and halo lie below it)
+  if (lane < KT) xs[(size_t)(a.smax + a.hmax) * KT + lane] = T(0);
+  int ticket = 0;
+  if (lane == 0) ticket = atomicAdd(&a.state[ST_TICKET], 1);
+  int task = __builtin_amdgcn_readlane(ticket, 0);
+  int4 h0 = make_int4(0, 0, 0, 0), h1 = h0;
+  if (task < ntasks) load_head(a.head, task / a.groups, h0, h1);
+  while (task < ntasks) {
     trace_mark(a.trace, a.trace_cap, task, 0);
-    const int r = task / a.groups, g = task - r * a.groups;
+    // the next ticket, in flight while this task runs
+    if (lane == 0) ticket = atomicAdd(&a.state[ST_TICKET], 1);
+    const int g = task % a.groups;
     const int c0 = g * KT;
     const int kc = a.k - c0 < KT ? a.k - c0 : KT;
-    const int base = a.rptr[r], nr = a.rptr[r + 1] - base;
-    const int e0 = a.eptr[base], ne = a.eptr[base + nr] - e0;
-    const int s0 = a.sptr[r], ns = a.sptr[r + 1] - s0 - 1;
-    const int h0 = a.hptr[r], nh = a.hptr[r + 1] - h0;
-    // 1. stage
-    stage<NT>(pcode, a.pcode + (size_t)PK * base, PK * nr);
-    stage<NT>(pval, a.pval + (size_t)PK * base, PK * nr);
-    stage<NT>(pcnt, a.pcnt + base, nr);
-    stage<NT>(lcode, a.ecode + e0, ne);
-    stage<NT>(lval, a.eval + e0, ne);
-    stage<NT>(lrow, a.rows + base, nr);
-    stage<NT>(ldiag, a.diag + base, nr);
-    stage<NT>(lsl, a.slices + s0, ns + 1);
-    stage<NT>(lhalo, a.halo + h0, nh);
-    for (int t = lane; t <= nr; t += NT) lep[t] = a.eptr[base + t] - e0;
-    block_sync<NT>();
-    stage_rows<NT, T, KT, false>(xs, a.B, lrow, nr, kc, c0, a.rs, a.cs);
-    // 2. inputs published (same column group of the regions read); one wave
-    // polls, the others wait at the barrier
-    if (lane < 64)
-      wait_units(a.dptr, a.dlist, r, a.tflags, target, a.state, a.timeout, a.groups, g);
-    if constexpr (NT > 64) __syncthreads();
+    const int nr = __builtin_amdgcn_readfirstlane(h0.x);
+    const int nh = __builtin_amdgcn_readfirstlane(h0.y);
+    const int ne = __builtin_amdgcn_readfirstlane(h0.z);
+    const int ns = __builtin_amdgcn_readfirstlane(h0.w);
+    const int nd = __builtin_amdgcn_readfirstlane(h1.x);
+    const RecLayout L = rec_layout(nr, nh, ne, ns, nd, VS);
+    const int *lrow = reinterpret_cast<const int *>(rb + L.rows);
+    const int *lhalo = reinterpret_cast<const int *>(rb + L.halo);
+    const int *ldep = reinterpret_cast<const int *>(rb + L.deps);
+    // 1. record, then the right-hand sides of its rows
+    dma_bytes(rb, a.rec + (size_t)__builtin_amdgcn_readfirstlane(h1.y) * 16,
+              __builtin_amdgcn_readfirstlane(h1.z) * 16);
+    vm_wait();
+    load_rows<T, KT, ES, false>(xs, a.B, lrow, nr, kc, c0, a.rs, a.cs, a.n);
+    // 2. inputs published (same column group of the regions read)
+    wait_list(ldep, nd, a.tflags, target, a.state, a.timeout, a.groups, g, a.R);
     trace_mark(a.trace, a.trace_cap, task, 1);
     // 3. halo
-    stage_rows<NT, T, KT, true>(xs + (size_t)a.smax * KT, a.X, lhalo, nh, kc, c0, a.rs,
-                                a.cs);
-    block_sync<NT>();
-    // 4. internal levels.  Per (row, column) pair: the row's packed slot
-    // codes and values (addresses known from the row alone, one round of LDS
-    // reads), then the dependency values, then the products in entry order
-    // (padding slots read slot 0 and are not accumulated), then the row's
-    // overflow entries in order, then the division.  VPL pairs per thread
-    // are in flight together.
-    for (int s = 0; s < ns; ++s) {
-      const int q1 = lsl[s + 1] * KT;
-      for (int t0 = lsl[s] * KT + lane; t0 < q1; t0 += NT * VPL) {
-        int ll[VPL], cc[VPL], cnt[VPL], cd[VPL][PK];
-        T acc[VPL], pv[VPL][PK], xv[VPL][PK];
-        bool ok[VPL];
-#pragma unroll
-        for (int v = 0; v < VPL; ++v) {
-          const int t = t0 + NT * v;
-          cc[v] = t & (KT - 1);
-          ok[v] = t < q1 && cc[v] < kc;
-          ll[v] = ok[v] ? t >> KS : 0;
-          cnt[v] = pcnt[ll[v]];
-#pragma unroll
-          for (int u = 0; u < PK; ++u) {
-            cd[v][u] = pcode[PK * ll[v] + u];
-            pv[v][u] = pval[PK * ll[v] + u];
-          }
-          acc[v] = ok[v] ? xs[t] : T(0);
-        }
-#pragma unroll
-        for (int v = 0; v < VPL; ++v)
-#pragma unroll
-          for (int u = 0; u < PK; ++u) xv[v][u] = xs[cd[v][u] * KT + cc[v]];
-#pragma unroll
-        for (int v = 0; v < VPL; ++v) {
-#pragma unroll
-          for (int u = 0; u < PK; ++u)
-            if (u < cnt[v]) acc[v] -= pv[v][u] * xv[v][u];
-          if (cnt[v] > PK)
-            for (int e = lep[ll[v]]; e < lep[ll[v] + 1]; ++e)
-              acc[v] -= lval[e] * xs[lcode[e] * KT + cc[v]];
-          if (ok[v]) xs[t0 + NT * v] = acc[v] / ldiag[ll[v]];
-        }
-      }
-      block_sync<NT>();
-    }
-    // 5. store; every wave waits for its own stores, then one thread
-    // publishes (the barrier also ends every read of this task's LDS)
-    for (int t = lane; t < nr * KT; t += NT) {
+    load_rows<T, KT, ES, true>(xs + (size_t)a.smax * KT, a.X, lhalo, nh, kc, c0, a.rs,
+                               a.cs, a.n);
+    vm_wait();
+    wave_lds_sync();
+    // 4. internal levels
+    if (ns <= 62)
+      solve_slices<T, KT, VPL, true>(xs, rb, L, nr, ns, ne, kc);
+    else
+      solve_slices<T, KT, VPL, false>(xs, rb, L, nr, ns, ne, kc);
+    wave_lds_sync();
+    // 5. store, fetch the next task's header, wait for both, publish
+    for (int t = lane; t < nr * KT; t += 64) {
       const int c = t & (KT - 1);
-      if (c < kc)
-        st_dev(&a.X[(int64_t)lrow[t >> KS] * a.rs + (int64_t)(c0 + c) * a.cs], xs[t]);
+      const int id = lrow[t >> KS];
+      if (c < kc && (unsigned)id < (unsigned)a.n)
+        st_dev(&a.X[(int64_t)id * a.rs + (int64_t)(c0 + c) * a.cs], xs[t]);
     }
+    const int next = __builtin_amdgcn_readlane(ticket, 0);
+    if (next < ntasks) load_head(a.head, next / a.groups, h0, h1);
     publish_fence();
-    if constexpr (NT > 64) __syncthreads();
     if (lane == 0) st_dev(&a.tflags[task], target);
     trace_mark(a.trace, a.trace_cap, task, 2);
+    task = next;
   }
   persistent_finish(a.state, a.total_waves);
 }
 
 template <typename T>
 __global__ void k_region_gather(int64_t m, const int *__restrict__ src,
-                                const T *__restrict__ from, T *to) {
+                                const int *__restrict__ dst, const T *__restrict__ from,
+                                T *to) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < m) to[t] = from[src[t]];
+  if (t < m) to[dst[t]] = from[src[t]];
 }
 
 template <typename T>
 int gather_one(symamd_handle h, symamd_plan p, RegionPlan &rp) {
-  const T *val = (const T *)p->d_val;
-  if (rp.entries > 0)
-    k_region_gather<T><<<(unsigned)((rp.entries + 255) / 256), 256, 0, h->stream>>>(
-        rp.entries, rp.d_esrc, val, (T *)rp.d_eval);
-  const int64_t packed = (int64_t)PK * p->n;
-  k_region_gather<T><<<(unsigned)((packed + 255) / 256), 256, 0, h->stream>>>(
-      packed, rp.d_psrc, val, (T *)rp.d_pval);
-  k_region_gather<T><<<(unsigned)((p->n + 255) / 256), 256, 0, h->stream>>>(
-      p->n, rp.d_dsrc, val, (T *)rp.d_diag);
+  if (rp.nvals > 0)
+    k_region_gather<T><<<(unsigned)((rp.nvals + 255) / 256), 256, 0, h->stream>>>(
+        rp.nvals, rp.d_vsrc, rp.d_vdst, (const T *)p->d_val, (T *)rp.d_rec);
   SYMAMD_HIP(hipGetLastError());
   return SYMAMD_OK;
 }
 
+// Bytes per direct global -> LDS load of a task's row values: 16 when every
+// row chunk (KT and, for the last group, k mod KT columns), the row stride
+// and the buffer are 16-byte aligned, else 4; 0 (staging through registers)
+// for strided columns.
+template <typename T, int KT>
+int load_bytes(const void *d_B, int k, int64_t rs, int64_t cs) {
+  if (k > 1 && cs != 1) return 0;
+  constexpr int vs = (int)sizeof(T);
+  const bool a16 = (KT * vs) % 16 == 0 && ((k % KT) * vs) % 16 == 0 && (rs * vs) % 16 == 0 &&
+                   reinterpret_cast<uintptr_t>(d_B) % 16 == 0;
+  return a16 ? 16 : 4;
+}
+
 // Tasks are taken by ticket, so the grid may be as large as the occupancy
 // estimate (a block that is not resident yet simply takes later tickets).
-// One wave per task: a four-wave block per task (NT = 256) did not complete
-// on the 32-column 128^3 solve and is not used until the cause is known.
-template <typename T, int KT>
-int region_launch(symamd_handle h, symamd_plan p, const RegionPlan &rp, void *d_B,
-                  int k, int64_t rs, int64_t cs, size_t lds) {
-  constexpr int NT = 64;
-  const void *kfn = (const void *)&k_regions<T, KT, NT>;
+template <typename T, int KT, int ES>
+int region_launch(symamd_handle h, symamd_plan p, const RegionPlan &rp, void *d_B, int k,
+                  int64_t rs, int64_t cs) {
+  const void *kfn = (const void *)&k_regions<T, KT, ES>;
+  const size_t lds = region_lds_bytes(rp, KT, (int)sizeof(T));
   int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, NT, lds) != hipSuccess ||
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, 64, lds) != hipSuccess ||
       per_cu < 1)
     per_cu = 1;
   const int groups = (k + KT - 1) / KT;
@@ -332,46 +435,43 @@ int region_launch(symamd_handle h, symamd_plan p, const RegionPlan &rp, void *d_
   a.cs = cs;
   a.B = (const T *)d_B;
   a.X = (T *)d_B;
-  a.rptr = rp.d_rptr;
-  a.rows = rp.d_rows;
-  a.sptr = rp.d_sptr;
-  a.slices = rp.d_slices;
-  a.hptr = rp.d_hptr;
-  a.halo = rp.d_halo;
-  a.dptr = rp.d_dptr;
-  a.dlist = rp.d_dlist;
-  a.eptr = rp.d_eptr;
-  a.ecode = rp.d_ecode;
-  a.pcnt = rp.d_pcnt;
-  a.pcode = rp.d_pcode;
-  a.pval = (const T *)rp.d_pval;
-  a.eval = (const T *)rp.d_eval;
-  a.diag = (const T *)rp.d_diag;
+  a.n = p->n;
+  a.rec = (const uint8_t *)rp.d_rec;
+  a.head = rp.d_rhead;
   a.tflags = rp.d_rflags;
   a.state = p->d_state;
   a.R = rp.R;
   a.groups = groups;
   a.smax = rp.smax;
   a.hmax = rp.hmax;
-  a.emax = rp.emax;
-  a.slmax = rp.slmax;
-  a.total_waves = (int)grid * (NT / 64);
+  a.xsb = xs_bytes(rp, KT, (int)sizeof(T));
+  a.total_waves = (int)grid;
   a.timeout = h->timeout_ticks();
   a.trace = h->trace;
   a.trace_cap = h->trace_cap;
   void *args[] = {&a};
-  SYMAMD_HIP(hipLaunchKernel(kfn, dim3((unsigned)grid), dim3(NT), args, lds, h->stream));
+  SYMAMD_HIP(hipLaunchKernel(kfn, dim3((unsigned)grid), dim3(64), args, lds, h->stream));
   return SYMAMD_OK;
 }
 
+template <typename T, int KT>
+int region_launch_es(symamd_handle h, symamd_plan p, const RegionPlan &rp, void *d_B, int k,
+                     int64_t rs, int64_t cs) {
+  const int es = load_bytes<T, KT>(d_B, k, rs, cs);
+  if (es == 0) return region_launch<T, KT, 0>(h, p, rp, d_B, k, rs, cs);
+  if constexpr ((KT * sizeof(T)) % 16 == 0)
+    if (es == 16) return region_launch<T, KT, 16>(h, p, rp, d_B, k, rs, cs);
+  return region_launch<T, KT, 4>(h, p, rp, d_B, k, rs, cs);
+}
+
 template <typename T>
-int region_dispatch(symamd_handle h, symamd_plan p, const RegionPlan &rp, void *d_B,
-                    int k, int64_t rs, int64_t cs, int kt, size_t lds) {
+int region_dispatch(symamd_handle h, symamd_plan p, const RegionPlan &rp, void *d_B, int k,
+                    int64_t rs, int64_t cs, int kt) {
   switch (kt) {
-    case 1: return region_launch<T, 1>(h, p, rp, d_B, k, rs, cs, lds);
-    case 2: return region_launch<T, 2>(h, p, rp, d_B, k, rs, cs, lds);
-    case 4: return region_launch<T, 4>(h, p, rp, d_B, k, rs, cs, lds);
-    default: return region_launch<T, 8>(h, p, rp, d_B, k, rs, cs, lds);
+    case 1: return region_launch_es<T, 1>(h, p, rp, d_B, k, rs, cs);
+    case 2: return region_launch_es<T, 2>(h, p, rp, d_B, k, rs, cs);
+    case 4: return region_launch_es<T, 4>(h, p, rp, d_B, k, rs, cs);
+    default: return region_launch_es<T, 8>(h, p, rp, d_B, k, rs, cs);
   }
 }
 
@@ -411,7 +511,7 @@ bool region_usable(symamd_handle h, symamd_plan p, int k) {
   if (k < 1 || k > KG_MAX * RegionPlan::MAX_GROUPS) return false;
   RegionPlan *rp = nullptr;
   if (plan_regions(h, p, region_cap(h, k), &rp) != SYMAMD_OK || rp->R == 0) return false;
-  const size_t vs = p->dtype == SYMAMD_F64 ? 8 : 4;
+  const int vs = p->dtype == SYMAMD_F64 ? 8 : 4;
   return region_lds_bytes(*rp, task_cols(k), vs) <= 64 * 1024;
 }
 
@@ -422,11 +522,8 @@ int launch_regions(symamd_handle h, symamd_plan p, void *d_B, int k, int64_t rs,
   RegionPlan *rp = nullptr;
   SYMAMD_TRY(plan_regions(h, p, region_cap(h, k), &rp));
   const int kt = task_cols(k);
-  const size_t vs = p->dtype == SYMAMD_F64 ? 8 : 4;
-  const size_t lds = region_lds_bytes(*rp, kt, vs);
-  if (p->dtype == SYMAMD_F64)
-    return region_dispatch<double>(h, p, *rp, d_B, k, rs, cs, kt, lds);
-  return region_dispatch<float>(h, p, *rp, d_B, k, rs, cs, kt, lds);
+  if (p->dtype == SYMAMD_F64) return region_dispatch<double>(h, p, *rp, d_B, k, rs, cs, kt);
+  return region_dispatch<float>(h, p, *rp, d_B, k, rs, cs, kt);
 }
 
 }  // namespace symamd

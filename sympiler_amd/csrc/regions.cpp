@@ -20,6 +20,7 @@
 #include <algorithm>
 #include <climits>
 #include <cstdint>
+#include <cstring>
 #include <numeric>
 #include <vector>
 
@@ -270,10 +271,7 @@ int plan_owners(symamd_handle h, symamd_plan p, UnitSet *us, int nw) {
 }
 
 void free_region_plan(RegionPlan &rp) {
-  void *ptrs[] = {rp.d_rptr, rp.d_rows, rp.d_sptr,  rp.d_slices, rp.d_hptr,
-                  rp.d_halo, rp.d_dptr, rp.d_dlist, rp.d_eptr,   rp.d_ecode,
-                  rp.d_eval, rp.d_esrc, rp.d_diag,  rp.d_dsrc,   rp.d_rflags,
-                  rp.d_pcnt, rp.d_pcode, rp.d_pval, rp.d_psrc};
+  void *ptrs[] = {rp.d_rec, rp.d_rhead, rp.d_vsrc, rp.d_vdst, rp.d_rflags};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   rp = RegionPlan{};
@@ -538,8 +536,8 @@ bool build_region_schedule(int n, int64_t nnz, int levels, const int *lvl_ptr,
   // slices, halos, entries, dependency lists (in ranks)
   std::vector<int> sptr(R + 1, 0), slices, hptr(R + 1, 0), halo, dptr(R + 1, 0), dlist;
   // entries: the first RegionPlan::PACK of every row packed per row (pcnt,
-  // pcode, psrc; padding slots read LDS slot 0 and the row's diagonal, and
-  // are never accumulated), the rest in the overflow list (eptr, ecode, esrc)
+  // pcode, psrc; padding slots read the all-zero LDS slot smax + hmax, and
+  // their values are zero), the rest in the overflow list (eptr, ecode, esrc)
   constexpr int PK = RegionPlan::PACK;
   std::vector<int> eptr(n + 1, 0), ecode, esrc, dsrc(n);
   std::vector<int> pcnt(n), pcode((size_t)PK * n, 0), psrc((size_t)PK * n, 0);
@@ -594,6 +592,8 @@ bool build_region_schedule(int n, int64_t nnz, int levels, const int *lvl_ptr,
     for (int e = gptr[g]; e < gptr[g + 1]; ++e) dlist.push_back(rank[glist[e]]);
     dptr[r + 1] = (int)dlist.size();
   }
+  for (int t = 0; t < n; ++t)
+    for (int u = pcnt[t]; u < PK; ++u) pcode[(size_t)PK * t + u] = smax + hmax;
   o.R = R;
   o.smax = smax;
   o.hmax = hmax;
@@ -614,6 +614,76 @@ bool build_region_schedule(int n, int64_t nnz, int levels, const int *lvl_ptr,
   o.pcnt.swap(pcnt);
   o.pcode.swap(pcode);
   o.psrc.swap(psrc);
+  return true;
+}
+
+bool pack_region_records(const HostRegions &o, int vs, HostRecords &out) {
+  constexpr int PK = RegionPlan::PACK;
+  const int R = o.R;
+  out.rhead.assign(8 * (size_t)R, 0);
+  out.recmax = 0;
+  int64_t units = 0;
+  for (int r = 0; r < R; ++r) {
+    const int a = o.rptr[r], nr = o.rptr[r + 1] - a;
+    const int nh = o.hptr[r + 1] - o.hptr[r];
+    const int ne = o.eptr[a + nr] - o.eptr[a];
+    const int ns = o.sptr[r + 1] - o.sptr[r] - 1;
+    const int nd = o.dptr[r + 1] - o.dptr[r];
+    const RecLayout L = rec_layout(nr, nh, ne, ns, nd, vs);
+    int *hd = &out.rhead[8 * (size_t)r];
+    hd[0] = nr;
+    hd[1] = nh;
+    hd[2] = ne;
+    hd[3] = ns;
+    hd[4] = nd;
+    hd[5] = (int)units;
+    hd[6] = L.bytes / 16;
+    out.recmax = std::max(out.recmax, L.bytes);
+    units += L.bytes / 16;
+    // value positions are counted in values of vs bytes
+    if (units > INT_MAX || units * 16 / vs > INT_MAX) return false;
+  }
+  out.bytes.assign((size_t)units * 16, 0);
+  out.vsrc.clear();
+  out.vdst.clear();
+  for (int r = 0; r < R; ++r) {
+    const int a = o.rptr[r], nr = o.rptr[r + 1] - a;
+    const int h0 = o.hptr[r], nh = o.hptr[r + 1] - h0;
+    const int e0 = o.eptr[a], ne = o.eptr[a + nr] - e0;
+    const int s0 = o.sptr[r], ns = o.sptr[r + 1] - s0 - 1;
+    const int d0 = o.dptr[r], nd = o.dptr[r + 1] - d0;
+    const RecLayout L = rec_layout(nr, nh, ne, ns, nd, vs);
+    uint8_t *rec = out.bytes.data() + (size_t)out.rhead[8 * (size_t)r + 5] * 16;
+    const int64_t vbase = (int64_t)out.rhead[8 * (size_t)r + 5] * 16 / vs;
+    auto put16 = [](uint8_t *p, int v) {
+      const uint16_t w = (uint16_t)v;
+      std::memcpy(p, &w, 2);
+    };
+    auto put32 = [](uint8_t *p, int v) { std::memcpy(p, &v, 4); };
+    for (int l = 0; l < nr; ++l) {
+      const int t = a + l;
+      for (int u = 0; u < PK; ++u) {
+        put16(rec + L.code + 2 * (PK * l + u), o.pcode[(size_t)PK * t + u]);
+        if (u >= o.pcnt[t]) continue;  // padding: value stays +0.0
+        out.vsrc.push_back(o.psrc[(size_t)PK * t + u]);
+        out.vdst.push_back((int)(vbase + (L.pval / vs) + PK * l + u));
+      }
+      out.vsrc.push_back(o.dsrc[t]);
+      out.vdst.push_back((int)(vbase + L.diag / vs + l));
+      rec[L.cnt + l] = (uint8_t)std::min(o.pcnt[t], PK + 1);
+      put32(rec + L.rows + 4 * l, o.rows[t]);
+      if (ne > 0) put32(rec + L.lep + 4 * l, o.eptr[t] - e0);
+    }
+    if (ne > 0) put32(rec + L.lep + 4 * nr, ne);
+    for (int e = 0; e < ne; ++e) {
+      put16(rec + L.lcode + 2 * e, o.ecode[e0 + e]);
+      out.vsrc.push_back(o.esrc[e0 + e]);
+      out.vdst.push_back((int)(vbase + L.lval / vs + e));
+    }
+    for (int q = 0; q < nh; ++q) put32(rec + L.halo + 4 * q, o.halo[h0 + q]);
+    for (int s = 0; s <= ns; ++s) put32(rec + L.slices + 4 * s, o.slices[s0 + s]);
+    for (int d = 0; d < nd; ++d) put32(rec + L.deps + 4 * d, o.dlist[d0 + d]);
+  }
   return true;
 }
 
@@ -643,29 +713,27 @@ int plan_regions(symamd_handle h, symamd_plan p, int cap, RegionPlan **out) {
     *out = &p->rplans.emplace(cap, rp).first->second;
     return SYMAMD_OK;
   }
+  const int vs = p->dtype == SYMAMD_F64 ? 8 : 4;
+  HostRecords hr;
+  if (o.smax + o.hmax >= REGION_SLOTS_MAX || !pack_region_records(o, vs, hr)) {
+    *out = &p->rplans.emplace(cap, rp).first->second;  // R = 0: not usable
+    return SYMAMD_OK;
+  }
   rp.R = o.R;
   rp.smax = o.smax;
   rp.hmax = o.hmax;
   rp.emax = o.emax;
   rp.slmax = o.slmax;
-  rp.entries = (int64_t)o.ecode.size();
-  const size_t vs = p->dtype == SYMAMD_F64 ? 8 : 4;
+  rp.recmax = hr.recmax;
+  rp.nvals = (int64_t)hr.vsrc.size();
   int rc = SYMAMD_OK;
-  if ((rc = upload(&rp.d_rptr, o.rptr)) || (rc = upload(&rp.d_rows, o.rows)) ||
-      (rc = upload(&rp.d_sptr, o.sptr)) || (rc = upload(&rp.d_slices, o.slices)) ||
-      (rc = upload(&rp.d_hptr, o.hptr)) || (rc = upload(&rp.d_halo, o.halo)) ||
-      (rc = upload(&rp.d_dptr, o.dptr)) || (rc = upload(&rp.d_dlist, o.dlist)) ||
-      (rc = upload(&rp.d_eptr, o.eptr)) || (rc = upload(&rp.d_ecode, o.ecode)) ||
-      (rc = upload(&rp.d_esrc, o.esrc)) || (rc = upload(&rp.d_dsrc, o.dsrc)) ||
-      (rc = upload(&rp.d_pcnt, o.pcnt)) || (rc = upload(&rp.d_pcode, o.pcode)) ||
-      (rc = upload(&rp.d_psrc, o.psrc))) {
+  if ((rc = upload(reinterpret_cast<uint8_t **>(&rp.d_rec), hr.bytes)) ||
+      (rc = upload(&rp.d_rhead, hr.rhead)) || (rc = upload(&rp.d_vsrc, hr.vsrc)) ||
+      (rc = upload(&rp.d_vdst, hr.vdst))) {
     free_region_plan(rp);
     return rc;
   }
-  if (hipMalloc(&rp.d_eval, vs * (o.ecode.empty() ? 1 : o.ecode.size())) != hipSuccess ||
-      hipMalloc(&rp.d_pval, vs * RegionPlan::PACK * n) != hipSuccess ||
-      hipMalloc(&rp.d_diag, vs * n) != hipSuccess ||
-      hipMalloc(&rp.d_rflags, sizeof(int) * rp.R * RegionPlan::MAX_GROUPS) != hipSuccess ||
+  if (hipMalloc(&rp.d_rflags, sizeof(int) * rp.R * RegionPlan::MAX_GROUPS) != hipSuccess ||
       hipMemset(rp.d_rflags, 0, sizeof(int) * rp.R * RegionPlan::MAX_GROUPS) != hipSuccess) {
     free_region_plan(rp);
     return fail(SYMAMD_ERR_ALLOC, "region schedule arrays");
