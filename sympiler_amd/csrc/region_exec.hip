@@ -183,66 +183,87 @@ __device__ __forceinline__ void load_rows(T *xs, const T *M, const int *rows, in
     stage_rows<T, KT, COH>(xs, M, rows, nr, kc, c0, rs, cs, n);
 }
 
-// Operands of a lane's VPL (row, column) pairs that do not depend on the
-// slice being solved: position, entry count, slot codes and values, the
-// diagonal and the right-hand side (a pair's slot is written only when its
-// own slice is solved).  A pair outside the slice reads row 0 and stores
-// nothing.
-template <typename T, int VPL>
+// CPL consecutive columns of one row, one LDS access (16-byte aligned for
+// CPL = 2: columns are taken in even pairs of a row of KT slots).
+template <typename T, int CPL>
+__device__ __forceinline__ void ld_cols(const T *p, T (&out)[CPL]) {
+  if constexpr (CPL == 1) {
+    out[0] = p[0];
+  } else {
+    static_assert(CPL == 2, "one or two columns per lane");
+    if constexpr (sizeof(T) == 8) {
+      const double2 v = *reinterpret_cast<const double2 *>(p);
+      out[0] = v.x;
+      out[1] = v.y;
+    } else {
+      const float2 v = *reinterpret_cast<const float2 *>(p);
+      out[0] = v.x;
+      out[1] = v.y;
+    }
+  }
+}
+
+// A lane's share of a slice: CPL consecutive columns of one row.  Operands
+// that do not depend on the slice being solved: position, entry count, slot
+// codes and values and the diagonal (shared by the columns), and the
+// right-hand sides (a row's slots are written only when its own slice is
+// solved).  A lane outside the slice reads row 0 and stores nothing.
+template <typename T, int CPL>
 struct Pairs {
-  int t[VPL], cc[VPL], cnt[VPL], ll[VPL];
-  uint2 cw[VPL];  // four uint16 slot codes, unpacked where they are used
-  bool ok[VPL];
-  T pv[VPL][PK], dg[VPL], acc[VPL];
+  int t, cnt, ll;
+  uint2 cw;  // four uint16 slot codes, unpacked where they are used
+  bool ok;
+  T pv[PK], dg, acc[CPL];
 };
 
-template <typename T, int KT, int VPL>
-__device__ __forceinline__ void load_pairs(Pairs<T, VPL> &m, int t0, int q1, int kc,
+template <typename T, int KT, int CPL>
+__device__ __forceinline__ void load_pairs(Pairs<T, CPL> &m, int t, int q1, int kc,
                                            const T *xs, const uint8_t *cnt8,
                                            const uint16_t *code, const T *pval,
                                            const T *ldiag) {
   constexpr int KS = log2c<KT>();
+  m.t = t;
+  m.ok = t < q1 && (t & (KT - 1)) < kc;
+  const int ll = m.ok ? t >> KS : 0;
+  m.ll = ll;
+  m.cnt = cnt8[ll];
+  m.cw = *reinterpret_cast<const uint2 *>(code + PK * ll);
 #pragma unroll
-  for (int v = 0; v < VPL; ++v) {
-    const int t = t0 + 64 * v;
-    m.t[v] = t;
-    m.cc[v] = t & (KT - 1);
-    m.ok[v] = t < q1 && m.cc[v] < kc;
-    const int ll = m.ok[v] ? t >> KS : 0;
-    m.ll[v] = ll;
-    m.cnt[v] = cnt8[ll];
-    m.cw[v] = *reinterpret_cast<const uint2 *>(code + PK * ll);
-#pragma unroll
-    for (int u = 0; u < PK; ++u) m.pv[v][u] = pval[PK * ll + u];
-    m.dg[v] = ldiag[ll];
-    m.acc[v] = xs[m.ok[v] ? t : 0];
-  }
+  for (int u = 0; u < PK; ++u) m.pv[u] = pval[PK * ll + u];
+  m.dg = ldiag[ll];
+  ld_cols<T, CPL>(xs + (m.ok ? t : 0), m.acc);
 }
 
-// The row arithmetic of the pairs: the four packed products in entry order
+// The row arithmetic, per column: the four packed products in entry order
 // (padding products are +0.0 * +0.0 from the zero slot, which leave the
 // value unchanged, so no test is needed), then the row's overflow entries in
-// order (bounds clamped to [0, ne]), then the division; the result goes to
-// the pair's slot.
-template <typename T, int KT, int VPL>
-__device__ __forceinline__ void solve_pairs(Pairs<T, VPL> &m, T *xs, const int *lep,
-                                            const T *lval, const uint16_t *lcode, int ne) {
-  T xv[VPL][PK];
+// order (bounds clamped to [0, ne]), then the division; results go to the
+// row's slots of the columns that exist.
+template <typename T, int KT, int CPL>
+__device__ __forceinline__ void solve_pairs(Pairs<T, CPL> &m, T *xs, const int *lep,
+                                            const T *lval, const uint16_t *lcode, int ne,
+                                            int kc) {
+  const int cc = m.t & (KT - 1);
+  const int cd[PK] = {(int)(m.cw.x & 0xffffu), (int)(m.cw.x >> 16), (int)(m.cw.y & 0xffffu),
+                      (int)(m.cw.y >> 16)};
+  T xv[PK][CPL];
 #pragma unroll
-  for (int v = 0; v < VPL; ++v) {
-    const int cd[PK] = {(int)(m.cw[v].x & 0xffffu), (int)(m.cw[v].x >> 16),
-                        (int)(m.cw[v].y & 0xffffu), (int)(m.cw[v].y >> 16)};
+  for (int u = 0; u < PK; ++u) ld_cols<T, CPL>(xs + cd[u] * KT + cc, xv[u]);
 #pragma unroll
-    for (int u = 0; u < PK; ++u) xv[v][u] = xs[cd[u] * KT + m.cc[v]];
-  }
+  for (int u = 0; u < PK; ++u)
 #pragma unroll
-  for (int v = 0; v < VPL; ++v) {
+    for (int j = 0; j < CPL; ++j) m.acc[j] -= m.pv[u] * xv[u][j];
+  if (m.cnt > PK)
+    for (int e = max(lep[m.ll], 0), e1 = min(lep[m.ll + 1], ne); e < e1; ++e) {
+      T xe[CPL];
+      ld_cols<T, CPL>(xs + lcode[e] * KT + cc, xe);
 #pragma unroll
-    for (int u = 0; u < PK; ++u) m.acc[v] -= m.pv[v][u] * xv[v][u];
-    if (m.cnt[v] > PK)
-      for (int e = max(lep[m.ll[v]], 0), e1 = min(lep[m.ll[v] + 1], ne); e < e1; ++e)
-        m.acc[v] -= lval[e] * xs[lcode[e] * KT + m.cc[v]];
-    if (m.ok[v]) xs[m.t[v]] = m.acc[v] / m.dg[v];
+      for (int j = 0; j < CPL; ++j) m.acc[j] -= lval[e] * xe[j];
+    }
+  if (m.ok) {
+#pragma unroll
+    for (int j = 0; j < CPL; ++j)
+      if (cc + j < kc) xs[m.t + j] = m.acc[j] / m.dg;
   }
 }
 
@@ -253,15 +274,15 @@ __device__ __forceinline__ void lds_order() {
   asm volatile("" ::: "memory");
 }
 
-// The internal levels of a task, slice after slice.  Each lane solves VPL
-// (row, column) pairs at a time; the first pairs of the next slice are
+// The internal levels of a task, slice after slice.  Each lane solves CPL
+// columns of one row at a time; the lane's share of the next slice is
 // loaded while this slice is solved, so only the dependency values sit on
 // the chain.  The wave's LDS operations complete in issue order, so the
 // next slice's reads see this slice's writes without a wait.  Slice bounds:
 // lane j holds lsl[j]; with SHORT (ns <= 62) every bound is read from that
 // register, otherwise those beyond lane 63 from LDS.  Bounds are clamped to
 // [0, nr].
-template <typename T, int KT, int VPL, bool SHORT>
+template <typename T, int KT, int CPL, bool SHORT>
 __device__ __forceinline__ void solve_slices(T *xs, const uint8_t *rb, const RecLayout &L,
                                              int nr, int ns, int ne, int kc) {
   const int lane = threadIdx.x;
@@ -282,18 +303,19 @@ __device__ __forceinline__ void solve_slices(T *xs, const uint8_t *rb, const Rec
       v = s < 64 ? __builtin_amdgcn_readlane(myb, s) : lsl[s];
     return min(max(v, 0), nr);
   };
-  Pairs<T, VPL> cur, nxt;
+  const int off = lane * CPL;
+  Pairs<T, CPL> cur, nxt;
   int b0 = bound(0), b1 = ns > 0 ? bound(1) : 0;
-  if (ns > 0) load_pairs<T, KT, VPL>(cur, b0 * KT + lane, b1 * KT, kc, xs, cnt8, code, pval, ldiag);
+  if (ns > 0) load_pairs<T, KT, CPL>(cur, b0 * KT + off, b1 * KT, kc, xs, cnt8, code, pval, ldiag);
   for (int s = 0; s < ns; ++s) {
     // after the last slice b2 == b1: the loads are harmless and unused
     const int b2 = s + 1 < ns ? bound(s + 2) : b1;
-    load_pairs<T, KT, VPL>(nxt, b1 * KT + lane, b2 * KT, kc, xs, cnt8, code, pval, ldiag);
-    solve_pairs<T, KT, VPL>(cur, xs, lep, lval, lcode, ne);
-    for (int t0 = b0 * KT + lane + 64 * VPL; t0 < b1 * KT; t0 += 64 * VPL) {
-      Pairs<T, VPL> more;
-      load_pairs<T, KT, VPL>(more, t0, b1 * KT, kc, xs, cnt8, code, pval, ldiag);
-      solve_pairs<T, KT, VPL>(more, xs, lep, lval, lcode, ne);
+    load_pairs<T, KT, CPL>(nxt, b1 * KT + off, b2 * KT, kc, xs, cnt8, code, pval, ldiag);
+    solve_pairs<T, KT, CPL>(cur, xs, lep, lval, lcode, ne, kc);
+    for (int t = b0 * KT + off + 64 * CPL; t < b1 * KT; t += 64 * CPL) {
+      Pairs<T, CPL> more;
+      load_pairs<T, KT, CPL>(more, t, b1 * KT, kc, xs, cnt8, code, pval, ldiag);
+      solve_pairs<T, KT, CPL>(more, xs, lep, lval, lcode, ne, kc);
     }
     lds_order();
     cur = nxt;
@@ -315,9 +337,9 @@ __global__ __launch_bounds__(64) void k_regions(RegionArgs<T> a) {
   extern __shared__ __align__(16) uint8_t smem[];
   constexpr int KS = log2c<KT>();
   constexpr int VS = (int)sizeof(T);
-  // two values per lane in flight for tasks of 8 columns (their slices hold
-  // up to 2 (row, column) pairs per lane), one for narrower tasks
-  constexpr int VPL = KT >= 8 ? 2 : 1;
+  // columns per lane in the slices: two when a task has two or more (the
+  // row's operands are then read once per two columns)
+  constexpr int CPL = KT >= 2 ? 2 : 1;
   T *xs = reinterpret_cast<T *>(smem);  // (smax + hmax) * KT values
   uint8_t *rb = smem + a.xsb;           // record image
   const int lane = threadIdx.x;
@@ -361,9 +383,9 @@ __global__ __launch_bounds__(64) void k_regions(RegionArgs<T> a) {
     wave_lds_sync();
     // 4. internal levels
     if (ns <= 62)
-      solve_slices<T, KT, VPL, true>(xs, rb, L, nr, ns, ne, kc);
+      solve_slices<T, KT, CPL, true>(xs, rb, L, nr, ns, ne, kc);
     else
-      solve_slices<T, KT, VPL, false>(xs, rb, L, nr, ns, ne, kc);
+      solve_slices<T, KT, CPL, false>(xs, rb, L, nr, ns, ne, kc);
     wave_lds_sync();
     // 5. store, fetch the next task's header, wait for both, publish
     for (int t = lane; t < nr * KT; t += 64) {
@@ -477,13 +499,12 @@ int region_dispatch(symamd_handle h, symamd_plan p, const RegionPlan &rp, void *
 
 }  // namespace
 
-// Default cap: 512 rows for one or two columns per task, 128 for eight
-// (measured on the 128^3 IC(0) factor: k=1 is fastest at 512 -- 1024 and 256
-// are slower -- and k=32 at 128, where one wave's LDS still allows seven
-// waves per CU).
+// Default cap: 512 rows for up to four columns per task, 256 for eight
+// (measured on the 128^3 IC(0) factor: k=1 is fastest at 512 -- 256 and 128
+// are slower -- and k=32 at 256, ahead of 128; at 512 its LDS exceeds 64 KiB).
 int region_cap(symamd_handle h, int k) {
   if (h->region_cap > 0) return h->region_cap;
-  const int cap = pow2_floor(1024 / task_cols(k));
+  const int cap = pow2_floor(2048 / task_cols(k));
   return cap > 512 ? 512 : (cap < 16 ? 16 : cap);
 }
 
