@@ -430,6 +430,10 @@ int symamd_create(symamd_handle *h, void *stream) {
                             x->device) != hipSuccess ||
       x->num_cus < 1)
     x->num_cus = 256;
+  if (hipDeviceGetAttribute(&x->lds_per_block, hipDeviceAttributeMaxSharedMemoryPerBlock,
+                            x->device) != hipSuccess ||
+      x->lds_per_block < 65536)
+    x->lds_per_block = 65536;
   *h = x;
   return SYMAMD_OK;
 }

@@ -232,6 +232,7 @@ struct symamd_handle_s {
   hipStream_t stream = nullptr;
   int device = 0;
   int num_cus = 0;
+  int lds_per_block = 65536;  // LDS bytes one workgroup may allocate
   double timeout_ms = 20000.0;
   int force_tickets = 0;  // persistent kernels take units from a ticket counter
   uint64_t *trace = nullptr;  // per-unit timestamps of sync-free launches

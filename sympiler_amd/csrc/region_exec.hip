@@ -32,8 +32,8 @@
 // running wave with all its inputs published (a wave holds at most one
 // further ticket, always above the one it is solving): the grid cannot
 // deadlock, whether or not all its blocks are resident (these blocks carry
-// up to 64 KiB of LDS each), and the watchdog of wait_list bounds every
-// wait.
+// up to the device's LDS per workgroup, 160 KiB on gfx950), and the watchdog
+// of wait_list bounds every wait.
 #include "device.h"
 
 namespace symamd {
@@ -42,6 +42,7 @@ namespace {
 
 constexpr int KG_MAX = 8;   // right-hand side columns per task
 constexpr int STAGE_U = 8;  // loads per lane in flight while staging (strided path)
+constexpr int STORE_U = 8;  // result stores per lane issued after one LDS wait
 constexpr int PK = RegionPlan::PACK;
 static_assert(PK == 4, "the slice loop unpacks four uint16 slot codes per row");
 // cache policy bits of a direct global -> LDS load: 16 = sc1, the bits of an
@@ -234,21 +235,28 @@ __device__ __forceinline__ void load_pairs(Pairs<T, CPL> &m, int t, int q1, int 
   ld_cols<T, CPL>(xs + (m.ok ? t : 0), m.acc);
 }
 
+// The dependency values of the pairs (rows of earlier slices or the halo,
+// final when the slice starts): the four packed slots of each column.
+template <typename T, int KT, int CPL>
+__device__ __forceinline__ void read_deps(const Pairs<T, CPL> &m, const T *xs,
+                                          T (&xv)[PK][CPL]) {
+  const int cc = m.t & (KT - 1);
+  const int cd[PK] = {(int)(m.cw.x & 0xffffu), (int)(m.cw.x >> 16), (int)(m.cw.y & 0xffffu),
+                      (int)(m.cw.y >> 16)};
+#pragma unroll
+  for (int u = 0; u < PK; ++u) ld_cols<T, CPL>(xs + cd[u] * KT + cc, xv[u]);
+}
+
 // The row arithmetic, per column: the four packed products in entry order
 // (padding products are +0.0 * +0.0 from the zero slot, which leave the
 // value unchanged, so no test is needed), then the row's overflow entries in
 // order (bounds clamped to [0, ne]), then the division; results go to the
 // row's slots of the columns that exist.
 template <typename T, int KT, int CPL>
-__device__ __forceinline__ void solve_pairs(Pairs<T, CPL> &m, T *xs, const int *lep,
-                                            const T *lval, const uint16_t *lcode, int ne,
-                                            int kc) {
+__device__ __forceinline__ void finish_pairs(Pairs<T, CPL> &m, const T (&xv)[PK][CPL], T *xs,
+                                             const int *lep, const T *lval,
+                                             const uint16_t *lcode, int ne, int kc) {
   const int cc = m.t & (KT - 1);
-  const int cd[PK] = {(int)(m.cw.x & 0xffffu), (int)(m.cw.x >> 16), (int)(m.cw.y & 0xffffu),
-                      (int)(m.cw.y >> 16)};
-  T xv[PK][CPL];
-#pragma unroll
-  for (int u = 0; u < PK; ++u) ld_cols<T, CPL>(xs + cd[u] * KT + cc, xv[u]);
 #pragma unroll
   for (int u = 0; u < PK; ++u)
 #pragma unroll
@@ -267,6 +275,19 @@ __device__ __forceinline__ void solve_pairs(Pairs<T, CPL> &m, T *xs, const int *
   }
 }
 
+// P groups of pairs of one slice: every dependency read first (rows of one
+// slice never read each other), then the arithmetic.
+template <typename T, int KT, int CPL, int P>
+__device__ __forceinline__ void solve_pairs(Pairs<T, CPL> (&m)[P], T *xs, const int *lep,
+                                            const T *lval, const uint16_t *lcode, int ne,
+                                            int kc) {
+  T xv[P][PK][CPL];
+#pragma unroll
+  for (int p = 0; p < P; ++p) read_deps<T, KT, CPL>(m[p], xs, xv[p]);
+#pragma unroll
+  for (int p = 0; p < P; ++p) finish_pairs<T, KT, CPL>(m[p], xv[p], xs, lep, lval, lcode, ne, kc);
+}
+
 // Keeps the compiler from moving LDS accesses across this point (the
 // hardware keeps one wave's LDS operations in order).
 __device__ __forceinline__ void lds_order() {
@@ -275,14 +296,14 @@ __device__ __forceinline__ void lds_order() {
 }
 
 // The internal levels of a task, slice after slice.  Each lane solves CPL
-// columns of one row at a time; the lane's share of the next slice is
-// loaded while this slice is solved, so only the dependency values sit on
-// the chain.  The wave's LDS operations complete in issue order, so the
-// next slice's reads see this slice's writes without a wait.  Slice bounds:
-// lane j holds lsl[j]; with SHORT (ns <= 62) every bound is read from that
-// register, otherwise those beyond lane 63 from LDS.  Bounds are clamped to
-// [0, nr].
-template <typename T, int KT, int CPL, bool SHORT>
+// columns of one row at a time, P such groups of the next slice loaded while
+// this slice is solved, so only the dependency values sit on the chain (a
+// slice with more than 64 * P groups loads the rest as it goes).  The
+// wave's LDS operations complete in issue order, so the next slice's reads
+// see this slice's writes without a wait.  Slice bounds: lane j holds
+// lsl[j]; with SHORT (ns <= 62) every bound is read from that register,
+// otherwise those beyond lane 63 from LDS.  Bounds are clamped to [0, nr].
+template <typename T, int KT, int CPL, int P, bool SHORT>
 __device__ __forceinline__ void solve_slices(T *xs, const uint8_t *rb, const RecLayout &L,
                                              int nr, int ns, int ne, int kc) {
   const int lane = threadIdx.x;
@@ -304,21 +325,31 @@ __device__ __forceinline__ void solve_slices(T *xs, const uint8_t *rb, const Rec
     return min(max(v, 0), nr);
   };
   const int off = lane * CPL;
-  Pairs<T, CPL> cur, nxt;
+  constexpr int STEP = 64 * CPL;  // slots per group of the wave
+  Pairs<T, CPL> cur[P], nxt[P];
   int b0 = bound(0), b1 = ns > 0 ? bound(1) : 0;
-  if (ns > 0) load_pairs<T, KT, CPL>(cur, b0 * KT + off, b1 * KT, kc, xs, cnt8, code, pval, ldiag);
+  if (ns > 0) {
+#pragma unroll
+    for (int p = 0; p < P; ++p)
+      load_pairs<T, KT, CPL>(cur[p], b0 * KT + off + p * STEP, b1 * KT, kc, xs, cnt8, code,
+                             pval, ldiag);
+  }
   for (int s = 0; s < ns; ++s) {
     // after the last slice b2 == b1: the loads are harmless and unused
     const int b2 = s + 1 < ns ? bound(s + 2) : b1;
-    load_pairs<T, KT, CPL>(nxt, b1 * KT + off, b2 * KT, kc, xs, cnt8, code, pval, ldiag);
-    solve_pairs<T, KT, CPL>(cur, xs, lep, lval, lcode, ne, kc);
-    for (int t = b0 * KT + off + 64 * CPL; t < b1 * KT; t += 64 * CPL) {
-      Pairs<T, CPL> more;
-      load_pairs<T, KT, CPL>(more, t, b1 * KT, kc, xs, cnt8, code, pval, ldiag);
-      solve_pairs<T, KT, CPL>(more, xs, lep, lval, lcode, ne, kc);
+#pragma unroll
+    for (int p = 0; p < P; ++p)
+      load_pairs<T, KT, CPL>(nxt[p], b1 * KT + off + p * STEP, b2 * KT, kc, xs, cnt8, code,
+                             pval, ldiag);
+    solve_pairs<T, KT, CPL, P>(cur, xs, lep, lval, lcode, ne, kc);
+    for (int t = b0 * KT + off + P * STEP; t < b1 * KT; t += STEP) {
+      Pairs<T, CPL> more[1];
+      load_pairs<T, KT, CPL>(more[0], t, b1 * KT, kc, xs, cnt8, code, pval, ldiag);
+      solve_pairs<T, KT, CPL, 1>(more, xs, lep, lval, lcode, ne, kc);
     }
     lds_order();
-    cur = nxt;
+#pragma unroll
+    for (int p = 0; p < P; ++p) cur[p] = nxt[p];
     b0 = b1;
     b1 = b2;
   }
@@ -340,6 +371,10 @@ __global__ __launch_bounds__(64) void k_regions(RegionArgs<T> a) {
   // columns per lane in the slices: two when a task has two or more (the
   // row's operands are then read once per two columns)
   constexpr int CPL = KT >= 2 ? 2 : 1;
+  // groups of pairs per lane prefetched per slice (two, for the eight-column
+  // tasks of the 128^3 IC(0) solve, measured 10% slower: the second group's
+  // loads are wasted on most slices and load the LDS of the CU)
+  constexpr int P = 1;
   T *xs = reinterpret_cast<T *>(smem);  // (smax + hmax) * KT values
   uint8_t *rb = smem + a.xsb;           // record image
   const int lane = threadIdx.x;
@@ -383,16 +418,28 @@ __global__ __launch_bounds__(64) void k_regions(RegionArgs<T> a) {
     wave_lds_sync();
     // 4. internal levels
     if (ns <= 62)
-      solve_slices<T, KT, CPL, true>(xs, rb, L, nr, ns, ne, kc);
+      solve_slices<T, KT, CPL, P, true>(xs, rb, L, nr, ns, ne, kc);
     else
-      solve_slices<T, KT, CPL, false>(xs, rb, L, nr, ns, ne, kc);
+      solve_slices<T, KT, CPL, P, false>(xs, rb, L, nr, ns, ne, kc);
     wave_lds_sync();
-    // 5. store, fetch the next task's header, wait for both, publish
-    for (int t = lane; t < nr * KT; t += 64) {
-      const int c = t & (KT - 1);
-      const int id = lrow[t >> KS];
-      if (c < kc && (unsigned)id < (unsigned)a.n)
-        st_dev(&a.X[(int64_t)id * a.rs + (int64_t)(c0 + c) * a.cs], xs[t]);
+    // 5. store (row ids and values of STORE_U stores read first, one wait),
+    // fetch the next task's header, wait for both, publish
+    for (int t0 = lane; t0 < nr * KT; t0 += 64 * STORE_U) {
+      int id[STORE_U];
+      T v[STORE_U];
+#pragma unroll
+      for (int u = 0; u < STORE_U; ++u) {
+        const int t = t0 + 64 * u;
+        const bool in = t < nr * KT && (t & (KT - 1)) < kc;
+        id[u] = in ? lrow[t >> KS] : -1;
+        v[u] = xs[in ? t : 0];
+      }
+#pragma unroll
+      for (int u = 0; u < STORE_U; ++u) {
+        const int c = (t0 + 64 * u) & (KT - 1);
+        if ((unsigned)id[u] < (unsigned)a.n)
+          st_dev(&a.X[(int64_t)id[u] * a.rs + (int64_t)(c0 + c) * a.cs], v[u]);
+      }
     }
     const int next = __builtin_amdgcn_readlane(ticket, 0);
     if (next < ntasks) load_head(a.head, next / a.groups, h0, h1);
@@ -441,6 +488,8 @@ int region_launch(symamd_handle h, symamd_plan p, const RegionPlan &rp, void *d_
                   int64_t rs, int64_t cs) {
   const void *kfn = (const void *)&k_regions<T, KT, ES>;
   const size_t lds = region_lds_bytes(rp, KT, (int)sizeof(T));
+  if (lds > 65536)
+    SYMAMD_HIP(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   int per_cu = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, 64, lds) != hipSuccess ||
       per_cu < 1)
@@ -533,7 +582,7 @@ bool region_usable(symamd_handle h, symamd_plan p, int k) {
   RegionPlan *rp = nullptr;
   if (plan_regions(h, p, region_cap(h, k), &rp) != SYMAMD_OK || rp->R == 0) return false;
   const int vs = p->dtype == SYMAMD_F64 ? 8 : 4;
-  return region_lds_bytes(*rp, task_cols(k), vs) <= 64 * 1024;
+  return region_lds_bytes(*rp, task_cols(k), vs) <= (size_t)h->lds_per_block;
 }
 
 int launch_regions(symamd_handle h, symamd_plan p, void *d_B, int k, int64_t rs,
