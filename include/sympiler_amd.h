@@ -107,11 +107,15 @@ int symamd_set_timeout_ms(symamd_handle h, double ms);
  *   SYMAMD_OPT_REGION_CAP (read at solve time): region size cap of the
  *     SYMAMD_EXEC_REGIONS schedule (0, the default, picks it from k: 512
  *     rows for k <= 4, 256 for k >= 8); each cap's schedule is built once
- *     per plan. */
+ *     per plan.
+ *   SYMAMD_OPT_REGION_WAVES (read at solve time): wavefronts per task of
+ *     the SYMAMD_EXEC_REGIONS schedule, 1, 2 or 4 for tasks of eight
+ *     columns (k >= 8; narrower tasks use one); 0, the default, picks it.
+ *     Results are identical either way. */
 enum { SYMAMD_OPT_TICKETS = 1, SYMAMD_OPT_TRACE_PTR = 2, SYMAMD_OPT_TRACE_CAP = 3,
        SYMAMD_OPT_MAX_WAVES = 4, SYMAMD_OPT_COLMAJOR_DIRECT = 5, SYMAMD_OPT_MARKS = 6,
        SYMAMD_OPT_REGIONS = 7, SYMAMD_OPT_REGION_ROWS = 8, SYMAMD_OPT_OWNERS = 9,
-       SYMAMD_OPT_REGION_CAP = 10 };
+       SYMAMD_OPT_REGION_CAP = 10, SYMAMD_OPT_REGION_WAVES = 11 };
 int symamd_set_option(symamd_handle h, int option, int64_t value);
 const char *symamd_last_error(void);
 /* Library version string. */

@@ -177,6 +177,11 @@ class Handle:
         """Region size cap of the region executor (0: chosen from k)."""
         _check(lib().symamd_set_option(self._h, 10, int(rows)))
 
+    def set_region_waves(self, waves: int):
+        """Wavefronts per task of the region executor: 1, 2 or 4 for tasks
+        of eight columns (0: chosen from k)."""
+        _check(lib().symamd_set_option(self._h, 11, int(waves)))
+
     def set_owners(self, on: bool):
         """Run plans with regions on the owner schedule (True, default) or
         deal their units like other plans (False)."""

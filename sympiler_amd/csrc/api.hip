@@ -471,6 +471,11 @@ int symamd_set_option(symamd_handle h, int option, int64_t value) {
     case SYMAMD_OPT_OWNERS:
       h->owners = value != 0;
       return SYMAMD_OK;
+    case SYMAMD_OPT_REGION_WAVES:
+      if (value != 0 && value != 1 && value != 2 && value != 4)
+        return fail(SYMAMD_ERR_INVALID_ARG, "region waves: 0, 1, 2 or 4");
+      h->region_waves = (int)value;
+      return SYMAMD_OK;
     case SYMAMD_OPT_REGION_CAP:
       if (value != 0 && (value < 2 || value > 4096))
         return fail(SYMAMD_ERR_INVALID_ARG, "region cap: 0 or 2..4096");

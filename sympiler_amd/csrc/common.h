@@ -244,6 +244,7 @@ struct symamd_handle_s {
   int region_rows = 512;    // region size cap
   int owners = 1;           // solve plans with regions on the owner schedule
   int region_cap = 0;       // region executor: region size cap (0: by k)
+  int region_waves = 0;     // waves per region task (0: chosen from k)
   uint64_t timeout_ticks() const { return (uint64_t)(timeout_ms * 1.0e5); }
 };
 

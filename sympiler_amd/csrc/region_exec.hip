@@ -95,11 +95,13 @@ __device__ __forceinline__ void vm_wait() { asm volatile("s_waitcnt vmcnt(0)" ::
 
 // LDS dst[0 .. bytes) = global src[0 .. bytes) (both 16-byte aligned, bytes
 // a multiple of 16): 16 bytes per lane per instruction, 1 KiB per
-// instruction, every instruction in flight together; vm_wait() completes
-// them.
-__device__ __forceinline__ void dma_bytes(uint8_t *dst, const uint8_t *src, int bytes) {
-  const int lane = threadIdx.x;
-  for (int o = 0; o < bytes; o += 1024)
+// instruction, every instruction in flight together; wave `wid` of NW takes
+// every NW-th KiB.  vm_wait() completes them.
+template <int NW>
+__device__ __forceinline__ void dma_bytes(uint8_t *dst, const uint8_t *src, int bytes,
+                                          int wid) {
+  const int lane = threadIdx.x & 63;
+  for (int o = wid * 1024; o < bytes; o += NW * 1024)
     if (o + 16 * lane < bytes)
       __builtin_amdgcn_global_load_lds((gmem_void *)(src + o + 16 * lane),
                                        (lds_void *)(dst + o), 16, 0, 0);
@@ -110,18 +112,19 @@ __device__ __forceinline__ void dma_bytes(uint8_t *dst, const uint8_t *src, int 
 // ES bytes per lane by direct global -> LDS loads, 64/LPR rows per
 // instruction.  The launch picks ES so that kc*sizeof(T), rs*sizeof(T) and
 // the buffer are ES-aligned.  `rows` lies in LDS (ids clamped below n).
-template <typename T, int KT, int ES, int AUX>
+// Wave `wid` of NW takes every NW-th batch of UB instructions.
+template <typename T, int KT, int ES, int AUX, int NW>
 __device__ __forceinline__ void dma_rows(T *xs, const T *M, const int *rows, int nr, int kc,
-                                         int c0, int64_t rs, int n) {
+                                         int c0, int64_t rs, int n, int wid) {
   constexpr int RB = KT * (int)sizeof(T);
   static_assert((ES == 16 || ES == 4) && RB % ES == 0, "row bytes: a multiple of the load size");
   constexpr int LPR = RB / ES;   // lanes per row
   constexpr int RPI = 64 / LPR;  // rows per instruction
   constexpr int UB = 8;          // instructions per batch: row ids read first
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & 63;
   const int lr = lane / LPR, part = lane - lr * LPR;
   const bool live = (part + 1) * ES <= kc * (int)sizeof(T);
-  for (int b0 = 0; b0 < nr; b0 += RPI * UB) {
+  for (int b0 = wid * RPI * UB; b0 < nr; b0 += NW * RPI * UB) {
     int rid[UB];
 #pragma unroll
     for (int u = 0; u < UB; ++u) {
@@ -146,18 +149,18 @@ __device__ __forceinline__ void dma_rows(T *xs, const T *M, const int *rows, int
 
 // Strided columns (column-major blocks): xs[l*KT + c] = M[rows[l]*rs +
 // (c0 + c)*cs] through registers, STAGE_U loads in flight; coherent loads
-// when COH.
-template <typename T, int KT, bool COH>
+// when COH; NT threads.
+template <typename T, int KT, bool COH, int NT>
 __device__ __forceinline__ void stage_rows(T *xs, const T *M, const int *rows, int nr,
                                            int kc, int c0, int64_t rs, int64_t cs, int n) {
   constexpr int KS = log2c<KT>();
-  const int lane = threadIdx.x;
+  const int tid = threadIdx.x;
   const int count = nr * KT;
-  for (int t0 = lane; t0 < count; t0 += 64 * STAGE_U) {
+  for (int t0 = tid; t0 < count; t0 += NT * STAGE_U) {
     T v[STAGE_U];
 #pragma unroll
     for (int u = 0; u < STAGE_U; ++u) {
-      const int t = t0 + 64 * u;
+      const int t = t0 + NT * u;
       const int c = t & (KT - 1);
       v[u] = T(0);
       if (t < count && c < kc) {
@@ -169,19 +172,32 @@ __device__ __forceinline__ void stage_rows(T *xs, const T *M, const int *rows, i
     }
 #pragma unroll
     for (int u = 0; u < STAGE_U; ++u) {
-      const int t = t0 + 64 * u;
+      const int t = t0 + NT * u;
       if (t < count) xs[t] = v[u];
     }
   }
 }
 
-template <typename T, int KT, int ES, bool COH>
+template <typename T, int KT, int ES, bool COH, int NW>
 __device__ __forceinline__ void load_rows(T *xs, const T *M, const int *rows, int nr, int kc,
-                                          int c0, int64_t rs, int64_t cs, int n) {
+                                          int c0, int64_t rs, int64_t cs, int n, int wid) {
   if constexpr (ES > 0)
-    dma_rows<T, KT, ES, COH ? AUX_COH : AUX_PLAIN>(xs, M, rows, nr, kc, c0, rs, n);
+    dma_rows<T, KT, ES, COH ? AUX_COH : AUX_PLAIN, NW>(xs, M, rows, nr, kc, c0, rs, n, wid);
   else
-    stage_rows<T, KT, COH>(xs, M, rows, nr, kc, c0, rs, cs, n);
+    stage_rows<T, KT, COH, 64 * NW>(xs, M, rows, nr, kc, c0, rs, cs, n);
+}
+
+// LDS writes of the task's waves are visible to all of them: one wave keeps
+// its LDS operations in order (only the compiler must not reorder); several
+// meet at a workgroup barrier.
+template <int NW>
+__device__ __forceinline__ void task_sync() {
+  if constexpr (NW == 1) {
+    __builtin_amdgcn_wave_barrier();
+    asm volatile("" ::: "memory");
+  } else {
+    __syncthreads();
+  }
 }
 
 // CPL consecutive columns of one row, one LDS access (16-byte aligned for
@@ -288,25 +304,19 @@ __device__ __forceinline__ void solve_pairs(Pairs<T, CPL> (&m)[P], T *xs, const 
   for (int p = 0; p < P; ++p) finish_pairs<T, KT, CPL>(m[p], xv[p], xs, lep, lval, lcode, ne, kc);
 }
 
-// Keeps the compiler from moving LDS accesses across this point (the
-// hardware keeps one wave's LDS operations in order).
-__device__ __forceinline__ void lds_order() {
-  __builtin_amdgcn_wave_barrier();
-  asm volatile("" ::: "memory");
-}
-
-// The internal levels of a task, slice after slice.  Each lane solves CPL
-// columns of one row at a time, P such groups of the next slice loaded while
-// this slice is solved, so only the dependency values sit on the chain (a
-// slice with more than 64 * P groups loads the rest as it goes).  The
-// wave's LDS operations complete in issue order, so the next slice's reads
-// see this slice's writes without a wait.  Slice bounds: lane j holds
+// The internal levels of a task, slice after slice.  Each lane of the NW
+// waves solves CPL columns of one row at a time, P such groups of the next
+// slice loaded while this slice is solved, so only the dependency values sit
+// on the chain (a slice with more than 64 * NW * P groups loads the rest as
+// it goes).  One wave's LDS operations complete in issue order, so with one
+// wave the next slice's reads see this slice's writes without a wait;
+// several waves meet at a barrier after each slice.  Slice bounds: lane j holds
 // lsl[j]; with SHORT (ns <= 62) every bound is read from that register,
 // otherwise those beyond lane 63 from LDS.  Bounds are clamped to [0, nr].
-template <typename T, int KT, int CPL, int P, bool SHORT>
+template <typename T, int KT, int CPL, int P, bool SHORT, int NW>
 __device__ __forceinline__ void solve_slices(T *xs, const uint8_t *rb, const RecLayout &L,
                                              int nr, int ns, int ne, int kc) {
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & 63;
   const T *pval = reinterpret_cast<const T *>(rb + L.pval);
   const T *ldiag = reinterpret_cast<const T *>(rb + L.diag);
   const T *lval = reinterpret_cast<const T *>(rb + L.lval);
@@ -324,8 +334,8 @@ __device__ __forceinline__ void solve_slices(T *xs, const uint8_t *rb, const Rec
       v = s < 64 ? __builtin_amdgcn_readlane(myb, s) : lsl[s];
     return min(max(v, 0), nr);
   };
-  const int off = lane * CPL;
-  constexpr int STEP = 64 * CPL;  // slots per group of the wave
+  const int off = (int)threadIdx.x * CPL;
+  constexpr int STEP = 64 * NW * CPL;  // slots per group of the task's waves
   Pairs<T, CPL> cur[P], nxt[P];
   int b0 = bound(0), b1 = ns > 0 ? bound(1) : 0;
   if (ns > 0) {
@@ -347,7 +357,7 @@ __device__ __forceinline__ void solve_slices(T *xs, const uint8_t *rb, const Rec
       load_pairs<T, KT, CPL>(more[0], t, b1 * KT, kc, xs, cnt8, code, pval, ldiag);
       solve_pairs<T, KT, CPL, 1>(more, xs, lep, lval, lcode, ne, kc);
     }
-    lds_order();
+    task_sync<NW>();
 #pragma unroll
     for (int p = 0; p < P; ++p) cur[p] = nxt[p];
     b0 = b1;
@@ -361,11 +371,15 @@ __device__ __forceinline__ void load_head(const int *head, int r, int4 &h0, int4
   h1 = p[1];
 }
 
-// One wave per task.  ES: bytes per direct global -> LDS load of the row
-// values (16 or 4), or 0 for strided columns.
-template <typename T, int KT, int ES>
-__global__ __launch_bounds__(64) void k_regions(RegionArgs<T> a) {
+// NW waves (one workgroup) per task.  ES: bytes per direct global -> LDS
+// load of the row values (16 or 4), or 0 for strided columns.  With several
+// waves every branch that reaches a barrier depends only on the task, which
+// all of them share; wave 0 alone waits for the inputs (its wait is bounded
+// by the watchdog) while the others wait at the next barrier.
+template <typename T, int KT, int ES, int NW>
+__global__ __launch_bounds__(64 * NW) void k_regions(RegionArgs<T> a) {
   extern __shared__ __align__(16) uint8_t smem[];
+  constexpr int NT = 64 * NW;
   constexpr int KS = log2c<KT>();
   constexpr int VS = (int)sizeof(T);
   // columns per lane in the slices: two when a task has two or more (the
@@ -375,22 +389,32 @@ __global__ __launch_bounds__(64) void k_regions(RegionArgs<T> a) {
   // tasks of the 128^3 IC(0) solve, measured 10% slower: the second group's
   // loads are wasted on most slices and load the LDS of the CU)
   constexpr int P = 1;
-  T *xs = reinterpret_cast<T *>(smem);  // (smax + hmax) * KT values
+  __shared__ int s_next[9];             // next task and its header (NW > 1)
+  T *xs = reinterpret_cast<T *>(smem);  // (smax + hmax + 1) * KT values
   uint8_t *rb = smem + a.xsb;           // record image
-  const int lane = threadIdx.x;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wid = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int target = ld_dev(&a.state[ST_EPOCH]) + 1;
   const int ntasks = a.R * a.groups;
   // the zero slot, never written afterwards (rows and halo lie below it)
-  if (lane < KT) xs[(size_t)(a.smax + a.hmax) * KT + lane] = T(0);
+  if (tid < KT) xs[(size_t)(a.smax + a.hmax) * KT + tid] = T(0);
   int ticket = 0;
-  if (lane == 0) ticket = atomicAdd(&a.state[ST_TICKET], 1);
-  int task = __builtin_amdgcn_readlane(ticket, 0);
+  if (tid == 0) ticket = atomicAdd(&a.state[ST_TICKET], 1);
+  int task;
+  if constexpr (NW == 1) {
+    task = __builtin_amdgcn_readlane(ticket, 0);
+  } else {
+    if (tid == 0) s_next[0] = ticket;
+    __syncthreads();
+    task = s_next[0];
+  }
   int4 h0 = make_int4(0, 0, 0, 0), h1 = h0;
   if (task < ntasks) load_head(a.head, task / a.groups, h0, h1);
   while (task < ntasks) {
-    trace_mark(a.trace, a.trace_cap, task, 0);
+    if (wid == 0) trace_mark(a.trace, a.trace_cap, task, 0);
     // the next ticket, in flight while this task runs
-    if (lane == 0) ticket = atomicAdd(&a.state[ST_TICKET], 1);
+    if (tid == 0) ticket = atomicAdd(&a.state[ST_TICKET], 1);
     const int g = task % a.groups;
     const int c0 = g * KT;
     const int kc = a.k - c0 < KT ? a.k - c0 : KT;
@@ -404,48 +428,69 @@ __global__ __launch_bounds__(64) void k_regions(RegionArgs<T> a) {
     const int *lhalo = reinterpret_cast<const int *>(rb + L.halo);
     const int *ldep = reinterpret_cast<const int *>(rb + L.deps);
     // 1. record, then the right-hand sides of its rows
-    dma_bytes(rb, a.rec + (size_t)__builtin_amdgcn_readfirstlane(h1.y) * 16,
-              __builtin_amdgcn_readfirstlane(h1.z) * 16);
+    dma_bytes<NW>(rb, a.rec + (size_t)__builtin_amdgcn_readfirstlane(h1.y) * 16,
+                  __builtin_amdgcn_readfirstlane(h1.z) * 16, wid);
     vm_wait();
-    load_rows<T, KT, ES, false>(xs, a.B, lrow, nr, kc, c0, a.rs, a.cs, a.n);
+    if constexpr (NW > 1) __syncthreads();
+    load_rows<T, KT, ES, false, NW>(xs, a.B, lrow, nr, kc, c0, a.rs, a.cs, a.n, wid);
     // 2. inputs published (same column group of the regions read)
-    wait_list(ldep, nd, a.tflags, target, a.state, a.timeout, a.groups, g, a.R);
-    trace_mark(a.trace, a.trace_cap, task, 1);
+    if (wid == 0) wait_list(ldep, nd, a.tflags, target, a.state, a.timeout, a.groups, g, a.R);
+    if constexpr (NW > 1) __syncthreads();
+    if (wid == 0) trace_mark(a.trace, a.trace_cap, task, 1);
     // 3. halo
-    load_rows<T, KT, ES, true>(xs + (size_t)a.smax * KT, a.X, lhalo, nh, kc, c0, a.rs,
-                               a.cs, a.n);
+    load_rows<T, KT, ES, true, NW>(xs + (size_t)a.smax * KT, a.X, lhalo, nh, kc, c0, a.rs,
+                                   a.cs, a.n, wid);
     vm_wait();
-    wave_lds_sync();
-    // 4. internal levels
-    if (ns <= 62)
-      solve_slices<T, KT, CPL, P, true>(xs, rb, L, nr, ns, ne, kc);
+    if constexpr (NW == 1)
+      wave_lds_sync();
     else
-      solve_slices<T, KT, CPL, P, false>(xs, rb, L, nr, ns, ne, kc);
-    wave_lds_sync();
+      __syncthreads();
+    // 4. internal levels (each slice ends with task_sync)
+    if (ns <= 62)
+      solve_slices<T, KT, CPL, P, true, NW>(xs, rb, L, nr, ns, ne, kc);
+    else
+      solve_slices<T, KT, CPL, P, false, NW>(xs, rb, L, nr, ns, ne, kc);
+    if constexpr (NW == 1) wave_lds_sync();
     // 5. store (row ids and values of STORE_U stores read first, one wait),
     // fetch the next task's header, wait for both, publish
-    for (int t0 = lane; t0 < nr * KT; t0 += 64 * STORE_U) {
+    for (int t0 = tid; t0 < nr * KT; t0 += NT * STORE_U) {
       int id[STORE_U];
       T v[STORE_U];
 #pragma unroll
       for (int u = 0; u < STORE_U; ++u) {
-        const int t = t0 + 64 * u;
+        const int t = t0 + NT * u;
         const bool in = t < nr * KT && (t & (KT - 1)) < kc;
         id[u] = in ? lrow[t >> KS] : -1;
         v[u] = xs[in ? t : 0];
       }
 #pragma unroll
       for (int u = 0; u < STORE_U; ++u) {
-        const int c = (t0 + 64 * u) & (KT - 1);
+        const int c = (t0 + NT * u) & (KT - 1);
         if ((unsigned)id[u] < (unsigned)a.n)
           st_dev(&a.X[(int64_t)id[u] * a.rs + (int64_t)(c0 + c) * a.cs], v[u]);
       }
     }
-    const int next = __builtin_amdgcn_readlane(ticket, 0);
-    if (next < ntasks) load_head(a.head, next / a.groups, h0, h1);
-    publish_fence();
-    if (lane == 0) st_dev(&a.tflags[task], target);
-    trace_mark(a.trace, a.trace_cap, task, 2);
+    int next;
+    if constexpr (NW == 1) {
+      next = __builtin_amdgcn_readlane(ticket, 0);
+      if (next < ntasks) load_head(a.head, next / a.groups, h0, h1);
+      publish_fence();
+    } else {
+      // wave 0 fetches the next task and its header into LDS; every wave
+      // waits for its stores; then one thread publishes
+      if (wid == 0) {
+        const int nx = __builtin_amdgcn_readlane(ticket, 0);
+        if (lane < 8) s_next[1 + lane] = nx < ntasks ? a.head[8 * (nx / a.groups) + lane] : 0;
+        if (lane == 0) s_next[0] = nx;
+      }
+      publish_fence();
+      __syncthreads();
+      next = s_next[0];
+      h0 = make_int4(s_next[1], s_next[2], s_next[3], s_next[4]);
+      h1 = make_int4(s_next[5], s_next[6], s_next[7], s_next[8]);
+    }
+    if (tid == 0) st_dev(&a.tflags[task], target);
+    if (wid == 0) trace_mark(a.trace, a.trace_cap, task, 2);
     task = next;
   }
   persistent_finish(a.state, a.total_waves);
@@ -483,15 +528,15 @@ int load_bytes(const void *d_B, int k, int64_t rs, int64_t cs) {
 
 // Tasks are taken by ticket, so the grid may be as large as the occupancy
 // estimate (a block that is not resident yet simply takes later tickets).
-template <typename T, int KT, int ES>
+template <typename T, int KT, int ES, int NW>
 int region_launch(symamd_handle h, symamd_plan p, const RegionPlan &rp, void *d_B, int k,
                   int64_t rs, int64_t cs) {
-  const void *kfn = (const void *)&k_regions<T, KT, ES>;
+  const void *kfn = (const void *)&k_regions<T, KT, ES, NW>;
   const size_t lds = region_lds_bytes(rp, KT, (int)sizeof(T));
   if (lds > 65536)
     SYMAMD_HIP(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, 64, lds) != hipSuccess ||
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, 64 * NW, lds) != hipSuccess ||
       per_cu < 1)
     per_cu = 1;
   const int groups = (k + KT - 1) / KT;
@@ -516,45 +561,57 @@ int region_launch(symamd_handle h, symamd_plan p, const RegionPlan &rp, void *d_
   a.smax = rp.smax;
   a.hmax = rp.hmax;
   a.xsb = xs_bytes(rp, KT, (int)sizeof(T));
-  a.total_waves = (int)grid;
+  a.total_waves = (int)grid * NW;
   a.timeout = h->timeout_ticks();
   a.trace = h->trace;
   a.trace_cap = h->trace_cap;
   void *args[] = {&a};
-  SYMAMD_HIP(hipLaunchKernel(kfn, dim3((unsigned)grid), dim3(64), args, lds, h->stream));
+  SYMAMD_HIP(hipLaunchKernel(kfn, dim3((unsigned)grid), dim3(64 * NW), args, lds, h->stream));
   return SYMAMD_OK;
 }
 
-template <typename T, int KT>
+template <typename T, int KT, int NW>
 int region_launch_es(symamd_handle h, symamd_plan p, const RegionPlan &rp, void *d_B, int k,
                      int64_t rs, int64_t cs) {
   const int es = load_bytes<T, KT>(d_B, k, rs, cs);
-  if (es == 0) return region_launch<T, KT, 0>(h, p, rp, d_B, k, rs, cs);
+  if (es == 0) return region_launch<T, KT, 0, NW>(h, p, rp, d_B, k, rs, cs);
   if constexpr ((KT * sizeof(T)) % 16 == 0)
-    if (es == 16) return region_launch<T, KT, 16>(h, p, rp, d_B, k, rs, cs);
-  return region_launch<T, KT, 4>(h, p, rp, d_B, k, rs, cs);
+    if (es == 16) return region_launch<T, KT, 16, NW>(h, p, rp, d_B, k, rs, cs);
+  return region_launch<T, KT, 4, NW>(h, p, rp, d_B, k, rs, cs);
 }
 
 template <typename T>
 int region_dispatch(symamd_handle h, symamd_plan p, const RegionPlan &rp, void *d_B, int k,
-                    int64_t rs, int64_t cs, int kt) {
+                    int64_t rs, int64_t cs, int kt, int nw) {
   switch (kt) {
-    case 1: return region_launch_es<T, 1>(h, p, rp, d_B, k, rs, cs);
-    case 2: return region_launch_es<T, 2>(h, p, rp, d_B, k, rs, cs);
-    case 4: return region_launch_es<T, 4>(h, p, rp, d_B, k, rs, cs);
-    default: return region_launch_es<T, 8>(h, p, rp, d_B, k, rs, cs);
+    case 1: return region_launch_es<T, 1, 1>(h, p, rp, d_B, k, rs, cs);
+    case 2: return region_launch_es<T, 2, 1>(h, p, rp, d_B, k, rs, cs);
+    case 4: return region_launch_es<T, 4, 1>(h, p, rp, d_B, k, rs, cs);
+    default:
+      if (nw == 4) return region_launch_es<T, 8, 4>(h, p, rp, d_B, k, rs, cs);
+      if (nw == 2) return region_launch_es<T, 8, 2>(h, p, rp, d_B, k, rs, cs);
+      return region_launch_es<T, 8, 1>(h, p, rp, d_B, k, rs, cs);
   }
+}
+
+// Waves per task: several only for tasks of eight columns (whose slices
+// need several passes of one wave); four by default (measured on the 128^3
+// IC(0) factor at k = 32: 0.84 ms with four waves at cap 512, 0.87 with two
+// at cap 256, 0.99 with one at cap 256).
+int region_waves(symamd_handle h, int k) {
+  if (task_cols(k) < KG_MAX) return 1;
+  return h->region_waves > 0 ? h->region_waves : 4;
 }
 
 }  // namespace
 
-// Default cap: 512 rows for up to four columns per task, 256 for eight
-// (measured on the 128^3 IC(0) factor: k=1 is fastest at 512 -- 256 and 128
-// are slower -- and k=32 at 256, ahead of 128; at 512 its LDS exceeds 64 KiB).
+// Default cap: 512 rows, or 256 for tasks of eight columns run by one wave
+// (measured on the 128^3 IC(0) factor: k=1 is fastest at 512 -- 1024 and
+// 256 are slower; k=32 with four waves per task at 512 -- 1024 is slower --
+// and with one wave at 256).
 int region_cap(symamd_handle h, int k) {
   if (h->region_cap > 0) return h->region_cap;
-  const int cap = pow2_floor(2048 / task_cols(k));
-  return cap > 512 ? 512 : (cap < 16 ? 16 : cap);
+  return task_cols(k) == KG_MAX && region_waves(h, k) == 1 ? 256 : 512;
 }
 
 bool region_preferred(symamd_handle h, symamd_plan p, int k, int layout) {
@@ -591,9 +648,9 @@ int launch_regions(symamd_handle h, symamd_plan p, void *d_B, int k, int64_t rs,
     return fail(SYMAMD_ERR_NOT_SUPPORTED, "no region schedule for this matrix and k");
   RegionPlan *rp = nullptr;
   SYMAMD_TRY(plan_regions(h, p, region_cap(h, k), &rp));
-  const int kt = task_cols(k);
-  if (p->dtype == SYMAMD_F64) return region_dispatch<double>(h, p, *rp, d_B, k, rs, cs, kt);
-  return region_dispatch<float>(h, p, *rp, d_B, k, rs, cs, kt);
+  const int kt = task_cols(k), nw = region_waves(h, k);
+  if (p->dtype == SYMAMD_F64) return region_dispatch<double>(h, p, *rp, d_B, k, rs, cs, kt, nw);
+  return region_dispatch<float>(h, p, *rp, d_B, k, rs, cs, kt, nw);
 }
 
 }  // namespace symamd

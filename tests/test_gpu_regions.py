@@ -154,6 +154,26 @@ def test_region_executor_grids(dev, k, layout):
         assert bitwise_equal(X.T if layout == "row" else X, X_ref)
 
 
+@pytest.mark.parametrize("waves", [2, 4])
+def test_region_executor_multiwave(dev, waves):
+    # tasks of eight columns solved by 2 or 4 waves (one workgroup) per task:
+    # full and partial column groups, direct and staged (column-major) loads,
+    # the default cap and 512
+    h = sa.Handle(timeout_ms=5000.0)
+    h.set_region_waves(waves)
+    for Lp, Li, Lx in (W.ic0_lap3d(24, 0.01), W.lap2d_tril(150, 97)):
+        n = len(Lp) - 1
+        plan = sa.Plan(*to_dev(Lp, Li, Lx, device=dev), handle=h)
+        for k, cap, layout in ((8, 0, "row"), (32, 512, "row"), (13, 0, "row"), (8, 0, "col")):
+            h.set_region_cap(cap)
+            B = W.uniform(60 + k, 0, n * k).reshape(k, n)
+            X_ref = oracle.sptrsm_csr(Lp, Li, Lx, B)
+            Bl = np.ascontiguousarray(B.T) if layout == "row" else B
+            for _ in range(2):  # second solve: next epoch of the task tags
+                X = _solve_regions(plan, Bl, dev, layout)
+                assert bitwise_equal(X.T if layout == "row" else X, X_ref), (k, cap, layout)
+
+
 def test_region_executor_irregular_falls_back(dev):
     # few entries inside regions and large halos: the region schedule does
     # not fit one wave's LDS and the solve runs on the sync-free executor
