@@ -10,7 +10,7 @@ re-initialises x before every timed run, demo/FusionDemo.cpp:59-62).
 
 Reported beside the headline (in "detail"): the single-RHS plain level-set
 solve (one launch per level), the same launches replayed from a hipGraph,
-the single-RHS sync-free solve, the fused adjoint (gb = L^-T gx and grad over
+the single-RHS sync-free solve, the adjoint (gb = L^-T gx and grad over
 L's pattern), and the column-major 32-RHS solve.  "cpu_baseline" times the
 reference's OpenMP level-set solve (oracle/_ref, the reference's own source
 compiled for this host; the C restatement in oracle/ when that is absent) on
@@ -317,16 +317,20 @@ def main():
         gx = xs.clone()
         gb = torch.empty_like(gx)
         gL = torch.empty(nnz, dtype=torch.float64, device=dev)
-        wa, ea = timed_steps(lambda: plan.adjoint(xs, gx, gb, gL, executor="syncfree"),
-                             lambda: None, 10, 2, stream, torch)
-        plan.check()
         ab = W.algorithmic_bytes_adjoint(n, nnz)
-        detail["adjoint"] = {"ms": round(1e3 * float(np.mean(wa)), 4),
-                             "kernel_ms": round(float(np.mean(ea)), 4),
-                             "rhs_solves_per_s": round(1.0 / float(np.mean(wa)), 1),
-                             "gbs": round(ab / (np.mean(ea) * 1e-3) / 1e9, 1),
-                             "vs_plain_levelset": round(1.0 / float(np.mean(wa)) / plain, 2)}
-        log(f"adjoint: {detail['adjoint']}")
+        # the automatic choice (the transposed region schedule on this matrix),
+        # then the sync-free adjoint for comparison; both include grad-L
+        for key, ex in (("adjoint", "auto"), ("adjoint_syncfree", "syncfree")):
+            wa, ea = timed_steps(lambda: plan.adjoint(xs, gx, gb, gL, executor=ex),
+                                 lambda: None, 10, 2, stream, torch)
+            plan.check()
+            detail[key] = {"executor": ex,
+                           "ms": round(1e3 * float(np.mean(wa)), 4),
+                           "kernel_ms": round(float(np.mean(ea)), 4),
+                           "rhs_solves_per_s": round(1.0 / float(np.mean(wa)), 1),
+                           "gbs": round(ab / (np.mean(ea) * 1e-3) / 1e9, 1),
+                           "vs_plain_levelset": round(1.0 / float(np.mean(wa)) / plain, 2)}
+            log(f"{key}: {detail[key]}")
 
         # the headline block on the region executor
         wr, er = timed_steps(lambda: plan.solve(X, layout=args.layout, executor="regions"),

@@ -55,6 +55,7 @@ void free_plan_arrays(symamd_plan p) {
       if (q) (void)hipFree(q);
   }
   for (auto &kv : p->rplans) free_region_plan(kv.second);
+  for (auto &kv : p->rplans_t) free_region_plan(kv.second);
   for (auto &kv : p->gcache) {
     if (kv.second.second) (void)hipGraphExecDestroy(kv.second.second);
     if (kv.second.first) (void)hipGraphDestroy(kv.second.first);
@@ -700,9 +701,14 @@ int symamd_adjoint(symamd_handle h, symamd_plan plan, const void *d_x,
     return fail(SYMAMD_ERR_INVALID_ARG, "symamd_adjoint: bad arguments");
   if (exec < SYMAMD_EXEC_AUTO || exec > SYMAMD_EXEC_REGIONS)
     return fail(SYMAMD_ERR_INVALID_ARG, "unknown executor %d", (int)exec);
+  // the region executor solves L^T on the transposed region schedule; a
+  // forced choice without a usable schedule runs sync-free, like forward
+  if (exec == SYMAMD_EXEC_REGIONS ? region_adjoint_usable(h, plan)
+                                  : exec == SYMAMD_EXEC_AUTO && region_adjoint_preferred(h, plan))
+    return launch_regions_adjoint(h, plan, d_x, d_gx, d_gb, d_gLx);
   Choice c = choose(h, plan, 1, SYMAMD_COL_MAJOR, exec, false);
   int e = c.exec == SYMAMD_EXEC_LEVELSET_GRAPH ? SYMAMD_EXEC_LEVELSET : c.exec;
-  if (e == SYMAMD_EXEC_REGIONS) e = SYMAMD_EXEC_SYNCFREE;  // forward solves only
+  if (e == SYMAMD_EXEC_REGIONS) e = SYMAMD_EXEC_SYNCFREE;
   return launch_adjoint(h, plan, d_x, d_gx, d_gb, d_gLx, e, c.lanes);
 }
 

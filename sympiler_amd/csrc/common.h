@@ -152,6 +152,7 @@ struct RegionPlan {
   int *d_vdst = nullptr;
   int64_t nvals = 0;
   int *d_rflags = nullptr;  // task ready tags, MAX_GROUPS per region
+  bool transposed = false;  // schedule of L^T x = g (values from the transposed copy)
 };
 
 // Sections of a region record, in bytes from its start, each 16-byte
@@ -211,6 +212,16 @@ struct HostRegions {
 bool build_region_schedule(int n, int64_t nnz, int levels, const int *lvl_ptr,
                            const int *perm, const int *crp, const int *cci, int cap,
                            HostRegions &out);
+// Builder inputs of the transposed solve from the plan's level sets (perm,
+// lvl_ptr) and transposed copy (trp, tci: column perm[q] of L at position q,
+// diagonal first): rows and positions numbered backwards, entries in
+// ascending row order of L with the diagonal last, levels reversed; vmap
+// maps each entry to its index in the transposed copy.
+void reversed_transpose(int n, int levels, const std::vector<int> &lvl_ptr,
+                        const std::vector<int> &perm, const std::vector<int> &trp,
+                        const std::vector<int> &tci, std::vector<int> &rlvl,
+                        std::vector<int> &rperm, std::vector<int> &rrp, std::vector<int> &rci,
+                        std::vector<int> &vmap);
 // Records of a schedule for values of vs bytes: the static bytes (values
 // zero), headers (8 ints per region, as RegionPlan::d_rhead) and the value
 // gather list.  False when the records would exceed the 32-bit offsets.
@@ -283,6 +294,7 @@ struct symamd_plan_s {
   int *d_state = nullptr;              // ST_WORDS ints
   std::map<int, symamd::UnitSet> units;  // rows-per-unit -> units
   std::map<int, symamd::RegionPlan> rplans;  // region cap -> region schedule
+  std::map<int, symamd::RegionPlan> rplans_t;  // same, transposed solve
   // schedule choices
   int lanes_override = 0;
   std::map<int64_t, symamd::Choice> tuned;  // key(k, layout) -> choice
@@ -352,15 +364,29 @@ int cluster_regions(int n, const int *Lp, const int *Li, int max_rows,
 int order_by_regions(symamd_handle h, symamd_plan p, const int *d_Lp, const int *d_Li);
 int plan_owners(symamd_handle h, symamd_plan p, UnitSet *us, int nw);
 // Region schedule of the plan for region cap `cap` (built on first use;
-// rp->R == 0 when the region graph has a cycle or there is nothing to gain).
-int plan_regions(symamd_handle h, symamd_plan p, int cap, RegionPlan **out);
+// rp->R == 0 when the region graph has a cycle or there is nothing to gain),
+// of the forward solve or, transposed, of L^T x = g.
+int plan_regions(symamd_handle h, symamd_plan p, int cap, RegionPlan **out,
+                 bool transposed = false);
 void free_region_plan(RegionPlan &rp);
-// Re-gathers the region schedules' values from the plan's level-ordered copy.
+// Re-gathers the region schedules' values from the plan's level-ordered
+// copies (the transposed copy for transposed schedules).
 int gather_region_values(symamd_handle h, symamd_plan p);
 // Region-schedule forward solve (region_exec.hip); SYMAMD_ERR_NOT_SUPPORTED
 // when the plan has no usable region schedule for k right-hand sides.
 int launch_regions(symamd_handle h, symamd_plan p, void *d_B, int k, int64_t rs,
                    int64_t cs);
+// Region-schedule adjoint: gb = L^-T gx (out of place, single right-hand
+// side) and, when d_gL is given, gL = -(gb[i] * x[j]) at every stored entry
+// (i, j) of L in the caller's CSR order.  SYMAMD_ERR_NOT_SUPPORTED when the
+// transposed region schedule is not usable.
+int launch_regions_adjoint(symamd_handle h, symamd_plan p, const void *d_x,
+                           const void *d_gx, void *d_gb, void *d_gL);
+// Whether the transposed region schedule exists (built on first use) and
+// fits the LDS of a workgroup; whether the automatic choice takes it (at
+// least 60 % of the entries inside regions).
+bool region_adjoint_usable(symamd_handle h, symamd_plan p);
+bool region_adjoint_preferred(symamd_handle h, symamd_plan p);
 // Region size cap used for k right-hand sides (the LDS of one wave holds
 // the region's and its halo's values of one column group per row), unless
 // the handle sets one.

@@ -57,8 +57,8 @@ template <typename T>
 struct RegionArgs {
   int k;
   int64_t rs, cs;  // element (i, c) at X[i*rs + c*cs]
-  const T *B;      // right-hand sides (the same buffer as X: in place)
-  T *X;
+  const T *B;      // right-hand sides (read only for the task's own rows)
+  T *X;            // solution (halo rows are read from it); may equal B
   int n;           // rows: row ids read from LDS are clamped below n
   const uint8_t *rec;  // region records
   const int *head;     // 8 ints per region (RegionPlan::d_rhead)
@@ -506,9 +506,10 @@ __global__ void k_region_gather(int64_t m, const int *__restrict__ src,
 
 template <typename T>
 int gather_one(symamd_handle h, symamd_plan p, RegionPlan &rp) {
+  const T *src = (const T *)(rp.transposed ? p->d_tval : p->d_val);
   if (rp.nvals > 0)
     k_region_gather<T><<<(unsigned)((rp.nvals + 255) / 256), 256, 0, h->stream>>>(
-        rp.nvals, rp.d_vsrc, rp.d_vdst, (const T *)p->d_val, (T *)rp.d_rec);
+        rp.nvals, rp.d_vsrc, rp.d_vdst, src, (T *)rp.d_rec);
   SYMAMD_HIP(hipGetLastError());
   return SYMAMD_OK;
 }
@@ -529,8 +530,8 @@ int load_bytes(const void *d_B, int k, int64_t rs, int64_t cs) {
 // Tasks are taken by ticket, so the grid may be as large as the occupancy
 // estimate (a block that is not resident yet simply takes later tickets).
 template <typename T, int KT, int ES, int NW>
-int region_launch(symamd_handle h, symamd_plan p, const RegionPlan &rp, void *d_B, int k,
-                  int64_t rs, int64_t cs) {
+int region_launch(symamd_handle h, symamd_plan p, const RegionPlan &rp, const void *d_B,
+                  void *d_X, int k, int64_t rs, int64_t cs) {
   const void *kfn = (const void *)&k_regions<T, KT, ES, NW>;
   const size_t lds = region_lds_bytes(rp, KT, (int)sizeof(T));
   if (lds > 65536)
@@ -550,7 +551,7 @@ int region_launch(symamd_handle h, symamd_plan p, const RegionPlan &rp, void *d_
   a.rs = rs;
   a.cs = cs;
   a.B = (const T *)d_B;
-  a.X = (T *)d_B;
+  a.X = (T *)d_X;
   a.n = p->n;
   a.rec = (const uint8_t *)rp.d_rec;
   a.head = rp.d_rhead;
@@ -571,27 +572,43 @@ int region_launch(symamd_handle h, symamd_plan p, const RegionPlan &rp, void *d_
 }
 
 template <typename T, int KT, int NW>
-int region_launch_es(symamd_handle h, symamd_plan p, const RegionPlan &rp, void *d_B, int k,
-                     int64_t rs, int64_t cs) {
-  const int es = load_bytes<T, KT>(d_B, k, rs, cs);
-  if (es == 0) return region_launch<T, KT, 0, NW>(h, p, rp, d_B, k, rs, cs);
+int region_launch_es(symamd_handle h, symamd_plan p, const RegionPlan &rp, const void *d_B,
+                     void *d_X, int k, int64_t rs, int64_t cs) {
+  int es = load_bytes<T, KT>(d_B, k, rs, cs);
+  if (d_X != d_B) es = es < load_bytes<T, KT>(d_X, k, rs, cs) ? es : load_bytes<T, KT>(d_X, k, rs, cs);
+  if (es == 0) return region_launch<T, KT, 0, NW>(h, p, rp, d_B, d_X, k, rs, cs);
   if constexpr ((KT * sizeof(T)) % 16 == 0)
-    if (es == 16) return region_launch<T, KT, 16, NW>(h, p, rp, d_B, k, rs, cs);
-  return region_launch<T, KT, 4, NW>(h, p, rp, d_B, k, rs, cs);
+    if (es == 16) return region_launch<T, KT, 16, NW>(h, p, rp, d_B, d_X, k, rs, cs);
+  return region_launch<T, KT, 4, NW>(h, p, rp, d_B, d_X, k, rs, cs);
 }
 
 template <typename T>
-int region_dispatch(symamd_handle h, symamd_plan p, const RegionPlan &rp, void *d_B, int k,
-                    int64_t rs, int64_t cs, int kt, int nw) {
+int region_dispatch(symamd_handle h, symamd_plan p, const RegionPlan &rp, const void *d_B,
+                    void *d_X, int k, int64_t rs, int64_t cs, int kt, int nw) {
   switch (kt) {
-    case 1: return region_launch_es<T, 1, 1>(h, p, rp, d_B, k, rs, cs);
-    case 2: return region_launch_es<T, 2, 1>(h, p, rp, d_B, k, rs, cs);
-    case 4: return region_launch_es<T, 4, 1>(h, p, rp, d_B, k, rs, cs);
+    case 1: return region_launch_es<T, 1, 1>(h, p, rp, d_B, d_X, k, rs, cs);
+    case 2: return region_launch_es<T, 2, 1>(h, p, rp, d_B, d_X, k, rs, cs);
+    case 4: return region_launch_es<T, 4, 1>(h, p, rp, d_B, d_X, k, rs, cs);
     default:
-      if (nw == 4) return region_launch_es<T, 8, 4>(h, p, rp, d_B, k, rs, cs);
-      if (nw == 2) return region_launch_es<T, 8, 2>(h, p, rp, d_B, k, rs, cs);
-      return region_launch_es<T, 8, 1>(h, p, rp, d_B, k, rs, cs);
+      if (nw == 4) return region_launch_es<T, 8, 4>(h, p, rp, d_B, d_X, k, rs, cs);
+      if (nw == 2) return region_launch_es<T, 8, 2>(h, p, rp, d_B, d_X, k, rs, cs);
+      return region_launch_es<T, 8, 1>(h, p, rp, d_B, d_X, k, rs, cs);
   }
+}
+
+// gL[orp[i] + t] = -(gb[i] * x[j]) for entry t (column j) of row i, in the
+// row's stored order; one thread per position of the level-ordered copy
+// (the same product as the sync-free adjoint).
+template <typename T>
+__global__ void k_grad_l(int n, const int *__restrict__ perm, const int *__restrict__ rp,
+                         const int *__restrict__ ci, const int *__restrict__ orp,
+                         const T *__restrict__ x, const T *__restrict__ gb, T *gL) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const int i = perm[q];
+  const int b = rp[q], len = rp[q + 1] - b, ob = orp[i];
+  const T g = gb[i];
+  for (int t = 0; t < len; ++t) gL[ob + t] = -(g * x[ci[b + t]]);
 }
 
 // Waves per task: several only for tasks of eight columns (whose slices
@@ -623,14 +640,15 @@ bool region_preferred(symamd_handle h, symamd_plan p, int k, int layout) {
 }
 
 int gather_region_values(symamd_handle h, symamd_plan p) {
-  for (auto &kv : p->rplans) {
-    RegionPlan &rp = kv.second;
-    if (rp.R == 0) continue;
-    if (p->dtype == SYMAMD_F64)
-      SYMAMD_TRY(gather_one<double>(h, p, rp));
-    else
-      SYMAMD_TRY(gather_one<float>(h, p, rp));
-  }
+  for (auto *plans : {&p->rplans, &p->rplans_t})
+    for (auto &kv : *plans) {
+      RegionPlan &rp = kv.second;
+      if (rp.R == 0) continue;
+      if (p->dtype == SYMAMD_F64)
+        SYMAMD_TRY(gather_one<double>(h, p, rp));
+      else
+        SYMAMD_TRY(gather_one<float>(h, p, rp));
+    }
   return SYMAMD_OK;
 }
 
@@ -649,8 +667,55 @@ int launch_regions(symamd_handle h, symamd_plan p, void *d_B, int k, int64_t rs,
   RegionPlan *rp = nullptr;
   SYMAMD_TRY(plan_regions(h, p, region_cap(h, k), &rp));
   const int kt = task_cols(k), nw = region_waves(h, k);
-  if (p->dtype == SYMAMD_F64) return region_dispatch<double>(h, p, *rp, d_B, k, rs, cs, kt, nw);
-  return region_dispatch<float>(h, p, *rp, d_B, k, rs, cs, kt, nw);
+  if (p->dtype == SYMAMD_F64)
+    return region_dispatch<double>(h, p, *rp, d_B, d_B, k, rs, cs, kt, nw);
+  return region_dispatch<float>(h, p, *rp, d_B, d_B, k, rs, cs, kt, nw);
+}
+
+namespace {
+
+// The transposed schedule for one right-hand side, when its LDS image fits.
+RegionPlan *adjoint_plan(symamd_handle h, symamd_plan p) {
+  RegionPlan *rp = nullptr;
+  if (plan_regions(h, p, region_cap(h, 1), &rp, true) != SYMAMD_OK || rp->R == 0)
+    return nullptr;
+  const int vs = p->dtype == SYMAMD_F64 ? 8 : 4;
+  return region_lds_bytes(*rp, 1, vs) <= (size_t)h->lds_per_block ? rp : nullptr;
+}
+
+}  // namespace
+
+bool region_adjoint_usable(symamd_handle h, symamd_plan p) {
+  return p->n > 0 && adjoint_plan(h, p) != nullptr;
+}
+
+bool region_adjoint_preferred(symamd_handle h, symamd_plan p) {
+  RegionPlan *rp = p->n > 0 ? adjoint_plan(h, p) : nullptr;
+  return rp && rp->inside >= 0.6;
+}
+
+int launch_regions_adjoint(symamd_handle h, symamd_plan p, const void *d_x,
+                           const void *d_gx, void *d_gb, void *d_gL) {
+  if (p->n == 0) return SYMAMD_OK;
+  RegionPlan *rp = adjoint_plan(h, p);
+  if (!rp)
+    return fail(SYMAMD_ERR_NOT_SUPPORTED, "no transposed region schedule for this matrix");
+  const unsigned blocks = (unsigned)((p->n + 255) / 256);
+  if (p->dtype == SYMAMD_F64) {
+    SYMAMD_TRY(region_dispatch<double>(h, p, *rp, d_gx, d_gb, 1, 1, 0, 1, 1));
+    if (d_gL)
+      k_grad_l<double><<<blocks, 256, 0, h->stream>>>(p->n, p->d_perm, p->d_rp, p->d_ci,
+                                                      p->d_orp, (const double *)d_x,
+                                                      (const double *)d_gb, (double *)d_gL);
+  } else {
+    SYMAMD_TRY(region_dispatch<float>(h, p, *rp, d_gx, d_gb, 1, 1, 0, 1, 1));
+    if (d_gL)
+      k_grad_l<float><<<blocks, 256, 0, h->stream>>>(p->n, p->d_perm, p->d_rp, p->d_ci,
+                                                     p->d_orp, (const float *)d_x,
+                                                     (const float *)d_gb, (float *)d_gL);
+  }
+  SYMAMD_HIP(hipGetLastError());
+  return SYMAMD_OK;
 }
 
 }  // namespace symamd

@@ -687,38 +687,98 @@ bool pack_region_records(const HostRegions &o, int vs, HostRecords &out) {
   return true;
 }
 
-// Device part: the schedule for cap `cap` (built once per plan and cap).
-int plan_regions(symamd_handle h, symamd_plan p, int cap, RegionPlan **out) {
-  auto it = p->rplans.find(cap);
-  if (it != p->rplans.end()) {
+// Inputs of the region builder for the transposed solve L^T x = g: with
+// rows and positions numbered backwards (row j' = n-1-j, position
+// q' = n-1-q), L^T is lower triangular.  Row j' holds column j of L: its
+// off-diagonal entries in ascending row order of L (the order
+// sptrsv_lt.cpp:14-16 accumulates them), then the diagonal (the builder
+// takes it last).  The forward levels reversed are a valid level function
+// (an entry (i, j) of L, i > j, has lev(i) > lev(j)), so position q' keeps
+// the level set of q, in reverse.  vmap: entry of the reversed copy ->
+// entry of the plan's transposed copy (CSC of L, diagonal first).
+void reversed_transpose(int n, int levels, const std::vector<int> &lvl_ptr,
+                        const std::vector<int> &perm, const std::vector<int> &trp,
+                        const std::vector<int> &tci, std::vector<int> &rlvl,
+                        std::vector<int> &rperm, std::vector<int> &rrp, std::vector<int> &rci,
+                        std::vector<int> &vmap) {
+  rlvl.assign(levels + 1, 0);
+  for (int l = 0; l <= levels; ++l) rlvl[l] = n - lvl_ptr[levels - l];
+  rperm.assign(n, 0);
+  rrp.assign(n + 1, 0);
+  rci.assign(tci.size(), 0);
+  vmap.assign(tci.size(), 0);
+  for (int qr = 0; qr < n; ++qr) {
+    const int q = n - 1 - qr;
+    rperm[qr] = n - 1 - perm[q];
+    const int t0 = trp[q], len = trp[q + 1] - t0;
+    const int r0 = rrp[qr];
+    for (int u = 1; u < len; ++u) {
+      rci[r0 + u - 1] = n - 1 - tci[t0 + u];
+      vmap[r0 + u - 1] = t0 + u;
+    }
+    rci[r0 + len - 1] = n - 1 - tci[t0];
+    vmap[r0 + len - 1] = t0;
+    rrp[qr + 1] = r0 + len;
+  }
+}
+
+// Device part: the schedule for cap `cap` (built once per plan and cap), of
+// the forward solve or (transposed) of L^T x = g.  A transposed schedule
+// names original rows in its records and gathers its values from the
+// plan's transposed copy.
+int plan_regions(symamd_handle h, symamd_plan p, int cap, RegionPlan **out,
+                 bool transposed) {
+  auto &plans = transposed ? p->rplans_t : p->rplans;
+  auto it = plans.find(cap);
+  if (it != plans.end()) {
     *out = &it->second;
     return SYMAMD_OK;
   }
   RegionPlan rp;
   rp.cap = cap;
+  rp.transposed = transposed;
   const int n = p->n;
   HostRegions o;
+  std::vector<int> vmap;
   if (n > 0 && p->levels >= 2) {
     std::vector<int> perm(n), crp(n + 1), cci(p->nnz);
+    if (transposed) SYMAMD_TRY(plan_transpose(h, p));
     SYMAMD_HIP(hipStreamSynchronize(h->stream));
     SYMAMD_HIP(hipMemcpy(perm.data(), p->d_perm, sizeof(int) * n, hipMemcpyDeviceToHost));
-    SYMAMD_HIP(hipMemcpy(crp.data(), p->d_rp, sizeof(int) * (n + 1), hipMemcpyDeviceToHost));
-    SYMAMD_HIP(hipMemcpy(cci.data(), p->d_ci, sizeof(int) * p->nnz, hipMemcpyDeviceToHost));
-    if (!build_region_schedule(n, p->nnz, p->levels, p->h_lvl_ptr.data(), perm.data(),
-                               crp.data(), cci.data(), cap, o))
-      o.R = 0;
+    SYMAMD_HIP(hipMemcpy(crp.data(), transposed ? p->d_trp : p->d_rp, sizeof(int) * (n + 1),
+                         hipMemcpyDeviceToHost));
+    SYMAMD_HIP(hipMemcpy(cci.data(), transposed ? p->d_tci : p->d_ci, sizeof(int) * p->nnz,
+                         hipMemcpyDeviceToHost));
+    bool ok;
+    if (transposed) {
+      std::vector<int> rlvl, rperm, rrp, rci;
+      reversed_transpose(n, p->levels, p->h_lvl_ptr, perm, crp, cci, rlvl, rperm, rrp, rci,
+                         vmap);
+      ok = build_region_schedule(n, p->nnz, p->levels, rlvl.data(), rperm.data(), rrp.data(),
+                                 rci.data(), cap, o);
+      if (ok) {
+        for (int &r : o.rows) r = n - 1 - r;
+        for (int &r : o.halo) r = n - 1 - r;
+      }
+    } else {
+      ok = build_region_schedule(n, p->nnz, p->levels, p->h_lvl_ptr.data(), perm.data(),
+                                 crp.data(), cci.data(), cap, o);
+    }
+    if (!ok) o.R = 0;
   }
   rp.inside = o.inside;
   if (o.R == 0) {
-    *out = &p->rplans.emplace(cap, rp).first->second;
+    *out = &plans.emplace(cap, rp).first->second;
     return SYMAMD_OK;
   }
   const int vs = p->dtype == SYMAMD_F64 ? 8 : 4;
   HostRecords hr;
   if (o.smax + o.hmax >= REGION_SLOTS_MAX || !pack_region_records(o, vs, hr)) {
-    *out = &p->rplans.emplace(cap, rp).first->second;  // R = 0: not usable
+    *out = &plans.emplace(cap, rp).first->second;  // R = 0: not usable
     return SYMAMD_OK;
   }
+  if (transposed)
+    for (int &v : hr.vsrc) v = vmap[v];
   rp.R = o.R;
   rp.smax = o.smax;
   rp.hmax = o.hmax;
@@ -738,7 +798,7 @@ int plan_regions(symamd_handle h, symamd_plan p, int cap, RegionPlan **out) {
     free_region_plan(rp);
     return fail(SYMAMD_ERR_ALLOC, "region schedule arrays");
   }
-  RegionPlan *stored = &p->rplans.emplace(cap, rp).first->second;
+  RegionPlan *stored = &plans.emplace(cap, rp).first->second;
   SYMAMD_TRY(gather_region_values(h, p));
   *out = stored;
   return SYMAMD_OK;

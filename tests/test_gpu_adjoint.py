@@ -39,7 +39,7 @@ def _grads(plan, dLx, b, executor="auto"):
     return x.detach(), Lx_t.grad, b_t.grad
 
 
-@pytest.mark.parametrize("executor", ["syncfree", "levelset"])
+@pytest.mark.parametrize("executor", ["syncfree", "levelset", "regions", "auto"])
 def test_adjoint_16cubed_bitwise_and_dense(dev, executor):
     Lp, Li, Lx, plan, dLx = _setup(16, dev)
     plan.set_lanes(1)

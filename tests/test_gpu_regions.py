@@ -174,6 +174,55 @@ def test_region_executor_multiwave(dev, waves):
                 assert bitwise_equal(X.T if layout == "row" else X, X_ref), (k, cap, layout)
 
 
+@pytest.mark.parametrize("mat", ["lap3d", "lap2d", "random"])
+def test_region_adjoint_bitwise(dev, mat):
+    # L^T solved on the transposed region schedule (rows numbered backwards),
+    # out of place, then grad-L: bitwise the reference's ltsolve and the
+    # closed form, on two calls, and again after a value update
+    import torch
+    if mat == "lap3d":
+        Lp, Li, Lx = W.ic0_lap3d(24, 0.01)
+    elif mat == "lap2d":
+        Lp, Li, Lx = W.lap2d_tril(150, 97)
+    else:  # irregular: halos and overflow entries, forced onto regions
+        Lp, Li, Lx = W.random_lower(20000, 0.0008, 1, seed=46)
+    n = len(Lp) - 1
+    plan = sa.Plan(*to_dev(Lp, Li, Lx, device=dev), handle=_rh())
+    x, gx = W.uniform(47, 0, n), W.uniform(48, 0, n)
+    dx, dgx = to_dev(x, gx, device=dev)
+    gL = torch.empty(len(Lx), dtype=torch.float64, device=dev)
+    for scale in (1.0, 1.0, 0.5):
+        if scale != 1.0:
+            Lx = Lx * scale
+            plan.update_values(to_dev(Lx, device=dev))
+        gb_ref, gL_ref = oracle.adjoint(Lp, Li, Lx, x, gx)
+        gb, _ = plan.adjoint(dx, dgx, gLx=gL, executor="regions")
+        plan.check()
+        assert bitwise_equal(gb.cpu().numpy(), gb_ref)
+        assert bitwise_equal(gL.cpu().numpy(), gL_ref)
+        assert bitwise_equal(dgx.cpu().numpy(), gx)  # gx is not written
+
+
+def test_region_adjoint_fp32_matches_syncfree(dev):
+    # FP32: the region kernel and the sync-free kernel (one lane per row)
+    # perform the same float operations in the same order
+    import torch
+    Lp, Li, Lx = W.ic0_lap3d(20, 0.01)
+    n = len(Lp) - 1
+    dLp, dLi, dLx = to_dev(Lp, Li, Lx, device=dev)
+    plan = sa.Plan(dLp, dLi, dLx.to(torch.float32), handle=_rh())
+    plan.set_lanes(1)
+    x, gx = (to_dev(W.uniform(s, 0, n), device=dev).to(torch.float32) for s in (49, 50))
+    outs = []
+    for ex in ("regions", "syncfree"):
+        gL = torch.empty(len(Lx), dtype=torch.float32, device=dev)
+        gb, _ = plan.adjoint(x, gx, gLx=gL, executor=ex)
+        plan.check()
+        outs.append((gb.cpu().numpy(), gL.cpu().numpy()))
+    assert bitwise_equal(outs[0][0], outs[1][0])
+    assert bitwise_equal(outs[0][1], outs[1][1])
+
+
 def test_region_executor_irregular_falls_back(dev):
     # few entries inside regions and large halos: the region schedule does
     # not fit one wave's LDS and the solve runs on the sync-free executor

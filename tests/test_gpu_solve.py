@@ -177,7 +177,7 @@ def test_batched_small(dev):
 
 
 # ---------------------------------------------------------------- adjoint
-@pytest.mark.parametrize("executor", ["syncfree", "levelset"])
+@pytest.mark.parametrize("executor", ["syncfree", "levelset", "regions", "auto"])
 def test_adjoint_bitwise(dev, executor):
     Lp, Li, Lx = W.ic0_lap3d(16)
     n = len(Lp) - 1
