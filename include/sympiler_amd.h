@@ -81,8 +81,9 @@ int symamd_set_timeout_ms(symamd_handle h, double ms);
  *     (0 = as many as fit the GPU, the default).
  *   SYMAMD_OPT_COLMAJOR_DIRECT: 1 = solve column-major blocks of k > 1
  *     right-hand sides in place (one cache line per row and column); 0 =
- *     (default) transpose them into a level-ordered row-major workspace owned
- *     by the plan, solve there and transpose back. */
+ *     (default) transpose them into a row-major workspace owned by the plan
+ *     (level-ordered for the unit executors, natural order for the region
+ *     executor), solve there and transpose back. */
 /*   SYMAMD_OPT_MARKS: 1 (default) = sync-free solves of a single
  *     right-hand side mark each row's right-hand side with a signalling-NaN
  *     pattern after reading it and publish a unit's ready tag without waiting
@@ -211,9 +212,9 @@ int symamd_plan_regions(symamd_plan plan, int *count, double *inside);
 /* The executor (and lanes per row) SYMAMD_EXEC_AUTO takes for a forward
  * solve of k right-hand sides in `layout`: the tuned choice if symamd_tune
  * ran, else SYMAMD_EXEC_REGIONS when the plan has a usable region schedule
- * with at least 60 % of L's off-diagonal entries inside regions (k = 1 or a
- * row-major block), else SYMAMD_EXEC_SYNCFREE.  May build the region
- * schedule. */
+ * with at least 60 % of L's off-diagonal entries inside regions (column-major
+ * blocks only through the workspace, see SYMAMD_OPT_COLMAJOR_DIRECT), else
+ * SYMAMD_EXEC_SYNCFREE.  May build the region schedule. */
 int symamd_choice(symamd_handle h, symamd_plan plan, int k, symamd_layout layout,
                   int *exec, int *lanes);
 int symamd_plan_destroy(symamd_plan plan);

@@ -44,7 +44,7 @@ void free_plan_arrays(symamd_plan p) {
   void *ptrs[] = {p->d_lvl_ptr, p->d_perm, p->d_rp,   p->d_ci,   p->d_val,
                   p->d_fmap,    p->d_orp,  p->d_trp,  p->d_tci,  p->d_tval,
                   p->d_tmap,    p->d_state, p->d_ipos, p->d_cpos, p->d_ws,
-                  p->d_lvl_set, p->d_preg};
+                  p->d_lvl_set, p->d_preg,   p->d_oci};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   for (auto &kv : p->units) {
@@ -335,7 +335,8 @@ bool through_workspace(symamd_handle h, int k, int layout) {
 // Everything a solve allocates or builds, done before any launch so that a
 // solve can be captured into a graph.
 int prepare_solve(symamd_handle h, symamd_plan p, int k, int layout, const Choice &c) {
-  if (c.exec == SYMAMD_EXEC_REGIONS && region_usable(h, p, k)) return SYMAMD_OK;
+  if (c.exec == SYMAMD_EXEC_REGIONS && region_usable(h, p, k))
+    return through_workspace(h, k, layout) ? prepare_workspace(h, p, k) : SYMAMD_OK;
   UnitSet *us = nullptr;
   SYMAMD_TRY(plan_units(h, p, forward_rows_per_unit(k, c.lanes), false, &us));
   if (through_workspace(h, k, layout)) SYMAMD_TRY(prepare_workspace(h, p, k));
@@ -346,9 +347,12 @@ int run_solve(symamd_handle h, symamd_plan p, void *d_B, int k, int64_t ld,
               int64_t rs, int64_t cs, int layout, int exec, int lanes) {
   if (exec == SYMAMD_EXEC_LEVELSET_GRAPH) exec = SYMAMD_EXEC_LEVELSET;
   if (exec == SYMAMD_EXEC_REGIONS) {
-    // the region schedule reads every layout in place (built beforehand by
-    // prepare_solve, so nothing is built here while capturing)
-    if (region_usable(h, p, k)) return launch_regions(h, p, d_B, k, rs, cs);
+    // column-major blocks go through the row-major workspace, the rest in
+    // place (schedule and workspace built beforehand by prepare_solve, so
+    // nothing is built here while capturing)
+    if (region_usable(h, p, k))
+      return through_workspace(h, k, layout) ? launch_regions_ws(h, p, d_B, k, ld)
+                                             : launch_regions(h, p, d_B, k, rs, cs);
     exec = SYMAMD_EXEC_SYNCFREE;
   }
   if (through_workspace(h, k, layout))

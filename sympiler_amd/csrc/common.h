@@ -285,6 +285,7 @@ struct symamd_plan_s {
   void *d_ws = nullptr;                // level-ordered row-major workspace
   size_t ws_bytes = 0;
   int *d_orp = nullptr;                // caller CSR row pointers, n+1
+  int *d_oci = nullptr;                // caller CSR column ids, nnz (built on first use)
   // transposed copy (CSC of L, level order), built on first adjoint
   int *d_trp = nullptr;
   int *d_tci = nullptr;
@@ -322,6 +323,10 @@ int plan_units(symamd_handle h, symamd_plan p, int rows, bool adjoint,
                UnitSet **out);
 // Ensures the transposed (CSC) copy exists.
 int plan_transpose(symamd_handle h, symamd_plan p);
+// Ensures the inverse of the level order (row -> position, d_ipos) exists.
+int plan_ipos(symamd_handle h, symamd_plan p);
+// Ensures the caller-order column ids (d_oci) exist.
+int plan_oci(symamd_handle h, symamd_plan p);
 
 // Kernel launchers (executors.hip / adjoint.hip / spmv.hip / inspector.hip)
 int launch_forward(symamd_handle h, symamd_plan p, void *d_B, int k,
@@ -376,6 +381,9 @@ int gather_region_values(symamd_handle h, symamd_plan p);
 // when the plan has no usable region schedule for k right-hand sides.
 int launch_regions(symamd_handle h, symamd_plan p, void *d_B, int k, int64_t rs,
                    int64_t cs);
+// The same for a column-major block through the workspace (natural row
+// order); prepare_workspace must have run.
+int launch_regions_ws(symamd_handle h, symamd_plan p, void *d_B, int k, int64_t ld);
 // Region-schedule adjoint: gb = L^-T gx (out of place, single right-hand
 // side) and, when d_gL is given, gL = -(gb[i] * x[j]) at every stored entry
 // (i, j) of L in the caller's CSR order.  SYMAMD_ERR_NOT_SUPPORTED when the
@@ -395,8 +403,8 @@ int region_cap(symamd_handle h, int k);
 // first use) and its buffers fit one wave's LDS.
 bool region_usable(symamd_handle h, symamd_plan p, int k);
 // Whether the automatic choice takes the region executor: a usable region
-// schedule with at least 60 % of L's off-diagonal entries inside regions,
-// for one right-hand side or a row-major block.
+// schedule with at least 60 % of L's off-diagonal entries inside regions
+// (column-major blocks only through the workspace).
 bool region_preferred(symamd_handle h, symamd_plan p, int k, int layout);
 int ilu_lower(symamd_handle h, int n, const int *d_Ap, const int *d_Ai,
               const double *d_Ax, const int *d_diags, int unit, int *d_Lp,

@@ -679,9 +679,10 @@ SolveArgs<T> adjoint_args(symamd_handle h, symamd_plan p, const void *d_x,
 }
 
 // Column-major right-hand sides <-> level-ordered row-major workspace,
-// W[ipos[i]*k + c] = B[i + c*ld], through LDS tiles of 64 rows x 32
-// columns: the column-major side moves 64 consecutive rows of one column
-// per access, the workspace side 32 consecutive columns of one row.
+// W[ipos[i]*k + c] = B[i + c*ld] (W[i*k + c] when ipos is null), through LDS
+// tiles of 64 rows x 32 columns: the column-major side moves 64 consecutive
+// rows of one column per access, the workspace side 32 consecutive columns
+// of one row.
 template <typename T, bool TO_WS>
 __global__ __launch_bounds__(256) void k_ws_transpose(int n, int k, int64_t ld, T *B,
                                                       const int *__restrict__ ipos,
@@ -701,13 +702,13 @@ __global__ __launch_bounds__(256) void k_ws_transpose(int n, int k, int64_t ld, 
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
       const int r = wr + 8 * m, i = i0 + r;
-      if (i < n && c0 + wc < k) W[(int64_t)ipos[i] * k + c0 + wc] = tile[r][wc];
+      if (i < n && c0 + wc < k) W[(int64_t)(ipos ? ipos[i] : i) * k + c0 + wc] = tile[r][wc];
     }
   } else {
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
       const int r = wr + 8 * m, i = i0 + r;
-      if (i < n && c0 + wc < k) tile[r][wc] = W[(int64_t)ipos[i] * k + c0 + wc];
+      if (i < n && c0 + wc < k) tile[r][wc] = W[(int64_t)(ipos ? ipos[i] : i) * k + c0 + wc];
     }
     __syncthreads();
 #pragma unroll
@@ -778,6 +779,29 @@ int launch_forward_ws(symamd_handle h, symamd_plan p, void *d_B, int k, int64_t 
     SYMAMD_TRY(fwd_dispatch<float>(h, p, a, exec, lanes, kt));
     k_ws_transpose<float, false><<<grid, 256, 0, h->stream>>>(p->n, k, ld, (float *)d_B,
                                                               p->d_ipos, (float *)p->d_ws);
+  }
+  SYMAMD_HIP(hipGetLastError());
+  return SYMAMD_OK;
+}
+
+// Column-major k > 1 on the region executor: transpose into the workspace
+// in natural row order, solve it as a row-major block (the region kernel's
+// direct LDS loads), transpose back.  prepare_workspace must have run.
+int launch_regions_ws(symamd_handle h, symamd_plan p, void *d_B, int k, int64_t ld) {
+  if (p->n == 0 || k == 0) return SYMAMD_OK;
+  dim3 grid((p->n + 63) / 64, (k + 31) / 32);
+  if (p->dtype == SYMAMD_F64) {
+    k_ws_transpose<double, true><<<grid, 256, 0, h->stream>>>(p->n, k, ld, (double *)d_B,
+                                                              nullptr, (double *)p->d_ws);
+    SYMAMD_TRY(launch_regions(h, p, p->d_ws, k, k, 1));
+    k_ws_transpose<double, false><<<grid, 256, 0, h->stream>>>(p->n, k, ld, (double *)d_B,
+                                                               nullptr, (double *)p->d_ws);
+  } else {
+    k_ws_transpose<float, true><<<grid, 256, 0, h->stream>>>(p->n, k, ld, (float *)d_B,
+                                                             nullptr, (float *)p->d_ws);
+    SYMAMD_TRY(launch_regions(h, p, p->d_ws, k, k, 1));
+    k_ws_transpose<float, false><<<grid, 256, 0, h->stream>>>(p->n, k, ld, (float *)d_B,
+                                                              nullptr, (float *)p->d_ws);
   }
   SYMAMD_HIP(hipGetLastError());
   return SYMAMD_OK;

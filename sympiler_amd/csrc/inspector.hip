@@ -520,6 +520,14 @@ __global__ void k_unit_of_pos(int U, const int *__restrict__ ub, int *uop) {
   for (int q = ub[u]; q < ub[u + 1]; ++q) uop[q] = u;
 }
 
+// Caller-order column ids from the level-ordered copy: entry t of the copy
+// sits at caller position fmap[t].
+__global__ void k_scatter_cols(int64_t nnz, const int *__restrict__ fmap,
+                               const int *__restrict__ ci, int *oci) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < nnz) oci[fmap[t]] = ci[t];
+}
+
 __global__ void k_inverse_perm(int n, const int *__restrict__ perm, int *ipos) {
   int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q < n) ipos[perm[q]] = q;
@@ -680,11 +688,7 @@ int plan_units(symamd_handle h, symamd_plan p, int rows, bool adjoint,
   const int U = us.count;
   SYMAMD_HIP(hipMalloc(&us.d_uflags, sizeof(int) * (U > 0 ? U : 1)));
   SYMAMD_HIP(hipMemsetAsync(us.d_uflags, 0, sizeof(int) * (U > 0 ? U : 1), h->stream));
-  if (!p->d_ipos) {
-    SYMAMD_HIP(hipMalloc(&p->d_ipos, sizeof(int) * (p->n > 0 ? p->n : 1)));
-    k_inverse_perm<<<blocks_for(p->n), 256, 0, h->stream>>>(p->n, p->d_perm, p->d_ipos);
-    SYMAMD_HIP(hipGetLastError());
-  }
+  SYMAMD_TRY(plan_ipos(h, p));
   int *d_uop = nullptr;
   SYMAMD_HIP(hipMallocAsync(&d_uop, sizeof(int) * (p->n > 0 ? p->n : 1), h->stream));
   k_unit_of_pos<<<blocks_for(U), 256, 0, h->stream>>>(U, us.d_ub, d_uop);
@@ -699,6 +703,27 @@ int plan_units(symamd_handle h, symamd_plan p, int rows, bool adjoint,
     return rc;
   }
   *out = &p->units.emplace(key, us).first->second;
+  return SYMAMD_OK;
+}
+
+int plan_oci(symamd_handle h, symamd_plan p) {
+  if (p->d_oci) return SYMAMD_OK;
+  SYMAMD_HIP(hipMalloc(&p->d_oci, sizeof(int) * (p->nnz > 0 ? p->nnz : 1)));
+  if (p->nnz > 0) {
+    k_scatter_cols<<<blocks_for(p->nnz), 256, 0, h->stream>>>(p->nnz, p->d_fmap, p->d_ci,
+                                                              p->d_oci);
+    SYMAMD_HIP(hipGetLastError());
+  }
+  return SYMAMD_OK;
+}
+
+int plan_ipos(symamd_handle h, symamd_plan p) {
+  if (p->d_ipos) return SYMAMD_OK;
+  SYMAMD_HIP(hipMalloc(&p->d_ipos, sizeof(int) * (p->n > 0 ? p->n : 1)));
+  if (p->n > 0) {
+    k_inverse_perm<<<blocks_for(p->n), 256, 0, h->stream>>>(p->n, p->d_perm, p->d_ipos);
+    SYMAMD_HIP(hipGetLastError());
+  }
   return SYMAMD_OK;
 }
 

@@ -596,19 +596,17 @@ int region_dispatch(symamd_handle h, symamd_plan p, const RegionPlan &rp, const 
   }
 }
 
-// gL[orp[i] + t] = -(gb[i] * x[j]) for entry t (column j) of row i, in the
-// row's stored order; one thread per position of the level-ordered copy
-// (the same product as the sync-free adjoint).
+// gL[e] = -(gb[i] * x[j]) for every entry e = (i, j) of the caller's CSR
+// (the same product as the sync-free adjoint).  One thread per row in the
+// caller's order: a wave reads and writes contiguous stretches of oci and gL.
 template <typename T>
-__global__ void k_grad_l(int n, const int *__restrict__ perm, const int *__restrict__ rp,
-                         const int *__restrict__ ci, const int *__restrict__ orp,
+__global__ void k_grad_l(int n, const int *__restrict__ orp, const int *__restrict__ oci,
                          const T *__restrict__ x, const T *__restrict__ gb, T *gL) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= n) return;
-  const int i = perm[q];
-  const int b = rp[q], len = rp[q + 1] - b, ob = orp[i];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int b = orp[i], e = orp[i + 1];
   const T g = gb[i];
-  for (int t = 0; t < len; ++t) gL[ob + t] = -(g * x[ci[b + t]]);
+  for (int t = b; t < e; ++t) gL[t] = -(g * x[oci[t]]);
 }
 
 // Waves per task: several only for tasks of eight columns (whose slices
@@ -632,7 +630,10 @@ int region_cap(symamd_handle h, int k) {
 }
 
 bool region_preferred(symamd_handle h, symamd_plan p, int k, int layout) {
-  if (k > 1 && layout == SYMAMD_COL_MAJOR) return false;
+  // column-major blocks in place would be staged through registers; they
+  // are solved through the row-major workspace unless the handle asks for
+  // in-place solves
+  if (k > 1 && layout == SYMAMD_COL_MAJOR && h->colmajor_direct) return false;
   if (!region_usable(h, p, k)) return false;
   RegionPlan *rp = nullptr;
   if (plan_regions(h, p, region_cap(h, k), &rp) != SYMAMD_OK) return false;
@@ -701,17 +702,18 @@ int launch_regions_adjoint(symamd_handle h, symamd_plan p, const void *d_x,
   if (!rp)
     return fail(SYMAMD_ERR_NOT_SUPPORTED, "no transposed region schedule for this matrix");
   const unsigned blocks = (unsigned)((p->n + 255) / 256);
+  if (d_gL) SYMAMD_TRY(plan_oci(h, p));
   if (p->dtype == SYMAMD_F64) {
     SYMAMD_TRY(region_dispatch<double>(h, p, *rp, d_gx, d_gb, 1, 1, 0, 1, 1));
     if (d_gL)
-      k_grad_l<double><<<blocks, 256, 0, h->stream>>>(p->n, p->d_perm, p->d_rp, p->d_ci,
-                                                      p->d_orp, (const double *)d_x,
+      k_grad_l<double><<<blocks, 256, 0, h->stream>>>(p->n, p->d_orp, p->d_oci,
+                                                      (const double *)d_x,
                                                       (const double *)d_gb, (double *)d_gL);
   } else {
     SYMAMD_TRY(region_dispatch<float>(h, p, *rp, d_gx, d_gb, 1, 1, 0, 1, 1));
     if (d_gL)
-      k_grad_l<float><<<blocks, 256, 0, h->stream>>>(p->n, p->d_perm, p->d_rp, p->d_ci,
-                                                     p->d_orp, (const float *)d_x,
+      k_grad_l<float><<<blocks, 256, 0, h->stream>>>(p->n, p->d_orp, p->d_oci,
+                                                     (const float *)d_x,
                                                      (const float *)d_gb, (float *)d_gL);
   }
   SYMAMD_HIP(hipGetLastError());

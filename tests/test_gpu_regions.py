@@ -142,11 +142,19 @@ def test_region_executor_c2(dev):
     assert bitwise_equal(X.T, X_ref)
 
 
-@pytest.mark.parametrize("k,layout", [(1, "row"), (3, "row"), (8, "col"), (64, "row")])
+@pytest.mark.parametrize("k,layout", [(1, "row"), (3, "row"), (8, "col"), (64, "row"),
+                                      (32, "col"), (5, "col-direct")])
 def test_region_executor_grids(dev, k, layout):
+    # column-major blocks go through the row-major workspace by default;
+    # "col-direct" solves them in place (strided staging through registers)
+    h = _rh()
+    if layout == "col-direct":
+        h = sa.Handle(timeout_ms=5000.0)
+        h.set_colmajor_direct(True)
+        layout = "col"
     for Lp, Li, Lx in (W.ic0_lap3d(24, 0.01), W.lap2d_tril(150, 97)):
         n = len(Lp) - 1
-        plan = sa.Plan(*to_dev(Lp, Li, Lx, device=dev), handle=_rh())
+        plan = sa.Plan(*to_dev(Lp, Li, Lx, device=dev), handle=h)
         B = W.uniform(41 + k, 0, n * k).reshape(k, n)
         X_ref = oracle.sptrsm_csr(Lp, Li, Lx, B)
         X = _solve_regions(plan, np.ascontiguousarray(B.T) if layout == "row" else B, dev,
