@@ -304,13 +304,12 @@ __device__ __forceinline__ void solve_pairs(Pairs<T, CPL> (&m)[P], T *xs, const 
   for (int p = 0; p < P; ++p) finish_pairs<T, KT, CPL>(m[p], xv[p], xs, lep, lval, lcode, ne, kc);
 }
 
-// The internal levels of a task, slice after slice.  Each lane of the NW
-// waves solves CPL columns of one row at a time, P such groups of the next
-// slice loaded while this slice is solved, so only the dependency values sit
-// on the chain (a slice with more than 64 * NW * P groups loads the rest as
-// it goes).  One wave's LDS operations complete in issue order, so with one
-// wave the next slice's reads see this slice's writes without a wait;
-// several waves meet at a barrier after each slice.  Slice bounds: lane j holds
+// The internal levels of a task, slice after slice.  Each lane solves CPL
+// columns of one row at a time, P such groups of the next slice loaded while
+// this slice is solved, so only the dependency values sit on the chain (a
+// slice with more groups than one pass of the lanes loads the rest as it
+// goes).  One wave's LDS operations complete in issue order, so a wave's
+// next slice sees its own writes of this slice without a wait.  Slice bounds: lane j holds
 // lsl[j]; with SHORT (ns <= 62) every bound is read from that register,
 // otherwise those beyond lane 63 from LDS.  Bounds are clamped to [0, nr].
 template <typename T, int KT, int CPL, int P, bool SHORT, int NW>
@@ -334,8 +333,14 @@ __device__ __forceinline__ void solve_slices(T *xs, const uint8_t *rb, const Rec
       v = s < 64 ? __builtin_amdgcn_readlane(myb, s) : lsl[s];
     return min(max(v, 0), nr);
   };
-  const int off = (int)threadIdx.x * CPL;
-  constexpr int STEP = 64 * NW * CPL;  // slots per group of the task's waves
+  // Column split (NW * CPL == KT): wave w owns columns w*CPL .. w*CPL+CPL-1
+  // of every row and its lanes take rows; a column depends only on the same
+  // column, so a wave reads only slots it wrote itself and the waves need no
+  // barrier between slices.  Otherwise the lanes of all NW waves share each
+  // slice's (row, column-pair) items and meet at a barrier after it.
+  constexpr bool SPLIT = NW * CPL == KT;
+  const int off = SPLIT ? lane * KT + ((int)threadIdx.x >> 6) * CPL : (int)threadIdx.x * CPL;
+  constexpr int STEP = SPLIT ? 64 * KT : 64 * NW * CPL;  // slots per pass of the task
   Pairs<T, CPL> cur[P], nxt[P];
   int b0 = bound(0), b1 = ns > 0 ? bound(1) : 0;
   if (ns > 0) {
@@ -357,7 +362,7 @@ __device__ __forceinline__ void solve_slices(T *xs, const uint8_t *rb, const Rec
       load_pairs<T, KT, CPL>(more[0], t, b1 * KT, kc, xs, cnt8, code, pval, ldiag);
       solve_pairs<T, KT, CPL, 1>(more, xs, lep, lval, lcode, ne, kc);
     }
-    task_sync<NW>();
+    task_sync<SPLIT ? 1 : NW>();
 #pragma unroll
     for (int p = 0; p < P; ++p) cur[p] = nxt[p];
     b0 = b1;
@@ -445,12 +450,16 @@ __global__ __launch_bounds__(64 * NW) void k_regions(RegionArgs<T> a) {
       wave_lds_sync();
     else
       __syncthreads();
-    // 4. internal levels (each slice ends with task_sync)
+    // 4. internal levels; then every wave's results are visible to all (the
+    // stores read every column)
     if (ns <= 62)
       solve_slices<T, KT, CPL, P, true, NW>(xs, rb, L, nr, ns, ne, kc);
     else
       solve_slices<T, KT, CPL, P, false, NW>(xs, rb, L, nr, ns, ne, kc);
-    if constexpr (NW == 1) wave_lds_sync();
+    if constexpr (NW == 1)
+      wave_lds_sync();
+    else
+      __syncthreads();
     // 5. store (row ids and values of STORE_U stores read first, one wait),
     // fetch the next task's header, wait for both, publish
     for (int t0 = tid; t0 < nr * KT; t0 += NT * STORE_U) {
