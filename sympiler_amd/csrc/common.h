@@ -139,6 +139,7 @@ struct RegionPlan {
   int hmax = 0;       // max halo rows (rows of other regions read)
   int emax = 0;       // max overflow entries (beyond PACK per row) in a region
   int slmax = 0;      // max internal levels (slices) of a region
+  int width = 0;      // most regions at one depth (longest-path level) of the region graph
   double inside = 0;  // fraction of off-diagonal entries inside regions
   // One record per region (byte layout rec_layout(), also its LDS image),
   // described by rhead[8r ..] = {rows, halo rows, overflow entries, slices,
@@ -205,6 +206,7 @@ constexpr int REGION_SLOTS_MAX = 65535;
 // and dsrc are positions in the level-ordered copy of the matrix.
 struct HostRegions {
   int R = 0, smax = 0, hmax = 0, emax = 0, slmax = 0;
+  int width = 0;  // most regions at one depth of the region graph
   double inside = 0;
   std::vector<int> rptr, rows, sptr, slices, hptr, halo, dptr, dlist, eptr, ecode, esrc,
       dsrc, pcnt, pcode, psrc;

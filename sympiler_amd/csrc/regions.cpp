@@ -519,6 +519,11 @@ bool build_region_schedule(int n, int64_t nnz, int levels, const int *lvl_ptr,
   std::iota(ids.begin(), ids.end(), 0);
   int maxdepth = 0;
   for (int r = 0; r < R; ++r) maxdepth = std::max(maxdepth, depth[r]);
+  {
+    std::vector<int> at(maxdepth + 1, 0);
+    for (int r = 0; r < R; ++r) ++at[depth[r]];
+    o.width = *std::max_element(at.begin(), at.end());
+  }
   const std::vector<int> o1 = counting_sort(ids, levels, [&](int r) { return first[r]; });
   const std::vector<int> order = counting_sort(o1, maxdepth + 1, [&](int r) { return depth[r]; });
   std::vector<int> rank(R);
@@ -785,6 +790,7 @@ int plan_regions(symamd_handle h, symamd_plan p, int cap, RegionPlan **out,
   rp.emax = o.emax;
   rp.slmax = o.slmax;
   rp.recmax = hr.recmax;
+  rp.width = o.width;
   rp.nvals = (int64_t)hr.vsrc.size();
   int rc = SYMAMD_OK;
   if ((rc = upload(reinterpret_cast<uint8_t **>(&rp.d_rec), hr.bytes)) ||
