@@ -331,6 +331,25 @@ def main():
                            "gbs": round(ab / (np.mean(ea) * 1e-3) / 1e9, 1),
                            "vs_plain_levelset": round(1.0 / float(np.mean(wa)) / plain, 2)}
             log(f"{key}: {detail[key]}")
+        # the block adjoint of the headline block: GB = L^-T GX for all its
+        # columns in one launch, then grad-L summed over the columns
+        kb = X.shape[1] if args.layout == "row" else X.shape[0]
+        GXb = X.clone()
+        GBb = torch.empty_like(GXb)
+        wb, eb = timed_steps(lambda: plan.adjoint_block(X, GXb, GBb, gL, layout=args.layout),
+                             lambda: None, 10, 2, stream, torch)
+        plan.check()
+        bb = W.algorithmic_bytes_adjoint_block(n, nnz, kb)
+        detail["adjoint_block"] = {"k": kb, "layout": args.layout,
+                                   "ms": round(1e3 * float(np.mean(wb)), 4),
+                                   "kernel_ms": round(float(np.mean(eb)), 4),
+                                   "rhs_solves_per_s": round(kb / float(np.mean(wb)), 1),
+                                   "gbs": round(bb / (np.mean(eb) * 1e-3) / 1e9, 1),
+                                   "frac_hbm_peak": round(bb / (np.mean(eb) * 1e-3) / 1e9
+                                                          / HBM_PEAK_GBS, 4),
+                                   "vs_plain_levelset": round(kb / float(np.mean(wb)) / plain, 2)}
+        log(f"adjoint_block: {detail['adjoint_block']}")
+        del GXb, GBb
 
         # the headline block on the region executor
         wr, er = timed_steps(lambda: plan.solve(X, layout=args.layout, executor="regions"),

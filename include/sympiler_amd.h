@@ -248,6 +248,20 @@ int symamd_graph_destroy(symamd_graph g);
 int symamd_adjoint(symamd_handle h, symamd_plan plan, const void *d_x,
                    const void *d_gx, void *d_gb, void *d_gLx,
                    symamd_executor exec);
+/* The same for a block of k right-hand sides, X = L^{-1} B, with the
+ * upstream gradient GX = dloss/dX (no counterpart in the reference, which
+ * has no adjoint; the batched form of symamd_adjoint):
+ *   GB = L^{-T} GX                      column by column
+ *   gLx[p] = -(GB[i,0]*X[j,0] + ... + GB[i,k-1]*X[j,k-1])
+ *                                       products added in ascending column,
+ *                                       for every stored entry p = (i, j).
+ * X, GX and GB share k, ld and layout; d_GB may alias d_GX; d_gLx may be
+ * NULL.  The region executor (SYMAMD_EXEC_AUTO when the matrix has a
+ * usable region schedule, or SYMAMD_EXEC_REGIONS) solves the whole block in
+ * one launch; otherwise the columns are solved one after another. */
+int symamd_adjoint_block(symamd_handle h, symamd_plan plan, const void *d_X,
+                         const void *d_GX, void *d_GB, void *d_gLx, int k, int64_t ld,
+                         symamd_layout layout, symamd_executor exec);
 
 /* ------------------------------------------------------------------ SpMV */
 /* y = A x (sparse_blas/spmv.cpp:12-21). FP64. */

@@ -68,6 +68,7 @@ def olib():
             "orc_levelset_csc": ([i, vp, vp, vp, vp], c_int),
             "orc_check_schedule": ([i, vp, vp, i, vp, vp, vp], c_int),
             "orc_adjoint": ([i, vp, vp, vp, vp, vp, vp, vp, vp, vp], None),
+            "orc_adjoint_block": ([i, i, vp, vp, vp, vp, vp, vp, vp, vp, vp], None),
             "orc_csr_to_csc": ([i, vp, vp, vp, vp, vp, vp, vp], None),
         }
         for name, (args, res) in sigs.items():
@@ -257,6 +258,21 @@ def adjoint(Lp, Li, Lx, x, gx):
     olib().orc_adjoint(n, _p(_i32(Lp)), _p(_i32(Li)), _p(Cp), _p(Ci), _p(Cx),
                        _p(_f64(x)), _p(_f64(gx)), _p(gb), _p(gL))
     return gb, gL
+
+
+def adjoint_block(Lp, Li, Lx, X, GX):
+    """(GB, gL) for row-major (n, k) blocks: GB = L^-T GX per column,
+    gL[p] = -(sum over c ascending of GB[i,c] X[j,c])."""
+    n = len(Lp) - 1
+    X = np.ascontiguousarray(X, np.float64)
+    GX = np.ascontiguousarray(GX, np.float64)
+    k = X.shape[1]
+    Cp, Ci, Cx, _ = csr_to_csc(Lp, Li, Lx)
+    GB = np.empty((n, k), np.float64)
+    gL = np.empty(len(Lx), np.float64)
+    olib().orc_adjoint_block(n, k, _p(_i32(Lp)), _p(_i32(Li)), _p(Cp), _p(Ci), _p(Cx),
+                             _p(X), _p(GX), _p(GB), _p(gL))
+    return GB, gL
 
 
 def prune_set(Lp, Li):

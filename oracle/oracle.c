@@ -367,6 +367,33 @@ void orc_adjoint(int n, const int *Lp, const int *Li, const int *Cp,
     for (int p = Lp[i]; p < Lp[i + 1]; p++) gL[p] = -(gb[i] * x[Li[p]]);
 }
 
+/* Adjoint of X = L^{-1} B for a row-major block of k right-hand sides
+ * (element (i, c) at [i*k + c]): GB = L^{-T} GX column by column (ltsolve,
+ * sptrsv_lt.cpp:11-20, on the CSC of L) and, for every stored entry
+ * p = (i, j) of the CSR of L,
+ *   gL[p] = -(GB[i,0]*X[j,0] + GB[i,1]*X[j,1] + ... + GB[i,k-1]*X[j,k-1])
+ * with the products added in ascending c, left to right (the gradient of a
+ * loss over the whole block).  The reference has no adjoint; for k = 1
+ * this is orc_adjoint. */
+void orc_adjoint_block(int n, int k, const int *Lp, const int *Li, const int *Cp,
+                       const int *Ci, const double *Cx, const double *X,
+                       const double *GX, double *GB, double *gL) {
+  double *col = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+  for (int c = 0; c < k; c++) {
+    for (int i = 0; i < n; i++) col[i] = GX[(size_t)i * k + c];
+    orc_ltsolve(n, Cp, Ci, Cx, col);
+    for (int i = 0; i < n; i++) GB[(size_t)i * k + c] = col[i];
+  }
+  free(col);
+  for (int i = 0; i < n; i++)
+    for (int p = Lp[i]; p < Lp[i + 1]; p++) {
+      const double *g = GB + (size_t)i * k, *x = X + (size_t)Li[p] * k;
+      double acc = g[0] * x[0];
+      for (int c = 1; c < k; c++) acc += g[c] * x[c];
+      gL[p] = -acc;
+    }
+}
+
 /* CSR -> CSC transpose (stable: entries of a column in ascending row order,
  * which puts the diagonal of a lower triangle first).  map[p] gives the CSC
  * position of CSR entry p (may be NULL). */

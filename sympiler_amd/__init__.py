@@ -88,6 +88,7 @@ def lib() -> ctypes.CDLL:
             "symamd_graph_launch": [vp, vp],
             "symamd_graph_destroy": [vp],
             "symamd_adjoint": [vp, vp, vp, vp, vp, vp, i32],
+            "symamd_adjoint_block": [vp, vp, vp, vp, vp, vp, i32, i64, i32, i32],
             "symamd_spmv_csr": [vp, i32, vp, vp, vp, vp, vp],
             "symamd_spmv_csr_axpby": [vp, i32, vp, vp, vp, vp, vp, vp, dbl, dbl],
             "symamd_residual": [vp, vp, vp, vp, i32, i64, i32, vp],
@@ -335,6 +336,22 @@ class Plan:
         _check(lib().symamd_adjoint(self.h._h, self._p, _ptr(x), _ptr(gx),
                                     _ptr(gb), _ptr(gLx), EXEC[executor]))
         return gb, gLx
+
+    def adjoint_block(self, X, GX, GB=None, gLx=None, layout="row", executor="auto"):
+        """Block adjoint of X = L^-1 B: GB = L^-T GX column by column and
+        gLx[p] = -(sum over columns c, ascending, of GB[i,c] X[j,c]).  X, GX
+        and GB are 2-D tensors in `layout` (as for solve)."""
+        import torch
+        _require_cuda(X, GX, GB, gLx)
+        if GB is None:
+            GB = torch.empty_like(GX)
+        k, ld, lay = self._layout(GX, layout)
+        for t in (X, GB):
+            if self._layout(t, layout) != (k, ld, lay):
+                raise ValueError("X, GX and GB must share shape and layout")
+        _check(lib().symamd_adjoint_block(self.h._h, self._p, _ptr(X), _ptr(GX), _ptr(GB),
+                                          _ptr(gLx), int(k), int(ld), lay, EXEC[executor]))
+        return GB, gLx
 
     def residual(self, X, B, layout="col"):
         """Relative residual ||L X - B|| / ||B|| per column (list of floats)."""

@@ -25,7 +25,9 @@ class TriangularSolve(torch.autograd.Function):
     """x = TriangularSolve.apply(Lx, b, plan, executor)
 
     Lx: values of L on the plan's pattern (caller CSR order), 1-D tensor
-    b:  right-hand side, 1-D tensor of length n
+    b:  right-hand side: a 1-D tensor of length n, or an (n, k) block whose
+        columns are k right-hand sides (row-major); the backward pass of a
+        block is the block adjoint, grad-L summed over the columns
     """
 
     @staticmethod
@@ -35,7 +37,10 @@ class TriangularSolve(torch.autograd.Function):
         Lx = Lx.detach().contiguous()
         plan.update_values(Lx)
         x = b.detach().contiguous().clone()
-        plan.solve(x, executor=executor)
+        if x.dim() == 1:
+            plan.solve(x, executor=executor)
+        else:
+            plan.solve(x, layout="row", executor=executor)
         ctx.plan = plan
         ctx.executor = executor
         ctx.save_for_backward(x)
@@ -49,7 +54,10 @@ class TriangularSolve(torch.autograd.Function):
         gb = torch.empty_like(gx)
         gL = torch.empty(plan.nnz, dtype=gx.dtype, device=gx.device) \
             if ctx.needs_input_grad[0] else None
-        plan.adjoint(x, gx, gb, gL, executor=ctx.executor)
+        if x.dim() == 1:
+            plan.adjoint(x, gx, gb, gL, executor=ctx.executor)
+        else:
+            plan.adjoint_block(x, gx, gb, gL, layout="row", executor=ctx.executor)
         return gL, (gb if ctx.needs_input_grad[1] else None), None, None
 
 

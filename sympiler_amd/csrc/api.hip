@@ -187,6 +187,26 @@ __global__ void k_fill(int64_t m, double v, void *dst, int is64) {
 int solve_impl(symamd_handle h, symamd_plan p, void *d_B, int k, int64_t ld,
                int layout, int exec, bool graph_ok);
 
+template <typename W>
+__global__ void k_copy_strided(int n, const W *src, int64_t ss, W *dst, int64_t ds) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[(int64_t)i * ds] = src[(int64_t)i * ss];
+}
+
+// dst[i*ds] = src[i*ss] for n elements of vs (4 or 8) bytes.
+int copy_strided(symamd_handle h, int n, size_t vs, const void *src, int64_t ss, void *dst,
+                 int64_t ds) {
+  const unsigned blocks = (unsigned)((n + 255) / 256);
+  if (vs == 8)
+    k_copy_strided<uint64_t><<<blocks, 256, 0, h->stream>>>(n, (const uint64_t *)src, ss,
+                                                            (uint64_t *)dst, ds);
+  else
+    k_copy_strided<uint32_t><<<blocks, 256, 0, h->stream>>>(n, (const uint32_t *)src, ss,
+                                                            (uint32_t *)dst, ds);
+  SYMAMD_HIP(hipGetLastError());
+  return SYMAMD_OK;
+}
+
 }  // namespace
 
 void set_error(const char *fmt, ...) {
@@ -351,7 +371,7 @@ int run_solve(symamd_handle h, symamd_plan p, void *d_B, int k, int64_t ld,
     // place (schedule and workspace built beforehand by prepare_solve, so
     // nothing is built here while capturing)
     if (region_usable(h, p, k))
-      return through_workspace(h, k, layout) ? launch_regions_ws(h, p, d_B, k, ld)
+      return through_workspace(h, k, layout) ? launch_regions_ws(h, p, d_B, d_B, k, ld, false)
                                              : launch_regions(h, p, d_B, k, rs, cs);
     exec = SYMAMD_EXEC_SYNCFREE;
   }
@@ -707,13 +727,61 @@ int symamd_adjoint(symamd_handle h, symamd_plan plan, const void *d_x,
     return fail(SYMAMD_ERR_INVALID_ARG, "unknown executor %d", (int)exec);
   // the region executor solves L^T on the transposed region schedule; a
   // forced choice without a usable schedule runs sync-free, like forward
-  if (exec == SYMAMD_EXEC_REGIONS ? region_adjoint_usable(h, plan)
-                                  : exec == SYMAMD_EXEC_AUTO && region_adjoint_preferred(h, plan))
-    return launch_regions_adjoint(h, plan, d_x, d_gx, d_gb, d_gLx);
+  if (exec == SYMAMD_EXEC_REGIONS ? region_adjoint_usable(h, plan, 1)
+                                  : exec == SYMAMD_EXEC_AUTO && region_adjoint_preferred(h, plan, 1)) {
+    SYMAMD_TRY(launch_regions_plan(h, plan, d_gx, d_gb, 1, 1, 0, true));
+    return d_gLx ? launch_grad_l(h, plan, d_x, d_gb, d_gLx, 1, 1, 0) : SYMAMD_OK;
+  }
   Choice c = choose(h, plan, 1, SYMAMD_COL_MAJOR, exec, false);
   int e = c.exec == SYMAMD_EXEC_LEVELSET_GRAPH ? SYMAMD_EXEC_LEVELSET : c.exec;
   if (e == SYMAMD_EXEC_REGIONS) e = SYMAMD_EXEC_SYNCFREE;
   return launch_adjoint(h, plan, d_x, d_gx, d_gb, d_gLx, e, c.lanes);
+}
+
+int symamd_adjoint_block(symamd_handle h, symamd_plan plan, const void *d_X,
+                         const void *d_GX, void *d_GB, void *d_gLx, int k, int64_t ld,
+                         symamd_layout layout, symamd_executor exec) {
+  SYMAMD_TRY(check_handle(h));
+  SYMAMD_TRY(check_plan(plan));
+  if (k < 0 || (k > 0 && plan->n > 0 && (!d_X || !d_GX || !d_GB)))
+    return fail(SYMAMD_ERR_INVALID_ARG, "symamd_adjoint_block: bad arguments");
+  if (exec < SYMAMD_EXEC_AUTO || exec > SYMAMD_EXEC_REGIONS)
+    return fail(SYMAMD_ERR_INVALID_ARG, "unknown executor %d", (int)exec);
+  int64_t rs = 0, cs = 0;
+  SYMAMD_TRY(strides(plan->n, k, ld, layout, &rs, &cs));
+  if (k == 0 || plan->n == 0) return SYMAMD_OK;
+  const bool regions = exec == SYMAMD_EXEC_REGIONS ? region_adjoint_usable(h, plan, k)
+                       : exec == SYMAMD_EXEC_AUTO && region_adjoint_preferred(h, plan, k);
+  if (regions) {
+    if (k > 1 && layout == SYMAMD_COL_MAJOR && !h->colmajor_direct) {
+      SYMAMD_TRY(prepare_workspace(h, plan, k));
+      SYMAMD_TRY(launch_regions_ws(h, plan, d_GX, d_GB, k, ld, true));
+    } else {
+      SYMAMD_TRY(launch_regions_plan(h, plan, d_GX, d_GB, k, rs, cs, true));
+    }
+  } else {
+    // one column at a time on the transposed unit executors, through
+    // contiguous vectors (column c of the block at offset c*cs, stride rs)
+    Choice c = choose(h, plan, 1, SYMAMD_COL_MAJOR, exec == SYMAMD_EXEC_REGIONS
+                                                        ? SYMAMD_EXEC_SYNCFREE : exec, false);
+    int e = c.exec == SYMAMD_EXEC_LEVELSET_GRAPH ? SYMAMD_EXEC_LEVELSET : c.exec;
+    if (e == SYMAMD_EXEC_REGIONS) e = SYMAMD_EXEC_SYNCFREE;
+    const size_t vs = plan->dtype == SYMAMD_F64 ? 8 : 4;
+    void *tin = nullptr, *tout = nullptr;
+    SYMAMD_HIP(hipMallocAsync(&tin, vs * plan->n, h->stream));
+    SYMAMD_HIP(hipMallocAsync(&tout, vs * plan->n, h->stream));
+    int rc = SYMAMD_OK;
+    for (int col = 0; col < k && rc == SYMAMD_OK; ++col) {
+      const int64_t off = (int64_t)col * cs;
+      rc = copy_strided(h, plan->n, vs, (const char *)d_GX + vs * off, rs, tin, 1);
+      if (rc == SYMAMD_OK) rc = launch_adjoint(h, plan, nullptr, tin, tout, nullptr, e, c.lanes);
+      if (rc == SYMAMD_OK) rc = copy_strided(h, plan->n, vs, tout, 1, (char *)d_GB + vs * off, rs);
+    }
+    (void)hipFreeAsync(tin, h->stream);
+    (void)hipFreeAsync(tout, h->stream);
+    SYMAMD_TRY(rc);
+  }
+  return d_gLx ? launch_grad_l(h, plan, d_X, d_GB, d_gLx, k, rs, cs) : SYMAMD_OK;
 }
 
 int symamd_spmv_csr(symamd_handle h, int n, const int *d_Ap, const int *d_Ai,

@@ -383,20 +383,25 @@ int gather_region_values(symamd_handle h, symamd_plan p);
 // when the plan has no usable region schedule for k right-hand sides.
 int launch_regions(symamd_handle h, symamd_plan p, void *d_B, int k, int64_t rs,
                    int64_t cs);
-// The same for a column-major block through the workspace (natural row
-// order); prepare_workspace must have run.
-int launch_regions_ws(symamd_handle h, symamd_plan p, void *d_B, int k, int64_t ld);
-// Region-schedule adjoint: gb = L^-T gx (out of place, single right-hand
-// side) and, when d_gL is given, gL = -(gb[i] * x[j]) at every stored entry
-// (i, j) of L in the caller's CSR order.  SYMAMD_ERR_NOT_SUPPORTED when the
-// transposed region schedule is not usable.
-int launch_regions_adjoint(symamd_handle h, symamd_plan p, const void *d_x,
-                           const void *d_gx, void *d_gb, void *d_gL);
-// Whether the transposed region schedule exists (built on first use) and
-// fits the LDS of a workgroup; whether the automatic choice takes it (at
-// least 60 % of the entries inside regions).
-bool region_adjoint_usable(symamd_handle h, symamd_plan p);
-bool region_adjoint_preferred(symamd_handle h, symamd_plan p);
+// The forward (or transposed) region solve of a column-major block from
+// d_in into d_out (which may be the same buffer) through the workspace in
+// natural row order; prepare_workspace must have run.
+int launch_regions_ws(symamd_handle h, symamd_plan p, const void *d_in, void *d_out, int k,
+                      int64_t ld, bool transposed);
+// The forward (or, transposed, L^T X = B) region solve from d_B into d_X
+// (which may be the same buffer); SYMAMD_ERR_NOT_SUPPORTED when the
+// schedule is not usable.
+int launch_regions_plan(symamd_handle h, symamd_plan p, const void *d_B, void *d_X, int k,
+                        int64_t rs, int64_t cs, bool transposed);
+// Whether the transposed region schedule for k right-hand sides exists
+// (built on first use) and fits the LDS of a workgroup; whether the
+// automatic choice takes it (at least 60 % of the entries inside regions).
+bool region_adjoint_usable(symamd_handle h, symamd_plan p, int k);
+bool region_adjoint_preferred(symamd_handle h, symamd_plan p, int k);
+// gL[e] = -(sum over c ascending of GB[i,c] * X[j,c]) for every entry
+// e = (i, j) of L in the caller's CSR order (element (r, c) at r*rs + c*cs).
+int launch_grad_l(symamd_handle h, symamd_plan p, const void *d_X, const void *d_GB,
+                  void *d_gL, int k, int64_t rs, int64_t cs);
 // Region size cap used for k right-hand sides (the LDS of one wave holds
 // the region's and its halo's values of one column group per row), unless
 // the handle sets one.

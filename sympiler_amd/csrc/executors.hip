@@ -784,23 +784,25 @@ int launch_forward_ws(symamd_handle h, symamd_plan p, void *d_B, int k, int64_t 
   return SYMAMD_OK;
 }
 
-// Column-major k > 1 on the region executor: transpose into the workspace
-// in natural row order, solve it as a row-major block (the region kernel's
-// direct LDS loads), transpose back.  prepare_workspace must have run.
-int launch_regions_ws(symamd_handle h, symamd_plan p, void *d_B, int k, int64_t ld) {
+// Column-major k > 1 on the region executor: transpose d_in into the
+// workspace in natural row order, solve it there as a row-major block (the
+// region kernel's direct LDS loads), transpose back into d_out.
+// prepare_workspace must have run.
+int launch_regions_ws(symamd_handle h, symamd_plan p, const void *d_in, void *d_out, int k,
+                      int64_t ld, bool transposed) {
   if (p->n == 0 || k == 0) return SYMAMD_OK;
   dim3 grid((p->n + 63) / 64, (k + 31) / 32);
   if (p->dtype == SYMAMD_F64) {
-    k_ws_transpose<double, true><<<grid, 256, 0, h->stream>>>(p->n, k, ld, (double *)d_B,
-                                                              nullptr, (double *)p->d_ws);
-    SYMAMD_TRY(launch_regions(h, p, p->d_ws, k, k, 1));
-    k_ws_transpose<double, false><<<grid, 256, 0, h->stream>>>(p->n, k, ld, (double *)d_B,
+    k_ws_transpose<double, true><<<grid, 256, 0, h->stream>>>(
+        p->n, k, ld, (double *)const_cast<void *>(d_in), nullptr, (double *)p->d_ws);
+    SYMAMD_TRY(launch_regions_plan(h, p, p->d_ws, p->d_ws, k, k, 1, transposed));
+    k_ws_transpose<double, false><<<grid, 256, 0, h->stream>>>(p->n, k, ld, (double *)d_out,
                                                                nullptr, (double *)p->d_ws);
   } else {
-    k_ws_transpose<float, true><<<grid, 256, 0, h->stream>>>(p->n, k, ld, (float *)d_B,
-                                                             nullptr, (float *)p->d_ws);
-    SYMAMD_TRY(launch_regions(h, p, p->d_ws, k, k, 1));
-    k_ws_transpose<float, false><<<grid, 256, 0, h->stream>>>(p->n, k, ld, (float *)d_B,
+    k_ws_transpose<float, true><<<grid, 256, 0, h->stream>>>(
+        p->n, k, ld, (float *)const_cast<void *>(d_in), nullptr, (float *)p->d_ws);
+    SYMAMD_TRY(launch_regions_plan(h, p, p->d_ws, p->d_ws, k, k, 1, transposed));
+    k_ws_transpose<float, false><<<grid, 256, 0, h->stream>>>(p->n, k, ld, (float *)d_out,
                                                               nullptr, (float *)p->d_ws);
   }
   SYMAMD_HIP(hipGetLastError());

@@ -611,19 +611,6 @@ int region_dispatch(symamd_handle h, symamd_plan p, const RegionPlan &rp, const 
   }
 }
 
-// gL[e] = -(gb[i] * x[j]) for every entry e = (i, j) of the caller's CSR
-// (the same product as the sync-free adjoint).  One thread per row in the
-// caller's order: a wave reads and writes contiguous stretches of oci and gL.
-template <typename T>
-__global__ void k_grad_l(int n, const int *__restrict__ orp, const int *__restrict__ oci,
-                         const T *__restrict__ x, const T *__restrict__ gb, T *gL) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int b = orp[i], e = orp[i + 1];
-  const T g = gb[i];
-  for (int t = b; t < e; ++t) gL[t] = -(g * x[oci[t]]);
-}
-
 // Waves per task: several only for tasks of eight columns (whose slices
 // need several passes of one wave); four by default (measured on the 128^3
 // IC(0) factor at k = 32: 0.84 ms with four waves at cap 512, 0.87 with two
@@ -676,63 +663,156 @@ bool region_usable(symamd_handle h, symamd_plan p, int k) {
   return region_lds_bytes(*rp, task_cols(k), vs) <= (size_t)h->lds_per_block;
 }
 
-int launch_regions(symamd_handle h, symamd_plan p, void *d_B, int k, int64_t rs,
-                   int64_t cs) {
-  if (!region_usable(h, p, k))
-    return fail(SYMAMD_ERR_NOT_SUPPORTED, "no region schedule for this matrix and k");
-  RegionPlan *rp = nullptr;
-  SYMAMD_TRY(plan_regions(h, p, region_cap(h, k), &rp));
-  const int kt = task_cols(k), nw = region_waves(h, k);
-  if (p->dtype == SYMAMD_F64)
-    return region_dispatch<double>(h, p, *rp, d_B, d_B, k, rs, cs, kt, nw);
-  return region_dispatch<float>(h, p, *rp, d_B, d_B, k, rs, cs, kt, nw);
-}
-
 namespace {
 
-// The transposed schedule for one right-hand side, when its LDS image fits.
-RegionPlan *adjoint_plan(symamd_handle h, symamd_plan p) {
+// The schedule of the forward (or transposed) solve for k right-hand sides,
+// when its LDS image fits a workgroup.
+RegionPlan *usable_plan(symamd_handle h, symamd_plan p, int k, bool transposed) {
+  if (p->n == 0 || k < 1 || k > KG_MAX * RegionPlan::MAX_GROUPS) return nullptr;
   RegionPlan *rp = nullptr;
-  if (plan_regions(h, p, region_cap(h, 1), &rp, true) != SYMAMD_OK || rp->R == 0)
+  if (plan_regions(h, p, region_cap(h, k), &rp, transposed) != SYMAMD_OK || rp->R == 0)
     return nullptr;
   const int vs = p->dtype == SYMAMD_F64 ? 8 : 4;
-  return region_lds_bytes(*rp, 1, vs) <= (size_t)h->lds_per_block ? rp : nullptr;
+  return region_lds_bytes(*rp, task_cols(k), vs) <= (size_t)h->lds_per_block ? rp : nullptr;
+}
+
+// gL[e] = -(GB[i,0]*X[j,0] + ... + GB[i,k-1]*X[j,k-1]) for every entry
+// e = (i, j) of the caller's CSR, products added in ascending column (for
+// k = 1, the sync-free adjoint's -(gb[i] * x[j])); element (r, c) of X and
+// GB at r*rs + c*cs.  Narrow blocks: one thread per row in the caller's
+// order, GC columns at a time with all their loads in flight.
+template <typename T, int GC>
+__global__ void k_grad_l(int n, int k, int64_t rs, int64_t cs, const int *__restrict__ orp,
+                         const int *__restrict__ oci, const T *__restrict__ X,
+                         const T *__restrict__ GB, T *gL) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const T *g = GB + (int64_t)i * rs;
+  for (int e = orp[i]; e < orp[i + 1]; ++e) {
+    const T *x = X + (int64_t)oci[e] * rs;
+    T acc = T(0);
+    for (int c0 = 0; c0 < k; c0 += GC) {
+      T gv[GC], xv[GC];
+#pragma unroll
+      for (int u = 0; u < GC; ++u) {
+        const int64_t o = (int64_t)(c0 + u < k ? c0 + u : c0) * cs;
+        gv[u] = g[o];
+        xv[u] = x[o];
+      }
+#pragma unroll
+      for (int u = 0; u < GC; ++u)
+        if (c0 + u < k) acc = c0 + u == 0 ? gv[u] * xv[u] : acc + gv[u] * xv[u];
+    }
+    gL[e] = -acc;
+  }
+}
+
+// Wide blocks (8 <= k <= 256): LPR lanes take the columns of one row
+// (coalesced loads of GB's row and of each entry's X row), 64/LPR rows per
+// wave; the products of up to EB entries of each row go to LDS, then one
+// lane per entry adds them in ascending column -- the same sums as above.
+constexpr int GRAD_EB = 8;
+template <typename T, int LPR>
+__global__ __launch_bounds__(256) void k_grad_l_wide(int n, int k, int64_t rs, int64_t cs,
+                                                     const int *__restrict__ orp,
+                                                     const int *__restrict__ oci,
+                                                     const T *__restrict__ X,
+                                                     const T *__restrict__ GB, T *gL) {
+  constexpr int RPW = 64 / LPR, EB = GRAD_EB;
+  extern __shared__ __align__(16) uint8_t gsm[];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  T *pw = reinterpret_cast<T *>(gsm) + (size_t)wid * RPW * EB * k;
+  const int r = lane / LPR, cl = lane - r * LPR;
+  const int64_t i64 = ((int64_t)blockIdx.x * 4 + wid) * RPW + r;
+  const bool live = i64 < n;
+  const int i = live ? (int)i64 : 0;
+  const int e0 = live ? orp[i] : 0, len = live ? orp[i + 1] - e0 : 0;
+  int mlen = len;  // most entries among the wave's rows
+#pragma unroll
+  for (int m = LPR; m < 64; m <<= 1) mlen = max(mlen, __shfl_xor(mlen, m));
+  const T *g = GB + (int64_t)i * rs;
+  for (int b = 0; b < mlen; b += EB) {
+#pragma unroll
+    for (int eb = 0; eb < EB; ++eb) {
+      const bool have = b + eb < len;
+      const T *x = X + (int64_t)(have ? oci[e0 + b + eb] : 0) * rs;
+      for (int c = cl; c < k; c += LPR)
+        pw[(r * EB + eb) * k + c] = have ? g[(int64_t)c * cs] * x[(int64_t)c * cs] : T(0);
+    }
+    __builtin_amdgcn_wave_barrier();
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    if (cl < EB && b + cl < len) {
+      const T *q = pw + (r * EB + cl) * k;
+      T acc = q[0];
+      for (int c = 1; c < k; ++c) acc += q[c];
+      gL[e0 + b + cl] = -acc;
+    }
+    __builtin_amdgcn_wave_barrier();
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+  }
+}
+
+template <typename T>
+int grad_dispatch(symamd_handle h, symamd_plan p, const T *X, const T *GB, T *gL, int k,
+                  int64_t rs, int64_t cs) {
+  const int n = p->n;
+  if (k >= 8 && k <= 256) {
+    const int lpr = k >= 64 ? 64 : pow2_ceil(k);
+    const int rows_per_block = 4 * (64 / lpr);
+    const unsigned blocks = (unsigned)((n + rows_per_block - 1) / rows_per_block);
+    const size_t lds = (size_t)4 * (64 / lpr) * GRAD_EB * k * sizeof(T);
+    switch (lpr) {
+      case 8: k_grad_l_wide<T, 8><<<blocks, 256, lds, h->stream>>>(n, k, rs, cs, p->d_orp, p->d_oci, X, GB, gL); break;
+      case 16: k_grad_l_wide<T, 16><<<blocks, 256, lds, h->stream>>>(n, k, rs, cs, p->d_orp, p->d_oci, X, GB, gL); break;
+      case 32: k_grad_l_wide<T, 32><<<blocks, 256, lds, h->stream>>>(n, k, rs, cs, p->d_orp, p->d_oci, X, GB, gL); break;
+      default: k_grad_l_wide<T, 64><<<blocks, 256, lds, h->stream>>>(n, k, rs, cs, p->d_orp, p->d_oci, X, GB, gL); break;
+    }
+  } else {
+    const unsigned blocks = (unsigned)((n + 255) / 256);
+    if (k == 1)
+      k_grad_l<T, 1><<<blocks, 256, 0, h->stream>>>(n, k, rs, cs, p->d_orp, p->d_oci, X, GB, gL);
+    else
+      k_grad_l<T, 8><<<blocks, 256, 0, h->stream>>>(n, k, rs, cs, p->d_orp, p->d_oci, X, GB, gL);
+  }
+  SYMAMD_HIP(hipGetLastError());
+  return SYMAMD_OK;
 }
 
 }  // namespace
 
-bool region_adjoint_usable(symamd_handle h, symamd_plan p) {
-  return p->n > 0 && adjoint_plan(h, p) != nullptr;
+int launch_regions_plan(symamd_handle h, symamd_plan p, const void *d_B, void *d_X, int k,
+                        int64_t rs, int64_t cs, bool transposed) {
+  RegionPlan *rp = usable_plan(h, p, k, transposed);
+  if (!rp) return fail(SYMAMD_ERR_NOT_SUPPORTED, "no region schedule for this matrix and k");
+  const int kt = task_cols(k), nw = region_waves(h, k);
+  if (p->dtype == SYMAMD_F64)
+    return region_dispatch<double>(h, p, *rp, d_B, d_X, k, rs, cs, kt, nw);
+  return region_dispatch<float>(h, p, *rp, d_B, d_X, k, rs, cs, kt, nw);
 }
 
-bool region_adjoint_preferred(symamd_handle h, symamd_plan p) {
-  RegionPlan *rp = p->n > 0 ? adjoint_plan(h, p) : nullptr;
+int launch_regions(symamd_handle h, symamd_plan p, void *d_B, int k, int64_t rs,
+                   int64_t cs) {
+  return launch_regions_plan(h, p, d_B, d_B, k, rs, cs, false);
+}
+
+bool region_adjoint_usable(symamd_handle h, symamd_plan p, int k) {
+  return usable_plan(h, p, k, true) != nullptr;
+}
+
+bool region_adjoint_preferred(symamd_handle h, symamd_plan p, int k) {
+  RegionPlan *rp = usable_plan(h, p, k, true);
   return rp && rp->inside >= 0.6;
 }
 
-int launch_regions_adjoint(symamd_handle h, symamd_plan p, const void *d_x,
-                           const void *d_gx, void *d_gb, void *d_gL) {
-  if (p->n == 0) return SYMAMD_OK;
-  RegionPlan *rp = adjoint_plan(h, p);
-  if (!rp)
-    return fail(SYMAMD_ERR_NOT_SUPPORTED, "no transposed region schedule for this matrix");
-  const unsigned blocks = (unsigned)((p->n + 255) / 256);
-  if (d_gL) SYMAMD_TRY(plan_oci(h, p));
-  if (p->dtype == SYMAMD_F64) {
-    SYMAMD_TRY(region_dispatch<double>(h, p, *rp, d_gx, d_gb, 1, 1, 0, 1, 1));
-    if (d_gL)
-      k_grad_l<double><<<blocks, 256, 0, h->stream>>>(p->n, p->d_orp, p->d_oci,
-                                                      (const double *)d_x,
-                                                      (const double *)d_gb, (double *)d_gL);
-  } else {
-    SYMAMD_TRY(region_dispatch<float>(h, p, *rp, d_gx, d_gb, 1, 1, 0, 1, 1));
-    if (d_gL)
-      k_grad_l<float><<<blocks, 256, 0, h->stream>>>(p->n, p->d_orp, p->d_oci,
-                                                     (const float *)d_x,
-                                                     (const float *)d_gb, (float *)d_gL);
-  }
-  SYMAMD_HIP(hipGetLastError());
-  return SYMAMD_OK;
+int launch_grad_l(symamd_handle h, symamd_plan p, const void *d_X, const void *d_GB,
+                  void *d_gL, int k, int64_t rs, int64_t cs) {
+  if (p->n == 0 || k == 0) return SYMAMD_OK;
+  SYMAMD_TRY(plan_oci(h, p));
+  if (p->dtype == SYMAMD_F64)
+    return grad_dispatch<double>(h, p, (const double *)d_X, (const double *)d_GB,
+                                 (double *)d_gL, k, rs, cs);
+  return grad_dispatch<float>(h, p, (const float *)d_X, (const float *)d_GB, (float *)d_gL,
+                              k, rs, cs);
 }
 
 }  // namespace symamd

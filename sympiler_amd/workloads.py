@@ -224,6 +224,13 @@ def algorithmic_bytes_forward(n, nnz, k, vbytes=8):
     return (vbytes + 4) * nnz + 4 * (n + 1) + 2 * vbytes * n * k
 
 
+def algorithmic_bytes_adjoint_block(n, nnz, k, vbytes=8):
+    """Block adjoint of k right-hand sides: L read once (12*nnz + 4*(n+1)
+    for FP64), X and GX read (8nk each), GB written (8nk), grad-L written
+    (8*nnz)."""
+    return (vbytes + 4) * nnz + 4 * (n + 1) + 3 * vbytes * n * k + vbytes * nnz
+
+
 def algorithmic_bytes_adjoint(n, nnz, vbytes=8):
     """20*nnz + 28*n + 4 for FP64: L read once (12*nnz + 4*(n+1)), gx and x
     read (8n each), gb written (8n), grad-L written (8*nnz)."""

@@ -135,3 +135,28 @@ def test_forward_fp32_matches_fp32_oracle_order(dev):
     x_ref = oracle.sptrsv_csr(Lp, Li, Lx, b)
     err = np.max(np.abs(x.cpu().numpy().astype(np.float64) - x_ref)) / np.max(np.abs(x_ref))
     assert err < 1e-5
+
+
+@pytest.mark.parametrize("executor", ["auto", "syncfree"])
+def test_block_autograd_bitwise_and_dense(dev, executor):
+    # b as an (n, k) block: forward = row-major SpTRSM, backward = block
+    # adjoint; bitwise the oracle (loss 0.5 ||X||_F^2, so GX = X) and within
+    # 1e-12 of the dense FP64 gradient
+    Lp, Li, Lx, plan, dLx = _setup(12, dev)
+    plan.set_lanes(1)
+    n, k = len(Lp) - 1, 8
+    B = W.uniform(60, 0, n * k).reshape(n, k)
+    X, gL, GB = _grads(plan, dLx, to_dev(np.ascontiguousarray(B), device=dev), executor)
+    X_ref = oracle.sptrsm_csr(Lp, Li, Lx, np.ascontiguousarray(B.T)).T
+    assert bitwise_equal(X.cpu().numpy(), X_ref)
+    GB_ref, gL_ref = oracle.adjoint_block(Lp, Li, Lx, X_ref, X_ref)
+    assert bitwise_equal(GB.cpu().numpy(), GB_ref)
+    assert bitwise_equal(gL.cpu().numpy(), gL_ref)
+    D = np.zeros((n, n))
+    rows = np.repeat(np.arange(n), np.diff(Lp))
+    D[rows, Li] = Lx
+    Xd = np.linalg.solve(D, B)
+    GBd = np.linalg.solve(D.T, Xd)
+    gLd = -(GBd @ Xd.T)[rows, Li]
+    assert np.max(np.abs(GB_ref - GBd)) / np.max(np.abs(GBd)) < 1e-12
+    assert np.max(np.abs(gL_ref - gLd)) / np.max(np.abs(gLd)) < 1e-12

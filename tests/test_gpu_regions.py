@@ -211,6 +211,34 @@ def test_region_adjoint_bitwise(dev, mat):
         assert bitwise_equal(dgx.cpu().numpy(), gx)  # gx is not written
 
 
+@pytest.mark.parametrize("k,layout,executor", [(1, "row", "regions"), (8, "row", "regions"),
+                                               (13, "row", "auto"), (32, "row", "regions"),
+                                               (8, "col", "regions"), (5, "col", "syncfree"),
+                                               (3, "row", "syncfree")])
+def test_block_adjoint_bitwise(dev, k, layout, executor):
+    # GB = L^-T GX for a block of k columns (region schedule in one launch,
+    # or the sync-free adjoint column by column) and gL summed over the
+    # columns in ascending order: bitwise the oracle, on two calls
+    import torch
+    for Lp, Li, Lx in (W.ic0_lap3d(20, 0.01), W.random_lower(6000, 0.002, 1, seed=51)):
+        n = len(Lp) - 1
+        plan = sa.Plan(*to_dev(Lp, Li, Lx, device=dev), handle=_rh())
+        X = W.uniform(52, 0, n * k).reshape(n, k)
+        GX = W.uniform(53, 0, n * k).reshape(n, k)
+        GB_ref, gL_ref = oracle.adjoint_block(Lp, Li, Lx, X, GX)
+        lay = (lambda a: np.ascontiguousarray(a)) if layout == "row" else \
+            (lambda a: np.ascontiguousarray(a.T))
+        dX, dGX = to_dev(lay(X), lay(GX), device=dev)
+        gL = torch.empty(len(Lx), dtype=torch.float64, device=dev)
+        for _ in range(2):
+            GB, _ = plan.adjoint_block(dX, dGX, gLx=gL, layout=layout, executor=executor)
+            plan.check()
+            GBh = GB.cpu().numpy()
+            assert bitwise_equal(GBh if layout == "row" else GBh.T, GB_ref)
+            assert bitwise_equal(gL.cpu().numpy(), gL_ref)
+        assert bitwise_equal(dGX.cpu().numpy(), lay(GX))  # GX is not written
+
+
 def test_region_adjoint_fp32_matches_syncfree(dev):
     # FP32: the region kernel and the sync-free kernel (one lane per row)
     # perform the same float operations in the same order

@@ -432,3 +432,35 @@ def test_csr_csc_round_trip():
     assert np.array_equal(Cx[mp], Lx)
     # diagonal first in every column of the CSC
     assert np.array_equal(Ci[Cp[:-1]], np.arange(500))
+
+
+def test_adjoint_block_pinned_by_single_columns_and_dense():
+    # block of k right-hand sides: GB equals the single-column adjoints, and
+    # gL equals the running sum (ascending column) of their gL -- exact,
+    # since (-a) + (-b) = -(a + b) in round-to-nearest -- and the dense FP64
+    # gradient of 0.5 ||L^-1 B||_F^2
+    Lp, Li, Lx = W.ic0_lap3d(12)
+    n = len(Lp) - 1
+    for k in (1, 3, 8):
+        B = W.uniform(30 + k, 0, n * k).reshape(n, k)
+        X = np.stack([oracle.sptrsv_csr(Lp, Li, Lx, B[:, c]) for c in range(k)], axis=1)
+        GX = W.uniform(40 + k, 0, n * k).reshape(n, k)
+        GB, gL = oracle.adjoint_block(Lp, Li, Lx, X, GX)
+        acc = None
+        for c in range(k):
+            gb, gl = oracle.adjoint(Lp, Li, Lx, X[:, c].copy(), GX[:, c].copy())
+            assert bitwise_equal(GB[:, c], gb)
+            acc = gl if acc is None else acc + gl
+        assert bitwise_equal(gL, acc)
+    # dense gradient of the Frobenius loss, k = 4
+    k = 4
+    B = W.uniform(50, 0, n * k).reshape(n, k)
+    D = np.zeros((n, n))
+    rows = np.repeat(np.arange(n), np.diff(Lp))
+    D[rows, Li] = Lx
+    Xd = np.linalg.solve(D, B)
+    GB, gL = oracle.adjoint_block(Lp, Li, Lx, Xd, Xd)
+    GBd = np.linalg.solve(D.T, Xd)
+    assert np.max(np.abs(GB - GBd)) / np.max(np.abs(GBd)) < 1e-12
+    gLd = -(GBd @ Xd.T)[rows, Li]
+    assert np.max(np.abs(gL - gLd)) / np.max(np.abs(gLd)) < 1e-12
