@@ -11,7 +11,9 @@ re-initialises x before every timed run, demo/FusionDemo.cpp:59-62).
 Reported beside the headline (in "detail"): the single-RHS plain level-set
 solve (one launch per level), the same launches replayed from a hipGraph,
 the single-RHS sync-free solve, the adjoint (gb = L^-T gx and grad over
-L's pattern), and the column-major 32-RHS solve.  "cpu_baseline" times the
+L's pattern; automatic choice and sync-free), the block adjoint of the
+headline block (GB = L^-T GX for all its columns, grad-L summed over them),
+and the column-major 32-RHS solve.  "cpu_baseline" times the
 reference's OpenMP level-set solve (oracle/_ref, the reference's own source
 compiled for this host; the C restatement in oracle/ when that is absent) on
 this host's cores for a bounded sample of right-hand sides.
