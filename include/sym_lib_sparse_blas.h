@@ -68,6 +68,27 @@ void sptrsv_csc_lbc(int n, int *Lp, int *Li, double *Lx, double *x,
  * null input, 1 otherwise. */
 int ltsolve(int n, int *Lp, int *Li, double *Lx, double *x);
 
+/* sparse_blas_lib.h:242-245: IC(0) of the CSC lower triangle (diagonal
+ * first), in place; every column is computed (the reference asserts that
+ * each pivot is positive).  Results are the serial spic0_csr's bit for bit;
+ * the _lbc variant (whose reference applies its updates with atomics in a
+ * thread-dependent order) returns the same factor, its schedule unused. */
+void spic0_csr(int n, double *val, int *rowPtr, int *rowIdx);
+void spic0_csr_lbc(int n, double *val, int *rowPtr, int *colIdx, int level_no,
+                   int *level_ptr, int *par_ptr, int *partition);
+/* sparse_blas_lib.h:256-265: left-looking IC(0) of the CSC lower triangle of
+ * A into Lx.  The prune set must be the row pattern of the strict lower part,
+ * ascending (std::invalid_argument otherwise).  spic0_csc_left returns false
+ * at the first pivot <= 0 having written only the columns before it; the LBC
+ * variant writes every column (its schedule unused) and returns false if any
+ * pivot failed.  Bit-identical to the reference. */
+bool spic0_csc_left(int n, const int *Ap, const int *Ai, const double *Ax,
+                    const int *prunePtr, const int *pruneSet, double *Lx);
+bool spic0_csc_left_LBC(int n, const int *Ap, const int *Ai, const double *Ax,
+                        const int *prunePtr, const int *pruneSet, const int nLevels,
+                        const int *levelPtr, const int *parPtr, const int *partition,
+                        double *Lx);
+
 /* The level-set inspector of the un-vendored aggregation submodule, with the
  * signature its call sites use (Catch_tests/sptrsv_tests.cpp:59,
  * demo/sptrsv_demo_utils.h:44-45): input the CSC of L, outputs new[]'d for

@@ -34,7 +34,8 @@ typedef enum {
   SYMAMD_ERR_HIP = 4,           /* a HIP runtime call failed                */
   SYMAMD_ERR_ALLOC = 5,         /* device or host allocation failed         */
   SYMAMD_ERR_TIMEOUT = 6,       /* a sync-free wait exceeded its watchdog   */
-  SYMAMD_ERR_NOT_SUPPORTED = 7
+  SYMAMD_ERR_NOT_SUPPORTED = 7,
+  SYMAMD_ERR_NOT_SPD = 8        /* IC(0): a pivot that is not positive      */
 } symamd_status;
 
 typedef enum { SYMAMD_F64 = 0, SYMAMD_F32 = 1 } symamd_dtype;
@@ -277,6 +278,29 @@ int symamd_spmv_csr_axpby(symamd_handle h, int n, const int *d_Ap,
 int symamd_residual(symamd_handle h, symamd_plan plan, const void *d_X,
                     const void *d_B, int k, int64_t ld, symamd_layout layout,
                     double *h_rel);
+
+/* ----------------------------------------------------------------- IC(0) */
+/* Zero-fill incomplete Cholesky on the plan's pattern (FP64 plans).  d_Ax
+ * holds the lower triangle of the SPD matrix A on the plan's CSR pattern
+ * (diagonal last, the caller's entry order); d_Lx (may equal d_Ax) receives
+ * L on the same pattern, ready for symamd_plan_update_values.  Variants:
+ *   SYMAMD_IC0_LEFT  sym_lib::spic0_csc_left and spic0_csc_left_LBC
+ *                    (sparse_blas/spic0.cpp:11-46, :48-99; textbook IC(0):
+ *                    L[r,c] = (A[r,c] - sum_s L[r,s] L[c,s]) / L[c,c] over
+ *                    every s < c both rows store, ascending)
+ *   SYMAMD_IC0_CSR   sym_lib::spic0_csr and spic0_csr_lbc
+ *                    (sparse_blas/spic0.cpp:106-127, :130-160; the same sum
+ *                    restricted to the s whose last stored row is r)
+ * Every entry is bit-identical to the serial reference routine.  Every row
+ * is computed; a pivot that fails the variant's test (LEFT: <= 0, as
+ * spic0_csc_left returns false; CSR: not > 0, spic0_csr's assertion) sets
+ * *bad_row to the smallest such row and the call returns SYMAMD_ERR_NOT_SPD
+ * (rows that read it hold NaN or inf); otherwise *bad_row = -1.  Blocks until
+ * the factor is complete.  The update lists are built on the first call of a
+ * variant and kept by the plan. */
+typedef enum { SYMAMD_IC0_LEFT = 0, SYMAMD_IC0_CSR = 1 } symamd_ic0_variant;
+int symamd_ic0(symamd_handle h, symamd_plan plan, symamd_ic0_variant variant,
+               const double *d_Ax, double *d_Lx, int *bad_row);
 
 /* ----------------------------------------------------------------- tuner */
 /* Time the candidate schedules for k right-hand sides in `layout` on

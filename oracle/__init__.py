@@ -18,6 +18,7 @@ Functions and the reference lines they follow:
   ltsolve              sparse_blas/sptrsv_lt.cpp:11-20
   spmv_csr / _axpby    sparse_blas/spmv.cpp:12-21 / :23-34
   spic0_csc_left       sparse_blas/spic0.cpp:11-46
+  spic0_csr            sparse_blas/spic0.cpp:106-127
   levelset_csr/_csc    build_levelSet_CSC (aggregation, absent); definition
                        of parsy/common/TreeUtils.cpp:89-141
   sptrsm_csr           k columns of sptrsv_csr
@@ -64,6 +65,7 @@ def olib():
             "orc_spmv_csr": ([i, vp, vp, vp, vp, vp], None),
             "orc_spmv_csr_axpby": ([i, vp, vp, vp, vp, vp, vp, d, d], None),
             "orc_spic0_csc_left": ([i, vp, vp, vp, vp, vp, vp], c_int),
+            "orc_spic0_csr": ([i, vp, vp, vp], c_int),
             "orc_levelset_csr": ([i, vp, vp, vp, vp, vp], c_int),
             "orc_levelset_csc": ([i, vp, vp, vp, vp], c_int),
             "orc_check_schedule": ([i, vp, vp, i, vp, vp, vp], c_int),
@@ -211,6 +213,13 @@ def spic0_csc_left(Ap, Ai, Ax, prunePtr, pruneSet):
     Lx = np.zeros(len(Ax), np.float64)
     ok = olib().orc_spic0_csc_left(len(Ap) - 1, _p(_i32(Ap)), _p(_i32(Ai)), _p(_f64(Ax)),
                                    _p(_i32(prunePtr)), _p(_i32(pruneSet)), _p(Lx))
+    return Lx, bool(ok)
+
+
+def spic0_csr(Ap, Ai, Ax):
+    """In place on a copy of the CSC values: (Lx, all pivots positive)."""
+    Lx = _f64(Ax).copy()
+    ok = olib().orc_spic0_csr(len(Ap) - 1, _p(Lx), _p(_i32(Ap)), _p(_i32(Ai)))
     return Lx, bool(ok)
 
 

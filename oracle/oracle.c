@@ -243,6 +243,31 @@ int orc_spic0_csc_left(int n, const int *Ap, const int *Ai, const double *Ax,
   return 1;
 }
 
+/* sparse_blas/spic0.cpp:106-127 (sym_lib::spic0_csr): right-looking IC(0)
+ * in place on the CSC lower triangle (diagonal first), loop for loop,
+ * including the update guard of :118 -- l advances from m only while the
+ * row of the NEXT entry is <= the target row, so a match is reached only at
+ * the last entry of column i (whose successor, the next column's diagonal,
+ * has row i + 1).  In place of the reference's assert(pivot > 0) this
+ * returns 0 when some pivot is not positive (every column is still
+ * computed, as in a build without assertions), 1 otherwise. */
+int orc_spic0_csr(int n, double *Lx, const int *Lp, const int *Li) {
+  int ok = 1;
+  for (int i = 0; i < n; i++) {
+    if (!(Lx[Lp[i]] > 0)) ok = 0;
+    Lx[Lp[i]] = sqrt(Lx[Lp[i]]);
+    double d = Lx[Lp[i]];
+    for (int m = Lp[i] + 1; m < Lp[i + 1]; m++) Lx[m] = Lx[m] / d;
+    for (int m = Lp[i] + 1; m < Lp[i + 1]; m++)
+      for (int k = Lp[Li[m]]; k < Lp[Li[m] + 1]; k++)
+        /* l + 1 < Lp[n] holds: a column with an off-diagonal m is not the
+         * last one */
+        for (int l = m; l < Lp[i + 1] && Li[l + 1] <= Li[k]; l++)
+          if (Li[l] == Li[k]) Lx[k] -= Lx[m] * Lx[l];
+  }
+  return ok;
+}
+
 /* --------------------------------------------------------- level sets */
 
 /* Level-set inspector on the CSR of L (the reference's build_levelSet_CSC

@@ -19,6 +19,8 @@ Reference mapping (reference include/sympiler/sparse_blas_lib.h):
   solve_lbc_redundant()  sym_lib::sptrsv_csr_lbc_redundant (sptrsv.cpp:78-106)
   ilu_lower()            the lower factor read by sym_lib::sptrsv_csr_ilu /
                          sptrsv_csr_ilu_lbc (sptrsv.cpp:28-38, :109-126)
+  Plan.ic0()             IC(0) on the plan's pattern (spic0.cpp:11-46,
+                         :106-127)
 """
 from __future__ import annotations
 
@@ -35,7 +37,7 @@ F64, F32 = 0, 1
 COL_MAJOR, ROW_MAJOR = 0, 1
 EXEC = {"auto": 0, "levelset": 1, "syncfree": 2, "graph": 3, "regions": 4}
 STATUS = {0: "OK", 1: "INVALID_ARG", 2: "STRUCTURE", 3: "ZERO_PIVOT", 4: "HIP",
-          5: "ALLOC", 6: "TIMEOUT", 7: "NOT_SUPPORTED"}
+          5: "ALLOC", 6: "TIMEOUT", 7: "NOT_SUPPORTED", 8: "NOT_SPD"}
 
 
 class SymAmdError(RuntimeError):
@@ -93,6 +95,7 @@ def lib() -> ctypes.CDLL:
             "symamd_spmv_csr_axpby": [vp, i32, vp, vp, vp, vp, vp, vp, dbl, dbl],
             "symamd_residual": [vp, vp, vp, vp, i32, i64, i32, vp],
             "symamd_tune": [vp, vp, i32, i32, c_char_p, i32],
+            "symamd_ic0": [vp, vp, i32, vp, vp, POINTER(i32)],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -290,6 +293,31 @@ class Plan:
         if Lx.dtype != self.dtype:
             raise ValueError("value dtype differs from the plan's")
         _check(lib().symamd_plan_update_values(self.h._h, self._p, _ptr(Lx)))
+
+    def ic0(self, Ax, variant="left", out=None, strict=True):
+        """Zero-fill incomplete Cholesky of the SPD matrix whose lower triangle
+        Ax holds on this plan's pattern (FP64, the caller's CSR order); returns
+        L on the same pattern, bit-identical to sym_lib::spic0_csc_left
+        (variant "left", sparse_blas/spic0.cpp:11-46) or sym_lib::spic0_csr
+        ("csr", :106-127).  out may be Ax (in place).  A pivot that fails the
+        variant's test raises ValueError when strict, else the factor is
+        returned with the failed row: (L, row or -1)."""
+        import torch
+        _require_cuda(Ax)
+        if Ax.dtype != torch.float64 or self.dtype != torch.float64:
+            raise ValueError("IC(0) needs an FP64 plan and values")
+        var = {"left": 0, "csr": 1}[variant]
+        if out is None:
+            out = torch.empty_like(Ax)
+        _require_cuda(out)
+        bad = c_int(-1)
+        rc = lib().symamd_ic0(self.h._h, self._p, var, _ptr(Ax), _ptr(out), byref(bad))
+        if rc == 8 and not strict:
+            return out, int(bad.value)
+        if rc == 8:
+            raise ValueError(f"IC(0) breakdown: pivot of row {bad.value} is not positive")
+        _check(rc)
+        return out if strict else (out, -1)
 
     # ---------------------------------------------------------------- solve
     @staticmethod

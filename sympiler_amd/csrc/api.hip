@@ -56,6 +56,7 @@ void free_plan_arrays(symamd_plan p) {
   }
   for (auto &kv : p->rplans) free_region_plan(kv.second);
   for (auto &kv : p->rplans_t) free_region_plan(kv.second);
+  free_ic0_pairs(p);
   for (auto &kv : p->gcache) {
     if (kv.second.second) (void)hipGraphExecDestroy(kv.second.second);
     if (kv.second.first) (void)hipGraphDestroy(kv.second.first);
@@ -938,6 +939,24 @@ int symamd_ilu_lower(symamd_handle h, int n, const int *d_Ap, const int *d_Ai,
     return fail(SYMAMD_ERR_INVALID_ARG, "symamd_ilu_lower: bad arguments");
   return ilu_lower(h, n, d_Ap, d_Ai, d_Ax, d_diags, unit_diag ? 1 : 0, d_Lp, d_Li, d_Lx,
                    nnz);
+}
+
+int symamd_ic0(symamd_handle h, symamd_plan plan, symamd_ic0_variant variant,
+               const double *d_Ax, double *d_Lx, int *bad_row) {
+  SYMAMD_TRY(check_handle(h));
+  SYMAMD_TRY(check_plan(plan));
+  if (!bad_row || (variant != SYMAMD_IC0_LEFT && variant != SYMAMD_IC0_CSR) ||
+      (plan->nnz > 0 && (!d_Ax || !d_Lx)))
+    return fail(SYMAMD_ERR_INVALID_ARG, "symamd_ic0: bad arguments");
+  if (plan->dtype != SYMAMD_F64)
+    return fail(SYMAMD_ERR_NOT_SUPPORTED, "symamd_ic0: FP64 plans only");
+  *bad_row = -1;
+  if (plan->n == 0) return SYMAMD_OK;
+  SYMAMD_TRY(ic0_factor(h, plan, (int)variant, d_Ax, d_Lx, bad_row));
+  SYMAMD_TRY(symamd_plan_check(h, plan));
+  if (*bad_row >= 0)
+    return fail(SYMAMD_ERR_NOT_SPD, "symamd_ic0: pivot of row %d is not positive", *bad_row);
+  return SYMAMD_OK;
 }
 
 }  // extern "C"

@@ -126,6 +126,15 @@ struct UnitSet {
   int64_t xdeps = 0;
 };
 
+// IC(0) update pairs of one variant (ic0.hip): entry p of the caller's CSR
+// subtracts Lx[pa[t]] * Lx[pb[t]] for t in [pp[p], pp[p+1]), in order.
+struct Ic0Pairs {
+  int64_t pairs = 0;
+  int64_t *d_pp = nullptr;  // nnz + 1
+  int *d_pa = nullptr;
+  int *d_pb = nullptr;
+};
+
 // Region schedule: each region (a compact cluster of rows, at most `cap`
 // rows) is solved by one wavefront in LDS, its internal levels in order;
 // regions wait for the regions they read from (a DAG, ranked
@@ -298,6 +307,9 @@ struct symamd_plan_s {
   std::map<int, symamd::UnitSet> units;  // rows-per-unit -> units
   std::map<int, symamd::RegionPlan> rplans;  // region cap -> region schedule
   std::map<int, symamd::RegionPlan> rplans_t;  // same, transposed solve
+  // IC(0): update pairs per variant, and the failed-pivot word
+  std::map<int, symamd::Ic0Pairs> ic0;
+  int *d_ic0_bad = nullptr;
   // schedule choices
   int lanes_override = 0;
   std::map<int64_t, symamd::Choice> tuned;  // key(k, layout) -> choice
@@ -413,6 +425,11 @@ bool region_usable(symamd_handle h, symamd_plan p, int k);
 // schedule with at least 60 % of L's off-diagonal entries inside regions
 // (column-major blocks only through the workspace).
 bool region_preferred(symamd_handle h, symamd_plan p, int k, int layout);
+// IC(0) of the plan's pattern (ic0.hip); *bad_row = smallest failed pivot
+// row or -1.  Blocks.
+int ic0_factor(symamd_handle h, symamd_plan p, int variant, const double *d_Ax,
+               double *d_Lx, int *bad_row);
+void free_ic0_pairs(symamd_plan p);
 int ilu_lower(symamd_handle h, int n, const int *d_Ap, const int *d_Ai,
               const double *d_Ax, const int *d_diags, int unit, int *d_Lp,
               int *d_Li, double *d_Lx, int *nnz_out);

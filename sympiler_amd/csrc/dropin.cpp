@@ -285,6 +285,113 @@ int ltsolve(int n, int *Lp, int *Li, double *Lx, double *x) {
   return 1;
 }
 
+// ---------------------------------------------------------------- IC(0)
+// The CSC lower triangle of A (diagonal first) the reference factors is the
+// CSR of the same triangle transposed: symamd_ic0 factors the CSR (diagonal
+// last) on the device and the values are mapped back to CSC positions.
+namespace {
+
+// IC(0) of the CSC lower triangle (Ap, Ai, Ax) with one of symamd_ic0's
+// variants: the factor in CSC order; returns the smallest column whose pivot
+// failed the variant's test, or -1.
+int ic0_csc(int n, const int *Ap, const int *Ai, const double *Ax, symamd_ic0_variant variant,
+            std::vector<double> &out) {
+  const int nnz = Ap[n];
+  std::vector<int> Rp, Ri;
+  std::vector<double> Rx;
+  csc_to_csr(n, Ap, Ai, Ax, Rp, Ri, Rx);
+  // CSR position of every CSC position (csc_to_csr fills rows in column order)
+  std::vector<int> fill(Rp.begin(), Rp.end() - 1), pos(nnz);
+  for (int c = 0; c < n; ++c)
+    for (int p = Ap[c]; p < Ap[c + 1]; ++p) pos[p] = fill[Ai[p]]++;
+  symamd_handle h = handle();
+  DevBuf<int> dLp(n + 1, Rp.data()), dLi(nnz, Ri.data());
+  DevBuf<double> dA(nnz, Rx.data());
+  Plan pl;
+  check(symamd_analyze(h, n, dLp.p, dLi.p, dA.p, SYMAMD_F64, &pl.p), "analyze");
+  int bad = -1;
+  const int rc = symamd_ic0(h, pl.p, variant, dA.p, dA.p, &bad);
+  if (rc != SYMAMD_ERR_NOT_SPD) check(rc, "ic0");
+  dA.download(Rx.data(), nnz);
+  out.resize(nnz);
+  for (int p = 0; p < nnz; ++p) out[p] = Rx[pos[p]];
+  return bad;
+}
+
+// The prune set spic0_csc_left walks must be the row pattern of the strict
+// lower part (for column c the s < c with L[c,s] != 0, ascending): the
+// factor here is defined by that pattern.
+void check_prune(int n, const int *Ap, const int *Ai, const int *prunePtr,
+                 const int *pruneSet) {
+  if (!prunePtr || (Ap[n] > n && !pruneSet) || prunePtr[0] != 0)
+    throw std::invalid_argument("spic0_csc_left: bad prune set");
+  std::vector<int> cnt(n + 1, 0);
+  for (int c = 0; c < n; ++c)
+    for (int p = Ap[c] + 1; p < Ap[c + 1]; ++p) cnt[Ai[p] + 1]++;
+  for (int c = 0; c < n; ++c) {
+    cnt[c + 1] += cnt[c];
+    if (prunePtr[c + 1] != cnt[c + 1])
+      throw std::invalid_argument(
+          "spic0_csc_left: the prune set must be the row pattern of the strict lower part");
+  }
+  std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+  for (int c = 0; c < n; ++c)
+    for (int p = Ap[c] + 1; p < Ap[c + 1]; ++p)
+      if (pruneSet[fill[Ai[p]]++] != c)
+        throw std::invalid_argument(
+            "spic0_csc_left: the prune set must be the row pattern of the strict lower "
+            "part, ascending");
+}
+
+}  // namespace
+
+bool spic0_csc_left(int n, const int *Ap, const int *Ai, const double *Ax,
+                    const int *prunePtr, const int *pruneSet, double *Lx) {
+  if (n <= 0) return true;
+  check_prune(n, Ap, Ai, prunePtr, pruneSet);
+  std::vector<double> f;
+  const int bad = ic0_csc(n, Ap, Ai, Ax, SYMAMD_IC0_LEFT, f);
+  // the reference returns false at the first failed pivot, having written
+  // the columns before it (a prefix of the CSC arrays) and nothing after
+  const int end = bad < 0 ? Ap[n] : Ap[bad];
+  std::memcpy(Lx, f.data(), sizeof(double) * (size_t)end);
+  return bad < 0;
+}
+
+bool spic0_csc_left_LBC(int n, const int *Ap, const int *Ai, const double *Ax,
+                        const int *prunePtr, const int *pruneSet, const int nLevels,
+                        const int *levelPtr, const int *parPtr, const int *partition,
+                        double *Lx) {
+  (void)nLevels;
+  (void)levelPtr;
+  (void)parPtr;
+  (void)partition;
+  if (n <= 0) return true;
+  check_prune(n, Ap, Ai, prunePtr, pruneSet);
+  std::vector<double> f;
+  // the LBC routine computes every column and reports a failed pivot
+  const int bad = ic0_csc(n, Ap, Ai, Ax, SYMAMD_IC0_LEFT, f);
+  std::memcpy(Lx, f.data(), sizeof(double) * (size_t)Ap[n]);
+  return bad < 0;
+}
+
+void spic0_csr(int n, double *val, int *rowPtr, int *rowIdx) {
+  if (n <= 0) return;
+  std::vector<double> f;
+  // every column is computed (the reference asserts pivot > 0)
+  (void)ic0_csc(n, rowPtr, rowIdx, val, SYMAMD_IC0_CSR, f);
+  std::memcpy(val, f.data(), sizeof(double) * (size_t)rowPtr[n]);
+}
+
+void spic0_csr_lbc(int n, double *val, int *rowPtr, int *colIdx, int level_no,
+                   int *level_ptr, int *par_ptr, int *partition) {
+  (void)level_no;
+  (void)level_ptr;
+  (void)par_ptr;
+  (void)partition;
+  spic0_csr(n, val, rowPtr, colIdx);
+}
+
 int build_levelSet_CSC(int n, const int *Lp, const int *Li, int *&level_ptr,
                        int *&level_set) {
   level_ptr = new int[n + 1];

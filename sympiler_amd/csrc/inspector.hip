@@ -604,9 +604,11 @@ int build_unit_deps(symamd_handle h, symamd_plan p, const UnitSet &us,
   SYMAMD_HIP(hipMallocAsync(&d_skeys, 8 * mm, h->stream));
   SYMAMD_HIP(hipMallocAsync(&d_flag, 4 * (mm + 1), h->stream));
   SYMAMD_HIP(hipMallocAsync(&d_pos, 4 * (mm + 1), h->stream));
-  k_unit_dep_keys<<<blocks_for(U), 256, 0, h->stream>>>(U, us.d_ub, rp, ci, skip_first,
-                                                        p->d_ipos, uop, d_off, d_keys);
-  SYMAMD_HIP(hipGetLastError());
+  if (U > 0) {
+    k_unit_dep_keys<<<blocks_for(U), 256, 0, h->stream>>>(U, us.d_ub, rp, ci, skip_first,
+                                                          p->d_ipos, uop, d_off, d_keys);
+    SYMAMD_HIP(hipGetLastError());
+  }
   unsigned ubits = 1;
   while ((1u << ubits) < (unsigned)U && ubits < 31) ++ubits;
   if (m > 0) {
@@ -627,9 +629,11 @@ int build_unit_deps(symamd_handle h, symamd_plan p, const UnitSet &us,
   SYMAMD_HIP(hipStreamSynchronize(h->stream));
   SYMAMD_HIP(hipMalloc(d_dlist, sizeof(int) * (uniq > 0 ? uniq : 1)));
   SYMAMD_HIP(hipMemsetAsync(d_cnt, 0, sizeof(int) * (U + 1), h->stream));
-  k_compact_deps<<<blocks_for(m), 256, 0, h->stream>>>(m, d_skeys, d_flag, d_pos,
-                                                        *d_dlist, d_cnt);
-  SYMAMD_HIP(hipGetLastError());
+  if (m > 0) {  // a diagonal matrix has no dependencies
+    k_compact_deps<<<blocks_for(m), 256, 0, h->stream>>>(m, d_skeys, d_flag, d_pos,
+                                                          *d_dlist, d_cnt);
+    SYMAMD_HIP(hipGetLastError());
+  }
   SYMAMD_TRY(scan_exclusive(h, d_cnt, *d_dptr, U + 1));
   SYMAMD_HIP(hipFreeAsync(d_keys, h->stream));
   SYMAMD_HIP(hipFreeAsync(d_skeys, h->stream));
@@ -691,8 +695,10 @@ int plan_units(symamd_handle h, symamd_plan p, int rows, bool adjoint,
   SYMAMD_TRY(plan_ipos(h, p));
   int *d_uop = nullptr;
   SYMAMD_HIP(hipMallocAsync(&d_uop, sizeof(int) * (p->n > 0 ? p->n : 1), h->stream));
-  k_unit_of_pos<<<blocks_for(U), 256, 0, h->stream>>>(U, us.d_ub, d_uop);
-  SYMAMD_HIP(hipGetLastError());
+  if (U > 0) {
+    k_unit_of_pos<<<blocks_for(U), 256, 0, h->stream>>>(U, us.d_ub, d_uop);
+    SYMAMD_HIP(hipGetLastError());
+  }
   int rc = build_unit_deps(h, p, us, rp, ci, adjoint ? 1 : 0, d_uop, &us.d_dptr,
                            &us.d_dlist, &us.deps);
   (void)hipFreeAsync(d_uop, h->stream);

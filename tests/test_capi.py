@@ -56,6 +56,10 @@ def test_sym_lib_dropins_exported_with_reference_mangling():
         "_ZN7sym_lib24sptrsv_csr_lbc_redundantEiPiS0_PdS1_S1_iS0_S0_S0_S1_",
         "_ZN7sym_lib14sptrsv_csr_iluEiPKiS1_PKdS1_Pd",
         "_ZN7sym_lib18sptrsv_csr_ilu_lbcEiPKiS1_PKdS1_PdiS1_S1_S1_",
+        "_ZN7sym_lib9spic0_csrEiPdPiS1_",
+        "_ZN7sym_lib13spic0_csr_lbcEiPdPiS1_iS1_S1_S1_",
+        "_ZN7sym_lib14spic0_csc_leftEiPKiS1_PKdS1_S1_Pd",
+        "_ZN7sym_lib18spic0_csc_left_LBCEiPKiS1_PKdS1_S1_iS1_S1_S1_Pd",
     }
     missing = want - exported(LIB)
     assert not missing, missing

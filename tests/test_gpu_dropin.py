@@ -132,3 +132,65 @@ def test_structure_errors_reported(L):
     with pytest.raises(sa.SymAmdError) as e:
         sa.Plan(*to_dev(Lp, bad, Lx, device="cuda:0"))
     assert e.value.code == 2
+
+
+def _csc_spd(Lp, Li, A):
+    Cp, Ci, Cx, _ = oracle.csr_to_csc(Lp, Li, A)
+    pp, ps = oracle.prune_set(Lp, Li)
+    return Cp, Ci, Cx, pp, ps
+
+
+def test_spic0_csc_left_and_lbc(L):
+    # sparse_blas/spic0.cpp:11-46 and :48-99 on host CSC arrays
+    Lp, Li, A = W.lap3d_tril(9, 8, 7, 6.01, -1.0)
+    Cp, Ci, Cx, pp, ps = _csc_spd(Lp, Li, A)
+    want, ok = oracle.spic0_csc_left(Cp, Ci, Cx, pp, ps)
+    assert ok
+    n = len(Cp) - 1
+    Lx = np.full(len(Cx), 7.0)
+    fn = L["_ZN7sym_lib14spic0_csc_leftEiPKiS1_PKdS1_S1_Pd"]
+    fn.restype = ctypes.c_bool
+    assert fn(c_int(n), P(Cp), P(Ci), P(Cx), P(pp), P(ps), P(Lx))
+    assert bitwise_equal(Lx, want)
+    lbc = L["_ZN7sym_lib18spic0_csc_left_LBCEiPKiS1_PKdS1_S1_iS1_S1_S1_Pd"]
+    lbc.restype = ctypes.c_bool
+    lp, ls, _ = oracle.levelset_csr(Lp, Li)
+    par = np.arange(len(lp), dtype=np.int32)  # one partition per level
+    Lx = np.zeros(len(Cx))
+    assert lbc(c_int(n), P(Cp), P(Ci), P(Cx), P(pp), P(ps), c_int(len(lp) - 1), P(par),
+               P(lp), P(ls), P(Lx))
+    assert bitwise_equal(Lx, want)
+
+
+def test_spic0_csc_left_breakdown_writes_the_prefix(L):
+    Lp, Li, A = W.lap2d_tril(6, 6, 2.2, -1.0)  # not positive definite
+    Cp, Ci, Cx, pp, ps = _csc_spd(Lp, Li, A)
+    want, ok = oracle.spic0_csc_left(Cp, Ci, Cx, pp, ps)
+    assert not ok
+    n = len(Cp) - 1
+    jf = int(np.argmax(~(want[Cp[:-1]] > 0)))
+    Lx = np.full(len(Cx), 7.0)
+    fn = L["_ZN7sym_lib14spic0_csc_leftEiPKiS1_PKdS1_S1_Pd"]
+    fn.restype = ctypes.c_bool
+    assert not fn(c_int(n), P(Cp), P(Ci), P(Cx), P(pp), P(ps), P(Lx))
+    assert bitwise_equal(Lx[:Cp[jf]], want[:Cp[jf]])
+    assert np.all(Lx[Cp[jf]:] == 7.0)
+
+
+def test_spic0_csr_and_lbc(L):
+    # sparse_blas/spic0.cpp:106-127 and :130-160, in place on host CSC arrays
+    Lp, Li, A = W.lap3d_tril(9, 8, 7, 6.01, -1.0)
+    Cp, Ci, Cx, _, _ = _csc_spd(Lp, Li, A)
+    want, ok = oracle.spic0_csr(Cp, Ci, Cx)
+    assert ok
+    n = len(Cp) - 1
+    val = Cx.copy()
+    L["_ZN7sym_lib9spic0_csrEiPdPiS1_"](c_int(n), P(val), P(Cp.copy()), P(Ci.copy()))
+    assert bitwise_equal(val, want)
+    lp, ls, _ = oracle.levelset_csr(Lp, Li)
+    par = np.arange(len(lp), dtype=np.int32)
+    val = Cx.copy()
+    L["_ZN7sym_lib13spic0_csr_lbcEiPdPiS1_iS1_S1_S1_"](
+        c_int(n), P(val), P(Cp.copy()), P(Ci.copy()), c_int(len(lp) - 1), P(par), P(lp),
+        P(ls))
+    assert bitwise_equal(val, want)
