@@ -133,6 +133,7 @@ struct Ic0Pairs {
   int64_t *d_pp = nullptr;  // nnz + 1
   int *d_pa = nullptr;
   int *d_pb = nullptr;
+  int *d_dpos = nullptr;    // position of L[c,c] for each entry (r, c)
 };
 
 // Region schedule: each region (a compact cluster of rows, at most `cap`
