@@ -96,8 +96,9 @@ def latency_floor(torch, sa, W, dev, stream, levels, ms_k, ms_1):
 def other_configs(torch, sa, W, dev, stream):
     """The other workloads, a few timed repetitions each (seconds of GPU
     time): Zipf-irregular SpTRSM (n = 1e6, k = 1/8/32/128, column-major), the
-    512-system uneven batch replayed from a hipGraph, and the 64^3 IC(0)
-    adjoint (forward vs forward+backward, FP64 and FP32)."""
+    512-system uneven batch replayed from a hipGraph, the 64^3 IC(0)
+    adjoint (forward vs forward+backward, FP64 and FP32) and the device IC(0)
+    factorization of the 128^3 Laplacian."""
     out = {}
     # Zipf(1.8) irregular lower triangle, B column-major (k, n)
     w3 = W.config3(k=1)
@@ -171,6 +172,30 @@ def other_configs(torch, sa, W, dev, stream):
         del plan5
     out["adjoint_64cubed_autograd"] = adj
     log(f"adjoint 64^3: {adj}")
+    del Lx_t, b_t
+
+    # IC(0) of the 128^3 7-point Laplacian on the device (symamd_ic0), both
+    # variants; checked against the package's CPU generator (spic0_csc_left
+    # order)
+    g6 = 128
+    Lp6, Li6, A6 = W.lap3d_tril(g6, g6, g6, 6.01, -1.0)
+    dA6 = torch.from_numpy(A6).to(dev)
+    plan6 = sa.Plan(torch.from_numpy(Lp6).to(dev), torch.from_numpy(Li6).to(dev), dA6)
+    ic0 = {"n": len(Lp6) - 1, "nnz": int(Lp6[-1]), "levels": plan6.levels}
+    for variant in ("left", "csr"):
+        t0 = time.perf_counter()
+        L6 = plan6.ic0(dA6, variant)
+        first = time.perf_counter() - t0
+        wi, _ = timed_steps(lambda: plan6.ic0(dA6, variant, out=L6), lambda: None, 5, 1,
+                            stream, torch)
+        ic0[variant] = {"first_call_ms": round(1e3 * first, 2),
+                        "factor_ms": round(1e3 * float(np.mean(wi)), 3)}
+        if variant == "left":
+            ic0[variant]["bitwise_vs_cpu_generator"] = bool(np.array_equal(
+                L6.cpu().numpy().view(np.int64), W.ic0(Lp6, Li6, A6).view(np.int64)))
+    out["ic0_lap3d_128cubed"] = ic0
+    log(f"ic0 128^3: {ic0}")
+    del plan6, dA6, L6
     return out
 
 
@@ -458,13 +483,25 @@ def cpu_baseline(Lp, Li, Lx, Bcols, Xgpu, args, k):
             worst = max(worst, float(np.max(np.abs(Xh[c] - x)) / np.max(np.abs(x))))
         done += 1
     rate = done / t_total
+    # the reference's IC(0) (spic0_csc_left, serial) on the 128^3 Laplacian,
+    # beside the device factor timed in the detail
+    ic0_ms = None
+    if kind == "reference":
+        from sympiler_amd import workloads as W
+        Lp6, Li6, A6 = W.lap3d_tril(128, 128, 128, 6.01, -1.0)
+        Cp, Ci, Cx, _ = oracle.csr_to_csc(Lp6, Li6, A6)
+        pp, ps = oracle.prune_set(Lp6, Li6)
+        t0 = time.perf_counter()
+        oracle.ref.spic0_csc_left(Cp, Ci, Cx, pp, ps)
+        ic0_ms = round(1e3 * (time.perf_counter() - t0), 1)
     log(f"cpu baseline ({kind}, {ncores} threads): "
         f"{rate:.2f} RHS-solves/s over {done} solves; GPU vs CPU max rel err {worst:.2e}")
     return {"value": round(rate, 3), "unit": "RHS-solves/s",
             "cores": ncores, "kind": kind,
             "sample": f"{done} single-RHS level-set solves of the same matrix "
                       f"({t_total:.1f}s)",
-            "gpu_vs_cpu_max_rel_err": worst}
+            "gpu_vs_cpu_max_rel_err": worst,
+            "ic0_spic0_csc_left_128cubed_ms_1_thread": ic0_ms}
 
 
 if __name__ == "__main__":

@@ -89,6 +89,16 @@ bool spic0_csc_left_LBC(int n, const int *Ap, const int *Ai, const double *Ax,
                         const int *levelPtr, const int *parPtr, const int *partition,
                         double *Lx);
 
+/* sparse_blas_lib.h:276-281: ILU(0) of a general CSR matrix (A_diag[i] =
+ * position of row i's diagonal) into l, bit-identical to the reference; every
+ * row is computed (the reference asserts non-zero U diagonals).  The _lbc
+ * variant returns the same factor; its schedule and tempvecs are unused. */
+void spilu0_csr(int n, int nnz, const int *Ap, const int *Ai, const double *Ax,
+                const int *A_diag, double *l);
+void spilu0_csr_lbc(int n, int nnz, const int *Ap, const int *Ai, const double *Ax,
+                    const int *A_diag, double *l, int level_no, const int *level_ptr,
+                    const int *par_ptr, const int *partition, int *tempvecs);
+
 /* The level-set inspector of the un-vendored aggregation submodule, with the
  * signature its call sites use (Catch_tests/sptrsv_tests.cpp:59,
  * demo/sptrsv_demo_utils.h:44-45): input the CSC of L, outputs new[]'d for

@@ -30,7 +30,7 @@ typedef enum {
   SYMAMD_OK = 0,
   SYMAMD_ERR_INVALID_ARG = 1,   /* null pointer, negative size, bad enum   */
   SYMAMD_ERR_STRUCTURE = 2,     /* not lower / diagonal not last / unsorted */
-  SYMAMD_ERR_ZERO_PIVOT = 3,    /* a zero diagonal entry                    */
+  SYMAMD_ERR_ZERO_PIVOT = 3,    /* a zero diagonal entry (or ILU(0) pivot)  */
   SYMAMD_ERR_HIP = 4,           /* a HIP runtime call failed                */
   SYMAMD_ERR_ALLOC = 5,         /* device or host allocation failed         */
   SYMAMD_ERR_TIMEOUT = 6,       /* a sync-free wait exceeded its watchdog   */
@@ -301,6 +301,23 @@ int symamd_residual(symamd_handle h, symamd_plan plan, const void *d_X,
 typedef enum { SYMAMD_IC0_LEFT = 0, SYMAMD_IC0_CSR = 1 } symamd_ic0_variant;
 int symamd_ic0(symamd_handle h, symamd_plan plan, symamd_ic0_variant variant,
                const double *d_Ax, double *d_Lx, int *bad_row);
+
+/* ILU(0) of a general CSR matrix A (n rows; columns ascending in each row;
+ * d_Adiag[i] = position of row i's diagonal), bit-identical to
+ * sym_lib::spilu0_csr and spilu0_csr_lbc (sparse_blas/spilu0.cpp:11-51,
+ * :54-110; decls sparse_blas_lib.h:276-281): d_l (may equal d_Ax) receives,
+ * on A's pattern, L strictly below the diagonal (unit diagonal implied) and U
+ * on and above it:
+ *   L[i,c] = (A[i,c] - sum_k L[i,k] U[k,c]) / U[c,c]   (c < i)
+ *   U[i,c] =  A[i,c] - sum_k L[i,k] U[k,c]             (c >= i)
+ * over the k < min(i, c) where A stores (i, k) and (k, c), ascending.  Every
+ * row is computed; a zero U[i,i] (the reference's assertion) sets *bad_row to
+ * the smallest such row and returns SYMAMD_ERR_ZERO_PIVOT, else *bad_row =
+ * -1.  SYMAMD_ERR_STRUCTURE when a row is not ascending or d_Adiag[i] is not
+ * its diagonal.  Builds its schedule from A's lower part on each call and
+ * blocks until the factor is complete. */
+int symamd_ilu0(symamd_handle h, int n, const int *d_Ap, const int *d_Ai,
+                const double *d_Ax, const int *d_Adiag, double *d_l, int *bad_row);
 
 /* ----------------------------------------------------------------- tuner */
 /* Time the candidate schedules for k right-hand sides in `layout` on

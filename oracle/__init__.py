@@ -19,6 +19,7 @@ Functions and the reference lines they follow:
   spmv_csr / _axpby    sparse_blas/spmv.cpp:12-21 / :23-34
   spic0_csc_left       sparse_blas/spic0.cpp:11-46
   spic0_csr            sparse_blas/spic0.cpp:106-127
+  spilu0_csr           sparse_blas/spilu0.cpp:11-51
   levelset_csr/_csc    build_levelSet_CSC (aggregation, absent); definition
                        of parsy/common/TreeUtils.cpp:89-141
   sptrsm_csr           k columns of sptrsv_csr
@@ -66,6 +67,7 @@ def olib():
             "orc_spmv_csr_axpby": ([i, vp, vp, vp, vp, vp, vp, d, d], None),
             "orc_spic0_csc_left": ([i, vp, vp, vp, vp, vp, vp], c_int),
             "orc_spic0_csr": ([i, vp, vp, vp], c_int),
+            "orc_spilu0_csr": ([i, i, vp, vp, vp, vp, vp], c_int),
             "orc_levelset_csr": ([i, vp, vp, vp, vp, vp], c_int),
             "orc_levelset_csc": ([i, vp, vp, vp, vp], c_int),
             "orc_check_schedule": ([i, vp, vp, i, vp, vp, vp], c_int),
@@ -221,6 +223,15 @@ def spic0_csr(Ap, Ai, Ax):
     Lx = _f64(Ax).copy()
     ok = olib().orc_spic0_csr(len(Ap) - 1, _p(Lx), _p(_i32(Ap)), _p(_i32(Ai)))
     return Lx, bool(ok)
+
+
+def spilu0_csr(Ap, Ai, Ax, A_diag):
+    """(l, first row with a zero U diagonal or -1; -2 on a structure error)."""
+    nnz = int(Ap[-1])
+    l = np.zeros(nnz, np.float64)
+    bad = olib().orc_spilu0_csr(len(Ap) - 1, nnz, _p(_i32(Ap)), _p(_i32(Ai)), _p(_f64(Ax)),
+                                _p(_i32(A_diag)), _p(l))
+    return l, int(bad)
 
 
 def levelset_csr(Lp, Li):

@@ -268,6 +268,42 @@ int orc_spic0_csr(int n, double *Lx, const int *Lp, const int *Li) {
   return ok;
 }
 
+/* sparse_blas/spilu0.cpp:11-51 (sym_lib::spilu0_csr): ILU(0) of a general
+ * CSR matrix, row by row, in place on a copy of A (l), with the row's
+ * column map iw (EMPTY = -1).  In place of the reference's assertions this
+ * returns -2 when a row's loop does not stop at A_diag[i] on the diagonal,
+ * else the first row whose U[i,i] is zero (every row still computed, as in
+ * a build without assertions), else -1. */
+int orc_spilu0_csr(int n, int nnz, const int *Ap, const int *Ai, const double *Ax,
+                   const int *A_diag, double *l) {
+  int *iw = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+  int bad = -1, jrow = 0;
+  for (int j = 0; j < n; j++) iw[j] = -1;
+  for (int k = 0; k < nnz; k++) l[k] = Ax[k];
+  for (int i = 0; i < n; i++) {
+    for (int k = Ap[i]; k < Ap[i + 1]; k++) iw[Ai[k]] = k;
+    int j;
+    for (j = Ap[i]; j < Ap[i + 1]; ++j) {
+      jrow = Ai[j];
+      if (i <= jrow) break;
+      double tl = l[j] / l[A_diag[jrow]];
+      l[j] = tl;
+      for (int jj = A_diag[jrow] + 1; jj < Ap[jrow + 1]; jj++) {
+        int jw = iw[Ai[jj]];
+        if (jw != -1) l[jw] = l[jw] - tl * l[jj];
+      }
+    }
+    if (A_diag[i] != j || jrow != i) {
+      free(iw);
+      return -2;
+    }
+    if (l[j] == 0.0 && bad < 0) bad = i;
+    for (int k = Ap[i]; k < Ap[i + 1]; k++) iw[Ai[k]] = -1;
+  }
+  free(iw);
+  return bad;
+}
+
 /* --------------------------------------------------------- level sets */
 
 /* Level-set inspector on the CSR of L (the reference's build_levelSet_CSC

@@ -21,6 +21,7 @@ Reference mapping (reference include/sympiler/sparse_blas_lib.h):
                          sptrsv_csr_ilu_lbc (sptrsv.cpp:28-38, :109-126)
   Plan.ic0()             IC(0) on the plan's pattern (spic0.cpp:11-46,
                          :106-127)
+  ilu0()                 sym_lib::spilu0_csr (spilu0.cpp:11-51)
 """
 from __future__ import annotations
 
@@ -96,6 +97,7 @@ def lib() -> ctypes.CDLL:
             "symamd_residual": [vp, vp, vp, vp, i32, i64, i32, vp],
             "symamd_tune": [vp, vp, i32, i32, c_char_p, i32],
             "symamd_ic0": [vp, vp, i32, vp, vp, POINTER(i32)],
+            "symamd_ilu0": [vp, i32, vp, vp, vp, vp, vp, POINTER(i32)],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -495,6 +497,31 @@ def ilu_lower(Ap, Ai, Ax, diags, unit_diag: bool, handle: Handle | None = None):
                                   1 if unit_diag else 0, _ptr(Lp), _ptr(Li), _ptr(Lx),
                                   byref(nnz)))
     return Lp, Li[: nnz.value], Lx[: nnz.value]
+
+
+def ilu0(Ap, Ai, Ax, diags, out=None, strict=True, handle: Handle | None = None):
+    """ILU(0) of a general CSR matrix A (diags[i] = position of row i's
+    diagonal) on A's pattern: L strictly below the diagonal (unit diagonal
+    implied), U on and above, bit-identical to sym_lib::spilu0_csr
+    (sparse_blas/spilu0.cpp:11-51).  out may be Ax (in place).  A zero U
+    diagonal raises ValueError when strict, else (l, row or -1) is
+    returned."""
+    import torch
+    _require_cuda(Ap, Ai, Ax, diags)
+    h = handle or default_handle()
+    n = Ap.numel() - 1
+    if out is None:
+        out = torch.empty_like(Ax)
+    _require_cuda(out)
+    bad = c_int(-1)
+    rc = lib().symamd_ilu0(h._h, n, _ptr(Ap), _ptr(Ai), _ptr(Ax), _ptr(diags), _ptr(out),
+                           byref(bad))
+    if rc == 3 and not strict:
+        return out, int(bad.value)
+    if rc == 3:
+        raise ValueError(f"ILU(0) breakdown: U[{bad.value},{bad.value}] is zero")
+    _check(rc)
+    return out if strict else (out, -1)
 
 
 def spmv_csr(Ap, Ai, Ax, x, y=None, z=None, a=None, b=None,

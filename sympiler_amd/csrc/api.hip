@@ -959,4 +959,15 @@ int symamd_ic0(symamd_handle h, symamd_plan plan, symamd_ic0_variant variant,
   return SYMAMD_OK;
 }
 
+int symamd_ilu0(symamd_handle h, int n, const int *d_Ap, const int *d_Ai, const double *d_Ax,
+                const int *d_Adiag, double *d_l, int *bad_row) {
+  SYMAMD_TRY(check_handle(h));
+  if (n < 0 || !bad_row || !d_Ap || (n > 0 && (!d_Ai || !d_Ax || !d_Adiag || !d_l)))
+    return fail(SYMAMD_ERR_INVALID_ARG, "symamd_ilu0: bad arguments");
+  SYMAMD_TRY(ilu0_factor(h, n, d_Ap, d_Ai, d_Ax, d_Adiag, d_l, bad_row));
+  if (*bad_row >= 0)
+    return fail(SYMAMD_ERR_ZERO_PIVOT, "symamd_ilu0: U[%d,%d] is zero", *bad_row, *bad_row);
+  return SYMAMD_OK;
+}
+
 }  // extern "C"

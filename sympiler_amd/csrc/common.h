@@ -126,7 +126,7 @@ struct UnitSet {
   int64_t xdeps = 0;
 };
 
-// IC(0) update pairs of one variant (ic0.hip): entry p of the caller's CSR
+// IC(0) update pairs of one variant (factor0.hip): entry p of the caller's CSR
 // subtracts Lx[pa[t]] * Lx[pb[t]] for t in [pp[p], pp[p+1]), in order.
 struct Ic0Pairs {
   int64_t pairs = 0;
@@ -426,11 +426,15 @@ bool region_usable(symamd_handle h, symamd_plan p, int k);
 // schedule with at least 60 % of L's off-diagonal entries inside regions
 // (column-major blocks only through the workspace).
 bool region_preferred(symamd_handle h, symamd_plan p, int k, int layout);
-// IC(0) of the plan's pattern (ic0.hip); *bad_row = smallest failed pivot
+// IC(0) of the plan's pattern (factor0.hip); *bad_row = smallest failed pivot
 // row or -1.  Blocks.
 int ic0_factor(symamd_handle h, symamd_plan p, int variant, const double *d_Ax,
                double *d_Lx, int *bad_row);
 void free_ic0_pairs(symamd_plan p);
+// ILU(0) of a general CSR matrix (factor0.hip); *bad_row = smallest row with
+// a zero U diagonal or -1.  Blocks.
+int ilu0_factor(symamd_handle h, int n, const int *d_Ap, const int *d_Ai, const double *d_Ax,
+                const int *d_Adiag, double *d_l, int *bad_row);
 int ilu_lower(symamd_handle h, int n, const int *d_Ap, const int *d_Ai,
               const double *d_Ax, const int *d_diags, int unit, int *d_Lp,
               int *d_Li, double *d_Lx, int *nnz_out);

@@ -392,6 +392,36 @@ void spic0_csr_lbc(int n, double *val, int *rowPtr, int *colIdx, int level_no,
   spic0_csr(n, val, rowPtr, colIdx);
 }
 
+// ---------------------------------------------------------------- ILU(0)
+// spilu0_csr copies A into l and factors l in place; every row is computed
+// (the reference asserts that each U[i,i] is non-zero).
+void spilu0_csr(int n, int nnz, const int *Ap, const int *Ai, const double *Ax,
+                const int *A_diag, double *l) {
+  if (n <= 0) {
+    if (nnz > 0) std::memcpy(l, Ax, sizeof(double) * (size_t)nnz);
+    return;
+  }
+  symamd_handle h = handle();
+  const size_t total = (size_t)(nnz > Ap[n] ? nnz : Ap[n]);
+  DevBuf<int> dAp(n + 1, Ap), dAi(Ap[n], Ai), dd(n, A_diag);
+  DevBuf<double> dl(total, Ax);
+  int bad = -1;
+  const int rc = symamd_ilu0(h, n, dAp.p, dAi.p, dl.p, dd.p, dl.p, &bad);
+  if (rc != SYMAMD_ERR_ZERO_PIVOT) check(rc, "spilu0_csr");
+  dl.download(l, (size_t)nnz);
+}
+
+void spilu0_csr_lbc(int n, int nnz, const int *Ap, const int *Ai, const double *Ax,
+                    const int *A_diag, double *l, int level_no, const int *level_ptr,
+                    const int *par_ptr, const int *partition, int *tempvecs) {
+  (void)level_no;
+  (void)level_ptr;
+  (void)par_ptr;
+  (void)partition;
+  (void)tempvecs;
+  spilu0_csr(n, nnz, Ap, Ai, Ax, A_diag, l);
+}
+
 int build_levelSet_CSC(int n, const int *Lp, const int *Li, int *&level_ptr,
                        int *&level_set) {
   level_ptr = new int[n + 1];

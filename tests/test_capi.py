@@ -60,6 +60,8 @@ def test_sym_lib_dropins_exported_with_reference_mangling():
         "_ZN7sym_lib13spic0_csr_lbcEiPdPiS1_iS1_S1_S1_",
         "_ZN7sym_lib14spic0_csc_leftEiPKiS1_PKdS1_S1_Pd",
         "_ZN7sym_lib18spic0_csc_left_LBCEiPKiS1_PKdS1_S1_iS1_S1_S1_Pd",
+        "_ZN7sym_lib10spilu0_csrEiiPKiS1_PKdS1_Pd",
+        "_ZN7sym_lib14spilu0_csr_lbcEiiPKiS1_PKdS1_PdiS1_S1_S1_Pi",
     }
     missing = want - exported(LIB)
     assert not missing, missing
