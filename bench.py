@@ -97,8 +97,8 @@ def other_configs(torch, sa, W, dev, stream):
     """The other workloads, a few timed repetitions each (seconds of GPU
     time): Zipf-irregular SpTRSM (n = 1e6, k = 1/8/32/128, column-major), the
     512-system uneven batch replayed from a hipGraph, the 64^3 IC(0)
-    adjoint (forward vs forward+backward, FP64 and FP32) and the device IC(0)
-    factorization of the 128^3 Laplacian."""
+    adjoint (forward vs forward+backward, FP64 and FP32), and the device IC(0)
+    and ILU(0) factorizations on the 128^3 grid."""
     out = {}
     # Zipf(1.8) irregular lower triangle, B column-major (k, n)
     w3 = W.config3(k=1)
@@ -196,6 +196,26 @@ def other_configs(torch, sa, W, dev, stream):
     out["ic0_lap3d_128cubed"] = ic0
     log(f"ic0 128^3: {ic0}")
     del plan6, dA6, L6
+
+    # ILU(0) of a general matrix on the full 7-point pattern of the 128^3
+    # grid: the one-shot call (schedule, pairs and factor) and the numeric
+    # factorization of an analysed pattern
+    Ap7, Ai7, Ax7, d7 = W.grid3d_general(128)
+    dAp7, dAi7, dAx7, dd7 = (torch.from_numpy(a).to(dev) for a in (Ap7, Ai7, Ax7, d7))
+    l7 = torch.empty_like(dAx7)
+    wl7, _ = timed_steps(lambda: sa.ilu0(dAp7, dAi7, dAx7, dd7, out=l7), lambda: None, 3, 1,
+                         stream, torch)
+    t0 = time.perf_counter()
+    iplan = sa.Ilu0Plan(dAp7, dAi7, dd7)
+    an_ms = 1e3 * (time.perf_counter() - t0)
+    wf7, _ = timed_steps(lambda: iplan.factor(dAx7, out=l7), lambda: None, 5, 1, stream, torch)
+    out["ilu0_grid3d_128cubed"] = {"n": len(Ap7) - 1, "nnz": int(Ap7[-1]),
+                                   "one_shot_ms": round(1e3 * float(np.mean(wl7)), 2),
+                                   "analyze_ms": round(an_ms, 2),
+                                   "factor_ms": round(1e3 * float(np.mean(wf7)), 3)}
+    del iplan
+    log(f"ilu0 128^3: {out['ilu0_grid3d_128cubed']}")
+    del dAp7, dAi7, dAx7, dd7, l7
     return out
 
 
@@ -494,6 +514,13 @@ def cpu_baseline(Lp, Li, Lx, Bcols, Xgpu, args, k):
         t0 = time.perf_counter()
         oracle.ref.spic0_csc_left(Cp, Ci, Cx, pp, ps)
         ic0_ms = round(1e3 * (time.perf_counter() - t0), 1)
+    # ILU(0) of the general 128^3 7-point matrix: the oracle's serial
+    # restatement of spilu0_csr (the reference's file is not buildable here)
+    from sympiler_amd import workloads as W
+    Ap7, Ai7, Ax7, d7 = W.grid3d_general(128)
+    t0 = time.perf_counter()
+    oracle.spilu0_csr(Ap7, Ai7, Ax7, d7)
+    ilu_ms = round(1e3 * (time.perf_counter() - t0), 1)
     log(f"cpu baseline ({kind}, {ncores} threads): "
         f"{rate:.2f} RHS-solves/s over {done} solves; GPU vs CPU max rel err {worst:.2e}")
     return {"value": round(rate, 3), "unit": "RHS-solves/s",
@@ -501,7 +528,8 @@ def cpu_baseline(Lp, Li, Lx, Bcols, Xgpu, args, k):
             "sample": f"{done} single-RHS level-set solves of the same matrix "
                       f"({t_total:.1f}s)",
             "gpu_vs_cpu_max_rel_err": worst,
-            "ic0_spic0_csc_left_128cubed_ms_1_thread": ic0_ms}
+            "ic0_spic0_csc_left_128cubed_ms_1_thread": ic0_ms,
+            "ilu0_port_spilu0_csr_grid3d_128cubed_ms_1_thread": ilu_ms}
 
 
 if __name__ == "__main__":

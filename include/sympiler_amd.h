@@ -318,6 +318,17 @@ int symamd_ic0(symamd_handle h, symamd_plan plan, symamd_ic0_variant variant,
  * blocks until the factor is complete. */
 int symamd_ilu0(symamd_handle h, int n, const int *d_Ap, const int *d_Ai,
                 const double *d_Ax, const int *d_Adiag, double *d_l, int *bad_row);
+/* The same in two steps for repeated factorizations of one pattern:
+ * symamd_ilu0_analyze checks the structure and builds the schedule and the
+ * update lists (the plan copies the pattern; the arrays may be freed after
+ * the call); symamd_ilu0_factor computes the factor (same results, errors
+ * and blocking as symamd_ilu0). */
+typedef struct symamd_ilu0_s *symamd_ilu0_plan;
+int symamd_ilu0_analyze(symamd_handle h, int n, const int *d_Ap, const int *d_Ai,
+                        const int *d_Adiag, symamd_ilu0_plan *plan);
+int symamd_ilu0_factor(symamd_handle h, symamd_ilu0_plan plan, const double *d_Ax,
+                       double *d_l, int *bad_row);
+int symamd_ilu0_destroy(symamd_ilu0_plan plan);
 
 /* ----------------------------------------------------------------- tuner */
 /* Time the candidate schedules for k right-hand sides in `layout` on

@@ -21,7 +21,7 @@ Reference mapping (reference include/sympiler/sparse_blas_lib.h):
                          sptrsv_csr_ilu_lbc (sptrsv.cpp:28-38, :109-126)
   Plan.ic0()             IC(0) on the plan's pattern (spic0.cpp:11-46,
                          :106-127)
-  ilu0()                 sym_lib::spilu0_csr (spilu0.cpp:11-51)
+  ilu0(), Ilu0Plan       sym_lib::spilu0_csr (spilu0.cpp:11-51)
 """
 from __future__ import annotations
 
@@ -98,6 +98,9 @@ def lib() -> ctypes.CDLL:
             "symamd_tune": [vp, vp, i32, i32, c_char_p, i32],
             "symamd_ic0": [vp, vp, i32, vp, vp, POINTER(i32)],
             "symamd_ilu0": [vp, i32, vp, vp, vp, vp, vp, POINTER(i32)],
+            "symamd_ilu0_analyze": [vp, i32, vp, vp, vp, POINTER(vp)],
+            "symamd_ilu0_factor": [vp, vp, vp, vp, POINTER(i32)],
+            "symamd_ilu0_destroy": [vp],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -522,6 +525,43 @@ def ilu0(Ap, Ai, Ax, diags, out=None, strict=True, handle: Handle | None = None)
         raise ValueError(f"ILU(0) breakdown: U[{bad.value},{bad.value}] is zero")
     _check(rc)
     return out if strict else (out, -1)
+
+
+class Ilu0Plan:
+    """ILU(0) of one pattern, factored many times: the structure check, the
+    schedule and the update lists are built once (symamd_ilu0_analyze), each
+    factor() is the numeric phase only (symamd_ilu0_factor); results as
+    ilu0()."""
+
+    def __init__(self, Ap, Ai, diags, handle: Handle | None = None):
+        _require_cuda(Ap, Ai, diags)
+        self.h = handle or default_handle()
+        self.n = Ap.numel() - 1
+        self._p = c_void_p()
+        _check(lib().symamd_ilu0_analyze(self.h._h, self.n, _ptr(Ap), _ptr(Ai), _ptr(diags),
+                                         byref(self._p)))
+
+    def factor(self, Ax, out=None, strict=True):
+        import torch
+        _require_cuda(Ax)
+        if out is None:
+            out = torch.empty_like(Ax)
+        _require_cuda(out)
+        bad = c_int(-1)
+        rc = lib().symamd_ilu0_factor(self.h._h, self._p, _ptr(Ax), _ptr(out), byref(bad))
+        if rc == 3 and not strict:
+            return out, int(bad.value)
+        if rc == 3:
+            raise ValueError(f"ILU(0) breakdown: U[{bad.value},{bad.value}] is zero")
+        _check(rc)
+        return out if strict else (out, -1)
+
+    def __del__(self):
+        try:
+            if self._p:
+                lib().symamd_ilu0_destroy(self._p)
+        except Exception:
+            pass
 
 
 def spmv_csr(Ap, Ai, Ax, x, y=None, z=None, a=None, b=None,

@@ -959,15 +959,41 @@ int symamd_ic0(symamd_handle h, symamd_plan plan, symamd_ic0_variant variant,
   return SYMAMD_OK;
 }
 
+int symamd_ilu0_analyze(symamd_handle h, int n, const int *d_Ap, const int *d_Ai,
+                        const int *d_Adiag, symamd_ilu0_plan *plan) {
+  SYMAMD_TRY(check_handle(h));
+  if (n < 0 || !plan || !d_Ap || (n > 0 && (!d_Ai || !d_Adiag)))
+    return fail(SYMAMD_ERR_INVALID_ARG, "symamd_ilu0_analyze: bad arguments");
+  *plan = nullptr;
+  return ilu0_analyze(h, n, d_Ap, d_Ai, d_Adiag, plan);
+}
+
+int symamd_ilu0_factor(symamd_handle h, symamd_ilu0_plan plan, const double *d_Ax,
+                       double *d_l, int *bad_row) {
+  SYMAMD_TRY(check_handle(h));
+  if (!plan || !bad_row || (plan->nnz > 0 && (!d_Ax || !d_l)))
+    return fail(SYMAMD_ERR_INVALID_ARG, "symamd_ilu0_factor: bad arguments");
+  SYMAMD_TRY(ilu0_numeric(h, plan, d_Ax, d_l, bad_row));
+  if (*bad_row >= 0)
+    return fail(SYMAMD_ERR_ZERO_PIVOT, "ILU(0): U[%d,%d] is zero", *bad_row, *bad_row);
+  return SYMAMD_OK;
+}
+
+int symamd_ilu0_destroy(symamd_ilu0_plan plan) {
+  ilu0_destroy(plan);
+  return SYMAMD_OK;
+}
+
 int symamd_ilu0(symamd_handle h, int n, const int *d_Ap, const int *d_Ai, const double *d_Ax,
                 const int *d_Adiag, double *d_l, int *bad_row) {
   SYMAMD_TRY(check_handle(h));
   if (n < 0 || !bad_row || !d_Ap || (n > 0 && (!d_Ai || !d_Ax || !d_Adiag || !d_l)))
     return fail(SYMAMD_ERR_INVALID_ARG, "symamd_ilu0: bad arguments");
-  SYMAMD_TRY(ilu0_factor(h, n, d_Ap, d_Ai, d_Ax, d_Adiag, d_l, bad_row));
-  if (*bad_row >= 0)
-    return fail(SYMAMD_ERR_ZERO_PIVOT, "symamd_ilu0: U[%d,%d] is zero", *bad_row, *bad_row);
-  return SYMAMD_OK;
+  symamd_ilu0_plan plan = nullptr;
+  SYMAMD_TRY(ilu0_analyze(h, n, d_Ap, d_Ai, d_Adiag, &plan));
+  const int rc = symamd_ilu0_factor(h, plan, d_Ax, d_l, bad_row);
+  ilu0_destroy(plan);
+  return rc;
 }
 
 }  // extern "C"

@@ -324,6 +324,18 @@ struct symamd_plan_s {
   std::map<std::string, std::pair<hipGraph_t, hipGraphExec_t>> gcache;
 };
 
+// ILU(0) plan: A's pattern (owned copies), the schedule of its lower part
+// and the update pairs of every entry (factor0.hip).
+struct symamd_ilu0_s {
+  int n = 0;
+  int64_t nnz = 0;
+  int *d_Ap = nullptr, *d_Ai = nullptr, *d_Adiag = nullptr;
+  symamd_plan sched = nullptr;
+  int64_t pairs = 0;
+  int64_t *d_pp = nullptr;
+  int *d_pa = nullptr, *d_pb = nullptr, *d_dpos = nullptr;
+};
+
 struct symamd_graph_s {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -431,10 +443,14 @@ bool region_preferred(symamd_handle h, symamd_plan p, int k, int layout);
 int ic0_factor(symamd_handle h, symamd_plan p, int variant, const double *d_Ax,
                double *d_Lx, int *bad_row);
 void free_ic0_pairs(symamd_plan p);
-// ILU(0) of a general CSR matrix (factor0.hip); *bad_row = smallest row with
-// a zero U diagonal or -1.  Blocks.
-int ilu0_factor(symamd_handle h, int n, const int *d_Ap, const int *d_Ai, const double *d_Ax,
-                const int *d_Adiag, double *d_l, int *bad_row);
+// ILU(0) of a general CSR matrix (factor0.hip): the schedule and update
+// pairs once, then numeric factorizations (*bad_row = smallest row with a
+// zero U diagonal or -1; blocks).
+int ilu0_analyze(symamd_handle h, int n, const int *d_Ap, const int *d_Ai, const int *d_Adiag,
+                 symamd_ilu0_s **out);
+int ilu0_numeric(symamd_handle h, symamd_ilu0_s *q, const double *d_Ax, double *d_l,
+                 int *bad_row);
+void ilu0_destroy(symamd_ilu0_s *q);
 int ilu_lower(symamd_handle h, int n, const int *d_Ap, const int *d_Ai,
               const double *d_Ax, const int *d_diags, int unit, int *d_Lp,
               int *d_Li, double *d_Lx, int *nnz_out);

@@ -200,20 +200,10 @@ struct FactorArgs {
   uint64_t timeout;
 };
 
-// Rows of at most SHORT_RB entries keep their values in registers: the
-// loads that do not depend on the row's own results (A, the pair ranges,
-// the divisors) are issued first, and the row is stored once at the end, so
-// no load waits behind a store.  (Variants that also held the pairs' indices
-// and values in registers measured slower on the 128^3 Laplacian IC(0): 5.0
-// and 6.4 ms against 4.0 ms.)
+// Rows of at most SHORT_RB entries keep their values in registers and the
+// lane's LDS slots, and are stored once at the end, so no load waits behind
+// a store.
 constexpr int SHORT_RB = 8;
-
-__device__ __forceinline__ double pick(const double (&v)[SHORT_RB], int i) {
-  double r = v[0];
-#pragma unroll
-  for (int j = 1; j < SHORT_RB; ++j) r = i == j ? v[j] : r;
-  return r;
-}
 
 // The pivot test of each routine: spic0_csc_left stops at f <= 0 (a NaN
 // pivot passes); spic0_csr asserts pivot > 0 (a NaN pivot fails);
@@ -236,39 +226,100 @@ __device__ __forceinline__ void finish_entry(const FactorArgs &a, int r, int off
   }
 }
 
-template <int MODE>
-__device__ __forceinline__ void short_row(const FactorArgs &a, int r, int b0, int m, int dr) {
-  double ax[SHORT_RB], dg[SHORT_RB], own[SHORT_RB];
+// A short row's first SHORT_PB pairs, staged in the lane's own LDS slots
+// (read back by pair index, which registers cannot be without scratch
+// memory): the offset in this row of the first operand, the offset of the
+// second when it is this row's too (the IC(0) diagonal) or -1, and the
+// other row's value otherwise.
+constexpr int SHORT_PB = 8;
+
+struct PairSlots {
+  double own[SHORT_RB];  // this row's finished entries, read by offset
+  double y[SHORT_PB];    // other rows' operands
+  int yp[SHORT_PB];      // their positions
+  int16_t ia[SHORT_PB];
+  int16_t ib[SHORT_PB];
+};
+
+// A short row in two phases.  prep: everything that depends only on the
+// structure and on A (values of A, pair ranges, divisor positions, the pairs
+// staged in the slots) -- issued before the wait for the rows it reads, so
+// those loads overlap the wait.  fin: one round of loads of the other rows'
+// values (divisors and pair operands), then the arithmetic in entry order,
+// then one store per entry.
+struct ShortRow {
+  double ax[SHORT_RB];
   int64_t pq[SHORT_RB + 1];
-  bool hd[SHORT_RB];
+  int dp[SHORT_RB];
+  int64_t t0;
+  bool staged;
+};
+
+template <int MODE>
+__device__ __forceinline__ void short_prep(const FactorArgs &a, int b0, int m, int dr,
+                                           ShortRow &w, PairSlots &ps) {
 #pragma unroll
-  for (int j = 0; j < SHORT_RB; ++j) {
-    ax[j] = j < m ? a.Ax[b0 + j] : 0.0;
-    own[j] = 0.0;
+  for (int j = 0; j < SHORT_RB; ++j) w.ax[j] = j < m ? a.Ax[b0 + j] : 0.0;
+#pragma unroll
+  for (int j = 0; j <= SHORT_RB; ++j) w.pq[j] = j <= m ? a.pp[b0 + j] : 0;
+#pragma unroll
+  for (int j = 0; j < SHORT_RB; ++j)
+    w.dp[j] = j < m && (MODE == MODE_ILU0 || j != dr) ? a.dpos[b0 + j] : -1;
+  w.t0 = w.pq[0];
+  int64_t t1 = 0;
+#pragma unroll
+  for (int j = 1; j <= SHORT_RB; ++j) t1 = j == m ? w.pq[j] : t1;
+  w.staged = t1 - w.t0 <= SHORT_PB;
+  if (w.staged) {
+    const int np = (int)(t1 - w.t0);
+#pragma unroll
+    for (int k = 0; k < SHORT_PB; ++k) {
+      const int ia = k < np ? a.pa[w.t0 + k] : b0;
+      const int ib = k < np ? a.pb[w.t0 + k] : b0;
+      ps.ia[k] = (int16_t)(ia - b0);
+      // another row's operand: its position, loaded after the wait
+      ps.ib[k] = (int16_t)(ib < b0 ? -1 : ib - b0);
+      ps.yp[k] = ib;
+    }
   }
+}
+
+template <int MODE>
+__device__ __forceinline__ void short_fin(const FactorArgs &a, int r, int b0, int m, int dr,
+                                          const ShortRow &w, PairSlots &ps) {
+  double dg[SHORT_RB], own[SHORT_RB];
 #pragma unroll
-  for (int j = 0; j <= SHORT_RB; ++j) pq[j] = j <= m ? a.pp[b0 + j] : 0;
+  for (int j = 0; j < SHORT_RB; ++j) dg[j] = w.dp[j] >= 0 ? ld_dev(&a.Lx[w.dp[j]]) : 1.0;
+  if (w.staged) {
+    double y[SHORT_PB];
 #pragma unroll
-  for (int j = 0; j < SHORT_RB; ++j) {
-    const int dp = j < m && (MODE == MODE_ILU0 || j != dr) ? a.dpos[b0 + j] : -1;
-    hd[j] = dp >= 0;
-    dg[j] = hd[j] ? ld_dev(&a.Lx[dp]) : 1.0;
+    for (int k = 0; k < SHORT_PB; ++k) y[k] = ps.ib[k] < 0 ? ld_dev(&a.Lx[ps.yp[k]]) : 0.0;
+#pragma unroll
+    for (int k = 0; k < SHORT_PB; ++k) ps.y[k] = y[k];
   }
 #pragma unroll
   for (int j = 0; j < SHORT_RB; ++j) {
     if (j < m) {
-      double v = ax[j];
-      // the pair's first operand is this row's; the second another row's,
-      // or this row's on the IC(0) diagonal (c == r)
-      const bool own_b = MODE == MODE_IC0 && j == dr;
-      for (int64_t t = pq[j]; t < pq[j + 1]; ++t) {
-        const int ib = a.pb[t];
-        const double x = pick(own, a.pa[t] - b0);
-        const double y = own_b ? pick(own, ib - b0) : ld_dev(&a.Lx[ib]);
-        v -= x * y;
+      double v = w.ax[j];
+      if (w.staged) {
+        for (int k = (int)(w.pq[j] - w.t0), ke = (int)(w.pq[j + 1] - w.t0); k < ke; ++k) {
+          const int kb = ps.ib[k];
+          const double y = kb >= 0 ? ps.own[kb] : ps.y[k];
+          v -= ps.own[ps.ia[k]] * y;
+        }
+      } else {
+        // the pair's first operand is this row's; the second another row's,
+        // or this row's on the IC(0) diagonal (c == r)
+        for (int64_t t = w.pq[j]; t < w.pq[j + 1]; ++t) {
+          const int ib = a.pb[t];
+          const double x = ps.own[a.pa[t] - b0];
+          const double y = ib >= b0 ? ps.own[ib - b0] : ld_dev(&a.Lx[ib]);
+          v -= x * y;
+        }
       }
-      finish_entry<MODE>(a, r, j, dr, hd[j], dg[j], v);
+      finish_entry<MODE>(a, r, j, dr, w.dp[j] >= 0, dg[j], v);
       own[j] = v;
+      ps.own[j] = v;
     }
   }
 #pragma unroll
@@ -296,6 +347,7 @@ __device__ __forceinline__ void long_row(const FactorArgs &a, int r, int b0, int
 // units carry this launch's tag.
 template <int MODE>
 __global__ __launch_bounds__(256) void k_factor0(FactorArgs a) {
+  __shared__ PairSlots slots[256];
   const int lane = threadIdx.x & 63;
   const int target = ld_dev(&a.state[ST_EPOCH]) + 1;
   int cursor = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
@@ -304,13 +356,19 @@ __global__ __launch_bounds__(256) void k_factor0(FactorArgs a) {
     if (u >= a.U) break;
     const int q = a.ub[u] + lane;
     const bool act = q < a.ub[u + 1];
+    int r = 0, b0 = 0, m = 0, dr = 0;
+    ShortRow w;
+    if (act) {
+      r = a.perm[q];
+      b0 = a.rp[r];
+      m = a.rp[r + 1] - b0;
+      dr = MODE == MODE_IC0 ? m - 1 : a.adiag[r] - b0;
+      if (m <= SHORT_RB) short_prep<MODE>(a, b0, m, dr, w, slots[threadIdx.x]);
+    }
     wait_units(a.dptr, a.dlist, u, a.uflags, target, a.state, a.timeout);
     if (act) {
-      const int r = a.perm[q];
-      const int b0 = a.rp[r], m = a.rp[r + 1] - b0;
-      const int dr = MODE == MODE_IC0 ? m - 1 : a.adiag[r] - b0;
       if (m <= SHORT_RB)
-        short_row<MODE>(a, r, b0, m, dr);
+        short_fin<MODE>(a, r, b0, m, dr, w, slots[threadIdx.x]);
       else
         long_row<MODE>(a, r, b0, m, dr);
     }
@@ -478,19 +536,43 @@ int ic0_factor(symamd_handle h, symamd_plan p, int variant, const double *d_Ax, 
   return launch_factor0<MODE_IC0>(h, p, a, bad_row);
 }
 
-int ilu0_factor(symamd_handle h, int n, const int *d_Ap, const int *d_Ai, const double *d_Ax,
-                const int *d_Adiag, double *d_l, int *bad_row) {
-  *bad_row = -1;
-  if (n == 0) return SYMAMD_OK;
+void ilu0_destroy(symamd_ilu0_s *q) {
+  if (!q) return;
+  void *ptrs[] = {q->d_Ap, q->d_Ai, q->d_Adiag, q->d_pp, q->d_pa, q->d_pb, q->d_dpos};
+  for (void *x : ptrs)
+    if (x) (void)hipFree(x);
+  if (q->sched) symamd_plan_destroy(q->sched);
+  delete q;
+}
+
+namespace {
+
+int ilu0_analyze_into(symamd_handle h, symamd_ilu0_s *q, const int *d_Ap, const int *d_Ai,
+                      const int *d_Adiag) {
+  const int n = q->n;
   int nnz32 = 0;
   SYMAMD_HIP(hipMemcpyAsync(&nnz32, d_Ap + n, sizeof(int), hipMemcpyDeviceToHost, h->stream));
   SYMAMD_HIP(hipStreamSynchronize(h->stream));
   const int64_t nnz = nnz32;
+  q->nnz = nnz;
+  // the plan keeps its own copy of A's pattern
+  SYMAMD_HIP(hipMalloc(&q->d_Ap, sizeof(int) * (n + 1)));
+  SYMAMD_HIP(hipMalloc(&q->d_Ai, sizeof(int) * (nnz > 0 ? nnz : 1)));
+  SYMAMD_HIP(hipMalloc(&q->d_Adiag, sizeof(int) * (n > 0 ? n : 1)));
+  SYMAMD_HIP(hipMemcpyAsync(q->d_Ap, d_Ap, sizeof(int) * (n + 1), hipMemcpyDeviceToDevice,
+                            h->stream));
+  if (nnz > 0)
+    SYMAMD_HIP(hipMemcpyAsync(q->d_Ai, d_Ai, sizeof(int) * nnz, hipMemcpyDeviceToDevice,
+                              h->stream));
+  if (n > 0)
+    SYMAMD_HIP(hipMemcpyAsync(q->d_Adiag, d_Adiag, sizeof(int) * n, hipMemcpyDeviceToDevice,
+                              h->stream));
+  if (n == 0) return SYMAMD_OK;
   Scratch sc;
   int *d_err = nullptr;
   SYMAMD_TRY(sc.alloc(&d_err, 1));
   SYMAMD_HIP(hipMemsetAsync(d_err, 0, sizeof(int), h->stream));
-  k_ilu_structure<<<blocks_of(n), 256, 0, h->stream>>>(n, d_Ap, d_Ai, d_Adiag, d_err);
+  k_ilu_structure<<<blocks_of(n), 256, 0, h->stream>>>(n, q->d_Ap, q->d_Ai, q->d_Adiag, d_err);
   SYMAMD_HIP(hipGetLastError());
   int err = 0;
   SYMAMD_HIP(hipMemcpyAsync(&err, d_err, sizeof(int), hipMemcpyDeviceToHost, h->stream));
@@ -498,55 +580,74 @@ int ilu0_factor(symamd_handle h, int n, const int *d_Ap, const int *d_Ai, const 
   if (err)
     return fail(SYMAMD_ERR_STRUCTURE,
                 "ILU(0): rows must be ascending with the diagonal at A_diag[i]");
-  // the schedule: the plan of A's lower part (row i waits for its k < i)
+  // the schedule: the plan of A's lower part with a unit diagonal (row i
+  // waits for the rows k < i it stores)
   int *d_Lp = nullptr, *d_Li = nullptr;
-  double *d_Lx = nullptr;
+  double *d_Lx = nullptr, *d_zero = nullptr;
   SYMAMD_TRY(sc.alloc(&d_Lp, n + 1));
   SYMAMD_TRY(sc.alloc(&d_Li, nnz));
   SYMAMD_TRY(sc.alloc(&d_Lx, nnz));
+  SYMAMD_TRY(sc.alloc(&d_zero, nnz));
+  SYMAMD_HIP(hipMemsetAsync(d_zero, 0, sizeof(double) * nnz, h->stream));
   int lnnz = 0;
-  SYMAMD_TRY(ilu_lower(h, n, d_Ap, d_Ai, d_Ax, d_Adiag, 0, d_Lp, d_Li, d_Lx, &lnnz));
-  symamd_plan p = nullptr;
-  SYMAMD_TRY(symamd_analyze(h, n, d_Lp, d_Li, d_Lx, SYMAMD_F64, &p));
-  struct PlanGuard {
-    symamd_plan p;
-    ~PlanGuard() { symamd_plan_destroy(p); }
-  } guard{p};
-  // pairs and divisors on A's pattern
-  int64_t *d_cnt = nullptr, *d_pp = nullptr;
-  int *d_dpos = nullptr, *d_pa = nullptr, *d_pb = nullptr;
+  SYMAMD_TRY(ilu_lower(h, n, q->d_Ap, q->d_Ai, d_zero, q->d_Adiag, 1, d_Lp, d_Li, d_Lx, &lnnz));
+  SYMAMD_TRY(symamd_analyze(h, n, d_Lp, d_Li, d_Lx, SYMAMD_F64, &q->sched));
+  // update pairs and divisors on A's pattern
+  int64_t *d_cnt = nullptr;
   SYMAMD_TRY(sc.alloc(&d_cnt, nnz + 1));
-  SYMAMD_TRY(sc.alloc(&d_pp, nnz + 1));
-  SYMAMD_TRY(sc.alloc(&d_dpos, nnz));
-  k_ilu_pairs<true><<<blocks_of(n + 1), 256, 0, h->stream>>>(n, d_Ap, d_Ai, d_Adiag, nnz,
-                                                              d_cnt, d_dpos, nullptr, nullptr);
+  SYMAMD_HIP(hipMalloc(&q->d_pp, sizeof(int64_t) * (nnz + 1)));
+  SYMAMD_HIP(hipMalloc(&q->d_dpos, sizeof(int) * nnz));
+  k_ilu_pairs<true><<<blocks_of(n + 1), 256, 0, h->stream>>>(
+      n, q->d_Ap, q->d_Ai, q->d_Adiag, nnz, d_cnt, q->d_dpos, nullptr, nullptr);
   SYMAMD_HIP(hipGetLastError());
-  SYMAMD_TRY(scan64(h, d_cnt, d_pp, nnz + 1, sc));
-  int64_t pairs = 0;
-  SYMAMD_HIP(hipMemcpyAsync(&pairs, d_pp + nnz, sizeof(int64_t), hipMemcpyDeviceToHost,
+  SYMAMD_TRY(scan64(h, d_cnt, q->d_pp, nnz + 1, sc));
+  SYMAMD_HIP(hipMemcpyAsync(&q->pairs, q->d_pp + nnz, sizeof(int64_t), hipMemcpyDeviceToHost,
                             h->stream));
   SYMAMD_HIP(hipStreamSynchronize(h->stream));
-  SYMAMD_TRY(sc.alloc(&d_pa, pairs));
-  SYMAMD_TRY(sc.alloc(&d_pb, pairs));
+  SYMAMD_HIP(hipMalloc(&q->d_pa, sizeof(int) * (q->pairs > 0 ? q->pairs : 1)));
+  SYMAMD_HIP(hipMalloc(&q->d_pb, sizeof(int) * (q->pairs > 0 ? q->pairs : 1)));
   // write cursors start at each entry's first pair
-  SYMAMD_HIP(hipMemcpyAsync(d_cnt, d_pp, sizeof(int64_t) * (nnz + 1), hipMemcpyDeviceToDevice,
-                            h->stream));
-  k_ilu_pairs<false><<<blocks_of(n), 256, 0, h->stream>>>(n, d_Ap, d_Ai, d_Adiag, nnz, d_cnt,
-                                                           nullptr, d_pa, d_pb);
+  SYMAMD_HIP(hipMemcpyAsync(d_cnt, q->d_pp, sizeof(int64_t) * (nnz + 1),
+                            hipMemcpyDeviceToDevice, h->stream));
+  k_ilu_pairs<false><<<blocks_of(n), 256, 0, h->stream>>>(
+      n, q->d_Ap, q->d_Ai, q->d_Adiag, nnz, d_cnt, nullptr, q->d_pa, q->d_pb);
   SYMAMD_HIP(hipGetLastError());
+  SYMAMD_HIP(hipStreamSynchronize(h->stream));
+  return SYMAMD_OK;
+}
+
+}  // namespace
+
+int ilu0_analyze(symamd_handle h, int n, const int *d_Ap, const int *d_Ai, const int *d_Adiag,
+                 symamd_ilu0_s **out) {
+  auto *q = new symamd_ilu0_s();
+  q->n = n;
+  const int rc = ilu0_analyze_into(h, q, d_Ap, d_Ai, d_Adiag);
+  if (rc != SYMAMD_OK) {
+    ilu0_destroy(q);
+    return rc;
+  }
+  *out = q;
+  return SYMAMD_OK;
+}
+
+int ilu0_numeric(symamd_handle h, symamd_ilu0_s *q, const double *d_Ax, double *d_l,
+                 int *bad_row) {
+  *bad_row = -1;
+  if (q->n == 0) return SYMAMD_OK;
   FactorArgs a{};
-  a.rp = d_Ap;
-  a.ci = d_Ai;
+  a.rp = q->d_Ap;
+  a.ci = q->d_Ai;
   a.Ax = d_Ax;
   a.Lx = d_l;
-  a.pp = d_pp;
-  a.pa = d_pa;
-  a.pb = d_pb;
-  a.dpos = d_dpos;
-  a.adiag = d_Adiag;
+  a.pp = q->d_pp;
+  a.pa = q->d_pa;
+  a.pb = q->d_pb;
+  a.dpos = q->d_dpos;
+  a.adiag = q->d_Adiag;
   a.variant = 0;
-  SYMAMD_TRY(launch_factor0<MODE_ILU0>(h, p, a, bad_row));
-  return symamd_plan_check(h, p);
+  SYMAMD_TRY(launch_factor0<MODE_ILU0>(h, q->sched, a, bad_row));
+  return symamd_plan_check(h, q->sched);
 }
 
 void free_ic0_pairs(symamd_plan p) {

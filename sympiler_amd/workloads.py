@@ -99,6 +99,33 @@ def lap3d_tril(nx, ny, nz, diag=6.0, off=-1.0):
     return Lp, Li, Lx
 
 
+def grid3d_general(g, seed=5):
+    """A general (non-symmetric) matrix on the full 7-point pattern of a
+    g^3 grid, row i = x + g*(y + g*z), columns ascending: off-diagonals
+    ~ U[-1, 0) (splitmix64), each diagonal 1 + its row's and column's
+    absolute sums (strictly diagonally dominant).  Returns (Ap, Ai, Ax,
+    diags) with diags[i] the position of row i's diagonal -- the input of
+    sym_lib::spilu0_csr."""
+    n = g ** 3
+    idx = np.arange(n, dtype=np.int64)
+    x, y, z = idx % g, (idx // g) % g, idx // (g * g)
+    offs = np.array([-g * g, -g, -1, 0, 1, g, g * g], np.int64)
+    ok = np.stack([z > 0, y > 0, x > 0, np.ones(n, bool), x < g - 1, y < g - 1, z < g - 1],
+                  axis=1)
+    cols = idx[:, None] + offs[None, :]
+    Ai = cols[ok].astype(np.int32)
+    cnt = ok.sum(axis=1)
+    Ap = np.zeros(n + 1, np.int32)
+    Ap[1:] = np.cumsum(cnt)
+    diags = (Ap[:-1] + ok[:, :3].sum(axis=1)).astype(np.int32)
+    Ax = uniform(seed, 0, len(Ai), -1.0, 0.0)
+    rows = np.repeat(idx, cnt)
+    w = np.abs(Ax)
+    w[diags] = 0.0
+    Ax[diags] = 1.0 + np.bincount(rows, w, n) + np.bincount(Ai, w, n)
+    return Ap, Ai, Ax, diags
+
+
 def ic0(Lp, Li, A_lower):
     """IC(0) factor values on the pattern of the lower triangle."""
     Lx = np.array(A_lower, dtype=np.float64, copy=True)

@@ -53,6 +53,18 @@ def test_ilu0_bitwise(dev, name):
     assert bitwise_equal(dAx.cpu().numpy(), Ax)
 
 
+def test_ilu0_plan_refactors_new_values(dev):
+    Ap, Ai, Ax, diags = grid(9, 8, 7, 13)
+    dAp, dAi, dd = to_dev(Ap, Ai, diags)
+    plan = sa.Ilu0Plan(dAp, dAi, dd)
+    for seed in (14, 15):
+        Bx = dominant(Ap, Ai, diags, seed)
+        want, bad = oracle.spilu0_csr(Ap, Ai, Bx, diags)
+        assert bad == -1
+        got = plan.factor(to_dev(Bx))
+        assert bitwise_equal(got.cpu().numpy(), want)
+
+
 def test_ilu0_zero_pivot_and_structure(dev):
     # 2 x 2 ones: U[1,1] = 0
     Ap = np.array([0, 2, 4], np.int32)
