@@ -55,6 +55,8 @@ def parse():
                     help="skip the other workloads (Zipf SpTRSM, batch, adjoint 64^3)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a device copy of 1 GiB (x5) to scale HBM counters")
+    ap.add_argument("--factors", action="store_true",
+                    help="only time the device IC(0) and ILU(0) factorizations")
     return ap.parse_args()
 
 
@@ -174,6 +176,14 @@ def other_configs(torch, sa, W, dev, stream):
     log(f"adjoint 64^3: {adj}")
     del Lx_t, b_t
 
+    out.update(factor_configs(torch, sa, W, dev, stream))
+    return out
+
+
+def factor_configs(torch, sa, W, dev, stream):
+    """Device IC(0) (both variants) of the 128^3 Laplacian and ILU(0) of the
+    general 128^3 7-point matrix, timed on the launch stream."""
+    out = {}
     # IC(0) of the 128^3 7-point Laplacian on the device (symamd_ic0), both
     # variants; checked against the package's CPU generator (spic0_csc_left
     # order)
@@ -250,6 +260,14 @@ def main():
     args = parse()
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.factors:
+        import torch
+        import sympiler_amd as sa
+        from sympiler_amd import workloads as W
+        dev = torch.device("cuda:0")
+        print(json.dumps({"factors": factor_configs(torch, sa, W, dev,
+                                                    torch.cuda.current_stream())}), flush=True)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; the multi-GPU "
