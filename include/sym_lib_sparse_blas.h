@@ -99,6 +99,17 @@ void spilu0_csr_lbc(int n, int nnz, const int *Ap, const int *Ai, const double *
                     const int *A_diag, double *l, int level_no, const int *level_ptr,
                     const int *par_ptr, const int *partition, int *tempvecs);
 
+/* sparse_blas_lib.h:196-214: forward solve on a supernodal (BCSC) lower
+ * triangle, bit-identical to the serial sptrsv_csc_block; the levelset
+ * variant (whose reference scatters with atomics) returns the same x, its
+ * schedule and temporaries unused.  Supernodes up to 1024 columns wide. */
+void sptrsv_csc_levelset_block(int n, const int *Lp, const int *Li, const int *nrows,
+                               double *Lx, double *x, const int *levelPtr, const int *levels,
+                               int n_lev, const int *supernodes, const int *sup2node,
+                               double **tempvecs);
+void sptrsv_csc_block(int n, const int *Lp, const int *Li, const int *nrows, double *Lx,
+                      double *x, int num_nodes, const int *supernodes);
+
 /* The level-set inspector of the un-vendored aggregation submodule, with the
  * signature its call sites use (Catch_tests/sptrsv_tests.cpp:59,
  * demo/sptrsv_demo_utils.h:44-45): input the CSC of L, outputs new[]'d for

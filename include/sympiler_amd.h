@@ -330,6 +330,27 @@ int symamd_ilu0_factor(symamd_handle h, symamd_ilu0_plan plan, const double *d_A
                        double *d_l, int *bad_row);
 int symamd_ilu0_destroy(symamd_ilu0_plan plan);
 
+/* ------------------------------------------------------ supernodal solve */
+/* Forward solve on a supernodal (BCSC) lower triangle, bit-identical to
+ * sym_lib::sptrsv_csc_block (sparse_blas/sptrsv.cpp:176-199, dense kernels
+ * dense_blas/BLAS.cpp lsolve_BLAS :3-96 and matvec_BLAS :286-349; decl
+ * sparse_blas_lib.h:212-214).  Layout (the reference's): supernode i owns
+ * columns h_sup[i] .. h_sup[i+1]-1 (width w, at most 1024) and
+ * h_nrows[i] >= w block rows; its values are a column-major h_nrows[i] x w
+ * block at Lx[h_Lp[i]] whose first w rows are its own columns, and row r >= w
+ * of the block is row h_Li[h_Lp[i] + r] of L, which must lie below the
+ * supernode (SYMAMD_ERR_STRUCTURE otherwise).  symamd_sn_analyze reads the
+ * HOST structure arrays (m = number of supernodes, h_sup[m] = n) and builds
+ * the schedule on the device; symamd_sn_solve overwrites d_x (b in, x out,
+ * FP64) asynchronously on the handle's stream; symamd_sn_check synchronises
+ * and reports a watchdog expiry. */
+typedef struct symamd_snplan_s *symamd_snplan;
+int symamd_sn_analyze(symamd_handle h, int n, int m, const int *h_sup, const int *h_nrows,
+                      const int *h_Lp, const int *h_Li, symamd_snplan *plan);
+int symamd_sn_solve(symamd_handle h, symamd_snplan plan, const double *d_Lx, double *d_x);
+int symamd_sn_check(symamd_handle h, symamd_snplan plan);
+int symamd_sn_destroy(symamd_snplan plan);
+
 /* ----------------------------------------------------------------- tuner */
 /* Time the candidate schedules for k right-hand sides in `layout` on
  * scratch data and remember the fastest in the plan (and in a process-wide

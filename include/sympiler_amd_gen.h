@@ -58,6 +58,20 @@ void symgen_batch_dims(int count, int lo, int hi, uint64_t seed, int *a,
 int64_t symgen_random_lower(int n, double density, int min_off, uint64_t seed,
                             int *Lp, int *Li, double *Lx);
 
+/* Supernodal (BCSC) lower triangle for sym_lib::sptrsv_csc_block
+ * (sparse_blas/sptrsv.cpp:176-199): supernode i owns columns
+ * supernodes[i] .. supernodes[i+1]-1 (widths ~ U{1..wmax}) and nrows[i] block
+ * rows (its own columns, then up to rmax distinct ascending rows drawn from
+ * the next `window` rows); its column-major block of values is at
+ * Lx[Lp[i]] with row ids Li[Lp[i] + j*nrows[i] + r].  Top-block diagonal
+ * 1 + U[0,1), entries below it U[-1,1)/nrows[i], strictly upper part NaN
+ * (never read by the solve).  With Lp == NULL only *num_nodes and the
+ * returned value count are produced; then supernodes/nrows need
+ * num_nodes+1 / num_nodes entries, Lp num_nodes+1, Li/Lx the count. */
+int64_t symgen_bcsc(int n, int wmax, int rmax, int window, uint64_t seed,
+                    int *num_nodes, int *supernodes, int *nrows, int *Lp,
+                    int *Li, double *Lx);
+
 #ifdef __cplusplus
 }
 #endif

@@ -20,6 +20,8 @@ Functions and the reference lines they follow:
   spic0_csc_left       sparse_blas/spic0.cpp:11-46
   spic0_csr            sparse_blas/spic0.cpp:106-127
   spilu0_csr           sparse_blas/spilu0.cpp:11-51
+  sptrsv_csc_block     sparse_blas/sptrsv.cpp:176-199 with dense_blas/BLAS.cpp
+                       lsolve_BLAS (:3-96) and matvec_BLAS (:286-349)
   levelset_csr/_csc    build_levelSet_CSC (aggregation, absent); definition
                        of parsy/common/TreeUtils.cpp:89-141
   sptrsm_csr           k columns of sptrsv_csr
@@ -68,6 +70,7 @@ def olib():
             "orc_spic0_csc_left": ([i, vp, vp, vp, vp, vp, vp], c_int),
             "orc_spic0_csr": ([i, vp, vp, vp], c_int),
             "orc_spilu0_csr": ([i, i, vp, vp, vp, vp, vp], c_int),
+            "orc_sptrsv_csc_block": ([i, vp, vp, vp, vp, vp, i, vp], None),
             "orc_levelset_csr": ([i, vp, vp, vp, vp, vp], c_int),
             "orc_levelset_csc": ([i, vp, vp, vp, vp], c_int),
             "orc_check_schedule": ([i, vp, vp, i, vp, vp, vp], c_int),
@@ -105,6 +108,7 @@ def rlib():
             "ref_sptrsv_csr_ilu": ([i, vp, vp, vp, vp, vp], None),
             "ref_sptrsv_csr_ilu_lbc": ([i, vp, vp, vp, vp, vp, i, vp, vp, vp], None),
             "ref_sptrsv_csc": ([i, vp, vp, vp, vp], None),
+            "ref_sptrsv_csc_block": ([i, vp, vp, vp, vp, vp, i, vp], None),
             "ref_ltsolve": ([i, vp, vp, vp, vp], c_int),
             "ref_spmv_csr": ([i, vp, vp, vp, vp, vp], None),
             "ref_spmv_csr_axpby": ([i, vp, vp, vp, vp, vp, vp, d, d], None),
@@ -232,6 +236,13 @@ def spilu0_csr(Ap, Ai, Ax, A_diag):
     bad = olib().orc_spilu0_csr(len(Ap) - 1, nnz, _p(_i32(Ap)), _p(_i32(Ai)), _p(_f64(Ax)),
                                 _p(_i32(A_diag)), _p(l))
     return l, int(bad)
+
+
+def sptrsv_csc_block(sup, nrows, Lp, Li, Lx, b):
+    x = _f64(b).copy()
+    olib().orc_sptrsv_csc_block(len(x), _p(_i32(Lp)), _p(_i32(Li)), _p(_i32(nrows)),
+                                _p(_f64(Lx)), _p(x), len(sup) - 1, _p(_i32(sup)))
+    return x
 
 
 def levelset_csr(Lp, Li):
@@ -397,6 +408,13 @@ class ref:
                                        _p(_f64(Ax)), _p(_i32(prunePtr)),
                                        _p(_i32(pruneSet)), _p(Lx))
         return Lx, bool(ok)
+
+    @staticmethod
+    def sptrsv_csc_block(sup, nrows, Lp, Li, Lx, b):
+        x = _f64(b).copy()
+        rlib().ref_sptrsv_csc_block(len(x), _p(_i32(Lp)), _p(_i32(Li)), _p(_i32(nrows)),
+                                    _p(_f64(Lx).copy()), _p(x), len(sup) - 1, _p(_i32(sup)))
+        return x
 
     @staticmethod
     def spic0_csr(Ap, Ai, Ax):

@@ -304,6 +304,77 @@ int orc_spilu0_csr(int n, int nnz, const int *Ap, const int *Ai, const double *A
   return bad;
 }
 
+/* ------------------------------------------------------ supernodal (BCSC) */
+
+/* sparse_blas/dense_blas/BLAS.cpp:3-96 (lsolve_BLAS): b <- L^-1 b for the
+ * w x w lower-triangular top of a column-major block M (leading dimension
+ * dim), in column blocks of 8, then 4, then one of 2, then a single column.
+ * Inside a block of B columns starting at c: x_j = (b[c+j] - x_0 M[c+j,c]
+ * - ... - x_{j-1} M[c+j,c+j-1]) / M[c+j,c+j], subtractions left to right;
+ * then every later row k of the top: b[k] = b[k] - x_0 M[k,c] - ... -
+ * x_{B-1} M[k,c+B-1]. */
+static void orc_lsolve_block(int dim, int w, const double *M, double *b) {
+  int col = 0;
+  while (col < w) {
+    int B = col < w - 7 ? 8 : col < w - 3 ? 4 : col < w - 1 ? 2 : 1;
+    double xv[8];
+    for (int j = 0; j < B; j++) {
+      double t = b[col + j];
+      for (int i = 0; i < j; i++) t = t - xv[i] * M[(col + j) + (size_t)(col + i) * dim];
+      xv[j] = t / M[(col + j) + (size_t)(col + j) * dim];
+    }
+    for (int j = 0; j < B; j++) b[col + j] = xv[j];
+    if (B > 1)
+      for (int k = col + B; k < w; k++) {
+        double t = b[k];
+        for (int i = 0; i < B; i++) t = t - xv[i] * M[k + (size_t)(col + i) * dim];
+        b[k] = t;
+      }
+    col += B;
+  }
+}
+
+/* sparse_blas/dense_blas/BLAS.cpp:286-349 (matvec_BLAS): b[k] += sum over
+ * the columns of M (nrow x ncol, leading dimension dim) times x, in column
+ * blocks of 8, then 4, then single columns; a block adds its products summed
+ * left to right: b[k] = b[k] + (((x_0 M[k,c] + x_1 M[k,c+1]) + ...)). */
+static void orc_matvec_block(int dim, int nrow, int ncol, const double *M, const double *x,
+                             double *b) {
+  int col = 0;
+  while (col < ncol) {
+    int B = col < ncol - 7 ? 8 : col < ncol - 3 ? 4 : 1;
+    for (int k = 0; k < nrow; k++) {
+      double s = x[col] * M[k + (size_t)col * dim];
+      for (int i = 1; i < B; i++) s = s + x[col + i] * M[k + (size_t)(col + i) * dim];
+      b[k] = b[k] + s;
+    }
+    col += B;
+  }
+}
+
+/* sparse_blas/sptrsv.cpp:176-199 (sym_lib::sptrsv_csc_block): forward solve
+ * on a supernodal lower triangle.  Supernode i owns columns supernodes[i] ..
+ * supernodes[i+1]-1 (width w) and nrows[i] block rows: a column-major dense
+ * block at Lx[Lp[i]] (leading dimension nrows[i]) whose first w rows are
+ * the supernode's own columns; row r of the block is row Li[Lp[i] + r] of L.
+ * Per supernode in order: the dense triangular solve of the top, the
+ * product of the rows below with the solved part into a zeroed temporary,
+ * then x[row] -= temporary, row by row. */
+void orc_sptrsv_csc_block(int n, const int *Lp, const int *Li, const int *nrows,
+                          const double *Lx, double *x, int num_nodes, const int *supernodes) {
+  double *t = (double *)calloc((size_t)(n > 0 ? n : 1), sizeof(double));
+  for (int i = 0; i < num_nodes; i++) {
+    int s = supernodes[i], w = supernodes[i + 1] - s, nr = nrows[i];
+    orc_lsolve_block(nr, w, Lx + Lp[i], x + s);
+    orc_matvec_block(nr, nr - w, w, Lx + Lp[i] + w, x + s, t);
+    for (int k = 0; k < nr - w; k++) {
+      x[Li[Lp[i] + w + k]] -= t[k];
+      t[k] = 0;
+    }
+  }
+  free(t);
+}
+
 /* --------------------------------------------------------- level sets */
 
 /* Level-set inspector on the CSR of L (the reference's build_levelSet_CSC

@@ -33,6 +33,9 @@ void sptrsv_csr_ilu_lbc(int n, const int *Lp, const int *Li, const double *Lx,
                         const int *level_ptr, const int *par_ptr,
                         const int *partition);  // :139-142
 void sptrsv_csc(int n, int *Lp, int *Li, double *Lx, double *x);  // :169-170
+void sptrsv_csc_block(int n, const int *Lp, const int *Li, const int *nrows,
+                      double *Lx, double *x, int num_nodes,
+                      const int *supernodes);  // :212-214
 int ltsolve(int n, int *Lp, int *Li, double *Lx, double *x);      // :330
 void spmv_csr(int n, const int *Ap, const int *Ai, const double *Ax,
               const double *x, double *y);  // :18-20
@@ -87,6 +90,10 @@ void ref_sptrsv_csr_ilu_lbc(int n, const int *Lp, const int *Li,
                             const int *par_ptr, const int *partition) {
   sym_lib::sptrsv_csr_ilu_lbc(n, Lp, Li, Lx, diags, x, level_no, level_ptr,
                               par_ptr, partition);
+}
+void ref_sptrsv_csc_block(int n, const int *Lp, const int *Li, const int *nrows,
+                          double *Lx, double *x, int num_nodes, const int *supernodes) {
+  sym_lib::sptrsv_csc_block(n, Lp, Li, nrows, Lx, x, num_nodes, supernodes);
 }
 void ref_sptrsv_csc(int n, int *Lp, int *Li, double *Lx, double *x) {
   sym_lib::sptrsv_csc(n, Lp, Li, Lx, x);

@@ -22,6 +22,7 @@ Reference mapping (reference include/sympiler/sparse_blas_lib.h):
   Plan.ic0()             IC(0) on the plan's pattern (spic0.cpp:11-46,
                          :106-127)
   ilu0(), Ilu0Plan       sym_lib::spilu0_csr (spilu0.cpp:11-51)
+  SupernodalPlan         sym_lib::sptrsv_csc_block (sptrsv.cpp:176-199)
 """
 from __future__ import annotations
 
@@ -101,6 +102,10 @@ def lib() -> ctypes.CDLL:
             "symamd_ilu0_analyze": [vp, i32, vp, vp, vp, POINTER(vp)],
             "symamd_ilu0_factor": [vp, vp, vp, vp, POINTER(i32)],
             "symamd_ilu0_destroy": [vp],
+            "symamd_sn_analyze": [vp, i32, i32, vp, vp, vp, vp, POINTER(vp)],
+            "symamd_sn_solve": [vp, vp, vp, vp],
+            "symamd_sn_check": [vp, vp],
+            "symamd_sn_destroy": [vp],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -560,6 +565,40 @@ class Ilu0Plan:
         try:
             if self._p:
                 lib().symamd_ilu0_destroy(self._p)
+        except Exception:
+            pass
+
+
+class SupernodalPlan:
+    """Forward solve on a supernodal (BCSC) lower triangle, bit-identical to
+    sym_lib::sptrsv_csc_block (sparse_blas/sptrsv.cpp:176-199).  The
+    structure (numpy int32: supernodes, nrows, Lp, Li) is analysed once;
+    solve(Lx, x) overwrites the device vector x (b in, x out)."""
+
+    def __init__(self, supernodes, nrows, Lp, Li, handle: Handle | None = None):
+        import numpy as np
+        self.h = handle or default_handle()
+        arrs = [np.ascontiguousarray(a, np.int32) for a in (supernodes, nrows, Lp, Li)]
+        self._keep = arrs
+        m = len(arrs[0]) - 1
+        self.n = int(arrs[0][-1]) if m >= 0 else 0
+        self._p = c_void_p()
+        _check(lib().symamd_sn_analyze(self.h._h, self.n, m,
+                                       *(a.ctypes.data_as(c_void_p) for a in arrs),
+                                       byref(self._p)))
+
+    def solve(self, Lx, x):
+        _require_cuda(Lx, x)
+        _check(lib().symamd_sn_solve(self.h._h, self._p, _ptr(Lx), _ptr(x)))
+        return x
+
+    def check(self):
+        _check(lib().symamd_sn_check(self.h._h, self._p))
+
+    def __del__(self):
+        try:
+            if self._p:
+                lib().symamd_sn_destroy(self._p)
         except Exception:
             pass
 

@@ -1018,4 +1018,30 @@ int symamd_ilu0(symamd_handle h, int n, const int *d_Ap, const int *d_Ai, const 
   return rc;
 }
 
+int symamd_sn_analyze(symamd_handle h, int n, int m, const int *h_sup, const int *h_nrows,
+                      const int *h_Lp, const int *h_Li, symamd_snplan *plan) {
+  SYMAMD_TRY(check_handle(h));
+  if (!plan) return fail(SYMAMD_ERR_INVALID_ARG, "symamd_sn_analyze: null plan");
+  *plan = nullptr;
+  return sn_analyze(h, n, m, h_sup, h_nrows, h_Lp, h_Li, plan);
+}
+
+int symamd_sn_solve(symamd_handle h, symamd_snplan plan, const double *d_Lx, double *d_x) {
+  SYMAMD_TRY(check_handle(h));
+  if (!plan || (plan->m > 0 && (!d_Lx || !d_x)))
+    return fail(SYMAMD_ERR_INVALID_ARG, "symamd_sn_solve: bad arguments");
+  return sn_solve(h, plan, d_Lx, d_x);
+}
+
+int symamd_sn_check(symamd_handle h, symamd_snplan plan) {
+  SYMAMD_TRY(check_handle(h));
+  if (!plan) return fail(SYMAMD_ERR_INVALID_ARG, "symamd_sn_check: null plan");
+  return sn_check(h, plan);
+}
+
+int symamd_sn_destroy(symamd_snplan plan) {
+  sn_destroy(plan);
+  return SYMAMD_OK;
+}
+
 }  // extern "C"

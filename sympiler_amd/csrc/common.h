@@ -336,6 +336,19 @@ struct symamd_ilu0_s {
   int *d_pa = nullptr, *d_pb = nullptr, *d_dpos = nullptr;
 };
 
+// Supernodal (BCSC) solve plan (supernodal.hip): the reference's supernode
+// arrays on the device, the temporaries' offsets, the per-row lists of the
+// temporaries each row subtracts, the dependency lists, tags and state.
+constexpr int SN_WMAX = 1024;  // widest supernode (LDS of one wave: 8 KiB)
+struct symamd_snplan_s {
+  int n = 0, m = 0;
+  int64_t tcount = 0, deps = 0;
+  int *d_sup = nullptr, *d_nrows = nullptr, *d_Lp = nullptr, *d_tb = nullptr;
+  int *d_cptr = nullptr, *d_cidx = nullptr, *d_dptr = nullptr, *d_dlist = nullptr;
+  double *d_T = nullptr;
+  int *d_flags = nullptr, *d_state = nullptr;
+};
+
 struct symamd_graph_s {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -451,6 +464,13 @@ int ilu0_analyze(symamd_handle h, int n, const int *d_Ap, const int *d_Ai, const
 int ilu0_numeric(symamd_handle h, symamd_ilu0_s *q, const double *d_Ax, double *d_l,
                  int *bad_row);
 void ilu0_destroy(symamd_ilu0_s *q);
+// Supernodal solve (supernodal.hip): analysis on host structure arrays,
+// asynchronous solve, watchdog check, destroy.
+int sn_analyze(symamd_handle h, int n, int m, const int *h_sup, const int *h_nrows,
+               const int *h_Lp, const int *h_Li, symamd_snplan_s **out);
+int sn_solve(symamd_handle h, symamd_snplan_s *q, const double *d_Lx, double *d_x);
+int sn_check(symamd_handle h, symamd_snplan_s *q);
+void sn_destroy(symamd_snplan_s *q);
 int ilu_lower(symamd_handle h, int n, const int *d_Ap, const int *d_Ai,
               const double *d_Ax, const int *d_diags, int unit, int *d_Lp,
               int *d_Li, double *d_Lx, int *nnz_out);

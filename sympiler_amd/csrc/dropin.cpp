@@ -6,6 +6,7 @@
 // per row (bit-identical to the serial reference), and x is copied back.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -420,6 +421,47 @@ void spilu0_csr_lbc(int n, int nnz, const int *Ap, const int *Ai, const double *
   (void)partition;
   (void)tempvecs;
   spilu0_csr(n, nnz, Ap, Ai, Ax, A_diag, l);
+}
+
+// ------------------------------------------------------------ supernodal
+// sptrsv_csc_block on the device: the structure is analysed from the host
+// arrays, Lx and x are copied in, x is copied back.
+void sptrsv_csc_block(int n, const int *Lp, const int *Li, const int *nrows, double *Lx,
+                      double *x, int num_nodes, const int *supernodes) {
+  if (n <= 0 || num_nodes <= 0) return;
+  symamd_handle h = handle();
+  int64_t vals = 0;
+  for (int i = 0; i < num_nodes; ++i)
+    vals = std::max<int64_t>(vals, (int64_t)Lp[i] +
+                                       (int64_t)nrows[i] * (supernodes[i + 1] - supernodes[i]));
+  symamd_snplan plan = nullptr;
+  check(symamd_sn_analyze(h, n, num_nodes, supernodes, nrows, Lp, Li, &plan), "sptrsv_csc_block");
+  struct Guard {
+    symamd_snplan p;
+    ~Guard() { symamd_sn_destroy(p); }
+  } guard{plan};
+  DevBuf<double> dLx((size_t)vals, Lx), dx(n, x);
+  check(symamd_sn_solve(h, plan, dLx.p, dx.p), "sptrsv_csc_block");
+  check(symamd_sn_check(h, plan), "sptrsv_csc_block");
+  dx.download(x, n);
+}
+
+// The level schedule and per-thread temporaries are not needed (the
+// reference scatters with atomics under it; the result here is the serial
+// routine's).
+void sptrsv_csc_levelset_block(int n, const int *Lp, const int *Li, const int *nrows,
+                               double *Lx, double *x, const int *levelPtr, const int *levels,
+                               int n_lev, const int *supernodes, const int *sup2node,
+                               double **tempvecs) {
+  (void)levelPtr;
+  (void)tempvecs;
+  // the supernode count: sup2node names every supernode once
+  int num_nodes = 0;
+  if (n_lev > 0)
+    for (int j = levelPtr[0]; j < levelPtr[n_lev]; ++j)
+      num_nodes = std::max(num_nodes, sup2node[j] + 1);
+  (void)levels;
+  sptrsv_csc_block(n, Lp, Li, nrows, Lx, x, num_nodes, supernodes);
 }
 
 int build_levelSet_CSC(int n, const int *Lp, const int *Li, int *&level_ptr,

@@ -263,4 +263,63 @@ int64_t symgen_random_lower(int n, double density, int min_off, uint64_t seed,
   return p;
 }
 
+// Supernodal (BCSC) lower triangle (structure from stream (seed, 4), values
+// from (seed, 5)): supernode i owns columns s_i .. s_i+w_i-1
+// (w_i ~ U{1..wmax}) and nrows_i = w_i + c_i block rows, the c_i ~
+// U{0..min(rmax, rows left)} rows below it distinct and ascending, drawn from
+// the next `window` rows.  Values: a column-major nrows_i x w_i block at
+// Lx[Lp[i]]; on the top block's diagonal 1 + U[0, 1), below it U[-1, 1) /
+// nrows_i; the top block's strictly upper part is never read by the solve
+// and holds NaN.  Li[Lp[i] + j*nrows_i + r] = row of block row r (every
+// column of the block repeats the row list).  With Lp == NULL only the
+// counts are returned: *num_nodes and the value count.
+int64_t symgen_bcsc(int n, int wmax, int rmax, int window, uint64_t seed, int *num_nodes,
+                    int *supernodes, int *nrows, int *Lp, int *Li, double *Lx) {
+  Stream st(seed, 4);   // structure
+  Stream sv(seed, 5);   // values (drawn only when filling)
+  std::vector<int> rows;
+  int64_t p = 0;
+  int i = 0;
+  for (int s = 0; s < n; ++i) {
+    int w = 1 + (int)(st.unit() * wmax);
+    if (w > wmax) w = wmax;
+    if (w > n - s) w = n - s;
+    const int e = s + w;  // first row below the supernode
+    const int span = std::min(window, n - e);
+    int c = (int)(st.unit() * (std::min(rmax, span) + 1));
+    if (c > span) c = span;
+    rows.clear();
+    while ((int)rows.size() < c) {  // c distinct rows of [e, e + span)
+      int r = e + (int)(st.unit() * span);
+      if (r >= e + span) r = e + span - 1;
+      if (std::find(rows.begin(), rows.end(), r) == rows.end()) rows.push_back(r);
+    }
+    std::sort(rows.begin(), rows.end());
+    const int nr = w + c;
+    if (Lp) {
+      supernodes[i] = s;
+      nrows[i] = nr;
+      Lp[i] = (int)p;
+      for (int j = 0; j < w; ++j)
+        for (int r = 0; r < nr; ++r) {
+          const int64_t q = p + (int64_t)j * nr + r;
+          Li[q] = r < w ? s + r : rows[r - w];
+          double v;
+          if (r < j) v = NAN;
+          else if (r == j) v = 1.0 + sv.unit();
+          else v = (2.0 * sv.unit() - 1.0) / nr;
+          Lx[q] = v;
+        }
+    }
+    p += (int64_t)nr * w;
+    s = e;
+  }
+  if (Lp) {
+    supernodes[i] = n;
+    Lp[i] = (int)p;
+  }
+  *num_nodes = i;
+  return p;
+}
+
 }  // extern "C"

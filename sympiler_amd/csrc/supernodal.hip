@@ -1,0 +1,284 @@
+// supernodal.hip -- forward solve on a supernodal (BCSC) lower triangle for
+// gfx950, bit-identical to the reference's blocked solve.
+//
+// Reference: sparse_blas/sptrsv.cpp:176-199 (sptrsv_csc_block) with the dense
+// kernels of sparse_blas/dense_blas/BLAS.cpp: lsolve_BLAS (:3-96) and
+// matvec_BLAS (:286-349); sparse_blas/sptrsv.cpp:239-279
+// (sptrsv_csc_levelset_block) is the same per-supernode routine under a level
+// schedule, scattering with atomics.
+//
+// Layout (the reference's): supernode i owns columns s_i .. s_i + w_i - 1 and
+// nrows_i block rows; its values are a column-major nrows_i x w_i block at
+// Lx[Lp[i]] whose first w_i rows are its own columns and whose row r is row
+// Li[Lp[i] + r] of L.  Per supernode the reference solves the dense top
+// (column blocks of 8, then 4, then 2, then 1; inside a block x_j = (b_j -
+// x_0 m_j0 - ... - x_{j-1} m_j,j-1) / m_jj left to right, then every later
+// top row subtracts the block's products left to right), multiplies the rows
+// below by the solved part (column blocks of 8, then 4, then 1; each block's
+// products summed left to right and added to a zeroed temporary), and
+// subtracts the temporary from x at those rows, supernode after supernode.
+//
+// Executor: one wavefront per supernode, supernodes taken by ticket in index
+// order (a supernode only reads supernodes before it, so every wait is for a
+// supernode a running wave already holds).  The subtraction into x is pulled
+// instead of pushed: each supernode writes its temporary to a buffer T and
+// publishes a tag; a supernode starts by waiting for the supernodes that
+// write to its own rows, then forms x at each own row as b minus their
+// temporaries in ascending supernode order -- the order the reference's
+// serial loop subtracts them in -- so x is bit-identical.  The top block is
+// solved in the wave's LDS: lane j of a column block holds row j's entries,
+// x_j is formed on lane j and broadcast; the rows below are one lane each.
+#include <algorithm>
+#include <vector>
+
+#include "device.h"
+
+namespace symamd {
+
+namespace {
+
+struct SnArgs {
+  const int *sup, *nrows, *Lp;  // the reference's supernode arrays
+  const double *Lx;
+  double *x;                    // right-hand side in, solution out
+  double *T;                    // temporaries: supernode i's at T[tb[i] ..]
+  const int *tb;                // nrows_i - w_i values each
+  const int *cptr, *cidx;       // row r subtracts T[cidx[cptr[r] .. cptr[r+1])]
+  const int *dptr, *dlist;      // supernodes writing to supernode i's rows
+  int *flags, *state;
+  int m, total_waves;
+  uint64_t timeout;
+};
+
+// Column block at `col` of a top of width w: 8 while 8 fit with one to
+// spare (matvec) or without (lsolve), else 4, else 2 (lsolve) or 1.
+__device__ __forceinline__ int lsolve_block(int col, int w) {
+  return col < w - 7 ? 8 : col < w - 3 ? 4 : col < w - 1 ? 2 : 1;
+}
+__device__ __forceinline__ int matvec_block(int col, int w) {
+  return col < w - 7 ? 8 : col < w - 3 ? 4 : 1;
+}
+
+__global__ __launch_bounds__(256) void k_supernodal(SnArgs a) {
+  __shared__ double slab[4][SN_WMAX];
+  double *xs = slab[threadIdx.x >> 6];
+  const int lane = threadIdx.x & 63;
+  const int target = ld_dev(&a.state[ST_EPOCH]) + 1;
+  for (;;) {
+    int i = 0;
+    if (lane == 0) i = atomicAdd(&a.state[ST_TICKET], 1);
+    i = __shfl(i, 0);
+    if (i >= a.m) break;
+    const int s = a.sup[i], w = a.sup[i + 1] - s, nr = a.nrows[i];
+    const double *M = a.Lx + a.Lp[i];
+    // 1. the supernodes that write to this one's rows
+    wait_units(a.dptr, a.dlist, i, a.flags, target, a.state, a.timeout);
+    // 2. own rows: b minus the temporaries, in ascending supernode order
+    for (int r = lane; r < w; r += 64) {
+      double v = a.x[s + r];
+      for (int c = a.cptr[s + r], ce = a.cptr[s + r + 1]; c < ce; ++c)
+        v -= ld_dev(&a.T[a.cidx[c]]);
+      xs[r] = v;
+    }
+    wave_lds_sync();
+    // 3. the dense top, column block by column block
+    for (int col = 0; col < w;) {
+      const int B = lsolve_block(col, w);
+      // lane j < B holds row col+j of the block's columns (and its diagonal)
+      double mrow[8];
+#pragma unroll
+      for (int c = 0; c < 8; ++c)
+        mrow[c] = lane < B && c <= lane ? M[(col + lane) + (int64_t)(col + c) * nr] : 1.0;
+      double xv[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        if (j < B) {
+          double t = xs[col + j];
+#pragma unroll
+          for (int c = 0; c < j; ++c) t = t - xv[c] * mrow[c];
+          const double q = t / mrow[j];
+          xv[j] = __shfl(q, j);
+        } else {
+          xv[j] = 0.0;
+        }
+      }
+      wave_lds_sync();
+      if (lane < B) xs[col + lane] = lane == 0 ? xv[0] : lane == 1 ? xv[1] : lane == 2 ? xv[2]
+                                   : lane == 3 ? xv[3] : lane == 4 ? xv[4] : lane == 5 ? xv[5]
+                                   : lane == 6 ? xv[6] : xv[7];
+      for (int k = col + B + lane; k < w; k += 64) {
+        double t = xs[k];
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+          if (c < B) t = t - xv[c] * M[k + (int64_t)(col + c) * nr];
+        xs[k] = t;
+      }
+      wave_lds_sync();
+      col += B;
+    }
+    // 4. the rows below: products with the solved top into T
+    const int tb = a.tb[i];
+    for (int k = lane; k < nr - w; k += 64) {
+      const double *Mk = M + w + k;
+      double t = 0.0;
+      for (int col = 0; col < w;) {
+        const int B = matvec_block(col, w);
+        double sum = xs[col] * Mk[(int64_t)col * nr];
+#pragma unroll
+        for (int c = 1; c < 8; ++c)
+          if (c < B) sum = sum + xs[col + c] * Mk[(int64_t)(col + c) * nr];
+        t = t + sum;
+        col += B;
+      }
+      st_dev(&a.T[tb + k], t);
+    }
+    for (int r = lane; r < w; r += 64) st_dev(&a.x[s + r], xs[r]);
+    // 5. publish
+    publish_fence();
+    if (lane == 0) st_dev(&a.flags[i], target);
+    wave_lds_sync();
+  }
+  persistent_finish(a.state, a.total_waves);
+}
+
+}  // namespace
+
+int sn_analyze(symamd_handle h, int n, int m, const int *h_sup, const int *h_nrows,
+               const int *h_Lp, const int *h_Li, symamd_snplan_s **out) {
+  if (n < 0 || m < 0 || (m > 0 && (!h_sup || !h_nrows || !h_Lp || !h_Li)))
+    return fail(SYMAMD_ERR_INVALID_ARG, "supernodal analyze: bad arguments");
+  if (m == 0 && n > 0)
+    return fail(SYMAMD_ERR_STRUCTURE, "supernodal analyze: no supernodes for %d columns", n);
+  if (m > 0 && (h_sup[0] != 0 || h_sup[m] != n))
+    return fail(SYMAMD_ERR_STRUCTURE, "supernodes must cover columns 0 .. n-1");
+  std::vector<int> tb(m + 1, 0), cnt(n + 1, 0);
+  for (int i = 0; i < m; ++i) {
+    const int s = h_sup[i], w = h_sup[i + 1] - s, nr = h_nrows[i];
+    if (w < 1 || nr < w)
+      return fail(SYMAMD_ERR_STRUCTURE, "supernode %d: width %d, %d rows", i, w, nr);
+    if (w > SN_WMAX)
+      return fail(SYMAMD_ERR_NOT_SUPPORTED, "supernode %d is %d columns wide (at most %d)", i,
+                  w, SN_WMAX);
+    if (i > 0 && h_Lp[i] < h_Lp[i - 1] + (int64_t)h_nrows[i - 1] * (h_sup[i] - h_sup[i - 1]))
+      return fail(SYMAMD_ERR_STRUCTURE, "supernode %d's values overlap the previous block", i);
+    // the rows below the top must lie below the supernode (the top's own
+    // row ids are not read, as in the reference)
+    for (int r = w; r < nr; ++r) {
+      const int row = h_Li[h_Lp[i] + r];
+      if (row < s + w || row >= n)
+        return fail(SYMAMD_ERR_STRUCTURE, "supernode %d: block row %d is row %d", i, r, row);
+      cnt[row + 1]++;
+    }
+    tb[i + 1] = tb[i] + (nr - w);
+  }
+  for (int r = 0; r < n; ++r) cnt[r + 1] += cnt[r];
+  // contributions per row in ascending supernode order; dependency lists
+  std::vector<int> cidx(cnt[n] > 0 ? cnt[n] : 1), fill(cnt.begin(), cnt.end() - 1);
+  std::vector<int> owner(n);
+  for (int i = 0; i < m; ++i)
+    for (int c = h_sup[i]; c < h_sup[i + 1]; ++c) owner[c] = i;
+  std::vector<std::vector<int>> deps(m);
+  for (int i = 0; i < m; ++i) {
+    const int w = h_sup[i + 1] - h_sup[i];
+    for (int k = 0; k < h_nrows[i] - w; ++k) {
+      const int row = h_Li[h_Lp[i] + w + k];
+      cidx[fill[row]++] = tb[i] + k;
+      std::vector<int> &d = deps[owner[row]];
+      if (d.empty() || d.back() != i) d.push_back(i);
+    }
+  }
+  std::vector<int> dptr(m + 1, 0), dlist;
+  for (int j = 0; j < m; ++j) {
+    std::sort(deps[j].begin(), deps[j].end());
+    deps[j].erase(std::unique(deps[j].begin(), deps[j].end()), deps[j].end());
+    dlist.insert(dlist.end(), deps[j].begin(), deps[j].end());
+    dptr[j + 1] = (int)dlist.size();
+  }
+  auto *q = new symamd_snplan_s();
+  q->n = n;
+  q->m = m;
+  q->tcount = tb[m];
+  int rc = SYMAMD_OK;
+  auto up = [&](int **dst, const int *src, size_t count) {
+    if (rc != SYMAMD_OK) return;
+    if (hipMalloc(dst, sizeof(int) * (count ? count : 1)) != hipSuccess ||
+        (count && hipMemcpy(*dst, src, sizeof(int) * count, hipMemcpyHostToDevice) != hipSuccess))
+      rc = fail(SYMAMD_ERR_ALLOC, "supernodal plan upload failed");
+  };
+  up(&q->d_sup, h_sup, m + 1);
+  up(&q->d_nrows, h_nrows, m);
+  up(&q->d_Lp, h_Lp, m + 1);
+  up(&q->d_tb, tb.data(), m + 1);
+  up(&q->d_cptr, cnt.data(), n + 1);
+  up(&q->d_cidx, cidx.data(), cnt[n]);
+  up(&q->d_dptr, dptr.data(), m + 1);
+  up(&q->d_dlist, dlist.data(), dlist.size());
+  if (rc == SYMAMD_OK &&
+      (hipMalloc(&q->d_T, sizeof(double) * (q->tcount > 0 ? q->tcount : 1)) != hipSuccess ||
+       hipMalloc(&q->d_flags, sizeof(int) * (m > 0 ? m : 1)) != hipSuccess ||
+       hipMalloc(&q->d_state, sizeof(int) * ST_WORDS) != hipSuccess ||
+       hipMemset(q->d_flags, 0, sizeof(int) * (m > 0 ? m : 1)) != hipSuccess ||
+       hipMemset(q->d_state, 0, sizeof(int) * ST_WORDS) != hipSuccess))
+    rc = fail(SYMAMD_ERR_ALLOC, "supernodal plan allocation failed");
+  if (rc != SYMAMD_OK) {
+    sn_destroy(q);
+    return rc;
+  }
+  q->deps = (int64_t)dlist.size();
+  (void)h;
+  *out = q;
+  return SYMAMD_OK;
+}
+
+int sn_solve(symamd_handle h, symamd_snplan_s *q, const double *d_Lx, double *d_x) {
+  if (q->m == 0) return SYMAMD_OK;
+  SnArgs a;
+  a.sup = q->d_sup;
+  a.nrows = q->d_nrows;
+  a.Lp = q->d_Lp;
+  a.Lx = d_Lx;
+  a.x = d_x;
+  a.T = q->d_T;
+  a.tb = q->d_tb;
+  a.cptr = q->d_cptr;
+  a.cidx = q->d_cidx;
+  a.dptr = q->d_dptr;
+  a.dlist = q->d_dlist;
+  a.flags = q->d_flags;
+  a.state = q->d_state;
+  a.m = q->m;
+  a.timeout = h->timeout_ticks();
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)&k_supernodal, 256,
+                                                   0) != hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  int64_t grid = (int64_t)per_cu * h->num_cus;
+  const int64_t want = ((int64_t)q->m + 3) / 4;
+  if (want < grid) grid = want;
+  if (grid < 1) grid = 1;
+  a.total_waves = (int)grid * 4;
+  // tickets are dealt in supernode order, so any grid size is safe
+  k_supernodal<<<(unsigned)grid, 256, 0, h->stream>>>(a);
+  SYMAMD_HIP(hipGetLastError());
+  return SYMAMD_OK;
+}
+
+int sn_check(symamd_handle h, symamd_snplan_s *q) {
+  int st[ST_WORDS];
+  SYMAMD_HIP(hipStreamSynchronize(h->stream));
+  SYMAMD_HIP(hipMemcpy(st, q->d_state, sizeof(st), hipMemcpyDeviceToHost));
+  if (st[ST_ERROR]) return fail(SYMAMD_ERR_TIMEOUT, "a supernodal wait exceeded the watchdog");
+  return SYMAMD_OK;
+}
+
+void sn_destroy(symamd_snplan_s *q) {
+  if (!q) return;
+  void *ptrs[] = {q->d_sup,   q->d_nrows, q->d_Lp,    q->d_tb,    q->d_cptr, q->d_cidx,
+                  q->d_dptr,  q->d_dlist, q->d_T,     q->d_flags, q->d_state};
+  for (void *x : ptrs)
+    if (x) (void)hipFree(x);
+  delete q;
+}
+
+}  // namespace symamd

@@ -65,6 +65,9 @@ def gen_lib():
         g.symgen_batch_dims.restype = None
         g.symgen_random_lower.argtypes = [c_int, c_double, c_int, c_uint64, vp, vp, vp]
         g.symgen_random_lower.restype = c_int64
+        g.symgen_bcsc.argtypes = [c_int, c_int, c_int, c_int, c_uint64, POINTER(c_int),
+                                  vp, vp, vp, vp, vp]
+        g.symgen_bcsc.restype = c_int64
         _g = g
     return _g
 
@@ -165,6 +168,25 @@ def random_lower(n, density, min_off=0, seed=0):
     Lx = np.empty(max(nnz, 1), np.float64)
     gen_lib().symgen_random_lower(n, density, min_off, seed, _p(Lp), _p(Li), _p(Lx))
     return Lp, Li[:nnz], Lx[:nnz]
+
+
+def bcsc(n, wmax=32, rmax=32, window=4096, seed=6):
+    """Supernodal (BCSC) lower triangle for sptrsv_csc_block (see
+    symgen_bcsc): (supernodes, nrows, Lp, Li, Lx)."""
+    nn = c_int()
+    cnt = gen_lib().symgen_bcsc(n, wmax, rmax, window, seed, ctypes.byref(nn), None, None,
+                                None, None, None)
+    if cnt >= 2 ** 31:
+        raise ValueError("supernodal value count exceeds int32 offsets")
+    m = nn.value
+    sup = np.empty(m + 1, np.int32)
+    nrows = np.empty(max(m, 1), np.int32)[:m]
+    Lp = np.empty(m + 1, np.int32)
+    Li = np.empty(max(cnt, 1), np.int32)[:cnt]
+    Lx = np.empty(max(cnt, 1), np.float64)[:cnt]
+    gen_lib().symgen_bcsc(n, wmax, rmax, window, seed, ctypes.byref(nn), _p(sup), _p(nrows),
+                          _p(Lp), _p(Li), _p(Lx))
+    return sup, nrows, Lp, Li, Lx
 
 
 @dataclass

@@ -62,6 +62,7 @@ def test_sym_lib_dropins_exported_with_reference_mangling():
         "_ZN7sym_lib18spic0_csc_left_LBCEiPKiS1_PKdS1_S1_iS1_S1_S1_Pd",
         "_ZN7sym_lib10spilu0_csrEiiPKiS1_PKdS1_Pd",
         "_ZN7sym_lib14spilu0_csr_lbcEiiPKiS1_PKdS1_PdiS1_S1_S1_Pi",
+        "_ZN7sym_lib16sptrsv_csc_blockEiPKiS1_S1_PdS2_iS1_",
     }
     missing = want - exported(LIB)
     assert not missing, missing
@@ -77,14 +78,14 @@ def test_mangled_names_match_the_reference_objects():
                "sptrsv_csr_lbc_redundant", "sptrsv_csr_ilu", "sptrsv_csr_ilu_lbc",
                "sptrsv_csc", "sptrsv_csc_levelset", "sptrsv_csc_lbc", "ltsolve",
                "spmv_csr", "spic0_csc_left", "spic0_csc_left_LBC", "spic0_csr",
-               "spic0_csr_lbc"}
+               "spic0_csr_lbc", "sptrsv_csc_block", "sptrsv_csc_levelset_block"}
 
     def fname(sym):  # the length-prefixed function name after "sym_lib"
         m = re.match(r"_ZN7sym_lib(\d+)", sym)
         return sym[m.end():m.end() + int(m.group(1))] if m else None
 
     ref = {s for s in exported(oracle.REF_PATH) if fname(s) in on_path}
-    assert len(ref) == 16, sorted(ref)  # spmv_csr has two overloads
+    assert len(ref) == 18, sorted(ref)  # spmv_csr has two overloads
     missing = ref - exported(LIB)
     assert not missing, missing
 

@@ -1,0 +1,78 @@
+"""Supernodal (BCSC) solve on the device (symamd_sn_* / SupernodalPlan and
+the sym_lib::sptrsv_csc_block drop-ins) against the oracle's restatement of
+sparse_blas/sptrsv.cpp:176-199: bit-identical on widths from 1 to past one
+wavefront and row blocks longer than a wavefront, repeated solves, and the
+structure errors."""
+import ctypes
+import os
+from ctypes import c_int, c_void_p
+
+import numpy as np
+import pytest
+
+import oracle
+import sympiler_amd as sa
+from sympiler_amd import workloads as W
+from tests.helpers import bitwise_equal, to_dev
+from tests.test_supernodal_oracle import CASES
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("case", CASES + [(60000, 64, 64, 8192, 7)])
+def test_supernodal_bitwise(dev, case):
+    n = case[0]
+    sup, nrows, Lp, Li, Lx = W.bcsc(*case)
+    print(f"[sn] case={case} supernodes={len(sup) - 1} values={len(Lx)}", flush=True)
+    b = W.uniform(case[-1], 0, n)
+    want = oracle.sptrsv_csc_block(sup, nrows, Lp, Li, Lx, b)
+    plan = sa.SupernodalPlan(sup, nrows, Lp, Li)
+    dLx = to_dev(Lx)
+    for _ in range(2):  # the second solve reuses the tags (next epoch)
+        x = to_dev(b.copy())
+        plan.solve(dLx, x)
+        plan.check()
+        assert bitwise_equal(x.cpu().numpy(), want)
+
+
+def test_supernodal_structure_errors(dev):
+    sup, nrows, Lp, Li, Lx = W.bcsc(300, 8, 8, 60, 1)
+    i = int(np.argmax(nrows - np.diff(sup)))  # a supernode with rows below
+    bad = Li.copy()
+    bad[Lp[i] + (sup[i + 1] - sup[i])] = sup[i]  # a row inside the supernode
+    with pytest.raises(sa.SymAmdError, match="block row"):
+        sa.SupernodalPlan(sup, nrows, Lp, bad)
+    wide = np.array([0, 1100], np.int32)
+    with pytest.raises(sa.SymAmdError, match="wide"):
+        sa.SupernodalPlan(wide, np.array([1100], np.int32), np.array([0, 1100 * 1100], np.int32),
+                          np.arange(1100, dtype=np.int32))
+
+
+LIB = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                   "sympiler_amd", "libsympiler_amd.so")
+
+
+def P(a):
+    return a.ctypes.data_as(c_void_p)
+
+
+def test_sptrsv_csc_block_dropins(dev):
+    L = ctypes.CDLL(LIB)
+    n = 3000
+    sup, nrows, Lp, Li, Lx = W.bcsc(n, 40, 24, 400, 2)
+    b = W.uniform(2, 0, n)
+    want = oracle.sptrsv_csc_block(sup, nrows, Lp, Li, Lx, b)
+    m = len(sup) - 1
+    x = b.copy()
+    L["_ZN7sym_lib16sptrsv_csc_blockEiPKiS1_S1_PdS2_iS1_"](
+        c_int(n), P(Lp), P(Li), P(nrows), P(Lx), P(x), c_int(m), P(sup))
+    assert bitwise_equal(x, want)
+    # the level-scheduled variant: one level per supernode, sup2node = identity
+    levelPtr = np.arange(m + 1, dtype=np.int32)
+    levels = sup[:-1].copy()
+    sup2node = np.arange(m, dtype=np.int32)
+    x = b.copy()
+    L["_ZN7sym_lib25sptrsv_csc_levelset_blockEiPKiS1_S1_PdS2_S1_S1_iS1_S1_PS2_"](
+        c_int(n), P(Lp), P(Li), P(nrows), P(Lx), P(x), P(levelPtr), P(levels), c_int(m),
+        P(sup), P(sup2node), None)
+    assert bitwise_equal(x, want)
