@@ -57,6 +57,8 @@ def parse():
                     help="only run a device copy of 1 GiB (x5) to scale HBM counters")
     ap.add_argument("--factors", action="store_true",
                     help="only time the device IC(0) and ILU(0) factorizations")
+    ap.add_argument("--supernodal", action="store_true",
+                    help="only time the supernodal (BCSC) solve")
     return ap.parse_args()
 
 
@@ -177,6 +179,30 @@ def other_configs(torch, sa, W, dev, stream):
     del Lx_t, b_t
 
     out.update(factor_configs(torch, sa, W, dev, stream))
+    out["supernodal_bcsc_tree13"] = supernodal_config(torch, sa, W, dev, stream)
+    return out
+
+
+def supernodal_config(torch, sa, W, dev, stream):
+    """Supernodal (BCSC) forward solve on a nested-dissection-shaped factor
+    (13 tree levels, 8,191 supernodes, 221,040 columns, 36 M block values):
+    symamd_sn_solve, bit-identical to sptrsv_csc_block."""
+    sup, nrows, Lp, Li, Lx = W.bcsc_tree(13)
+    n = int(sup[-1])
+    b = W.uniform(8, 0, n)
+    plan = sa.SupernodalPlan(sup, nrows, Lp, Li)
+    dLx = torch.from_numpy(Lx).to(dev)
+    db = torch.from_numpy(b).to(dev)
+    x = db.clone()
+    wt, et = timed_steps(lambda: plan.solve(dLx, x), lambda: x.copy_(db), 10, 2, stream, torch)
+    plan.check()
+    ms = float(np.mean(et))
+    byts = 8 * len(Lx) + 16 * n
+    out = {"n": n, "supernodes": len(sup) - 1, "values": len(Lx),
+           "widest": int(np.diff(sup).max()), "kernel_ms": round(ms, 4),
+           "gbs": round(byts / (ms * 1e-3) / 1e9, 1)}
+    log(f"supernodal tree13: {out}")
+    del plan, dLx, db, x
     return out
 
 
@@ -261,13 +287,14 @@ def main():
     if args.calibrate_copy:
         calibrate_copy()
         return
-    if args.factors:
+    if args.factors or args.supernodal:
         import torch
         import sympiler_amd as sa
         from sympiler_amd import workloads as W
         dev = torch.device("cuda:0")
-        print(json.dumps({"factors": factor_configs(torch, sa, W, dev,
-                                                    torch.cuda.current_stream())}), flush=True)
+        fn = factor_configs if args.factors else supernodal_config
+        print(json.dumps({"factors" if args.factors else "supernodal":
+                          fn(torch, sa, W, dev, torch.cuda.current_stream())}), flush=True)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; the multi-GPU "
@@ -539,6 +566,14 @@ def cpu_baseline(Lp, Li, Lx, Bcols, Xgpu, args, k):
     t0 = time.perf_counter()
     oracle.spilu0_csr(Ap7, Ai7, Ax7, d7)
     ilu_ms = round(1e3 * (time.perf_counter() - t0), 1)
+    # the supernodal solve of the detail: the reference's serial
+    # sptrsv_csc_block (or its restatement)
+    sup8, nr8, Lp8, Li8, Lx8 = W.bcsc_tree(13)
+    b8 = W.uniform(8, 0, int(sup8[-1]))
+    blk = oracle.ref.sptrsv_csc_block if kind == "reference" else oracle.sptrsv_csc_block
+    t0 = time.perf_counter()
+    blk(sup8, nr8, Lp8, Li8, Lx8, b8)
+    sn_ms = round(1e3 * (time.perf_counter() - t0), 1)
     log(f"cpu baseline ({kind}, {ncores} threads): "
         f"{rate:.2f} RHS-solves/s over {done} solves; GPU vs CPU max rel err {worst:.2e}")
     return {"value": round(rate, 3), "unit": "RHS-solves/s",
@@ -547,7 +582,8 @@ def cpu_baseline(Lp, Li, Lx, Bcols, Xgpu, args, k):
                       f"({t_total:.1f}s)",
             "gpu_vs_cpu_max_rel_err": worst,
             "ic0_spic0_csc_left_128cubed_ms_1_thread": ic0_ms,
-            "ilu0_port_spilu0_csr_grid3d_128cubed_ms_1_thread": ilu_ms}
+            "ilu0_port_spilu0_csr_grid3d_128cubed_ms_1_thread": ilu_ms,
+            "sptrsv_csc_block_tree13_ms_1_thread": sn_ms}
 
 
 if __name__ == "__main__":

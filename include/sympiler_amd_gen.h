@@ -72,6 +72,17 @@ int64_t symgen_bcsc(int n, int wmax, int rmax, int window, uint64_t seed,
                     int *num_nodes, int *supernodes, int *nrows, int *Lp,
                     int *Li, double *Lx);
 
+/* Supernodal lower triangle shaped like a nested-dissection Cholesky factor:
+ * a complete binary tree of `levels` levels of supernodes in postorder; a
+ * node h levels above the leaves is min(wmax, round(wleaf 2^(h/2))) columns
+ * wide and has about rowfac x its width rows below its top, drawn from its
+ * ancestors' columns.  Layout and values as symgen_bcsc.  With Lp == NULL
+ * only *num_nodes, *n_out and the returned value count are produced. */
+int64_t symgen_bcsc_tree(int levels, int wleaf, int wmax, double rowfac,
+                         uint64_t seed, int *num_nodes, int *n_out,
+                         int *supernodes, int *nrows, int *Lp, int *Li,
+                         double *Lx);
+
 #ifdef __cplusplus
 }
 #endif

@@ -411,9 +411,12 @@ class ref:
 
     @staticmethod
     def sptrsv_csc_block(sup, nrows, Lp, Li, Lx, b):
+        # the reference only reads Lx (lsolve_BLAS / matvec_BLAS), so it is
+        # passed without a copy
         x = _f64(b).copy()
         rlib().ref_sptrsv_csc_block(len(x), _p(_i32(Lp)), _p(_i32(Li)), _p(_i32(nrows)),
-                                    _p(_f64(Lx).copy()), _p(x), len(sup) - 1, _p(_i32(sup)))
+                                    _p(np.ascontiguousarray(Lx, np.float64)), _p(x),
+                                    len(sup) - 1, _p(_i32(sup)))
         return x
 
     @staticmethod

@@ -322,4 +322,90 @@ int64_t symgen_bcsc(int n, int wmax, int rmax, int window, uint64_t seed, int *n
   return p;
 }
 
+// Supernodal lower triangle shaped like a nested-dissection Cholesky factor:
+// a complete binary tree of `levels` levels of supernodes numbered in
+// postorder (children's columns before their parent's); a node h levels
+// above the leaves is min(wmax, round(wleaf * 2^(h/2))) columns wide and
+// stores below its top about rowfac x its width rows drawn from its
+// ancestors' columns (all of them if fewer), ascending.  Values as
+// symgen_bcsc; streams (seed, 6) for the structure and (seed, 7) for values.
+int64_t symgen_bcsc_tree(int levels, int wleaf, int wmax, double rowfac, uint64_t seed,
+                         int *num_nodes, int *n_out, int *supernodes, int *nrows, int *Lp,
+                         int *Li, double *Lx) {
+  Stream st(seed, 6), sv(seed, 7);
+  const int m = (1 << levels) - 1;
+  // postorder: node heights and widths, then column ranges
+  std::vector<int> height, parent, order;  // order[k] = node id in postorder
+  height.resize(m);
+  parent.assign(m, -1);
+  // heap numbering: node t has children 2t+1, 2t+2
+  std::vector<int> stack{0}, visited(m, 0);
+  while (!stack.empty()) {
+    const int t = stack.back();
+    const int l = 2 * t + 1, r = 2 * t + 2;
+    if (l < m && !visited[t]) {
+      visited[t] = 1;
+      parent[l] = t;
+      parent[r] = t;
+      stack.push_back(r);
+      stack.push_back(l);
+    } else {
+      stack.pop_back();
+      order.push_back(t);
+    }
+  }
+  for (int t = m - 1; t >= 0; --t) {
+    const int l = 2 * t + 1;
+    height[t] = l < m ? height[l] + 1 : 0;
+  }
+  std::vector<int> width(m), first(m), post(m);
+  int col = 0;
+  for (int k = 0; k < m; ++k) {
+    const int t = order[k];
+    post[t] = k;
+    width[t] = std::min(wmax, std::max(1, (int)std::lround(wleaf * std::pow(2.0, 0.5 * height[t]))));
+    first[t] = col;
+    col += width[t];
+  }
+  const int n = col;
+  std::vector<int> anc, rows;
+  int64_t p = 0;
+  for (int k = 0; k < m; ++k) {
+    const int t = order[k], w = width[t], s = first[t];
+    anc.clear();
+    for (int a = parent[t]; a >= 0; a = parent[a])
+      for (int c = first[a]; c < first[a] + width[a]; ++c) anc.push_back(c);
+    std::sort(anc.begin(), anc.end());
+    int want = (int)std::lround(rowfac * w);
+    if (want > (int)anc.size()) want = (int)anc.size();
+    // a uniform subset of the ancestors' columns: partial Fisher-Yates
+    for (int j = 0; j < want; ++j) {
+      const int q = j + (int)(st.unit() * (anc.size() - j));
+      std::swap(anc[j], anc[q < (int)anc.size() ? q : (int)anc.size() - 1]);
+    }
+    rows.assign(anc.begin(), anc.begin() + want);
+    std::sort(rows.begin(), rows.end());
+    const int nr = w + want;
+    if (Lp) {
+      supernodes[k] = s;
+      nrows[k] = nr;
+      Lp[k] = (int)p;
+      for (int j = 0; j < w; ++j)
+        for (int r = 0; r < nr; ++r) {
+          const int64_t q = p + (int64_t)j * nr + r;
+          Li[q] = r < w ? s + r : rows[r - w];
+          Lx[q] = r < j ? NAN : r == j ? 1.0 + sv.unit() : (2.0 * sv.unit() - 1.0) / nr;
+        }
+    }
+    p += (int64_t)nr * w;
+  }
+  if (Lp) {
+    supernodes[m] = n;
+    Lp[m] = (int)p;
+  }
+  *num_nodes = m;
+  *n_out = n;
+  return p;
+}
+
 }  // extern "C"

@@ -68,6 +68,9 @@ def gen_lib():
         g.symgen_bcsc.argtypes = [c_int, c_int, c_int, c_int, c_uint64, POINTER(c_int),
                                   vp, vp, vp, vp, vp]
         g.symgen_bcsc.restype = c_int64
+        g.symgen_bcsc_tree.argtypes = [c_int, c_int, c_int, c_double, c_uint64,
+                                       POINTER(c_int), POINTER(c_int), vp, vp, vp, vp, vp]
+        g.symgen_bcsc_tree.restype = c_int64
         _g = g
     return _g
 
@@ -186,6 +189,25 @@ def bcsc(n, wmax=32, rmax=32, window=4096, seed=6):
     Lx = np.empty(max(cnt, 1), np.float64)[:cnt]
     gen_lib().symgen_bcsc(n, wmax, rmax, window, seed, ctypes.byref(nn), _p(sup), _p(nrows),
                           _p(Lp), _p(Li), _p(Lx))
+    return sup, nrows, Lp, Li, Lx
+
+
+def bcsc_tree(levels=12, wleaf=16, wmax=512, rowfac=2.0, seed=8):
+    """Supernodal lower triangle shaped like a nested-dissection factor (see
+    symgen_bcsc_tree): (supernodes, nrows, Lp, Li, Lx), n = supernodes[-1]."""
+    nn, n = c_int(), c_int()
+    cnt = gen_lib().symgen_bcsc_tree(levels, wleaf, wmax, rowfac, seed, ctypes.byref(nn),
+                                     ctypes.byref(n), None, None, None, None, None)
+    if cnt >= 2 ** 31:
+        raise ValueError("supernodal value count exceeds int32 offsets")
+    m = nn.value
+    sup = np.empty(m + 1, np.int32)
+    nrows = np.empty(m, np.int32)
+    Lp = np.empty(m + 1, np.int32)
+    Li = np.empty(cnt, np.int32)
+    Lx = np.empty(cnt, np.float64)
+    gen_lib().symgen_bcsc_tree(levels, wleaf, wmax, rowfac, seed, ctypes.byref(nn),
+                               ctypes.byref(n), _p(sup), _p(nrows), _p(Lp), _p(Li), _p(Lx))
     return sup, nrows, Lp, Li, Lx
 
 
