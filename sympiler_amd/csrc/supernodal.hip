@@ -59,6 +59,10 @@ __device__ __forceinline__ int matvec_block(int col, int w) {
   return col < w - 7 ? 8 : col < w - 3 ? 4 : 1;
 }
 
+// Rows per lane processed together in the top's updates and the product
+// below (their loads in flight at once).
+constexpr int SN_ROWS = 4;
+
 __global__ __launch_bounds__(256) void k_supernodal(SnArgs a) {
   __shared__ double slab[4][SN_WMAX];
   double *xs = slab[threadIdx.x >> 6];
@@ -106,31 +110,62 @@ __global__ __launch_bounds__(256) void k_supernodal(SnArgs a) {
       if (lane < B) xs[col + lane] = lane == 0 ? xv[0] : lane == 1 ? xv[1] : lane == 2 ? xv[2]
                                    : lane == 3 ? xv[3] : lane == 4 ? xv[4] : lane == 5 ? xv[5]
                                    : lane == 6 ? xv[6] : xv[7];
-      for (int k = col + B + lane; k < w; k += 64) {
-        double t = xs[k];
+      // SN_ROWS rows per lane at a time: all their entries in flight at once
+      for (int k0 = col + B + lane; k0 < w; k0 += 64 * SN_ROWS) {
+        double mk[SN_ROWS][8];
 #pragma unroll
-        for (int c = 0; c < 8; ++c)
-          if (c < B) t = t - xv[c] * M[k + (int64_t)(col + c) * nr];
-        xs[k] = t;
+        for (int u = 0; u < SN_ROWS; ++u) {
+          const int k = k0 + 64 * u;
+#pragma unroll
+          for (int c = 0; c < 8; ++c)
+            mk[u][c] = k < w && c < B ? M[k + (int64_t)(col + c) * nr] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < SN_ROWS; ++u) {
+          const int k = k0 + 64 * u;
+          if (k < w) {
+            double t = xs[k];
+#pragma unroll
+            for (int c = 0; c < 8; ++c)
+              if (c < B) t = t - xv[c] * mk[u][c];
+            xs[k] = t;
+          }
+        }
       }
       wave_lds_sync();
       col += B;
     }
     // 4. the rows below: products with the solved top into T
     const int tb = a.tb[i];
-    for (int k = lane; k < nr - w; k += 64) {
-      const double *Mk = M + w + k;
-      double t = 0.0;
+    // SN_ROWS rows per lane at a time; each column block's entries of those
+    // rows are loaded together, each row's sum kept in the reference's order
+    for (int k0 = w + lane; k0 < nr; k0 += 64 * SN_ROWS) {
+      double t[SN_ROWS];
+#pragma unroll
+      for (int u = 0; u < SN_ROWS; ++u) t[u] = 0.0;
       for (int col = 0; col < w;) {
         const int B = matvec_block(col, w);
-        double sum = xs[col] * Mk[(int64_t)col * nr];
+        double mk[SN_ROWS][8];
 #pragma unroll
-        for (int c = 1; c < 8; ++c)
-          if (c < B) sum = sum + xs[col + c] * Mk[(int64_t)(col + c) * nr];
-        t = t + sum;
+        for (int u = 0; u < SN_ROWS; ++u) {
+          const int k = k0 + 64 * u;
+#pragma unroll
+          for (int c = 0; c < 8; ++c)
+            mk[u][c] = k < nr && c < B ? M[k + (int64_t)(col + c) * nr] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < SN_ROWS; ++u) {
+          double sum = xs[col] * mk[u][0];
+#pragma unroll
+          for (int c = 1; c < 8; ++c)
+            if (c < B) sum = sum + xs[col + c] * mk[u][c];
+          t[u] = t[u] + sum;
+        }
         col += B;
       }
-      st_dev(&a.T[tb + k], t);
+#pragma unroll
+      for (int u = 0; u < SN_ROWS; ++u)
+        if (k0 + 64 * u < nr) st_dev(&a.T[tb + k0 + 64 * u - w], t[u]);
     }
     for (int r = lane; r < w; r += 64) st_dev(&a.x[s + r], xs[r]);
     // 5. publish
