@@ -93,6 +93,17 @@ __global__ __launch_bounds__(256) void k_supernodal(SnArgs a) {
 #pragma unroll
       for (int c = 0; c < 8; ++c)
         mrow[c] = lane < B && c <= lane ? M[(col + lane) + (int64_t)(col + c) * nr] : 1.0;
+      // the first SN_ROWS rows per lane of the update, loaded with the block
+      // (they do not depend on its values)
+      double mk[SN_ROWS][8];
+      const int k0 = col + B + lane;
+#pragma unroll
+      for (int u = 0; u < SN_ROWS; ++u) {
+        const int k = k0 + 64 * u;
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+          mk[u][c] = k < w && c < B ? M[k + (int64_t)(col + c) * nr] : 0.0;
+      }
       double xv[8];
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
@@ -111,18 +122,19 @@ __global__ __launch_bounds__(256) void k_supernodal(SnArgs a) {
                                    : lane == 3 ? xv[3] : lane == 4 ? xv[4] : lane == 5 ? xv[5]
                                    : lane == 6 ? xv[6] : xv[7];
       // SN_ROWS rows per lane at a time: all their entries in flight at once
-      for (int k0 = col + B + lane; k0 < w; k0 += 64 * SN_ROWS) {
-        double mk[SN_ROWS][8];
+      for (int kc = k0; kc < w; kc += 64 * SN_ROWS) {
+        if (kc != k0) {
 #pragma unroll
-        for (int u = 0; u < SN_ROWS; ++u) {
-          const int k = k0 + 64 * u;
+          for (int u = 0; u < SN_ROWS; ++u) {
+            const int k = kc + 64 * u;
 #pragma unroll
-          for (int c = 0; c < 8; ++c)
-            mk[u][c] = k < w && c < B ? M[k + (int64_t)(col + c) * nr] : 0.0;
+            for (int c = 0; c < 8; ++c)
+              mk[u][c] = k < w && c < B ? M[k + (int64_t)(col + c) * nr] : 0.0;
+          }
         }
 #pragma unroll
         for (int u = 0; u < SN_ROWS; ++u) {
-          const int k = k0 + 64 * u;
+          const int k = kc + 64 * u;
           if (k < w) {
             double t = xs[k];
 #pragma unroll
