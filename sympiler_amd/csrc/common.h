@@ -340,8 +340,10 @@ struct symamd_ilu0_s {
 // arrays on the device, the temporaries' offsets, the per-row lists of the
 // temporaries each row subtracts, the dependency lists, tags and state.
 constexpr int SN_WMAX = 1024;  // widest supernode (LDS of one wave: 8 KiB)
+constexpr int SN_CHUNK = 256;  // rows below a top per task
 struct symamd_snplan_s {
-  int n = 0, m = 0;
+  int n = 0, m = 0, ntasks = 0;
+  int *d_task = nullptr;       // (supernode, chunk) per task
   int64_t tcount = 0, deps = 0;
   int *d_sup = nullptr, *d_nrows = nullptr, *d_Lp = nullptr, *d_tb = nullptr;
   int *d_cptr = nullptr, *d_cidx = nullptr, *d_dptr = nullptr, *d_dlist = nullptr;

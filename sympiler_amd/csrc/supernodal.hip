@@ -18,9 +18,13 @@
 // products summed left to right and added to a zeroed temporary), and
 // subtracts the temporary from x at those rows, supernode after supernode.
 //
-// Executor: one wavefront per supernode, supernodes taken by ticket in index
-// order (a supernode only reads supernodes before it, so every wait is for a
-// supernode a running wave already holds).  The subtraction into x is pulled
+// Executor: one wavefront per task, tasks taken by ticket in order.  A
+// supernode's first task solves its top and multiplies the first SN_CHUNK
+// rows below; each further SN_CHUNK rows below are a task of their own that
+// reads the solved top from memory, so the rows below a wide supernode are
+// multiplied by several waves.  A task only waits for earlier tasks (the
+// producers of its rows' temporaries, or its supernode's top), which running
+// waves already hold.  The subtraction into x is pulled
 // instead of pushed: each supernode writes its temporary to a buffer T and
 // publishes a tag; a supernode starts by waiting for the supernodes that
 // write to its own rows, then forms x at each own row as b minus their
@@ -44,9 +48,10 @@ struct SnArgs {
   double *T;                    // temporaries: supernode i's at T[tb[i] ..]
   const int *tb;                // nrows_i - w_i values each
   const int *cptr, *cidx;       // row r subtracts T[cidx[cptr[r] .. cptr[r+1])]
-  const int *dptr, *dlist;      // supernodes writing to supernode i's rows
-  int *flags, *state;
-  int m, total_waves;
+  const int2 *task;             // task t: (supernode, chunk of its rows below)
+  const int *dptr, *dlist;      // tasks that task t waits for
+  int *flags, *state;           // one tag per task
+  int ntasks, total_waves;
   uint64_t timeout;
 };
 
@@ -62,6 +67,7 @@ __device__ __forceinline__ int matvec_block(int col, int w) {
 // Rows per lane processed together in the top's updates and the product
 // below (their loads in flight at once).
 constexpr int SN_ROWS = 4;
+static_assert(SN_CHUNK == 64 * SN_ROWS, "a chunk of rows below is one pass of a wave");
 
 __global__ __launch_bounds__(256) void k_supernodal(SnArgs a) {
   __shared__ double slab[4][SN_WMAX];
@@ -69,89 +75,101 @@ __global__ __launch_bounds__(256) void k_supernodal(SnArgs a) {
   const int lane = threadIdx.x & 63;
   const int target = ld_dev(&a.state[ST_EPOCH]) + 1;
   for (;;) {
-    int i = 0;
-    if (lane == 0) i = atomicAdd(&a.state[ST_TICKET], 1);
-    i = __shfl(i, 0);
-    if (i >= a.m) break;
+    int tk = 0;
+    if (lane == 0) tk = atomicAdd(&a.state[ST_TICKET], 1);
+    tk = __shfl(tk, 0);
+    if (tk >= a.ntasks) break;
+    const int2 td = a.task[tk];
+    const int i = __builtin_amdgcn_readfirstlane(td.x);
+    const int chunk = __builtin_amdgcn_readfirstlane(td.y);
     const int s = a.sup[i], w = a.sup[i + 1] - s, nr = a.nrows[i];
     const double *M = a.Lx + a.Lp[i];
-    // 1. the supernodes that write to this one's rows
-    wait_units(a.dptr, a.dlist, i, a.flags, target, a.state, a.timeout);
-    // 2. own rows: b minus the temporaries, in ascending supernode order
-    for (int r = lane; r < w; r += 64) {
-      double v = a.x[s + r];
-      for (int c = a.cptr[s + r], ce = a.cptr[s + r + 1]; c < ce; ++c)
-        v -= ld_dev(&a.T[a.cidx[c]]);
-      xs[r] = v;
-    }
-    wave_lds_sync();
-    // 3. the dense top, column block by column block
-    for (int col = 0; col < w;) {
-      const int B = lsolve_block(col, w);
-      // lane j < B holds row col+j of the block's columns (and its diagonal)
-      double mrow[8];
-#pragma unroll
-      for (int c = 0; c < 8; ++c)
-        mrow[c] = lane < B && c <= lane ? M[(col + lane) + (int64_t)(col + c) * nr] : 1.0;
-      // the first SN_ROWS rows per lane of the update, loaded with the block
-      // (they do not depend on its values)
-      double mk[SN_ROWS][8];
-      const int k0 = col + B + lane;
-#pragma unroll
-      for (int u = 0; u < SN_ROWS; ++u) {
-        const int k = k0 + 64 * u;
-#pragma unroll
-        for (int c = 0; c < 8; ++c)
-          mk[u][c] = k < w && c < B ? M[k + (int64_t)(col + c) * nr] : 0.0;
-      }
-      double xv[8];
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        if (j < B) {
-          double t = xs[col + j];
-#pragma unroll
-          for (int c = 0; c < j; ++c) t = t - xv[c] * mrow[c];
-          const double q = t / mrow[j];
-          xv[j] = __shfl(q, j);
-        } else {
-          xv[j] = 0.0;
-        }
+    // 1. the tasks that write to this task's inputs (for a later chunk of the
+    // rows below: the supernode's own top)
+    wait_units(a.dptr, a.dlist, tk, a.flags, target, a.state, a.timeout);
+    if (chunk > 0) {
+      // a later chunk of the rows below: the solved top from memory
+      for (int r = lane; r < w; r += 64) xs[r] = ld_dev(&a.x[s + r]);
+      wave_lds_sync();
+    } else {
+      // 2. own rows: b minus the temporaries, in ascending supernode order
+      for (int r = lane; r < w; r += 64) {
+        double v = a.x[s + r];
+        for (int c = a.cptr[s + r], ce = a.cptr[s + r + 1]; c < ce; ++c)
+          v -= ld_dev(&a.T[a.cidx[c]]);
+        xs[r] = v;
       }
       wave_lds_sync();
-      if (lane < B) xs[col + lane] = lane == 0 ? xv[0] : lane == 1 ? xv[1] : lane == 2 ? xv[2]
-                                   : lane == 3 ? xv[3] : lane == 4 ? xv[4] : lane == 5 ? xv[5]
-                                   : lane == 6 ? xv[6] : xv[7];
-      // SN_ROWS rows per lane at a time: all their entries in flight at once
-      for (int kc = k0; kc < w; kc += 64 * SN_ROWS) {
-        if (kc != k0) {
+      // 3. the dense top, column block by column block
+      for (int col = 0; col < w;) {
+        const int B = lsolve_block(col, w);
+        // lane j < B holds row col+j of the block's columns (and its diagonal)
+        double mrow[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+          mrow[c] = lane < B && c <= lane ? M[(col + lane) + (int64_t)(col + c) * nr] : 1.0;
+        // the first SN_ROWS rows per lane of the update, loaded with the block
+        // (they do not depend on its values)
+        double mk[SN_ROWS][8];
+        const int k0 = col + B + lane;
+#pragma unroll
+        for (int u = 0; u < SN_ROWS; ++u) {
+          const int k = k0 + 64 * u;
+#pragma unroll
+          for (int c = 0; c < 8; ++c)
+            mk[u][c] = k < w && c < B ? M[k + (int64_t)(col + c) * nr] : 0.0;
+        }
+        double xv[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          if (j < B) {
+            double t = xs[col + j];
+#pragma unroll
+            for (int c = 0; c < j; ++c) t = t - xv[c] * mrow[c];
+            const double q = t / mrow[j];
+            xv[j] = __shfl(q, j);
+          } else {
+            xv[j] = 0.0;
+          }
+        }
+        wave_lds_sync();
+        if (lane < B) xs[col + lane] = lane == 0 ? xv[0] : lane == 1 ? xv[1] : lane == 2 ? xv[2]
+                                     : lane == 3 ? xv[3] : lane == 4 ? xv[4] : lane == 5 ? xv[5]
+                                     : lane == 6 ? xv[6] : xv[7];
+        // SN_ROWS rows per lane at a time: all their entries in flight at once
+        for (int kc = k0; kc < w; kc += 64 * SN_ROWS) {
+          if (kc != k0) {
+#pragma unroll
+            for (int u = 0; u < SN_ROWS; ++u) {
+              const int k = kc + 64 * u;
+#pragma unroll
+              for (int c = 0; c < 8; ++c)
+                mk[u][c] = k < w && c < B ? M[k + (int64_t)(col + c) * nr] : 0.0;
+            }
+          }
 #pragma unroll
           for (int u = 0; u < SN_ROWS; ++u) {
             const int k = kc + 64 * u;
+            if (k < w) {
+              double t = xs[k];
 #pragma unroll
-            for (int c = 0; c < 8; ++c)
-              mk[u][c] = k < w && c < B ? M[k + (int64_t)(col + c) * nr] : 0.0;
+              for (int c = 0; c < 8; ++c)
+                if (c < B) t = t - xv[c] * mk[u][c];
+              xs[k] = t;
+            }
           }
         }
-#pragma unroll
-        for (int u = 0; u < SN_ROWS; ++u) {
-          const int k = kc + 64 * u;
-          if (k < w) {
-            double t = xs[k];
-#pragma unroll
-            for (int c = 0; c < 8; ++c)
-              if (c < B) t = t - xv[c] * mk[u][c];
-            xs[k] = t;
-          }
-        }
+        wave_lds_sync();
+        col += B;
       }
-      wave_lds_sync();
-      col += B;
+      for (int r = lane; r < w; r += 64) st_dev(&a.x[s + r], xs[r]);
     }
-    // 4. the rows below: products with the solved top into T
+    // 4. this task's chunk of the rows below: products with the solved top
+    // into T; SN_ROWS rows per lane, each column block's entries of those
+    // rows loaded together, each row's sum kept in the reference's order
     const int tb = a.tb[i];
-    // SN_ROWS rows per lane at a time; each column block's entries of those
-    // rows are loaded together, each row's sum kept in the reference's order
-    for (int k0 = w + lane; k0 < nr; k0 += 64 * SN_ROWS) {
+    {
+      const int k0 = w + chunk * SN_CHUNK + lane;
       double t[SN_ROWS];
 #pragma unroll
       for (int u = 0; u < SN_ROWS; ++u) t[u] = 0.0;
@@ -179,10 +197,9 @@ __global__ __launch_bounds__(256) void k_supernodal(SnArgs a) {
       for (int u = 0; u < SN_ROWS; ++u)
         if (k0 + 64 * u < nr) st_dev(&a.T[tb + k0 + 64 * u - w], t[u]);
     }
-    for (int r = lane; r < w; r += 64) st_dev(&a.x[s + r], xs[r]);
     // 5. publish
     publish_fence();
-    if (lane == 0) st_dev(&a.flags[i], target);
+    if (lane == 0) st_dev(&a.flags[tk], target);
     wave_lds_sync();
   }
   persistent_finish(a.state, a.total_waves);
@@ -221,21 +238,36 @@ int sn_analyze(symamd_handle h, int n, int m, const int *h_sup, const int *h_nro
   for (int r = 0; r < n; ++r) cnt[r + 1] += cnt[r];
   // contributions per row in ascending supernode order; dependency lists
   std::vector<int> cidx(cnt[n] > 0 ? cnt[n] : 1), fill(cnt.begin(), cnt.end() - 1);
+  // tasks: supernode i's top (with the first SN_CHUNK rows below), then one
+  // task per further SN_CHUNK rows below
+  std::vector<int> first(m + 1, 0);
+  for (int i = 0; i < m; ++i) {
+    const int below = h_nrows[i] - (h_sup[i + 1] - h_sup[i]);
+    first[i + 1] = first[i] + std::max(1, (below + SN_CHUNK - 1) / SN_CHUNK);
+  }
+  const int ntasks = first[m];
+  std::vector<int2> task(ntasks > 0 ? ntasks : 1);
+  for (int i = 0; i < m; ++i)
+    for (int t = first[i]; t < first[i + 1]; ++t) task[t] = make_int2(i, t - first[i]);
   std::vector<int> owner(n);
   for (int i = 0; i < m; ++i)
     for (int c = h_sup[i]; c < h_sup[i + 1]; ++c) owner[c] = i;
-  std::vector<std::vector<int>> deps(m);
+  // a top task waits for the tasks writing its rows' temporaries, a later
+  // chunk for its supernode's top
+  std::vector<std::vector<int>> deps(ntasks);
   for (int i = 0; i < m; ++i) {
     const int w = h_sup[i + 1] - h_sup[i];
     for (int k = 0; k < h_nrows[i] - w; ++k) {
       const int row = h_Li[h_Lp[i] + w + k];
       cidx[fill[row]++] = tb[i] + k;
-      std::vector<int> &d = deps[owner[row]];
-      if (d.empty() || d.back() != i) d.push_back(i);
+      std::vector<int> &d = deps[first[owner[row]]];
+      const int producer = first[i] + k / SN_CHUNK;
+      if (d.empty() || d.back() != producer) d.push_back(producer);
     }
+    for (int t = first[i] + 1; t < first[i + 1]; ++t) deps[t].push_back(first[i]);
   }
-  std::vector<int> dptr(m + 1, 0), dlist;
-  for (int j = 0; j < m; ++j) {
+  std::vector<int> dptr(ntasks + 1, 0), dlist;
+  for (int j = 0; j < ntasks; ++j) {
     std::sort(deps[j].begin(), deps[j].end());
     deps[j].erase(std::unique(deps[j].begin(), deps[j].end()), deps[j].end());
     dlist.insert(dlist.end(), deps[j].begin(), deps[j].end());
@@ -245,6 +277,7 @@ int sn_analyze(symamd_handle h, int n, int m, const int *h_sup, const int *h_nro
   q->n = n;
   q->m = m;
   q->tcount = tb[m];
+  q->ntasks = ntasks;
   int rc = SYMAMD_OK;
   auto up = [&](int **dst, const int *src, size_t count) {
     if (rc != SYMAMD_OK) return;
@@ -258,13 +291,14 @@ int sn_analyze(symamd_handle h, int n, int m, const int *h_sup, const int *h_nro
   up(&q->d_tb, tb.data(), m + 1);
   up(&q->d_cptr, cnt.data(), n + 1);
   up(&q->d_cidx, cidx.data(), cnt[n]);
-  up(&q->d_dptr, dptr.data(), m + 1);
+  up(&q->d_dptr, dptr.data(), ntasks + 1);
   up(&q->d_dlist, dlist.data(), dlist.size());
+  up(&q->d_task, reinterpret_cast<const int *>(task.data()), 2 * (size_t)ntasks);
   if (rc == SYMAMD_OK &&
       (hipMalloc(&q->d_T, sizeof(double) * (q->tcount > 0 ? q->tcount : 1)) != hipSuccess ||
-       hipMalloc(&q->d_flags, sizeof(int) * (m > 0 ? m : 1)) != hipSuccess ||
+       hipMalloc(&q->d_flags, sizeof(int) * (ntasks > 0 ? ntasks : 1)) != hipSuccess ||
        hipMalloc(&q->d_state, sizeof(int) * ST_WORDS) != hipSuccess ||
-       hipMemset(q->d_flags, 0, sizeof(int) * (m > 0 ? m : 1)) != hipSuccess ||
+       hipMemset(q->d_flags, 0, sizeof(int) * (ntasks > 0 ? ntasks : 1)) != hipSuccess ||
        hipMemset(q->d_state, 0, sizeof(int) * ST_WORDS) != hipSuccess))
     rc = fail(SYMAMD_ERR_ALLOC, "supernodal plan allocation failed");
   if (rc != SYMAMD_OK) {
@@ -291,9 +325,10 @@ int sn_solve(symamd_handle h, symamd_snplan_s *q, const double *d_Lx, double *d_
   a.cidx = q->d_cidx;
   a.dptr = q->d_dptr;
   a.dlist = q->d_dlist;
+  a.task = reinterpret_cast<const int2 *>(q->d_task);
   a.flags = q->d_flags;
   a.state = q->d_state;
-  a.m = q->m;
+  a.ntasks = q->ntasks;
   a.timeout = h->timeout_ticks();
   int per_cu = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)&k_supernodal, 256,
@@ -301,7 +336,7 @@ int sn_solve(symamd_handle h, symamd_snplan_s *q, const double *d_Lx, double *d_
       per_cu < 1)
     per_cu = 1;
   int64_t grid = (int64_t)per_cu * h->num_cus;
-  const int64_t want = ((int64_t)q->m + 3) / 4;
+  const int64_t want = ((int64_t)q->ntasks + 3) / 4;
   if (want < grid) grid = want;
   if (grid < 1) grid = 1;
   a.total_waves = (int)grid * 4;
@@ -321,8 +356,8 @@ int sn_check(symamd_handle h, symamd_snplan_s *q) {
 
 void sn_destroy(symamd_snplan_s *q) {
   if (!q) return;
-  void *ptrs[] = {q->d_sup,   q->d_nrows, q->d_Lp,    q->d_tb,    q->d_cptr, q->d_cidx,
-                  q->d_dptr,  q->d_dlist, q->d_T,     q->d_flags, q->d_state};
+  void *ptrs[] = {q->d_sup,  q->d_nrows, q->d_Lp, q->d_tb,    q->d_cptr,  q->d_cidx,
+                  q->d_dptr, q->d_dlist, q->d_T,  q->d_flags, q->d_state, q->d_task};
   for (void *x : ptrs)
     if (x) (void)hipFree(x);
   delete q;
