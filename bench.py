@@ -201,8 +201,17 @@ def supernodal_config(torch, sa, W, dev, stream):
     out = {"n": n, "supernodes": len(sup) - 1, "values": len(Lx),
            "widest": int(np.diff(sup).max()), "kernel_ms": round(ms, 4),
            "gbs": round(byts / (ms * 1e-3) / 1e9, 1)}
+    # 32 right-hand sides, column-major (symamd_sn_solve_block)
+    k = 32
+    B0 = torch.from_numpy(W.uniform(8, 1, k * n).reshape(k, n)).to(dev)
+    X = B0.clone()
+    wk, ek = timed_steps(lambda: plan.solve(dLx, X), lambda: X.copy_(B0), 5, 1, stream, torch)
+    plan.check()
+    mk = float(np.mean(ek))
+    out["k32"] = {"kernel_ms": round(mk, 4), "rhs_solves_per_s": round(k / (mk * 1e-3), 1),
+                  "vs_k1_rhs_per_s": round(k * ms / mk, 2)}
     log(f"supernodal tree13: {out}")
-    del plan, dLx, db, x
+    del plan, dLx, db, x, B0, X
     return out
 
 

@@ -348,6 +348,12 @@ typedef struct symamd_snplan_s *symamd_snplan;
 int symamd_sn_analyze(symamd_handle h, int n, int m, const int *h_sup, const int *h_nrows,
                       const int *h_Lp, const int *h_Li, symamd_snplan *plan);
 int symamd_sn_solve(symamd_handle h, symamd_snplan plan, const double *d_Lx, double *d_x);
+/* k right-hand sides at once, column-major (column c at d_X + c*ld, ld >= n;
+ * the layout the reference's multi-RHS supernodal solves use,
+ * parsy/triangularSolve/Triangular_BCSC.cpp:62-121): each column's result is
+ * bit-identical to sptrsv_csc_block on that column. */
+int symamd_sn_solve_block(symamd_handle h, symamd_snplan plan, const double *d_Lx,
+                          double *d_X, int k, int64_t ld);
 int symamd_sn_check(symamd_handle h, symamd_snplan plan);
 int symamd_sn_destroy(symamd_snplan plan);
 

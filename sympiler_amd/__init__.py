@@ -104,6 +104,7 @@ def lib() -> ctypes.CDLL:
             "symamd_ilu0_destroy": [vp],
             "symamd_sn_analyze": [vp, i32, i32, vp, vp, vp, vp, POINTER(vp)],
             "symamd_sn_solve": [vp, vp, vp, vp],
+            "symamd_sn_solve_block": [vp, vp, vp, vp, i32, i64],
             "symamd_sn_check": [vp, vp],
             "symamd_sn_destroy": [vp],
         }
@@ -588,8 +589,18 @@ class SupernodalPlan:
                                        byref(self._p)))
 
     def solve(self, Lx, x):
-        _require_cuda(Lx, x)
-        _check(lib().symamd_sn_solve(self.h._h, self._p, _ptr(Lx), _ptr(x)))
+        """x: (n,) or a column-major block (k, n) (row c = column c of X),
+        overwritten with the solution."""
+        _require_cuda(Lx)
+        if x.dim() == 1:
+            _require_cuda(x)
+            _check(lib().symamd_sn_solve(self.h._h, self._p, _ptr(Lx), _ptr(x)))
+        else:
+            if not x.is_cuda or x.stride(1) != 1 or x.dtype != Lx.dtype:
+                raise ValueError("x: a (k, n) device block with unit stride along n")
+            k, n = x.shape
+            _check(lib().symamd_sn_solve_block(self.h._h, self._p, _ptr(Lx), _ptr(x), k,
+                                               x.stride(0)))
         return x
 
     def check(self):

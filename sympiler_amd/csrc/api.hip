@@ -1030,7 +1030,15 @@ int symamd_sn_solve(symamd_handle h, symamd_snplan plan, const double *d_Lx, dou
   SYMAMD_TRY(check_handle(h));
   if (!plan || (plan->m > 0 && (!d_Lx || !d_x)))
     return fail(SYMAMD_ERR_INVALID_ARG, "symamd_sn_solve: bad arguments");
-  return sn_solve(h, plan, d_Lx, d_x);
+  return sn_solve(h, plan, d_Lx, d_x, 1, 0);
+}
+
+int symamd_sn_solve_block(symamd_handle h, symamd_snplan plan, const double *d_Lx,
+                          double *d_X, int k, int64_t ld) {
+  SYMAMD_TRY(check_handle(h));
+  if (!plan || k < 0 || (k > 1 && ld < plan->n) || (plan->m > 0 && k > 0 && (!d_Lx || !d_X)))
+    return fail(SYMAMD_ERR_INVALID_ARG, "symamd_sn_solve_block: bad arguments");
+  return sn_solve(h, plan, d_Lx, d_X, k, ld);
 }
 
 int symamd_sn_check(symamd_handle h, symamd_snplan plan) {

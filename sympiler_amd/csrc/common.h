@@ -342,7 +342,7 @@ struct symamd_ilu0_s {
 constexpr int SN_WMAX = 1024;  // widest supernode (LDS of one wave: 8 KiB)
 constexpr int SN_CHUNK = 256;  // rows below a top per task
 struct symamd_snplan_s {
-  int n = 0, m = 0, ntasks = 0;
+  int n = 0, m = 0, ntasks = 0, kcap = 1;  // kcap: columns d_T and d_flags hold
   int *d_task = nullptr;       // (supernode, chunk) per task
   int64_t tcount = 0, deps = 0;
   int *d_sup = nullptr, *d_nrows = nullptr, *d_Lp = nullptr, *d_tb = nullptr;
@@ -470,7 +470,8 @@ void ilu0_destroy(symamd_ilu0_s *q);
 // asynchronous solve, watchdog check, destroy.
 int sn_analyze(symamd_handle h, int n, int m, const int *h_sup, const int *h_nrows,
                const int *h_Lp, const int *h_Li, symamd_snplan_s **out);
-int sn_solve(symamd_handle h, symamd_snplan_s *q, const double *d_Lx, double *d_x);
+int sn_solve(symamd_handle h, symamd_snplan_s *q, const double *d_Lx, double *d_x, int k,
+             int64_t ldx);
 int sn_check(symamd_handle h, symamd_snplan_s *q);
 void sn_destroy(symamd_snplan_s *q);
 int ilu_lower(symamd_handle h, int n, const int *d_Ap, const int *d_Ai,

@@ -50,8 +50,10 @@ struct SnArgs {
   const int *cptr, *cidx;       // row r subtracts T[cidx[cptr[r] .. cptr[r+1])]
   const int2 *task;             // task t: (supernode, chunk of its rows below)
   const int *dptr, *dlist;      // tasks that task t waits for
-  int *flags, *state;           // one tag per task
+  int *flags, *state;           // one tag per (column, task)
   int ntasks, total_waves;
+  int k;                        // right-hand sides: column c at x + c*ldx
+  int64_t ldx, tstride;         // its temporaries at T + c*tstride
   uint64_t timeout;
 };
 
@@ -75,10 +77,17 @@ __global__ __launch_bounds__(256) void k_supernodal(SnArgs a) {
   const int lane = threadIdx.x & 63;
   const int target = ld_dev(&a.state[ST_EPOCH]) + 1;
   for (;;) {
-    int tk = 0;
-    if (lane == 0) tk = atomicAdd(&a.state[ST_TICKET], 1);
-    tk = __shfl(tk, 0);
-    if (tk >= a.ntasks) break;
+    // ticket -> (task, column), task-major: the k columns of one task run
+    // side by side (sharing its block in the caches) and every wait is for
+    // an earlier ticket
+    int tt = 0;
+    if (lane == 0) tt = atomicAdd(&a.state[ST_TICKET], 1);
+    tt = __shfl(tt, 0);
+    if (tt >= a.ntasks * a.k) break;
+    const int tk = tt / a.k, colk = tt - tk * a.k;
+    double *const x = a.x + colk * a.ldx;
+    double *const T = a.T + colk * a.tstride;
+    int *const flags = a.flags + (int64_t)colk * a.ntasks;
     const int2 td = a.task[tk];
     const int i = __builtin_amdgcn_readfirstlane(td.x);
     const int chunk = __builtin_amdgcn_readfirstlane(td.y);
@@ -86,17 +95,17 @@ __global__ __launch_bounds__(256) void k_supernodal(SnArgs a) {
     const double *M = a.Lx + a.Lp[i];
     // 1. the tasks that write to this task's inputs (for a later chunk of the
     // rows below: the supernode's own top)
-    wait_units(a.dptr, a.dlist, tk, a.flags, target, a.state, a.timeout);
+    wait_units(a.dptr, a.dlist, tk, flags, target, a.state, a.timeout);
     if (chunk > 0) {
       // a later chunk of the rows below: the solved top from memory
-      for (int r = lane; r < w; r += 64) xs[r] = ld_dev(&a.x[s + r]);
+      for (int r = lane; r < w; r += 64) xs[r] = ld_dev(&x[s + r]);
       wave_lds_sync();
     } else {
       // 2. own rows: b minus the temporaries, in ascending supernode order
       for (int r = lane; r < w; r += 64) {
-        double v = a.x[s + r];
+        double v = x[s + r];
         for (int c = a.cptr[s + r], ce = a.cptr[s + r + 1]; c < ce; ++c)
-          v -= ld_dev(&a.T[a.cidx[c]]);
+          v -= ld_dev(&T[a.cidx[c]]);
         xs[r] = v;
       }
       wave_lds_sync();
@@ -162,7 +171,7 @@ __global__ __launch_bounds__(256) void k_supernodal(SnArgs a) {
         wave_lds_sync();
         col += B;
       }
-      for (int r = lane; r < w; r += 64) st_dev(&a.x[s + r], xs[r]);
+      for (int r = lane; r < w; r += 64) st_dev(&x[s + r], xs[r]);
     }
     // 4. this task's chunk of the rows below: products with the solved top
     // into T; SN_ROWS rows per lane, each column block's entries of those
@@ -195,11 +204,11 @@ __global__ __launch_bounds__(256) void k_supernodal(SnArgs a) {
       }
 #pragma unroll
       for (int u = 0; u < SN_ROWS; ++u)
-        if (k0 + 64 * u < nr) st_dev(&a.T[tb + k0 + 64 * u - w], t[u]);
+        if (k0 + 64 * u < nr) st_dev(&T[tb + k0 + 64 * u - w], t[u]);
     }
     // 5. publish
     publish_fence();
-    if (lane == 0) st_dev(&a.flags[tk], target);
+    if (lane == 0) st_dev(&flags[tk], target);
     wave_lds_sync();
   }
   persistent_finish(a.state, a.total_waves);
@@ -311,8 +320,29 @@ int sn_analyze(symamd_handle h, int n, int m, const int *h_sup, const int *h_nro
   return SYMAMD_OK;
 }
 
-int sn_solve(symamd_handle h, symamd_snplan_s *q, const double *d_Lx, double *d_x) {
-  if (q->m == 0) return SYMAMD_OK;
+int sn_solve(symamd_handle h, symamd_snplan_s *q, const double *d_Lx, double *d_x, int k,
+             int64_t ldx) {
+  if (q->m == 0 || k == 0) return SYMAMD_OK;
+  // temporaries and tags for k columns
+  if (k > q->kcap) {
+    double *T = nullptr;
+    int *flags = nullptr;
+    const int64_t tc = q->tcount > 0 ? q->tcount : 1;
+    const int64_t nt = q->ntasks > 0 ? q->ntasks : 1;
+    if (hipMalloc(&T, sizeof(double) * tc * k) != hipSuccess ||
+        hipMalloc(&flags, sizeof(int) * nt * k) != hipSuccess) {
+      if (T) (void)hipFree(T);
+      return fail(SYMAMD_ERR_ALLOC, "supernodal temporaries for %d columns", k);
+    }
+    // cleared tags read as "not ready" in every epoch
+    SYMAMD_HIP(hipMemsetAsync(flags, 0, sizeof(int) * nt * k, h->stream));
+    SYMAMD_HIP(hipStreamSynchronize(h->stream));
+    if (q->d_T) (void)hipFree(q->d_T);
+    if (q->d_flags) (void)hipFree(q->d_flags);
+    q->d_T = T;
+    q->d_flags = flags;
+    q->kcap = k;
+  }
   SnArgs a;
   a.sup = q->d_sup;
   a.nrows = q->d_nrows;
@@ -329,6 +359,9 @@ int sn_solve(symamd_handle h, symamd_snplan_s *q, const double *d_Lx, double *d_
   a.flags = q->d_flags;
   a.state = q->d_state;
   a.ntasks = q->ntasks;
+  a.k = k;
+  a.ldx = ldx;
+  a.tstride = q->tcount > 0 ? q->tcount : 1;
   a.timeout = h->timeout_ticks();
   int per_cu = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)&k_supernodal, 256,
@@ -336,11 +369,11 @@ int sn_solve(symamd_handle h, symamd_snplan_s *q, const double *d_Lx, double *d_
       per_cu < 1)
     per_cu = 1;
   int64_t grid = (int64_t)per_cu * h->num_cus;
-  const int64_t want = ((int64_t)q->ntasks + 3) / 4;
+  const int64_t want = ((int64_t)q->ntasks * k + 3) / 4;
   if (want < grid) grid = want;
   if (grid < 1) grid = 1;
   a.total_waves = (int)grid * 4;
-  // tickets are dealt in supernode order, so any grid size is safe
+  // tickets are dealt in task order, so any grid size is safe
   k_supernodal<<<(unsigned)grid, 256, 0, h->stream>>>(a);
   SYMAMD_HIP(hipGetLastError());
   return SYMAMD_OK;

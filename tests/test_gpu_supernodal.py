@@ -35,6 +35,30 @@ def test_supernodal_bitwise(dev, case):
         assert bitwise_equal(x.cpu().numpy(), want)
 
 
+@pytest.mark.parametrize("k", [3, 32])
+def test_supernodal_block_columns_bitwise(dev, k):
+    n = 3000
+    sup, nrows, Lp, Li, Lx = W.bcsc(n, 100, 70, 1500, 3)
+    B = W.uniform(11, 0, k * (n + 7)).reshape(k, n + 7)  # column-major, ld = n + 7
+    plan = sa.SupernodalPlan(sup, nrows, Lp, Li)
+    dLx = to_dev(Lx)
+    for _ in range(2):
+        X = to_dev(B.copy())
+        plan.solve(dLx, X[:, :n])
+        plan.check()
+        got = X.cpu().numpy()
+        for c in range(k):
+            want = oracle.sptrsv_csc_block(sup, nrows, Lp, Li, Lx, B[c, :n])
+            assert bitwise_equal(got[c, :n], want), c
+        assert bitwise_equal(got[:, n:], B[:, n:])  # the padding is untouched
+    # the single-column solve after a block one (tags of every column reused)
+    x = to_dev(B[0, :n].copy())
+    plan.solve(dLx, x)
+    plan.check()
+    assert bitwise_equal(x.cpu().numpy(), oracle.sptrsv_csc_block(sup, nrows, Lp, Li, Lx,
+                                                                  B[0, :n]))
+
+
 def test_supernodal_structure_errors(dev):
     sup, nrows, Lp, Li, Lx = W.bcsc(300, 8, 8, 60, 1)
     i = int(np.argmax(nrows - np.diff(sup)))  # a supernode with rows below
