@@ -233,35 +233,53 @@ __global__ void k_collen(int n, const int *__restrict__ perm,
 // the counts into uoff, to write the unit starts (ub).
 // With a region order (preg != null) a unit also ends where the region of
 // the positions changes, so every unit lies inside one region.
+// One wavefront scans a level: the lengths (and regions) of 64 positions are
+// loaded together, then the greedy cut runs over them on uniform values.
 __device__ __forceinline__ int unit_scan(const int *__restrict__ rp,
                                          const int *__restrict__ preg, int q0,
                                          int q1, int rows, int emax,
                                          int *ub_out) {
-  int cnt = 0, in_rows = 0, ents = 0;
-  for (int q = q0; q < q1; ++q) {
-    const int len = rp[q + 1] - rp[q];
-    if (in_rows == rows || (in_rows > 0 && ents + len > emax) ||
-        (in_rows > 0 && preg && preg[q] != preg[q - 1])) {
-      in_rows = 0;
-      ents = 0;
+  const int lane = threadIdx.x & 63;
+  int cnt = 0, in_rows = 0, ents = 0, prev = 0;
+  for (int b = q0; b < q1; b += 64) {
+    const int q = b + lane;
+    int len = 0, reg = 0;
+    if (q < q1) {
+      len = rp[q + 1] - rp[q];
+      reg = preg ? preg[q] : 0;
     }
-    if (in_rows == 0) {
-      if (ub_out) ub_out[cnt] = q;
-      ++cnt;
+    const int m = q1 - b < 64 ? q1 - b : 64;
+    for (int j = 0; j < m; ++j) {
+      const int lj = __builtin_amdgcn_readlane(len, j);
+      const int rj = __builtin_amdgcn_readlane(reg, j);
+      if (in_rows == rows || (in_rows > 0 && ents + lj > emax) ||
+          (in_rows > 0 && preg && rj != prev)) {
+        in_rows = 0;
+        ents = 0;
+      }
+      if (in_rows == 0) {
+        if (ub_out && lane == 0) ub_out[cnt] = b + j;
+        ++cnt;
+      }
+      ++in_rows;
+      ents += lj;
+      prev = rj;
     }
-    ++in_rows;
-    ents += len;
   }
   return cnt;
 }
 
+// One wavefront per level (wave nlev writes the trailing zero).
 __global__ void k_units_per_level(int nlev, const int *__restrict__ lp,
                                   const int *__restrict__ rp,
                                   const int *__restrict__ preg, int rows,
                                   int emax, int *upl) {
-  int l = blockIdx.x * blockDim.x + threadIdx.x;
-  if (l < nlev) upl[l] = unit_scan(rp, preg, lp[l], lp[l + 1], rows, emax, nullptr);
-  if (l == nlev) upl[l] = 0;
+  const int l = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  if (l < nlev) {
+    const int c = unit_scan(rp, preg, lp[l], lp[l + 1], rows, emax, nullptr);
+    if ((threadIdx.x & 63) == 0) upl[l] = c;
+  }
+  if (l == nlev && (threadIdx.x & 63) == 0) upl[l] = 0;
 }
 
 __global__ void k_unit_bounds(int nlev, int U, int n, const int *__restrict__ lp,
@@ -269,9 +287,9 @@ __global__ void k_unit_bounds(int nlev, int U, int n, const int *__restrict__ lp
                               const int *__restrict__ preg,
                               const int *__restrict__ uoff, int rows, int emax,
                               int *ub) {
-  int l = blockIdx.x * blockDim.x + threadIdx.x;
+  const int l = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   if (l < nlev) unit_scan(rp, preg, lp[l], lp[l + 1], rows, emax, ub + uoff[l]);
-  if (l == nlev) ub[U] = n;
+  if (l == nlev && (threadIdx.x & 63) == 0) ub[U] = n;
 }
 
 inline int blocks_for(int64_t m, int bs = 256) {
@@ -669,7 +687,7 @@ int plan_units(symamd_handle h, symamd_plan p, int rows, bool adjoint,
     int *d_upl = nullptr, *d_uoff = nullptr;
     SYMAMD_HIP(hipMallocAsync(&d_upl, sizeof(int) * (nlev + 1), h->stream));
     SYMAMD_HIP(hipMallocAsync(&d_uoff, sizeof(int) * (nlev + 1), h->stream));
-    k_units_per_level<<<blocks_for(nlev + 1), 256, 0, h->stream>>>(
+    k_units_per_level<<<blocks_for(nlev + 1, 4), 256, 0, h->stream>>>(
         nlev, p->d_lvl_ptr, rp, p->d_preg, rows, UNIT_EMAX, d_upl);
     SYMAMD_HIP(hipGetLastError());
     SYMAMD_TRY(scan_exclusive(h, d_upl, d_uoff, nlev + 1));
@@ -679,7 +697,7 @@ int plan_units(symamd_handle h, symamd_plan p, int rows, bool adjoint,
     const int U = us.h_uoff[nlev];
     us.count = U;
     SYMAMD_HIP(hipMalloc(&us.d_ub, sizeof(int) * (U + 1)));
-    k_unit_bounds<<<blocks_for(nlev + 1), 256, 0, h->stream>>>(
+    k_unit_bounds<<<blocks_for(nlev + 1, 4), 256, 0, h->stream>>>(
         nlev, U, p->n, p->d_lvl_ptr, rp, p->d_preg, d_uoff, rows, UNIT_EMAX, us.d_ub);
     SYMAMD_HIP(hipGetLastError());
     SYMAMD_HIP(hipFreeAsync(d_upl, h->stream));
