@@ -81,10 +81,12 @@ Choice heuristic(symamd_plan p, int k) {
   return c;
 }
 
-// Longest row of L (atomicMax into *out, zero on entry).
+// Longest row of L (zero on entry): the wave's maximum, one atomic per wave.
 __global__ void k_max_row(int n, const int *__restrict__ Lp, int *out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) atomicMax(out, Lp[i + 1] - Lp[i]);
+  int v = i < n ? Lp[i + 1] - Lp[i] : 0;
+  for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off));
+  if ((threadIdx.x & 63) == 0) atomicMax(out, v);
 }
 
 // forward: a forward solve (the region executor is a candidate)
