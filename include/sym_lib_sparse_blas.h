@@ -16,6 +16,8 @@
 #ifndef SYM_LIB_SPARSE_BLAS_H
 #define SYM_LIB_SPARSE_BLAS_H
 #ifdef __cplusplus
+#include <cstddef>
+
 namespace sym_lib {
 
 /* sparse_blas_lib.h:18-20 */
@@ -116,6 +118,18 @@ void sptrsv_csc_block(int n, const int *Lp, const int *Li, const int *nrows, dou
  * the caller to delete[], returns the number of levels. */
 int build_levelSet_CSC(int n, const int *Lp, const int *Li, int *&level_ptr,
                        int *&level_set);
+
+namespace parsy {
+/* parsy/symbolic/InspectionLevel_06.h:13-17: tree-level coarsening of an
+ * elimination tree (host only, on symamd_tree_coarsen); the three arrays are
+ * new[]'d for the caller to delete[] and equal the reference's.  Throws
+ * std::invalid_argument on a cycle, a parent index >= n, a NaN cost or a
+ * divRate that the reference would never return from. */
+int getCoarseLevelSet_6(size_t n, const int *eTree, const int *blk2col, int &finaLevelNo,
+                        int *&finaLevelPtr, int *&parLevelSet, int &partNo,
+                        int *&finalPartPtr, int *&finalNodePtr, int innerParts,
+                        int minLevelDist, int divRate, double *nodeCost);
+}  // namespace parsy
 
 }  // namespace sym_lib
 #endif

@@ -148,6 +148,25 @@ int symamd_coarsen(symamd_handle h, int n, const int *d_Lp, const int *d_Li,
                    int *d_par_ptr, int *d_partition, int *level_no,
                    int *part_no);
 
+/* Load-balanced level coarsening of an elimination tree (host arrays; no
+ * device work): `parent[i]` is node i's parent, negative for a root;
+ * `node_cost[i]` its cost.  Writes the same 3-array schedule as
+ * symamd_coarsen -- level_ptr (n+1 ints), part_ptr (n+1) and partition (n)
+ * -- with inside each part every node after its children, and the counts of
+ * coarse levels and parts.  The arrays equal those of
+ * sym_lib::parsy::getCoarseLevelSet_6(n, parent, -, ..., inner_parts,
+ * min_level_dist, div_rate, node_cost)
+ * (sparse_blas/parsy/symbolic/InspectionLevel_06.cpp:17; callers
+ * symbolic_phase.cpp:692 on the supernodal tree and :787 on the column
+ * tree).  SYMAMD_ERR_STRUCTURE if parent has a cycle; SYMAMD_ERR_INVALID_ARG
+ * for a parent index >= n or equal to its node, a NaN cost, or div_rate 0
+ * or 1 with inner_parts > 1 on a tree of more than two levels (input the
+ * reference never returns from).  n = 0 gives no levels. */
+int symamd_tree_coarsen(int n, const int *parent, const double *node_cost,
+                        int inner_parts, int min_level_dist, int div_rate,
+                        int *level_ptr, int *part_ptr, int *partition,
+                        int *level_no, int *part_no);
+
 /* Execute a caller-given coarsened schedule on a device CSR of L (FP64), in
  * place on d_x, with the semantics of sym_lib::sptrsv_csr_lbc
  * (sparse_blas/sptrsv.cpp:57-75): coarse levels in order, partitions of a

@@ -115,6 +115,7 @@ def rlib():
             "ref_spic0_csc_left": ([i, vp, vp, vp, vp, vp, vp], c_int),
             "ref_spic0_csr": ([i, vp, vp, vp], None),
             "ref_getLevelSet": ([i, vp, vp, vp], c_int),
+            "ref_getCoarseLevelSet_6": ([i, vp, vp, i, i, i, vp, vp, vp, vp, vp], None),
             "ref_set_threads": ([i], None),
         }
         for name, (args, res) in sigs.items():
@@ -436,3 +437,21 @@ class ref:
         ls = np.zeros(max(n, 1), np.int32)
         nl = rlib().ref_getLevelSet(n, _p(_i32(parent)), _p(lp), _p(ls))
         return lp[: nl + 1], ls[:n]
+
+    @staticmethod
+    def getCoarseLevelSet_6(parent, cost, inner_parts, min_level_dist, div_rate):
+        """(level_ptr, part_ptr, partition) of the reference's tree-level
+        coarsening (parsy/symbolic/InspectionLevel_06.cpp:17).  The reference
+        build keeps its assertions, one of which compares an integer sum of
+        the costs with the float one: give integer-valued costs."""
+        n = len(parent)
+        lp = np.zeros(n + 1, np.int32)
+        pp = np.zeros(n + 1, np.int32)
+        pt = np.zeros(max(n, 1), np.int32)
+        nl, npart = ctypes.c_int(0), ctypes.c_int(0)
+        c = _f64(cost).copy()
+        rlib().ref_getCoarseLevelSet_6(n, _p(_i32(parent)), _p(c), int(inner_parts),
+                                       int(min_level_dist), int(div_rate), _p(lp),
+                                       _p(pp), _p(pt), ctypes.byref(nl),
+                                       ctypes.byref(npart))
+        return lp[: nl.value + 1], pp[: npart.value + 1], pt[:n]

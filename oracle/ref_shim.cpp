@@ -49,6 +49,12 @@ void spic0_csr(int n, double *val, int *rowPtr, int *rowIdx);  // :242
 namespace parsy {
 // include/sympiler/parsy/common/TreeUtils.h (impl TreeUtils.cpp:89-141)
 int getLevelSet(size_t n, const int *inTree, int *levelPtr, int *levelSet);
+// include/sympiler/parsy/symbolic/InspectionLevel_06.h:13-17
+int getCoarseLevelSet_6(size_t n, const int *eTree, const int *blk2col,
+                        int &finaLevelNo, int *&finaLevelPtr, int *&parLevelSet,
+                        int &partNo, int *&finalPartPtr, int *&finalNodePtr,
+                        int innerParts, int minLevelDist, int divRate,
+                        double *nodeCost);
 }  // namespace parsy
 }  // namespace sym_lib
 
@@ -120,5 +126,24 @@ void ref_spic0_csr(int n, double *val, int *rowPtr, int *rowIdx) {
 }
 int ref_getLevelSet(int n, const int *inTree, int *levelPtr, int *levelSet) {
   return sym_lib::parsy::getLevelSet((size_t)n, inTree, levelPtr, levelSet);
+}
+// Copies the three new[]'d arrays out (sizes n+1, n+1, n) and frees them.
+void ref_getCoarseLevelSet_6(int n, const int *eTree, double *nodeCost,
+                             int innerParts, int minLevelDist, int divRate,
+                             int *levelPtr, int *partPtr, int *partition,
+                             int *levelNo, int *partNo) {
+  int *lp = nullptr, *unused = nullptr, *pp = nullptr, *pt = nullptr;
+  int nl = 0, np = 0;
+  sym_lib::parsy::getCoarseLevelSet_6((size_t)n, eTree, nullptr, nl, lp, unused,
+                                      np, pp, pt, innerParts, minLevelDist,
+                                      divRate, nodeCost);
+  for (int i = 0; i <= nl; ++i) levelPtr[i] = lp[i];
+  for (int i = 0; i <= np; ++i) partPtr[i] = pp[i];
+  for (int i = 0; i < pp[np]; ++i) partition[i] = pt[i];
+  *levelNo = nl;
+  *partNo = np;
+  delete[] lp;
+  delete[] pp;
+  delete[] pt;
 }
 }

@@ -72,6 +72,8 @@ def lib() -> ctypes.CDLL:
             "symamd_levelset": [vp, i32, vp, vp, vp, vp, POINTER(i32)],
             "symamd_coarsen": [vp, i32, vp, vp, i32, i32, vp, vp, vp,
                                POINTER(i32), POINTER(i32)],
+            "symamd_tree_coarsen": [i32, vp, vp, i32, i32, i32, vp, vp, vp,
+                                    POINTER(i32), POINTER(i32)],
             "symamd_solve_lbc": [vp, i32, vp, vp, vp, vp, i32, vp, vp, vp],
             "symamd_solve_lbc_redundant": [vp, i32, vp, vp, vp, vp, vp, i32, vp, vp, vp],
             "symamd_ilu_lower": [vp, i32, vp, vp, vp, vp, i32, vp, vp, vp,
@@ -453,6 +455,29 @@ def coarsen(Lp, Li, span=4, max_part_cost=4000, handle: Handle | None = None):
     _check(lib().symamd_coarsen(h._h, n, _ptr(Lp), _ptr(Li), int(span),
                                 int(max_part_cost), _ptr(lp), _ptr(pp),
                                 _ptr(part), byref(lno), byref(pno)))
+    return lp[: lno.value + 1], pp[: pno.value + 1], part[:n]
+
+
+def tree_coarsen(parent, node_cost, inner_parts, min_level_dist, div_rate):
+    """Tree-level coarsening of an elimination tree (host arrays, no device
+    work): parent[i] < 0 marks a root.  Returns numpy int32 arrays
+    (level_ptr, part_ptr, partition) equal to those of
+    sym_lib::parsy::getCoarseLevelSet_6 (parsy/symbolic/InspectionLevel_06.cpp:17)."""
+    import numpy as np
+    parent = np.ascontiguousarray(parent, dtype=np.int32)
+    cost = np.ascontiguousarray(node_cost, dtype=np.float64)
+    n = parent.size
+    if cost.size != n:
+        raise ValueError("node_cost must have one entry per node")
+    lp = np.zeros(n + 1, np.int32)
+    pp = np.zeros(n + 1, np.int32)
+    part = np.zeros(max(n, 1), np.int32)
+    lno, pno = c_int(), c_int()
+    _check(lib().symamd_tree_coarsen(
+        n, parent.ctypes.data_as(c_void_p), cost.ctypes.data_as(c_void_p),
+        int(inner_parts), int(min_level_dist), int(div_rate),
+        lp.ctypes.data_as(c_void_p), pp.ctypes.data_as(c_void_p),
+        part.ctypes.data_as(c_void_p), byref(lno), byref(pno)))
     return lp[: lno.value + 1], pp[: pno.value + 1], part[:n]
 
 

@@ -483,4 +483,32 @@ int build_levelSet_CSC(int n, const int *Lp, const int *Li, int *&level_ptr,
   return levels;
 }
 
+namespace parsy {
+
+// Host-only: the coarsening never touches the device.  The arrays are
+// allocated with new[] as the reference's are (the caller deletes them);
+// parLevelSet and blk2col are not used by the reference either.
+int getCoarseLevelSet_6(size_t n, const int *eTree, const int *blk2col, int &finaLevelNo,
+                        int *&finaLevelPtr, int *&parLevelSet, int &partNo,
+                        int *&finalPartPtr, int *&finalNodePtr, int innerParts,
+                        int minLevelDist, int divRate, double *nodeCost) {
+  (void)blk2col;
+  (void)parLevelSet;
+  if (n > (size_t)0x7ffffffe) throw std::invalid_argument("getCoarseLevelSet_6: n too large");
+  const int nn = (int)n;
+  finaLevelPtr = new int[nn + 1];
+  finalPartPtr = new int[nn + 1];
+  finalNodePtr = new int[nn > 0 ? nn : 1];
+  const int rc = symamd_tree_coarsen(nn, eTree, nodeCost, innerParts, minLevelDist, divRate,
+                                     finaLevelPtr, finalPartPtr, finalNodePtr, &finaLevelNo,
+                                     &partNo);
+  if (rc == SYMAMD_ERR_STRUCTURE)
+    throw std::invalid_argument("getCoarseLevelSet_6: eTree has a cycle");
+  if (rc != SYMAMD_OK)
+    throw std::invalid_argument(
+        "getCoarseLevelSet_6: bad parent index or NaN cost, or divRate < 2 with innerParts > 1");
+  return finaLevelNo;
+}
+
+}  // namespace parsy
 }  // namespace sym_lib

@@ -113,3 +113,45 @@ def lbc_schedules(lp, ls):
     coarse = (np.arange(len(cl), dtype=np.int32),
               np.array([lp[c] for c in cl], np.int32), np.asarray(ls, np.int32))
     return {"per_row": per_row, "coarse3": coarse}
+
+
+def elimination_tree(Lp, Li):
+    """Elimination tree of the symmetric matrix whose lower triangle is the
+    CSR (Lp, Li): parent[j] = the smallest i > j connected to j through
+    rows < i (Liu's algorithm with path compression).  Every entry L[i, j]
+    (i > j) has i among j's ancestors, so a schedule valid for the tree is
+    valid for the solve with L."""
+    n = len(Lp) - 1
+    parent = np.full(n, -1, np.int32)
+    anc = np.full(n, -1, np.int64)
+    for i in range(n):
+        for j in Li[Lp[i]:Lp[i + 1]]:
+            j = int(j)
+            while j != -1 and j < i:
+                nxt = int(anc[j])
+                anc[j] = i
+                if nxt == -1:
+                    parent[j] = i
+                j = nxt
+    return parent
+
+
+def postorder_binary_tree(levels):
+    """Parent array of a complete binary tree of 2**levels - 1 nodes
+    numbered in postorder (children before parents), the supernodal tree
+    shape of sympiler_amd.workloads.bcsc_tree."""
+    parent = []
+
+    def build(depth):
+        if depth == 1:
+            parent.append(-1)
+            return len(parent) - 1
+        a = build(depth - 1)
+        b = build(depth - 1)
+        parent.append(-1)
+        me = len(parent) - 1
+        parent[a] = parent[b] = me
+        return me
+
+    build(levels)
+    return np.asarray(parent, np.int32)

@@ -63,6 +63,7 @@ def test_sym_lib_dropins_exported_with_reference_mangling():
         "_ZN7sym_lib10spilu0_csrEiiPKiS1_PKdS1_Pd",
         "_ZN7sym_lib14spilu0_csr_lbcEiiPKiS1_PKdS1_PdiS1_S1_S1_Pi",
         "_ZN7sym_lib16sptrsv_csc_blockEiPKiS1_S1_PdS2_iS1_",
+        "_ZN7sym_lib5parsy19getCoarseLevelSet_6EmPKiS2_RiRPiS5_S3_S5_S5_iiiPd",
     }
     missing = want - exported(LIB)
     assert not missing, missing
@@ -86,6 +87,10 @@ def test_mangled_names_match_the_reference_objects():
 
     ref = {s for s in exported(oracle.REF_PATH) if fname(s) in on_path}
     assert len(ref) == 18, sorted(ref)  # spmv_csr has two overloads
+    # the tree-level coarsening of parsy (InspectionLevel_06.cpp:17)
+    ref |= {s for s in exported(oracle.REF_PATH)
+            if s.startswith("_ZN7sym_lib5parsy19getCoarseLevelSet_6")}
+    assert len(ref) == 19, sorted(ref)
     missing = ref - exported(LIB)
     assert not missing, missing
 
