@@ -129,6 +129,13 @@ int getCoarseLevelSet_6(size_t n, const int *eTree, const int *blk2col, int &fin
                         int *&finaLevelPtr, int *&parLevelSet, int &partNo,
                         int *&finalPartPtr, int *&finalNodePtr, int innerParts,
                         int minLevelDist, int divRate, double *nodeCost);
+/* parsy/triangularSolve/Triangular_BCSC.h:18-19: forward solve on parsy's
+ * supernodal storage (per-column size_t value and row-list offsets), on the
+ * device through sptrsv_csc_block; x bit-identical to the reference's
+ * (its dense kernels order every operation as lsolve_BLAS/matvec_BLAS).
+ * Returns 1, or 0 for a null Lp, Li or x as the reference does. */
+int blockedLsolve(int n, size_t *Lp, int *Li, double *Lx, int NNZ, size_t *Li_ptr,
+                  int *col2sup, int *sup2col, int supNo, double *x);
 }  // namespace parsy
 
 }  // namespace sym_lib

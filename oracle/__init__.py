@@ -116,6 +116,10 @@ def rlib():
             "ref_spic0_csr": ([i, vp, vp, vp], None),
             "ref_getLevelSet": ([i, vp, vp, vp], c_int),
             "ref_getCoarseLevelSet_6": ([i, vp, vp, i, i, i, vp, vp, vp, vp, vp], None),
+            "ref_lsolve_BLAS": ([i, i, vp, vp], None),
+            "ref_matvec_BLAS": ([i, i, i, vp, vp, vp], None),
+            "ref_parsy_dlsolve": ([i, i, vp, vp], None),
+            "ref_parsy_dmatvec": ([i, i, i, vp, vp, vp], None),
             "ref_set_threads": ([i], None),
         }
         for name, (args, res) in sigs.items():
@@ -437,6 +441,28 @@ class ref:
         ls = np.zeros(max(n, 1), np.int32)
         nl = rlib().ref_getLevelSet(n, _p(_i32(parent)), _p(lp), _p(ls))
         return lp[: nl + 1], ls[:n]
+
+    @staticmethod
+    def dense_kernels(M, ldm, ncol, nrow, rhs, vec, out, parsy):
+        """The dense kernels of the supernodal solves on one column-major
+        block M (leading dimension ldm): the triangular solve of its top
+        ncol x ncol part on a copy of rhs, and the product of the nrow x ncol
+        part at M[ncol:] with vec into a copy of out.  parsy=False runs
+        sym_lib's lsolve_BLAS/matvec_BLAS (dense_blas/BLAS.cpp:3, :286),
+        parsy=True parsy's dlsolve_blas_nonUnit/dmatvec_blas
+        (parsy/triangularSolve/BLAS.cpp:12, :123)."""
+        M = _f64(M).copy()
+        r = _f64(rhs).copy()
+        o = _f64(out).copy()
+        v = _f64(vec).copy()
+        L = rlib()
+        if parsy:
+            L.ref_parsy_dlsolve(ldm, ncol, _p(M), _p(r))
+            L.ref_parsy_dmatvec(ldm, nrow, ncol, _p(M[ncol:]), _p(v), _p(o))
+        else:
+            L.ref_lsolve_BLAS(ldm, ncol, _p(M), _p(r))
+            L.ref_matvec_BLAS(ldm, nrow, ncol, _p(M[ncol:]), _p(v), _p(o))
+        return r, o
 
     @staticmethod
     def getCoarseLevelSet_6(parent, cost, inner_parts, min_level_dist, div_rate):

@@ -14,6 +14,11 @@
 #include <cstdint>
 #include <omp.h>
 
+// dense_blas/BLAS.cpp:3 and :286 (the kernels of sptrsv_csc_block; global
+// namespace in the reference)
+void lsolve_BLAS(int dim, int num_col, double *M, double *b);
+void matvec_BLAS(int dim, int nrow, int ncol, double *M, double *x, double *b);
+
 namespace sym_lib {
 void sptrsv_csr(int n, int *Lp, int *Li, double *Lx, double *x);  // :126-127
 void sptrsv_csr_levelset(int n, const int *Lp, const int *Li,
@@ -47,6 +52,10 @@ bool spic0_csc_left(int n, const int *Ap, const int *Ai, const double *Ax,
                     double *Lx);  // :256-259
 void spic0_csr(int n, double *val, int *rowPtr, int *rowIdx);  // :242
 namespace parsy {
+// include/sympiler/parsy/triangularSolve/BLAS.h:12-19 (the kernels of
+// parsy::blockedLsolve, Triangular_BCSC.cpp:24-56)
+void dlsolve_blas_nonUnit(int ldm, int ncol, double *M, double *rhs);
+void dmatvec_blas(int ldm, int nrow, int ncol, double *M, double *vec, double *Mxvec);
 // include/sympiler/parsy/common/TreeUtils.h (impl TreeUtils.cpp:89-141)
 int getLevelSet(size_t n, const int *inTree, int *levelPtr, int *levelSet);
 // include/sympiler/parsy/symbolic/InspectionLevel_06.h:13-17
@@ -126,6 +135,18 @@ void ref_spic0_csr(int n, double *val, int *rowPtr, int *rowIdx) {
 }
 int ref_getLevelSet(int n, const int *inTree, int *levelPtr, int *levelSet) {
   return sym_lib::parsy::getLevelSet((size_t)n, inTree, levelPtr, levelSet);
+}
+void ref_lsolve_BLAS(int dim, int ncol, double *M, double *b) {
+  ::lsolve_BLAS(dim, ncol, M, b);
+}
+void ref_matvec_BLAS(int dim, int nrow, int ncol, double *M, double *x, double *b) {
+  ::matvec_BLAS(dim, nrow, ncol, M, x, b);
+}
+void ref_parsy_dlsolve(int ldm, int ncol, double *M, double *rhs) {
+  sym_lib::parsy::dlsolve_blas_nonUnit(ldm, ncol, M, rhs);
+}
+void ref_parsy_dmatvec(int ldm, int nrow, int ncol, double *M, double *vec, double *out) {
+  sym_lib::parsy::dmatvec_blas(ldm, nrow, ncol, M, vec, out);
 }
 // Copies the three new[]'d arrays out (sizes n+1, n+1, n) and frees them.
 void ref_getCoarseLevelSet_6(int n, const int *eTree, double *nodeCost,

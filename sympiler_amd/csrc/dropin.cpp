@@ -510,5 +510,43 @@ int getCoarseLevelSet_6(size_t n, const int *eTree, const int *blk2col, int &fin
   return finaLevelNo;
 }
 
+// parsy's supernodal storage (Triangular_BCSC.cpp:24-56): per-column value
+// and row-list offsets (size_t), a supernode's row list at Li[Li_ptr[c0]].
+// Its dense kernels (parsy/triangularSolve/BLAS.cpp) order every operation
+// as sym_lib's lsolve_BLAS / matvec_BLAS do, so the solve is
+// sptrsv_csc_block on the same blocks: the row lists move to the value
+// offsets that format indexes them by.  col2sup and NNZ are not read (as in
+// the reference); the supernodes must cover the n columns.
+int blockedLsolve(int n, size_t *Lp, int *Li, double *Lx, int NNZ, size_t *Li_ptr,
+                  int *col2sup, int *sup2col, int supNo, double *x) {
+  (void)NNZ;
+  (void)col2sup;
+  if (!Lp || !Li || !x) return 0;
+  if (supNo < 0 || !Li_ptr || !sup2col || !Lx)
+    throw std::invalid_argument("blockedLsolve: bad arguments");
+  std::vector<int> sup(supNo + 1), nrows(supNo), off(supNo);
+  size_t span = 0;
+  for (int i = 0; i < supNo; ++i) {
+    const int c0 = sup2col[i], c1 = sup2col[i + 1];
+    if (c0 < 0 || c1 <= c0 || c1 > n)
+      throw std::invalid_argument("blockedLsolve: bad sup2col");
+    const size_t rows = Li_ptr[c1] - Li_ptr[c0];
+    if (rows < (size_t)(c1 - c0) || Lp[c0] + rows > (size_t)0x7fffffff)
+      throw std::invalid_argument("blockedLsolve: bad row count or more than 2^31-1 values");
+    sup[i] = c0;
+    nrows[i] = (int)rows;
+    off[i] = (int)Lp[c0];
+    span = std::max(span, (size_t)Lp[c0] + rows);
+  }
+  sup[supNo] = supNo > 0 ? sup2col[supNo] : 0;
+  std::vector<int> rows_at(span > 0 ? span : 1, 0);
+  for (int i = 0; i < supNo; ++i)
+    for (int r = sup[i + 1] - sup[i]; r < nrows[i]; ++r)
+      rows_at[(size_t)off[i] + r] = Li[Li_ptr[sup[i]] + r];
+  sym_lib::sptrsv_csc_block(n, off.data(), rows_at.data(), nrows.data(), Lx, x, supNo,
+                            sup.data());
+  return 1;
+}
+
 }  // namespace parsy
 }  // namespace sym_lib

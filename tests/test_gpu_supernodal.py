@@ -100,3 +100,25 @@ def test_sptrsv_csc_block_dropins(dev):
         c_int(n), P(Lp), P(Li), P(nrows), P(Lx), P(x), P(levelPtr), P(levels), c_int(m),
         P(sup), P(sup2node), None)
     assert bitwise_equal(x, want)
+
+
+@pytest.mark.parametrize("case", [(3000, 40, 24, 400, 2), (20000, 100, 130, 4096, 5)])
+def test_parsy_blocked_lsolve_dropin(dev, case):
+    """sym_lib::parsy::blockedLsolve on parsy's storage of the same factor:
+    bit-identical to the sptrsv_csc_block restatement (the two reference
+    solves share their dense kernels' operation order)."""
+    from tests.test_supernodal_oracle import to_parsy
+    L = ctypes.CDLL(LIB)
+    n = case[0]
+    sup, nrows, Lp, Li, Lx = W.bcsc(*case)
+    b = W.uniform(case[-1], 0, n)
+    want = oracle.sptrsv_csc_block(sup, nrows, Lp, Li, Lx, b)
+    vp, rows, rp = to_parsy(sup, nrows, Lp, Li, n)
+    col2sup = np.repeat(np.arange(len(sup) - 1, dtype=np.int32), np.diff(sup))
+    x = b.copy()
+    fn = L["_ZN7sym_lib5parsy13blockedLsolveEiPmPiPdiS1_S2_S2_iS3_"]
+    fn.restype = c_int
+    r = fn(c_int(n), P(vp), P(rows), P(Lx), c_int(len(Lx)), P(rp), P(col2sup), P(sup),
+           c_int(len(sup) - 1), P(x))
+    assert r == 1
+    assert bitwise_equal(x, want)

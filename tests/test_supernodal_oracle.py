@@ -48,3 +48,48 @@ def test_block_solve_layout_against_dense(case):
     x = oracle.sptrsv_csc_block(sup, nrows, Lp, Li, Lx, b)
     L = dense(sup, nrows, Lp, Li, Lx, n)
     assert np.max(np.abs(L @ x - b)) <= 1e-13 * np.max(np.abs(b))
+
+
+@pytest.mark.skipif(not oracle.ref_available(), reason="oracle/_ref not built")
+def test_parsy_dense_kernels_equal_sym_lib_kernels():
+    """parsy::blockedLsolve (Triangular_BCSC.cpp:24-56) runs
+    dlsolve_blas_nonUnit / dmatvec_blas (parsy/triangularSolve/BLAS.cpp:12,
+    :123); sptrsv_csc_block runs lsolve_BLAS / matvec_BLAS
+    (dense_blas/BLAS.cpp:3, :286).  Both pairs, compiled from the reference,
+    give bit-identical results on every block shape (8/4/2/1 column tails,
+    empty and long row blocks, padded leading dimensions, a product added to
+    a nonzero vector): the two solves differ only in their storage format."""
+    rng = np.random.default_rng(0)
+    for ncol in list(range(1, 26)) + [63, 64, 100]:
+        for nrow in (0, 1, 3, 7, 8, 9, 17, 40):
+            ldm = ncol + nrow + int(rng.integers(0, 3))
+            M = rng.standard_normal(ldm * ncol)
+            M[np.arange(ncol) * (ldm + 1)] += 4.0
+            args = (M, ldm, ncol, nrow, rng.standard_normal(ncol),
+                    rng.standard_normal(ncol), rng.standard_normal(max(nrow, 1)))
+            a = oracle.ref.dense_kernels(*args, parsy=False)
+            b = oracle.ref.dense_kernels(*args, parsy=True)
+            assert bitwise_equal(a[0], b[0]) and bitwise_equal(a[1], b[1]), (ncol, nrow)
+
+
+def to_parsy(sup, nrows, Lp, Li, n):
+    """The same supernodal factor in parsy's storage (Triangular_BCSC.cpp:
+    24-56): per-column value offsets Lp (size_t) and row-list offsets Li_ptr
+    (size_t), each supernode's row list (its own columns, then the rows
+    below) at Li[Li_ptr[first column]]."""
+    m = len(sup) - 1
+    vp = np.zeros(n + 1, np.uint64)
+    rp = np.zeros(n + 1, np.uint64)
+    rows = []
+    total_rows = 0
+    for i in range(m):
+        c0, c1, nr = int(sup[i]), int(sup[i + 1]), int(nrows[i])
+        for c in range(c0, c1):
+            vp[c] = int(Lp[i]) + (c - c0) * nr
+            rp[c] = total_rows
+        rows.extend(range(c0, c1))
+        rows.extend(int(v) for v in Li[int(Lp[i]) + (c1 - c0):int(Lp[i]) + nr])
+        total_rows += nr
+    vp[n] = int(Lp[m - 1]) + (int(sup[m]) - int(sup[m - 1])) * int(nrows[m - 1]) if m else 0
+    rp[n] = total_rows
+    return vp, np.asarray(rows, np.int32), rp
