@@ -136,6 +136,15 @@ int getCoarseLevelSet_6(size_t n, const int *eTree, const int *blk2col, int &fin
  * Returns 1, or 0 for a null Lp, Li or x as the reference does. */
 int blockedLsolve(int n, size_t *Lp, int *Li, double *Lx, int NNZ, size_t *Li_ptr,
                   int *col2sup, int *sup2col, int supNo, double *x);
+/* Triangular_BCSC.h (impl Triangular_BCSC.cpp:512-579): the leveled variant
+ * over a getCoarseLevelSet_6 schedule of the supernodes.  The reference
+ * subtracts with atomics in thread order; here x is blockedLsolve's (serial
+ * order), within rounding of the reference's.  Throws std::invalid_argument
+ * unless the schedule lists every supernode once. */
+int H2LeveledBlockedLsolve(int n, size_t *Lp, int *Li, double *Lx, int NNZ, size_t *Li_ptr,
+                           int *col2sup, int *sup2col, int supNo, double *x, int levels,
+                           int *levelPtr, int *levelSet, int parts, int *parPtr,
+                           int *partition, int chunk);
 }  // namespace parsy
 
 }  // namespace sym_lib

@@ -548,5 +548,42 @@ int blockedLsolve(int n, size_t *Lp, int *Li, double *Lx, int NNZ, size_t *Li_pt
   return 1;
 }
 
+// The leveled variant (Triangular_BCSC.cpp:512-579): the reference solves the
+// supernodes of each part of a coarse level in parallel and subtracts their
+// products from x with atomics, so its subtraction order -- and the last
+// bits of x -- follow the thread schedule.  Here the same supernodes are
+// solved by the dependency-driven kernel in the serial order of
+// blockedLsolve (x within rounding of the reference's).  The schedule
+// (levelPtr / parPtr / partition, from getCoarseLevelSet_6) must list every
+// supernode once, as the reference needs for a complete solve; levelSet and
+// chunk are not read by the reference either.
+int H2LeveledBlockedLsolve(int n, size_t *Lp, int *Li, double *Lx, int NNZ, size_t *Li_ptr,
+                           int *col2sup, int *sup2col, int supNo, double *x, int levels,
+                           int *levelPtr, int *levelSet, int parts, int *parPtr,
+                           int *partition, int chunk) {
+  (void)levelSet;
+  (void)parts;
+  (void)chunk;
+  if (!Lp || !Li || !x) return 0;
+  if (levels < 0 || (levels > 0 && (!levelPtr || !parPtr || !partition)))
+    throw std::invalid_argument("H2LeveledBlockedLsolve: bad schedule arguments");
+  std::vector<char> seen(supNo > 0 ? supNo : 1, 0);
+  long listed = 0;
+  for (int l = 0; l < levels; ++l)
+    for (int p = levelPtr[l]; p < levelPtr[l + 1]; ++p)
+      for (int k = parPtr[p]; k < parPtr[p + 1]; ++k) {
+        const int s = partition[k];
+        if (s < 0 || s >= supNo || seen[s])
+          throw std::invalid_argument(
+              "H2LeveledBlockedLsolve: the schedule must list every supernode once");
+        seen[s] = 1;
+        ++listed;
+      }
+  if (listed != supNo)
+    throw std::invalid_argument(
+        "H2LeveledBlockedLsolve: the schedule must list every supernode once");
+  return blockedLsolve(n, Lp, Li, Lx, NNZ, Li_ptr, col2sup, sup2col, supNo, x);
+}
+
 }  // namespace parsy
 }  // namespace sym_lib

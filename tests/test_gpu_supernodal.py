@@ -122,3 +122,38 @@ def test_parsy_blocked_lsolve_dropin(dev, case):
            c_int(len(sup) - 1), P(x))
     assert r == 1
     assert bitwise_equal(x, want)
+
+
+def test_parsy_leveled_blocked_lsolve_with_tree_schedule(dev):
+    """The tree-level coarsening's schedule of the supernodal tree feeding
+    sym_lib::parsy::H2LeveledBlockedLsolve: every supernode's consumers are
+    its tree ancestors, and x equals the serial order's bit for bit (the
+    reference's atomics reorder the subtractions; it agrees to rounding)."""
+    from tests.helpers import postorder_binary_tree, rel_err
+    from tests.test_supernodal_oracle import to_parsy
+    levels = 9
+    sup, nrows, Lp, Li, Lx = W.bcsc_tree(levels, 4, 64, 2.0, 3)
+    n, m = int(sup[-1]), len(sup) - 1
+    parent = postorder_binary_tree(levels)
+    col2sup = np.repeat(np.arange(m, dtype=np.int32), np.diff(sup))
+    for i in range(m):
+        anc, a = set(), parent[i]
+        while a >= 0:
+            anc.add(int(a))
+            a = parent[a]
+        w = sup[i + 1] - sup[i]
+        assert {int(col2sup[r]) for r in Li[Lp[i] + w:Lp[i] + nrows[i]]} <= anc
+    cost = (np.diff(sup) * nrows).astype(np.float64)
+    lp, pp, part = sa.tree_coarsen(parent, cost, 8, 1, 2)
+    b = W.uniform(3, 0, n)
+    want = oracle.sptrsv_csc_block(sup, nrows, Lp, Li, Lx, b)
+    vp, rows, rp = to_parsy(sup, nrows, Lp, Li, n)
+    x = b.copy()
+    fn = ctypes.CDLL(LIB)["_ZN7sym_lib5parsy22H2LeveledBlockedLsolveEiPmPiPdiS1_S2_S2_iS3_iS2_S2_iS2_S2_i"]
+    fn.restype = c_int
+    r = fn(c_int(n), P(vp), P(rows), P(Lx), c_int(len(Lx)), P(rp), P(col2sup), P(sup),
+           c_int(m), P(x), c_int(len(lp) - 1), P(lp), None, c_int(len(pp) - 1), P(pp),
+           P(part), c_int(1))
+    assert r == 1
+    assert bitwise_equal(x, want)
+    assert rel_err(x, want) <= 1e-12
