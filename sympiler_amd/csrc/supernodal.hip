@@ -102,10 +102,21 @@ __global__ __launch_bounds__(256) void k_supernodal(SnArgs a) {
       wave_lds_sync();
     } else {
       // 2. own rows: b minus the temporaries, in ascending supernode order
+      // Eight temporaries are loaded before any is subtracted (the index of
+      // a slot past the row's list is clamped to its last entry and the
+      // value discarded), so a long list costs one memory round trip per
+      // eight entries; the subtractions keep the list's order.
       for (int r = lane; r < w; r += 64) {
         double v = x[s + r];
-        for (int c = a.cptr[s + r], ce = a.cptr[s + r + 1]; c < ce; ++c)
-          v -= ld_dev(&T[a.cidx[c]]);
+        const int ce = a.cptr[s + r + 1];
+        for (int c0 = a.cptr[s + r]; c0 < ce; c0 += 8) {
+          double t[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) t[u] = ld_dev(&T[a.cidx[min(c0 + u, ce - 1)]]);
+#pragma unroll
+          for (int u = 0; u < 8; ++u)
+            if (c0 + u < ce) v -= t[u];
+        }
         xs[r] = v;
       }
       wave_lds_sync();
@@ -183,6 +194,31 @@ __global__ __launch_bounds__(256) void k_supernodal(SnArgs a) {
 #pragma unroll
       for (int u = 0; u < SN_ROWS; ++u) t[u] = 0.0;
       for (int col = 0; col < w;) {
+        if (w - col >= 16) {
+          // two full blocks of eight: all their entries in flight together,
+          // each block's sum formed and added as the reference orders it
+          double mk2[2][SN_ROWS][8];
+#pragma unroll
+          for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int u = 0; u < SN_ROWS; ++u) {
+              const int k = k0 + 64 * u;
+#pragma unroll
+              for (int c = 0; c < 8; ++c)
+                mk2[h][u][c] = k < nr ? M[k + (int64_t)(col + 8 * h + c) * nr] : 0.0;
+            }
+#pragma unroll
+          for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int u = 0; u < SN_ROWS; ++u) {
+              double sum = xs[col + 8 * h] * mk2[h][u][0];
+#pragma unroll
+              for (int c = 1; c < 8; ++c) sum = sum + xs[col + 8 * h + c] * mk2[h][u][c];
+              t[u] = t[u] + sum;
+            }
+          col += 16;
+          continue;
+        }
         const int B = matvec_block(col, w);
         double mk[SN_ROWS][8];
 #pragma unroll
