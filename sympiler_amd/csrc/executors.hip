@@ -111,8 +111,11 @@ __device__ __forceinline__ void st_x(T *p, T v) {
 }
 
 // Dependency values loaded per lane and row before any is consumed: a row
-// with at most DEP_CHUNK (strided) entries needs one memory round trip.
-constexpr int DEP_CHUNK = 4;
+// with at most dep_chunk(G) (strided) entries needs one memory round trip.
+// Eight when several lanes share a row (long rows), else four: with one lane
+// per row the rows are short and eight slots only add dead loads.
+template <int G>
+constexpr int dep_chunk() { return G >= 8 ? 8 : 4; }
 
 // The mark: a signalling-NaN bit pattern that no arithmetic produces
 // (operations return quiet NaNs).  In the marked sync-free protocol a unit
@@ -181,6 +184,7 @@ __device__ __forceinline__ void finish_rows(const SolveArgs<T> &a,
                                             int iters, const int (&qpos)[NR],
                                             Solved<T, NR> &sv) {
   constexpr bool SYNC = SY != SY_LEVEL;
+  constexpr int DC = dep_chunk<G>();
   const int lane = threadIdx.x & 63;
   const int s = (lane / KT) % G;
   const int c = lane % KT;
@@ -198,14 +202,14 @@ __device__ __forceinline__ void finish_rows(const SolveArgs<T> &a,
     for (int r = 0; r < NR; ++r) acc[r] = G == 1 ? bv[r] : T(0);
     // Branch-free chunk: dead slots load the lane's own right-hand side
     // address (always valid) and are discarded, so the compiler keeps all
-    // DEP_CHUNK * NR loads in flight instead of waiting at every branch.
+    // dep_chunk<G>() * NR loads in flight instead of waiting at every branch.
     const int64_t coff_safe = colok ? coff : 0;
-    for (int t0 = 0; t0 < iters; t0 += DEP_CHUNK) {
-      T xd[NR][DEP_CHUNK];
+    for (int t0 = 0; t0 < iters; t0 += DC) {
+      T xd[NR][DC];
 #pragma unroll
       for (int r = 0; r < NR; ++r)
 #pragma unroll
-        for (int t = 0; t < DEP_CHUNK; ++t) {
+        for (int t = 0; t < DC; ++t) {
           const int j = jb[r] + (t0 + t) * G;
           const bool live = j < je[r];
           const int d = cis[live ? j : 0];
@@ -221,14 +225,14 @@ __device__ __forceinline__ void finish_rows(const SolveArgs<T> &a,
 #pragma unroll
           for (int r = 0; r < NR; ++r)
 #pragma unroll
-            for (int t = 0; t < DEP_CHUNK; ++t)
+            for (int t = 0; t < DC; ++t)
               again |= (jb[r] + (t0 + t) * G < je[r]) && colok && is_mark(xd[r][t]);
           if (!__any(again)) break;
           __builtin_amdgcn_s_sleep(1);
 #pragma unroll
           for (int r = 0; r < NR; ++r)
 #pragma unroll
-            for (int t = 0; t < DEP_CHUNK; ++t) {
+            for (int t = 0; t < DC; ++t) {
               const int j = jb[r] + (t0 + t) * G;
               if (j < je[r] && colok && is_mark(xd[r][t]))
                 xd[r][t] = ld_dev(&a.X[(int64_t)cis[j] * a.rs + coff]);
@@ -242,7 +246,7 @@ __device__ __forceinline__ void finish_rows(const SolveArgs<T> &a,
 #pragma unroll
       for (int r = 0; r < NR; ++r)
 #pragma unroll
-        for (int t = 0; t < DEP_CHUNK; ++t) {
+        for (int t = 0; t < DC; ++t) {
           const int j = jb[r] + (t0 + t) * G;
           const bool live = j < je[r];
           const T v = vals[live ? j : 0];
