@@ -75,11 +75,13 @@ def calibrate_copy():
                       "bytes_written_per_copy": 1 << 30, "copies": 5}))
 
 
-def latency_floor(torch, sa, W, dev, stream, levels, ms_k, ms_1):
-    """Per-level synchronisation cost of the sync-free executor: a chain
-    (bidiagonal) system with one row per level has no parallelism and almost
-    no bytes, so its solve time / levels is the cost of one dependent step.
-    T_floor = levels * t_sync bounds any solve with that many levels."""
+def unit_executor_handoff(torch, sa, W, dev, stream, levels, ms_1):
+    """Cost of one dependent cross-wave step of the unit (sync-free)
+    executor: a chain (bidiagonal) system with one row per level has no
+    parallelism and almost no bytes, so its solve time / levels is the
+    hand-off cost t_sync, and levels * t_sync bounds any unit-executor solve
+    with that many levels.  It does not bound the region executor, whose
+    critical path crosses only region-to-region hand-offs."""
     out = {}
     Lp, Li, Lx = W.lap2d_tril(1, levels)  # bidiagonal: `levels` levels
     plan = sa.Plan(*(torch.from_numpy(a).to(dev) for a in (Lp, Li, Lx)))
@@ -90,10 +92,8 @@ def latency_floor(torch, sa, W, dev, stream, levels, ms_k, ms_1):
                             lambda: X.copy_(B0), 10, 2, stream, torch)
         plan.check()
         t = float(np.mean(ev))
-        out[f"k{k}"] = {"chain_ms": round(t, 4), "t_sync_us": round(1e3 * t / levels, 3),
-                        "floor_ms": round(t, 4)}
-    out["headline_ms_over_floor"] = round(ms_k / out["k32"]["floor_ms"], 2)
-    out["single_rhs_ms_over_floor"] = round(ms_1 / out["k1"]["floor_ms"], 2)
+        out[f"k{k}"] = {"chain_ms": round(t, 4), "t_sync_us": round(1e3 * t / levels, 3)}
+    out["syncfree_k1_ms_over_chain"] = round(ms_1 / out["k1"]["chain_ms"], 2)
     return out
 
 
@@ -152,6 +152,7 @@ def other_configs(torch, sa, W, dev, stream):
     n5 = len(Lp5) - 1
     b5 = torch.from_numpy(W.uniform(4, 0, n5)).to(dev)
     adj = {"n": n5, "nnz": int(Lp5[-1])}
+    cold5 = ColdCache(torch, dev)
     for name, dt in (("fp64", torch.float64), ("fp32", torch.float32)):
         dLx = torch.from_numpy(Lx5).to(dev).to(dt)
         plan5 = sa.Plan(torch.from_numpy(Lp5).to(dev), torch.from_numpy(Li5).to(dev), dLx)
@@ -169,10 +170,14 @@ def other_configs(torch, sa, W, dev, stream):
 
         wf, _ = timed_steps(fwd, lambda: None, 10, 2, stream, torch)
         wb, _ = timed_steps(fwdbwd, lambda: None, 10, 2, stream, torch)
+        wfc, _ = timed_steps(fwd, cold5(), 10, 2, stream, torch)
+        wbc, _ = timed_steps(fwdbwd, cold5(), 10, 2, stream, torch)
         plan5.check()
         adj[name] = {"forward_ms": round(1e3 * float(np.mean(wf)), 4),
                      "forward_backward_ms": round(1e3 * float(np.mean(wb)), 4),
-                     "ratio": round(float(np.mean(wb)) / float(np.mean(wf)), 2)}
+                     "ratio": round(float(np.mean(wb)) / float(np.mean(wf)), 2),
+                     "cold_forward_ms": round(1e3 * float(np.mean(wfc)), 4),
+                     "cold_forward_backward_ms": round(1e3 * float(np.mean(wbc)), 4)}
         del plan5
     out["adjoint_64cubed_autograd"] = adj
     log(f"adjoint 64^3: {adj}")
@@ -262,6 +267,21 @@ def factor_configs(torch, sa, W, dev, stream):
     log(f"ilu0 128^3: {out['ilu0_grid3d_128cubed']}")
     del dAp7, dAi7, dAx7, dd7, l7
     return out
+
+
+class ColdCache:
+    """Evicts the caches between repetitions: writes a 1 GiB buffer (four
+    times the 256 MB MALL) before each timed step, outside the timed region."""
+
+    def __init__(self, torch, dev):
+        self.buf = torch.empty(1 << 27, dtype=torch.float64, device=dev)
+
+    def __call__(self, reset=None):
+        def r():
+            if reset is not None:
+                reset()
+            self.buf.fill_(0.5)
+        return r
 
 
 def log(msg):
@@ -400,6 +420,20 @@ def main():
                              "gbs": round(bytes1 / (np.mean(e1) * 1e-3) / 1e9, 1),
                              "residual": float(r)}
             log(f"k=1 {name}: {singles[name]}")
+        # the single-RHS working set (142 MB) fits the 256 MB MALL: the same
+        # solves with the caches evicted before each one
+        cold = ColdCache(torch, dev)
+        for name, ex in (("syncfree_cold", "syncfree"), ("regions_cold", "regions"),
+                         ("levelset_graph_cold", "graph")):
+            plan.set_lanes(1)
+            w1, e1 = timed_steps(lambda: plan.solve(x1, executor=ex), cold(reset1), 10, 2,
+                                 stream, torch)
+            plan.check()
+            singles[name] = {"ms": round(1e3 * float(np.mean(w1)), 4),
+                             "kernel_ms": round(float(np.mean(e1)), 4),
+                             "rhs_solves_per_s": round(1.0 / float(np.mean(w1)), 1),
+                             "gbs": round(bytes1 / (np.mean(e1) * 1e-3) / 1e9, 1)}
+            log(f"k=1 {name}: {singles[name]}")
         plan.set_lanes(0)
         detail["single_rhs"] = singles
         # the plain level-set baseline: one kernel per level, launched eagerly
@@ -409,9 +443,9 @@ def main():
         detail["plain_levelset_rhs_solves_per_s"] = plain
         detail["headline_vs_plain_levelset"] = round(value / plain, 2)
         detail["headline_frac_hbm_peak"] = round(achieved / HBM_PEAK_GBS, 4)
-        detail["latency_floor"] = latency_floor(torch, sa, W, dev, stream, plan.levels,
-                                                ms_kernel, singles["syncfree"]["kernel_ms"])
-        log(f"latency floor: {detail['latency_floor']}")
+        detail["unit_executor_handoff"] = unit_executor_handoff(
+            torch, sa, W, dev, stream, plan.levels, singles["syncfree"]["kernel_ms"])
+        log(f"unit executor hand-off: {detail['unit_executor_handoff']}")
 
         # fused adjoint: gb = L^-T gx, grad over L's pattern; loss 0.5||x||^2
         xs = x1.clone()
@@ -421,9 +455,11 @@ def main():
         ab = W.algorithmic_bytes_adjoint(n, nnz)
         # the automatic choice (the transposed region schedule on this matrix),
         # then the sync-free adjoint for comparison; both include grad-L
-        for key, ex in (("adjoint", "auto"), ("adjoint_syncfree", "syncfree")):
+        for key, ex, rs in (("adjoint", "auto", lambda: None),
+                            ("adjoint_cold", "auto", cold()),
+                            ("adjoint_syncfree", "syncfree", lambda: None)):
             wa, ea = timed_steps(lambda: plan.adjoint(xs, gx, gb, gL, executor=ex),
-                                 lambda: None, 10, 2, stream, torch)
+                                 rs, 10, 2, stream, torch)
             plan.check()
             detail[key] = {"executor": ex,
                            "ms": round(1e3 * float(np.mean(wa)), 4),
@@ -530,12 +566,24 @@ def measured_traffic(workload, k, layout, kernel):
     return None
 
 
+def cpu_model():
+    """The host CPU's model name and counts as lscpu reports them."""
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        keep = ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core", "CPU(s)")
+        return "; ".join(line.strip() for line in out.splitlines()
+                         if line.split(":")[0].strip() in keep)
+    except Exception as e:  # lscpu missing: say so rather than guess
+        return f"lscpu unavailable ({e})"
+
+
 def cpu_baseline(Lp, Li, Lx, Bcols, Xgpu, args, k):
     """Reference OpenMP level-set solve on this host, one right-hand side
     at a time (the reference has no multi-RHS CSR solve), for a bounded
     sample; also checks the GPU solution of the sampled columns."""
     import oracle
-    ncores = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16")), 16)
+    ncores = len(os.sched_getaffinity(0))
     lp, ls, _ = oracle.levelset_csr(Lp, Li)
     if oracle.ref_available():
         kind = "reference"
@@ -557,6 +605,21 @@ def cpu_baseline(Lp, Li, Lx, Bcols, Xgpu, args, k):
             worst = max(worst, float(np.max(np.abs(Xh[c] - x)) / np.max(np.abs(x))))
         done += 1
     rate = done / t_total
+    # the C restatement (oracle.c) beside the reference build on the same
+    # matrix, cores and right-hand side: their time ratio relates a port
+    # timing on another host to the reference's
+    ratio = None
+    if kind == "reference":
+        b = Bcols[0]
+        tr, tp = [], []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            oracle.ref.sptrsv_csr_levelset(Lp, Li, Lx, b, lp, ls)
+            tr.append(time.perf_counter() - t0)
+            t0 = time.perf_counter()
+            oracle.sptrsv_csr_levelset(Lp, Li, Lx, b, lp, ls, ncores)
+            tp.append(time.perf_counter() - t0)
+        ratio = round(min(tp) / min(tr), 3)
     # the reference's IC(0) (spic0_csc_left, serial) on the 128^3 Laplacian,
     # beside the device factor timed in the detail
     ic0_ms = None
@@ -586,7 +649,9 @@ def cpu_baseline(Lp, Li, Lx, Bcols, Xgpu, args, k):
     log(f"cpu baseline ({kind}, {ncores} threads): "
         f"{rate:.2f} RHS-solves/s over {done} solves; GPU vs CPU max rel err {worst:.2e}")
     return {"value": round(rate, 3), "unit": "RHS-solves/s",
-            "cores": ncores, "kind": kind,
+            "cores": ncores, "kind": kind, "cpu": cpu_model(),
+            "host_logical_cpus": os.cpu_count(),
+            "port_over_reference_time": ratio,
             "sample": f"{done} single-RHS level-set solves of the same matrix "
                       f"({t_total:.1f}s)",
             "gpu_vs_cpu_max_rel_err": worst,

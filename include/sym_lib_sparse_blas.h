@@ -119,6 +119,19 @@ void sptrsv_csc_block(int n, const int *Lp, const int *Li, const int *nrows, dou
 int build_levelSet_CSC(int n, const int *Lp, const int *Li, int *&level_ptr,
                        int *&level_set);
 
+/* The coarsened-schedule (LBC) inspector of the un-vendored aggregation
+ * submodule, with the arguments of its only call site
+ * (demo/sptrsv_demo_utils.h:83-88; n, lC, lR = the CSR of L, stype, then
+ * the outputs and lp = threads, cp = -1, ic = 4000 from
+ * demo/sptrsv_demo.cpp:37, and the row nnz as cost, :80-82).  The parameter
+ * types follow parsy's getCoarseLevelSet_6 (size_t n, const int arrays), the
+ * submodule's declaration being absent.  Built on the device
+ * (symamd_coarsen); arrays new[]'d for the caller to delete[]; returns 1. */
+int get_coarse_levelSet_DAG_CSC_tree(size_t n, const int *lC, const int *lR, int stype,
+                                     int &finaLevelNo, int *&finaLevelPtr, int &partNo,
+                                     int *&finalPartPtr, int *&finalNodePtr, int innerParts,
+                                     int minLevelDist, int divRate, double *nodeCost);
+
 namespace parsy {
 /* parsy/symbolic/InspectionLevel_06.h:13-17: tree-level coarsening of an
  * elimination tree (host only, on symamd_tree_coarsen); the three arrays are

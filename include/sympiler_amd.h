@@ -113,11 +113,18 @@ int symamd_set_timeout_ms(symamd_handle h, double ms);
  *   SYMAMD_OPT_REGION_WAVES (read at solve time): wavefronts per task of
  *     the SYMAMD_EXEC_REGIONS schedule, 1, 2 or 4 for tasks of eight
  *     columns (k >= 8; narrower tasks use one); 0, the default, picks it.
+ *     Results are identical either way.
+ *   SYMAMD_OPT_REGION_BAND (read at solve time): a region task publishes
+ *     its progress every this many internal levels (slices), so a region
+ *     that reads it starts on its first levels while the rest is solved
+ *     (0, the default, = 2; a region has at most 31 bands, wider ones when
+ *     it has more levels); each value's schedule is built once per plan.
  *     Results are identical either way. */
 enum { SYMAMD_OPT_TICKETS = 1, SYMAMD_OPT_TRACE_PTR = 2, SYMAMD_OPT_TRACE_CAP = 3,
        SYMAMD_OPT_MAX_WAVES = 4, SYMAMD_OPT_COLMAJOR_DIRECT = 5, SYMAMD_OPT_MARKS = 6,
        SYMAMD_OPT_REGIONS = 7, SYMAMD_OPT_REGION_ROWS = 8, SYMAMD_OPT_OWNERS = 9,
-       SYMAMD_OPT_REGION_CAP = 10, SYMAMD_OPT_REGION_WAVES = 11 };
+       SYMAMD_OPT_REGION_CAP = 10, SYMAMD_OPT_REGION_WAVES = 11,
+       SYMAMD_OPT_REGION_BAND = 12 };
 int symamd_set_option(symamd_handle h, int option, int64_t value);
 const char *symamd_last_error(void);
 /* Library version string. */
@@ -135,18 +142,31 @@ int symamd_levelset(symamd_handle h, int n, const int *d_Lp, const int *d_Li,
                     int *d_level_ptr, int *d_level_set, int *levels);
 
 /* Coarsened 3-array schedule (level_ptr / par_ptr / partition), the format
- * sym_lib::sptrsv_csr_lbc consumes (sparse_blas/sptrsv.cpp:57-75): `span`
- * consecutive levels form one coarse level; inside it the rows are split
- * into partitions closed under dependencies (every dependency of a row in a
- * partition is in an earlier coarse level or earlier in the same
- * partition), merged greedily up to `max_part_cost` (cost = row nnz).
- * Output arrays are device arrays of size n+1 (level_ptr, par_ptr) and n
- * (partition).  Replaces the un-vendored get_coarse_levelSet_DAG_CSC_tree
- * (call site demo/sptrsv_demo_utils.h:83-88). */
+ * sym_lib::sptrsv_csr_lbc consumes (sparse_blas/sptrsv.cpp:57-75), built on
+ * the GPU from a device CSR lower triangle (diagonal last).  Replaces the
+ * un-vendored get_coarse_levelSet_DAG_CSC_tree (call site
+ * demo/sptrsv_demo_utils.h:83-88; the demo passes inner_parts = threads,
+ * min_level_dist = -1, div_rate = 4000, demo/sptrsv_demo.cpp:37, and the row
+ * nnz as cost, :80-82), with the same parameters:
+ *   d_cost          per-row cost (device, n doubles); NULL = row nnz
+ *   inner_parts     parts (threads) per coarse level, 1..4096
+ *   min_level_dist  > 0: that many consecutive dependency levels form each
+ *                   coarse level; <= 0: chosen from the DAG -- the largest
+ *                   block of 2^t levels starting at a coarse level's first
+ *                   level whose weakly connected components number at least
+ *                   inner_parts, none costlier than cost/inner_parts
+ *   div_rate        the largest block tried is the largest 2^t <= div_rate
+ * Inside a coarse level the components (dependency edges that stay inside
+ * it) are packed worst-fit (largest first, into the first least-loaded
+ * part); a part lists its rows in (level, row id) order; empty parts are
+ * dropped.  Every dependency then points to an earlier coarse level or to
+ * an earlier row of the same part.  Outputs are device arrays of n+1
+ * (level_ptr, par_ptr) and n (partition) ints; *level_no coarse levels,
+ * *part_no parts.  Deterministic (costs summed in 1/1024 fixed point). */
 int symamd_coarsen(symamd_handle h, int n, const int *d_Lp, const int *d_Li,
-                   int span, int max_part_cost, int *d_level_ptr,
-                   int *d_par_ptr, int *d_partition, int *level_no,
-                   int *part_no);
+                   const double *d_cost, int inner_parts, int min_level_dist,
+                   int div_rate, int *d_level_ptr, int *d_par_ptr, int *d_partition,
+                   int *level_no, int *part_no);
 
 /* Load-balanced level coarsening of an elimination tree (host arrays; no
  * device work): `parent[i]` is node i's parent, negative for a root;

@@ -26,8 +26,10 @@ Reference mapping (reference include/sympiler/sparse_blas_lib.h):
 """
 from __future__ import annotations
 
+import atexit
 import ctypes
 import os
+import weakref
 from ctypes import (POINTER, byref, c_char_p, c_double, c_int, c_int64,
                     c_void_p)
 
@@ -51,6 +53,27 @@ class SymAmdError(RuntimeError):
 
 
 _lib = None
+# Native objects still alive; destroyed explicitly at interpreter exit (in
+# creation-reverse order: graphs, plans, then handles), before module
+# teardown, so no device memory is freed from a finalizer while the runtime
+# and the profiler tools that wrap it are shutting down.
+_live = {"graph": weakref.WeakSet(), "plan": weakref.WeakSet(), "handle": weakref.WeakSet()}
+_closing = False
+
+
+def _release_all():
+    global _closing
+    if _lib is not None:
+        for kind in ("graph", "plan", "handle"):
+            for obj in list(_live[kind]):
+                try:
+                    obj.close()
+                except Exception:
+                    pass
+    _closing = True
+
+
+atexit.register(_release_all)
 
 
 def lib() -> ctypes.CDLL:
@@ -70,7 +93,7 @@ def lib() -> ctypes.CDLL:
             "symamd_set_timeout_ms": [vp, dbl],
             "symamd_set_option": [vp, i32, i64],
             "symamd_levelset": [vp, i32, vp, vp, vp, vp, POINTER(i32)],
-            "symamd_coarsen": [vp, i32, vp, vp, i32, i32, vp, vp, vp,
+            "symamd_coarsen": [vp, i32, vp, vp, vp, i32, i32, i32, vp, vp, vp,
                                POINTER(i32), POINTER(i32)],
             "symamd_tree_coarsen": [i32, vp, vp, i32, i32, i32, vp, vp, vp,
                                     POINTER(i32), POINTER(i32)],
@@ -159,6 +182,7 @@ class Handle:
         self._h = c_void_p()
         _check(lib().symamd_create(byref(self._h), c_void_p(stream.cuda_stream)))
         self.stream = stream
+        _live["handle"].add(self)
         if timeout_ms is not None:
             _check(lib().symamd_set_timeout_ms(self._h, float(timeout_ms)))
 
@@ -199,6 +223,10 @@ class Handle:
         of eight columns (0: chosen from k)."""
         _check(lib().symamd_set_option(self._h, 11, int(waves)))
 
+    def set_region_band(self, slices: int):
+        """Slices per published band of the region executor (0: default)."""
+        _check(lib().symamd_set_option(self._h, 12, int(slices)))
+
     def set_owners(self, on: bool):
         """Run plans with regions on the owner schedule (True, default) or
         deal their units like other plans (False)."""
@@ -217,10 +245,15 @@ class Handle:
         _check(lib().symamd_set_option(self._h, 2, buf.data_ptr()))
         _check(lib().symamd_set_option(self._h, 3, buf.numel()))
 
-    def __del__(self):
+    def close(self):
+        """Destroy the native handle (idempotent)."""
         if getattr(self, "_h", None) and _lib is not None:
             _lib.symamd_destroy(self._h)
-            self._h = None
+        self._h = None
+
+    def __del__(self):
+        if not _closing:
+            self.close()
 
 
 _default_handle = None
@@ -263,15 +296,23 @@ class Plan:
                                                 _ptr(Lp), _ptr(Li), _ptr(Lx),
                                                 _dtype_code(Lx.dtype),
                                                 byref(self._p)))
+        _live["plan"].add(self)
         n, nnz, lev, mx = c_int(), c_int64(), c_int(), c_int()
         _check(lib().symamd_plan_info(self._p, byref(n), byref(nnz), byref(lev),
                                       byref(mx)))
         self.nnz, self.levels, self.max_level_rows = nnz.value, lev.value, mx.value
+        self.values_generation = 0  # incremented by every update_values
 
-    def __del__(self):
+    def close(self):
+        """Destroy the native plan (idempotent)."""
         if getattr(self, "_p", None) and _lib is not None:
             _lib.symamd_plan_destroy(self._p)
-            self._p = None
+        self._p = None
+
+    def __del__(self):
+        if not _closing:
+            self.close()
+
 
     # ----------------------------------------------------------- inspection
     def level_sets(self):
@@ -306,6 +347,7 @@ class Plan:
         if Lx.dtype != self.dtype:
             raise ValueError("value dtype differs from the plan's")
         _check(lib().symamd_plan_update_values(self.h._h, self._p, _ptr(Lx)))
+        self.values_generation += 1
 
     def ic0(self, Ax, variant="left", out=None, strict=True):
         """Zero-fill incomplete Cholesky of the SPD matrix whose lower triangle
@@ -416,14 +458,20 @@ class Graph:
 
     def __init__(self, plan: Plan, g, B):
         self.plan, self._g, self._B = plan, g, B
+        _live["graph"].add(self)
 
     def launch(self):
         _check(lib().symamd_graph_launch(self.plan.h._h, self._g))
 
-    def __del__(self):
+    def close(self):
+        """Destroy the native graph (idempotent)."""
         if getattr(self, "_g", None) and _lib is not None:
             _lib.symamd_graph_destroy(self._g)
-            self._g = None
+        self._g = None
+
+    def __del__(self):
+        if not _closing:
+            self.close()
 
 
 # ------------------------------------------------------------ free functions
@@ -442,18 +490,23 @@ def levelset(Lp, Li, handle: Handle | None = None):
     return lp[: lev.value + 1], ls[:n]
 
 
-def coarsen(Lp, Li, span=4, max_part_cost=4000, handle: Handle | None = None):
-    """Coarsened (level_ptr, par_ptr, partition) schedule, int32 cuda tensors."""
+def coarsen(Lp, Li, inner_parts=6, min_level_dist=-1, div_rate=4000, cost=None,
+            handle: Handle | None = None):
+    """Coarsened (level_ptr, par_ptr, partition) schedule built on the GPU,
+    int32 cuda tensors; the parameters of get_coarse_levelSet_DAG_CSC_tree
+    (demo/sptrsv_demo_utils.h:83-88; defaults are the demo's,
+    demo/sptrsv_demo.cpp:37).  cost: per-row float64 cuda tensor (None: row
+    nnz)."""
     import torch
-    _require_cuda(Lp, Li)
+    _require_cuda(Lp, Li, cost)
     h = handle or default_handle()
     n = Lp.numel() - 1
     lp = torch.empty(n + 1, dtype=torch.int32, device=Lp.device)
     pp = torch.empty(n + 1, dtype=torch.int32, device=Lp.device)
     part = torch.empty(max(n, 1), dtype=torch.int32, device=Lp.device)
     lno, pno = c_int(), c_int()
-    _check(lib().symamd_coarsen(h._h, n, _ptr(Lp), _ptr(Li), int(span),
-                                int(max_part_cost), _ptr(lp), _ptr(pp),
+    _check(lib().symamd_coarsen(h._h, n, _ptr(Lp), _ptr(Li), _ptr(cost), int(inner_parts),
+                                int(min_level_dist), int(div_rate), _ptr(lp), _ptr(pp),
                                 _ptr(part), byref(lno), byref(pno)))
     return lp[: lno.value + 1], pp[: pno.value + 1], part[:n]
 
@@ -571,6 +624,7 @@ class Ilu0Plan:
         self._p = c_void_p()
         _check(lib().symamd_ilu0_analyze(self.h._h, self.n, _ptr(Ap), _ptr(Ai), _ptr(diags),
                                          byref(self._p)))
+        _live["plan"].add(self)
 
     def factor(self, Ax, out=None, strict=True):
         import torch
@@ -587,12 +641,14 @@ class Ilu0Plan:
         _check(rc)
         return out if strict else (out, -1)
 
+    def close(self):
+        if getattr(self, "_p", None) and _lib is not None:
+            _lib.symamd_ilu0_destroy(self._p)
+        self._p = None
+
     def __del__(self):
-        try:
-            if self._p:
-                lib().symamd_ilu0_destroy(self._p)
-        except Exception:
-            pass
+        if not _closing:
+            self.close()
 
 
 class SupernodalPlan:
@@ -612,6 +668,7 @@ class SupernodalPlan:
         _check(lib().symamd_sn_analyze(self.h._h, self.n, m,
                                        *(a.ctypes.data_as(c_void_p) for a in arrs),
                                        byref(self._p)))
+        _live["plan"].add(self)
 
     def solve(self, Lx, x):
         """x: (n,) or a column-major block (k, n) (row c = column c of X),
@@ -631,12 +688,14 @@ class SupernodalPlan:
     def check(self):
         _check(lib().symamd_sn_check(self.h._h, self._p))
 
+    def close(self):
+        if getattr(self, "_p", None) and _lib is not None:
+            _lib.symamd_sn_destroy(self._p)
+        self._p = None
+
     def __del__(self):
-        try:
-            if self._p:
-                lib().symamd_sn_destroy(self._p)
-        except Exception:
-            pass
+        if not _closing:
+            self.close()
 
 
 def spmv_csr(Ap, Ai, Ax, x, y=None, z=None, a=None, b=None,

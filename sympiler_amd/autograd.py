@@ -13,6 +13,12 @@ runs the hand-written adjoint kernel (symamd_adjoint):
 
 (from d x = -L^{-1} (dL) x).  FP64 and FP32 are supported; the plan's dtype
 must match the values'.
+
+One plan may serve several forwards with different values before their
+backwards run: each forward gives the plan its values and notes the plan's
+value generation; a backward that finds the generation moved on (another
+forward, or any update_values, ran since) gives the plan the values its own
+forward used (saved with the output) again.
 """
 from __future__ import annotations
 
@@ -36,6 +42,7 @@ class TriangularSolve(torch.autograd.Function):
             raise ValueError("Lx and b must have the plan's dtype")
         Lx = Lx.detach().contiguous()
         plan.update_values(Lx)
+        ctx.values_generation = plan.values_generation
         x = b.detach().contiguous().clone()
         if x.dim() == 1:
             plan.solve(x, executor=executor)
@@ -43,13 +50,15 @@ class TriangularSolve(torch.autograd.Function):
             plan.solve(x, layout="row", executor=executor)
         ctx.plan = plan
         ctx.executor = executor
-        ctx.save_for_backward(x)
+        ctx.save_for_backward(x, Lx)
         return x
 
     @staticmethod
     def backward(ctx, gx):
-        (x,) = ctx.saved_tensors
+        x, Lx = ctx.saved_tensors
         plan: Plan = ctx.plan
+        if plan.values_generation != ctx.values_generation:
+            plan.update_values(Lx)
         gx = gx.contiguous()
         gb = torch.empty_like(gx)
         gL = torch.empty(plan.nnz, dtype=gx.dtype, device=gx.device) \

@@ -261,112 +261,6 @@ int fail(int code, const char *fmt, ...) {
   return code;
 }
 
-// --------------------------------------------------------------- coarsen
-// Host implementation of the coarsened schedule.  Levels come from the GPU
-// inspector; `span` consecutive levels form one coarse level; inside it the
-// rows are grouped into the connected components of the dependency edges
-// that stay inside the coarse level (union-find), so no dependency crosses
-// two partitions of one coarse level; components are then packed in order
-// of their first row into partitions of cost (row nnz) at most max_cost.
-// Rows inside a partition are listed by (level, row id), which respects
-// every dependency inside it.
-int coarsen(symamd_handle h, int n, const int *d_Lp, const int *d_Li, int span,
-            int max_cost, int *d_level_ptr, int *d_par_ptr, int *d_partition,
-            int *level_no, int *part_no) {
-  if (span < 1) span = 1;
-  std::vector<int> Lp(n + 1), Li, lp, ls(n > 0 ? n : 1);
-  SYMAMD_HIP(hipMemcpy(Lp.data(), d_Lp, sizeof(int) * (n + 1), hipMemcpyDeviceToHost));
-  Li.resize(Lp[n] > 0 ? Lp[n] : 1);
-  SYMAMD_HIP(hipMemcpy(Li.data(), d_Li, sizeof(int) * Lp[n], hipMemcpyDeviceToHost));
-  int *d_state = nullptr, *d_lp = nullptr, *d_ls = nullptr;
-  SYMAMD_HIP(hipMalloc(&d_state, sizeof(int) * ST_WORDS));
-  SYMAMD_HIP(hipMalloc(&d_lp, sizeof(int) * (n + 1)));
-  SYMAMD_HIP(hipMalloc(&d_ls, sizeof(int) * (n > 0 ? n : 1)));
-  int levels = 0;
-  int rc = check_structure(h, n, d_Lp, d_Li, d_state);
-  if (rc == SYMAMD_OK)
-    rc = compute_levels(h, n, d_Lp, d_Li, d_state, d_lp, d_ls, &levels, &lp);
-  if (rc == SYMAMD_OK && n > 0)
-    if (hipMemcpy(ls.data(), d_ls, sizeof(int) * n, hipMemcpyDeviceToHost) != hipSuccess)
-      rc = fail(SYMAMD_ERR_HIP, "copy of level set failed");
-  (void)hipFree(d_state);
-  (void)hipFree(d_lp);
-  (void)hipFree(d_ls);
-  if (rc != SYMAMD_OK) return rc;
-  std::vector<int> lev(n > 0 ? n : 1), parent(n > 0 ? n : 1);
-  for (int l = 0; l < levels; ++l)
-    for (int q = lp[l]; q < lp[l + 1]; ++q) lev[ls[q]] = l;
-  auto find = [&](int x) {
-    while (parent[x] != x) {
-      parent[x] = parent[parent[x]];
-      x = parent[x];
-    }
-    return x;
-  };
-  for (int i = 0; i < n; ++i) parent[i] = i;
-  for (int i = 0; i < n; ++i)
-    for (int j = Lp[i]; j < Lp[i + 1] - 1; ++j) {
-      int d = Li[j];
-      if (lev[d] / span == lev[i] / span) {
-        int a = find(i), b = find(d);
-        if (a != b) parent[std::max(a, b)] = std::min(a, b);
-      }
-    }
-  const int ncoarse = (levels + span - 1) / span;
-  std::vector<int> out_lp(1, 0), out_pp(1, 0), out_part;
-  out_part.reserve(n);
-  std::vector<int> comp_slot(n > 0 ? n : 1, -1);
-  for (int cl = 0; cl < ncoarse; ++cl) {
-    const int q0 = lp[cl * span], q1 = lp[std::min(levels, (cl + 1) * span)];
-    // components in order of first appearance (level order)
-    std::vector<int> roots;
-    std::vector<std::vector<int>> members;
-    std::vector<long long> cost;
-    for (int q = q0; q < q1; ++q) {
-      int i = ls[q], r = find(i);
-      if (comp_slot[r] < 0) {
-        comp_slot[r] = (int)roots.size();
-        roots.push_back(r);
-        members.emplace_back();
-        cost.push_back(0);
-      }
-      int s = comp_slot[r];
-      members[s].push_back(i);  // ls is (level, row) ordered already
-      cost[s] += Lp[i + 1] - Lp[i];
-    }
-    int parts_here = 0;
-    long long acc = 0;
-    std::vector<int> cur;
-    auto flush = [&]() {
-      if (cur.empty()) return;
-      // merged components keep (level, row id) order
-      std::stable_sort(cur.begin(), cur.end(), [&](int a, int b) {
-        return lev[a] != lev[b] ? lev[a] < lev[b] : a < b;
-      });
-      out_part.insert(out_part.end(), cur.begin(), cur.end());
-      out_pp.push_back((int)out_part.size());
-      ++parts_here;
-      cur.clear();
-      acc = 0;
-    };
-    for (size_t s = 0; s < roots.size(); ++s) {
-      if (!cur.empty() && max_cost > 0 && acc + cost[s] > max_cost) flush();
-      cur.insert(cur.end(), members[s].begin(), members[s].end());
-      acc += cost[s];
-    }
-    flush();
-    for (int r : roots) comp_slot[r] = -1;
-    out_lp.push_back(out_lp.back() + parts_here);
-  }
-  *level_no = ncoarse;
-  *part_no = out_lp.back();
-  SYMAMD_HIP(hipMemcpy(d_level_ptr, out_lp.data(), sizeof(int) * out_lp.size(), hipMemcpyHostToDevice));
-  SYMAMD_HIP(hipMemcpy(d_par_ptr, out_pp.data(), sizeof(int) * out_pp.size(), hipMemcpyHostToDevice));
-  if (n > 0)
-    SYMAMD_HIP(hipMemcpy(d_partition, out_part.data(), sizeof(int) * n, hipMemcpyHostToDevice));
-  return SYMAMD_OK;
-}
-
 namespace {
 
 std::string graph_key(void *d_B, int k, int64_t rs, int64_t cs, const Choice &c) {
@@ -529,6 +423,11 @@ int symamd_set_option(symamd_handle h, int option, int64_t value) {
     case SYMAMD_OPT_OWNERS:
       h->owners = value != 0;
       return SYMAMD_OK;
+    case SYMAMD_OPT_REGION_BAND:
+      if (value < 0 || value > 255)
+        return fail(SYMAMD_ERR_INVALID_ARG, "region band: 0..255 slices");
+      h->region_band = (int)value;
+      return SYMAMD_OK;
     case SYMAMD_OPT_REGION_WAVES:
       if (value != 0 && value != 1 && value != 2 && value != 4)
         return fail(SYMAMD_ERR_INVALID_ARG, "region waves: 0, 1, 2 or 4");
@@ -581,15 +480,15 @@ int symamd_levelset(symamd_handle h, int n, const int *d_Lp, const int *d_Li,
 }
 
 int symamd_coarsen(symamd_handle h, int n, const int *d_Lp, const int *d_Li,
-                   int span, int max_part_cost, int *d_level_ptr,
-                   int *d_par_ptr, int *d_partition, int *level_no,
-                   int *part_no) {
+                   const double *d_cost, int inner_parts, int min_level_dist,
+                   int div_rate, int *d_level_ptr, int *d_par_ptr, int *d_partition,
+                   int *level_no, int *part_no) {
   SYMAMD_TRY(check_handle(h));
   if (n < 0 || !level_no || !part_no || !d_Lp || !d_level_ptr || !d_par_ptr ||
       (n > 0 && (!d_Li || !d_partition)))
     return fail(SYMAMD_ERR_INVALID_ARG, "symamd_coarsen: bad arguments");
-  return coarsen(h, n, d_Lp, d_Li, span, max_part_cost, d_level_ptr, d_par_ptr,
-                 d_partition, level_no, part_no);
+  return lbc_schedule(h, n, d_Lp, d_Li, d_cost, inner_parts, min_level_dist, div_rate,
+                      d_level_ptr, d_par_ptr, d_partition, level_no, part_no);
 }
 
 int symamd_analyze(symamd_handle h, int n, const int *d_Lp, const int *d_Li,

@@ -162,7 +162,10 @@ struct RegionPlan {
   int *d_vsrc = nullptr;
   int *d_vdst = nullptr;
   int64_t nvals = 0;
-  int *d_rflags = nullptr;  // task ready tags, MAX_GROUPS per region
+  // task progress tags, MAX_GROUPS per region: (launch target << 8) | bands
+  // published, so a value from an earlier launch never satisfies a wait
+  int64_t *d_rflags = nullptr;
+  int band = 0;             // slices per band the schedule was built with
   bool transposed = false;  // schedule of L^T x = g (values from the transposed copy)
 };
 
@@ -179,10 +182,11 @@ struct RegionPlan {
 // padding product is +0.0 * +0.0 and subtracting it leaves any value
 // unchanged, bit for bit.
 struct RecLayout {
-  int pval, diag, lval, code, cnt, rows, halo, slices, deps, lep, lcode, bytes;
+  int pval, diag, lval, code, cnt, rows, halo, slices, bands, hband, wptr, deps, lep, lcode,
+      bytes;
 };
 __host__ __device__ inline RecLayout rec_layout(int nr, int nh, int ne, int ns, int nd,
-                                                int vs) {
+                                                int nb, int vs) {
   constexpr int PK = RegionPlan::PACK;
   RecLayout L;
   int o = 0;
@@ -198,6 +202,9 @@ __host__ __device__ inline RecLayout rec_layout(int nr, int nh, int ne, int ns, 
   put(L.rows, nr * 4);
   put(L.halo, nh * 4);
   put(L.slices, (ns + 1) * 4);
+  put(L.bands, (nb + 1) * 4);
+  put(L.hband, (nb + 1) * 4);
+  put(L.wptr, (nb + 1) * 4);
   put(L.deps, nd * 4);
   put(L.lep, ne > 0 ? (nr + 1) * 4 : 0);
   put(L.lcode, ne * 2);
@@ -207,6 +214,12 @@ __host__ __device__ inline RecLayout rec_layout(int nr, int nh, int ne, int ns, 
 // Slot codes are uint16: a region with its halo and the zero slot may use
 // at most this many LDS slots.
 constexpr int REGION_SLOTS_MAX = 65535;
+// A region publishes its progress band by band (a band is a few consecutive
+// slices, at most REGION_BANDS_MAX per region).  A wait entry packs the rank
+// of the region read (low REGION_RANK_BITS bits) with the number of its
+// bands that must be published (the high bits).
+constexpr int REGION_BANDS_MAX = 31;
+constexpr int REGION_RANK_BITS = 26;
 
 // Host copy of a region schedule, before packing into records.  Rows are
 // region-major in rank order; entries of every row: the first PACK packed
@@ -220,10 +233,15 @@ struct HostRegions {
   double inside = 0;
   std::vector<int> rptr, rows, sptr, slices, hptr, halo, dptr, dlist, eptr, ecode, esrc,
       dsrc, pcnt, pcode, psrc;
+  // bands: per region nb+1 entries at bptr[r]: the first slice of each band
+  // (bands), its first halo row (hband, halo sorted by the band that first
+  // reads it) and its first wait entry (wptr, into the region's dlist,
+  // whose entries are rank | bands_needed << REGION_RANK_BITS)
+  std::vector<int> bptr, bands, hband, wptr;
 };
 bool build_region_schedule(int n, int64_t nnz, int levels, const int *lvl_ptr,
                            const int *perm, const int *crp, const int *cci, int cap,
-                           HostRegions &out);
+                           int band_slices, HostRegions &out);
 // Builder inputs of the transposed solve from the plan's level sets (perm,
 // lvl_ptr) and transposed copy (trp, tci: column perm[q] of L at position q,
 // diagonal first): rows and positions numbered backwards, entries in
@@ -268,6 +286,7 @@ struct symamd_handle_s {
   int owners = 1;           // solve plans with regions on the owner schedule
   int region_cap = 0;       // region executor: region size cap (0: by k)
   int region_waves = 0;     // waves per region task (0: chosen from k)
+  int region_band = 0;      // slices per published band (0: the default)
   uint64_t timeout_ticks() const { return (uint64_t)(timeout_ms * 1.0e5); }
 };
 
@@ -306,8 +325,9 @@ struct symamd_plan_s {
   // persistent-kernel state
   int *d_state = nullptr;              // ST_WORDS ints
   std::map<int, symamd::UnitSet> units;  // rows-per-unit -> units
-  std::map<int, symamd::RegionPlan> rplans;  // region cap -> region schedule
-  std::map<int, symamd::RegionPlan> rplans_t;  // same, transposed solve
+  // region schedules keyed by cap * 256 + slices per band
+  std::map<int64_t, symamd::RegionPlan> rplans;
+  std::map<int64_t, symamd::RegionPlan> rplans_t;  // same, transposed solve
   // IC(0): update pairs per variant, and the failed-pivot word
   std::map<int, symamd::Ic0Pairs> ic0;
   int *d_ic0_bad = nullptr;
@@ -395,9 +415,11 @@ int forward_rows_per_unit(int k, int lanes);
 int prepare_workspace(symamd_handle h, symamd_plan p, int k);
 int launch_forward_ws(symamd_handle h, symamd_plan p, void *d_B, int k, int64_t ld,
                       int exec, int lanes);
-int coarsen(symamd_handle h, int n, const int *d_Lp, const int *d_Li,
-            int span, int max_cost, int *d_level_ptr, int *d_par_ptr,
-            int *d_partition, int *level_no, int *part_no);
+// Coarsened 3-array schedule on the device (lbc.hip).
+int lbc_schedule(symamd_handle h, int n, const int *d_Lp, const int *d_Li,
+                 const double *d_cost, int parts, int min_level_dist, int div_rate,
+                 int *d_level_ptr, int *d_par_ptr, int *d_partition, int *level_no,
+                 int *part_no);
 // d_b == nullptr: sptrsv_csr_lbc arithmetic in place on d_x; else the
 // sptrsv_csr_lbc_redundant arithmetic with right-hand side d_b.
 int launch_lbc(symamd_handle h, int n, const int *d_Lp, const int *d_Li,
@@ -446,6 +468,7 @@ int launch_grad_l(symamd_handle h, symamd_plan p, const void *d_X, const void *d
 // the region's and its halo's values of one column group per row), unless
 // the handle sets one.
 int region_cap(symamd_handle h, int k);
+int region_band(symamd_handle h);
 // True when a region schedule for k right-hand sides exists (building it on
 // first use) and its buffers fit one wave's LDS.
 bool region_usable(symamd_handle h, symamd_plan p, int k);

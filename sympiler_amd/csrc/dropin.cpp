@@ -453,15 +453,30 @@ void sptrsv_csc_levelset_block(int n, const int *Lp, const int *Li, const int *n
                                double *Lx, double *x, const int *levelPtr, const int *levels,
                                int n_lev, const int *supernodes, const int *sup2node,
                                double **tempvecs) {
-  (void)levelPtr;
   (void)tempvecs;
-  // the supernode count: sup2node names every supernode once
-  int num_nodes = 0;
-  if (n_lev > 0)
-    for (int j = levelPtr[0]; j < levelPtr[n_lev]; ++j)
-      num_nodes = std::max(num_nodes, sup2node[j] + 1);
-  (void)levels;
-  sptrsv_csc_block(n, Lp, Li, nrows, Lx, x, num_nodes, supernodes);
+  if (n_lev < 0 || (n_lev > 0 && (!levelPtr || !levels || !sup2node || !supernodes)))
+    throw std::invalid_argument("sptrsv_csc_levelset_block: bad schedule arguments");
+  if (n_lev == 0) return;
+  // the schedule must name supernodes 0 .. m-1 once each (levels[j] being
+  // supernode sup2node[j]'s first column); the serial order then gives the
+  // result of any valid level schedule
+  const int j0 = levelPtr[0], j1 = levelPtr[n_lev];
+  if (j0 < 0 || j1 < j0)
+    throw std::invalid_argument("sptrsv_csc_levelset_block: bad level pointers");
+  const int m = j1 - j0;
+  std::vector<char> seen(m > 0 ? m : 1, 0);
+  for (int i = 0; i < n_lev; ++i)
+    if (levelPtr[i + 1] < levelPtr[i])
+      throw std::invalid_argument("sptrsv_csc_levelset_block: bad level pointers");
+  for (int j = j0; j < j1; ++j) {
+    const int s = sup2node[j];
+    if (s < 0 || s >= m || seen[s] || levels[j] != supernodes[s])
+      throw std::invalid_argument(
+          "sptrsv_csc_levelset_block: the schedule must list every supernode once, "
+          "with levels[j] == supernodes[sup2node[j]]");
+    seen[s] = 1;
+  }
+  sptrsv_csc_block(n, Lp, Li, nrows, Lx, x, m, supernodes);
 }
 
 int build_levelSet_CSC(int n, const int *Lp, const int *Li, int *&level_ptr,
@@ -481,6 +496,37 @@ int build_levelSet_CSC(int n, const int *Lp, const int *Li, int *&level_ptr,
   dlp.download(level_ptr, levels + 1);
   dls.download(level_set, n);
   return levels;
+}
+
+// The coarsened-schedule inspector of the un-vendored aggregation submodule,
+// with the arguments its call site passes (demo/sptrsv_demo_utils.h:83-88:
+// the CSR arrays of L, its storage type, the three schedule outputs, then
+// lp / cp / ic and the per-row cost).  The schedule is built on the device
+// (symamd_coarsen); stype is accepted and not read: the arrays are the CSR
+// of the lower triangle, as that call site passes them.  The three arrays
+// are new[]'d for the caller to delete[].  Returns 1.
+int get_coarse_levelSet_DAG_CSC_tree(size_t n, const int *lC, const int *lR, int stype,
+                                     int &finaLevelNo, int *&finaLevelPtr, int &partNo,
+                                     int *&finalPartPtr, int *&finalNodePtr, int innerParts,
+                                     int minLevelDist, int divRate, double *nodeCost) {
+  (void)stype;
+  if (n > (size_t)0x7ffffffe)
+    throw std::invalid_argument("get_coarse_levelSet_DAG_CSC_tree: n too large");
+  const int nn = (int)n;
+  symamd_handle h = handle();
+  DevBuf<int> dLp(nn + 1, lC), dLi(nn > 0 ? lC[nn] : 0, lR), dlp(nn + 1), dpp(nn + 1),
+      dpart(nn);
+  DevBuf<double> dcost(nodeCost ? nn : 0, nodeCost);
+  check(symamd_coarsen(h, nn, dLp.p, dLi.p, nodeCost ? dcost.p : nullptr, innerParts,
+                       minLevelDist, divRate, dlp.p, dpp.p, dpart.p, &finaLevelNo, &partNo),
+        "get_coarse_levelSet_DAG_CSC_tree");
+  finaLevelPtr = new int[nn + 1];
+  finalPartPtr = new int[nn + 1];
+  finalNodePtr = new int[nn > 0 ? nn : 1];
+  dlp.download(finaLevelPtr, finaLevelNo + 1);
+  dpp.download(finalPartPtr, partNo + 1);
+  dpart.download(finalNodePtr, nn);
+  return 1;
 }
 
 namespace parsy {

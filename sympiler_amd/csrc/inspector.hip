@@ -13,6 +13,9 @@
 #include <rocprim/device/device_scan.hpp>
 #include <rocprim/iterator/counting_iterator.hpp>
 
+#include <map>
+#include <mutex>
+
 #include "common.h"
 
 namespace symamd {
@@ -387,10 +390,82 @@ int ilu_lower(symamd_handle h, int n, const int *d_Ap, const int *d_Ai,
   return rc;
 }
 
+namespace {
+
+// v[0 .. m) = value (a kernel, not a memset node, so callers may capture it).
+__global__ void k_fill_int(int64_t m, int value, int *v) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < m) v[t] = value;
+}
+
+// Levels and level sets with the caller's buffers: lev and skey of n ints,
+// the sort's temporary storage tmp of tmp_bytes (nullptr: allocated here).
+int levels_core(symamd_handle h, int n, const int *d_Lp, const int *d_Li, int *d_state,
+                int *d_lev, int *d_skey, void *tmp, size_t tmp_bytes, int *d_level_ptr,
+                int *d_level_set, int *levels, std::vector<int> *h_level_ptr) {
+  *levels = 0;
+  if (n == 0) {
+    k_fill_int<<<1, 64, 0, h->stream>>>(1, 0, d_level_ptr);
+    SYMAMD_HIP(hipGetLastError());
+    SYMAMD_HIP(hipStreamSynchronize(h->stream));
+    if (h_level_ptr) h_level_ptr->assign(1, 0);
+    return SYMAMD_OK;
+  }
+  k_fill_int<<<blocks_for(n), 256, 0, h->stream>>>(n, -1, d_lev);
+  k_fill_int<<<1, 64, 0, h->stream>>>(ST_WORDS, 0, d_state);
+  int waves_needed = (n + 63) / 64;
+  int grid = (waves_needed + 3) / 4;
+  int cap = h->num_cus * 8;
+  if (grid > cap) grid = cap;
+  k_levels<<<grid, 256, 0, h->stream>>>(n, d_Lp, d_Li, d_lev, d_state,
+                                        h->timeout_ticks());
+  SYMAMD_HIP(hipGetLastError());
+  int st[ST_WORDS];
+  SYMAMD_HIP(hipMemcpyAsync(st, d_state, sizeof(st), hipMemcpyDeviceToHost,
+                            h->stream));
+  SYMAMD_HIP(hipStreamSynchronize(h->stream));
+  if (st[ST_ERROR]) return fail(SYMAMD_ERR_TIMEOUT, "level inspector watchdog expired");
+  const int nlev = st[ST_SCRATCH] + 1;
+  unsigned end_bit = 1;
+  while ((1u << end_bit) <= (unsigned)(nlev - 1) && end_bit < 31) ++end_bit;
+  size_t bytes = 0;
+  rocprim::counting_iterator<int> rows(0);
+  SYMAMD_HIP(rocprim::radix_sort_pairs(nullptr, bytes, (unsigned *)d_lev,
+                                       (unsigned *)d_skey, rows, d_level_set,
+                                       (size_t)n, 0, end_bit, h->stream));
+  void *own = nullptr;
+  if (!tmp) {
+    SYMAMD_HIP(hipMallocAsync(&own, bytes ? bytes : 4, h->stream));
+    tmp = own;
+  } else if (bytes > tmp_bytes) {
+    return fail(SYMAMD_ERR_NOT_SUPPORTED,
+                "level sets of %d rows need %zu bytes of sort storage, more than the %zu "
+                "of the fixed buffer", n, bytes, tmp_bytes);
+  }
+  SYMAMD_HIP(rocprim::radix_sort_pairs(tmp, bytes, (unsigned *)d_lev,
+                                       (unsigned *)d_skey, rows, d_level_set,
+                                       (size_t)n, 0, end_bit, h->stream));
+  k_level_ptr<<<blocks_for(n), 256, 0, h->stream>>>(n, nlev, d_skey,
+                                                    d_level_ptr);
+  SYMAMD_HIP(hipGetLastError());
+  if (own) SYMAMD_HIP(hipFreeAsync(own, h->stream));
+  if (h_level_ptr) {
+    h_level_ptr->resize(nlev + 1);
+    SYMAMD_HIP(hipMemcpyAsync(h_level_ptr->data(), d_level_ptr,
+                              sizeof(int) * (nlev + 1), hipMemcpyDeviceToHost,
+                              h->stream));
+  }
+  SYMAMD_HIP(hipStreamSynchronize(h->stream));
+  *levels = nlev;
+  return SYMAMD_OK;
+}
+
+}  // namespace
+
 int check_structure(symamd_handle h, int n, const int *d_Lp, const int *d_Li,
                     int *d_state) {
   if (n == 0) return SYMAMD_OK;
-  SYMAMD_HIP(hipMemsetAsync(d_state + ST_SCRATCH, 0, sizeof(int), h->stream));
+  k_fill_int<<<1, 64, 0, h->stream>>>(1, 0, d_state + ST_SCRATCH);
   k_check_structure<<<blocks_for(n), 256, 0, h->stream>>>(n, d_Lp, d_Li,
                                                           d_state + ST_SCRATCH);
   SYMAMD_HIP(hipGetLastError());
@@ -407,64 +482,17 @@ int check_structure(symamd_handle h, int n, const int *d_Lp, const int *d_Li,
 int compute_levels(symamd_handle h, int n, const int *d_Lp, const int *d_Li,
                    int *d_state, int *d_level_ptr, int *d_level_set,
                    int *levels, std::vector<int> *h_level_ptr) {
-  *levels = 0;
-  if (n == 0) {
-    int zero = 0;
-    SYMAMD_HIP(hipMemcpyAsync(d_level_ptr, &zero, sizeof(int),
-                              hipMemcpyHostToDevice, h->stream));
-    SYMAMD_HIP(hipStreamSynchronize(h->stream));
-    if (h_level_ptr) h_level_ptr->assign(1, 0);
-    return SYMAMD_OK;
-  }
   int *d_lev = nullptr, *d_skey = nullptr;
-  SYMAMD_HIP(hipMallocAsync(&d_lev, sizeof(int) * n, h->stream));
-  SYMAMD_HIP(hipMallocAsync(&d_skey, sizeof(int) * n, h->stream));
-  SYMAMD_HIP(hipMemsetAsync(d_lev, 0xff, sizeof(int) * n, h->stream));
-  SYMAMD_HIP(hipMemsetAsync(d_state, 0, sizeof(int) * ST_WORDS, h->stream));
-  int waves_needed = (n + 63) / 64;
-  int grid = (waves_needed + 3) / 4;
-  int cap = h->num_cus * 8;
-  if (grid > cap) grid = cap;
-  k_levels<<<grid, 256, 0, h->stream>>>(n, d_Lp, d_Li, d_lev, d_state,
-                                        h->timeout_ticks());
-  SYMAMD_HIP(hipGetLastError());
-  int st[ST_WORDS];
-  SYMAMD_HIP(hipMemcpyAsync(st, d_state, sizeof(st), hipMemcpyDeviceToHost,
-                            h->stream));
-  SYMAMD_HIP(hipStreamSynchronize(h->stream));
-  if (st[ST_ERROR]) {
-    (void)hipFreeAsync(d_lev, h->stream);
-    (void)hipFreeAsync(d_skey, h->stream);
-    return fail(SYMAMD_ERR_TIMEOUT, "level inspector watchdog expired");
+  if (n > 0) {
+    SYMAMD_HIP(hipMallocAsync(&d_lev, sizeof(int) * n, h->stream));
+    SYMAMD_HIP(hipMallocAsync(&d_skey, sizeof(int) * n, h->stream));
   }
-  const int nlev = st[ST_SCRATCH] + 1;
-  unsigned end_bit = 1;
-  while ((1u << end_bit) <= (unsigned)(nlev - 1) && end_bit < 31) ++end_bit;
-  size_t bytes = 0;
-  rocprim::counting_iterator<int> rows(0);
-  SYMAMD_HIP(rocprim::radix_sort_pairs(nullptr, bytes, (unsigned *)d_lev,
-                                       (unsigned *)d_skey, rows, d_level_set,
-                                       (size_t)n, 0, end_bit, h->stream));
-  void *tmp = nullptr;
-  SYMAMD_HIP(hipMallocAsync(&tmp, bytes ? bytes : 4, h->stream));
-  SYMAMD_HIP(rocprim::radix_sort_pairs(tmp, bytes, (unsigned *)d_lev,
-                                       (unsigned *)d_skey, rows, d_level_set,
-                                       (size_t)n, 0, end_bit, h->stream));
-  k_level_ptr<<<blocks_for(n), 256, 0, h->stream>>>(n, nlev, d_skey,
-                                                    d_level_ptr);
-  SYMAMD_HIP(hipGetLastError());
-  SYMAMD_HIP(hipFreeAsync(tmp, h->stream));
-  SYMAMD_HIP(hipFreeAsync(d_lev, h->stream));
-  SYMAMD_HIP(hipFreeAsync(d_skey, h->stream));
-  if (h_level_ptr) {
-    h_level_ptr->resize(nlev + 1);
-    SYMAMD_HIP(hipMemcpyAsync(h_level_ptr->data(), d_level_ptr,
-                              sizeof(int) * (nlev + 1), hipMemcpyDeviceToHost,
-                              h->stream));
-  }
-  SYMAMD_HIP(hipStreamSynchronize(h->stream));
-  *levels = nlev;
-  return SYMAMD_OK;
+  const int rc = levels_core(h, n, d_Lp, d_Li, d_state, d_lev, d_skey, nullptr, 0, d_level_ptr,
+                             d_level_set, levels, h_level_ptr);
+  if (d_lev) (void)hipFreeAsync(d_lev, h->stream);
+  if (d_skey) (void)hipFreeAsync(d_skey, h->stream);
+  if (n > 0) SYMAMD_HIP(hipStreamSynchronize(h->stream));
+  return rc;
 }
 
 int build_plan_arrays(symamd_handle h, symamd_plan p, const int *d_Lp,

@@ -62,7 +62,7 @@ struct RegionArgs {
   int n;           // rows: row ids read from LDS are clamped below n
   const uint8_t *rec;  // region records
   const int *head;     // 8 ints per region (RegionPlan::d_rhead)
-  int *tflags;         // task tags: region rank * groups + group
+  int64_t *tflags;     // task progress: region rank * groups + group
   int *state;
   int R, groups, smax, hmax;
   int xsb;  // LDS bytes of the value slots; the record image follows
@@ -314,7 +314,7 @@ __device__ __forceinline__ void solve_pairs(Pairs<T, CPL> (&m)[P], T *xs, const 
 // otherwise those beyond lane 63 from LDS.  Bounds are clamped to [0, nr].
 template <typename T, int KT, int CPL, int P, bool SHORT, int NW>
 __device__ __forceinline__ void solve_slices(T *xs, const uint8_t *rb, const RecLayout &L,
-                                             int nr, int ns, int ne, int kc) {
+                                             int nr, int ns, int sb, int se, int ne, int kc) {
   const int lane = threadIdx.x & 63;
   const T *pval = reinterpret_cast<const T *>(rb + L.pval);
   const T *ldiag = reinterpret_cast<const T *>(rb + L.diag);
@@ -342,16 +342,16 @@ __device__ __forceinline__ void solve_slices(T *xs, const uint8_t *rb, const Rec
   const int off = SPLIT ? lane * KT + ((int)threadIdx.x >> 6) * CPL : (int)threadIdx.x * CPL;
   constexpr int STEP = SPLIT ? 64 * KT : 64 * NW * CPL;  // slots per pass of the task
   Pairs<T, CPL> cur[P], nxt[P];
-  int b0 = bound(0), b1 = ns > 0 ? bound(1) : 0;
-  if (ns > 0) {
+  int b0 = bound(sb), b1 = se > sb ? bound(sb + 1) : 0;
+  if (se > sb) {
 #pragma unroll
     for (int p = 0; p < P; ++p)
       load_pairs<T, KT, CPL>(cur[p], b0 * KT + off + p * STEP, b1 * KT, kc, xs, cnt8, code,
                              pval, ldiag);
   }
-  for (int s = 0; s < ns; ++s) {
+  for (int s = sb; s < se; ++s) {
     // after the last slice b2 == b1: the loads are harmless and unused
-    const int b2 = s + 1 < ns ? bound(s + 2) : b1;
+    const int b2 = s + 1 < se ? bound(s + 2) : b1;
 #pragma unroll
     for (int p = 0; p < P; ++p)
       load_pairs<T, KT, CPL>(nxt[p], b1 * KT + off + p * STEP, b2 * KT, kc, xs, cnt8, code,
@@ -368,6 +368,40 @@ __device__ __forceinline__ void solve_slices(T *xs, const uint8_t *rb, const Rec
     b0 = b1;
     b1 = b2;
   }
+}
+
+// Wait until every entry of list[0 .. count) is satisfied: entry e names
+// the region of rank e & (2^REGION_RANK_BITS - 1), whose task of column
+// group g must have published e >> REGION_RANK_BITS bands in this launch
+// (flags[rank * groups + g] >= base + bands).  One lane per entry, one
+// coherent load each; a wait longer than `timeout` ticks records an error
+// and gives up.  Ranks outside [0, R) are read as rank 0.
+__device__ __forceinline__ void wait_bands(const int *list, int count, const int64_t *flags,
+                                           int64_t base, int *state, uint64_t timeout,
+                                           int groups, int g, int R) {
+  const int lane = threadIdx.x & 63;
+  for (int b0 = 0; b0 < count; b0 += 64) {
+    const int t = b0 + lane;
+    bool pending = t < count;
+    const int e = pending ? list[t] : 0;
+    int rank = e & ((1 << REGION_RANK_BITS) - 1);
+    rank = rank < R ? rank : 0;
+    const int64_t want = base + (e >> REGION_RANK_BITS);
+    const int64_t *f = &flags[(int64_t)rank * groups + g];
+    if (pending) pending = ld_dev(f) < want;
+    if (!__any(pending)) continue;
+    const uint64_t t0 = wall_ticks();
+    int rounds = 0;
+    while (__any(pending)) {
+      __builtin_amdgcn_s_sleep(1);
+      if (pending) pending = ld_dev(f) < want;
+      if ((++rounds & 63) == 0 && wall_ticks() - t0 > timeout) {
+        if (pending) atomicOr(&state[ST_ERROR], 1);
+        pending = false;
+      }
+    }
+  }
+  asm volatile("" ::: "memory");
 }
 
 __device__ __forceinline__ void load_head(const int *head, int r, int4 &h0, int4 &h1) {
@@ -401,6 +435,7 @@ __global__ __launch_bounds__(64 * NW) void k_regions(RegionArgs<T> a) {
   const int lane = tid & 63;
   const int wid = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int target = ld_dev(&a.state[ST_EPOCH]) + 1;
+  const int64_t base = (int64_t)target << 8;  // progress of this launch
   const int ntasks = a.R * a.groups;
   // the zero slot, never written afterwards (rows and halo lie below it)
   if (tid < KT) xs[(size_t)(a.smax + a.hmax) * KT + tid] = T(0);
@@ -428,55 +463,77 @@ __global__ __launch_bounds__(64 * NW) void k_regions(RegionArgs<T> a) {
     const int ne = __builtin_amdgcn_readfirstlane(h0.z);
     const int ns = __builtin_amdgcn_readfirstlane(h0.w);
     const int nd = __builtin_amdgcn_readfirstlane(h1.x);
-    const RecLayout L = rec_layout(nr, nh, ne, ns, nd, VS);
+    const int nb = __builtin_amdgcn_readfirstlane(h1.w);
+    const RecLayout L = rec_layout(nr, nh, ne, ns, nd, nb, VS);
     const int *lrow = reinterpret_cast<const int *>(rb + L.rows);
     const int *lhalo = reinterpret_cast<const int *>(rb + L.halo);
     const int *ldep = reinterpret_cast<const int *>(rb + L.deps);
+    const int *lsl = reinterpret_cast<const int *>(rb + L.slices);
+    const int *lband = reinterpret_cast<const int *>(rb + L.bands);
+    const int *lhb = reinterpret_cast<const int *>(rb + L.hband);
+    const int *lwp = reinterpret_cast<const int *>(rb + L.wptr);
+    int64_t *myflag = &a.tflags[task];
     // 1. record, then the right-hand sides of its rows
     dma_bytes<NW>(rb, a.rec + (size_t)__builtin_amdgcn_readfirstlane(h1.y) * 16,
                   __builtin_amdgcn_readfirstlane(h1.z) * 16, wid);
     vm_wait();
     if constexpr (NW > 1) __syncthreads();
     load_rows<T, KT, ES, false, NW>(xs, a.B, lrow, nr, kc, c0, a.rs, a.cs, a.n, wid);
-    // 2. inputs published (same column group of the regions read)
-    if (wid == 0) wait_list(ldep, nd, a.tflags, target, a.state, a.timeout, a.groups, g, a.R);
-    if constexpr (NW > 1) __syncthreads();
-    if (wid == 0) trace_mark(a.trace, a.trace_cap, task, 1);
-    // 3. halo
-    load_rows<T, KT, ES, true, NW>(xs + (size_t)a.smax * KT, a.X, lhalo, nh, kc, c0, a.rs,
-                                   a.cs, a.n, wid);
-    vm_wait();
-    if constexpr (NW == 1)
-      wave_lds_sync();
-    else
-      __syncthreads();
-    // 4. internal levels; then every wave's results are visible to all (the
-    // stores read every column)
-    if (ns <= 62)
-      solve_slices<T, KT, CPL, P, true, NW>(xs, rb, L, nr, ns, ne, kc);
-    else
-      solve_slices<T, KT, CPL, P, false, NW>(xs, rb, L, nr, ns, ne, kc);
-    if constexpr (NW == 1)
-      wave_lds_sync();
-    else
-      __syncthreads();
-    // 5. store (row ids and values of STORE_U stores read first, one wait),
-    // fetch the next task's header, wait for both, publish
-    for (int t0 = tid; t0 < nr * KT; t0 += NT * STORE_U) {
-      int id[STORE_U];
-      T v[STORE_U];
+    // band by band: wait for the bands of other regions it reads, load its
+    // halo rows, publish the previous band (its stores are complete once the
+    // halo loads are), solve its slices, store its rows
+    for (int b = 0; b < nb; ++b) {
+      const int sb = __builtin_amdgcn_readfirstlane(lband[b]);
+      const int se = __builtin_amdgcn_readfirstlane(lband[b + 1]);
+      const int hb0 = __builtin_amdgcn_readfirstlane(lhb[b]);
+      const int hb1 = __builtin_amdgcn_readfirstlane(lhb[b + 1]);
+      const int wb0 = __builtin_amdgcn_readfirstlane(lwp[b]);
+      const int wb1 = __builtin_amdgcn_readfirstlane(lwp[b + 1]);
+      // 2. inputs published (same column group of the regions read)
+      if (wid == 0 && wb1 > wb0)
+        wait_bands(ldep + wb0, wb1 - wb0, a.tflags, base, a.state, a.timeout, a.groups, g, a.R);
+      if constexpr (NW > 1) __syncthreads();
+      if (b == 0 && wid == 0) trace_mark(a.trace, a.trace_cap, task, 1);
+      // 3. this band's halo rows
+      if (hb1 > hb0)
+        load_rows<T, KT, ES, true, NW>(xs + (size_t)(a.smax + hb0) * KT, a.X, lhalo + hb0,
+                                       hb1 - hb0, kc, c0, a.rs, a.cs, a.n, wid);
+      vm_wait();
+      if constexpr (NW == 1)
+        wave_lds_sync();
+      else
+        __syncthreads();
+      if (b > 0 && tid == 0) st_dev(myflag, base + b);
+      // 4. this band's slices; then every wave's results are visible to all
+      // (the stores read every column)
+      if (ns <= 62)
+        solve_slices<T, KT, CPL, P, true, NW>(xs, rb, L, nr, ns, sb, se, ne, kc);
+      else
+        solve_slices<T, KT, CPL, P, false, NW>(xs, rb, L, nr, ns, sb, se, ne, kc);
+      if constexpr (NW == 1)
+        wave_lds_sync();
+      else
+        __syncthreads();
+      // 5. store this band's rows (row ids and values of STORE_U stores read
+      // first, one wait)
+      const int r0 = min(max(lsl[min(max(sb, 0), ns)], 0), nr);
+      const int r1 = min(max(lsl[min(max(se, 0), ns)], 0), nr);
+      for (int t0 = r0 * KT + tid; t0 < r1 * KT; t0 += NT * STORE_U) {
+        int id[STORE_U];
+        T v[STORE_U];
 #pragma unroll
-      for (int u = 0; u < STORE_U; ++u) {
-        const int t = t0 + NT * u;
-        const bool in = t < nr * KT && (t & (KT - 1)) < kc;
-        id[u] = in ? lrow[t >> KS] : -1;
-        v[u] = xs[in ? t : 0];
-      }
+        for (int u = 0; u < STORE_U; ++u) {
+          const int t = t0 + NT * u;
+          const bool in = t < r1 * KT && (t & (KT - 1)) < kc;
+          id[u] = in ? lrow[t >> KS] : -1;
+          v[u] = xs[in ? t : 0];
+        }
 #pragma unroll
-      for (int u = 0; u < STORE_U; ++u) {
-        const int c = (t0 + NT * u) & (KT - 1);
-        if ((unsigned)id[u] < (unsigned)a.n)
-          st_dev(&a.X[(int64_t)id[u] * a.rs + (int64_t)(c0 + c) * a.cs], v[u]);
+        for (int u = 0; u < STORE_U; ++u) {
+          const int c = (t0 + NT * u) & (KT - 1);
+          if ((unsigned)id[u] < (unsigned)a.n)
+            st_dev(&a.X[(int64_t)id[u] * a.rs + (int64_t)(c0 + c) * a.cs], v[u]);
+        }
       }
     }
     int next;
@@ -498,7 +555,7 @@ __global__ __launch_bounds__(64 * NW) void k_regions(RegionArgs<T> a) {
       h0 = make_int4(s_next[1], s_next[2], s_next[3], s_next[4]);
       h1 = make_int4(s_next[5], s_next[6], s_next[7], s_next[8]);
     }
-    if (tid == 0) st_dev(&a.tflags[task], target);
+    if (tid == 0) st_dev(myflag, base + nb);
     if (wid == 0) trace_mark(a.trace, a.trace_cap, task, 2);
     task = next;
   }
@@ -629,6 +686,12 @@ int region_waves(symamd_handle h, int k) {
 int region_cap(symamd_handle h, int k) {
   if (h->region_cap > 0) return h->region_cap;
   return task_cols(k) == KG_MAX && region_waves(h, k) == 1 ? 256 : 512;
+}
+
+// Slices per published band (the handle's choice, else 2).
+int region_band(symamd_handle h) {
+  const int b = h->region_band > 0 ? h->region_band : 2;
+  return b < 255 ? b : 255;
 }
 
 bool region_preferred(symamd_handle h, symamd_plan p, int k, int layout) {

@@ -451,7 +451,8 @@ int make_acyclic(int n, const int *Lp, const int *Li, const std::vector<int> &le
 // make_acyclic; kept as a guard).
 bool build_region_schedule(int n, int64_t nnz, int levels, const int *lvl_ptr,
                            const int *perm, const int *crp_, const int *cci_, int cap,
-                           HostRegions &o) {
+                           int band_slices, HostRegions &o) {
+  if (band_slices < 1) band_slices = 1;
   const std::vector<int> crp(crp_, crp_ + n + 1), cci(cci_, cci_ + nnz);
   std::vector<int> ipos(n), lev(n), Lp(n + 1, 0), Li(nnz);
   for (int q = 0; q < n; ++q) ipos[perm[q]] = q;
@@ -538,39 +539,94 @@ bool build_region_schedule(int n, int64_t nnz, int levels, const int *lvl_ptr,
   for (int r = 0; r < R; ++r) smax = std::max(smax, rptr[r + 1] - rptr[r]);
   for (int r = 0; r < R; ++r)
     for (int t = rptr[r]; t < rptr[r + 1]; ++t) lidx[rows[t]] = t - rptr[r];
-  // slices, halos, entries, dependency lists (in ranks)
-  std::vector<int> sptr(R + 1, 0), slices, hptr(R + 1, 0), halo, dptr(R + 1, 0), dlist;
+  // slices and bands: band b of a region covers its slices
+  // [b*bs, (b+1)*bs) with bs = band_slices, or more when the region would
+  // have more than REGION_BANDS_MAX bands
+  std::vector<int> sptr(R + 1, 0), slices, bptr(R + 1, 0), bands, band_of(n);
+  int slmax = 0;
+  for (int r = 0; r < R; ++r) {
+    const int a = rptr[r], b = rptr[r + 1];
+    const size_t s0 = slices.size();
+    for (int t = a; t < b; ++t)
+      if (t == a || lev[rows[t]] != lev[rows[t - 1]]) slices.push_back(t - a);
+    slices.push_back(b - a);
+    sptr[r + 1] = (int)slices.size();
+    const int ns = sptr[r + 1] - sptr[r] - 1;
+    slmax = std::max(slmax, ns);
+    const int bs = std::max(band_slices, (ns + REGION_BANDS_MAX - 1) / REGION_BANDS_MAX);
+    const int nb = (ns + bs - 1) / bs;
+    for (int k = 0; k <= nb; ++k) bands.push_back(std::min(k * bs, ns));
+    bptr[r + 1] = (int)bands.size();
+    for (int sl = 0; sl < ns; ++sl)
+      for (int t = a + slices[s0 + sl]; t < a + slices[s0 + sl + 1]; ++t)
+        band_of[rows[t]] = sl / bs;
+  }
+  // halos (ordered by the band that first reads them, then row), entries,
+  // and the waits of each band: (rank of a region read, bands of it that
+  // must be published), emitted where a region's requirement first rises
+  std::vector<int> hptr(R + 1, 0), halo, hband, dptr(R + 1, 0), dlist, wptr;
   // entries: the first RegionPlan::PACK of every row packed per row (pcnt,
   // pcode, psrc; padding slots read the all-zero LDS slot smax + hmax, and
   // their values are zero), the rest in the overflow list (eptr, ecode, esrc)
   constexpr int PK = RegionPlan::PACK;
   std::vector<int> eptr(n + 1, 0), ecode, esrc, dsrc(n);
   std::vector<int> pcnt(n), pcode((size_t)PK * n, 0), psrc((size_t)PK * n, 0);
-  std::vector<int> hslot(n, -1), hstamp(n, -1);
-  int hmax = 0, emax = 0, slmax = 0;
+  std::vector<int> hslot(n, -1), hstamp(n, -1), first_use(n, 0), need_max(R, 0),
+      need_stamp(R, -1);
+  int hmax = 0, emax = 0;
   for (int r = 0; r < R; ++r) {
     const int g = order[r];  // cluster id of rank r
     const int a = rptr[r], b = rptr[r + 1];
-    for (int t = a; t < b; ++t)
-      if (t == a || lev[rows[t]] != lev[rows[t - 1]]) slices.push_back(t - a);
-    slices.push_back(b - a);
-    sptr[r + 1] = (int)slices.size();
-    slmax = std::max(slmax, sptr[r + 1] - sptr[r] - 1);
+    const int nb = bptr[r + 1] - bptr[r] - 1;
     const size_t h0 = halo.size();
     for (int t = a; t < b; ++t) {
       const int i = rows[t];
       for (int j = Lp[i]; j < Lp[i + 1] - 1; ++j) {
         const int c = Li[j];
-        if (reg[c] != g && hstamp[c] != r) {
+        if (reg[c] == g) continue;
+        if (hstamp[c] != r) {
           hstamp[c] = r;
           halo.push_back(c);
+          first_use[c] = band_of[i];
+        } else {
+          first_use[c] = std::min(first_use[c], band_of[i]);
         }
       }
     }
-    std::sort(halo.begin() + h0, halo.end());
+    std::sort(halo.begin() + h0, halo.end(), [&](int x, int y) {
+      return first_use[x] != first_use[y] ? first_use[x] < first_use[y] : x < y;
+    });
     for (size_t k = h0; k < halo.size(); ++k) hslot[halo[k]] = (int)(k - h0);
     hptr[r + 1] = (int)halo.size();
     hmax = std::max(hmax, hptr[r + 1] - hptr[r]);
+    // halo bounds and waits per band
+    size_t k = h0;
+    const int dl0 = (int)dlist.size();
+    for (int bb = 0; bb <= nb; ++bb) {
+      hband.push_back((int)(k - h0));
+      wptr.push_back((int)dlist.size() - dl0);
+      if (bb == nb) break;
+      const size_t k0 = k;
+      while (k < halo.size() && first_use[halo[k]] == bb) ++k;
+      // the largest requirement of this band per region read, in halo order
+      std::vector<int> touched;
+      for (size_t q = k0; q < k; ++q) {
+        const int c = halo[q];
+        const int u = rank[reg[c]];
+        const int need = band_of[c] + 1;
+        if (need_stamp[u] != r) {
+          need_stamp[u] = r;
+          need_max[u] = 0;
+        }
+        if (need > need_max[u]) {
+          if (std::find(touched.begin(), touched.end(), u) == touched.end()) touched.push_back(u);
+          need_max[u] = need;
+        }
+      }
+      std::sort(touched.begin(), touched.end());
+      for (int u : touched) dlist.push_back(u | (need_max[u] << REGION_RANK_BITS));
+    }
+    dptr[r + 1] = (int)dlist.size();
     const int e0 = (int)ecode.size();
     for (int t = a; t < b; ++t) {
       const int i = rows[t];
@@ -594,9 +650,8 @@ bool build_region_schedule(int n, int64_t nnz, int levels, const int *lvl_ptr,
       eptr[t + 1] = (int)ecode.size();
     }
     emax = std::max(emax, (int)ecode.size() - e0);
-    for (int e = gptr[g]; e < gptr[g + 1]; ++e) dlist.push_back(rank[glist[e]]);
-    dptr[r + 1] = (int)dlist.size();
   }
+  if (R >= (1 << REGION_RANK_BITS)) return false;  // ranks do not fit the wait entries
   for (int t = 0; t < n; ++t)
     for (int u = pcnt[t]; u < PK; ++u) pcode[(size_t)PK * t + u] = smax + hmax;
   o.R = R;
@@ -612,6 +667,10 @@ bool build_region_schedule(int n, int64_t nnz, int levels, const int *lvl_ptr,
   o.halo.swap(halo);
   o.dptr.swap(dptr);
   o.dlist.swap(dlist);
+  o.bptr.swap(bptr);
+  o.bands.swap(bands);
+  o.hband.swap(hband);
+  o.wptr.swap(wptr);
   o.eptr.swap(eptr);
   o.ecode.swap(ecode);
   o.esrc.swap(esrc);
@@ -634,7 +693,8 @@ bool pack_region_records(const HostRegions &o, int vs, HostRecords &out) {
     const int ne = o.eptr[a + nr] - o.eptr[a];
     const int ns = o.sptr[r + 1] - o.sptr[r] - 1;
     const int nd = o.dptr[r + 1] - o.dptr[r];
-    const RecLayout L = rec_layout(nr, nh, ne, ns, nd, vs);
+    const int nb = o.bptr[r + 1] - o.bptr[r] - 1;
+    const RecLayout L = rec_layout(nr, nh, ne, ns, nd, nb, vs);
     int *hd = &out.rhead[8 * (size_t)r];
     hd[0] = nr;
     hd[1] = nh;
@@ -643,6 +703,7 @@ bool pack_region_records(const HostRegions &o, int vs, HostRecords &out) {
     hd[4] = nd;
     hd[5] = (int)units;
     hd[6] = L.bytes / 16;
+    hd[7] = nb;
     out.recmax = std::max(out.recmax, L.bytes);
     units += L.bytes / 16;
     // value positions are counted in values of vs bytes
@@ -657,7 +718,8 @@ bool pack_region_records(const HostRegions &o, int vs, HostRecords &out) {
     const int e0 = o.eptr[a], ne = o.eptr[a + nr] - e0;
     const int s0 = o.sptr[r], ns = o.sptr[r + 1] - s0 - 1;
     const int d0 = o.dptr[r], nd = o.dptr[r + 1] - d0;
-    const RecLayout L = rec_layout(nr, nh, ne, ns, nd, vs);
+    const int b0 = o.bptr[r], nb = o.bptr[r + 1] - b0 - 1;
+    const RecLayout L = rec_layout(nr, nh, ne, ns, nd, nb, vs);
     uint8_t *rec = out.bytes.data() + (size_t)out.rhead[8 * (size_t)r + 5] * 16;
     const int64_t vbase = (int64_t)out.rhead[8 * (size_t)r + 5] * 16 / vs;
     auto put16 = [](uint8_t *p, int v) {
@@ -688,6 +750,11 @@ bool pack_region_records(const HostRegions &o, int vs, HostRecords &out) {
     for (int q = 0; q < nh; ++q) put32(rec + L.halo + 4 * q, o.halo[h0 + q]);
     for (int s = 0; s <= ns; ++s) put32(rec + L.slices + 4 * s, o.slices[s0 + s]);
     for (int d = 0; d < nd; ++d) put32(rec + L.deps + 4 * d, o.dlist[d0 + d]);
+    for (int k = 0; k <= nb; ++k) {
+      put32(rec + L.bands + 4 * k, o.bands[b0 + k]);
+      put32(rec + L.hband + 4 * k, o.hband[b0 + k]);
+      put32(rec + L.wptr + 4 * k, o.wptr[b0 + k]);
+    }
   }
   return true;
 }
@@ -734,13 +801,16 @@ void reversed_transpose(int n, int levels, const std::vector<int> &lvl_ptr,
 int plan_regions(symamd_handle h, symamd_plan p, int cap, RegionPlan **out,
                  bool transposed) {
   auto &plans = transposed ? p->rplans_t : p->rplans;
-  auto it = plans.find(cap);
+  const int band = region_band(h);
+  const int64_t key = (int64_t)cap * 256 + band;
+  auto it = plans.find(key);
   if (it != plans.end()) {
     *out = &it->second;
     return SYMAMD_OK;
   }
   RegionPlan rp;
   rp.cap = cap;
+  rp.band = band;
   rp.transposed = transposed;
   const int n = p->n;
   HostRegions o;
@@ -760,26 +830,26 @@ int plan_regions(symamd_handle h, symamd_plan p, int cap, RegionPlan **out,
       reversed_transpose(n, p->levels, p->h_lvl_ptr, perm, crp, cci, rlvl, rperm, rrp, rci,
                          vmap);
       ok = build_region_schedule(n, p->nnz, p->levels, rlvl.data(), rperm.data(), rrp.data(),
-                                 rci.data(), cap, o);
+                                 rci.data(), cap, band, o);
       if (ok) {
         for (int &r : o.rows) r = n - 1 - r;
         for (int &r : o.halo) r = n - 1 - r;
       }
     } else {
       ok = build_region_schedule(n, p->nnz, p->levels, p->h_lvl_ptr.data(), perm.data(),
-                                 crp.data(), cci.data(), cap, o);
+                                 crp.data(), cci.data(), cap, band, o);
     }
     if (!ok) o.R = 0;
   }
   rp.inside = o.inside;
   if (o.R == 0) {
-    *out = &plans.emplace(cap, rp).first->second;
+    *out = &plans.emplace(key, rp).first->second;
     return SYMAMD_OK;
   }
   const int vs = p->dtype == SYMAMD_F64 ? 8 : 4;
   HostRecords hr;
   if (o.smax + o.hmax >= REGION_SLOTS_MAX || !pack_region_records(o, vs, hr)) {
-    *out = &plans.emplace(cap, rp).first->second;  // R = 0: not usable
+    *out = &plans.emplace(key, rp).first->second;  // R = 0: not usable
     return SYMAMD_OK;
   }
   if (transposed)
@@ -799,12 +869,14 @@ int plan_regions(symamd_handle h, symamd_plan p, int cap, RegionPlan **out,
     free_region_plan(rp);
     return rc;
   }
-  if (hipMalloc(&rp.d_rflags, sizeof(int) * rp.R * RegionPlan::MAX_GROUPS) != hipSuccess ||
-      hipMemset(rp.d_rflags, 0, sizeof(int) * rp.R * RegionPlan::MAX_GROUPS) != hipSuccess) {
+  if (hipMalloc(&rp.d_rflags, sizeof(int64_t) * rp.R * RegionPlan::MAX_GROUPS) !=
+          hipSuccess ||
+      hipMemset(rp.d_rflags, 0, sizeof(int64_t) * rp.R * RegionPlan::MAX_GROUPS) !=
+          hipSuccess) {
     free_region_plan(rp);
     return fail(SYMAMD_ERR_ALLOC, "region schedule arrays");
   }
-  RegionPlan *stored = &plans.emplace(cap, rp).first->second;
+  RegionPlan *stored = &plans.emplace(key, rp).first->second;
   SYMAMD_TRY(gather_region_values(h, p));
   *out = stored;
   return SYMAMD_OK;

@@ -359,6 +359,11 @@ int sn_analyze(symamd_handle h, int n, int m, const int *h_sup, const int *h_nro
 int sn_solve(symamd_handle h, symamd_snplan_s *q, const double *d_Lx, double *d_x, int k,
              int64_t ldx) {
   if (q->m == 0 || k == 0) return SYMAMD_OK;
+  // tickets run over (task, column) pairs in 32-bit counters
+  if ((int64_t)q->ntasks * k >= (int64_t)INT_MAX)
+    return fail(SYMAMD_ERR_NOT_SUPPORTED,
+                "supernodal solve: %d tasks x %d columns exceed the 32-bit ticket range; "
+                "solve the columns in smaller batches", q->ntasks, k);
   // temporaries and tags for k columns
   if (k > q->kcap) {
     double *T = nullptr;

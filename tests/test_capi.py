@@ -64,6 +64,7 @@ def test_sym_lib_dropins_exported_with_reference_mangling():
         "_ZN7sym_lib14spilu0_csr_lbcEiiPKiS1_PKdS1_PdiS1_S1_S1_Pi",
         "_ZN7sym_lib16sptrsv_csc_blockEiPKiS1_S1_PdS2_iS1_",
         "_ZN7sym_lib5parsy19getCoarseLevelSet_6EmPKiS2_RiRPiS5_S3_S5_S5_iiiPd",
+        "_ZN7sym_lib32get_coarse_levelSet_DAG_CSC_treeEmPKiS1_iRiRPiS2_S4_S4_iiiPd",
         "_ZN7sym_lib5parsy13blockedLsolveEiPmPiPdiS1_S2_S2_iS3_",
         "_ZN7sym_lib5parsy22H2LeveledBlockedLsolveEiPmPiPdiS1_S2_S2_iS3_iS2_S2_iS2_S2_i",
     }

@@ -160,3 +160,35 @@ def test_block_autograd_bitwise_and_dense(dev, executor):
     gLd = -(GBd @ Xd.T)[rows, Li]
     assert np.max(np.abs(GB_ref - GBd)) / np.max(np.abs(GBd)) < 1e-12
     assert np.max(np.abs(gL_ref - gLd)) / np.max(np.abs(gLd)) < 1e-12
+
+
+def test_two_forwards_one_plan_then_backward(dev):
+    """Two forwards with different values on one plan, one backward through
+    both: each gradient is taken against the values of its own forward."""
+    import torch
+    Lp, Li, Lx, plan, dLx = _setup(12, dev)
+    plan.set_lanes(1)
+    n = len(Lp) - 1
+    Lx2 = Lx * 1.25  # a second matrix on the same pattern
+    b1, b2 = W.uniform(4, 0, n), W.uniform(4, 1, n)
+    L1 = dLx.clone().requires_grad_(True)
+    L2 = to_dev(Lx2, device=dev).requires_grad_(True)
+    t1 = to_dev(b1, device=dev).requires_grad_(True)
+    t2 = to_dev(b2, device=dev).requires_grad_(True)
+    x1 = TriangularSolve.apply(L1, t1, plan, "syncfree")
+    x2 = TriangularSolve.apply(L2, t2, plan, "syncfree")
+    (0.5 * (x1 * x1).sum() + 0.5 * (x2 * x2).sum()).backward()
+    plan.check()
+    for Lv, bv, xt, Lt, bt in ((Lx, b1, x1, L1, t1), (Lx2, b2, x2, L2, t2)):
+        x_ref = oracle.sptrsv_csr(Lp, Li, Lv, bv)
+        gb_ref, gL_ref = oracle.adjoint(Lp, Li, Lv, x_ref, x_ref)
+        assert bitwise_equal(xt.detach().cpu().numpy(), x_ref)
+        assert bitwise_equal(bt.grad.cpu().numpy(), gb_ref)
+        assert bitwise_equal(Lt.grad.cpu().numpy(), gL_ref)
+    # in-place change of the values after a forward: the plan notices
+    with torch.no_grad():
+        L1.mul_(0.5)
+    x3 = TriangularSolve.apply(L1.detach(), to_dev(b1, device=dev), plan, "syncfree")
+    plan.check()
+    x3_ref = oracle.sptrsv_csr(Lp, Li, Lx * 0.5, b1)
+    assert bitwise_equal(x3.cpu().numpy(), x3_ref)

@@ -81,32 +81,3 @@ def test_tuner_picks_a_correct_schedule(dev):
     plan.solve(dB, layout="row")
     plan.check()
     assert rel_err(dB.cpu().numpy().T, X_ref) <= RTOL64
-
-
-@pytest.mark.parametrize("span,cost", [(1, 0), (3, 4000), (8, 200), (1000, 10 ** 9)])
-def test_coarsened_schedule_is_valid_and_solves(dev, span, cost):
-    Lp, Li, Lx = W.random_lower(5000, 0.001, 1, seed=12)
-    dLp, dLi, dLx = to_dev(Lp, Li, Lx, device=dev)
-    lp, pp, part = sa.coarsen(dLp, dLi, span=span, max_part_cost=cost)
-    lp, pp, part = lp.cpu().numpy(), pp.cpu().numpy(), part.cpu().numpy()
-    assert oracle.check_schedule(Lp, Li, lp, pp, part) == 0
-    b = W.uniform(1, 0, 5000)
-    x_ref = oracle.sptrsv_csr(Lp, Li, Lx, b)
-    assert bitwise_equal(oracle.sptrsv_csr_lbc(Lp, Li, Lx, b, lp, pp, part), x_ref)
-    x = to_dev(b, device=dev)
-    sa.solve_lbc(dLp, dLi, dLx, x, lp, to_dev(pp, device=dev), to_dev(part, device=dev))
-    import torch
-    torch.cuda.synchronize()
-    assert bitwise_equal(x.cpu().numpy(), x_ref)
-
-
-def test_coarsened_grid_partitions(dev):
-    # 2D grid, 8 levels per coarse level: one connected component per
-    # coarse level, split into partitions by the cost bound is not allowed
-    # (the component must stay whole), so each coarse level is one partition
-    Lp, Li, Lx = W.lap2d_tril(40, 30)
-    dLp, dLi = to_dev(Lp, Li, device=dev)
-    lp, pp, part = sa.coarsen(dLp, dLi, span=8, max_part_cost=10)
-    lp, pp, part = lp.cpu().numpy(), pp.cpu().numpy(), part.cpu().numpy()
-    assert oracle.check_schedule(Lp, Li, lp, pp, part) == 0
-    assert len(lp) - 1 == (40 + 30 - 1 + 7) // 8

@@ -39,7 +39,7 @@ def _schedules(Lp, Li, dev):
     """The two helper schedules plus the library's own coarsening."""
     lp, ls, _ = oracle.levelset_csr(Lp, Li)
     out = dict(lbc_schedules(lp, ls))
-    lvl, pp, part = sa.coarsen(*to_dev(Lp, Li, device=dev), span=4, max_part_cost=400)
+    lvl, pp, part = sa.coarsen(*to_dev(Lp, Li, device=dev), inner_parts=8, min_level_dist=4)
     s = (lvl.cpu().numpy(), pp.cpu().numpy(), part.cpu().numpy())
     assert oracle.check_schedule(Lp, Li, *s) == 0
     out["coarsen"] = s

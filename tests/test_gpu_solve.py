@@ -163,6 +163,28 @@ def test_sptrsm(dev, k, layout):
             assert bitwise_equal(X, X_ref), (ex, k, layout, direct)
 
 
+@pytest.mark.parametrize("k,lanes", [(8, 8), (4, 16)])
+@pytest.mark.parametrize("layout", ["col", "row"])
+def test_sptrsm_multilane_long_rows(dev, k, lanes, layout):
+    """Several lanes per row with k > 1 on the unit executors: rows of up to
+    257 entries (several chunks of dependency loads, dead slots in the last
+    one), both layouts, at the 1e-12 bar."""
+    n = 6000
+    Lp, Li, Lx = W.zipf_lower(n, 1.8, 256, 3000, 9)
+    assert int(np.diff(Lp).max()) > 64
+    B = W.uniform(13, 0, n * k).reshape(k, n)
+    X_ref = oracle.sptrsm_csr(Lp, Li, Lx, B)
+    plan = _plan(Lp, Li, Lx, dev)
+    for ex in ("syncfree", "levelset"):
+        plan.set_lanes(lanes)
+        dB = to_dev(B if layout == "col" else np.ascontiguousarray(B.T), device=dev)
+        plan.solve(dB, layout=layout, executor=ex)
+        plan.check()
+        X = dB.cpu().numpy()
+        X = X.T if layout == "row" else X
+        assert rel_err(X, X_ref) <= RTOL64, (ex, k, lanes, layout)
+
+
 # ---------------------------------------------------------------- batched
 def test_batched_small(dev):
     w = W.config4(count=24, lo=8, hi=40, seed=3)

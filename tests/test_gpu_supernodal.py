@@ -102,6 +102,44 @@ def test_sptrsv_csc_block_dropins(dev):
     assert bitwise_equal(x, want)
 
 
+def test_sptrsv_csc_levelset_block_tree_schedule(dev):
+    """sptrsv_csc_levelset_block with several supernodes per level, listed
+    in descending order inside each level (sup2node not the identity):
+    bit-identical to the serial sptrsv_csc_block."""
+    from tests.helpers import postorder_binary_tree
+    L = ctypes.CDLL(LIB)
+    levels = 8
+    sup, nrows, Lp, Li, Lx = W.bcsc_tree(levels, 4, 64, 2.0, 5)
+    n, m = int(sup[-1]), len(sup) - 1
+    parent = postorder_binary_tree(levels)
+    depth = np.zeros(m, np.int64)  # height above the leaves: a valid level
+    for i in range(m):  # postorder: children before parents
+        if parent[i] >= 0:
+            depth[parent[i]] = max(depth[parent[i]], depth[i] + 1)
+    order = np.lexsort((-np.arange(m), depth)).astype(np.int32)
+    levelPtr = np.searchsorted(depth[order], np.arange(depth.max() + 2)).astype(np.int32)
+    b = W.uniform(5, 0, n)
+    want = oracle.sptrsv_csc_block(sup, nrows, Lp, Li, Lx, b)
+    x = b.copy()
+    L["_ZN7sym_lib25sptrsv_csc_levelset_blockEiPKiS1_S1_PdS2_S1_S1_iS1_S1_PS2_"](
+        c_int(n), P(Lp), P(Li), P(nrows), P(Lx), P(x), P(levelPtr),
+        P(np.ascontiguousarray(sup[order])), c_int(len(levelPtr) - 1), P(sup), P(order), None)
+    assert bitwise_equal(x, want)
+
+
+def test_supernodal_ticket_range_rejected(dev):
+    """k columns whose (task, column) tickets would pass 2^31 - 1 are
+    refused before any device work."""
+    sup, nrows, Lp, Li, Lx = W.bcsc(300, 8, 8, 60, 1)
+    plan = sa.SupernodalPlan(sup, nrows, Lp, Li)
+    dLx = to_dev(Lx)
+    x = to_dev(W.uniform(1, 0, 300))
+    k = (2 ** 31 - 1) // 2  # far more tickets than 2^31 for any task count >= 2
+    rc = sa.lib().symamd_sn_solve_block(plan.h._h, plan._p, c_void_p(dLx.data_ptr()),
+                                        c_void_p(x.data_ptr()), c_int(k), 300)
+    assert rc == 7 and b"ticket" in sa.lib().symamd_last_error()
+
+
 @pytest.mark.parametrize("case", [(3000, 40, 24, 400, 2), (20000, 100, 130, 4096, 5)])
 def test_parsy_blocked_lsolve_dropin(dev, case):
     """sym_lib::parsy::blockedLsolve on parsy's storage of the same factor:
