@@ -117,8 +117,9 @@ int symamd_set_timeout_ms(symamd_handle h, double ms);
  *   SYMAMD_OPT_REGION_BAND (read at solve time): a region task publishes
  *     its progress every this many internal levels (slices), so a region
  *     that reads it starts on its first levels while the rest is solved
- *     (0, the default, = 2; a region has at most 31 bands, wider ones when
- *     it has more levels); each value's schedule is built once per plan.
+ *     (0, the default, = 255: one band; a region has at most 31 bands, wider
+ *     ones when it has more levels); each value's schedule is built once per
+ *     plan.
  *     Results are identical either way. */
 enum { SYMAMD_OPT_TICKETS = 1, SYMAMD_OPT_TRACE_PTR = 2, SYMAMD_OPT_TRACE_CAP = 3,
        SYMAMD_OPT_MAX_WAVES = 4, SYMAMD_OPT_COLMAJOR_DIRECT = 5, SYMAMD_OPT_MARKS = 6,
@@ -407,6 +408,21 @@ int symamd_tune(symamd_handle h, symamd_plan plan, int k,
 /* Low-level schedule override used by the tuner and by tests: lanes per row
  * (1..64, power of two) for the given executor; 0 restores the heuristic. */
 int symamd_plan_set_lanes(symamd_plan plan, int lanes_per_row);
+
+/* ----------------------------------------------------------- diagnostics */
+/* The region executor's schedule for a HOST CSR lower triangle (diagonal
+ * last, FP64 values), built on the host by the steps the device path takes:
+ * region cap `cap`, `band_slices` slices per published band, the forward
+ * solve or (transposed != 0) L^T x = g.  info[8] receives {regions, max
+ * rows per region, max halo rows, largest record (bytes), total record
+ * bytes, max slices per region, levels, widest region depth}; when rec is
+ * not NULL it receives the records (info[4] bytes, values filled) and rhead
+ * the 8-int header of every region.  No device work; for tests that emulate
+ * the region kernel on the CPU.  SYMAMD_ERR_NOT_SUPPORTED when the matrix
+ * has no usable region schedule. */
+int symamd_region_schedule_host(int n, const int *Lp, const int *Li, const double *Lx,
+                                int cap, int band_slices, int transposed, int *info,
+                                uint8_t *rec, int *rhead);
 
 #ifdef __cplusplus
 }

@@ -688,9 +688,10 @@ int region_cap(symamd_handle h, int k) {
   return task_cols(k) == KG_MAX && region_waves(h, k) == 1 ? 256 : 512;
 }
 
-// Slices per published band (the handle's choice, else 2).
+// Slices per published band (the handle's choice, else 255: one band for
+// regions of up to 255 levels).
 int region_band(symamd_handle h) {
-  const int b = h->region_band > 0 ? h->region_band : 2;
+  const int b = h->region_band > 0 ? h->region_band : 255;
   return b < 255 ? b : 255;
 }
 

@@ -883,3 +883,95 @@ int plan_regions(symamd_handle h, symamd_plan p, int cap, RegionPlan **out,
 }
 
 }  // namespace symamd
+
+// ------------------------------------------------------------ diagnostics
+// The region schedule of a host CSR lower triangle (diagonal last), built on
+// the host by the same steps the device path takes (level sets with rows
+// ascending inside a level, the level-ordered copy, and for the transposed
+// solve the level-ordered CSC), packed with FP64 values.  For tests that
+// emulate the region kernel on the CPU.
+extern "C" int symamd_region_schedule_host(int n, const int *Lp, const int *Li,
+                                           const double *Lx, int cap, int band_slices,
+                                           int transposed, int *info, uint8_t *rec,
+                                           int *rhead) {
+  using namespace symamd;
+  if (n < 1 || !Lp || !Li || !Lx || !info || cap < 2)
+    return fail(SYMAMD_ERR_INVALID_ARG, "symamd_region_schedule_host: bad arguments");
+  const int64_t nnz = Lp[n];
+  std::vector<int> lev(n, 0);
+  int levels = 0;
+  for (int i = 0; i < n; ++i) {
+    int l = 0;
+    for (int64_t j = Lp[i]; j < Lp[i + 1] - 1; ++j) l = std::max(l, lev[Li[j]] + 1);
+    lev[i] = l;
+    levels = std::max(levels, l + 1);
+  }
+  std::vector<int> lvl_ptr(levels + 1, 0), perm(n);
+  for (int i = 0; i < n; ++i) lvl_ptr[lev[i] + 1]++;
+  for (int l = 0; l < levels; ++l) lvl_ptr[l + 1] += lvl_ptr[l];
+  {
+    std::vector<int> fill(lvl_ptr.begin(), lvl_ptr.end() - 1);
+    for (int i = 0; i < n; ++i) perm[fill[lev[i]]++] = i;
+  }
+  // level-ordered copy (row perm[q] at position q), and its values
+  std::vector<int> crp(n + 1, 0), cci(nnz);
+  std::vector<double> cval(nnz);
+  for (int q = 0; q < n; ++q) {
+    const int i = perm[q];
+    crp[q + 1] = crp[q] + (int)(Lp[i + 1] - Lp[i]);
+    for (int64_t j = Lp[i]; j < Lp[i + 1]; ++j) {
+      cci[crp[q] + (j - Lp[i])] = Li[j];
+      cval[crp[q] + (j - Lp[i])] = Lx[j];
+    }
+  }
+  HostRegions o;
+  std::vector<int> vmap;
+  std::vector<double> tval;
+  bool ok;
+  if (transposed) {
+    // level-ordered CSC: column perm[q] at position q, rows ascending
+    std::vector<int> cnt(n + 1, 0), trp(n + 1, 0), tci(nnz);
+    tval.resize(nnz);
+    for (int64_t j = 0; j < nnz; ++j) cnt[Li[j]]++;
+    std::vector<int> ipos(n);
+    for (int q = 0; q < n; ++q) ipos[perm[q]] = q;
+    for (int q = 0; q < n; ++q) trp[q + 1] = trp[q] + cnt[perm[q]];
+    std::vector<int> fill(trp.begin(), trp.end() - 1);
+    for (int i = 0; i < n; ++i)
+      for (int64_t j = Lp[i]; j < Lp[i + 1]; ++j) {
+        const int t = fill[ipos[Li[j]]]++;
+        tci[t] = i;
+        tval[t] = Lx[j];
+      }
+    std::vector<int> rlvl, rperm, rrp, rci;
+    reversed_transpose(n, levels, lvl_ptr, perm, trp, tci, rlvl, rperm, rrp, rci, vmap);
+    ok = build_region_schedule(n, nnz, levels, rlvl.data(), rperm.data(), rrp.data(), rci.data(),
+                               cap, band_slices, o);
+    if (ok) {
+      for (int &r : o.rows) r = n - 1 - r;
+      for (int &r : o.halo) r = n - 1 - r;
+    }
+  } else {
+    ok = build_region_schedule(n, nnz, levels, lvl_ptr.data(), perm.data(), crp.data(),
+                               cci.data(), cap, band_slices, o);
+  }
+  HostRecords hr;
+  if (!ok || o.R == 0 || !pack_region_records(o, 8, hr))
+    return fail(SYMAMD_ERR_NOT_SUPPORTED, "symamd_region_schedule_host: no region schedule");
+  info[0] = o.R;
+  info[1] = o.smax;
+  info[2] = o.hmax;
+  info[3] = hr.recmax;
+  info[4] = (int)hr.bytes.size();
+  info[5] = o.slmax;
+  info[6] = levels;
+  info[7] = o.width;
+  if (rec) {
+    std::memcpy(rec, hr.bytes.data(), hr.bytes.size());
+    double *vals = reinterpret_cast<double *>(rec);
+    for (size_t t = 0; t < hr.vsrc.size(); ++t)
+      vals[hr.vdst[t]] = transposed ? tval[vmap[hr.vsrc[t]]] : cval[hr.vsrc[t]];
+  }
+  if (rhead) std::memcpy(rhead, hr.rhead.data(), sizeof(int) * hr.rhead.size());
+  return SYMAMD_OK;
+}
