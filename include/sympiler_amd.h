@@ -53,9 +53,11 @@ typedef enum {
   SYMAMD_EXEC_SYNCFREE = 2,  /* one persistent launch, per-unit ready flags     */
   SYMAMD_EXEC_LEVELSET_GRAPH = 3, /* the per-level launches replayed as a hipGraph */
   SYMAMD_EXEC_REGIONS = 4    /* one persistent launch over compact regions of rows,
-                                each solved by one wavefront in LDS (forward
-                                solves, 1 <= k <= 64; other cases, and matrices
-                                whose region graph has a cycle, run SYNCFREE) */
+                                each solved by one workgroup in LDS (forward
+                                and adjoint solves, 1 <= k <= 64, at the largest
+                                region cap whose task fits the LDS; other cases,
+                                and matrices with no such schedule, run
+                                SYNCFREE) */
 } symamd_executor;
 
 typedef struct symamd_handle_s *symamd_handle;
@@ -107,9 +109,10 @@ int symamd_set_timeout_ms(symamd_handle h, double ms);
  *     regions run the owner schedule; 0 = they deal units like other plans
  *     (for comparison; results are identical).
  *   SYMAMD_OPT_REGION_CAP (read at solve time): region size cap of the
- *     SYMAMD_EXEC_REGIONS schedule (0, the default, picks it from k: 512
- *     rows for k <= 4, 256 for k >= 8); each cap's schedule is built once
- *     per plan.
+ *     SYMAMD_EXEC_REGIONS schedule (0, the default: 512 rows, or 256 for
+ *     tasks of eight columns run by one wavefront, halved -- down to 32 --
+ *     until a task's rows, halo and record fit a workgroup's LDS; a cap set
+ *     here is used alone); each cap's schedule is built once per plan.
  *   SYMAMD_OPT_REGION_WAVES (read at solve time): wavefronts per task of
  *     the SYMAMD_EXEC_REGIONS schedule, 1, 2 or 4 for tasks of eight
  *     columns (k >= 8; narrower tasks use one); 0, the default, picks it.
@@ -120,12 +123,19 @@ int symamd_set_timeout_ms(symamd_handle h, double ms);
  *     (0, the default, = 255: one band; a region has at most 31 bands, wider
  *     ones when it has more levels); each value's schedule is built once per
  *     plan.
+ *     Results are identical either way.
+ *   SYMAMD_OPT_REGION_MW (read at solve time): 1 (default) = region tasks
+ *     whose right-hand sides are loaded directly (row-major blocks, single
+ *     vectors) run with one extra wavefront per task that does all of its
+ *     waiting, loading, storing and publishing, so the solving wavefronts
+ *     never wait for memory; the default band is then 2 slices.  0 = every
+ *     wavefront of a task does both (default band: one per region).
  *     Results are identical either way. */
 enum { SYMAMD_OPT_TICKETS = 1, SYMAMD_OPT_TRACE_PTR = 2, SYMAMD_OPT_TRACE_CAP = 3,
        SYMAMD_OPT_MAX_WAVES = 4, SYMAMD_OPT_COLMAJOR_DIRECT = 5, SYMAMD_OPT_MARKS = 6,
        SYMAMD_OPT_REGIONS = 7, SYMAMD_OPT_REGION_ROWS = 8, SYMAMD_OPT_OWNERS = 9,
        SYMAMD_OPT_REGION_CAP = 10, SYMAMD_OPT_REGION_WAVES = 11,
-       SYMAMD_OPT_REGION_BAND = 12 };
+       SYMAMD_OPT_REGION_BAND = 12, SYMAMD_OPT_REGION_MW = 13 };
 int symamd_set_option(symamd_handle h, int option, int64_t value);
 const char *symamd_last_error(void);
 /* Library version string. */
@@ -250,6 +260,12 @@ int symamd_plan_levels(symamd_plan plan, int *h_level_ptr, int *h_level_set);
  * L's off-diagonal entries whose two rows share a region (computed whenever
  * the clustering ran). */
 int symamd_plan_regions(symamd_plan plan, int *count, double *inside);
+/* The region schedule SYMAMD_EXEC_REGIONS runs for a forward (transposed =
+ * 0) or adjoint (1) solve of k right-hand sides: its region cap and its task
+ * count (regions times column groups of up to eight columns), or 0 and 0
+ * when such a solve runs SYMAMD_EXEC_SYNCFREE.  May build the schedule. */
+int symamd_region_schedule(symamd_handle h, symamd_plan plan, int k, int transposed,
+                           int *cap, int *tasks);
 /* The executor (and lanes per row) SYMAMD_EXEC_AUTO takes for a forward
  * solve of k right-hand sides in `layout`: the tuned choice if symamd_tune
  * ran, else SYMAMD_EXEC_REGIONS when the plan has a usable region schedule
@@ -394,6 +410,19 @@ int symamd_sn_solve(symamd_handle h, symamd_snplan plan, const double *d_Lx, dou
  * bit-identical to sptrsv_csc_block on that column. */
 int symamd_sn_solve_block(symamd_handle h, symamd_snplan plan, const double *d_Lx,
                           double *d_X, int k, int64_t ld);
+/* The transposed solve L^T X = B on the same plan and storage, k
+ * column-major right-hand sides (k = 1: a vector): the semantics of parsy's
+ * blockedLTsolve (parsy/triangularSolve/Triangular_BCSC.cpp:125-186) and of
+ * its multi-RHS and level-scheduled variants (:187-249, :742-808,
+ * :811-880), supernodes from the last to the first: per supernode, each top
+ * column's right-hand side minus the sum over the rows below, ascending, of
+ * the block entry times the solved value at that row; then the top's
+ * transposed triangular solve from its last column (x_c /= M[c,c]; each
+ * x_r, r < c, loses M[c,r] x_c).  The reference's own order depends on the
+ * BLAS it links (none is present here), so this order is the library's,
+ * the same in every column. */
+int symamd_sn_solve_transposed(symamd_handle h, symamd_snplan plan, const double *d_Lx,
+                               double *d_X, int k, int64_t ld);
 int symamd_sn_check(symamd_handle h, symamd_snplan plan);
 int symamd_sn_destroy(symamd_snplan plan);
 

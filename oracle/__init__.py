@@ -71,6 +71,7 @@ def olib():
             "orc_spic0_csr": ([i, vp, vp, vp], c_int),
             "orc_spilu0_csr": ([i, i, vp, vp, vp, vp, vp], c_int),
             "orc_sptrsv_csc_block": ([i, vp, vp, vp, vp, vp, i, vp], None),
+            "orc_sptrsv_csc_block_t": ([i, vp, vp, vp, vp, vp, i, vp], None),
             "orc_levelset_csr": ([i, vp, vp, vp, vp, vp], c_int),
             "orc_levelset_csc": ([i, vp, vp, vp, vp], c_int),
             "orc_check_schedule": ([i, vp, vp, i, vp, vp, vp], c_int),
@@ -247,6 +248,15 @@ def sptrsv_csc_block(sup, nrows, Lp, Li, Lx, b):
     x = _f64(b).copy()
     olib().orc_sptrsv_csc_block(len(x), _p(_i32(Lp)), _p(_i32(Li)), _p(_i32(nrows)),
                                 _p(_f64(Lx)), _p(x), len(sup) - 1, _p(_i32(sup)))
+    return x
+
+
+def sptrsv_csc_block_t(sup, nrows, Lp, Li, Lx, b):
+    """L^T x = b on the supernodal storage (blockedLTsolve's semantics, this
+    build's operation order: see orc_sptrsv_csc_block_t)."""
+    x = _f64(b).copy()
+    olib().orc_sptrsv_csc_block_t(len(x), _p(_i32(Lp)), _p(_i32(Li)), _p(_i32(nrows)),
+                                  _p(_f64(Lx)), _p(x), len(sup) - 1, _p(_i32(sup)))
     return x
 
 

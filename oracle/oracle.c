@@ -375,6 +375,42 @@ void orc_sptrsv_csc_block(int n, const int *Lp, const int *Li, const int *nrows,
   free(t);
 }
 
+/* Transposed solve L^T x = b on the same supernodal storage: the semantics
+ * of parsy's blockedLTsolve (sparse_blas/parsy/triangularSolve/
+ * Triangular_BCSC.cpp:125-186), supernodes from the last to the first:
+ *   1. the top's right-hand side minus the rows below times the solved
+ *      values at those rows (its CBLAS path: dgemv "T", alpha -1, beta 1);
+ *      here, per top column c, t = sum over the rows r below, ascending, of
+ *      M[r, c] * x[row r] (the first product, then adding in order), and
+ *      x[s+c] = x[s+c] - t;
+ *   2. the top's transposed triangular solve (dtrsm "L","L","T","N"); here
+ *      column-oriented from the last column: x[s+c] /= M[c, c], then every
+ *      x[s+r], r < c, loses M[c, r] * x[s+c].
+ * The reference's own result depends on the BLAS it links (absent here;
+ * its SYM_BLAS fallback is marked unfinished, :163), so this order is this
+ * build's definition, pinned by dense solves and exact integer cases. */
+void orc_sptrsv_csc_block_t(int n, const int *Lp, const int *Li, const int *nrows,
+                            const double *Lx, double *x, int num_nodes,
+                            const int *supernodes) {
+  (void)n;
+  for (int i = num_nodes - 1; i >= 0; i--) {
+    const int s = supernodes[i], w = supernodes[i + 1] - s, nr = nrows[i];
+    const double *M = Lx + Lp[i];
+    for (int c = 0; c < w; c++) {
+      double t = 0.0;
+      for (int r = w; r < nr; r++) {
+        const double p = M[r + (size_t)c * nr] * x[Li[Lp[i] + r]];
+        t = r == w ? p : t + p;
+      }
+      if (nr > w) x[s + c] = x[s + c] - t;
+    }
+    for (int c = w - 1; c >= 0; c--) {
+      x[s + c] = x[s + c] / M[c + (size_t)c * nr];
+      for (int r = 0; r < c; r++) x[s + r] = x[s + r] - M[c + (size_t)r * nr] * x[s + c];
+    }
+  }
+}
+
 /* --------------------------------------------------------- level sets */
 
 /* Level-set inspector on the CSR of L (the reference's build_levelSet_CSC

@@ -108,6 +108,7 @@ def lib() -> ctypes.CDLL:
                                  POINTER(i32)],
             "symamd_plan_levels": [vp, vp, vp],
             "symamd_plan_regions": [vp, POINTER(i32), POINTER(dbl)],
+            "symamd_region_schedule": [vp, vp, i32, i32, POINTER(i32), POINTER(i32)],
             "symamd_choice": [vp, vp, i32, i32, POINTER(i32), POINTER(i32)],
             "symamd_plan_destroy": [vp],
             "symamd_plan_set_lanes": [vp, i32],
@@ -130,6 +131,7 @@ def lib() -> ctypes.CDLL:
             "symamd_sn_analyze": [vp, i32, i32, vp, vp, vp, vp, POINTER(vp)],
             "symamd_sn_solve": [vp, vp, vp, vp],
             "symamd_sn_solve_block": [vp, vp, vp, vp, i32, i64],
+            "symamd_sn_solve_transposed": [vp, vp, vp, vp, i32, i64],
             "symamd_sn_check": [vp, vp],
             "symamd_sn_destroy": [vp],
         }
@@ -226,6 +228,10 @@ class Handle:
     def set_region_band(self, slices: int):
         """Slices per published band of the region executor (0: default)."""
         _check(lib().symamd_set_option(self._h, 12, int(slices)))
+
+    def set_region_mw(self, on: bool):
+        """Region tasks with a separate memory wavefront (True, default)."""
+        _check(lib().symamd_set_option(self._h, 13, int(bool(on))))
 
     def set_owners(self, on: bool):
         """Run plans with regions on the owner schedule (True, default) or
@@ -338,6 +344,15 @@ class Plan:
         cnt, inside = c_int(), c_double()
         _check(lib().symamd_plan_regions(self._p, byref(cnt), byref(inside)))
         return cnt.value, inside.value
+
+    def region_schedule(self, k=1, transposed=False):
+        """(region cap, tasks) of the schedule the region executor runs for
+        a forward (or, transposed, adjoint) solve of k right-hand sides;
+        (0, 0) when such a solve runs sync-free."""
+        cap, tasks = c_int(), c_int()
+        _check(lib().symamd_region_schedule(self.h._h, self._p, int(k), int(bool(transposed)),
+                                            byref(cap), byref(tasks)))
+        return cap.value, tasks.value
 
     def set_lanes(self, lanes: int):
         _check(lib().symamd_plan_set_lanes(self._p, int(lanes)))
@@ -683,6 +698,16 @@ class SupernodalPlan:
             k, n = x.shape
             _check(lib().symamd_sn_solve_block(self.h._h, self._p, _ptr(Lx), _ptr(x), k,
                                                x.stride(0)))
+        return x
+
+    def solve_transposed(self, Lx, x):
+        """L^T X = B in place: x is (n,) or a column-major block (k, n)
+        (parsy's blockedLTsolve semantics, Triangular_BCSC.cpp:125-186)."""
+        _require_cuda(Lx)
+        if not x.is_cuda or x.stride(-1) != 1 or x.dtype != Lx.dtype:
+            raise ValueError("x: a device vector or (k, n) block with unit stride along n")
+        k, ld = (1, x.numel()) if x.dim() == 1 else (x.shape[0], x.stride(0))
+        _check(lib().symamd_sn_solve_transposed(self.h._h, self._p, _ptr(Lx), _ptr(x), k, ld))
         return x
 
     def check(self):

@@ -110,7 +110,10 @@ Choice choose(symamd_handle h, symamd_plan p, int k, int layout, int exec, bool 
   } else {
     c = heuristic(p, k);
     c.exec = exec;
-    if (it != p->tuned.end() && it->second.exec == exec) c.lanes = it->second.lanes;
+    if (it != p->tuned.end() && it->second.exec == exec) {
+      c.lanes = it->second.lanes;
+      c.tile = it->second.tile;
+    }
   }
   if (p->lanes_override > 0) c.lanes = p->lanes_override;
   return c;
@@ -265,8 +268,8 @@ namespace {
 
 std::string graph_key(void *d_B, int k, int64_t rs, int64_t cs, const Choice &c) {
   char buf[160];
-  snprintf(buf, sizeof buf, "%p/%d/%lld/%lld/%d/%d", d_B, k, (long long)rs,
-           (long long)cs, c.exec, c.lanes);
+  snprintf(buf, sizeof buf, "%p/%d/%lld/%lld/%d/%d/%d", d_B, k, (long long)rs,
+           (long long)cs, c.exec, c.lanes, c.tile);
   return buf;
 }
 
@@ -285,13 +288,15 @@ int prepare_solve(symamd_handle h, symamd_plan p, int k, int layout, const Choic
   if (c.exec == SYMAMD_EXEC_REGIONS && region_usable(h, p, k))
     return through_workspace(h, k, layout) ? prepare_workspace(h, p, k) : SYMAMD_OK;
   UnitSet *us = nullptr;
-  SYMAMD_TRY(plan_units(h, p, forward_rows_per_unit(k, c.lanes), false, &us));
+  SYMAMD_TRY(plan_units(h, p, forward_rows_per_unit(k, c.lanes, c.tile), false, &us));
+  const int kt = tile_cols(k, c.tile);
+  SYMAMD_TRY(unit_flags(h, us, (k + kt - 1) / kt));
   if (through_workspace(h, k, layout)) SYMAMD_TRY(prepare_workspace(h, p, k));
   return SYMAMD_OK;
 }
 
 int run_solve(symamd_handle h, symamd_plan p, void *d_B, int k, int64_t ld,
-              int64_t rs, int64_t cs, int layout, int exec, int lanes) {
+              int64_t rs, int64_t cs, int layout, int exec, int lanes, int tile) {
   if (exec == SYMAMD_EXEC_LEVELSET_GRAPH) exec = SYMAMD_EXEC_LEVELSET;
   if (exec == SYMAMD_EXEC_REGIONS) {
     // column-major blocks go through the row-major workspace, the rest in
@@ -303,8 +308,8 @@ int run_solve(symamd_handle h, symamd_plan p, void *d_B, int k, int64_t ld,
     exec = SYMAMD_EXEC_SYNCFREE;
   }
   if (through_workspace(h, k, layout))
-    return launch_forward_ws(h, p, d_B, k, ld, exec, lanes);
-  return launch_forward(h, p, d_B, k, rs, cs, exec, lanes);
+    return launch_forward_ws(h, p, d_B, k, ld, exec, lanes, tile);
+  return launch_forward(h, p, d_B, k, rs, cs, exec, lanes, tile);
 }
 
 int capture_solve(symamd_handle h, symamd_plan p, void *d_B, int k, int64_t ld,
@@ -319,7 +324,7 @@ int capture_solve(symamd_handle h, symamd_plan p, void *d_B, int k, int64_t ld,
   if (hipStreamBeginCapture(cap, hipStreamCaptureModeRelaxed) != hipSuccess) {
     rc = fail(SYMAMD_ERR_HIP, "hipStreamBeginCapture failed");
   } else {
-    rc = run_solve(h, p, d_B, k, ld, rs, cs, layout, c.exec, c.lanes);
+    rc = run_solve(h, p, d_B, k, ld, rs, cs, layout, c.exec, c.lanes, c.tile);
     hipGraph_t graph = nullptr;
     hipError_t e = hipStreamEndCapture(cap, &graph);
     if (rc == SYMAMD_OK && e != hipSuccess)
@@ -356,7 +361,7 @@ int solve_impl(symamd_handle h, symamd_plan p, void *d_B, int k, int64_t ld,
     return SYMAMD_OK;
   }
   SYMAMD_TRY(prepare_solve(h, p, k, layout, c));
-  return run_solve(h, p, d_B, k, ld, rs, cs, layout, c.exec, c.lanes);
+  return run_solve(h, p, d_B, k, ld, rs, cs, layout, c.exec, c.lanes, c.tile);
 }
 
 }  // namespace
@@ -422,6 +427,9 @@ int symamd_set_option(symamd_handle h, int option, int64_t value) {
       return SYMAMD_OK;
     case SYMAMD_OPT_OWNERS:
       h->owners = value != 0;
+      return SYMAMD_OK;
+    case SYMAMD_OPT_REGION_MW:
+      h->region_mw = value != 0;
       return SYMAMD_OK;
     case SYMAMD_OPT_REGION_BAND:
       if (value < 0 || value > 255)
@@ -575,6 +583,18 @@ int symamd_plan_regions(symamd_plan plan, int *count, double *inside) {
   return SYMAMD_OK;
 }
 
+int symamd_region_schedule(symamd_handle h, symamd_plan plan, int k, int transposed,
+                           int *cap, int *tasks) {
+  SYMAMD_TRY(check_handle(h));
+  SYMAMD_TRY(check_plan(plan));
+  if (k < 1) return fail(SYMAMD_ERR_INVALID_ARG, "k must be >= 1");
+  int c = 0, r = 0;
+  region_schedule_info(h, plan, k, transposed != 0, &c, &r);
+  if (cap) *cap = c;
+  if (tasks) *tasks = r;
+  return SYMAMD_OK;
+}
+
 int symamd_plan_destroy(symamd_plan plan) {
   if (!plan) return SYMAMD_OK;
   (void)hipDeviceSynchronize();
@@ -661,8 +681,10 @@ int symamd_adjoint(symamd_handle h, symamd_plan plan, const void *d_x,
   // forced choice without a usable schedule runs sync-free, like forward
   if (exec == SYMAMD_EXEC_REGIONS ? region_adjoint_usable(h, plan, 1)
                                   : exec == SYMAMD_EXEC_AUTO && region_adjoint_preferred(h, plan, 1)) {
-    SYMAMD_TRY(launch_regions_plan(h, plan, d_gx, d_gb, 1, 1, 0, true));
-    return d_gLx ? launch_grad_l(h, plan, d_x, d_gb, d_gLx, 1, 1, 0) : SYMAMD_OK;
+    // grad-L is written by the solve itself where its kernel supports it
+    bool fused = false;
+    SYMAMD_TRY(launch_regions_plan(h, plan, d_gx, d_gb, 1, 1, 0, true, d_x, d_gLx, &fused));
+    return d_gLx && !fused ? launch_grad_l(h, plan, d_x, d_gb, d_gLx, 1, 1, 0) : SYMAMD_OK;
   }
   Choice c = choose(h, plan, 1, SYMAMD_COL_MAJOR, exec, false);
   int e = c.exec == SYMAMD_EXEC_LEVELSET_GRAPH ? SYMAMD_EXEC_LEVELSET : c.exec;
@@ -761,7 +783,6 @@ int symamd_tune(symamd_handle h, symamd_plan plan, int k, symamd_layout layout,
   hipEvent_t e0, e1;
   SYMAMD_HIP(hipEventCreate(&e0));
   SYMAMD_HIP(hipEventCreate(&e1));
-  const int kt = pow2_ceil(k < 64 ? k : 64);
   const int gmax_rows = std::max(1, pow2_ceil((int)std::ceil(plan->avg_row_nnz)) * 2);
   std::string log;
   Choice best;
@@ -770,35 +791,46 @@ int symamd_tune(symamd_handle h, symamd_plan plan, int k, symamd_layout layout,
   plan->lanes_override = 0;
   const int execs[] = {SYMAMD_EXEC_SYNCFREE, SYMAMD_EXEC_LEVELSET_GRAPH, SYMAMD_EXEC_REGIONS};
   int rc = SYMAMD_OK;
+  // candidates: the executor; for the unit executors the column tile (all
+  // of k up to 64 columns, or narrower tiles of 8 to 32 columns solved as
+  // separate work items) and the lanes per row that share a row's entries
   for (int ex : execs) {
-    // the region executor has no lanes-per-row choice and needs a schedule
+    // the region executor has no lanes-per-row or tile choice and needs a schedule
     if (ex == SYMAMD_EXEC_REGIONS && !region_usable(h, plan, k)) continue;
-    const int gmax = ex == SYMAMD_EXEC_REGIONS ? 1 : std::min(64, gmax_rows);
-    for (int g = 1; g * kt <= 64 && g <= gmax; g *= 2) {
-      Choice c;
-      c.exec = ex;
-      c.lanes = g;
-      plan->tuned[tune_key(k, layout)] = c;
-      rc = solve_impl(h, plan, d_B, k, ld, layout, SYMAMD_EXEC_AUTO, true);  // warm / capture
-      if (rc != SYMAMD_OK) break;
-      float ms = 1e30f;
-      for (int rep = 0; rep < 3 && rc == SYMAMD_OK; ++rep) {
-        (void)hipEventRecord(e0, h->stream);
-        rc = solve_impl(h, plan, d_B, k, ld, layout, SYMAMD_EXEC_AUTO, true);
-        (void)hipEventRecord(e1, h->stream);
-        (void)hipEventSynchronize(e1);
-        float t = 0;
-        (void)hipEventElapsedTime(&t, e0, e1);
-        ms = std::min(ms, t);
+    std::vector<int> tiles = {0};
+    if (ex == SYMAMD_EXEC_SYNCFREE)
+      for (int t = 8; t < tile_cols(k, 0); t *= 2) tiles.push_back(t);
+    for (int tile : tiles) {
+      const int kt = tile_cols(k, tile);
+      const int gmax = ex == SYMAMD_EXEC_REGIONS ? 1 : std::min(64, gmax_rows);
+      for (int g = 1; g * kt <= 64 && g <= gmax; g *= 2) {
+        Choice c;
+        c.exec = ex;
+        c.lanes = g;
+        c.tile = tile;
+        plan->tuned[tune_key(k, layout)] = c;
+        rc = solve_impl(h, plan, d_B, k, ld, layout, SYMAMD_EXEC_AUTO, true);  // warm / capture
+        if (rc != SYMAMD_OK) break;
+        float ms = 1e30f;
+        for (int rep = 0; rep < 3 && rc == SYMAMD_OK; ++rep) {
+          (void)hipEventRecord(e0, h->stream);
+          rc = solve_impl(h, plan, d_B, k, ld, layout, SYMAMD_EXEC_AUTO, true);
+          (void)hipEventRecord(e1, h->stream);
+          (void)hipEventSynchronize(e1);
+          float t = 0;
+          (void)hipEventElapsedTime(&t, e0, e1);
+          ms = std::min(ms, t);
+        }
+        if (rc != SYMAMD_OK) break;
+        char line[128];
+        snprintf(line, sizeof line, "exec=%d lanes=%d tile=%d ms=%.4f\n", ex, g, kt, ms);
+        log += line;
+        if (ms < best_ms * 0.97f) {  // prefer earlier (fewer lanes, sync-free) on ties
+          best_ms = ms;
+          best = c;
+        }
       }
       if (rc != SYMAMD_OK) break;
-      char line[128];
-      snprintf(line, sizeof line, "exec=%d lanes=%d ms=%.4f\n", ex, g, ms);
-      log += line;
-      if (ms < best_ms * 0.97f) {  // prefer earlier (fewer lanes, sync-free) on ties
-        best_ms = ms;
-        best = c;
-      }
     }
     if (rc != SYMAMD_OK) break;
   }
@@ -826,7 +858,8 @@ int symamd_tune(symamd_handle h, symamd_plan plan, int k, symamd_layout layout,
   plan->tuned[tune_key(k, layout)] = best;
   g_tune_cache[plan->fingerprint][tune_key(k, layout)] = best;
   char line[128];
-  snprintf(line, sizeof line, "best exec=%d lanes=%d ms=%.4f\n", best.exec, best.lanes, best_ms);
+  snprintf(line, sizeof line, "best exec=%d lanes=%d tile=%d ms=%.4f\n", best.exec, best.lanes,
+           tile_cols(k, best.tile), best_ms);
   log += line;
   if (summary && summary_len > 0) {
     strncpy(summary, log.c_str(), summary_len - 1);
@@ -948,6 +981,14 @@ int symamd_sn_solve_block(symamd_handle h, symamd_snplan plan, const double *d_L
   if (!plan || k < 0 || (k > 1 && ld < plan->n) || (plan->m > 0 && k > 0 && (!d_Lx || !d_X)))
     return fail(SYMAMD_ERR_INVALID_ARG, "symamd_sn_solve_block: bad arguments");
   return sn_solve(h, plan, d_Lx, d_X, k, ld);
+}
+
+int symamd_sn_solve_transposed(symamd_handle h, symamd_snplan plan, const double *d_Lx,
+                               double *d_X, int k, int64_t ld) {
+  SYMAMD_TRY(check_handle(h));
+  if (!plan || k < 0 || (k > 1 && ld < plan->n) || (plan->m > 0 && k > 0 && (!d_Lx || !d_X)))
+    return fail(SYMAMD_ERR_INVALID_ARG, "symamd_sn_solve_transposed: bad arguments");
+  return sn_solve(h, plan, d_Lx, d_X, k, ld, true);
 }
 
 int symamd_sn_check(symamd_handle h, symamd_snplan plan) {

@@ -112,7 +112,8 @@ struct UnitSet {
   int *d_ub = nullptr;      // unit u covers positions [ub[u], ub[u+1])
   int *d_dptr = nullptr;    // dependency units of u:
   int *d_dlist = nullptr;   //   dlist[dptr[u] .. dptr[u+1])
-  int *d_uflags = nullptr;  // ready tags, one per unit
+  int *d_uflags = nullptr;  // ready tags, one per unit and column tile
+  int flag_tiles = 1;       // column tiles the tags have room for
   int64_t deps = 0;
   // owner schedule (plans with regions): unit u belongs to wave
   // owner(region of u) of `nw` co-resident waves; wave w solves
@@ -166,6 +167,7 @@ struct RegionPlan {
   // published, so a value from an earlier launch never satisfies a wait
   int64_t *d_rflags = nullptr;
   int band = 0;             // slices per band the schedule was built with
+  int gp = 0;               // records carry grad-L positions (rec_layout's gp)
   bool transposed = false;  // schedule of L^T x = g (values from the transposed copy)
 };
 
@@ -183,10 +185,14 @@ struct RegionPlan {
 // unchanged, bit for bit.
 struct RecLayout {
   int pval, diag, lval, code, cnt, rows, halo, slices, bands, hband, wptr, deps, lep, lcode,
-      bytes;
+      gpos, bytes;
 };
+// gp: the record also carries, for the transposed (adjoint) solve, the
+// caller's CSR position of every entry of L it holds (PACK + 1 per row --
+// the packed entries, padding 0, then the diagonal -- then one per overflow
+// entry), so the solve writes grad-L itself.
 __host__ __device__ inline RecLayout rec_layout(int nr, int nh, int ne, int ns, int nd,
-                                                int nb, int vs) {
+                                                int nb, int vs, int gp = 0) {
   constexpr int PK = RegionPlan::PACK;
   RecLayout L;
   int o = 0;
@@ -208,12 +214,16 @@ __host__ __device__ inline RecLayout rec_layout(int nr, int nh, int ne, int ns, 
   put(L.deps, nd * 4);
   put(L.lep, ne > 0 ? (nr + 1) * 4 : 0);
   put(L.lcode, ne * 2);
+  put(L.gpos, gp ? ((PK + 1) * nr + ne) * 4 : 0);
   L.bytes = o;
   return L;
 }
 // Slot codes are uint16: a region with its halo and the zero slot may use
 // at most this many LDS slots.
 constexpr int REGION_SLOTS_MAX = 65535;
+// Smallest region cap tried when a schedule's task does not fit a
+// workgroup's LDS at the default cap.
+constexpr int REGION_CAP_MIN = 32;
 // A region publishes its progress band by band (a band is a few consecutive
 // slices, at most REGION_BANDS_MAX per region).  A wait entry packs the rank
 // of the region read (low REGION_RANK_BITS bits) with the number of its
@@ -260,11 +270,16 @@ struct HostRecords {
   std::vector<int> rhead, vsrc, vdst;
   int recmax = 0;
 };
-bool pack_region_records(const HostRegions &o, int vs, HostRecords &out);
+// gmap (may be null): maps the builder's entry positions (psrc, esrc,
+// dsrc) to the caller's CSR positions; when given, the records carry them
+// (rec_layout's gp section).
+bool pack_region_records(const HostRegions &o, int vs, HostRecords &out,
+                         const std::vector<int> *gmap = nullptr);
 
 struct Choice {
   int exec = SYMAMD_EXEC_SYNCFREE;
   int lanes = 1;  // lanes cooperating on one row's nonzeros
+  int tile = 0;   // columns per tile of the unit executors (0: from k)
 };
 
 }  // namespace symamd
@@ -287,6 +302,7 @@ struct symamd_handle_s {
   int region_cap = 0;       // region executor: region size cap (0: by k)
   int region_waves = 0;     // waves per region task (0: chosen from k)
   int region_band = 0;      // slices per published band (0: the default)
+  int region_mw = 1;        // region executor: memory-wave kernel where it applies
   uint64_t timeout_ticks() const { return (uint64_t)(timeout_ms * 1.0e5); }
 };
 
@@ -369,6 +385,9 @@ struct symamd_snplan_s {
   int *d_cptr = nullptr, *d_cidx = nullptr, *d_dptr = nullptr, *d_dlist = nullptr;
   double *d_T = nullptr;
   int *d_flags = nullptr, *d_state = nullptr;
+  // transposed solve: rows below (temporaries order), the supernodes owning
+  // them per supernode
+  int *d_rowb = nullptr, *d_tdptr = nullptr, *d_tdlist = nullptr;
 };
 
 struct symamd_graph_s {
@@ -392,7 +411,7 @@ int plan_oci(symamd_handle h, symamd_plan p);
 
 // Kernel launchers (executors.hip / adjoint.hip / spmv.hip / inspector.hip)
 int launch_forward(symamd_handle h, symamd_plan p, void *d_B, int k,
-                   int64_t rs, int64_t cs, int exec, int lanes);
+                   int64_t rs, int64_t cs, int exec, int lanes, int tile);
 int launch_adjoint(symamd_handle h, symamd_plan p, const void *d_x,
                    const void *d_gx, void *d_gb, void *d_gL, int exec,
                    int lanes);
@@ -410,11 +429,14 @@ int launch_spmv(symamd_handle h, int n, const int *d_Ap, const int *d_Ai,
 int plan_residual(symamd_handle h, symamd_plan p, const void *d_X,
                   const void *d_B, int k, int64_t rs, int64_t cs,
                   double *h_rel);
-int forward_rows_per_unit(int k, int lanes);
+int forward_rows_per_unit(int k, int lanes, int tile);
+int tile_cols(int k, int tile);
+// Ready tags of a unit set for `tiles` column tiles (grown on demand).
+int unit_flags(symamd_handle h, UnitSet *us, int tiles);
 // Column-major multi-RHS solves through a level-ordered workspace.
 int prepare_workspace(symamd_handle h, symamd_plan p, int k);
 int launch_forward_ws(symamd_handle h, symamd_plan p, void *d_B, int k, int64_t ld,
-                      int exec, int lanes);
+                      int exec, int lanes, int tile);
 // Coarsened 3-array schedule on the device (lbc.hip).
 int lbc_schedule(symamd_handle h, int n, const int *d_Lp, const int *d_Li,
                  const double *d_cost, int parts, int min_level_dist, int div_rate,
@@ -432,10 +454,11 @@ int cluster_regions(int n, const int *Lp, const int *Li, int max_rows,
                     std::vector<int> &region);
 int order_by_regions(symamd_handle h, symamd_plan p, const int *d_Lp, const int *d_Li);
 int plan_owners(symamd_handle h, symamd_plan p, UnitSet *us, int nw);
-// Region schedule of the plan for region cap `cap` (built on first use;
-// rp->R == 0 when the region graph has a cycle or there is nothing to gain),
-// of the forward solve or, transposed, of L^T x = g.
-int plan_regions(symamd_handle h, symamd_plan p, int cap, RegionPlan **out,
+// Region schedule of the plan for region cap `cap` and the band of k
+// right-hand sides (built on first use; rp->R == 0 when the region graph
+// has a cycle or there is nothing to gain), of the forward solve or,
+// transposed, of L^T x = g.
+int plan_regions(symamd_handle h, symamd_plan p, int cap, int k, RegionPlan **out,
                  bool transposed = false);
 void free_region_plan(RegionPlan &rp);
 // Re-gathers the region schedules' values from the plan's level-ordered
@@ -454,7 +477,8 @@ int launch_regions_ws(symamd_handle h, symamd_plan p, const void *d_in, void *d_
 // (which may be the same buffer); SYMAMD_ERR_NOT_SUPPORTED when the
 // schedule is not usable.
 int launch_regions_plan(symamd_handle h, symamd_plan p, const void *d_B, void *d_X, int k,
-                        int64_t rs, int64_t cs, bool transposed);
+                        int64_t rs, int64_t cs, bool transposed, const void *d_xf = nullptr,
+                        void *d_gL = nullptr, bool *fused = nullptr);
 // Whether the transposed region schedule for k right-hand sides exists
 // (built on first use) and fits the LDS of a workgroup; whether the
 // automatic choice takes it (at least 60 % of the entries inside regions).
@@ -468,10 +492,14 @@ int launch_grad_l(symamd_handle h, symamd_plan p, const void *d_X, const void *d
 // the region's and its halo's values of one column group per row), unless
 // the handle sets one.
 int region_cap(symamd_handle h, int k);
-int region_band(symamd_handle h);
+int region_band(symamd_handle h, int k);
 // True when a region schedule for k right-hand sides exists (building it on
 // first use) and its buffers fit one wave's LDS.
 bool region_usable(symamd_handle h, symamd_plan p, int k);
+// Cap and task count (regions times column groups) of the schedule a region
+// solve of k right-hand sides runs; 0 and 0 when it runs sync-free.
+void region_schedule_info(symamd_handle h, symamd_plan p, int k, bool transposed, int *cap,
+                          int *tasks);
 // Whether the automatic choice takes the region executor: a usable region
 // schedule with at least 60 % of L's off-diagonal entries inside regions
 // (column-major blocks only through the workspace).
@@ -494,7 +522,7 @@ void ilu0_destroy(symamd_ilu0_s *q);
 int sn_analyze(symamd_handle h, int n, int m, const int *h_sup, const int *h_nrows,
                const int *h_Lp, const int *h_Li, symamd_snplan_s **out);
 int sn_solve(symamd_handle h, symamd_snplan_s *q, const double *d_Lx, double *d_x, int k,
-             int64_t ldx);
+             int64_t ldx, bool transposed = false);
 int sn_check(symamd_handle h, symamd_snplan_s *q);
 void sn_destroy(symamd_snplan_s *q);
 int ilu_lower(symamd_handle h, int n, const int *d_Ap, const int *d_Ai,

@@ -14,6 +14,27 @@ __device__ __forceinline__ void trace_mark(uint64_t *trace, int64_t cap,
     trace[3 * (int64_t)u + slot] = wall_ticks();
 }
 
+// The mark: a signalling-NaN bit pattern that no arithmetic produces
+// (operations return quiet NaNs).  In the marked protocols a unit (or region task)
+// overwrites its rows' right-hand sides with it right after reading them,
+// so a solved value can be told apart from one not yet stored.
+__device__ __forceinline__ bool is_mark(double v) {
+  return __double_as_longlong(v) == (long long)0x7FF4D0D0CAFEF00Dull;
+}
+__device__ __forceinline__ bool is_mark(float v) {
+  return __float_as_uint(v) == 0x7FA0CAFEu;
+}
+template <typename T>
+__device__ __forceinline__ T mark_value();
+template <>
+__device__ __forceinline__ double mark_value<double>() {
+  return __longlong_as_double((long long)0x7FF4D0D0CAFEF00Dull);
+}
+template <>
+__device__ __forceinline__ float mark_value<float>() {
+  return __uint_as_float(0x7FA0CAFEu);
+}
+
 // LDS writes of the wave are complete and visible to all its lanes.
 __device__ __forceinline__ void wave_lds_sync() {
   asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");

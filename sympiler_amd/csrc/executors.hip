@@ -82,6 +82,7 @@ struct SolveArgs {
   // dependencies owned by other waves
   const int *wptr, *wlist;
   int U;
+  int tiles;                 // column tiles of KT columns: work item = unit * tiles + tile
   int ticketed;              // 1: units from the ticket counter
   int reverse;               // 1: units visited from last to first
   uint64_t timeout;
@@ -112,31 +113,13 @@ __device__ __forceinline__ void st_x(T *p, T v) {
 
 // Dependency values loaded per lane and row before any is consumed: a row
 // with at most dep_chunk(G) (strided) entries needs one memory round trip.
-// Eight when several lanes share a row (long rows), else four: with one lane
-// per row the rows are short and eight slots only add dead loads.
-template <int G>
-constexpr int dep_chunk() { return G >= 8 ? 8 : 4; }
+// Eight when several lanes share a row (long rows) or a row has 32 or more
+// column lanes (few rows per wave, each with every load of a chunk shared
+// by the whole tile), else four: with one lane per row and few columns the
+// rows are short and eight slots only add dead loads.
+template <int G, int KT>
+constexpr int dep_chunk() { return G >= 8 || KT >= 32 ? 8 : 4; }
 
-// The mark: a signalling-NaN bit pattern that no arithmetic produces
-// (operations return quiet NaNs).  In the marked sync-free protocol a unit
-// overwrites its rows' right-hand sides with it right after reading them,
-// so a solved value can be told apart from one not yet stored.
-__device__ __forceinline__ bool is_mark(double v) {
-  return __double_as_longlong(v) == (long long)0x7FF4D0D0CAFEF00Dull;
-}
-__device__ __forceinline__ bool is_mark(float v) {
-  return __float_as_uint(v) == 0x7FA0CAFEu;
-}
-template <typename T>
-__device__ __forceinline__ T mark_value();
-template <>
-__device__ __forceinline__ double mark_value<double>() {
-  return __longlong_as_double((long long)0x7FF4D0D0CAFEF00Dull);
-}
-template <>
-__device__ __forceinline__ float mark_value<float>() {
-  return __uint_as_float(0x7FA0CAFEu);
-}
 
 // Rows a lane finished in the transposed solve, kept for the grad-L pass
 // that runs after the unit is published (off the dependency chain).
@@ -182,17 +165,17 @@ __device__ __forceinline__ void finish_rows(const SolveArgs<T> &a,
                                             const int (&je)[NR], const T (&diag)[NR],
                                             T (&bv)[NR], const bool (&act)[NR],
                                             int iters, const int (&qpos)[NR],
-                                            Solved<T, NR> &sv) {
+                                            Solved<T, NR> &sv, int cb, int ce) {
   constexpr bool SYNC = SY != SY_LEVEL;
-  constexpr int DC = dep_chunk<G>();
+  constexpr int DC = dep_chunk<G, KT>();
   const int lane = threadIdx.x & 63;
   const int s = (lane / KT) % G;
   const int c = lane % KT;
-  for (int c0 = 0; c0 < a.k; c0 += KT) {
+  for (int c0 = cb; c0 < ce; c0 += KT) {
     const int col = c0 + c;
     const bool colok = col < a.k;
     const int64_t coff = (int64_t)col * a.cs;
-    if (c0 > 0) {
+    if (c0 > cb) {
 #pragma unroll
       for (int r = 0; r < NR; ++r)
         bv[r] = (act[r] && colok) ? a.B[(int64_t)i[r] * a.rs + coff] : T(0);
@@ -202,7 +185,7 @@ __device__ __forceinline__ void finish_rows(const SolveArgs<T> &a,
     for (int r = 0; r < NR; ++r) acc[r] = G == 1 ? bv[r] : T(0);
     // Branch-free chunk: dead slots load the lane's own right-hand side
     // address (always valid) and are discarded, so the compiler keeps all
-    // dep_chunk<G>() * NR loads in flight instead of waiting at every branch.
+    // dep_chunk<G, KT>() * NR loads in flight instead of waiting at every branch.
     const int64_t coff_safe = colok ? coff : 0;
     for (int t0 = 0; t0 < iters; t0 += DC) {
       T xd[NR][DC];
@@ -283,7 +266,7 @@ __device__ __forceinline__ void finish_rows(const SolveArgs<T> &a,
 // (FLAT loads count against both wait counters and would serialize the
 // dependency loads).
 template <typename T, int G, int KT, int NR, int SY, bool ADJ>
-__device__ __forceinline__ void unit_rows(const SolveArgs<T> &a, int u, int q0,
+__device__ __forceinline__ void unit_rows(const SolveArgs<T> &a, int u, int tile, int q0,
                                           int nrows, const Slab<T> &sl,
                                           const int *cis, const T *vals,
                                           int target, Solved<T, NR> &sv) {
@@ -297,7 +280,7 @@ __device__ __forceinline__ void unit_rows(const SolveArgs<T> &a, int u, int q0,
   T diag[NR], bv[NR];
   bool act[NR];
   int iters = 0;
-  const int col0 = c;
+  const int col0 = tile * KT + c;
   const bool colok0 = col0 < a.k;
 #pragma unroll
   for (int r = 0; r < NR; ++r) {
@@ -336,22 +319,25 @@ __device__ __forceinline__ void unit_rows(const SolveArgs<T> &a, int u, int q0,
     publish_fence();
   }
   if constexpr (SY != SY_LEVEL) {
-    wait_units(a.dptr, a.dlist, u, a.uflags, target, a.state, a.timeout);
-    trace_mark(a.trace, a.trace_cap, u, 1);
+    // the same column tile of the units it depends on
+    const int d0 = a.dptr[u];
+    wait_list(a.dlist + d0, a.dptr[u + 1] - d0, a.uflags, target, a.state, a.timeout, a.tiles,
+              tile);
+    trace_mark(a.trace, a.trace_cap, u * a.tiles + tile, 1);
   }
   // 3. solve
   int qpos[NR];
 #pragma unroll
   for (int r = 0; r < NR; ++r) qpos[r] = q0 + rr + r * RPW;
   finish_rows<T, G, KT, NR, SY, ADJ>(a, cis, vals, i, jb, je, diag, bv, act, iters,
-                                     qpos, sv);
+                                     qpos, sv, tile * KT, tile * KT + KT);
 }
 
 // Solve unit u with one wave.  ADJ selects the transposed solve (diagonal
 // first in each row of the copy); its finished rows come back in `sv` for
 // the grad-L pass.
 template <typename T, int G, int KT, int NR, int SY, bool ADJ>
-__device__ __forceinline__ void solve_unit(const SolveArgs<T> &a, int u,
+__device__ __forceinline__ void solve_unit(const SolveArgs<T> &a, int u, int tile,
                                            Slab<T> &sl, int target,
                                            Solved<T, NR> &sv) {
   const int lane = threadIdx.x & 63;
@@ -370,10 +356,10 @@ __device__ __forceinline__ void solve_unit(const SolveArgs<T> &a, int u,
       sl.val[t] = a.val[e0 + t];
     }
     wave_lds_sync();
-    unit_rows<T, G, KT, NR, SY, ADJ>(a, u, q0, nrows, sl, sl.ci, sl.val, target, sv);
+    unit_rows<T, G, KT, NR, SY, ADJ>(a, u, tile, q0, nrows, sl, sl.ci, sl.val, target, sv);
   } else {
     wave_lds_sync();
-    unit_rows<T, G, KT, NR, SY, ADJ>(a, u, q0, nrows, sl, a.ci + e0, a.val + e0,
+    unit_rows<T, G, KT, NR, SY, ADJ>(a, u, tile, q0, nrows, sl, a.ci + e0, a.val + e0,
                                      target, sv);
   }
 }
@@ -395,25 +381,31 @@ __global__ __launch_bounds__(256) void k_syncfree(SolveArgs<T> a) {
     w1 = a.wptr[wave + 1];
   }
   cursor = a.wlist ? w0 : cursor;
+  // work items: unit-major, column tile minor (a unit's tiles are solved
+  // independently, each waiting for the same tile of its dependencies)
+  const int items = a.U * a.tiles;
   for (;;) {
-    int u;
+    int u, tile = 0;
     if (a.wlist) {
       if (cursor >= w1) break;
       const int t = cursor++;
       u = a.wlist[a.reverse ? w1 - 1 - (t - w0) : t];
     } else {
       const int t = next_unit(a.state, a.ticketed, cursor, stride);
-      if (t >= a.U) break;
-      u = a.reverse ? a.U - 1 - t : t;
+      if (t >= items) break;
+      const int ul = t / a.tiles;
+      tile = t - ul * a.tiles;
+      u = a.reverse ? a.U - 1 - ul : ul;
     }
-    trace_mark(a.trace, a.trace_cap, u, 0);
+    const int item = u * a.tiles + tile;
+    trace_mark(a.trace, a.trace_cap, item, 0);
     Solved<T, NR> sv;
-    solve_unit<T, G, KT, NR, SY, ADJ>(a, u, sl, target, sv);
+    solve_unit<T, G, KT, NR, SY, ADJ>(a, u, tile, sl, target, sv);
     // marked protocol: the tag is only a hint (consumers re-check each value
     // against the mark), so it need not wait for the value stores
     if constexpr (SY != SY_MARKS) publish_fence();
-    if (lane == 0) st_dev(&a.uflags[u], target);
-    trace_mark(a.trace, a.trace_cap, u, 2);
+    if (lane == 0) st_dev(&a.uflags[item], target);
+    trace_mark(a.trace, a.trace_cap, item, 2);
     if constexpr (ADJ) grad_rows<T, G, NR>(a, sv);
   }
   persistent_finish(a.state, a.total_waves);
@@ -452,7 +444,7 @@ __global__ __launch_bounds__(256) void k_level(SolveArgs<T> a, int q0, int q1) {
   }
   Solved<T, 1> sv;
   finish_rows<T, G, KT, 1, SY_LEVEL, ADJ>(a, a.ci, a.val, i, jb, je, diag, bv, act,
-                                          iters, qpos, sv);
+                                          iters, qpos, sv, 0, a.k);
   if constexpr (ADJ) grad_rows<T, G, 1>(a, sv);
 }
 
@@ -528,7 +520,7 @@ int launch_persistent(symamd_handle h, symamd_plan p, UnitSet *us, const void *k
   a.wptr = nullptr;
   a.wlist = nullptr;
   const bool coop = !capturing(h->stream) && !h->force_tickets;
-  if (coop && p->d_preg && h->owners) {
+  if (coop && p->d_preg && h->owners && a.tiles == 1) {
     const int grid = resident_grid(h, kfn, 1 << 30);
     SolveArgs<T> b = a;
     b.total_waves = grid * 4;
@@ -544,7 +536,7 @@ int launch_persistent(symamd_handle h, symamd_plan p, UnitSet *us, const void *k
       return SYMAMD_OK;
     (void)hipGetLastError();
   }
-  const int grid = resident_grid(h, kfn, a.U);
+  const int grid = resident_grid(h, kfn, a.U * a.tiles);
   a.total_waves = grid * 4;
   if (coop) {
     a.ticketed = 0;
@@ -560,12 +552,38 @@ int launch_persistent(symamd_handle h, symamd_plan p, UnitSet *us, const void *k
   return SYMAMD_OK;
 }
 
+}  // namespace
+
+// Ready tags for `tiles` column tiles of every unit (tag unit * tiles +
+// tile), grown on demand; new tags are cleared (never "ready").  Nothing is
+// allocated when the capacity suffices, as inside graph capture, where the
+// solve's preparation has grown it beforehand.
+int unit_flags(symamd_handle h, UnitSet *us, int tiles) {
+  if (tiles <= us->flag_tiles) return SYMAMD_OK;
+  if (capturing(h->stream))
+    return fail(SYMAMD_ERR_NOT_SUPPORTED, "unit tags for %d column tiles not prepared", tiles);
+  const size_t cnt = (size_t)(us->count > 0 ? us->count : 1) * tiles;
+  int *f = nullptr;
+  SYMAMD_HIP(hipStreamSynchronize(h->stream));
+  SYMAMD_HIP(hipMalloc(&f, sizeof(int) * cnt));
+  SYMAMD_HIP(hipMemset(f, 0, sizeof(int) * cnt));
+  (void)hipFree(us->d_uflags);
+  us->d_uflags = f;
+  us->flag_tiles = tiles;
+  return SYMAMD_OK;
+}
+
+namespace {
+
 template <typename T, int G, int KT, int NR, bool ADJ>
 int run(symamd_handle h, symamd_plan p, SolveArgs<T> a, int exec) {
   constexpr int R = NR * 64 / (G * KT);
   static_assert(R <= UNIT_ROWS_MAX, "unit rows exceed the LDS slab");
   UnitSet *us = nullptr;
   SYMAMD_TRY(plan_units(h, p, R, ADJ, &us));
+  a.tiles = exec == SYMAMD_EXEC_SYNCFREE ? (a.k + KT - 1) / KT : 1;
+  if (a.tiles < 1) a.tiles = 1;
+  SYMAMD_TRY(unit_flags(h, us, a.tiles));
   a.ub = us->d_ub;
   a.U = us->count;
   a.dptr = us->d_dptr;
@@ -760,9 +778,9 @@ int prepare_workspace(symamd_handle h, symamd_plan p, int k) {
 // there with rows = positions (coalesced right-hand sides, dependency ids
 // in position numbering), transpose back.  prepare_workspace must have run.
 int launch_forward_ws(symamd_handle h, symamd_plan p, void *d_B, int k, int64_t ld,
-                      int exec, int lanes) {
+                      int exec, int lanes, int tile) {
   if (p->n == 0 || k == 0) return SYMAMD_OK;
-  const int kt = pow2_ceil(k < 64 ? k : 64);
+  const int kt = tile_cols(k, tile);
   if (lanes * kt > 64) lanes = 64 / kt;
   dim3 grid((p->n + 63) / 64, (k + 31) / 32);
   if (p->dtype == SYMAMD_F64) {
@@ -813,8 +831,16 @@ int launch_regions_ws(symamd_handle h, symamd_plan p, const void *d_in, void *d_
   return SYMAMD_OK;
 }
 
-int forward_rows_per_unit(int k, int lanes) {
+// Columns per tile: `tile` when given (a power of two, at most 64), else
+// the columns of k up to 64.
+int tile_cols(int k, int tile) {
   int kt = pow2_ceil(k < 64 ? (k > 0 ? k : 1) : 64);
+  if (tile > 0 && tile < kt) kt = pow2_ceil(tile);
+  return kt;
+}
+
+int forward_rows_per_unit(int k, int lanes, int tile) {
+  const int kt = tile_cols(k, tile);
   if (lanes * kt > 64) lanes = 64 / kt;
   const int lpr = lanes * kt;
   return rows_per_lane(lpr) * 64 / lpr;
@@ -840,9 +866,9 @@ int launch_lbc(symamd_handle h, int n, const int *d_Lp, const int *d_Li,
 }
 
 int launch_forward(symamd_handle h, symamd_plan p, void *d_B, int k, int64_t rs,
-                   int64_t cs, int exec, int lanes) {
+                   int64_t cs, int exec, int lanes, int tile) {
   if (p->n == 0 || k == 0) return SYMAMD_OK;
-  const int kt = pow2_ceil(k < 64 ? k : 64);
+  const int kt = tile_cols(k, tile);
   if (lanes * kt > 64) lanes = 64 / kt;
   if (p->dtype == SYMAMD_F64)
     return fwd_dispatch<double>(h, p, forward_args<double>(h, p, d_B, k, rs, cs), exec,

@@ -66,6 +66,12 @@ struct RegionArgs {
   int *state;
   int R, groups, smax, hmax;
   int xsb;  // LDS bytes of the value slots; the record image follows
+  // transposed single right-hand side with grad-L (records with positions):
+  // the forward solution of the task's rows is loaded to LDS at xfb and
+  // gL[p] = -(gb[i] * x[j]) is written for every entry the task holds
+  int gp, xfb;
+  const T *xf;
+  T *gL;
   int total_waves;
   uint64_t timeout;
   // optional per-task trace (100 MHz clock): [3t] task taken, [3t+1] its
@@ -87,8 +93,15 @@ int task_cols(int k) { return k >= KG_MAX ? KG_MAX : pow2_ceil(k < 1 ? 1 : k); }
 int xs_bytes(const RegionPlan &rp, int kt, int vs) {
   return (int)((((int64_t)rp.smax + rp.hmax + 1) * kt * vs + 15) & ~int64_t(15));
 }
-size_t region_lds_bytes(const RegionPlan &rp, int kt, int vs) {
+size_t region_image_bytes(const RegionPlan &rp, int kt, int vs) {
   return (size_t)xs_bytes(rp, kt, vs) + (size_t)rp.recmax;
+}
+// Everything a task may hold: the slots and record image and, for records
+// that carry grad-L positions (single right-hand sides), the forward
+// solution of the region's rows after them.
+size_t region_lds_bytes(const RegionPlan &rp, int kt, int vs) {
+  const size_t base = region_image_bytes(rp, kt, vs);
+  return rp.gp && kt == 1 ? ((base + 15) & ~(size_t)15) + (size_t)rp.smax * vs : base;
 }
 
 __device__ __forceinline__ void vm_wait() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
@@ -464,7 +477,7 @@ __global__ __launch_bounds__(64 * NW) void k_regions(RegionArgs<T> a) {
     const int ns = __builtin_amdgcn_readfirstlane(h0.w);
     const int nd = __builtin_amdgcn_readfirstlane(h1.x);
     const int nb = __builtin_amdgcn_readfirstlane(h1.w);
-    const RecLayout L = rec_layout(nr, nh, ne, ns, nd, nb, VS);
+    const RecLayout L = rec_layout(nr, nh, ne, ns, nd, nb, VS, a.gp);
     const int *lrow = reinterpret_cast<const int *>(rb + L.rows);
     const int *lhalo = reinterpret_cast<const int *>(rb + L.halo);
     const int *ldep = reinterpret_cast<const int *>(rb + L.deps);
@@ -562,6 +575,347 @@ __global__ __launch_bounds__(64 * NW) void k_regions(RegionArgs<T> a) {
   persistent_finish(a.state, a.total_waves);
 }
 
+// ----------------------------------------------- memory-wave specialization
+// The same tasks with the memory traffic taken off the compute waves: a
+// workgroup holds NC compute waves and one memory wave.  The memory wave
+// takes the tickets, copies the record, loads the right-hand sides, and band
+// by band waits for the bands of the regions the band reads, loads its halo
+// rows, and -- as the compute waves finish a band -- stores that band's rows,
+// waits for the stores and publishes the band.  The compute waves only solve
+// slices from LDS: they wait for an LDS counter of loaded bands and count the
+// bands they finished in another, so a compute wave never waits for a
+// global store, and a band is published as soon as its stores land while
+// the next band is being solved.  Every wait is bounded by the watchdog.
+template <typename T>
+__device__ __forceinline__ int lds_get(const int *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// One round of the waits of bands b0 .. b1-1 (at most 64 entries read):
+// the first of these bands that is not ready yet (b1 when all are ready).
+// Entry e of the region's list belongs to band b when wptr[b] <= e <
+// wptr[b+1]; it is satisfied when flags[rank * groups + g] >= base + bands.
+__device__ __forceinline__ int bands_ready(const int *list, const int *wptr, int b0, int b1,
+                                           const int64_t *flags, int64_t base, int groups,
+                                           int g, int R, bool marked_only) {
+  const int lane = threadIdx.x & 63;
+  const int e0 = wptr[b0];
+  const int e1 = min(wptr[b1], e0 + 64);
+  // bands whose entries are not all read in this round count as not ready
+  int first = b1;
+  if (wptr[b1] > e1) {
+    first = b0;
+    while (first < b1 && wptr[first + 1] <= e1) ++first;
+  }
+  const int e = e0 + lane;
+  int bad = INT_MAX;
+  if (e < e1) {
+    const int v = list[e];
+    int rank = v & ((1 << REGION_RANK_BITS) - 1);
+    rank = rank < R ? rank : 0;
+    const int64_t want = base + (marked_only ? 1 : (v >> REGION_RANK_BITS));
+    if (ld_dev(&flags[(int64_t)rank * groups + g]) < want) {
+      int b = b0;
+      while (wptr[b + 1] <= e) ++b;
+      bad = b;
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) bad = min(bad, __shfl_xor(bad, off));
+  return min(first, bad);
+}
+
+// One round over every entry of list[0 .. count): true when each region it
+// names has published that its rows are marked (flag >= base + 1).
+__device__ __forceinline__ bool all_marked(const int *list, int count, const int64_t *flags,
+                                           int64_t base, int groups, int g, int R) {
+  const int lane = threadIdx.x & 63;
+  bool pending = false;
+  for (int e = lane; e < count; e += 64) {
+    int rank = list[e] & ((1 << REGION_RANK_BITS) - 1);
+    rank = rank < R ? rank : 0;
+    pending |= ld_dev(&flags[(int64_t)rank * groups + g]) < base + 1;
+  }
+  return !__any(pending);
+}
+
+// grad-L of the rows of bands [b0, b1) of a task of the transposed solve:
+// every entry (i, j) of L the task holds gets -(gb[i] * x[j]), gb[i] final
+// in LDS (the task's own rows, or its halo, loaded and no longer marked
+// before the band that reads it was solved) and x[j] the row's forward
+// solution (in LDS at xfl); positions from the record's gp section.
+template <typename T>
+__device__ __forceinline__ void grad_bands(T *gL, const T *xs, const T *xfl, const uint8_t *rb,
+                                           const RecLayout &L, const int *lsl, const int *lband,
+                                           int b0, int b1, int nr, int ns, int ne) {
+  constexpr int PK = RegionPlan::PACK;
+  const int lane = threadIdx.x & 63;
+  const int *gpos = reinterpret_cast<const int *>(rb + L.gpos);
+  const uint16_t *code = reinterpret_cast<const uint16_t *>(rb + L.code);
+  const uint8_t *cnt8 = rb + L.cnt;
+  const int *lep = reinterpret_cast<const int *>(rb + L.lep);
+  const uint16_t *lcode = reinterpret_cast<const uint16_t *>(rb + L.lcode);
+  const int r0 = min(max(lsl[min(lband[b0], ns)], 0), nr);
+  const int r1 = min(max(lsl[min(lband[b1], ns)], 0), nr);
+  for (int l = r0 + lane; l < r1; l += 64) {
+    const T xj = xfl[l];
+    const int *gl = gpos + (PK + 1) * l;
+    const int c = cnt8[l];
+#pragma unroll
+    for (int u = 0; u < PK; ++u)
+      if (u < c) gL[gl[u]] = -(xs[code[PK * l + u]] * xj);
+    if (c > PK)
+      for (int e = max(lep[l], 0), e1 = min(lep[l + 1], ne); e < e1; ++e)
+        gL[gpos[(PK + 1) * nr + e]] = -(xs[lcode[e]] * xj);
+    gL[gl[PK]] = -(xs[l] * xj);
+  }
+}
+
+// MK (single right-hand sides): the marked protocol.  A task marks its rows
+// in X (after reading their right-hand sides) and publishes "marked" once;
+// a task that reads them waits only for that, loads its halo values as soon
+// as it reaches their band and loads again any value still carrying the
+// mark.  Stores are never waited for, and no band is published: a value is
+// ready when it is no longer the mark.
+template <typename T, int KT, int ES, int NC, bool MK>
+__global__ __launch_bounds__(64 * (NC + 1)) void k_regions_mw(RegionArgs<T> a) {
+  extern __shared__ __align__(16) uint8_t smem[];
+  constexpr int KS = log2c<KT>();
+  constexpr int VS = (int)sizeof(T);
+  constexpr int CPL = KT >= 2 ? 2 : 1;
+  constexpr int P = 1;
+  constexpr int LOOK = 8;  // bands whose waits one round polls
+  static_assert(NC == 1 || NC * CPL == KT, "compute waves split the columns");
+  static_assert(!MK || KT == 1, "the marked protocol is for single right-hand sides");
+  // [0] task, [1..8] its header, [9] bands loaded, [10 + w] bands compute
+  // wave w finished
+  __shared__ int s_ctl[10 + NC];
+  T *xs = reinterpret_cast<T *>(smem);
+  uint8_t *rb = smem + a.xsb;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wid = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const bool memw = wid == NC;
+  const int target = ld_dev(&a.state[ST_EPOCH]) + 1;
+  const int64_t base = (int64_t)target << 8;
+  const int ntasks = a.R * a.groups;
+  if (tid < KT) xs[(size_t)(a.smax + a.hmax) * KT + tid] = T(0);
+  if (memw) {
+    int t = 0;
+    if (lane == 0) t = atomicAdd(&a.state[ST_TICKET], 1);
+    t = __builtin_amdgcn_readlane(t, 0);
+    if (lane < 8) s_ctl[1 + lane] = t < ntasks ? a.head[8 * (t / a.groups) + lane] : 0;
+    if (lane == 0) s_ctl[0] = t;
+  }
+  __syncthreads();
+  int task = s_ctl[0];
+  while (task < ntasks) {
+    const int nr = s_ctl[1], nh = s_ctl[2], ne = s_ctl[3], ns = s_ctl[4], nd = s_ctl[5];
+    const int ru = s_ctl[6], rn = s_ctl[7], nb = s_ctl[8];
+    const int g = task % a.groups;
+    const int c0 = g * KT;
+    const int kc = a.k - c0 < KT ? a.k - c0 : KT;
+    const RecLayout L = rec_layout(nr, nh, ne, ns, nd, nb, VS, a.gp);
+    const int *lrow = reinterpret_cast<const int *>(rb + L.rows);
+    const int *lhalo = reinterpret_cast<const int *>(rb + L.halo);
+    const int *ldep = reinterpret_cast<const int *>(rb + L.deps);
+    const int *lsl = reinterpret_cast<const int *>(rb + L.slices);
+    const int *lband = reinterpret_cast<const int *>(rb + L.bands);
+    const int *lhb = reinterpret_cast<const int *>(rb + L.hband);
+    const int *lwp = reinterpret_cast<const int *>(rb + L.wptr);
+    if (wid == 0) trace_mark(a.trace, a.trace_cap, task, 0);
+    // 1. the record, every wave a share of it
+    dma_bytes<NC + 1>(rb, a.rec + (size_t)ru * 16, rn * 16, wid);
+    vm_wait();
+    if (tid <= NC) s_ctl[9 + tid] = 0;
+    __syncthreads();
+    if (memw) {
+      int ticket = 0;
+      if (lane == 0) ticket = atomicAdd(&a.state[ST_TICKET], 1);
+      int64_t *myflag = &a.tflags[task];
+      // 2. the right-hand sides of the rows (waited for with the first halo);
+      // marked protocol: then mark the rows and publish that they are marked
+      load_rows<T, KT, ES, false, 1>(xs, a.B, lrow, nr, kc, c0, a.rs, a.cs, a.n, 0);
+      if constexpr (MK) {
+        // grad-L: the forward solution of the rows, beside the record
+        if (a.gL)
+          load_rows<T, KT, ES, false, 1>(reinterpret_cast<T *>(smem + a.xfb), a.xf, lrow, nr, kc,
+                                         c0, a.rs, a.cs, a.n, 0);
+        vm_wait();
+        for (int t = lane; t < nr; t += 64) {
+          const int id = lrow[t];
+          if ((unsigned)id < (unsigned)a.n)
+            st_dev(&a.X[(int64_t)id * a.rs + (int64_t)c0 * a.cs], mark_value<T>());
+        }
+        vm_wait();
+        if (lane == 0) st_dev(myflag, base + 1);
+      }
+      // 3. rounds: issue the stores of every band all compute waves have
+      // finished and the halo loads of every band whose waits are met, wait
+      // once for all of them, then publish and signal
+      int loaded = 0, issued = 0, stored = 0;
+      bool halo_in = false;  // marked protocol: band `loaded`'s halo is in LDS
+      int graded = 0;        // marked protocol with grad-L: bands whose grad-L is written
+      uint64_t t0 = wall_ticks();
+      bool gave_up = false;
+      while (stored < nb) {
+        int done = nb;
+#pragma unroll
+        for (int w = 0; w < NC; ++w) done = min(done, lds_get<int>(&s_ctl[10 + w]));
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+        const bool stores = done > issued;
+        if (stores) {
+          const int r0 = min(max(lsl[min(lband[issued], ns)], 0), nr);
+          const int r1 = min(max(lsl[min(lband[done], ns)], 0), nr);
+          for (int t = r0 * KT + lane; t < r1 * KT; t += 64) {
+            const int c = t & (KT - 1);
+            const int id = lrow[t >> KS];
+            if (c < kc && (unsigned)id < (unsigned)a.n)
+              st_dev(&a.X[(int64_t)id * a.rs + (int64_t)(c0 + c) * a.cs], xs[t]);
+          }
+          issued = done;
+        }
+        if constexpr (MK) {
+          // stores are not waited for: a reader re-loads what is still marked
+          stored = issued;
+          bool moved = stores;
+          if (loaded < nb) {
+            if (!halo_in) {
+              // once every region read has marked its rows (a region stays
+              // marked for the rest of the launch), every halo value of the
+              // task is loaded at once; values still marked are loaded again
+              // band by band below
+              if (gave_up || all_marked(ldep, nd, a.tflags, base, a.groups, g, a.R)) {
+                if (nh > 0)
+                  load_rows<T, KT, ES, true, 1>(xs + (size_t)a.smax * KT, a.X, lhalo, nh, kc, c0,
+                                                a.rs, a.cs, a.n, 0);
+                vm_wait();
+                wave_lds_sync();
+                halo_in = true;
+                moved = true;
+              }
+            }
+            if (halo_in) {
+              const int hb0 = lhb[loaded], hb1 = lhb[loaded + 1];
+              bool marked = false;
+              for (int t = hb0 + lane; t < hb1; t += 64) {
+                T *slot = &xs[(size_t)(a.smax + t) * KT];
+                if (is_mark(*slot)) {
+                  const int id = lhalo[t];
+                  const T v = (unsigned)id < (unsigned)a.n
+                                  ? ld_dev(&a.X[(int64_t)id * a.rs + (int64_t)c0 * a.cs])
+                                  : T(0);
+                  *slot = v;
+                  marked |= gave_up ? false : is_mark(v);
+                }
+              }
+              wave_lds_sync();
+              if (!__any(marked)) {
+                if (loaded == 0 && lane == 0) trace_mark(a.trace, a.trace_cap, task, 1);
+                ++loaded;
+                if (lane == 0)
+                  __hip_atomic_store(&s_ctl[9], loaded, __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_WORKGROUP);
+                moved = true;
+              }
+            }
+          }
+          if (moved) {
+            t0 = wall_ticks();
+          } else if (a.gL && graded < stored) {
+            // grad-L only in rounds that would otherwise wait: a band a
+            // round, off the path of the halo values and the stores
+            grad_bands<T>(a.gL, xs, reinterpret_cast<const T *>(smem + a.xfb), rb, L, lsl, lband,
+                          graded, graded + 1, nr, ns, ne);
+            ++graded;
+          } else {
+            __builtin_amdgcn_s_sleep(1);
+            if (!gave_up && wall_ticks() - t0 > a.timeout) {
+              if (lane == 0) atomicOr(&a.state[ST_ERROR], 1);
+              gave_up = true;
+            }
+          }
+          continue;
+        }
+        int ready = loaded;
+        if (loaded < nb) {
+          const int b1 = min(nb, loaded + LOOK);
+          ready = gave_up ? b1 : bands_ready(ldep, lwp, loaded, b1, a.tflags, base, a.groups,
+                                              g, a.R, false);
+          if (ready > loaded) {
+            const int hb0 = lhb[loaded], hb1 = lhb[ready];
+            if (hb1 > hb0)
+              load_rows<T, KT, ES, true, 1>(xs + (size_t)(a.smax + hb0) * KT, a.X, lhalo + hb0,
+                                            hb1 - hb0, kc, c0, a.rs, a.cs, a.n, 0);
+          }
+        }
+        if (stores || ready > loaded) {
+          vm_wait();
+          if (issued > stored) {
+            stored = issued;
+            if (lane == 0) st_dev(myflag, base + stored);
+          }
+          if (ready > loaded) {
+            if (loaded == 0 && lane == 0) trace_mark(a.trace, a.trace_cap, task, 1);
+            loaded = ready;
+            asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+            if (lane == 0)
+              __hip_atomic_store(&s_ctl[9], loaded, __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_WORKGROUP);
+          }
+          t0 = wall_ticks();
+        } else {
+          __builtin_amdgcn_s_sleep(1);
+          if (!gave_up && wall_ticks() - t0 > a.timeout) {
+            // a wait that does not end: record it and stop waiting (the
+            // compute waves then finish the task with what is loaded)
+            if (lane == 0) atomicOr(&a.state[ST_ERROR], 1);
+            gave_up = true;
+          }
+        }
+      }
+      if constexpr (MK)
+        if (a.gL && graded < nb)
+          grad_bands<T>(a.gL, xs, reinterpret_cast<const T *>(smem + a.xfb), rb, L, lsl, lband,
+                        graded, nb, nr, ns, ne);
+      // the next task and its header
+      const int nx = __builtin_amdgcn_readlane(ticket, 0);
+      if (lane < 8) s_ctl[1 + lane] = nx < ntasks ? a.head[8 * (nx / a.groups) + lane] : 0;
+      if (lane == 0) s_ctl[0] = nx;
+      if (lane == 0) trace_mark(a.trace, a.trace_cap, task, 2);
+    } else {
+      // compute wave `wid`: columns wid*CPL .. (column split) of every row
+      for (int b = 0; b < nb; ++b) {
+        if (lds_get<int>(&s_ctl[9]) < b + 1) {
+          const uint64_t t0 = wall_ticks();
+          int rounds = 0;
+          while (lds_get<int>(&s_ctl[9]) < b + 1) {
+            __builtin_amdgcn_s_sleep(1);
+            if ((++rounds & 255) == 0 && wall_ticks() - t0 > 4 * a.timeout) {
+              if (lane == 0) atomicOr(&a.state[ST_ERROR], 4);
+              break;
+            }
+          }
+        }
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+        const int sb = __builtin_amdgcn_readfirstlane(lband[b]);
+        const int se = __builtin_amdgcn_readfirstlane(lband[b + 1]);
+        if (ns <= 62)
+          solve_slices<T, KT, CPL, P, true, NC>(xs, rb, L, nr, ns, sb, se, ne, kc);
+        else
+          solve_slices<T, KT, CPL, P, false, NC>(xs, rb, L, nr, ns, sb, se, ne, kc);
+        wave_lds_sync();
+        if (lane == 0)
+          __hip_atomic_store(&s_ctl[10 + wid], b + 1, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_WORKGROUP);
+      }
+    }
+    __syncthreads();
+    task = s_ctl[0];
+  }
+  persistent_finish(a.state, a.total_waves);
+}
+
 template <typename T>
 __global__ void k_region_gather(int64_t m, const int *__restrict__ src,
                                 const int *__restrict__ dst, const T *__restrict__ from,
@@ -599,22 +953,38 @@ int load_bytes(const void *d_B, int k, int64_t rs, int64_t cs) {
 // depth of the region graph: more blocks would only wait, and their polls
 // slow the rest (the single-column 128^3 IC(0) solve: 0.415 ms with 256
 // blocks, 0.432 ms with the 1024 that fit).
-template <typename T, int KT, int ES, int NW>
+// With the memory-wave kernel (MW) a block has NW compute waves and one
+// memory wave; its tasks overlap across depths of the region graph (a
+// region starts once the first bands it reads are published), so the grid
+// may hold up to four times the widest depth's tasks.
+template <typename T, int KT, int ES, int NW, bool MW>
 int region_launch(symamd_handle h, symamd_plan p, const RegionPlan &rp, const void *d_B,
-                  void *d_X, int k, int64_t rs, int64_t cs) {
-  const void *kfn = (const void *)&k_regions<T, KT, ES, NW>;
-  const size_t lds = region_lds_bytes(rp, KT, (int)sizeof(T));
+                  void *d_X, int k, int64_t rs, int64_t cs, const void *d_xf = nullptr,
+                  void *d_gL = nullptr) {
+  const void *kfn;
+  if constexpr (MW)
+    kfn = (const void *)&k_regions_mw<T, KT, ES, NW, KT == 1>;
+  else
+    kfn = (const void *)&k_regions<T, KT, ES, NW>;
+  constexpr int WAVES = NW + (MW ? 1 : 0);
+  // grad-L from the solve: marked single right-hand side kernel only
+  const bool grad = MW && KT == 1 && d_gL && rp.gp;
+  const size_t lds_base = region_image_bytes(rp, KT, (int)sizeof(T));
+  const size_t xfb = (lds_base + 15) & ~(size_t)15;
+  const size_t lds = grad ? xfb + (size_t)rp.smax * sizeof(T) : lds_base;
   if (lds > 65536)
     SYMAMD_HIP(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, 64 * NW, lds) != hipSuccess ||
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, 64 * WAVES, lds) !=
+          hipSuccess ||
       per_cu < 1)
     per_cu = 1;
   const int groups = (k + KT - 1) / KT;
   const int64_t ntasks = (int64_t)rp.R * groups;
   int64_t grid = (int64_t)per_cu * h->num_cus;
   const int64_t wide = (int64_t)rp.width * groups;
-  if (wide > 0 && grid > wide + wide / 3) grid = wide + wide / 3;
+  const int64_t wcap = MW ? 4 * wide : wide + wide / 3;
+  if (wide > 0 && grid > wcap) grid = wcap;
   if (h->max_waves > 0 && h->max_waves < grid) grid = h->max_waves;
   if (grid > ntasks) grid = ntasks;
   if (grid < 1) grid = 1;
@@ -634,37 +1004,64 @@ int region_launch(symamd_handle h, symamd_plan p, const RegionPlan &rp, const vo
   a.smax = rp.smax;
   a.hmax = rp.hmax;
   a.xsb = xs_bytes(rp, KT, (int)sizeof(T));
-  a.total_waves = (int)grid * NW;
+  a.gp = rp.gp;
+  a.xfb = (int)xfb;
+  a.xf = grad ? (const T *)d_xf : nullptr;
+  a.gL = grad ? (T *)d_gL : nullptr;
+  a.total_waves = (int)grid * WAVES;
   a.timeout = h->timeout_ticks();
   a.trace = h->trace;
   a.trace_cap = h->trace_cap;
   void *args[] = {&a};
-  SYMAMD_HIP(hipLaunchKernel(kfn, dim3((unsigned)grid), dim3(64 * NW), args, lds, h->stream));
+  SYMAMD_HIP(hipLaunchKernel(kfn, dim3((unsigned)grid), dim3(64 * WAVES), args, lds,
+                            h->stream));
   return SYMAMD_OK;
 }
 
 template <typename T, int KT, int NW>
 int region_launch_es(symamd_handle h, symamd_plan p, const RegionPlan &rp, const void *d_B,
-                     void *d_X, int k, int64_t rs, int64_t cs) {
+                     void *d_X, int k, int64_t rs, int64_t cs, const void *d_xf, void *d_gL,
+                     bool *fused) {
   int es = load_bytes<T, KT>(d_B, k, rs, cs);
   if (d_X != d_B) es = es < load_bytes<T, KT>(d_X, k, rs, cs) ? es : load_bytes<T, KT>(d_X, k, rs, cs);
-  if (es == 0) return region_launch<T, KT, 0, NW>(h, p, rp, d_B, d_X, k, rs, cs);
+  // the memory-wave kernel: direct loads only, one compute wave or the
+  // column split of eight-column tasks
+  // (measured on the 128^3 IC(0) factor: single right-hand sides gain from
+  // the marked protocol; eight-column tasks were slower with a memory wave,
+  // 1.1 vs 0.85 ms at k = 32)
+  constexpr bool MWOK = KT == 1;
+  if (es == 0) return region_launch<T, KT, 0, NW, false>(h, p, rp, d_B, d_X, k, rs, cs);
   if constexpr ((KT * sizeof(T)) % 16 == 0)
-    if (es == 16) return region_launch<T, KT, 16, NW>(h, p, rp, d_B, d_X, k, rs, cs);
-  return region_launch<T, KT, 4, NW>(h, p, rp, d_B, d_X, k, rs, cs);
+    if (es == 16) {
+      if constexpr (MWOK)
+        if (h->region_mw) {
+          *fused = d_gL && rp.gp;
+          return region_launch<T, KT, 16, NW, true>(h, p, rp, d_B, d_X, k, rs, cs, d_xf, d_gL);
+        }
+      return region_launch<T, KT, 16, NW, false>(h, p, rp, d_B, d_X, k, rs, cs);
+    }
+  if constexpr (MWOK)
+    if (h->region_mw) {
+      *fused = d_gL && rp.gp;
+      return region_launch<T, KT, 4, NW, true>(h, p, rp, d_B, d_X, k, rs, cs, d_xf, d_gL);
+    }
+  return region_launch<T, KT, 4, NW, false>(h, p, rp, d_B, d_X, k, rs, cs);
 }
 
 template <typename T>
 int region_dispatch(symamd_handle h, symamd_plan p, const RegionPlan &rp, const void *d_B,
-                    void *d_X, int k, int64_t rs, int64_t cs, int kt, int nw) {
+                    void *d_X, int k, int64_t rs, int64_t cs, int kt, int nw,
+                    const void *d_xf, void *d_gL, bool *fused) {
   switch (kt) {
-    case 1: return region_launch_es<T, 1, 1>(h, p, rp, d_B, d_X, k, rs, cs);
-    case 2: return region_launch_es<T, 2, 1>(h, p, rp, d_B, d_X, k, rs, cs);
-    case 4: return region_launch_es<T, 4, 1>(h, p, rp, d_B, d_X, k, rs, cs);
+    case 1: return region_launch_es<T, 1, 1>(h, p, rp, d_B, d_X, k, rs, cs, d_xf, d_gL, fused);
+    case 2: return region_launch_es<T, 2, 1>(h, p, rp, d_B, d_X, k, rs, cs, d_xf, d_gL, fused);
+    case 4: return region_launch_es<T, 4, 1>(h, p, rp, d_B, d_X, k, rs, cs, d_xf, d_gL, fused);
     default:
-      if (nw == 4) return region_launch_es<T, 8, 4>(h, p, rp, d_B, d_X, k, rs, cs);
-      if (nw == 2) return region_launch_es<T, 8, 2>(h, p, rp, d_B, d_X, k, rs, cs);
-      return region_launch_es<T, 8, 1>(h, p, rp, d_B, d_X, k, rs, cs);
+      if (nw == 4)
+        return region_launch_es<T, 8, 4>(h, p, rp, d_B, d_X, k, rs, cs, d_xf, d_gL, fused);
+      if (nw == 2)
+        return region_launch_es<T, 8, 2>(h, p, rp, d_B, d_X, k, rs, cs, d_xf, d_gL, fused);
+      return region_launch_es<T, 8, 1>(h, p, rp, d_B, d_X, k, rs, cs, d_xf, d_gL, fused);
   }
 }
 
@@ -688,11 +1085,30 @@ int region_cap(symamd_handle h, int k) {
   return task_cols(k) == KG_MAX && region_waves(h, k) == 1 ? 256 : 512;
 }
 
-// Slices per published band (the handle's choice, else 255: one band for
-// regions of up to 255 levels).
-int region_band(symamd_handle h) {
-  const int b = h->region_band > 0 ? h->region_band : 255;
+// Slices per band: the handle's choice, else 4 where the memory-wave kernel
+// runs (single right-hand sides: halo values loaded band by band as the
+// task reaches them) and 255 (one band for regions of up to 255 levels)
+// elsewhere.
+int region_band(symamd_handle h, int k) {
+  const int b = h->region_band > 0 ? h->region_band : h->region_mw && k == 1 ? 4 : 255;
   return b < 255 ? b : 255;
+}
+
+// The schedule of the forward (or transposed) solve for k right-hand sides
+// whose task fits a workgroup's LDS: at the default cap, else at half the
+// cap, and so on down to REGION_CAP_MIN rows (a matrix whose rows read many
+// distinct rows outside their region needs smaller regions, whose halos are
+// smaller).  A cap the handle fixes is tried alone.  Null when none fits.
+static RegionPlan *fitted_plan(symamd_handle h, symamd_plan p, int k, bool transposed) {
+  if (p->n == 0 || k < 1 || k > KG_MAX * RegionPlan::MAX_GROUPS) return nullptr;
+  const int vs = p->dtype == SYMAMD_F64 ? 8 : 4;
+  const int kt = task_cols(k);
+  for (int cap = region_cap(h, k);; cap /= 2) {
+    RegionPlan *rp = nullptr;
+    if (plan_regions(h, p, cap, k, &rp, transposed) != SYMAMD_OK) return nullptr;
+    if (rp->R > 0 && region_lds_bytes(*rp, kt, vs) <= (size_t)h->lds_per_block) return rp;
+    if (h->region_cap > 0 || cap / 2 < REGION_CAP_MIN) return nullptr;
+  }
 }
 
 bool region_preferred(symamd_handle h, symamd_plan p, int k, int layout) {
@@ -700,10 +1116,8 @@ bool region_preferred(symamd_handle h, symamd_plan p, int k, int layout) {
   // are solved through the row-major workspace unless the handle asks for
   // in-place solves
   if (k > 1 && layout == SYMAMD_COL_MAJOR && h->colmajor_direct) return false;
-  if (!region_usable(h, p, k)) return false;
-  RegionPlan *rp = nullptr;
-  if (plan_regions(h, p, region_cap(h, k), &rp) != SYMAMD_OK) return false;
-  return rp->inside >= 0.6;
+  RegionPlan *rp = fitted_plan(h, p, k, false);
+  return rp && rp->inside >= 0.6;
 }
 
 int gather_region_values(symamd_handle h, symamd_plan p) {
@@ -720,24 +1134,20 @@ int gather_region_values(symamd_handle h, symamd_plan p) {
 }
 
 bool region_usable(symamd_handle h, symamd_plan p, int k) {
-  if (k < 1 || k > KG_MAX * RegionPlan::MAX_GROUPS) return false;
-  RegionPlan *rp = nullptr;
-  if (plan_regions(h, p, region_cap(h, k), &rp) != SYMAMD_OK || rp->R == 0) return false;
-  const int vs = p->dtype == SYMAMD_F64 ? 8 : 4;
-  return region_lds_bytes(*rp, task_cols(k), vs) <= (size_t)h->lds_per_block;
+  return fitted_plan(h, p, k, false) != nullptr;
+}
+
+void region_schedule_info(symamd_handle h, symamd_plan p, int k, bool transposed, int *cap,
+                          int *tasks) {
+  RegionPlan *rp = fitted_plan(h, p, k, transposed);
+  *cap = rp ? rp->cap : 0;
+  *tasks = rp ? rp->R * ((k + KG_MAX - 1) / KG_MAX) : 0;
 }
 
 namespace {
 
-// The schedule of the forward (or transposed) solve for k right-hand sides,
-// when its LDS image fits a workgroup.
 RegionPlan *usable_plan(symamd_handle h, symamd_plan p, int k, bool transposed) {
-  if (p->n == 0 || k < 1 || k > KG_MAX * RegionPlan::MAX_GROUPS) return nullptr;
-  RegionPlan *rp = nullptr;
-  if (plan_regions(h, p, region_cap(h, k), &rp, transposed) != SYMAMD_OK || rp->R == 0)
-    return nullptr;
-  const int vs = p->dtype == SYMAMD_F64 ? 8 : 4;
-  return region_lds_bytes(*rp, task_cols(k), vs) <= (size_t)h->lds_per_block ? rp : nullptr;
+  return fitted_plan(h, p, k, transposed);
 }
 
 // gL[e] = -(GB[i,0]*X[j,0] + ... + GB[i,k-1]*X[j,k-1]) for every entry
@@ -845,13 +1255,20 @@ int grad_dispatch(symamd_handle h, symamd_plan p, const T *X, const T *GB, T *gL
 }  // namespace
 
 int launch_regions_plan(symamd_handle h, symamd_plan p, const void *d_B, void *d_X, int k,
-                        int64_t rs, int64_t cs, bool transposed) {
+                        int64_t rs, int64_t cs, bool transposed, const void *d_xf,
+                        void *d_gL, bool *fused) {
+  bool f = false;
+  if (fused) *fused = false;
   RegionPlan *rp = usable_plan(h, p, k, transposed);
   if (!rp) return fail(SYMAMD_ERR_NOT_SUPPORTED, "no region schedule for this matrix and k");
   const int kt = task_cols(k), nw = region_waves(h, k);
+  int rc;
   if (p->dtype == SYMAMD_F64)
-    return region_dispatch<double>(h, p, *rp, d_B, d_X, k, rs, cs, kt, nw);
-  return region_dispatch<float>(h, p, *rp, d_B, d_X, k, rs, cs, kt, nw);
+    rc = region_dispatch<double>(h, p, *rp, d_B, d_X, k, rs, cs, kt, nw, d_xf, d_gL, &f);
+  else
+    rc = region_dispatch<float>(h, p, *rp, d_B, d_X, k, rs, cs, kt, nw, d_xf, d_gL, &f);
+  if (fused) *fused = f;
+  return rc;
 }
 
 int launch_regions(symamd_handle h, symamd_plan p, void *d_B, int k, int64_t rs,

@@ -681,7 +681,9 @@ bool build_region_schedule(int n, int64_t nnz, int levels, const int *lvl_ptr,
   return true;
 }
 
-bool pack_region_records(const HostRegions &o, int vs, HostRecords &out) {
+bool pack_region_records(const HostRegions &o, int vs, HostRecords &out,
+                         const std::vector<int> *gmap) {
+  const int gp = gmap ? 1 : 0;
   constexpr int PK = RegionPlan::PACK;
   const int R = o.R;
   out.rhead.assign(8 * (size_t)R, 0);
@@ -694,7 +696,7 @@ bool pack_region_records(const HostRegions &o, int vs, HostRecords &out) {
     const int ns = o.sptr[r + 1] - o.sptr[r] - 1;
     const int nd = o.dptr[r + 1] - o.dptr[r];
     const int nb = o.bptr[r + 1] - o.bptr[r] - 1;
-    const RecLayout L = rec_layout(nr, nh, ne, ns, nd, nb, vs);
+    const RecLayout L = rec_layout(nr, nh, ne, ns, nd, nb, vs, gp);
     int *hd = &out.rhead[8 * (size_t)r];
     hd[0] = nr;
     hd[1] = nh;
@@ -719,7 +721,7 @@ bool pack_region_records(const HostRegions &o, int vs, HostRecords &out) {
     const int s0 = o.sptr[r], ns = o.sptr[r + 1] - s0 - 1;
     const int d0 = o.dptr[r], nd = o.dptr[r + 1] - d0;
     const int b0 = o.bptr[r], nb = o.bptr[r + 1] - b0 - 1;
-    const RecLayout L = rec_layout(nr, nh, ne, ns, nd, nb, vs);
+    const RecLayout L = rec_layout(nr, nh, ne, ns, nd, nb, vs, gp);
     uint8_t *rec = out.bytes.data() + (size_t)out.rhead[8 * (size_t)r + 5] * 16;
     const int64_t vbase = (int64_t)out.rhead[8 * (size_t)r + 5] * 16 / vs;
     auto put16 = [](uint8_t *p, int v) {
@@ -754,6 +756,17 @@ bool pack_region_records(const HostRegions &o, int vs, HostRecords &out) {
       put32(rec + L.bands + 4 * k, o.bands[b0 + k]);
       put32(rec + L.hband + 4 * k, o.hband[b0 + k]);
       put32(rec + L.wptr + 4 * k, o.wptr[b0 + k]);
+    }
+    if (gp) {
+      const std::vector<int> &gm = *gmap;
+      for (int l = 0; l < nr; ++l) {
+        const int t = a + l;
+        for (int u = 0; u < PK; ++u)
+          put32(rec + L.gpos + 4 * ((PK + 1) * l + u),
+                u < o.pcnt[t] ? gm[o.psrc[(size_t)PK * t + u]] : 0);
+        put32(rec + L.gpos + 4 * ((PK + 1) * l + PK), gm[o.dsrc[t]]);
+      }
+      for (int e = 0; e < ne; ++e) put32(rec + L.gpos + 4 * ((PK + 1) * nr + e), gm[o.esrc[e0 + e]]);
     }
   }
   return true;
@@ -798,10 +811,10 @@ void reversed_transpose(int n, int levels, const std::vector<int> &lvl_ptr,
 // the forward solve or (transposed) of L^T x = g.  A transposed schedule
 // names original rows in its records and gathers its values from the
 // plan's transposed copy.
-int plan_regions(symamd_handle h, symamd_plan p, int cap, RegionPlan **out,
+int plan_regions(symamd_handle h, symamd_plan p, int cap, int k, RegionPlan **out,
                  bool transposed) {
   auto &plans = transposed ? p->rplans_t : p->rplans;
-  const int band = region_band(h);
+  const int band = region_band(h, k);
   const int64_t key = (int64_t)cap * 256 + band;
   auto it = plans.find(key);
   if (it != plans.end()) {
@@ -847,14 +860,25 @@ int plan_regions(symamd_handle h, symamd_plan p, int cap, RegionPlan **out,
     return SYMAMD_OK;
   }
   const int vs = p->dtype == SYMAMD_F64 ? 8 : 4;
+  // single right-hand side adjoints write grad-L from the solve: their
+  // records carry every entry's position in the caller's CSR
+  std::vector<int> gmap;
+  if (transposed && k == 1 && p->nnz > 0) {
+    std::vector<int> tmap(p->nnz);
+    SYMAMD_HIP(hipMemcpy(tmap.data(), p->d_tmap, sizeof(int) * p->nnz, hipMemcpyDeviceToHost));
+    gmap.resize(vmap.size());
+    for (size_t v = 0; v < vmap.size(); ++v) gmap[v] = tmap[vmap[v]];
+  }
   HostRecords hr;
-  if (o.smax + o.hmax >= REGION_SLOTS_MAX || !pack_region_records(o, vs, hr)) {
+  if (o.smax + o.hmax >= REGION_SLOTS_MAX ||
+      !pack_region_records(o, vs, hr, gmap.empty() ? nullptr : &gmap)) {
     *out = &plans.emplace(key, rp).first->second;  // R = 0: not usable
     return SYMAMD_OK;
   }
   if (transposed)
     for (int &v : hr.vsrc) v = vmap[v];
   rp.R = o.R;
+  rp.gp = gmap.empty() ? 0 : 1;
   rp.smax = o.smax;
   rp.hmax = o.hmax;
   rp.emax = o.emax;

@@ -52,6 +52,10 @@ struct SnArgs {
   const int *dptr, *dlist;      // tasks that task t waits for
   int *flags, *state;           // one tag per (column, task)
   int ntasks, total_waves;
+  // transposed solve: rows below each supernode (at its temporaries'
+  // offsets tb), and the supernodes owning them that it waits for
+  const int *rowb, *tdptr, *tdlist;
+  int m;
   int k;                        // right-hand sides: column c at x + c*ldx
   int64_t ldx, tstride;         // its temporaries at T + c*tstride
   uint64_t timeout;
@@ -250,6 +254,110 @@ __global__ __launch_bounds__(256) void k_supernodal(SnArgs a) {
   persistent_finish(a.state, a.total_waves);
 }
 
+
+// ---------------------------------------------------- transposed solve
+// L^T x = b on the same storage (the semantics of parsy's blockedLTsolve,
+// sparse_blas/parsy/triangularSolve/Triangular_BCSC.cpp:125-186), supernodes
+// from the last to the first; one wave per (supernode, column) ticket,
+// supernode-major, so every wait is for an earlier ticket.  Per supernode:
+// wait for the supernodes owning its rows below (solved earlier); per top
+// column c (lanes), t = sum over the rows below, ascending, of M[r, c] times
+// x at that row (the rows' x staged in LDS chunk by chunk), x[s+c] -= t;
+// then the top's transposed solve in LDS from the last column: x_c /= M[c, c]
+// and every x_r, r < c, loses M[c, r] * x_c (the next column's entries are
+// loaded while this one is finished).  The order is the oracle's
+// (orc_sptrsv_csc_block_t).
+constexpr int SN_TCHUNK = 256;   // rows below staged per pass
+constexpr int SN_TCOLS = SN_WMAX / 64;  // top columns per lane
+
+__global__ __launch_bounds__(256) void k_supernodal_t(SnArgs a) {
+  __shared__ double slab[4][SN_WMAX];
+  __shared__ double below[4][SN_TCHUNK];
+  double *xs = slab[threadIdx.x >> 6];
+  double *xb = below[threadIdx.x >> 6];
+  const int lane = threadIdx.x & 63;
+  const int target = ld_dev(&a.state[ST_EPOCH]) + 1;
+  for (;;) {
+    int tt = 0;
+    if (lane == 0) tt = atomicAdd(&a.state[ST_TICKET], 1);
+    tt = __shfl(tt, 0);
+    if (tt >= a.m * a.k) break;
+    const int ti = tt / a.k, colk = tt - ti * a.k;
+    const int i = a.m - 1 - ti;
+    double *const x = a.x + colk * a.ldx;
+    int *const flags = a.flags + (int64_t)colk * a.ntasks;
+    wait_units(a.tdptr, a.tdlist, i, flags, target, a.state, a.timeout);
+    const int s = a.sup[i], w = a.sup[i + 1] - s, nr = a.nrows[i];
+    const double *M = a.Lx + a.Lp[i];
+    const int *rows = a.rowb + a.tb[i];
+    // 1. the rows below: one running sum per top column, rows ascending
+    double t[SN_TCOLS];
+#pragma unroll
+    for (int j = 0; j < SN_TCOLS; ++j) t[j] = 0.0;
+    for (int r0 = w; r0 < nr; r0 += SN_TCHUNK) {
+      const int cnt = min(SN_TCHUNK, nr - r0);
+      for (int r = lane; r < cnt; r += 64) xb[r] = ld_dev(&x[rows[r0 - w + r]]);
+      wave_lds_sync();
+      for (int r = 0; r < cnt; ++r) {
+        const double v = xb[r];
+        const int row = r0 + r;
+#pragma unroll
+        for (int j = 0; j < SN_TCOLS; ++j) {
+          const int c = lane + 64 * j;
+          if (c < w) {
+            const double p = M[row + (int64_t)c * nr] * v;
+            t[j] = row == w ? p : t[j] + p;
+          }
+        }
+      }
+      wave_lds_sync();
+    }
+#pragma unroll
+    for (int j = 0; j < SN_TCOLS; ++j) {
+      const int c = lane + 64 * j;
+      if (c < w) xs[c] = nr > w ? x[s + c] - t[j] : x[s + c];
+    }
+    wave_lds_sync();
+    // 2. the top, transposed, from the last column
+    double mc[SN_TCOLS];
+    const int c1 = w - 1;
+#pragma unroll
+    for (int j = 0; j < SN_TCOLS; ++j) {
+      const int r = lane + 64 * j;
+      mc[j] = r < c1 ? M[c1 + (int64_t)r * nr] : 0.0;
+    }
+    double dg = M[c1 + (int64_t)c1 * nr];
+    for (int c = w - 1; c >= 0; --c) {
+      // the next column's entries (row c-1 of the top) and diagonal, in
+      // flight while this column is finished
+      double mn[SN_TCOLS];
+      const int cn = c - 1;
+#pragma unroll
+      for (int j = 0; j < SN_TCOLS; ++j) {
+        const int r = lane + 64 * j;
+        mn[j] = cn >= 0 && r < cn ? M[cn + (int64_t)r * nr] : 0.0;
+      }
+      const double dn = cn >= 0 ? M[cn + (int64_t)cn * nr] : 1.0;
+      const double y = xs[c] / dg;
+#pragma unroll
+      for (int j = 0; j < SN_TCOLS; ++j) {
+        const int r = lane + 64 * j;
+        if (r < c) xs[r] = xs[r] - mc[j] * y;
+      }
+      if (lane == (c & 63)) xs[c] = y;
+      wave_lds_sync();
+#pragma unroll
+      for (int j = 0; j < SN_TCOLS; ++j) mc[j] = mn[j];
+      dg = dn;
+    }
+    for (int r = lane; r < w; r += 64) st_dev(&x[s + r], xs[r]);
+    publish_fence();
+    if (lane == 0) st_dev(&flags[i], target);
+    wave_lds_sync();
+  }
+  persistent_finish(a.state, a.total_waves);
+}
+
 }  // namespace
 
 int sn_analyze(symamd_handle h, int n, int m, const int *h_sup, const int *h_nrows,
@@ -311,6 +419,22 @@ int sn_analyze(symamd_handle h, int n, int m, const int *h_sup, const int *h_nro
     }
     for (int t = first[i] + 1; t < first[i + 1]; ++t) deps[t].push_back(first[i]);
   }
+  // transposed solve: the rows below of each supernode (in temporaries
+  // order) and the supernodes owning them, which it waits for
+  std::vector<int> rowb(tb[m] > 0 ? tb[m] : 1), tdptr(m + 1, 0), tdlist;
+  for (int i = 0; i < m; ++i) {
+    const int w = h_sup[i + 1] - h_sup[i];
+    std::vector<int> own;
+    for (int k = 0; k < h_nrows[i] - w; ++k) {
+      const int row = h_Li[h_Lp[i] + w + k];
+      rowb[tb[i] + k] = row;
+      own.push_back(owner[row]);
+    }
+    std::sort(own.begin(), own.end());
+    own.erase(std::unique(own.begin(), own.end()), own.end());
+    tdlist.insert(tdlist.end(), own.begin(), own.end());
+    tdptr[i + 1] = (int)tdlist.size();
+  }
   std::vector<int> dptr(ntasks + 1, 0), dlist;
   for (int j = 0; j < ntasks; ++j) {
     std::sort(deps[j].begin(), deps[j].end());
@@ -339,6 +463,9 @@ int sn_analyze(symamd_handle h, int n, int m, const int *h_sup, const int *h_nro
   up(&q->d_dptr, dptr.data(), ntasks + 1);
   up(&q->d_dlist, dlist.data(), dlist.size());
   up(&q->d_task, reinterpret_cast<const int *>(task.data()), 2 * (size_t)ntasks);
+  up(&q->d_rowb, rowb.data(), tb[m]);
+  up(&q->d_tdptr, tdptr.data(), m + 1);
+  up(&q->d_tdlist, tdlist.data(), tdlist.size());
   if (rc == SYMAMD_OK &&
       (hipMalloc(&q->d_T, sizeof(double) * (q->tcount > 0 ? q->tcount : 1)) != hipSuccess ||
        hipMalloc(&q->d_flags, sizeof(int) * (ntasks > 0 ? ntasks : 1)) != hipSuccess ||
@@ -357,7 +484,7 @@ int sn_analyze(symamd_handle h, int n, int m, const int *h_sup, const int *h_nro
 }
 
 int sn_solve(symamd_handle h, symamd_snplan_s *q, const double *d_Lx, double *d_x, int k,
-             int64_t ldx) {
+             int64_t ldx, bool transposed) {
   if (q->m == 0 || k == 0) return SYMAMD_OK;
   // tickets run over (task, column) pairs in 32-bit counters
   if ((int64_t)q->ntasks * k >= (int64_t)INT_MAX)
@@ -397,6 +524,10 @@ int sn_solve(symamd_handle h, symamd_snplan_s *q, const double *d_Lx, double *d_
   a.dptr = q->d_dptr;
   a.dlist = q->d_dlist;
   a.task = reinterpret_cast<const int2 *>(q->d_task);
+  a.rowb = q->d_rowb;
+  a.tdptr = q->d_tdptr;
+  a.tdlist = q->d_tdlist;
+  a.m = q->m;
   a.flags = q->d_flags;
   a.state = q->d_state;
   a.ntasks = q->ntasks;
@@ -404,18 +535,21 @@ int sn_solve(symamd_handle h, symamd_snplan_s *q, const double *d_Lx, double *d_
   a.ldx = ldx;
   a.tstride = q->tcount > 0 ? q->tcount : 1;
   a.timeout = h->timeout_ticks();
+  const void *kfn = transposed ? (const void *)&k_supernodal_t : (const void *)&k_supernodal;
   int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)&k_supernodal, 256,
-                                                   0) != hipSuccess ||
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, 256, 0) != hipSuccess ||
       per_cu < 1)
     per_cu = 1;
   int64_t grid = (int64_t)per_cu * h->num_cus;
-  const int64_t want = ((int64_t)q->ntasks * k + 3) / 4;
+  const int64_t want = ((int64_t)(transposed ? q->m : q->ntasks) * k + 3) / 4;
   if (want < grid) grid = want;
   if (grid < 1) grid = 1;
   a.total_waves = (int)grid * 4;
   // tickets are dealt in task order, so any grid size is safe
-  k_supernodal<<<(unsigned)grid, 256, 0, h->stream>>>(a);
+  if (transposed)
+    k_supernodal_t<<<(unsigned)grid, 256, 0, h->stream>>>(a);
+  else
+    k_supernodal<<<(unsigned)grid, 256, 0, h->stream>>>(a);
   SYMAMD_HIP(hipGetLastError());
   return SYMAMD_OK;
 }
@@ -430,8 +564,9 @@ int sn_check(symamd_handle h, symamd_snplan_s *q) {
 
 void sn_destroy(symamd_snplan_s *q) {
   if (!q) return;
-  void *ptrs[] = {q->d_sup,  q->d_nrows, q->d_Lp, q->d_tb,    q->d_cptr,  q->d_cidx,
-                  q->d_dptr, q->d_dlist, q->d_T,  q->d_flags, q->d_state, q->d_task};
+  void *ptrs[] = {q->d_sup,   q->d_nrows, q->d_Lp,    q->d_tb,     q->d_cptr,
+                  q->d_cidx,  q->d_dptr,  q->d_dlist, q->d_T,      q->d_flags,
+                  q->d_state, q->d_task,  q->d_rowb,  q->d_tdptr, q->d_tdlist};
   for (void *x : ptrs)
     if (x) (void)hipFree(x);
   delete q;

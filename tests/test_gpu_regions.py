@@ -196,6 +196,7 @@ def test_region_adjoint_bitwise(dev, mat):
         Lp, Li, Lx = W.random_lower(20000, 0.0008, 1, seed=46)
     n = len(Lp) - 1
     plan = sa.Plan(*to_dev(Lp, Li, Lx, device=dev), handle=_rh())
+    assert plan.region_schedule(1, transposed=True)[1] > 0  # runs on regions
     x, gx = W.uniform(47, 0, n), W.uniform(48, 0, n)
     dx, dgx = to_dev(x, gx, device=dev)
     gL = torch.empty(len(Lx), dtype=torch.float64, device=dev)
@@ -209,6 +210,30 @@ def test_region_adjoint_bitwise(dev, mat):
         assert bitwise_equal(gb.cpu().numpy(), gb_ref)
         assert bitwise_equal(gL.cpu().numpy(), gL_ref)
         assert bitwise_equal(dgx.cpu().numpy(), gx)  # gx is not written
+
+
+@pytest.mark.parametrize("k", [1, 8])
+def test_region_cap_halved_to_fit(dev, k):
+    # rows that read many rows outside their region: at the default cap a
+    # task's halo and record overflow the workgroup's LDS, so the schedule
+    # is built at a smaller cap, and the solves stay on regions and bitwise
+    Lp, Li, Lx = W.random_lower(20000, 0.0008, 1, seed=46)
+    n = len(Lp) - 1
+    plan = sa.Plan(*to_dev(Lp, Li, Lx, device=dev), handle=_rh())
+    for transposed in (False, True):
+        cap, tasks = plan.region_schedule(k, transposed=transposed)
+        assert 32 <= cap < 512 and tasks > 0, (transposed, cap, tasks)
+    B = W.uniform(49, 0, n * k).reshape(k, n)
+    X_ref = oracle.sptrsm_csr(Lp, Li, Lx, B)
+    for _ in range(2):  # second solve: next epoch of the region tags
+        X = _solve_regions(plan, B[0] if k == 1 else np.ascontiguousarray(B.T), dev)
+        assert bitwise_equal(X if k == 1 else X.T, X_ref[0] if k == 1 else X_ref)
+    if k == 1:
+        x0 = W.uniform(50, 0, n)
+        gb_ref, gL_ref = oracle.adjoint(Lp, Li, Lx, x0, B[0])
+        gb, gL = plan.adjoint(*to_dev(x0, B[0], device=dev), executor="regions")
+        plan.check()
+        assert bitwise_equal(gb.cpu().numpy(), gb_ref)
 
 
 @pytest.mark.parametrize("k,layout,executor", [(1, "row", "regions"), (8, "row", "regions"),

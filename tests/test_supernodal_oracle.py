@@ -93,3 +93,47 @@ def to_parsy(sup, nrows, Lp, Li, n):
     vp[n] = int(Lp[m - 1]) + (int(sup[m]) - int(sup[m - 1])) * int(nrows[m - 1]) if m else 0
     rp[n] = total_rows
     return vp, np.asarray(rows, np.int32), rp
+
+
+# ------------------------------------------------- transposed (blockedLTsolve)
+@pytest.mark.parametrize("case", CASES)
+def test_transposed_block_solve_against_dense(case):
+    """L^T x = b on the supernodal storage (parsy's blockedLTsolve,
+    Triangular_BCSC.cpp:125-186; its BLAS is absent here, so the restatement
+    is pinned by the mathematics): the dense transposed solve of the
+    assembled L."""
+    n = case[0]
+    sup, nrows, Lp, Li, Lx = W.bcsc(*case)
+    b = W.uniform(case[-1], 1, n)
+    x = oracle.sptrsv_csc_block_t(sup, nrows, Lp, Li, Lx, b)
+    L = dense(sup, nrows, Lp, Li, Lx, n)
+    assert np.max(np.abs(L.T @ x - b)) <= 1e-12 * np.max(np.abs(b))
+
+
+def test_transposed_block_solve_exact_integers():
+    """Small integer entries, unit diagonal and an integer solution: every
+    operation is exact, so x must equal x_true exactly -- a wrong entry,
+    transposition or order of rows below shows as an integer error."""
+    n = 600
+    sup, nrows, Lp, Li, Lx = W.bcsc(n, 12, 16, 200, 9)
+    rng = np.random.default_rng(1)
+    Lx = rng.integers(-2, 3, len(Lx)).astype(np.float64)
+    for i in range(len(sup) - 1):  # unit diagonal of every top
+        w, nr = sup[i + 1] - sup[i], nrows[i]
+        for j in range(w):
+            Lx[Lp[i] + j * nr + j] = 1.0
+    L = dense(sup, nrows, Lp, Li, Lx, n)
+    x_true = rng.integers(-3, 4, n).astype(np.float64)
+    b = L.T @ x_true  # exact: small integers
+    x = oracle.sptrsv_csc_block_t(sup, nrows, Lp, Li, Lx, b)
+    assert np.array_equal(x, x_true)
+
+
+def test_transposed_is_the_adjoint_of_the_forward_solve():
+    """<L^-T b, c> = <b, L^-1 c> (to rounding): the transposed restatement is
+    the adjoint of the pinned forward one."""
+    sup, nrows, Lp, Li, Lx = W.bcsc(2000, 40, 24, 400, 2)
+    b, c = W.uniform(3, 0, 2000), W.uniform(4, 0, 2000)
+    u = oracle.sptrsv_csc_block_t(sup, nrows, Lp, Li, Lx, b)
+    v = oracle.sptrsv_csc_block(sup, nrows, Lp, Li, Lx, c)
+    assert abs(u @ c - b @ v) <= 1e-12 * max(abs(u @ c), 1.0)
