@@ -152,6 +152,18 @@ const char *symamd_version(void);
 int symamd_levelset(symamd_handle h, int n, const int *d_Lp, const int *d_Li,
                     int *d_level_ptr, int *d_level_set, int *levels);
 
+/* symamd_levelset without any allocation.  No counterpart in the reference;
+ * for callers that cannot allocate device memory (code under graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * results, errors and blocking as symamd_levelset, for n <= 2,097,152 rows
+ * (SYMAMD_ERR_NOT_SUPPORTED above that, before any device work).  Its
+ * working memory (counters, per-row levels, sort keys and sort storage) is
+ * one fixed 48 MiB device array per GPU, held until the process ends and
+ * shared by every call on that GPU: calls that overlap on one GPU (other
+ * threads or streams) race and must be serialised by the caller. */
+int symamd_levelset_noalloc(symamd_handle h, int n, const int *d_Lp, const int *d_Li,
+                            int *d_level_ptr, int *d_level_set, int *levels);
+
 /* Coarsened 3-array schedule (level_ptr / par_ptr / partition), the format
  * sym_lib::sptrsv_csr_lbc consumes (sparse_blas/sptrsv.cpp:57-75), built on
  * the GPU from a device CSR lower triangle (diagonal last).  Replaces the

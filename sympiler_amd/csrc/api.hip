@@ -487,6 +487,14 @@ int symamd_levelset(symamd_handle h, int n, const int *d_Lp, const int *d_Li,
   return rc;
 }
 
+int symamd_levelset_noalloc(symamd_handle h, int n, const int *d_Lp, const int *d_Li,
+                            int *d_level_ptr, int *d_level_set, int *levels) {
+  SYMAMD_TRY(check_handle(h));
+  if (n < 0 || !levels || (n > 0 && (!d_Lp || !d_Li || !d_level_ptr || !d_level_set)))
+    return fail(SYMAMD_ERR_INVALID_ARG, "symamd_levelset_noalloc: bad arguments");
+  return levelset_noalloc(h, n, d_Lp, d_Li, d_level_ptr, d_level_set, levels);
+}
+
 int symamd_coarsen(symamd_handle h, int n, const int *d_Lp, const int *d_Li,
                    const double *d_cost, int inner_parts, int min_level_dist,
                    int div_rate, int *d_level_ptr, int *d_par_ptr, int *d_partition,

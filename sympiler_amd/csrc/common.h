@@ -423,6 +423,9 @@ int build_plan_arrays(symamd_handle h, symamd_plan p, const int *d_Lp,
 int gather_values(symamd_handle h, symamd_plan p, const void *d_Lx);
 int check_structure(symamd_handle h, int n, const int *d_Lp, const int *d_Li,
                     int *d_state);
+// Level sets with every buffer in a fixed device array (inspector.hip).
+int levelset_noalloc(symamd_handle h, int n, const int *d_Lp, const int *d_Li,
+                     int *d_level_ptr, int *d_level_set, int *levels);
 int launch_spmv(symamd_handle h, int n, const int *d_Ap, const int *d_Ai,
                 const double *d_Ax, const double *d_x, const double *d_y,
                 double *d_z, double a, double b, int mode);

@@ -495,6 +495,66 @@ int compute_levels(symamd_handle h, int n, const int *d_Lp, const int *d_Li,
   return rc;
 }
 
+// ------------------------------------------------ allocation-free level sets
+// The level-set inspector for callers that cannot allocate device memory:
+// its state words, per-row levels, sort keys and sort storage live in one
+// fixed device array (each GPU holds its own copy for the life of the
+// process).  Host code reaches the array only through the address
+// hipGetSymbolAddress gives for the current device, looked up once per
+// device and kept.
+constexpr int NOALLOC_MAX_ROWS = 1 << 21;
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)48 << 20;
+static __device__ __align__(256) uint8_t scratch[NOALLOC_SCRATCH_BYTES];
+
+namespace {
+std::mutex g_scratch_mu;
+std::map<int, uint8_t *> g_scratch_addr;  // device -> its scratch address
+
+int scratch_address(int device, uint8_t **out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  auto it = g_scratch_addr.find(device);
+  if (it != g_scratch_addr.end()) {
+    *out = it->second;
+    return SYMAMD_OK;
+  }
+  int cur = -1;
+  SYMAMD_HIP(hipGetDevice(&cur));
+  if (cur != device)
+    return fail(SYMAMD_ERR_INVALID_ARG,
+                "symamd_levelset_noalloc: the handle's device (%d) is not the current one (%d)",
+                device, cur);
+  uint8_t *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return fail(SYMAMD_ERR_HIP, "symamd_levelset_noalloc: hipGetSymbolAddress failed: %s",
+                hipGetErrorString(e));
+  g_scratch_addr[device] = p;
+  *out = p;
+  return SYMAMD_OK;
+}
+
+inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+}  // namespace
+
+int levelset_noalloc(symamd_handle h, int n, const int *d_Lp, const int *d_Li,
+                     int *d_level_ptr, int *d_level_set, int *levels) {
+  if (n > NOALLOC_MAX_ROWS)
+    return fail(SYMAMD_ERR_NOT_SUPPORTED,
+                "symamd_levelset_noalloc: %d rows exceed the fixed buffer's limit of %d "
+                "(use symamd_levelset)", n, NOALLOC_MAX_ROWS);
+  uint8_t *base = nullptr;
+  SYMAMD_TRY(scratch_address(h->device, &base));
+  int *d_state = reinterpret_cast<int *>(base);
+  const size_t o_lev = 256;
+  const size_t o_skey = o_lev + align256(sizeof(int) * (size_t)n);
+  const size_t o_tmp = o_skey + align256(sizeof(int) * (size_t)n);
+  int *d_lev = reinterpret_cast<int *>(base + o_lev);
+  int *d_skey = reinterpret_cast<int *>(base + o_skey);
+  SYMAMD_TRY(check_structure(h, n, d_Lp, d_Li, d_state));
+  return levels_core(h, n, d_Lp, d_Li, d_state, d_lev, d_skey, base + o_tmp,
+                     NOALLOC_SCRATCH_BYTES - o_tmp, d_level_ptr, d_level_set, levels, nullptr);
+}
+
 int build_plan_arrays(symamd_handle h, symamd_plan p, const int *d_Lp,
                       const int *d_Li, const void *d_Lx) {
   const int n = p->n;
