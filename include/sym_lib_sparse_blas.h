@@ -149,6 +149,27 @@ int getCoarseLevelSet_6(size_t n, const int *eTree, const int *blk2col, int &fin
  * Returns 1, or 0 for a null Lp, Li or x as the reference does. */
 int blockedLsolve(int n, size_t *Lp, int *Li, double *Lx, int NNZ, size_t *Li_ptr,
                   int *col2sup, int *sup2col, int supNo, double *x);
+/* Triangular_BCSC.h:20-22 (impl Triangular_BCSC.cpp:62-121): n_rhs
+ * column-major right-hand sides x[r + j*n], each column bit-identical to
+ * blockedLsolve on it.  The reference's builds transpose the diagonal block
+ * in their dtrsm and one of them solves only the first column; every column
+ * of L X = B is solved here.  max_col (the reference's scratch stride) is not
+ * read. */
+int blockedLsolve_mrhs(int n, size_t *Lp, int *Li, double *Lx, int NNZ, size_t *Li_ptr,
+                       int *col2sup, int *sup2col, int supNo, double *x, int n_rhs,
+                       int max_col = -1);
+/* Triangular_BCSC.h:26-27 (impl Triangular_BCSC.cpp:124-184): L^T x = b on
+ * the same storage, supernodes from the last (symamd_sn_solve_transposed;
+ * the operation order is the library's where the reference's is its BLAS's,
+ * so x agrees with the reference's to rounding). */
+int blockedLTsolve(int n, size_t *Lp, int *Li, double *Lx, int NNZ, size_t *Li_ptr,
+                   int *col2sup, int *sup2col, int supNo, double *x);
+/* Triangular_BCSC.h:28-30 (impl :186-249): m_rhs column-major right-hand
+ * sides of L^T X = B, each column as blockedLTsolve solves it (the
+ * reference's CBLAS build solves only the first column's diagonal blocks). */
+int blockedLTsolve_mrhs(int n, size_t *Lp, int *Li, double *Lx, int NNZ, size_t *Li_ptr,
+                        int *col2sup, int *sup2col, int supNo, double *x, int m_rhs,
+                        int max_col = -1);
 /* Triangular_BCSC.h (impl Triangular_BCSC.cpp:512-579): the leveled variant
  * over a getCoarseLevelSet_6 schedule of the supernodes.  The reference
  * subtracts with atomics in thread order; here x is blockedLsolve's (serial
@@ -158,6 +179,31 @@ int H2LeveledBlockedLsolve(int n, size_t *Lp, int *Li, double *Lx, int NNZ, size
                            int *col2sup, int *sup2col, int supNo, double *x, int levels,
                            int *levelPtr, int *levelSet, int parts, int *parPtr,
                            int *partition, int chunk);
+/* Triangular_BCSC.h:80-83 (impl :583-653): n_rhs columns on a leveled
+ * schedule, x as blockedLsolve_mrhs gives it; the schedule is checked as
+ * H2LeveledBlockedLsolve checks it. */
+int H2LeveledBlockedLsolve_mrhs(int n, size_t *Lp, int *Li, double *Lx, int NNZ,
+                                size_t *Li_ptr, int *col2sup, int *sup2col, int supNo,
+                                double *x, int n_rhs, int levels, int *levelPtr, int *levelSet,
+                                int parts, int *parPtr, int *partition, int chunk,
+                                int max_col = -1);
+/* Triangular_BCSC.h:95-98 (impl :741-808): L^T x = b on a leveled schedule
+ * (walked from its last level), x as blockedLTsolve gives it; the schedule is
+ * checked as H2LeveledBlockedLsolve checks it. */
+int H2LeveledBlockedLTsolve(int n, size_t *Lp, int *Li, double *Lx, int NNZ, size_t *Li_ptr,
+                            int *col2sup, int *sup2col, int supNo, double *x, int levels,
+                            int *levelPtr, int *levelSet, int parts, int *parPtr,
+                            int *partition, int chunk);
+/* Triangular_BCSC.h:99-102 (impl :810-880): m_rhs columns of L^T X = B on a
+ * leveled schedule, x as blockedLTsolve_mrhs gives it.  The reference numbers
+ * partition entry i as the columns sup2col[i-1] .. sup2col[i]-1 here (unlike
+ * its other leveled solves), which skips the last supernode; supernodes are
+ * numbered here as in the other leveled solves. */
+int H2LeveledBlockedLTsolve_mrhs(int n, size_t *Lp, int *Li, double *Lx, int NNZ,
+                                 size_t *Li_ptr, int *col2sup, int *sup2col, int supNo,
+                                 double *x, int m_rhs, int levels, int *levelPtr,
+                                 int *levelSet, int parts, int *parPtr, int *partition,
+                                 int chunk, int max_col = -1);
 }  // namespace parsy
 
 }  // namespace sym_lib

@@ -424,26 +424,42 @@ void spilu0_csr_lbc(int n, int nnz, const int *Ap, const int *Ai, const double *
 }
 
 // ------------------------------------------------------------ supernodal
-// sptrsv_csc_block on the device: the structure is analysed from the host
-// arrays, Lx and x are copied in, x is copied back.
-void sptrsv_csc_block(int n, const int *Lp, const int *Li, const int *nrows, double *Lx,
-                      double *x, int num_nodes, const int *supernodes) {
-  if (n <= 0 || num_nodes <= 0) return;
+namespace {
+// One supernodal solve on the device: the structure is analysed from the
+// host arrays, Lx and the k column-major right-hand sides (leading dimension
+// n; k = 1: a vector) are copied in, forward (L X = B) or transposed
+// (L^T X = B), and X is copied back.
+void sn_device_solve(const char *who, int n, const int *Lp, const int *Li, const int *nrows,
+                     const double *Lx, double *x, int num_nodes, const int *supernodes, int k,
+                     bool transposed) {
+  if (n <= 0 || num_nodes <= 0 || k <= 0) return;
   symamd_handle h = handle();
   int64_t vals = 0;
   for (int i = 0; i < num_nodes; ++i)
     vals = std::max<int64_t>(vals, (int64_t)Lp[i] +
                                        (int64_t)nrows[i] * (supernodes[i + 1] - supernodes[i]));
   symamd_snplan plan = nullptr;
-  check(symamd_sn_analyze(h, n, num_nodes, supernodes, nrows, Lp, Li, &plan), "sptrsv_csc_block");
+  check(symamd_sn_analyze(h, n, num_nodes, supernodes, nrows, Lp, Li, &plan), who);
   struct Guard {
     symamd_snplan p;
     ~Guard() { symamd_sn_destroy(p); }
   } guard{plan};
-  DevBuf<double> dLx((size_t)vals, Lx), dx(n, x);
-  check(symamd_sn_solve(h, plan, dLx.p, dx.p), "sptrsv_csc_block");
-  check(symamd_sn_check(h, plan), "sptrsv_csc_block");
-  dx.download(x, n);
+  const size_t xs = (size_t)n * (size_t)k;
+  DevBuf<double> dLx((size_t)vals, Lx), dx(xs, x);
+  if (transposed)
+    check(symamd_sn_solve_transposed(h, plan, dLx.p, dx.p, k, n), who);
+  else if (k == 1)
+    check(symamd_sn_solve(h, plan, dLx.p, dx.p), who);
+  else
+    check(symamd_sn_solve_block(h, plan, dLx.p, dx.p, k, n), who);
+  check(symamd_sn_check(h, plan), who);
+  dx.download(x, xs);
+}
+}  // namespace
+
+void sptrsv_csc_block(int n, const int *Lp, const int *Li, const int *nrows, double *Lx,
+                      double *x, int num_nodes, const int *supernodes) {
+  sn_device_solve("sptrsv_csc_block", n, Lp, Li, nrows, Lx, x, num_nodes, supernodes, 1, false);
 }
 
 // The level schedule and per-thread temporaries are not needed (the
@@ -557,41 +573,138 @@ int getCoarseLevelSet_6(size_t n, const int *eTree, const int *blk2col, int &fin
 }
 
 // parsy's supernodal storage (Triangular_BCSC.cpp:24-56): per-column value
-// and row-list offsets (size_t), a supernode's row list at Li[Li_ptr[c0]].
-// Its dense kernels (parsy/triangularSolve/BLAS.cpp) order every operation
-// as sym_lib's lsolve_BLAS / matvec_BLAS do, so the solve is
-// sptrsv_csc_block on the same blocks: the row lists move to the value
-// offsets that format indexes them by.  col2sup and NNZ are not read (as in
-// the reference); the supernodes must cover the n columns.
-int blockedLsolve(int n, size_t *Lp, int *Li, double *Lx, int NNZ, size_t *Li_ptr,
-                  int *col2sup, int *sup2col, int supNo, double *x) {
-  (void)NNZ;
-  (void)col2sup;
-  if (!Lp || !Li || !x) return 0;
-  if (supNo < 0 || !Li_ptr || !sup2col || !Lx)
-    throw std::invalid_argument("blockedLsolve: bad arguments");
-  std::vector<int> sup(supNo + 1), nrows(supNo), off(supNo);
+// and row-list offsets (size_t), a supernode's row list at Li[Li_ptr[c0]],
+// supernode i owning columns sup2col[i] .. sup2col[i+1]-1.  Converted to
+// sym_lib's block storage (the row lists move to the value offsets that
+// format indexes them by); the values stay where they are.
+namespace {
+struct ParsyBlocks {
+  std::vector<int> sup, nrows, off, rows_at;
+};
+
+ParsyBlocks parsy_blocks(const char *who, int n, const size_t *Lp, const int *Li,
+                         const size_t *Li_ptr, const int *sup2col, int supNo) {
+  ParsyBlocks b;
+  b.sup.resize(supNo + 1);
+  b.nrows.resize(supNo);
+  b.off.resize(supNo);
   size_t span = 0;
   for (int i = 0; i < supNo; ++i) {
     const int c0 = sup2col[i], c1 = sup2col[i + 1];
     if (c0 < 0 || c1 <= c0 || c1 > n)
-      throw std::invalid_argument("blockedLsolve: bad sup2col");
+      throw std::invalid_argument(std::string(who) + ": bad sup2col");
     const size_t rows = Li_ptr[c1] - Li_ptr[c0];
     if (rows < (size_t)(c1 - c0) || Lp[c0] + rows > (size_t)0x7fffffff)
-      throw std::invalid_argument("blockedLsolve: bad row count or more than 2^31-1 values");
-    sup[i] = c0;
-    nrows[i] = (int)rows;
-    off[i] = (int)Lp[c0];
+      throw std::invalid_argument(std::string(who) +
+                                  ": bad row count or more than 2^31-1 values");
+    b.sup[i] = c0;
+    b.nrows[i] = (int)rows;
+    b.off[i] = (int)Lp[c0];
     span = std::max(span, (size_t)Lp[c0] + rows);
   }
-  sup[supNo] = supNo > 0 ? sup2col[supNo] : 0;
-  std::vector<int> rows_at(span > 0 ? span : 1, 0);
+  b.sup[supNo] = supNo > 0 ? sup2col[supNo] : 0;
+  b.rows_at.assign(span > 0 ? span : 1, 0);
   for (int i = 0; i < supNo; ++i)
-    for (int r = sup[i + 1] - sup[i]; r < nrows[i]; ++r)
-      rows_at[(size_t)off[i] + r] = Li[Li_ptr[sup[i]] + r];
-  sym_lib::sptrsv_csc_block(n, off.data(), rows_at.data(), nrows.data(), Lx, x, supNo,
-                            sup.data());
+    for (int r = b.sup[i + 1] - b.sup[i]; r < b.nrows[i]; ++r)
+      b.rows_at[(size_t)b.off[i] + r] = Li[Li_ptr[b.sup[i]] + r];
+  return b;
+}
+
+// k column-major right-hand sides of leading dimension n, forward or
+// transposed.  0 for a null Lp, Li or x, as the reference returns.
+int parsy_solve(const char *who, int n, const size_t *Lp, const int *Li, const double *Lx,
+                const size_t *Li_ptr, const int *sup2col, int supNo, double *x, int k,
+                bool transposed) {
+  if (!Lp || !Li || !x) return 0;
+  if (supNo < 0 || k < 0 || !Li_ptr || !sup2col || !Lx)
+    throw std::invalid_argument(std::string(who) + ": bad arguments");
+  ParsyBlocks b = parsy_blocks(who, n, Lp, Li, Li_ptr, sup2col, supNo);
+  sn_device_solve(who, n, b.off.data(), b.rows_at.data(), b.nrows.data(), Lx, x, supNo,
+                  b.sup.data(), k, transposed);
   return 1;
+}
+
+// The leveled variants take a getCoarseLevelSet_6 schedule of the supernodes
+// (levelPtr / parPtr / partition); it must list every supernode once, as the
+// reference needs for a complete solve.
+void check_parsy_schedule(const char *who, int supNo, int levels, const int *levelPtr,
+                          const int *parPtr, const int *partition) {
+  if (levels < 0 || (levels > 0 && (!levelPtr || !parPtr || !partition)))
+    throw std::invalid_argument(std::string(who) + ": bad schedule arguments");
+  std::vector<char> seen(supNo > 0 ? supNo : 1, 0);
+  long listed = 0;
+  for (int l = 0; l < levels; ++l)
+    for (int p = levelPtr[l]; p < levelPtr[l + 1]; ++p)
+      for (int k = parPtr[p]; k < parPtr[p + 1]; ++k) {
+        const int s = partition[k];
+        if (s < 0 || s >= supNo || seen[s])
+          throw std::invalid_argument(std::string(who) +
+                                      ": the schedule must list every supernode once");
+        seen[s] = 1;
+        ++listed;
+      }
+  if (listed != supNo)
+    throw std::invalid_argument(std::string(who) +
+                                ": the schedule must list every supernode once");
+}
+}  // namespace
+
+// Forward solve (Triangular_BCSC.cpp:23-60).  Its dense kernels
+// (parsy/triangularSolve/BLAS.cpp) order every operation as sym_lib's
+// lsolve_BLAS / matvec_BLAS do, so the solve is sptrsv_csc_block on the same
+// blocks, bit for bit.  col2sup and NNZ are not read (as in the reference);
+// the supernodes must cover the n columns.
+int blockedLsolve(int n, size_t *Lp, int *Li, double *Lx, int NNZ, size_t *Li_ptr,
+                  int *col2sup, int *sup2col, int supNo, double *x) {
+  (void)NNZ;
+  (void)col2sup;
+  return parsy_solve("blockedLsolve", n, Lp, Li, Lx, Li_ptr, sup2col, supNo, x, 1, false);
+}
+
+// n_rhs column-major right-hand sides x[r + j*n] (Triangular_BCSC.cpp:62-121),
+// each column bit-identical to blockedLsolve on it.  The reference's own
+// multi-column code departs from that intent in both of its builds: each
+// passes a transposed flag to the diagonal block's dtrsm (CblasTrans at :85,
+// "T" at :88), which solves with the block's transpose in a forward solve,
+// and the non-CBLAS build solves only the first column there (:88, ione).
+// This entry point solves every column of L X = B.  max_col only strides the
+// reference's scratch and is not read.
+int blockedLsolve_mrhs(int n, size_t *Lp, int *Li, double *Lx, int NNZ, size_t *Li_ptr,
+                       int *col2sup, int *sup2col, int supNo, double *x, int n_rhs,
+                       int max_col) {
+  (void)NNZ;
+  (void)col2sup;
+  (void)max_col;
+  return parsy_solve("blockedLsolve_mrhs", n, Lp, Li, Lx, Li_ptr, sup2col, supNo, x, n_rhs,
+                     false);
+}
+
+// Backward solve L^T x = b (Triangular_BCSC.cpp:124-184), supernodes from
+// the last: each top column's value minus the products of the block below
+// with the solved values of its rows, then the top's transposed triangular
+// solve.  The reference hands both steps to the BLAS it links (dgemv "T",
+// dtrsm "L","L","T"), whose operation order is the BLAS's; here the order is
+// the library's (symamd_sn_solve_transposed), the same in every column, and
+// x agrees with the reference's to rounding.
+int blockedLTsolve(int n, size_t *Lp, int *Li, double *Lx, int NNZ, size_t *Li_ptr,
+                   int *col2sup, int *sup2col, int supNo, double *x) {
+  (void)NNZ;
+  (void)col2sup;
+  return parsy_solve("blockedLTsolve", n, Lp, Li, Lx, Li_ptr, sup2col, supNo, x, 1, true);
+}
+
+// m_rhs column-major right-hand sides of L^T X = B (Triangular_BCSC.cpp:
+// 186-249), each column as blockedLTsolve solves it.  The reference's CBLAS
+// build solves only the first column in the diagonal block's dtrsm (:233,
+// ione); this entry point solves every column.  max_col is not read.
+int blockedLTsolve_mrhs(int n, size_t *Lp, int *Li, double *Lx, int NNZ, size_t *Li_ptr,
+                        int *col2sup, int *sup2col, int supNo, double *x, int m_rhs,
+                        int max_col) {
+  (void)NNZ;
+  (void)col2sup;
+  (void)max_col;
+  return parsy_solve("blockedLTsolve_mrhs", n, Lp, Li, Lx, Li_ptr, sup2col, supNo, x, m_rhs,
+                     true);
 }
 
 // The leveled variant (Triangular_BCSC.cpp:512-579): the reference solves the
@@ -599,10 +712,8 @@ int blockedLsolve(int n, size_t *Lp, int *Li, double *Lx, int NNZ, size_t *Li_pt
 // products from x with atomics, so its subtraction order -- and the last
 // bits of x -- follow the thread schedule.  Here the same supernodes are
 // solved by the dependency-driven kernel in the serial order of
-// blockedLsolve (x within rounding of the reference's).  The schedule
-// (levelPtr / parPtr / partition, from getCoarseLevelSet_6) must list every
-// supernode once, as the reference needs for a complete solve; levelSet and
-// chunk are not read by the reference either.
+// blockedLsolve (x within rounding of the reference's).  levelSet and chunk
+// are not read by the reference either.
 int H2LeveledBlockedLsolve(int n, size_t *Lp, int *Li, double *Lx, int NNZ, size_t *Li_ptr,
                            int *col2sup, int *sup2col, int supNo, double *x, int levels,
                            int *levelPtr, int *levelSet, int parts, int *parPtr,
@@ -611,24 +722,67 @@ int H2LeveledBlockedLsolve(int n, size_t *Lp, int *Li, double *Lx, int NNZ, size
   (void)parts;
   (void)chunk;
   if (!Lp || !Li || !x) return 0;
-  if (levels < 0 || (levels > 0 && (!levelPtr || !parPtr || !partition)))
-    throw std::invalid_argument("H2LeveledBlockedLsolve: bad schedule arguments");
-  std::vector<char> seen(supNo > 0 ? supNo : 1, 0);
-  long listed = 0;
-  for (int l = 0; l < levels; ++l)
-    for (int p = levelPtr[l]; p < levelPtr[l + 1]; ++p)
-      for (int k = parPtr[p]; k < parPtr[p + 1]; ++k) {
-        const int s = partition[k];
-        if (s < 0 || s >= supNo || seen[s])
-          throw std::invalid_argument(
-              "H2LeveledBlockedLsolve: the schedule must list every supernode once");
-        seen[s] = 1;
-        ++listed;
-      }
-  if (listed != supNo)
-    throw std::invalid_argument(
-        "H2LeveledBlockedLsolve: the schedule must list every supernode once");
+  check_parsy_schedule("H2LeveledBlockedLsolve", supNo, levels, levelPtr, parPtr, partition);
   return blockedLsolve(n, Lp, Li, Lx, NNZ, Li_ptr, col2sup, sup2col, supNo, x);
+}
+
+// n_rhs columns on a leveled schedule (Triangular_BCSC.cpp:583-653): x as
+// blockedLsolve_mrhs gives it.  As in blockedLsolve_mrhs, the reference's
+// builds transpose the diagonal block in its dtrsm (CblasConjTrans at :616,
+// "T" at :619) and the non-CBLAS one solves only the first column (:619);
+// every column of L X = B is solved here.
+int H2LeveledBlockedLsolve_mrhs(int n, size_t *Lp, int *Li, double *Lx, int NNZ,
+                                size_t *Li_ptr, int *col2sup, int *sup2col, int supNo,
+                                double *x, int n_rhs, int levels, int *levelPtr, int *levelSet,
+                                int parts, int *parPtr, int *partition, int chunk,
+                                int max_col) {
+  (void)levelSet;
+  (void)parts;
+  (void)chunk;
+  if (!Lp || !Li || !x) return 0;
+  check_parsy_schedule("H2LeveledBlockedLsolve_mrhs", supNo, levels, levelPtr, parPtr,
+                       partition);
+  return blockedLsolve_mrhs(n, Lp, Li, Lx, NNZ, Li_ptr, col2sup, sup2col, supNo, x, n_rhs,
+                            max_col);
+}
+
+// L^T x = b on a leveled schedule (Triangular_BCSC.cpp:741-808; levels
+// walked from the last, parts' supernodes from the last): every supernode
+// still waits for the supernodes its rows below belong to, so x is
+// blockedLTsolve's.
+int H2LeveledBlockedLTsolve(int n, size_t *Lp, int *Li, double *Lx, int NNZ, size_t *Li_ptr,
+                            int *col2sup, int *sup2col, int supNo, double *x, int levels,
+                            int *levelPtr, int *levelSet, int parts, int *parPtr,
+                            int *partition, int chunk) {
+  (void)levelSet;
+  (void)parts;
+  (void)chunk;
+  if (!Lp || !Li || !x) return 0;
+  check_parsy_schedule("H2LeveledBlockedLTsolve", supNo, levels, levelPtr, parPtr, partition);
+  return blockedLTsolve(n, Lp, Li, Lx, NNZ, Li_ptr, col2sup, sup2col, supNo, x);
+}
+
+// m_rhs columns of L^T X = B on a leveled schedule (Triangular_BCSC.cpp:
+// 810-880).  The reference numbers the supernode of partition entry i as
+// columns sup2col[i-1] .. sup2col[i]-1 there (:836-837), unlike the other
+// leveled solves (sup2col[i] .. sup2col[i+1]-1), so on a schedule of
+// supernodes 0 .. supNo-1 it skips the last supernode; its CBLAS build also
+// updates and solves only the first column (dgemv at :858, dtrsm with ione
+// at :871).  This entry point numbers supernodes as the others do and
+// solves every column: x as blockedLTsolve_mrhs gives it.
+int H2LeveledBlockedLTsolve_mrhs(int n, size_t *Lp, int *Li, double *Lx, int NNZ,
+                                 size_t *Li_ptr, int *col2sup, int *sup2col, int supNo,
+                                 double *x, int m_rhs, int levels, int *levelPtr,
+                                 int *levelSet, int parts, int *parPtr, int *partition,
+                                 int chunk, int max_col) {
+  (void)levelSet;
+  (void)parts;
+  (void)chunk;
+  if (!Lp || !Li || !x) return 0;
+  check_parsy_schedule("H2LeveledBlockedLTsolve_mrhs", supNo, levels, levelPtr, parPtr,
+                       partition);
+  return blockedLTsolve_mrhs(n, Lp, Li, Lx, NNZ, Li_ptr, col2sup, sup2col, supNo, x, m_rhs,
+                             max_col);
 }
 
 }  // namespace parsy

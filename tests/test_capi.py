@@ -67,6 +67,12 @@ def test_sym_lib_dropins_exported_with_reference_mangling():
         "_ZN7sym_lib32get_coarse_levelSet_DAG_CSC_treeEmPKiS1_iRiRPiS2_S4_S4_iiiPd",
         "_ZN7sym_lib5parsy13blockedLsolveEiPmPiPdiS1_S2_S2_iS3_",
         "_ZN7sym_lib5parsy22H2LeveledBlockedLsolveEiPmPiPdiS1_S2_S2_iS3_iS2_S2_iS2_S2_i",
+        "_ZN7sym_lib5parsy18blockedLsolve_mrhsEiPmPiPdiS1_S2_S2_iS3_ii",
+        "_ZN7sym_lib5parsy14blockedLTsolveEiPmPiPdiS1_S2_S2_iS3_",
+        "_ZN7sym_lib5parsy19blockedLTsolve_mrhsEiPmPiPdiS1_S2_S2_iS3_ii",
+        "_ZN7sym_lib5parsy27H2LeveledBlockedLsolve_mrhsEiPmPiPdiS1_S2_S2_iS3_iiS2_S2_iS2_S2_ii",
+        "_ZN7sym_lib5parsy23H2LeveledBlockedLTsolveEiPmPiPdiS1_S2_S2_iS3_iS2_S2_iS2_S2_i",
+        "_ZN7sym_lib5parsy28H2LeveledBlockedLTsolve_mrhsEiPmPiPdiS1_S2_S2_iS3_iiS2_S2_iS2_S2_ii",
     }
     missing = want - exported(LIB)
     assert not missing, missing
@@ -94,6 +100,21 @@ def test_mangled_names_match_the_reference_objects():
     ref |= {s for s in exported(oracle.REF_PATH)
             if s.startswith("_ZN7sym_lib5parsy19getCoarseLevelSet_6")}
     assert len(ref) == 19, sorted(ref)
+    missing = ref - exported(LIB)
+    assert not missing, missing
+
+
+def test_parsy_solves_match_the_reference_declarations():
+    # oracle/_ref/parsy_decls.o takes the address of every supernodal solve
+    # declared in the reference's parsy/triangularSolve/Triangular_BCSC.h
+    # (built from that header by oracle/Makefile): its undefined symbols are
+    # the reference's mangled names, all exported by the library
+    obj = os.path.join(ROOT, "oracle", "_ref", "parsy_decls.o")
+    if not os.path.exists(obj):
+        pytest.skip("oracle/_ref/parsy_decls.o not built")
+    out = subprocess.run(["nm", "-u", obj], capture_output=True, text=True, check=True).stdout
+    ref = {ln.split()[-1] for ln in out.splitlines() if "_ZN7sym_lib5parsy" in ln}
+    assert len(ref) == 8, sorted(ref)
     missing = ref - exported(LIB)
     assert not missing, missing
 

@@ -195,3 +195,124 @@ def test_parsy_leveled_blocked_lsolve_with_tree_schedule(dev):
     assert r == 1
     assert bitwise_equal(x, want)
     assert rel_err(x, want) <= 1e-12
+
+
+@pytest.mark.parametrize("case", CASES + [(60000, 64, 64, 8192, 7), (20000, 300, 600, 4096, 8)])
+def test_supernodal_transposed_bitwise(dev, case):
+    """L^T x = b on the same plan (blockedLTsolve's semantics): bit-identical
+    to the oracle's restatement (pinned by dense solves and exact integer
+    cases), on tops wider than one column chunk and blocks with more rows
+    below than one pass of the rows."""
+    n = case[0]
+    sup, nrows, Lp, Li, Lx = W.bcsc(*case)
+    print(f"[snT] case={case} supernodes={len(sup) - 1} values={len(Lx)}", flush=True)
+    b = W.uniform(case[-1], 1, n)
+    want = oracle.sptrsv_csc_block_t(sup, nrows, Lp, Li, Lx, b)
+    plan = sa.SupernodalPlan(sup, nrows, Lp, Li)
+    dLx = to_dev(Lx)
+    for _ in range(2):  # the second solve reuses the tags (next epoch)
+        x = to_dev(b.copy())
+        plan.solve_transposed(dLx, x)
+        plan.check()
+        assert bitwise_equal(x.cpu().numpy(), want)
+    # the forward solve after the transposed one, on the same plan
+    x = to_dev(b.copy())
+    plan.solve(dLx, x)
+    plan.check()
+    assert bitwise_equal(x.cpu().numpy(), oracle.sptrsv_csc_block(sup, nrows, Lp, Li, Lx, b))
+
+
+@pytest.mark.parametrize("k", [1, 3, 32])
+def test_supernodal_transposed_block_columns_bitwise(dev, k):
+    n = 3000
+    sup, nrows, Lp, Li, Lx = W.bcsc(n, 100, 70, 1500, 3)
+    B = W.uniform(12, 0, k * (n + 7)).reshape(k, n + 7)  # column-major, ld = n + 7
+    plan = sa.SupernodalPlan(sup, nrows, Lp, Li)
+    dLx = to_dev(Lx)
+    for _ in range(2):
+        X = to_dev(B.copy())
+        plan.solve_transposed(dLx, X[:, :n])
+        plan.check()
+        got = X.cpu().numpy()
+        for c in range(k):
+            want = oracle.sptrsv_csc_block_t(sup, nrows, Lp, Li, Lx, B[c, :n])
+            assert bitwise_equal(got[c, :n], want), c
+        assert bitwise_equal(got[:, n:], B[:, n:])  # the padding is untouched
+
+
+def _parsy_args(sup, nrows, Lp, Li, Lx, n):
+    from tests.test_supernodal_oracle import to_parsy
+    vp, rows, rp = to_parsy(sup, nrows, Lp, Li, n)
+    col2sup = np.repeat(np.arange(len(sup) - 1, dtype=np.int32), np.diff(sup))
+    return [c_int(n), P(vp), P(rows), P(Lx), c_int(len(Lx)), P(rp), P(col2sup), P(sup),
+            c_int(len(sup) - 1)], (vp, rows, rp, col2sup)
+
+
+MANGLED = {
+    "blockedLsolve_mrhs": "_ZN7sym_lib5parsy18blockedLsolve_mrhsEiPmPiPdiS1_S2_S2_iS3_ii",
+    "blockedLTsolve": "_ZN7sym_lib5parsy14blockedLTsolveEiPmPiPdiS1_S2_S2_iS3_",
+    "blockedLTsolve_mrhs": "_ZN7sym_lib5parsy19blockedLTsolve_mrhsEiPmPiPdiS1_S2_S2_iS3_ii",
+    "H2LeveledBlockedLsolve_mrhs":
+        "_ZN7sym_lib5parsy27H2LeveledBlockedLsolve_mrhsEiPmPiPdiS1_S2_S2_iS3_iiS2_S2_iS2_S2_ii",
+    "H2LeveledBlockedLTsolve":
+        "_ZN7sym_lib5parsy23H2LeveledBlockedLTsolveEiPmPiPdiS1_S2_S2_iS3_iS2_S2_iS2_S2_i",
+    "H2LeveledBlockedLTsolve_mrhs":
+        "_ZN7sym_lib5parsy28H2LeveledBlockedLTsolve_mrhsEiPmPiPdiS1_S2_S2_iS3_iiS2_S2_iS2_S2_ii",
+}
+
+
+def _fn(name):
+    f = ctypes.CDLL(LIB)[MANGLED[name]]
+    f.restype = c_int
+    return f
+
+
+@pytest.mark.parametrize("case", [(3000, 40, 24, 400, 2), (20000, 100, 130, 4096, 5)])
+def test_parsy_blocked_transposed_and_multi_rhs_dropins(dev, case):
+    """blockedLTsolve on parsy's storage: the transposed restatement bit for
+    bit; blockedLsolve_mrhs / blockedLTsolve_mrhs: every column of x[r + j*n]
+    as the single-column solves give it (the reference's own multi-column
+    builds solve only the first column or the transposed diagonal block)."""
+    n = case[0]
+    sup, nrows, Lp, Li, Lx = W.bcsc(*case)
+    args, keep = _parsy_args(sup, nrows, Lp, Li, Lx, n)
+    b = W.uniform(case[-1], 2, n)
+    x = b.copy()
+    assert _fn("blockedLTsolve")(*args, P(x)) == 1
+    assert bitwise_equal(x, oracle.sptrsv_csc_block_t(sup, nrows, Lp, Li, Lx, b))
+    k = 4
+    B = W.uniform(case[-1], 3, k * n).reshape(k, n)  # column j at x[j*n]
+    for name, ref in (("blockedLsolve_mrhs", oracle.sptrsv_csc_block),
+                      ("blockedLTsolve_mrhs", oracle.sptrsv_csc_block_t)):
+        X = B.copy()
+        assert _fn(name)(*args, P(X), c_int(k), c_int(-1)) == 1
+        for j in range(k):
+            assert bitwise_equal(X[j], ref(sup, nrows, Lp, Li, Lx, B[j])), (name, j)
+    # a null right-hand side returns 0, as the reference does
+    assert _fn("blockedLTsolve")(*args, None) == 0
+
+
+def test_parsy_leveled_transposed_and_multi_rhs_with_tree_schedule(dev):
+    """The leveled variants on the tree-level coarsening's schedule of the
+    supernodal tree: x as the serial solves give it, every column."""
+    from tests.helpers import postorder_binary_tree
+    levels = 9
+    sup, nrows, Lp, Li, Lx = W.bcsc_tree(levels, 4, 64, 2.0, 3)
+    n, m = int(sup[-1]), len(sup) - 1
+    parent = postorder_binary_tree(levels)
+    cost = (np.diff(sup) * nrows).astype(np.float64)
+    lp, pp, part = sa.tree_coarsen(parent, cost, 8, 1, 2)
+    sched = [c_int(len(lp) - 1), P(lp), None, c_int(len(pp) - 1), P(pp), P(part), c_int(1)]
+    args, keep = _parsy_args(sup, nrows, Lp, Li, Lx, n)
+    b = W.uniform(5, 0, n)
+    x = b.copy()
+    assert _fn("H2LeveledBlockedLTsolve")(*args, P(x), *sched) == 1
+    assert bitwise_equal(x, oracle.sptrsv_csc_block_t(sup, nrows, Lp, Li, Lx, b))
+    k = 3
+    B = W.uniform(6, 0, k * n).reshape(k, n)
+    for name, ref in (("H2LeveledBlockedLsolve_mrhs", oracle.sptrsv_csc_block),
+                      ("H2LeveledBlockedLTsolve_mrhs", oracle.sptrsv_csc_block_t)):
+        X = B.copy()
+        assert _fn(name)(*args, P(X), c_int(k), *sched[:-1], c_int(1), c_int(-1)) == 1
+        for j in range(k):
+            assert bitwise_equal(X[j], ref(sup, nrows, Lp, Li, Lx, B[j])), (name, j)
