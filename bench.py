@@ -59,7 +59,72 @@ def parse():
                     help="only time the device IC(0) and ILU(0) factorizations")
     ap.add_argument("--supernodal", action="store_true",
                     help="only time the supernodal (BCSC) solve")
+    ap.add_argument("--leg", choices=["k1", "adjoint", "adjoint_block", "zipf", "batch"],
+                    help="only run one workload (for profiling): the single right-hand "
+                         "side region solve or the adjoint of the --grid factor, its "
+                         "block adjoint at --k, the Zipf SpTRSM at --k (column-major, "
+                         "tuned), or the 512-system batch replayed from a graph")
     return ap.parse_args()
+
+
+def run_leg(args):
+    """One workload alone, --warmup untimed and --steps timed repetitions,
+    printed as one JSON line (HIP event time per repetition on the launch
+    stream, and the algorithmic bytes of one repetition)."""
+    import torch
+
+    import sympiler_amd as sa
+    from sympiler_amd import workloads as W
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+    leg, k = args.leg, args.k
+    out = {"leg": leg}
+    up = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    reset = lambda: None
+    if leg in ("k1", "adjoint", "adjoint_block"):
+        Lp, Li, Lx = W.ic0_lap3d(args.grid, 0.01)
+        n, nnz = len(Lp) - 1, int(Lp[-1])
+        plan = sa.Plan(up(Lp), up(Li), up(Lx))
+        out["workload"] = f"ic0_lap3d_{args.grid}^3"
+        if leg == "k1":
+            b, x = up(W.uniform(1, 0, n)), up(np.zeros(n))
+            fn, reset = (lambda: plan.solve(x, executor="regions")), (lambda: x.copy_(b))
+            nbytes = W.algorithmic_bytes_forward(n, nnz, 1)
+        elif leg == "adjoint":
+            xs, gx = up(W.uniform(1, 0, n)), up(W.uniform(2, 0, n))
+            gb, gL = torch.empty_like(gx), torch.empty(nnz, dtype=torch.float64, device=dev)
+            fn = lambda: plan.adjoint(xs, gx, gb, gL)
+            nbytes = W.algorithmic_bytes_adjoint(n, nnz)
+        else:
+            X, GX = up(W.uniform(1, 0, n * k).reshape(n, k)), up(W.uniform(2, 0, n * k).reshape(n, k))
+            GB, gL = torch.empty_like(GX), torch.empty(nnz, dtype=torch.float64, device=dev)
+            fn = lambda: plan.adjoint_block(X, GX, GB, gL, layout="row")
+            nbytes = W.algorithmic_bytes_adjoint_block(n, nnz, k)
+            out["k"] = k
+    elif leg == "zipf":
+        w3 = W.config3(k=1)
+        plan = sa.Plan(up(w3.Lp), up(w3.Li), up(w3.Lx))
+        B0 = up(W.uniform(2, 0, w3.n * k).reshape(k, w3.n))
+        X = torch.empty_like(B0)
+        out.update(workload="zipf_n1e6", k=k,
+                   tuned=plan.tune(k=k, layout="col").strip().splitlines()[-1])
+        fn, reset = (lambda: plan.solve(X, layout="col")), (lambda: X.copy_(B0))
+        nbytes = W.algorithmic_bytes_forward(w3.n, w3.nnz, k)
+    else:
+        w4 = W.config4()
+        plan = sa.Plan(up(w4.Lp), up(w4.Li), up(w4.Lx), row_offsets=w4.row_offsets)
+        b4 = up(w4.b)
+        X4 = b4.clone()
+        out.update(workload="batch512", tuned=plan.tune(k=1).strip().splitlines()[-1])
+        g = plan.graph(X4)
+        fn, reset = g.launch, (lambda: X4.copy_(b4))
+        nbytes = W.algorithmic_bytes_forward(w4.n, w4.nnz, 1)
+    _, evs = timed_steps(fn, reset, args.steps, args.warmup, stream, torch)
+    plan.check()
+    ms = float(np.mean(evs))
+    out.update(steps=args.steps, warmup=args.warmup, ms=round(ms, 4),
+               algorithmic_bytes=int(nbytes), gbs=round(nbytes / (ms * 1e-3) / 1e9, 1))
+    print(json.dumps(out), flush=True)
 
 
 def calibrate_copy():
@@ -315,6 +380,9 @@ def main():
     args = parse()
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.leg:
+        run_leg(args)
         return
     if args.factors or args.supernodal:
         import torch

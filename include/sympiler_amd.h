@@ -322,8 +322,11 @@ int symamd_adjoint(symamd_handle h, symamd_plan plan, const void *d_x,
  * has no adjoint; the batched form of symamd_adjoint):
  *   GB = L^{-T} GX                      column by column
  *   gLx[p] = -(GB[i,0]*X[j,0] + ... + GB[i,k-1]*X[j,k-1])
- *                                       products added in ascending column,
- *                                       for every stored entry p = (i, j).
+ *                                       for every stored entry p = (i, j),
+ * the k products added pairwise: in groups of G columns (G = the smallest
+ * power of two >= k when k <= 32, else 32; a group padded with +0.0 past
+ * column k), each reduced by halving (s[c] += s[c + h] for c < h, h = G/2,
+ * .., 1), group sums added in ascending order (k = 1: -(gb[i] * x[j])).
  * X, GX and GB share k, ld and layout; d_GB may alias d_GX; d_gLx may be
  * NULL.  The region executor (SYMAMD_EXEC_AUTO when the matrix has a
  * usable region schedule, or SYMAMD_EXEC_REGIONS) solves the whole block in

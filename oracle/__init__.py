@@ -77,6 +77,7 @@ def olib():
             "orc_check_schedule": ([i, vp, vp, i, vp, vp, vp], c_int),
             "orc_adjoint": ([i, vp, vp, vp, vp, vp, vp, vp, vp, vp], None),
             "orc_adjoint_block": ([i, i, vp, vp, vp, vp, vp, vp, vp, vp, vp], None),
+            "orc_pairwise_dot": ([vp, vp, i], ctypes.c_double),
             "orc_csr_to_csc": ([i, vp, vp, vp, vp, vp, vp, vp], None),
         }
         for name, (args, res) in sigs.items():
@@ -308,7 +309,7 @@ def adjoint(Lp, Li, Lx, x, gx):
 
 def adjoint_block(Lp, Li, Lx, X, GX):
     """(GB, gL) for row-major (n, k) blocks: GB = L^-T GX per column,
-    gL[p] = -(sum over c ascending of GB[i,c] X[j,c])."""
+    gL[p] = -(pairwise sum over c of GB[i,c] X[j,c]) (pairwise_dot's order)."""
     n = len(Lp) - 1
     X = np.ascontiguousarray(X, np.float64)
     GX = np.ascontiguousarray(GX, np.float64)
@@ -319,6 +320,14 @@ def adjoint_block(Lp, Li, Lx, X, GX):
     olib().orc_adjoint_block(n, k, _p(_i32(Lp)), _p(_i32(Li)), _p(Cp), _p(Ci), _p(Cx),
                              _p(X), _p(GX), _p(GB), _p(gL))
     return GB, gL
+
+
+def pairwise_dot(g, x):
+    """The k products g[c]*x[c] added in the block adjoint's pairwise order:
+    groups of G columns (pow2_ceil(k) for k <= 32, else 32) reduced by
+    halving, group sums in ascending order."""
+    g, x = _f64(g), _f64(x)
+    return float(olib().orc_pairwise_dot(_p(g), _p(x), len(g)))
 
 
 def prune_set(Lp, Li):

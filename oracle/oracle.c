@@ -535,14 +535,35 @@ void orc_adjoint(int n, const int *Lp, const int *Li, const int *Cp,
     for (int p = Lp[i]; p < Lp[i + 1]; p++) gL[p] = -(gb[i] * x[Li[p]]);
 }
 
+/* The pairwise sum of the k products g[c]*x[c]: columns in groups of G
+ * (G = the smallest power of two >= k when k <= 32, else 32), a group's
+ * values s[0..G) (+0.0 past column k) reduced by halving -- for h = G/2,
+ * G/4, .., 1: s[c] = s[c] + s[c+h] for c < h -- and the group sums added in
+ * ascending group order.  The reference has no adjoint, so this order is
+ * the library's own choice (a tree fits a wavefront's lanes). */
+static double pairwise_dot(const double *g, const double *x, int k) {
+  int G = 1;
+  if (k > 32)
+    G = 32;
+  else
+    while (G < k) G <<= 1;
+  double acc = 0.0, s[32];
+  for (int g0 = 0; g0 < k; g0 += G) {
+    for (int c = 0; c < G; c++) s[c] = g0 + c < k ? g[g0 + c] * x[g0 + c] : 0.0;
+    for (int h = G / 2; h >= 1; h /= 2)
+      for (int c = 0; c < h; c++) s[c] = s[c] + s[c + h];
+    acc = g0 == 0 ? s[0] : acc + s[0];
+  }
+  return acc;
+}
+
 /* Adjoint of X = L^{-1} B for a row-major block of k right-hand sides
  * (element (i, c) at [i*k + c]): GB = L^{-T} GX column by column (ltsolve,
  * sptrsv_lt.cpp:11-20, on the CSC of L) and, for every stored entry
  * p = (i, j) of the CSR of L,
- *   gL[p] = -(GB[i,0]*X[j,0] + GB[i,1]*X[j,1] + ... + GB[i,k-1]*X[j,k-1])
- * with the products added in ascending c, left to right (the gradient of a
- * loss over the whole block).  The reference has no adjoint; for k = 1
- * this is orc_adjoint. */
+ *   gL[p] = -(pairwise sum over c of GB[i,c]*X[j,c])
+ * (the gradient of a loss over the whole block; pairwise_dot's order).  For
+ * k = 1 this is orc_adjoint. */
 void orc_adjoint_block(int n, int k, const int *Lp, const int *Li, const int *Cp,
                        const int *Ci, const double *Cx, const double *X,
                        const double *GX, double *GB, double *gL) {
@@ -554,12 +575,13 @@ void orc_adjoint_block(int n, int k, const int *Lp, const int *Li, const int *Cp
   }
   free(col);
   for (int i = 0; i < n; i++)
-    for (int p = Lp[i]; p < Lp[i + 1]; p++) {
-      const double *g = GB + (size_t)i * k, *x = X + (size_t)Li[p] * k;
-      double acc = g[0] * x[0];
-      for (int c = 1; c < k; c++) acc += g[c] * x[c];
-      gL[p] = -acc;
-    }
+    for (int p = Lp[i]; p < Lp[i + 1]; p++)
+      gL[p] = -pairwise_dot(GB + (size_t)i * k, X + (size_t)Li[p] * k, k);
+}
+
+/* k values g[c]*x[c] added in pairwise_dot's order (exposed for tests). */
+double orc_pairwise_dot(const double *g, const double *x, int k) {
+  return pairwise_dot(g, x, k);
 }
 
 /* CSR -> CSC transpose (stable: entries of a column in ascending row order,

@@ -1150,52 +1150,43 @@ RegionPlan *usable_plan(symamd_handle h, symamd_plan p, int k, bool transposed) 
   return fitted_plan(h, p, k, transposed);
 }
 
-// gL[e] = -(GB[i,0]*X[j,0] + ... + GB[i,k-1]*X[j,k-1]) for every entry
-// e = (i, j) of the caller's CSR, products added in ascending column (for
-// k = 1, the sync-free adjoint's -(gb[i] * x[j])); element (r, c) of X and
-// GB at r*rs + c*cs.  Narrow blocks: one thread per row in the caller's
-// order, GC columns at a time with all their loads in flight.
-template <typename T, int GC>
-__global__ void k_grad_l(int n, int k, int64_t rs, int64_t cs, const int *__restrict__ orp,
-                         const int *__restrict__ oci, const T *__restrict__ X,
-                         const T *__restrict__ GB, T *gL) {
+// gL[e] = -(sum over columns c of GB[i,c]*X[j,c]) for every entry e = (i, j)
+// of the caller's CSR (element (r, c) of X and GB at r*rs + c*cs), the k
+// products added pairwise: in groups of G columns (G = pow2_ceil(k) when
+// k <= 32, else 32; a group padded with +0.0 past column k), each group
+// reduced by halving -- s[c] += s[c + h] for c < h, h = G/2, G/4, .., 1 --
+// and the group sums added in ascending group order.  For k = 1 this is the
+// sync-free adjoint's -(gb[i] * x[j]).
+//
+// Single right-hand sides: one thread per row in the caller's order.
+template <typename T>
+__global__ void k_grad_l(int n, int64_t rs, const int *__restrict__ orp,
+                         const int *__restrict__ oci, const T *__restrict__ x,
+                         const T *__restrict__ gb, T *gL) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const T *g = GB + (int64_t)i * rs;
-  for (int e = orp[i]; e < orp[i + 1]; ++e) {
-    const T *x = X + (int64_t)oci[e] * rs;
-    T acc = T(0);
-    for (int c0 = 0; c0 < k; c0 += GC) {
-      T gv[GC], xv[GC];
-#pragma unroll
-      for (int u = 0; u < GC; ++u) {
-        const int64_t o = (int64_t)(c0 + u < k ? c0 + u : c0) * cs;
-        gv[u] = g[o];
-        xv[u] = x[o];
-      }
-#pragma unroll
-      for (int u = 0; u < GC; ++u)
-        if (c0 + u < k) acc = c0 + u == 0 ? gv[u] * xv[u] : acc + gv[u] * xv[u];
-    }
-    gL[e] = -acc;
-  }
+  const T g = gb[(int64_t)i * rs];
+  for (int e = orp[i]; e < orp[i + 1]; ++e) gL[e] = -(g * x[(int64_t)oci[e] * rs]);
 }
 
-// Wide blocks (8 <= k <= 256): LPR lanes take the columns of one row
-// (coalesced loads of GB's row and of each entry's X row), 64/LPR rows per
-// wave; the products of up to EB entries of each row go to LDS, then one
-// lane per entry adds them in ascending column -- the same sums as above.
-constexpr int GRAD_EB = 8;
-template <typename T, int LPR>
+// Blocks (k >= 2): a group of G = LPR * CPL columns (G = pow2_ceil(k) up to
+// 32) is spread over LPR lanes, lane l holding columns l, l + LPR, ..,
+// l + (CPL-1)*LPR; 64/LPR rows per wave, GRAD_EU entries of each row at a
+// time with all their X loads in flight together.  The halving steps of
+// distance h >= LPR pair two columns of one lane (c < h exactly when the
+// column's slot m < h/LPR); the steps below LPR pair lanes by butterfly:
+// after the step of distance h every lane holds s[l] + s[l ^ h], which for
+// l < h is the halving order's s[l] + s[l + h] (addition is commutative bit
+// for bit), so lane 0 of a row ends with the group's pairwise sum.
+constexpr int GRAD_EU = 4;
+template <typename T, int LPR, int CPL>
 __global__ __launch_bounds__(256) void k_grad_l_wide(int n, int k, int64_t rs, int64_t cs,
                                                      const int *__restrict__ orp,
                                                      const int *__restrict__ oci,
                                                      const T *__restrict__ X,
                                                      const T *__restrict__ GB, T *gL) {
-  constexpr int RPW = 64 / LPR, EB = GRAD_EB;
-  extern __shared__ __align__(16) uint8_t gsm[];
+  constexpr int RPW = 64 / LPR, EU = GRAD_EU, G = LPR * CPL;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  T *pw = reinterpret_cast<T *>(gsm) + (size_t)wid * RPW * EB * k;
   const int r = lane / LPR, cl = lane - r * LPR;
   const int64_t i64 = ((int64_t)blockIdx.x * 4 + wid) * RPW + r;
   const bool live = i64 < n;
@@ -1205,24 +1196,55 @@ __global__ __launch_bounds__(256) void k_grad_l_wide(int n, int k, int64_t rs, i
 #pragma unroll
   for (int m = LPR; m < 64; m <<= 1) mlen = max(mlen, __shfl_xor(mlen, m));
   const T *g = GB + (int64_t)i * rs;
-  for (int b = 0; b < mlen; b += EB) {
+  const int groups = (k + G - 1) / G;
+  for (int e = 0; e < mlen; e += EU) {
+    bool have[EU];
+    const T *xr[EU];
 #pragma unroll
-    for (int eb = 0; eb < EB; ++eb) {
-      const bool have = b + eb < len;
-      const T *x = X + (int64_t)(have ? oci[e0 + b + eb] : 0) * rs;
-      for (int c = cl; c < k; c += LPR)
-        pw[(r * EB + eb) * k + c] = have ? g[(int64_t)c * cs] * x[(int64_t)c * cs] : T(0);
+    for (int u = 0; u < EU; ++u) {
+      have[u] = e + u < len;
+      xr[u] = X + (int64_t)(have[u] ? oci[e0 + e + u] : 0) * rs;
     }
-    __builtin_amdgcn_wave_barrier();
-    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-    if (cl < EB && b + cl < len) {
-      const T *q = pw + (r * EB + cl) * k;
-      T acc = q[0];
-      for (int c = 1; c < k; ++c) acc += q[c];
-      gL[e0 + b + cl] = -acc;
+    T acc[EU];
+    for (int grp = 0; grp < groups; ++grp) {
+      bool cin[CPL];
+      int64_t o[CPL];
+      T gv[CPL], sv[EU][CPL];
+#pragma unroll
+      for (int m = 0; m < CPL; ++m) {
+        const int c = grp * G + m * LPR + cl;
+        cin[m] = c < k;
+        o[m] = (int64_t)(cin[m] ? c : 0) * cs;
+        gv[m] = g[o[m]];
+      }
+#pragma unroll
+      for (int u = 0; u < EU; ++u)
+#pragma unroll
+        for (int m = 0; m < CPL; ++m) sv[u][m] = xr[u][o[m]];
+#pragma unroll
+      for (int u = 0; u < EU; ++u)
+#pragma unroll
+        for (int m = 0; m < CPL; ++m) sv[u][m] = cin[m] ? gv[m] * sv[u][m] : T(0);
+      // halving steps inside a lane (distance h = hm * LPR)
+#pragma unroll
+      for (int hm = CPL / 2; hm >= 1; hm >>= 1)
+#pragma unroll
+        for (int u = 0; u < EU; ++u)
+#pragma unroll
+          for (int m = 0; m < hm; ++m) sv[u][m] = sv[u][m] + sv[u][m + hm];
+      // then across the row's lanes
+#pragma unroll
+      for (int h = LPR / 2; h >= 1; h >>= 1)
+#pragma unroll
+        for (int u = 0; u < EU; ++u) sv[u][0] = sv[u][0] + __shfl_xor(sv[u][0], h);
+#pragma unroll
+      for (int u = 0; u < EU; ++u) acc[u] = grp == 0 ? sv[u][0] : acc[u] + sv[u][0];
     }
-    __builtin_amdgcn_wave_barrier();
-    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    if (cl == 0) {
+#pragma unroll
+      for (int u = 0; u < EU; ++u)
+        if (have[u]) gL[e0 + e + u] = -acc[u];
+    }
   }
 }
 
@@ -1230,23 +1252,23 @@ template <typename T>
 int grad_dispatch(symamd_handle h, symamd_plan p, const T *X, const T *GB, T *gL, int k,
                   int64_t rs, int64_t cs) {
   const int n = p->n;
-  if (k >= 8 && k <= 256) {
-    const int lpr = k >= 64 ? 64 : pow2_ceil(k);
+  if (k == 1) {
+    const unsigned blocks = (unsigned)((n + 255) / 256);
+    k_grad_l<T><<<blocks, 256, 0, h->stream>>>(n, rs, p->d_orp, p->d_oci, X, GB, gL);
+  } else {
+    // the group width G = pow2_ceil(k) up to 32, four columns per lane
+    // (two for G = 2)
+    const int G = k >= 32 ? 32 : pow2_ceil(k);
+    const int lpr = G >= 4 ? G / 4 : 1;
     const int rows_per_block = 4 * (64 / lpr);
     const unsigned blocks = (unsigned)((n + rows_per_block - 1) / rows_per_block);
-    const size_t lds = (size_t)4 * (64 / lpr) * GRAD_EB * k * sizeof(T);
-    switch (lpr) {
-      case 8: k_grad_l_wide<T, 8><<<blocks, 256, lds, h->stream>>>(n, k, rs, cs, p->d_orp, p->d_oci, X, GB, gL); break;
-      case 16: k_grad_l_wide<T, 16><<<blocks, 256, lds, h->stream>>>(n, k, rs, cs, p->d_orp, p->d_oci, X, GB, gL); break;
-      case 32: k_grad_l_wide<T, 32><<<blocks, 256, lds, h->stream>>>(n, k, rs, cs, p->d_orp, p->d_oci, X, GB, gL); break;
-      default: k_grad_l_wide<T, 64><<<blocks, 256, lds, h->stream>>>(n, k, rs, cs, p->d_orp, p->d_oci, X, GB, gL); break;
+    switch (G) {
+      case 2: k_grad_l_wide<T, 1, 2><<<blocks, 256, 0, h->stream>>>(n, k, rs, cs, p->d_orp, p->d_oci, X, GB, gL); break;
+      case 4: k_grad_l_wide<T, 1, 4><<<blocks, 256, 0, h->stream>>>(n, k, rs, cs, p->d_orp, p->d_oci, X, GB, gL); break;
+      case 8: k_grad_l_wide<T, 2, 4><<<blocks, 256, 0, h->stream>>>(n, k, rs, cs, p->d_orp, p->d_oci, X, GB, gL); break;
+      case 16: k_grad_l_wide<T, 4, 4><<<blocks, 256, 0, h->stream>>>(n, k, rs, cs, p->d_orp, p->d_oci, X, GB, gL); break;
+      default: k_grad_l_wide<T, 8, 4><<<blocks, 256, 0, h->stream>>>(n, k, rs, cs, p->d_orp, p->d_oci, X, GB, gL); break;
     }
-  } else {
-    const unsigned blocks = (unsigned)((n + 255) / 256);
-    if (k == 1)
-      k_grad_l<T, 1><<<blocks, 256, 0, h->stream>>>(n, k, rs, cs, p->d_orp, p->d_oci, X, GB, gL);
-    else
-      k_grad_l<T, 8><<<blocks, 256, 0, h->stream>>>(n, k, rs, cs, p->d_orp, p->d_oci, X, GB, gL);
   }
   SYMAMD_HIP(hipGetLastError());
   return SYMAMD_OK;

@@ -239,11 +239,14 @@ def test_region_cap_halved_to_fit(dev, k):
 @pytest.mark.parametrize("k,layout,executor", [(1, "row", "regions"), (8, "row", "regions"),
                                                (13, "row", "auto"), (32, "row", "regions"),
                                                (8, "col", "regions"), (5, "col", "syncfree"),
-                                               (3, "row", "syncfree")])
+                                               (3, "row", "syncfree"), (2, "row", "syncfree"),
+                                               (40, "row", "regions"), (64, "col", "regions"),
+                                               (70, "row", "auto")])
 def test_block_adjoint_bitwise(dev, k, layout, executor):
     # GB = L^-T GX for a block of k columns (region schedule in one launch,
     # or the sync-free adjoint column by column) and gL summed over the
-    # columns in ascending order: bitwise the oracle, on two calls
+    # columns pairwise (groups of up to 32 columns, padded past k): bitwise
+    # the oracle, on two calls
     import torch
     for Lp, Li, Lx in (W.ic0_lap3d(20, 0.01), W.random_lower(6000, 0.002, 1, seed=51)):
         n = len(Lp) - 1

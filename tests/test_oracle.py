@@ -435,13 +435,13 @@ def test_csr_csc_round_trip():
 
 
 def test_adjoint_block_pinned_by_single_columns_and_dense():
-    # block of k right-hand sides: GB equals the single-column adjoints, and
-    # gL equals the running sum (ascending column) of their gL -- exact,
-    # since (-a) + (-b) = -(a + b) in round-to-nearest -- and the dense FP64
-    # gradient of 0.5 ||L^-1 B||_F^2
+    # block of k right-hand sides: GB equals the single-column adjoints; for
+    # k <= 2 the pairwise sum is the running sum, so gL equals the sum of the
+    # single-column gL exactly ((-a) + (-b) = -(a + b) in round-to-nearest);
+    # and the dense FP64 gradient of 0.5 ||L^-1 B||_F^2
     Lp, Li, Lx = W.ic0_lap3d(12)
     n = len(Lp) - 1
-    for k in (1, 3, 8):
+    for k in (1, 2, 3, 8):
         B = W.uniform(30 + k, 0, n * k).reshape(n, k)
         X = np.stack([oracle.sptrsv_csr(Lp, Li, Lx, B[:, c]) for c in range(k)], axis=1)
         GX = W.uniform(40 + k, 0, n * k).reshape(n, k)
@@ -451,7 +451,10 @@ def test_adjoint_block_pinned_by_single_columns_and_dense():
             gb, gl = oracle.adjoint(Lp, Li, Lx, X[:, c].copy(), GX[:, c].copy())
             assert bitwise_equal(GB[:, c], gb)
             acc = gl if acc is None else acc + gl
-        assert bitwise_equal(gL, acc)
+        if k <= 2:
+            assert bitwise_equal(gL, acc)
+        else:
+            assert np.max(np.abs(gL - acc)) <= 1e-14 * np.max(np.abs(acc))
     # dense gradient of the Frobenius loss, k = 4
     k = 4
     B = W.uniform(50, 0, n * k).reshape(n, k)
@@ -464,3 +467,42 @@ def test_adjoint_block_pinned_by_single_columns_and_dense():
     assert np.max(np.abs(GB - GBd)) / np.max(np.abs(GBd)) < 1e-12
     gLd = -(GBd @ Xd.T)[rows, Li]
     assert np.max(np.abs(gL - gLd)) / np.max(np.abs(gLd)) < 1e-12
+
+
+def test_adjoint_block_exact_on_integers():
+    # small integers: every product and partial sum is exact in any order,
+    # so gL is the integer dot product of GB's row i and X's row j for every
+    # entry (i, j) -- a wrong operand, row or transposition shows exactly
+    Lp, Li, Lx = W.ic0_lap3d(8)
+    n = len(Lp) - 1
+    rng = np.random.default_rng(5)
+    rows = np.repeat(np.arange(n), np.diff(Lp))
+    for k in (3, 32, 40, 70):
+        X = rng.integers(-4, 5, (n, k)).astype(np.float64)
+        GX = rng.integers(-4, 5, (n, k)).astype(np.float64)
+        GB, gL = oracle.adjoint_block(Lp, Li, Lx, X, GX)
+        Gi = rng.integers(-4, 5, (n, k)).astype(np.float64)  # integer stand-in for GB
+        gLi = np.array([-oracle.pairwise_dot(Gi[i], X[j]) for i, j in zip(rows, Li)])
+        want = -np.einsum("ec,ec->e", Gi[rows].astype(np.int64), X[Li].astype(np.int64))
+        assert np.array_equal(gLi, want.astype(np.float64)), k
+        # and the block adjoint applies that sum to its own GB rows
+        got = np.array([-oracle.pairwise_dot(GB[i], X[j]) for i, j in zip(rows[:50], Li[:50])])
+        assert bitwise_equal(gL[:50], got)
+
+
+def test_pairwise_order_known_answers():
+    # hand-worked sums whose value depends on the order (u = 2^-53, half an
+    # ulp of 1: 1 + u rounds to 1, u + u = 2u and 1 + 2u are exact)
+    u = 2.0 ** -53
+    one = lambda k: np.ones(k)
+    # k = 4, one tree of four: (1 + -1) + (u + u) = 2u; left to right gives u
+    assert oracle.pairwise_dot(one(4), np.array([1.0, u, -1.0, u])) == 2 * u
+    # k = 3 pads to four: (u + u) + (1 + 0) = 1 + 2u; left to right gives 1
+    assert oracle.pairwise_dot(one(3), np.array([u, 1.0, u])) == 1.0 + 2 * u
+    # k = 40: groups of 32 -- group 0 sums to 1, group 1 (columns 32, 33) to
+    # 2u, total 1 + 2u; one tree of 64 would pair column 0 with 32 (1 + u = 1)
+    x = np.zeros(40)
+    x[0], x[32], x[33] = 1.0, u, u
+    assert oracle.pairwise_dot(one(40), x) == 1.0 + 2 * u
+    # k = 2: the single addition, and the products are g[c] * x[c]
+    assert oracle.pairwise_dot(np.array([3.0, 0.5]), np.array([2.0, 4.0])) == 8.0
