@@ -26,6 +26,39 @@ void spmv_csr(int n, const int *Ap, const int *Ai, const double *Ax,
 /* sparse_blas_lib.h:32-33 */
 void spmv_csr(int n, const int *Ap, const int *Ai, const double *Ax,
               const double *x, const double *y, double *z, double a, double b);
+/* sparse_blas_lib.h:35-37: the same products as the 6-argument spmv_csr. */
+void spmv_csr_parallel(int n, const int *Ap, const int *Ai, const double *Ax,
+                       const double *x, double *y);
+/* sparse_blas_lib.h:39-41: y[j] = row j of A times x for the rows of the
+ * supernodes (rows supernodes[0] .. supernodes[nodes]-1), each from +0.0. */
+void spmv_csr_block(int n, const int *Ap, const int *Ai, const double *Ax, int nodes,
+                    const int *supernodes, const double *x, double *y);
+/* sparse_blas_lib.h:50-52: y += A x for a CSC matrix (n columns); y has as
+ * many entries as A has rows (the largest row index + 1). */
+void spmv_csc(int n, const int *Ap, const int *Ai, const double *Ax, const double *x,
+              double *y);
+/* sparse_blas_lib.h:64-65: z += A x, then z = a*z + b*y (square A). */
+void spmv_csc(int n, const int *Ap, const int *Ai, const double *Ax, const double *x,
+              const double *y, double *z, double a, double b);
+/* sparse_blas_lib.h:67-69: the same with per-row a[i], b[i]. */
+void spmv_csc(int n, const int *Ap, const int *Ai, const double *Ax, const double *x,
+              const double *y, double *z, double *a, double *b);
+/* sparse_blas_lib.h:82-85: part l takes the columns ids[sets[l] ..
+ * sets[l+1]), its products summed from +0.0, and y (n entries, square A) +=
+ * the parts' sums in part order -- bit-identical to the reference. */
+void spmv_csc_lbc(int n_parts, const int *sets, const int *ids, int n, const int *Ap,
+                  const int *Ai, const double *Ax, const double *x, double *y);
+/* sparse_blas_lib.h:86-89: the same with n_parts = set_ptr[1]. */
+void spmv_csc_lbc(const int *set_ptr, const int *sets, const int *ids, int n, const int *Ap,
+                  const int *Ai, const double *Ax, const double *x, double *y);
+/* sparse_blas_lib.h:98-99: the reference adds with atomics in thread order;
+ * here y is spmv_csc's (column order), equal to the reference's to rounding. */
+void spmv_csc_parallel(int n, const int *Ap, const int *Ai, const double *Ax,
+                       const double *x, double *y);
+/* sparse_blas_lib.h:111-114: y += the columns supernodes[0] ..
+ * supernodes[nodes]-1 of A times x, column by column. */
+void spmv_csc_block(int n, const int *Ap, const int *Ai, const double *Ax, const double *x,
+                    double *y, int nodes, const int *supernodes);
 
 /* sparse_blas_lib.h:126-127 */
 void sptrsv_csr(int n, int *Lp, int *Li, double *Lx, double *x);

@@ -128,8 +128,9 @@ int symamd_set_timeout_ms(symamd_handle h, double ms);
  *     whose right-hand sides are loaded directly (row-major blocks, single
  *     vectors) run with one extra wavefront per task that does all of its
  *     waiting, loading, storing and publishing, so the solving wavefronts
- *     never wait for memory; the default band is then 2 slices.  0 = every
- *     wavefront of a task does both (default band: one per region).
+ *     never wait for memory; the default band is then 4 slices for single
+ *     right-hand sides.  0 = every wavefront of a task does both (default
+ *     band: one per region).
  *     Results are identical either way. */
 enum { SYMAMD_OPT_TICKETS = 1, SYMAMD_OPT_TRACE_PTR = 2, SYMAMD_OPT_TRACE_CAP = 3,
        SYMAMD_OPT_MAX_WAVES = 4, SYMAMD_OPT_COLMAJOR_DIRECT = 5, SYMAMD_OPT_MARKS = 6,
@@ -344,6 +345,29 @@ int symamd_spmv_csr_axpby(symamd_handle h, int n, const int *d_Ap,
                           const int *d_Ai, const double *d_Ax,
                           const double *d_x, const double *d_y, double *d_z,
                           double a, double b);
+/* y += A x over the columns c0 .. c1-1 of a CSC matrix A (n columns, m rows:
+ * column c's rows Ai[Ap[c] .. Ap[c+1]), values Ax), each y[r] receiving its
+ * products in ascending column order -- the order of the reference's
+ * column-by-column scatter, so y is bit-identical to sym_lib::spmv_csc
+ * (sparse_blas/spmv.cpp:73-81; c0 = 0, c1 = n) and spmv_csc_block (:171-186;
+ * the supernodes' column range).  FP64, blocking.  SYMAMD_ERR_STRUCTURE for a
+ * row index outside [0, m). */
+int symamd_spmv_csc(symamd_handle h, int n, int m, const int *d_Ap, const int *d_Ai,
+                    const double *d_Ax, const double *d_x, double *d_y, int c0, int c1);
+/* The LBC form of the same product (sym_lib::spmv_csc_lbc, spmv.cpp:122-167):
+ * part l takes the columns ids[sets[l] .. sets[l+1]) (d_sets: nparts + 1
+ * nondecreasing bounds, d_ids: column ids, device arrays); each part's
+ * products are summed from +0.0 in that order, then y[r] += the parts' sums in
+ * part order (every part, as the reference's reduction does).  Bit-identical
+ * to the reference's result.  FP64, blocking. */
+int symamd_spmv_csc_parts(symamd_handle h, int n, int m, const int *d_Ap, const int *d_Ai,
+                          const double *d_Ax, const double *d_x, double *d_y, int nparts,
+                          const int *d_sets, const int *d_ids);
+/* z[i] = a_i * z[i] + b_i * y[i] for i < n, a_i = d_a[i] (or a when d_a is
+ * NULL), b_i likewise: the finishing step of the reference's axpby forms of
+ * spmv_csc (spmv.cpp:85-109).  FP64, on the stream. */
+int symamd_axpby(symamd_handle h, int n, const double *d_a, const double *d_b, double a,
+                 double b, const double *d_y, double *d_z);
 /* Relative residual ||L X - B||_2 / ||B||_2 of every column c < k, written to
  * h_rel[c], using the plan's copy of L (blocking). */
 int symamd_residual(symamd_handle h, symamd_plan plan, const void *d_X,

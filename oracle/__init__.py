@@ -17,6 +17,10 @@ Functions and the reference lines they follow:
   sptrsv_csc           sparse_blas/sptrsv.cpp:129-137
   ltsolve              sparse_blas/sptrsv_lt.cpp:11-20
   spmv_csr / _axpby    sparse_blas/spmv.cpp:12-21 / :23-34
+  spmv_csr_block       sparse_blas/spmv.cpp:59-71
+  spmv_csc / _axpby    sparse_blas/spmv.cpp:73-81 / :85-109
+  spmv_csc_lbc         sparse_blas/spmv.cpp:122-143
+  spmv_csc_block       sparse_blas/spmv.cpp:171-186
   spic0_csc_left       sparse_blas/spic0.cpp:11-46
   spic0_csr            sparse_blas/spic0.cpp:106-127
   spilu0_csr           sparse_blas/spilu0.cpp:11-51
@@ -67,6 +71,11 @@ def olib():
             "orc_sptrsm_csr": ([i, vp, vp, vp, vp, i, i64, i64], None),
             "orc_spmv_csr": ([i, vp, vp, vp, vp, vp], None),
             "orc_spmv_csr_axpby": ([i, vp, vp, vp, vp, vp, vp, d, d], None),
+            "orc_spmv_csr_block": ([i, vp, vp, vp, i, vp, vp, vp], None),
+            "orc_spmv_csc": ([i, vp, vp, vp, vp, vp], None),
+            "orc_spmv_csc_axpby": ([i, vp, vp, vp, vp, vp, vp, d, d, vp, vp], None),
+            "orc_spmv_csc_lbc": ([i, vp, vp, i, vp, vp, vp, vp, vp], None),
+            "orc_spmv_csc_block": ([i, vp, vp, vp, vp, vp, i, vp], None),
             "orc_spic0_csc_left": ([i, vp, vp, vp, vp, vp, vp], c_int),
             "orc_spic0_csr": ([i, vp, vp, vp], c_int),
             "orc_spilu0_csr": ([i, i, vp, vp, vp, vp, vp], c_int),
@@ -114,6 +123,12 @@ def rlib():
             "ref_ltsolve": ([i, vp, vp, vp, vp], c_int),
             "ref_spmv_csr": ([i, vp, vp, vp, vp, vp], None),
             "ref_spmv_csr_axpby": ([i, vp, vp, vp, vp, vp, vp, d, d], None),
+            "ref_spmv_csr_block": ([i, vp, vp, vp, i, vp, vp, vp], None),
+            "ref_spmv_csc": ([i, vp, vp, vp, vp, vp], None),
+            "ref_spmv_csc_axpby": ([i, vp, vp, vp, vp, vp, vp, d, d], None),
+            "ref_spmv_csc_axpby_vec": ([i, vp, vp, vp, vp, vp, vp, vp, vp], None),
+            "ref_spmv_csc_lbc": ([i, vp, vp, i, vp, vp, vp, vp, vp], None),
+            "ref_spmv_csc_block": ([i, vp, vp, vp, vp, vp, i, vp], None),
             "ref_spic0_csc_left": ([i, vp, vp, vp, vp, vp, vp], c_int),
             "ref_spic0_csr": ([i, vp, vp, vp], None),
             "ref_getLevelSet": ([i, vp, vp, vp], c_int),
@@ -220,6 +235,49 @@ def spmv_csr_axpby(Ap, Ai, Ax, x, y, z, a, b):
     olib().orc_spmv_csr_axpby(len(Ap) - 1, _p(_i32(Ap)), _p(_i32(Ai)), _p(_f64(Ax)),
                               _p(_f64(x)), _p(_f64(y)), _p(z), a, b)
     return z
+
+
+def spmv_csr_block(Ap, Ai, Ax, supernodes, x, y):
+    """spmv_csr_block: y (copied) with the supernodes' rows set to A x."""
+    y = _f64(y).copy()
+    olib().orc_spmv_csr_block(len(Ap) - 1, _p(_i32(Ap)), _p(_i32(Ai)), _p(_f64(Ax)),
+                              len(supernodes) - 1, _p(_i32(supernodes)), _p(_f64(x)), _p(y))
+    return y
+
+
+def spmv_csc(Cp, Ci, Cx, x, y):
+    """spmv_csc: y (copied) += A x, A in CSC with len(Cp) - 1 columns."""
+    y = _f64(y).copy()
+    olib().orc_spmv_csc(len(Cp) - 1, _p(_i32(Cp)), _p(_i32(Ci)), _p(_f64(Cx)), _p(_f64(x)),
+                        _p(y))
+    return y
+
+
+def spmv_csc_axpby(Cp, Ci, Cx, x, y, z, a, b):
+    """spmv_csc's axpby forms: z (copied) += A x, then a z + b y; a and b
+    scalars or per-row arrays."""
+    z = _f64(z).copy()
+    vec = not np.isscalar(a)
+    av, bv = (_f64(a), _f64(b)) if vec else (None, None)
+    olib().orc_spmv_csc_axpby(len(Cp) - 1, _p(_i32(Cp)), _p(_i32(Ci)), _p(_f64(Cx)),
+                              _p(_f64(x)), _p(_f64(y)), _p(z), 0.0 if vec else float(a),
+                              0.0 if vec else float(b), _p(av) if vec else None,
+                              _p(bv) if vec else None)
+    return z
+
+
+def spmv_csc_lbc(sets, ids, Cp, Ci, Cx, x, y):
+    y = _f64(y).copy()
+    olib().orc_spmv_csc_lbc(len(sets) - 1, _p(_i32(sets)), _p(_i32(ids)), len(Cp) - 1,
+                            _p(_i32(Cp)), _p(_i32(Ci)), _p(_f64(Cx)), _p(_f64(x)), _p(y))
+    return y
+
+
+def spmv_csc_block(Cp, Ci, Cx, x, y, supernodes):
+    y = _f64(y).copy()
+    olib().orc_spmv_csc_block(len(Cp) - 1, _p(_i32(Cp)), _p(_i32(Ci)), _p(_f64(Cx)),
+                              _p(_f64(x)), _p(y), len(supernodes) - 1, _p(_i32(supernodes)))
+    return y
 
 
 def spic0_csc_left(Ap, Ai, Ax, prunePtr, pruneSet):
@@ -424,6 +482,48 @@ class ref:
         rlib().ref_spmv_csr_axpby(len(Ap) - 1, _p(_i32(Ap)), _p(_i32(Ai)), _p(_f64(Ax)),
                                   _p(_f64(x)), _p(_f64(y)), _p(z), a, b)
         return z
+
+    @staticmethod
+    def spmv_csr_block(Ap, Ai, Ax, supernodes, x, y):
+        y = _f64(y).copy()
+        rlib().ref_spmv_csr_block(len(Ap) - 1, _p(_i32(Ap)), _p(_i32(Ai)), _p(_f64(Ax)),
+                                  len(supernodes) - 1, _p(_i32(supernodes)), _p(_f64(x)),
+                                  _p(y))
+        return y
+
+    @staticmethod
+    def spmv_csc(Cp, Ci, Cx, x, y):
+        y = _f64(y).copy()
+        rlib().ref_spmv_csc(len(Cp) - 1, _p(_i32(Cp)), _p(_i32(Ci)), _p(_f64(Cx)),
+                            _p(_f64(x)), _p(y))
+        return y
+
+    @staticmethod
+    def spmv_csc_axpby(Cp, Ci, Cx, x, y, z, a, b):
+        z = _f64(z).copy()
+        if np.isscalar(a):
+            rlib().ref_spmv_csc_axpby(len(Cp) - 1, _p(_i32(Cp)), _p(_i32(Ci)), _p(_f64(Cx)),
+                                      _p(_f64(x)), _p(_f64(y)), _p(z), float(a), float(b))
+        else:
+            rlib().ref_spmv_csc_axpby_vec(len(Cp) - 1, _p(_i32(Cp)), _p(_i32(Ci)),
+                                          _p(_f64(Cx)), _p(_f64(x)), _p(_f64(y)), _p(z),
+                                          _p(_f64(a).copy()), _p(_f64(b).copy()))
+        return z
+
+    @staticmethod
+    def spmv_csc_lbc(sets, ids, Cp, Ci, Cx, x, y):
+        y = _f64(y).copy()
+        rlib().ref_spmv_csc_lbc(len(sets) - 1, _p(_i32(sets)), _p(_i32(ids)), len(Cp) - 1,
+                                _p(_i32(Cp)), _p(_i32(Ci)), _p(_f64(Cx)), _p(_f64(x)), _p(y))
+        return y
+
+    @staticmethod
+    def spmv_csc_block(Cp, Ci, Cx, x, y, supernodes):
+        y = _f64(y).copy()
+        rlib().ref_spmv_csc_block(len(Cp) - 1, _p(_i32(Cp)), _p(_i32(Ci)), _p(_f64(Cx)),
+                                  _p(_f64(x)), _p(y), len(supernodes) - 1,
+                                  _p(_i32(supernodes)))
+        return y
 
     @staticmethod
     def spic0_csc_left(Ap, Ai, Ax, prunePtr, pruneSet):

@@ -204,6 +204,65 @@ void orc_spmv_csr_axpby(int n, const int *Ap, const int *Ai, const double *Ax,
   }
 }
 
+/* sparse_blas/spmv.cpp:59-71 (sym_lib::spmv_csr_block): the rows of the
+ * supernodes, each y[j] from 0 then += entries ascending. */
+void orc_spmv_csr_block(int n, const int *Ap, const int *Ai, const double *Ax, int nodes,
+                        const int *supernodes, const double *x, double *y) {
+  (void)n;
+  for (int i = 0; i < nodes; i++)
+    for (int j = supernodes[i]; j < supernodes[i + 1]; j++) {
+      y[j] = 0;
+      for (int k = Ap[j]; k < Ap[j + 1]; k++) y[j] += Ax[k] * x[Ai[k]];
+    }
+}
+
+/* sparse_blas/spmv.cpp:73-81 (sym_lib::spmv_csc, 6 args): y[Ai[j]] +=
+ * Ax[j] * x[i], column by column. */
+void orc_spmv_csc(int n, const int *Ap, const int *Ai, const double *Ax, const double *x,
+                  double *y) {
+  for (int i = 0; i < n; i++)
+    for (int j = Ap[i]; j < Ap[i + 1]; j++) y[Ai[j]] += Ax[j] * x[i];
+}
+
+/* sparse_blas/spmv.cpp:85-95 and :97-109 (sym_lib::spmv_csc with scalars a, b
+ * or per-row arrays av, bv): z += A x column by column, then
+ * z[i] = a z[i] + b y[i]. */
+void orc_spmv_csc_axpby(int n, const int *Ap, const int *Ai, const double *Ax,
+                        const double *x, const double *y, double *z, double a, double b,
+                        const double *av, const double *bv) {
+  for (int i = 0; i < n; i++)
+    for (int j = Ap[i]; j < Ap[i + 1]; j++) z[Ai[j]] += Ax[j] * x[i];
+  for (int i = 0; i < n; i++)
+    z[i] = (av ? av[i] : a) * z[i] + (bv ? bv[i] : b) * y[i];
+}
+
+/* sparse_blas/spmv.cpp:122-143 (sym_lib::spmv_csc_lbc): part l scatters the
+ * columns ids[sets[l] .. sets[l+1]) into its own zeroed buffer, then y[i] +=
+ * buffer_0[i], buffer_1[i], ... in part order (every part, every row). */
+void orc_spmv_csc_lbc(int n_parts, const int *sets, const int *ids, int n, const int *Ap,
+                      const int *Ai, const double *Ax, const double *x, double *y) {
+  double *red = (double *)calloc((size_t)(n_parts > 0 ? n_parts : 1) * (size_t)(n > 0 ? n : 1),
+                                 sizeof(double));
+  for (int l = 0; l < n_parts; l++)
+    for (int k = sets[l]; k < sets[l + 1]; ++k) {
+      const int i = ids[k];
+      for (int j = Ap[i]; j < Ap[i + 1]; j++) red[(size_t)l * n + Ai[j]] += Ax[j] * x[i];
+    }
+  for (int m = 0; m < n_parts; ++m)
+    for (int i = 0; i < n; ++i) y[i] += red[(size_t)m * n + i];
+  free(red);
+}
+
+/* sparse_blas/spmv.cpp:171-186 (sym_lib::spmv_csc_block): the columns of the
+ * supernodes, column by column, scattered into y. */
+void orc_spmv_csc_block(int n, const int *Ap, const int *Ai, const double *Ax,
+                        const double *x, double *y, int nodes, const int *supernodes) {
+  (void)n;
+  for (int i = 0; i < nodes; i++)
+    for (int j = supernodes[i]; j < supernodes[i + 1]; j++)
+      for (int k = Ap[j]; k < Ap[j + 1]; k++) y[Ai[k]] += Ax[k] * x[j];
+}
+
 /* ---------------------------------------------------------------- IC(0) */
 
 /* sparse_blas/spic0.cpp:11-46 (sym_lib::spic0_csc_left): left-looking IC(0)

@@ -47,6 +47,18 @@ void spmv_csr(int n, const int *Ap, const int *Ai, const double *Ax,
 void spmv_csr(int n, const int *Ap, const int *Ai, const double *Ax,
               const double *x, const double *y, double *z, double a,
               double b);  // :32-33
+void spmv_csr_block(int n, const int *Ap, const int *Ai, const double *Ax, int nodes,
+                    const int *supernodes, const double *x, double *y);  // :39-41
+void spmv_csc(int n, const int *Ap, const int *Ai, const double *Ax, const double *x,
+              double *y);  // :50-52
+void spmv_csc(int n, const int *Ap, const int *Ai, const double *Ax, const double *x,
+              const double *y, double *z, double a, double b);  // :64-65
+void spmv_csc(int n, const int *Ap, const int *Ai, const double *Ax, const double *x,
+              const double *y, double *z, double *a, double *b);  // :67-69
+void spmv_csc_lbc(int n_parts, const int *sets, const int *ids, int n, const int *Ap,
+                  const int *Ai, const double *Ax, const double *x, double *y);  // :82-85
+void spmv_csc_block(int n, const int *Ap, const int *Ai, const double *Ax, const double *x,
+                    double *y, int nodes, const int *supernodes);  // :111-114
 bool spic0_csc_left(int n, const int *Ap, const int *Ai, const double *Ax,
                     const int *prunePtr, const int *pruneSet,
                     double *Lx);  // :256-259
@@ -124,6 +136,31 @@ void ref_spmv_csr_axpby(int n, const int *Ap, const int *Ai, const double *Ax,
                         const double *x, const double *y, double *z, double a,
                         double b) {
   sym_lib::spmv_csr(n, Ap, Ai, Ax, x, y, z, a, b);
+}
+void ref_spmv_csr_block(int n, const int *Ap, const int *Ai, const double *Ax, int nodes,
+                        const int *supernodes, const double *x, double *y) {
+  sym_lib::spmv_csr_block(n, Ap, Ai, Ax, nodes, supernodes, x, y);
+}
+void ref_spmv_csc(int n, const int *Ap, const int *Ai, const double *Ax, const double *x,
+                  double *y) {
+  sym_lib::spmv_csc(n, Ap, Ai, Ax, x, y);
+}
+void ref_spmv_csc_axpby(int n, const int *Ap, const int *Ai, const double *Ax,
+                        const double *x, const double *y, double *z, double a, double b) {
+  sym_lib::spmv_csc(n, Ap, Ai, Ax, x, y, z, a, b);
+}
+void ref_spmv_csc_axpby_vec(int n, const int *Ap, const int *Ai, const double *Ax,
+                            const double *x, const double *y, double *z, double *a,
+                            double *b) {
+  sym_lib::spmv_csc(n, Ap, Ai, Ax, x, y, z, a, b);
+}
+void ref_spmv_csc_lbc(int n_parts, const int *sets, const int *ids, int n, const int *Ap,
+                      const int *Ai, const double *Ax, const double *x, double *y) {
+  sym_lib::spmv_csc_lbc(n_parts, sets, ids, n, Ap, Ai, Ax, x, y);
+}
+void ref_spmv_csc_block(int n, const int *Ap, const int *Ai, const double *Ax,
+                        const double *x, double *y, int nodes, const int *supernodes) {
+  sym_lib::spmv_csc_block(n, Ap, Ai, Ax, x, y, nodes, supernodes);
 }
 int ref_spic0_csc_left(int n, const int *Ap, const int *Ai, const double *Ax,
                        const int *prunePtr, const int *pruneSet, double *Lx) {

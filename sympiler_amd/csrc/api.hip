@@ -495,6 +495,42 @@ int symamd_levelset_noalloc(symamd_handle h, int n, const int *d_Lp, const int *
   return levelset_noalloc(h, n, d_Lp, d_Li, d_level_ptr, d_level_set, levels);
 }
 
+int symamd_spmv_csc(symamd_handle h, int n, int m, const int *d_Ap, const int *d_Ai,
+                    const double *d_Ax, const double *d_x, double *d_y, int c0, int c1) {
+  SYMAMD_TRY(check_handle(h));
+  if (n < 0 || m < 0 || c0 < 0 || c1 < c0 || c1 > n ||
+      (c1 > c0 && (!d_Ap || !d_Ai || !d_Ax || !d_x)) || (m > 0 && !d_y))
+    return fail(SYMAMD_ERR_INVALID_ARG, "symamd_spmv_csc: bad arguments");
+  if (c1 == c0) return SYMAMD_OK;
+  return spmv_pull(h, n, m, d_Ap, d_Ai, d_Ax, d_x, d_y, nullptr, 0, c0, c1 - c0, 0, nullptr);
+}
+
+int symamd_spmv_csc_parts(symamd_handle h, int n, int m, const int *d_Ap, const int *d_Ai,
+                          const double *d_Ax, const double *d_x, double *d_y, int nparts,
+                          const int *d_sets, const int *d_ids) {
+  SYMAMD_TRY(check_handle(h));
+  if (n < 0 || m < 0 || nparts < 0 || (nparts > 0 && !d_sets) || (m > 0 && !d_y))
+    return fail(SYMAMD_ERR_INVALID_ARG, "symamd_spmv_csc_parts: bad arguments");
+  if (nparts == 0) return SYMAMD_OK;
+  int ends[2] = {0, 0};
+  SYMAMD_HIP(hipMemcpyAsync(&ends[0], d_sets, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  SYMAMD_HIP(hipMemcpyAsync(&ends[1], d_sets + nparts, sizeof(int), hipMemcpyDeviceToHost,
+                            h->stream));
+  SYMAMD_HIP(hipStreamSynchronize(h->stream));
+  const int k0 = ends[0], V = ends[1] - ends[0];
+  if (k0 < 0 || V < 0 || (V > 0 && (!d_ids || !d_Ap || !d_Ai || !d_Ax || !d_x)))
+    return fail(SYMAMD_ERR_INVALID_ARG, "symamd_spmv_csc_parts: bad part bounds or arrays");
+  return spmv_pull(h, n, m, d_Ap, d_Ai, d_Ax, d_x, d_y, d_ids, k0, 0, V, nparts, d_sets);
+}
+
+int symamd_axpby(symamd_handle h, int n, const double *d_a, const double *d_b, double a,
+                 double b, const double *d_y, double *d_z) {
+  SYMAMD_TRY(check_handle(h));
+  if (n < 0 || (n > 0 && (!d_y || !d_z)))
+    return fail(SYMAMD_ERR_INVALID_ARG, "symamd_axpby: bad arguments");
+  return launch_axpby(h, n, d_a, d_b, a, b, d_y, d_z);
+}
+
 int symamd_coarsen(symamd_handle h, int n, const int *d_Lp, const int *d_Li,
                    const double *d_cost, int inner_parts, int min_level_dist,
                    int div_rate, int *d_level_ptr, int *d_par_ptr, int *d_partition,

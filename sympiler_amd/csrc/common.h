@@ -426,6 +426,15 @@ int check_structure(symamd_handle h, int n, const int *d_Lp, const int *d_Li,
 // Level sets with every buffer in a fixed device array (inspector.hip).
 int levelset_noalloc(symamd_handle h, int n, const int *d_Lp, const int *d_Li,
                      int *d_level_ptr, int *d_level_set, int *levels);
+// The column-oriented SpMV forms as row pulls (spmv_csc.hip): y[r] (+)= the
+// products of the V visited columns, ids[k0 .. k0+V) or c0 .. c0+V-1, in
+// visit order; with d_sets, part by part (sums from +0.0 added in order).
+int spmv_pull(symamd_handle h, int n, int m, const int *d_Ap, const int *d_Ai,
+              const double *d_Ax, const double *d_x, double *d_y, const int *d_ids, int k0,
+              int c0, int V, int nparts, const int *d_sets);
+// z[i] = a_i * z[i] + b_i * y[i] (a_i = d_a[i] when d_a is given, else a).
+int launch_axpby(symamd_handle h, int n, const double *d_a, const double *d_b, double a,
+                 double b, const double *d_y, double *d_z);
 int launch_spmv(symamd_handle h, int n, const int *d_Ap, const int *d_Ai,
                 const double *d_Ax, const double *d_x, const double *d_y,
                 double *d_z, double a, double b, int mode);

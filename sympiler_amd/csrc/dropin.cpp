@@ -148,6 +148,136 @@ void spmv_csr(int n, const int *Ap, const int *Ai, const double *Ax,
   dz.download(z, n);
 }
 
+void spmv_csr_parallel(int n, const int *Ap, const int *Ai, const double *Ax,
+                       const double *x, double *y) {
+  spmv_csr(n, Ap, Ai, Ax, x, y);
+}
+
+// The rows of the supernodes are one contiguous range: the 6-argument
+// spmv_csr on it (row pointers stay absolute offsets into Ai/Ax).
+void spmv_csr_block(int n, const int *Ap, const int *Ai, const double *Ax, int nodes,
+                    const int *supernodes, const double *x, double *y) {
+  if (nodes <= 0) return;
+  const int r0 = supernodes[0], r1 = supernodes[nodes];
+  if (r0 < 0 || r1 < r0 || r1 > n)
+    throw std::invalid_argument("spmv_csr_block: supernode rows outside [0, n)");
+  if (r1 == r0) return;
+  const int e0 = Ap[r0], e1 = Ap[r1];
+  int ncol = n;
+  for (int p = e0; p < e1; ++p) ncol = Ai[p] + 1 > ncol ? Ai[p] + 1 : ncol;
+  std::vector<int> rp(r1 - r0 + 1);
+  for (int r = r0; r <= r1; ++r) rp[r - r0] = Ap[r] - e0;
+  DevBuf<int> dAp(rp.size(), rp.data()), dAi(e1 - e0, Ai + e0);
+  DevBuf<double> dAx(e1 - e0, Ax + e0), dx(ncol, x), dy(r1 - r0);
+  symamd_handle h = handle();
+  check(symamd_spmv_csr(h, r1 - r0, dAp.p, dAi.p, dAx.p, dx.p, dy.p), "spmv_csr_block");
+  hcheck(hipDeviceSynchronize(), "spmv_csr_block");
+  dy.download(y + r0, r1 - r0);
+}
+
+namespace {
+
+// Rows of a CSC matrix: the largest row index + 1 (at least `floor`).
+int csc_rows(int n, const int *Ap, const int *Ai, int floor) {
+  int m = floor;
+  for (int p = 0; p < Ap[n]; ++p) m = Ai[p] + 1 > m ? Ai[p] + 1 : m;
+  return m;
+}
+
+// y (m entries, updated in place) += columns c0 .. c1-1 of the CSC matrix.
+void csc_product(const char *who, int n, int m, const int *Ap, const int *Ai,
+                 const double *Ax, const double *x, double *y, int c0, int c1) {
+  const int nnz = Ap[n];
+  DevBuf<int> dAp(n + 1, Ap), dAi(nnz, Ai);
+  DevBuf<double> dAx(nnz, Ax), dx(n, x), dy(m, y);
+  symamd_handle h = handle();
+  check(symamd_spmv_csc(h, n, m, dAp.p, dAi.p, dAx.p, dx.p, dy.p, c0, c1), who);
+  dy.download(y, m);
+}
+
+// z += A x, then z = a z + b y with scalars (av, bv NULL) or per-row values.
+void csc_axpby(const char *who, int n, const int *Ap, const int *Ai, const double *Ax,
+               const double *x, const double *y, double *z, double a, double b,
+               const double *av, const double *bv) {
+  if (csc_rows(n, Ap, Ai, n) > n)
+    throw std::invalid_argument(std::string(who) + ": a row index lies outside [0, n)");
+  const int nnz = Ap[n];
+  DevBuf<int> dAp(n + 1, Ap), dAi(nnz, Ai);
+  DevBuf<double> dAx(nnz, Ax), dx(n, x), dy(n, y), dz(n, z);
+  DevBuf<double> da(av ? n : 0, av), db(bv ? n : 0, bv);
+  symamd_handle h = handle();
+  check(symamd_spmv_csc(h, n, n, dAp.p, dAi.p, dAx.p, dx.p, dz.p, 0, n), who);
+  check(symamd_axpby(h, n, av ? da.p : nullptr, bv ? db.p : nullptr, a, b, dy.p, dz.p), who);
+  hcheck(hipDeviceSynchronize(), who);
+  dz.download(z, n);
+}
+
+void csc_lbc(const char *who, int n_parts, const int *sets, const int *ids, int n,
+             const int *Ap, const int *Ai, const double *Ax, const double *x, double *y) {
+  if (n_parts <= 0 || n <= 0) return;
+  for (int l = 0; l < n_parts; ++l)
+    if (sets[l + 1] < sets[l] || sets[l] < 0)
+      throw std::invalid_argument(std::string(who) + ": part bounds must be nondecreasing");
+  for (int k = sets[0]; k < sets[n_parts]; ++k)
+    if (ids[k] < 0 || ids[k] >= n)
+      throw std::invalid_argument(std::string(who) + ": a column id lies outside [0, n)");
+  if (csc_rows(n, Ap, Ai, n) > n)
+    throw std::invalid_argument(std::string(who) + ": a row index lies outside [0, n)");
+  const int nnz = Ap[n], nid = sets[n_parts];
+  DevBuf<int> dAp(n + 1, Ap), dAi(nnz, Ai), dsets(n_parts + 1, sets), dids(nid, ids);
+  DevBuf<double> dAx(nnz, Ax), dx(n, x), dy(n, y);
+  symamd_handle h = handle();
+  check(symamd_spmv_csc_parts(h, n, n, dAp.p, dAi.p, dAx.p, dx.p, dy.p, n_parts, dsets.p,
+                              dids.p),
+        who);
+  dy.download(y, n);
+}
+
+}  // namespace
+
+void spmv_csc(int n, const int *Ap, const int *Ai, const double *Ax, const double *x,
+              double *y) {
+  if (n <= 0) return;
+  csc_product("spmv_csc", n, csc_rows(n, Ap, Ai, 0), Ap, Ai, Ax, x, y, 0, n);
+}
+
+void spmv_csc(int n, const int *Ap, const int *Ai, const double *Ax, const double *x,
+              const double *y, double *z, double a, double b) {
+  if (n <= 0) return;
+  csc_axpby("spmv_csc", n, Ap, Ai, Ax, x, y, z, a, b, nullptr, nullptr);
+}
+
+void spmv_csc(int n, const int *Ap, const int *Ai, const double *Ax, const double *x,
+              const double *y, double *z, double *a, double *b) {
+  if (n <= 0) return;
+  if (!a || !b) throw std::invalid_argument("spmv_csc: null a or b");
+  csc_axpby("spmv_csc", n, Ap, Ai, Ax, x, y, z, 0.0, 0.0, a, b);
+}
+
+void spmv_csc_lbc(int n_parts, const int *sets, const int *ids, int n, const int *Ap,
+                  const int *Ai, const double *Ax, const double *x, double *y) {
+  csc_lbc("spmv_csc_lbc", n_parts, sets, ids, n, Ap, Ai, Ax, x, y);
+}
+
+void spmv_csc_lbc(const int *set_ptr, const int *sets, const int *ids, int n, const int *Ap,
+                  const int *Ai, const double *Ax, const double *x, double *y) {
+  csc_lbc("spmv_csc_lbc", set_ptr[1], sets, ids, n, Ap, Ai, Ax, x, y);
+}
+
+void spmv_csc_parallel(int n, const int *Ap, const int *Ai, const double *Ax,
+                       const double *x, double *y) {
+  spmv_csc(n, Ap, Ai, Ax, x, y);
+}
+
+void spmv_csc_block(int n, const int *Ap, const int *Ai, const double *Ax, const double *x,
+                    double *y, int nodes, const int *supernodes) {
+  if (nodes <= 0 || n <= 0) return;
+  const int c0 = supernodes[0], c1 = supernodes[nodes];
+  if (c0 < 0 || c1 < c0 || c1 > n)
+    throw std::invalid_argument("spmv_csc_block: supernode columns outside [0, n)");
+  csc_product("spmv_csc_block", n, csc_rows(n, Ap, Ai, 0), Ap, Ai, Ax, x, y, c0, c1);
+}
+
 void sptrsv_csr(int n, int *Lp, int *Li, double *Lx, double *x) {
   solve_csr_host(n, Lp, Li, Lx, x);
 }

@@ -52,6 +52,15 @@ def test_sym_lib_dropins_exported_with_reference_mangling():
         "_ZN7sym_lib7ltsolveEiPiS0_PdS1_",
         "_ZN7sym_lib8spmv_csrEiPKiS1_PKdS3_Pd",
         "_ZN7sym_lib8spmv_csrEiPKiS1_PKdS3_S3_Pddd",
+        "_ZN7sym_lib17spmv_csr_parallelEiPKiS1_PKdS3_Pd",
+        "_ZN7sym_lib14spmv_csr_blockEiPKiS1_PKdiS1_S3_Pd",
+        "_ZN7sym_lib8spmv_cscEiPKiS1_PKdS3_Pd",
+        "_ZN7sym_lib8spmv_cscEiPKiS1_PKdS3_S3_Pddd",
+        "_ZN7sym_lib8spmv_cscEiPKiS1_PKdS3_S3_PdS4_S4_",
+        "_ZN7sym_lib12spmv_csc_lbcEiPKiS1_iS1_S1_PKdS3_Pd",
+        "_ZN7sym_lib12spmv_csc_lbcEPKiS1_S1_iS1_S1_PKdS3_Pd",
+        "_ZN7sym_lib17spmv_csc_parallelEiPKiS1_PKdS3_Pd",
+        "_ZN7sym_lib14spmv_csc_blockEiPKiS1_PKdS3_PdiS1_",
         "_ZN7sym_lib18build_levelSet_CSCEiPKiS1_RPiS3_",
         "_ZN7sym_lib24sptrsv_csr_lbc_redundantEiPiS0_PdS1_S1_iS0_S0_S0_S1_",
         "_ZN7sym_lib14sptrsv_csr_iluEiPKiS1_PKdS1_Pd",
@@ -88,18 +97,20 @@ def test_mangled_names_match_the_reference_objects():
                "sptrsv_csr_lbc_redundant", "sptrsv_csr_ilu", "sptrsv_csr_ilu_lbc",
                "sptrsv_csc", "sptrsv_csc_levelset", "sptrsv_csc_lbc", "ltsolve",
                "spmv_csr", "spic0_csc_left", "spic0_csc_left_LBC", "spic0_csr",
-               "spic0_csr_lbc", "sptrsv_csc_block", "sptrsv_csc_levelset_block"}
+               "spic0_csr_lbc", "sptrsv_csc_block", "sptrsv_csc_levelset_block",
+               "spmv_csr_parallel", "spmv_csr_block", "spmv_csc", "spmv_csc_lbc",
+               "spmv_csc_parallel", "spmv_csc_block"}
 
     def fname(sym):  # the length-prefixed function name after "sym_lib"
         m = re.match(r"_ZN7sym_lib(\d+)", sym)
         return sym[m.end():m.end() + int(m.group(1))] if m else None
 
     ref = {s for s in exported(oracle.REF_PATH) if fname(s) in on_path}
-    assert len(ref) == 18, sorted(ref)  # spmv_csr has two overloads
+    assert len(ref) == 27, sorted(ref)  # spmv_csr: two overloads, spmv_csc: three, _lbc: two
     # the tree-level coarsening of parsy (InspectionLevel_06.cpp:17)
     ref |= {s for s in exported(oracle.REF_PATH)
             if s.startswith("_ZN7sym_lib5parsy19getCoarseLevelSet_6")}
-    assert len(ref) == 19, sorted(ref)
+    assert len(ref) == 28, sorted(ref)
     missing = ref - exported(LIB)
     assert not missing, missing
 
