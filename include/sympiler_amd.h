@@ -109,10 +109,12 @@ int symamd_set_timeout_ms(symamd_handle h, double ms);
  *     regions run the owner schedule; 0 = they deal units like other plans
  *     (for comparison; results are identical).
  *   SYMAMD_OPT_REGION_CAP (read at solve time): region size cap of the
- *     SYMAMD_EXEC_REGIONS schedule (0, the default: 512 rows, or 256 for
- *     tasks of eight columns run by one wavefront, halved -- down to 32 --
- *     until a task's rows, halo and record fit a workgroup's LDS; a cap set
- *     here is used alone); each cap's schedule is built once per plan.
+ *     SYMAMD_EXEC_REGIONS schedule (0, the default: the cap symamd_tune
+ *     chose for the plan, else 1024 rows for single right-hand sides on the
+ *     memory-wavefront kernel, 256 for tasks of eight columns run by one
+ *     wavefront and 512 otherwise; halved -- down to 32 -- until a task's
+ *     rows, halo and record fit a workgroup's LDS; a cap set here is used
+ *     alone); each cap's schedule is built once per plan.
  *   SYMAMD_OPT_REGION_WAVES (read at solve time): wavefronts per task of
  *     the SYMAMD_EXEC_REGIONS schedule, 1, 2 or 4 for tasks of eight
  *     columns (k >= 8; narrower tasks use one); 0, the default, picks it.
@@ -469,7 +471,12 @@ int symamd_sn_destroy(symamd_snplan plan);
 /* Time the candidate schedules for k right-hand sides in `layout` on
  * scratch data and remember the fastest in the plan (and in a process-wide
  * cache keyed by the matrix fingerprint); SYMAMD_EXEC_AUTO then uses it.
- * Writes a human-readable summary (may be NULL). */
+ * Candidates: the sync-free unit executor (lanes per row, and for k > 8
+ * column tiles of 8 to 32 columns solved as separate work items), the
+ * level-set executor replayed from a graph (lanes per row) and the region
+ * executor (for one right-hand side at region caps 512 and 1024; the chosen
+ * cap also serves the plan's adjoint).  Writes a human-readable summary
+ * (may be NULL). */
 int symamd_tune(symamd_handle h, symamd_plan plan, int k,
                 symamd_layout layout, char *summary, int summary_len);
 

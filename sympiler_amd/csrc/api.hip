@@ -119,6 +119,19 @@ Choice choose(symamd_handle h, symamd_plan p, int k, int layout, int exec, bool 
   return c;
 }
 
+}  // namespace
+
+int tuned_region_cap(symamd_plan p, int k) {
+  for (int layout : {SYMAMD_ROW_MAJOR, SYMAMD_COL_MAJOR}) {
+    auto it = p->tuned.find(tune_key(k, layout));
+    if (it != p->tuned.end() && it->second.exec == SYMAMD_EXEC_REGIONS && it->second.cap > 0)
+      return it->second.cap;
+  }
+  return 0;
+}
+
+namespace {
+
 int check_handle(symamd_handle h) {
   if (!h) return fail(SYMAMD_ERR_INVALID_ARG, "null handle");
   return SYMAMD_OK;
@@ -268,8 +281,8 @@ namespace {
 
 std::string graph_key(void *d_B, int k, int64_t rs, int64_t cs, const Choice &c) {
   char buf[160];
-  snprintf(buf, sizeof buf, "%p/%d/%lld/%lld/%d/%d/%d", d_B, k, (long long)rs,
-           (long long)cs, c.exec, c.lanes, c.tile);
+  snprintf(buf, sizeof buf, "%p/%d/%lld/%lld/%d/%d/%d/%d", d_B, k, (long long)rs,
+           (long long)cs, c.exec, c.lanes, c.tile, c.cap);
   return buf;
 }
 
@@ -844,7 +857,12 @@ int symamd_tune(symamd_handle h, symamd_plan plan, int k, symamd_layout layout,
     std::vector<int> tiles = {0};
     if (ex == SYMAMD_EXEC_SYNCFREE)
       for (int t = 8; t < tile_cols(k, 0); t *= 2) tiles.push_back(t);
-    for (int tile : tiles) {
+    // region caps: for single right-hand sides 512 and 1024 rows (whichever
+    // schedule fits; the cap of the handle, when set, alone)
+    std::vector<int> caps = {0};
+    if (ex == SYMAMD_EXEC_REGIONS && k == 1 && h->region_cap == 0) caps = {512, 1024};
+    for (int tile : tiles)
+    for (int cap : caps) {
       const int kt = tile_cols(k, tile);
       const int gmax = ex == SYMAMD_EXEC_REGIONS ? 1 : std::min(64, gmax_rows);
       for (int g = 1; g * kt <= 64 && g <= gmax; g *= 2) {
@@ -852,6 +870,7 @@ int symamd_tune(symamd_handle h, symamd_plan plan, int k, symamd_layout layout,
         c.exec = ex;
         c.lanes = g;
         c.tile = tile;
+        c.cap = cap;
         plan->tuned[tune_key(k, layout)] = c;
         rc = solve_impl(h, plan, d_B, k, ld, layout, SYMAMD_EXEC_AUTO, true);  // warm / capture
         if (rc != SYMAMD_OK) break;
@@ -867,7 +886,8 @@ int symamd_tune(symamd_handle h, symamd_plan plan, int k, symamd_layout layout,
         }
         if (rc != SYMAMD_OK) break;
         char line[128];
-        snprintf(line, sizeof line, "exec=%d lanes=%d tile=%d ms=%.4f\n", ex, g, kt, ms);
+        snprintf(line, sizeof line, "exec=%d lanes=%d tile=%d cap=%d ms=%.4f\n", ex, g, kt,
+                 cap, ms);
         log += line;
         if (ms < best_ms * 0.97f) {  // prefer earlier (fewer lanes, sync-free) on ties
           best_ms = ms;
@@ -902,8 +922,8 @@ int symamd_tune(symamd_handle h, symamd_plan plan, int k, symamd_layout layout,
   plan->tuned[tune_key(k, layout)] = best;
   g_tune_cache[plan->fingerprint][tune_key(k, layout)] = best;
   char line[128];
-  snprintf(line, sizeof line, "best exec=%d lanes=%d tile=%d ms=%.4f\n", best.exec, best.lanes,
-           tile_cols(k, best.tile), best_ms);
+  snprintf(line, sizeof line, "best exec=%d lanes=%d tile=%d cap=%d ms=%.4f\n", best.exec,
+           best.lanes, tile_cols(k, best.tile), best.cap, best_ms);
   log += line;
   if (summary && summary_len > 0) {
     strncpy(summary, log.c_str(), summary_len - 1);

@@ -280,6 +280,7 @@ struct Choice {
   int exec = SYMAMD_EXEC_SYNCFREE;
   int lanes = 1;  // lanes cooperating on one row's nonzeros
   int tile = 0;   // columns per tile of the unit executors (0: from k)
+  int cap = 0;    // region cap of the region executor (0: the default)
 };
 
 }  // namespace symamd
@@ -503,7 +504,9 @@ int launch_grad_l(symamd_handle h, symamd_plan p, const void *d_X, const void *d
 // Region size cap used for k right-hand sides (the LDS of one wave holds
 // the region's and its halo's values of one column group per row), unless
 // the handle sets one.
-int region_cap(symamd_handle h, int k);
+int region_cap(symamd_handle h, symamd_plan p, int k);
+// The region cap the tuner chose for k right-hand sides (either layout), or 0.
+int tuned_region_cap(symamd_plan p, int k);
 int region_band(symamd_handle h, int k);
 // True when a region schedule for k right-hand sides exists (building it on
 // first use) and its buffers fit one wave's LDS.

@@ -1076,12 +1076,18 @@ int region_waves(symamd_handle h, int k) {
 
 }  // namespace
 
-// Default cap: 512 rows, or 256 for tasks of eight columns run by one wave
-// (measured on the 128^3 IC(0) factor: k=1 is fastest at 512 -- 1024 and
-// 256 are slower; k=32 with four waves per task at 512 -- 1024 is slower --
-// and with one wave at 256).
-int region_cap(symamd_handle h, int k) {
+// Region cap of a solve of k right-hand sides: the handle's, else the
+// plan's tuned one, else 1024 rows for single right-hand sides on the
+// memory-wave kernel, 256 for tasks of eight columns run by one wave and
+// 512 otherwise (measured on the IC(0) factors of the 128^3 and 64^3
+// Laplacians: k = 1 with the memory wave 0.309 / 0.137 ms at 1024 vs
+// 0.323 / 0.142 ms at 512 -- the 512-system batch of 2D grids prefers 512,
+// which the tuner finds; k = 32 with four waves per task is fastest at 512).
+int region_cap(symamd_handle h, symamd_plan p, int k) {
   if (h->region_cap > 0) return h->region_cap;
+  const int tuned = p ? tuned_region_cap(p, k) : 0;
+  if (tuned > 0) return tuned;
+  if (k == 1 && h->region_mw) return 1024;
   return task_cols(k) == KG_MAX && region_waves(h, k) == 1 ? 256 : 512;
 }
 
@@ -1103,7 +1109,7 @@ static RegionPlan *fitted_plan(symamd_handle h, symamd_plan p, int k, bool trans
   if (p->n == 0 || k < 1 || k > KG_MAX * RegionPlan::MAX_GROUPS) return nullptr;
   const int vs = p->dtype == SYMAMD_F64 ? 8 : 4;
   const int kt = task_cols(k);
-  for (int cap = region_cap(h, k);; cap /= 2) {
+  for (int cap = region_cap(h, p, k);; cap /= 2) {
     RegionPlan *rp = nullptr;
     if (plan_regions(h, p, cap, k, &rp, transposed) != SYMAMD_OK) return nullptr;
     if (rp->R > 0 && region_lds_bytes(*rp, kt, vs) <= (size_t)h->lds_per_block) return rp;
