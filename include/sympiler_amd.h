@@ -52,12 +52,24 @@ typedef enum {
   SYMAMD_EXEC_LEVELSET = 1,  /* one kernel launch per level (plain level set)   */
   SYMAMD_EXEC_SYNCFREE = 2,  /* one persistent launch, per-unit ready flags     */
   SYMAMD_EXEC_LEVELSET_GRAPH = 3, /* the per-level launches replayed as a hipGraph */
-  SYMAMD_EXEC_REGIONS = 4    /* one persistent launch over compact regions of rows,
+  SYMAMD_EXEC_REGIONS = 4,   /* one persistent launch over compact regions of rows,
                                 each solved by one workgroup in LDS (forward
                                 and adjoint solves, 1 <= k <= 64, at the largest
                                 region cap whose task fits the LDS; other cases,
                                 and matrices with no such schedule, run
                                 SYNCFREE) */
+  SYMAMD_EXEC_WIDE = 5,      /* disabled: returns SYMAMD_ERR_NOT_SUPPORTED (a
+                                persistent executor with one workgroup per row;
+                                its first run exceeded the watchdog and the
+                                cause is not established) */
+  SYMAMD_EXEC_LEVELSET_SPLIT = 6 /* one launch per level, replayed as a hipGraph:
+                                rows with at most T off-diagonal entries one
+                                lane per row and column in entry order
+                                (bit-identical), each longer row by a
+                                1024-thread workgroup over its entries (sum
+                                re-associated, within 1e-12); T from
+                                SYMAMD_OPT_SPLIT_LONG; forward solves
+                                (adjoints run LEVELSET) */
 } symamd_executor;
 
 typedef struct symamd_handle_s *symamd_handle;
@@ -133,12 +145,18 @@ int symamd_set_timeout_ms(symamd_handle h, double ms);
  *     never wait for memory; the default band is then 4 slices for single
  *     right-hand sides.  0 = every wavefront of a task does both (default
  *     band: one per region).
- *     Results are identical either way. */
+ *     Results are identical either way.
+ *   SYMAMD_OPT_SPLIT_LONG (read at solve time): SYMAMD_EXEC_LEVELSET_SPLIT
+ *     solves rows with more than this many off-diagonal entries with a
+ *     whole workgroup each (0, the default: 16 for column tiles of up to 16
+ *     columns, 32 for wider ones); each value's lists are built once per
+ *     plan.  Rows at or under it are bit-identical to the serial solve. */
 enum { SYMAMD_OPT_TICKETS = 1, SYMAMD_OPT_TRACE_PTR = 2, SYMAMD_OPT_TRACE_CAP = 3,
        SYMAMD_OPT_MAX_WAVES = 4, SYMAMD_OPT_COLMAJOR_DIRECT = 5, SYMAMD_OPT_MARKS = 6,
        SYMAMD_OPT_REGIONS = 7, SYMAMD_OPT_REGION_ROWS = 8, SYMAMD_OPT_OWNERS = 9,
        SYMAMD_OPT_REGION_CAP = 10, SYMAMD_OPT_REGION_WAVES = 11,
-       SYMAMD_OPT_REGION_BAND = 12, SYMAMD_OPT_REGION_MW = 13 };
+       SYMAMD_OPT_REGION_BAND = 12, SYMAMD_OPT_REGION_MW = 13,
+       SYMAMD_OPT_SPLIT_LONG = 14 };
 int symamd_set_option(symamd_handle h, int option, int64_t value);
 const char *symamd_last_error(void);
 /* Library version string. */

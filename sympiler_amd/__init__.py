@@ -39,7 +39,8 @@ GEN_PATH = os.path.join(_HERE, "libsymamd_gen.so")
 
 F64, F32 = 0, 1
 COL_MAJOR, ROW_MAJOR = 0, 1
-EXEC = {"auto": 0, "levelset": 1, "syncfree": 2, "graph": 3, "regions": 4}
+EXEC = {"auto": 0, "levelset": 1, "syncfree": 2, "graph": 3, "regions": 4, "wide": 5,
+        "split": 6}
 STATUS = {0: "OK", 1: "INVALID_ARG", 2: "STRUCTURE", 3: "ZERO_PIVOT", 4: "HIP",
           5: "ALLOC", 6: "TIMEOUT", 7: "NOT_SUPPORTED", 8: "NOT_SPD"}
 
@@ -232,6 +233,11 @@ class Handle:
     def set_region_mw(self, on: bool):
         """Region tasks with a separate memory wavefront (True, default)."""
         _check(lib().symamd_set_option(self._h, 13, int(bool(on))))
+
+    def set_split_long(self, entries: int):
+        """Off-diagonal entries above which the split level executor gives a
+        row a whole workgroup (0: by column-tile width)."""
+        _check(lib().symamd_set_option(self._h, 14, int(entries)))
 
     def set_owners(self, on: bool):
         """Run plans with regions on the owner schedule (True, default) or

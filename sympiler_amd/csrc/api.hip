@@ -54,6 +54,8 @@ void free_plan_arrays(symamd_plan p) {
     for (void *q : up)
       if (q) (void)hipFree(q);
   }
+  for (auto &kv : p->split)
+    if (kv.second.d_rec) (void)hipFree(kv.second.d_rec);
   for (auto &kv : p->rplans) free_region_plan(kv.second);
   for (auto &kv : p->rplans_t) free_region_plan(kv.second);
   free_ic0_pairs(p);
@@ -139,6 +141,16 @@ int check_handle(symamd_handle h) {
 
 int check_plan(symamd_plan p) {
   if (!p) return fail(SYMAMD_ERR_INVALID_ARG, "null plan");
+  return SYMAMD_OK;
+}
+
+int check_exec(int exec) {
+  if (exec == SYMAMD_EXEC_WIDE)
+    return fail(SYMAMD_ERR_NOT_SUPPORTED,
+                "the persistent workgroup-per-row executor is disabled (its first run "
+                "exceeded the watchdog; cause not established)");
+  if (exec < SYMAMD_EXEC_AUTO || exec > SYMAMD_EXEC_LEVELSET_SPLIT)
+    return fail(SYMAMD_ERR_INVALID_ARG, "unknown executor %d", exec);
   return SYMAMD_OK;
 }
 
@@ -300,6 +312,11 @@ bool through_workspace(symamd_handle h, int k, int layout) {
 int prepare_solve(symamd_handle h, symamd_plan p, int k, int layout, const Choice &c) {
   if (c.exec == SYMAMD_EXEC_REGIONS && region_usable(h, p, k))
     return through_workspace(h, k, layout) ? prepare_workspace(h, p, k) : SYMAMD_OK;
+  if (c.exec == SYMAMD_EXEC_LEVELSET_SPLIT) {
+    SplitLists *sl = nullptr;
+    SYMAMD_TRY(split_lists(h, p, split_threshold(h, tile_cols(k, c.tile)), &sl));
+    return through_workspace(h, k, layout) ? prepare_workspace(h, p, k) : SYMAMD_OK;
+  }
   UnitSet *us = nullptr;
   SYMAMD_TRY(plan_units(h, p, forward_rows_per_unit(k, c.lanes, c.tile), false, &us));
   const int kt = tile_cols(k, c.tile);
@@ -311,6 +328,7 @@ int prepare_solve(symamd_handle h, symamd_plan p, int k, int layout, const Choic
 int run_solve(symamd_handle h, symamd_plan p, void *d_B, int k, int64_t ld,
               int64_t rs, int64_t cs, int layout, int exec, int lanes, int tile) {
   if (exec == SYMAMD_EXEC_LEVELSET_GRAPH) exec = SYMAMD_EXEC_LEVELSET;
+  if (exec == SYMAMD_EXEC_LEVELSET_SPLIT) lanes = 1;
   if (exec == SYMAMD_EXEC_REGIONS) {
     // column-major blocks go through the row-major workspace, the rest in
     // place (schedule and workspace built beforehand by prepare_solve, so
@@ -361,8 +379,11 @@ int solve_impl(symamd_handle h, symamd_plan p, void *d_B, int k, int64_t ld,
   SYMAMD_TRY(strides(p->n, k, ld, layout, &rs, &cs));
   if (k == 0 || p->n == 0) return SYMAMD_OK;
   Choice c = choose(h, p, k, layout, exec, true);
-  if (c.exec == SYMAMD_EXEC_LEVELSET_GRAPH && graph_ok) {
+  if ((c.exec == SYMAMD_EXEC_LEVELSET_GRAPH || c.exec == SYMAMD_EXEC_LEVELSET_SPLIT) &&
+      graph_ok) {
     std::string key = graph_key(d_B, k, rs, cs, c) + (through_workspace(h, k, layout) ? "/ws" : "");
+    if (c.exec == SYMAMD_EXEC_LEVELSET_SPLIT)
+      key += "/split" + std::to_string(split_threshold(h, tile_cols(k, c.tile)));
     auto it = p->gcache.find(key);
     if (it == p->gcache.end()) {
       hipGraph_t g = nullptr;
@@ -443,6 +464,11 @@ int symamd_set_option(symamd_handle h, int option, int64_t value) {
       return SYMAMD_OK;
     case SYMAMD_OPT_REGION_MW:
       h->region_mw = value != 0;
+      return SYMAMD_OK;
+    case SYMAMD_OPT_SPLIT_LONG:
+      if (value < 0 || value > (1 << 30))
+        return fail(SYMAMD_ERR_INVALID_ARG, "split threshold out of range");
+      h->split_long = (int)value;
       return SYMAMD_OK;
     case SYMAMD_OPT_REGION_BAND:
       if (value < 0 || value > 255)
@@ -675,8 +701,7 @@ int symamd_solve(symamd_handle h, symamd_plan plan, void *d_B, int k, int64_t ld
   SYMAMD_TRY(check_plan(plan));
   if (k < 0 || (k > 0 && plan->n > 0 && !d_B))
     return fail(SYMAMD_ERR_INVALID_ARG, "symamd_solve: bad arguments");
-  if (exec < SYMAMD_EXEC_AUTO || exec > SYMAMD_EXEC_REGIONS)
-    return fail(SYMAMD_ERR_INVALID_ARG, "unknown executor %d", (int)exec);
+  SYMAMD_TRY(check_exec(exec));
   return solve_impl(h, plan, d_B, k, ld, layout, exec, true);
 }
 
@@ -696,6 +721,7 @@ int symamd_graph_create(symamd_handle h, symamd_plan plan, void *d_B, int k,
   SYMAMD_TRY(check_handle(h));
   SYMAMD_TRY(check_plan(plan));
   if (!g || k < 1 || !d_B) return fail(SYMAMD_ERR_INVALID_ARG, "symamd_graph_create: bad arguments");
+  SYMAMD_TRY(check_exec(exec));
   int64_t rs = 0, cs = 0;
   SYMAMD_TRY(strides(plan->n, k, ld, layout, &rs, &cs));
   Choice c = choose(h, plan, k, layout, exec, true);
@@ -732,8 +758,7 @@ int symamd_adjoint(symamd_handle h, symamd_plan plan, const void *d_x,
   SYMAMD_TRY(check_plan(plan));
   if (plan->n > 0 && (!d_x || !d_gx || !d_gb))
     return fail(SYMAMD_ERR_INVALID_ARG, "symamd_adjoint: bad arguments");
-  if (exec < SYMAMD_EXEC_AUTO || exec > SYMAMD_EXEC_REGIONS)
-    return fail(SYMAMD_ERR_INVALID_ARG, "unknown executor %d", (int)exec);
+  SYMAMD_TRY(check_exec(exec));
   // the region executor solves L^T on the transposed region schedule; a
   // forced choice without a usable schedule runs sync-free, like forward
   if (exec == SYMAMD_EXEC_REGIONS ? region_adjoint_usable(h, plan, 1)
@@ -744,7 +769,8 @@ int symamd_adjoint(symamd_handle h, symamd_plan plan, const void *d_x,
     return d_gLx && !fused ? launch_grad_l(h, plan, d_x, d_gb, d_gLx, 1, 1, 0) : SYMAMD_OK;
   }
   Choice c = choose(h, plan, 1, SYMAMD_COL_MAJOR, exec, false);
-  int e = c.exec == SYMAMD_EXEC_LEVELSET_GRAPH ? SYMAMD_EXEC_LEVELSET : c.exec;
+  int e = c.exec == SYMAMD_EXEC_LEVELSET_GRAPH || c.exec == SYMAMD_EXEC_LEVELSET_SPLIT
+              ? SYMAMD_EXEC_LEVELSET : c.exec;
   if (e == SYMAMD_EXEC_REGIONS) e = SYMAMD_EXEC_SYNCFREE;
   return launch_adjoint(h, plan, d_x, d_gx, d_gb, d_gLx, e, c.lanes);
 }
@@ -756,8 +782,7 @@ int symamd_adjoint_block(symamd_handle h, symamd_plan plan, const void *d_X,
   SYMAMD_TRY(check_plan(plan));
   if (k < 0 || (k > 0 && plan->n > 0 && (!d_X || !d_GX || !d_GB)))
     return fail(SYMAMD_ERR_INVALID_ARG, "symamd_adjoint_block: bad arguments");
-  if (exec < SYMAMD_EXEC_AUTO || exec > SYMAMD_EXEC_REGIONS)
-    return fail(SYMAMD_ERR_INVALID_ARG, "unknown executor %d", (int)exec);
+  SYMAMD_TRY(check_exec(exec));
   int64_t rs = 0, cs = 0;
   SYMAMD_TRY(strides(plan->n, k, ld, layout, &rs, &cs));
   if (k == 0 || plan->n == 0) return SYMAMD_OK;
@@ -775,7 +800,8 @@ int symamd_adjoint_block(symamd_handle h, symamd_plan plan, const void *d_X,
     // contiguous vectors (column c of the block at offset c*cs, stride rs)
     Choice c = choose(h, plan, 1, SYMAMD_COL_MAJOR, exec == SYMAMD_EXEC_REGIONS
                                                         ? SYMAMD_EXEC_SYNCFREE : exec, false);
-    int e = c.exec == SYMAMD_EXEC_LEVELSET_GRAPH ? SYMAMD_EXEC_LEVELSET : c.exec;
+    int e = c.exec == SYMAMD_EXEC_LEVELSET_GRAPH || c.exec == SYMAMD_EXEC_LEVELSET_SPLIT
+                ? SYMAMD_EXEC_LEVELSET : c.exec;
     if (e == SYMAMD_EXEC_REGIONS) e = SYMAMD_EXEC_SYNCFREE;
     const size_t vs = plan->dtype == SYMAMD_F64 ? 8 : 4;
     void *tin = nullptr, *tout = nullptr;
@@ -846,13 +872,16 @@ int symamd_tune(symamd_handle h, symamd_plan plan, int k, symamd_layout layout,
   float best_ms = 1e30f;
   const int saved_override = plan->lanes_override;
   plan->lanes_override = 0;
-  const int execs[] = {SYMAMD_EXEC_SYNCFREE, SYMAMD_EXEC_LEVELSET_GRAPH, SYMAMD_EXEC_REGIONS};
+  const int execs[] = {SYMAMD_EXEC_SYNCFREE, SYMAMD_EXEC_LEVELSET_GRAPH, SYMAMD_EXEC_REGIONS,
+                       SYMAMD_EXEC_LEVELSET_SPLIT};
   int rc = SYMAMD_OK;
   // candidates: the executor; for the unit executors the column tile (all
   // of k up to 64 columns, or narrower tiles of 8 to 32 columns solved as
   // separate work items) and the lanes per row that share a row's entries
   for (int ex : execs) {
-    // the region executor has no lanes-per-row or tile choice and needs a schedule
+    // the region executor has no lanes-per-row or tile choice and needs a
+    // schedule; the split level executor has one lane per row (its long rows
+    // are split by the schedule itself) and no tile choice
     if (ex == SYMAMD_EXEC_REGIONS && !region_usable(h, plan, k)) continue;
     std::vector<int> tiles = {0};
     if (ex == SYMAMD_EXEC_SYNCFREE)
@@ -864,7 +893,8 @@ int symamd_tune(symamd_handle h, symamd_plan plan, int k, symamd_layout layout,
     for (int tile : tiles)
     for (int cap : caps) {
       const int kt = tile_cols(k, tile);
-      const int gmax = ex == SYMAMD_EXEC_REGIONS ? 1 : std::min(64, gmax_rows);
+      const int gmax = ex == SYMAMD_EXEC_REGIONS || ex == SYMAMD_EXEC_LEVELSET_SPLIT
+                           ? 1 : std::min(64, gmax_rows);
       for (int g = 1; g * kt <= 64 && g <= gmax; g *= 2) {
         Choice c;
         c.exec = ex;

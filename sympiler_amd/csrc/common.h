@@ -304,7 +304,16 @@ struct symamd_handle_s {
   int region_waves = 0;     // waves per region task (0: chosen from k)
   int region_band = 0;      // slices per published band (0: the default)
   int region_mw = 1;        // region executor: memory-wave kernel where it applies
+  int split_long = 0;       // split level executor: long-row threshold (0: by tile width)
   uint64_t timeout_ticks() const { return (uint64_t)(timeout_ms * 1.0e5); }
+};
+
+// Long rows of the split level executor for one threshold: records {q,
+// first entry, diagonal entry, row} grouped by level (device) and their
+// levels+1 offsets (host).
+struct SplitLists {
+  int4 *d_rec = nullptr;
+  std::vector<int> ptr;
 };
 
 struct symamd_plan_s {
@@ -331,6 +340,8 @@ struct symamd_plan_s {
   int *d_ipos = nullptr;               // row -> position
   int *d_cpos = nullptr;               // column ids of the copy, in positions
   void *d_ws = nullptr;                // level-ordered row-major workspace
+  // split level executor, per long-row threshold
+  std::map<int, SplitLists> split;
   size_t ws_bytes = 0;
   int *d_orp = nullptr;                // caller CSR row pointers, n+1
   int *d_oci = nullptr;                // caller CSR column ids, nnz (built on first use)
@@ -448,6 +459,11 @@ int tile_cols(int k, int tile);
 int unit_flags(symamd_handle h, UnitSet *us, int tiles);
 // Column-major multi-RHS solves through a level-ordered workspace.
 int prepare_workspace(symamd_handle h, symamd_plan p, int k);
+// The split level executor: its long-row threshold for a column tile of kt
+// columns, and the long-row lists of a threshold (built once per plan and
+// threshold, outside graph capture).
+int split_threshold(symamd_handle h, int kt);
+int split_lists(symamd_handle h, symamd_plan p, int thr, SplitLists **out);
 int launch_forward_ws(symamd_handle h, symamd_plan p, void *d_B, int k, int64_t ld,
                       int exec, int lanes, int tile);
 // Coarsened 3-array schedule on the device (lbc.hip).

@@ -448,6 +448,147 @@ __global__ __launch_bounds__(256) void k_level(SolveArgs<T> a, int q0, int q1) {
   if constexpr (ADJ) grad_rows<T, G, 1>(a, sv);
 }
 
+// One launch per level, one lane per row and column (forward solves): a
+// wave solves 64/KT consecutive rows of [q0, q1) for column tile
+// blockIdx.y, so the tiles of a row run side by side instead of one after
+// another.  Each lane keeps LC_DEPS dependency values in flight and has the
+// column ids of the next chunk loading meanwhile; the products are
+// subtracted in ascending entry order, the serial solve's order, so the
+// result is bit-identical.  With KT == 64 a wave holds one row and every
+// row-structure load is wave-uniform.
+constexpr int LC_DEPS = 16;
+
+// Row q (level-ordered position) for one column: x = (b - sum L*x) / diag
+// with the products subtracted in entry order.  Rows with more than
+// `max_off` off-diagonal entries are left alone (another workgroup solves
+// them).
+template <typename T>
+__device__ __forceinline__ void serial_row(const SolveArgs<T> &a, int q, bool rowok,
+                                           bool colok, int64_t coff, int max_off) {
+  int i = 0, jb = 0, je = 0;
+  T dg = T(1), acc = T(0);
+  if (rowok) {
+    jb = a.rp[q];
+    je = a.rp[q + 1] - 1;  // the diagonal
+    if (je - jb > max_off) rowok = false;
+  }
+  if (!rowok) jb = je = 0;  // no loop; nothing stored
+  if (rowok) {
+    i = a.perm ? a.perm[q] : q;
+    dg = a.val[je];
+    acc = a.B[(int64_t)i * a.rs + coff];
+  }
+  const int64_t own = (int64_t)i * a.rs + coff;  // a valid address for dead slots
+  int d[LC_DEPS];
+#pragma unroll
+  for (int t = 0; t < LC_DEPS; ++t) d[t] = a.ci[jb + t < je ? jb + t : je];
+  for (int e0 = jb; e0 < je; e0 += LC_DEPS) {
+    T xv[LC_DEPS], vv[LC_DEPS];
+#pragma unroll
+    for (int t = 0; t < LC_DEPS; ++t) {
+      const bool live = e0 + t < je;
+      xv[t] = a.X[live ? (int64_t)d[t] * a.rs + coff : own];
+      vv[t] = a.val[live ? e0 + t : je];
+    }
+    // the next chunk's column ids, loading while this chunk is summed
+#pragma unroll
+    for (int t = 0; t < LC_DEPS; ++t) {
+      const int e = e0 + LC_DEPS + t;
+      d[t] = a.ci[e < je ? e : je];
+    }
+#pragma unroll
+    for (int t = 0; t < LC_DEPS; ++t) {
+      const T prod = vv[t] * xv[t];
+      acc = e0 + t < je ? acc - prod : acc;
+    }
+  }
+  if (rowok && colok) a.X[own] = acc / dg;
+}
+
+template <typename T, int KT>
+__global__ __launch_bounds__(256) void k_level_cols(SolveArgs<T> a, int q0, int q1) {
+  constexpr int RPW = 64 / KT;
+  const int lane = threadIdx.x & 63;
+  const int qw = q0 + (int)(blockIdx.x * 4 + (threadIdx.x >> 6)) * RPW;
+  if (qw >= q1) return;  // whole wave past the level
+  int q = qw + lane / KT;
+  if constexpr (RPW == 1) q = __builtin_amdgcn_readfirstlane(q);
+  const int col = (int)blockIdx.y * KT + lane % KT;
+  const bool colok = col < a.k;
+  serial_row<T>(a, q, q < q1, colok, (int64_t)(colok ? col : 0) * a.cs, 0x7fffffff);
+}
+
+// Split level executor: one launch per level.  Blocks [0, nlong) each solve
+// one row with more than max_off off-diagonal entries (records lrec[]:
+// position, first entry, diagonal entry, row), its entries dealt over
+// SPLIT_W wavefronts x 64/KT lane slices, LC_DEPS loads in flight per lane;
+// the slice sums meet in a fixed tree (lanes, then wavefronts in order), so
+// the row's sum is re-associated but the same on every run.  The remaining
+// blocks solve the other rows of the level one lane per row and column in
+// entry order (bit-identical), 64/KT rows per wavefront.  Column tile =
+// blockIdx.y.  (Eight wavefronts per group measured slower at k = 32 and
+// 128: fewer slices per long row.)
+constexpr int SPLIT_W = 16;
+
+template <typename T, int KT>
+__device__ __forceinline__ void split_rows(const SolveArgs<T> &a, int q0, int q1,
+                                           const int4 *__restrict__ lrec, int nlong,
+                                           int max_off) {
+  constexpr int SPW = 64 / KT;
+  __shared__ T part[SPLIT_W][KT];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int col = (int)blockIdx.y * KT + lane % KT;
+  const bool colok = col < a.k;
+  const int64_t coff = (int64_t)(colok ? col : 0) * a.cs;
+  if ((int)blockIdx.x >= nlong) {
+    const int qw = q0 + (((int)blockIdx.x - nlong) * SPLIT_W + wid) * SPW;
+    if (qw >= q1) return;  // whole wave past the level
+    int q = qw + lane / KT;
+    if constexpr (SPW == 1) q = __builtin_amdgcn_readfirstlane(q);
+    serial_row<T>(a, q, q < q1, colok, coff, max_off);
+    return;
+  }
+  constexpr int S = SPLIT_W * SPW;  // entry slices of the row
+  const int4 r = lrec[blockIdx.x];
+  const int i = a.perm ? r.w : r.x;
+  const int jb = r.y, je = r.z;
+  const int64_t own = (int64_t)i * a.rs + coff;
+  const T bv = a.B[own], dg = a.val[je];  // in flight with the first chunk
+  T acc = T(0);
+  for (int e0 = jb + wid * SPW + lane / KT; e0 < je; e0 += S * LC_DEPS) {
+    T xv[LC_DEPS], vv[LC_DEPS];
+#pragma unroll
+    for (int t = 0; t < LC_DEPS; ++t) {
+      const int e = e0 + t * S;
+      const bool live = e < je;
+      const int d = a.ci[live ? e : je];
+      xv[t] = a.X[live ? (int64_t)d * a.rs + coff : own];
+      vv[t] = a.val[live ? e : je];
+    }
+#pragma unroll
+    for (int t = 0; t < LC_DEPS; ++t)
+      if (e0 + t * S < je) acc += vv[t] * xv[t];
+  }
+  // the wavefront's slices of each column, then the wavefronts in order
+#pragma unroll
+  for (int off = KT; off < 64; off <<= 1) acc += __shfl_xor(acc, off);
+  if (lane < KT) part[wid][lane] = acc;
+  __syncthreads();
+  if (wid == 0 && lane < KT) {
+    T sum = part[0][lane];
+#pragma unroll
+    for (int w = 1; w < SPLIT_W; ++w) sum += part[w][lane];
+    if (colok) a.X[own] = (bv - sum) / dg;
+  }
+}
+
+template <typename T, int KT>
+__global__ __launch_bounds__(64 * SPLIT_W) void k_level_split(SolveArgs<T> a, int q0, int q1,
+                                                            const int4 *__restrict__ lrec,
+                                                            int nlong, int max_off) {
+  split_rows<T, KT>(a, q0, q1, lrec, nlong, max_off);
+}
+
 // ------------------------------------------------- caller schedules
 // Executes a caller-given 3-array coarsened schedule with the semantics of
 // sparse_blas/sptrsv.cpp:57-75 (sptrsv_csr_lbc): one launch per coarse
@@ -575,8 +716,33 @@ int unit_flags(symamd_handle h, UnitSet *us, int tiles) {
 
 namespace {
 
+template <typename T, int KT>
+int run_split(symamd_handle h, symamd_plan p, SolveArgs<T> a) {
+  const int thr = split_threshold(h, KT);
+  SplitLists *sl = nullptr;
+  SYMAMD_TRY(split_lists(h, p, thr, &sl));
+  const int tiles = (a.k + KT - 1) / KT;
+  if (tiles > 65535)
+    return fail(SYMAMD_ERR_NOT_SUPPORTED, "split level executor: %d column tiles", tiles);
+  constexpr int RPB = SPLIT_W * (64 / KT);  // short rows per workgroup
+  for (int l = 0; l < p->levels; ++l) {
+    const int q0 = p->h_lvl_ptr[l], q1 = p->h_lvl_ptr[l + 1];
+    if (q1 <= q0) continue;
+    const int l0 = sl->ptr[l], nl = sl->ptr[l + 1] - l0;
+    const dim3 grid((unsigned)(nl + (q1 - q0 + RPB - 1) / RPB), (unsigned)tiles);
+    k_level_split<T, KT><<<grid, 64 * SPLIT_W, 0, h->stream>>>(a, q0, q1, sl->d_rec + l0, nl,
+                                                                thr);
+  }
+  SYMAMD_HIP(hipGetLastError());
+  return SYMAMD_OK;
+}
+
 template <typename T, int G, int KT, int NR, bool ADJ>
 int run(symamd_handle h, symamd_plan p, SolveArgs<T> a, int exec) {
+  if (exec == SYMAMD_EXEC_LEVELSET_SPLIT) {
+    if constexpr (G == 1 && !ADJ) return run_split<T, KT>(h, p, a);
+    return fail(SYMAMD_ERR_INVALID_ARG, "the split level executor runs forward solves, one lane per row");
+  }
   constexpr int R = NR * 64 / (G * KT);
   static_assert(R <= UNIT_ROWS_MAX, "unit rows exceed the LDS slab");
   UnitSet *us = nullptr;
@@ -602,6 +768,20 @@ int run(symamd_handle h, symamd_plan p, SolveArgs<T> a, int exec) {
                              (const void *)&k_syncfree<T, G, KT, NR, ADJ, SY_FLAGS>, a);
   }
   constexpr int LPR = G * KT;
+  const int tiles = (a.k + KT - 1) / KT;
+  if constexpr (G == 1 && !ADJ) {
+    if (tiles <= 65535) {
+      constexpr int RPB = 4 * (64 / KT);  // rows per workgroup
+      for (int l = 0; l < p->levels; ++l) {
+        const int q0 = p->h_lvl_ptr[l], q1 = p->h_lvl_ptr[l + 1];
+        if (q1 <= q0) continue;
+        const dim3 grid((unsigned)((q1 - q0 + RPB - 1) / RPB), (unsigned)tiles);
+        k_level_cols<T, KT><<<grid, 256, 0, h->stream>>>(a, q0, q1);
+      }
+      SYMAMD_HIP(hipGetLastError());
+      return SYMAMD_OK;
+    }
+  }
   for (int l0 = 0; l0 < p->levels; ++l0) {
     const int l = ADJ ? p->levels - 1 - l0 : l0;
     const int q0 = p->h_lvl_ptr[l], q1 = p->h_lvl_ptr[l + 1];
@@ -748,6 +928,56 @@ __global__ void k_cols_to_positions(int64_t nnz, const int *__restrict__ ci,
 }
 
 }  // namespace
+
+// Long-row threshold of the split level executor: the handle's, else 16
+// off-diagonal entries for column tiles of up to 16 columns and 32 for wider
+// ones (measured on the Zipf(1.8) n = 1e6 matrix: at k = 8 the lower
+// threshold took 3.4 ms against 4.8; at k = 32 and 128 it took 4.9 and 7.7 ms
+// against 4.4 and 6.8).
+int split_threshold(symamd_handle h, int kt) {
+  if (h->split_long > 0) return h->split_long;
+  return kt <= 16 ? 16 : 32;
+}
+
+// Long rows of every level for the split level executor: the positions q
+// with more than `thr` off-diagonal entries in the level-ordered copy,
+// grouped by level, as records {q, first entry, diagonal entry, row}.
+int split_lists(symamd_handle h, symamd_plan p, int thr, SplitLists **out) {
+  auto it = p->split.find(thr);
+  if (it != p->split.end()) {
+    *out = &it->second;
+    return SYMAMD_OK;
+  }
+  if (capturing(h->stream))
+    return fail(SYMAMD_ERR_NOT_SUPPORTED, "split level lists not prepared before capture");
+  const int n = p->n;
+  std::vector<int> rp(n + 1), perm(n);
+  SYMAMD_HIP(hipStreamSynchronize(h->stream));
+  SYMAMD_HIP(hipMemcpy(rp.data(), p->d_rp, sizeof(int) * (n + 1), hipMemcpyDeviceToHost));
+  if (n > 0)
+    SYMAMD_HIP(hipMemcpy(perm.data(), p->d_perm, sizeof(int) * n, hipMemcpyDeviceToHost));
+  std::vector<int> ptr(p->levels + 1, 0);
+  std::vector<int4> pos;
+  for (int l = 0; l < p->levels; ++l) {
+    for (int q = p->h_lvl_ptr[l]; q < p->h_lvl_ptr[l + 1]; ++q)
+      if (rp[q + 1] - 1 - rp[q] > thr) pos.push_back(make_int4(q, rp[q], rp[q + 1] - 1, perm[q]));
+    ptr[l + 1] = (int)pos.size();
+  }
+  int4 *d = nullptr;
+  if (!pos.empty()) {
+    SYMAMD_HIP(hipMalloc(&d, sizeof(int4) * pos.size()));
+    if (hipMemcpy(d, pos.data(), sizeof(int4) * pos.size(), hipMemcpyHostToDevice) !=
+        hipSuccess) {
+      (void)hipFree(d);
+      return fail(SYMAMD_ERR_HIP, "split level lists: copy failed");
+    }
+  }
+  auto &e = p->split[thr];
+  e.d_rec = d;
+  e.ptr = std::move(ptr);
+  *out = &e;
+  return SYMAMD_OK;
+}
 
 int prepare_workspace(symamd_handle h, symamd_plan p, int k) {
   if (!p->d_ipos) {

@@ -185,6 +185,72 @@ def test_sptrsm_multilane_long_rows(dev, k, lanes, layout):
         assert rel_err(X, X_ref) <= RTOL64, (ex, k, lanes, layout)
 
 
+@pytest.mark.parametrize("k", [1, 5, 64, 130])
+@pytest.mark.parametrize("layout", ["col", "row"])
+def test_level_one_lane_long_rows(dev, k, layout):
+    """The one-lane-per-row level path on rows of up to 257 entries (many
+    16-entry chunks, a partial last chunk, rows of one entry), column tiles
+    side by side (k = 130: three 64-column tiles, the last one partial):
+    bit-identical to the serial solve, eager and replayed from a graph."""
+    n = 6000
+    Lp, Li, Lx = W.zipf_lower(n, 1.8, 256, 3000, 9)
+    assert int(np.diff(Lp).max()) > 64 and int(np.diff(Lp).min()) == 1
+    B = W.uniform(17, 0, n * k).reshape(k, n)
+    X_ref = oracle.sptrsm_csr(Lp, Li, Lx, B)
+    Bl = B if layout == "col" else np.ascontiguousarray(B.T)
+    plan = _plan(Lp, Li, Lx, dev)
+    plan.set_lanes(1)
+    for ex in ("levelset", "graph", "graph"):
+        dB = to_dev(Bl.copy(), device=dev)
+        print(f"[level-one-lane] k={k} layout={layout} executor={ex}", flush=True)
+        plan.solve(dB, layout=layout, executor=ex)
+        plan.check()
+        X = dB.cpu().numpy()
+        assert bitwise_equal(X.T if layout == "row" else X, X_ref), (ex, k, layout)
+
+
+@pytest.mark.parametrize("k", [1, 5, 64, 130])
+@pytest.mark.parametrize("layout", ["col", "row"])
+def test_split_level_executor(dev, k, layout):
+    """The split level executor: rows with more than 32 off-diagonal entries
+    solved by a 1024-thread workgroup (sums re-associated: the 1e-12 bar),
+    the others one lane per row and column.  A matrix with no long row is
+    bit-identical.  The solve is captured once and replayed."""
+    mats = {"zipf": W.zipf_lower(6000, 1.8, 256, 3000, 9), "grid": W.ic0_lap3d(12, 0.01)}
+    for name, (Lp, Li, Lx) in mats.items():
+        n = len(Lp) - 1
+        has_long = bool((np.diff(Lp) - 1 > 32).any())
+        assert has_long == (name == "zipf")
+        B = W.uniform(19 + k, 0, n * k).reshape(k, n)
+        X_ref = oracle.sptrsm_csr(Lp, Li, Lx, B)
+        Bl = B if layout == "col" else np.ascontiguousarray(B.T)
+        plan = _plan(Lp, Li, Lx, dev)
+        for rep in range(3):
+            dB = to_dev(Bl.copy(), device=dev)
+            print(f"[split] {name} n={n} k={k} layout={layout} solve {rep}", flush=True)
+            plan.solve(dB, layout=layout, executor="split")
+            plan.check()
+            X = dB.cpu().numpy()
+            X = X.T if layout == "row" else X
+            if has_long:
+                assert rel_err(X, X_ref) <= RTOL64, (name, k, layout, rep)
+            else:
+                assert bitwise_equal(X, X_ref), (name, k, layout, rep)
+
+
+def test_tuner_considers_the_split_level_executor(dev):
+    Lp, Li, Lx = W.zipf_lower(6000, 1.8, 256, 3000, 9)
+    plan = _plan(Lp, Li, Lx, dev)
+    log = plan.tune(k=64, layout="col")
+    assert "exec=6" in log
+    n = len(Lp) - 1
+    B = W.uniform(71, 0, n * 64).reshape(64, n)
+    X = to_dev(B.copy(), device=dev)
+    plan.solve(X, layout="col")
+    plan.check()
+    assert rel_err(X.cpu().numpy(), oracle.sptrsm_csr(Lp, Li, Lx, B)) <= RTOL64
+
+
 # ---------------------------------------------------------------- batched
 def test_batched_small(dev):
     w = W.config4(count=24, lo=8, hi=40, seed=3)
@@ -247,3 +313,65 @@ def test_structure_errors(dev):
     with pytest.raises(sa.SymAmdError) as e:
         _plan(Lp, Li, zero, dev)
     assert e.value.code == 3
+
+
+WIDE_DISABLED = ("the persistent workgroup-per-row executor exceeded the watchdog on its "
+                 "first run (zipf n=4000, k=1, row-major, cooperative round-robin deal); "
+                 "no cause was established, so the executor is disabled")
+
+
+def test_wide_row_executor_is_disabled(dev):
+    Lp, Li, Lx = W.lap2d_tril(1, 40)
+    plan = sa.Plan(*to_dev(Lp, Li, Lx, device=dev))
+    X = to_dev(np.ones(len(Lp) - 1), device=dev)
+    with pytest.raises(sa.SymAmdError) as e:
+        plan.solve(X, executor="wide")
+    assert e.value.code == 7
+
+
+@pytest.mark.skip(reason=WIDE_DISABLED)
+@pytest.mark.parametrize("k", [1, 5, 32, 64, 128, 130])
+@pytest.mark.parametrize("layout", ["row", "col"])
+def test_wide_row_executor(dev, k, layout):
+    # one workgroup per row, all k columns, the row's entries split over four
+    # wavefronts (sums re-associated: the 1e-12 bar), on matrices with long,
+    # uneven rows, a plain grid and a chain; two solves (next epoch of the
+    # row tags) and a graph replay (ticketed deal)
+    import torch
+    mats = {"zipf": W.zipf_lower(4000, 1.8, 256, 1500, 7),
+            "grid": W.ic0_lap3d(12, 0.01), "chain": W.lap2d_tril(1, 300)}
+    for name, (Lp, Li, Lx) in mats.items():
+        n = len(Lp) - 1
+        plan = sa.Plan(*to_dev(Lp, Li, Lx, device=dev))
+        B = W.uniform(61 + k, 0, n * k).reshape(k, n)
+        X_ref = oracle.sptrsm_csr(Lp, Li, Lx, B)
+        print(f"[wide] {name} n={n} k={k} layout={layout}", flush=True)
+        Bl = np.ascontiguousarray(B.T) if layout == "row" else B
+        for _ in range(2):
+            X = to_dev(Bl.copy(), device=dev)
+            plan.solve(X, layout=layout, executor="wide")
+            plan.check()
+            got = X.cpu().numpy()
+            assert rel_err(got.T if layout == "row" else got, X_ref) <= RTOL64, name
+        X = to_dev(Bl.copy(), device=dev)
+        g = plan.graph(X, layout=layout, executor="wide")
+        for _ in range(3):
+            X.copy_(to_dev(Bl, device=dev))
+            g.launch()
+            plan.check()
+            got = X.cpu().numpy()
+            assert rel_err(got.T if layout == "row" else got, X_ref) <= RTOL64, name
+
+
+@pytest.mark.skip(reason=WIDE_DISABLED)
+def test_tuner_considers_the_wide_row_executor(dev):
+    Lp, Li, Lx = W.zipf_lower(4000, 1.8, 256, 1500, 7)
+    plan = sa.Plan(*to_dev(Lp, Li, Lx, device=dev))
+    log = plan.tune(k=64, layout="col")
+    assert "exec=5" in log
+    n = len(Lp) - 1
+    B = W.uniform(71, 0, n * 64).reshape(64, n)
+    X = to_dev(B.copy(), device=dev)
+    plan.solve(X, layout="col")
+    plan.check()
+    assert rel_err(X.cpu().numpy(), oracle.sptrsm_csr(Lp, Li, Lx, B)) <= RTOL64
