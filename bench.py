@@ -353,6 +353,40 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def dropin_times(W, Lp, Li, Lx, b):
+    """sym_lib::sptrsv_csr bound by its mangled C++ name and called on host
+    arrays, as a program relinked against the library calls it (wall clock,
+    PCIe copies included): the first call on a pattern (upload, analysis,
+    solve, download) and later calls with new values on the same pattern
+    (the kept analysis: the pattern compared, values and right-hand side
+    uploaded, values gathered, solve, download).  For the 100 x 100 grid
+    Laplacian (the reference demo's scale) and the headline factor."""
+    import ctypes
+    lib = ctypes.CDLL(os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                                   "sympiler_amd", "libsympiler_amd.so"))
+    fn = lib["_ZN7sym_lib10sptrsv_csrEiPiS0_PdS1_"]
+    P = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    w1 = W.config1()
+    out = {}
+    for name, (p, i, v, bb) in (("lap2d_100x100", (w1.Lp, w1.Li, w1.Lx, w1.b)),
+                                ("ic0_lap3d_128", (Lp, Li, Lx, b))):
+        n = len(p) - 1
+        vals = [v, v * 1.5]               # alternate values on the same pattern
+        x = bb.copy()
+        t0 = time.perf_counter()
+        fn(ctypes.c_int(n), P(p), P(i), P(vals[0]), P(x))
+        first = time.perf_counter() - t0
+        times = []
+        for r in range(6):
+            x = bb.copy()
+            t0 = time.perf_counter()
+            fn(ctypes.c_int(n), P(p), P(i), P(vals[(r + 1) % 2]), P(x))
+            times.append(time.perf_counter() - t0)
+        out[name] = {"n": n, "nnz": int(p[-1]), "first_call_ms": round(1e3 * first, 3),
+                     "repeat_call_ms": round(1e3 * float(np.median(times)), 3)}
+    return out
+
+
 def timed_steps(fn, reset, steps, warmup, stream, torch):
     """Run warmup + steps; each step: reset (untimed), sync, fn, sync.
     Returns (wall seconds per step list, event ms per step list)."""
@@ -580,6 +614,10 @@ def main():
             log(f"col-major k={k}: {detail['col_major_k']}")
             del Bc, Xc
 
+    if not args.no_detail:
+        detail["dropin_sptrsv_csr"] = dropin_times(W, Lp, Li, Lx, Bcols[0])
+        log(f"drop-in sptrsv_csr: {detail['dropin_sptrsv_csr']}")
+
     if not args.no_detail and not args.no_configs:
         detail["configs"] = other_configs(torch, sa, W, dev, stream)
 
@@ -714,6 +752,19 @@ def cpu_baseline(Lp, Li, Lx, Bcols, Xgpu, args, k):
     t0 = time.perf_counter()
     blk(sup8, nr8, Lp8, Li8, Lx8, b8)
     sn_ms = round(1e3 * (time.perf_counter() - t0), 1)
+    # the reference's serial sptrsv_csr on the drop-in timing's two matrices
+    # (what a relinked call replaces)
+    serial = {}
+    w1 = W.config1()
+    ser = oracle.ref.sptrsv_csr if kind == "reference" else oracle.sptrsv_csr
+    for name, (p, i, v, bb) in (("lap2d_100x100", (w1.Lp, w1.Li, w1.Lx, w1.b)),
+                                ("ic0_lap3d_128", (Lp, Li, Lx, Bcols[0]))):
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            ser(p, i, v, bb)
+            ts.append(time.perf_counter() - t0)
+        serial[name] = round(1e3 * min(ts), 3)
     log(f"cpu baseline ({kind}, {ncores} threads): "
         f"{rate:.2f} RHS-solves/s over {done} solves; GPU vs CPU max rel err {worst:.2e}")
     return {"value": round(rate, 3), "unit": "RHS-solves/s",
@@ -725,7 +776,8 @@ def cpu_baseline(Lp, Li, Lx, Bcols, Xgpu, args, k):
             "gpu_vs_cpu_max_rel_err": worst,
             "ic0_spic0_csc_left_128cubed_ms_1_thread": ic0_ms,
             "ilu0_port_spilu0_csr_grid3d_128cubed_ms_1_thread": ilu_ms,
-            "sptrsv_csc_block_tree13_ms_1_thread": sn_ms}
+            "sptrsv_csc_block_tree13_ms_1_thread": sn_ms,
+            "sptrsv_csr_ms_1_thread": serial}
 
 
 if __name__ == "__main__":

@@ -1,9 +1,13 @@
 // dropin.cpp -- sym_lib:: entry points with the reference's signatures
 // (include/sympiler/sparse_blas_lib.h), implemented on the C ABI.
 //
-// Each call is blocking and stateless, as in the reference: host arrays are
-// copied to the device, the plan is built, the HIP kernels run with one lane
-// per row (bit-identical to the serial reference), and x is copied back.
+// Each call is blocking, as in the reference: host arrays are copied to the
+// device, the HIP kernels run with one lane per row (bit-identical to the
+// serial reference), and x is copied back.  The analysis of a matrix is kept
+// (per thread, the DROPIN_CACHE most recently used patterns): a later call
+// whose n, Lp and Li equal a kept pattern's exactly only uploads the values
+// and the right-hand side, as an iterative solver calling the same factor's
+// solve every step does.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -60,6 +64,96 @@ struct Plan {
   }
 };
 
+// ------------------------------------------------------- analysis cache
+// A kept analysis: the pattern (host copies, for exact matching), its device
+// arrays, a right-hand-side buffer and the plan.
+struct Kept {
+  int n = 0, nnz = 0;
+  std::vector<int> Lp, Li;
+  int *dLp = nullptr, *dLi = nullptr;
+  double *dLx = nullptr, *dx = nullptr;
+  symamd_plan plan = nullptr;
+  uint64_t used = 0;
+};
+constexpr int DROPIN_CACHE = 4;
+
+// Per thread; never destroyed at exit (the device memory goes with the
+// process, and no HIP call runs while the runtime shuts down).
+std::vector<Kept *> &kept() {
+  thread_local std::vector<Kept *> *k = new std::vector<Kept *>();
+  return *k;
+}
+
+void release(Kept *e) {
+  if (e->plan) (void)symamd_plan_destroy(e->plan);
+  void *bufs[] = {e->dLp, e->dLi, e->dLx, e->dx};
+  for (void *b : bufs)
+    if (b) (void)hipFree(b);
+  delete e;
+}
+
+// The kept analysis of (n, Lp, Li) with the values Lx (and, when x is given,
+// the right-hand side) uploaded; analysed now if no kept pattern matches,
+// evicting the least recently used one beyond DROPIN_CACHE.
+Kept *analysis(symamd_handle h, int n, const int *Lp, const int *Li, const double *Lx,
+               const double *x) {
+  thread_local uint64_t clock = 0;
+  auto &ks = kept();
+  const int nnz = Lp[n];
+  Kept *e = nullptr;
+  for (Kept *c : ks)
+    if (c->n == n && c->nnz == nnz && std::memcmp(c->Lp.data(), Lp, sizeof(int) * (n + 1)) == 0 &&
+        std::memcmp(c->Li.data(), Li, sizeof(int) * nnz) == 0) {
+      e = c;
+      break;
+    }
+  if (e) {
+    hcheck(hipMemcpy(e->dLx, Lx, sizeof(double) * (nnz ? nnz : 1), hipMemcpyHostToDevice),
+           "upload");
+    check(symamd_plan_update_values(h, e->plan, e->dLx), "update values");
+  } else {
+    if ((int)ks.size() >= DROPIN_CACHE) {
+      auto lru = std::min_element(ks.begin(), ks.end(),
+                                  [](const Kept *a, const Kept *b) { return a->used < b->used; });
+      release(*lru);
+      ks.erase(lru);
+    }
+    auto *c = new Kept();
+    c->n = n;
+    c->nnz = nnz;
+    c->Lp.assign(Lp, Lp + n + 1);
+    c->Li.assign(Li, Li + nnz);
+    const hipError_t a0 = hipMalloc(&c->dLp, sizeof(int) * (n + 1));
+    const hipError_t a1 = hipMalloc(&c->dLi, sizeof(int) * (nnz ? nnz : 1));
+    const hipError_t a2 = hipMalloc(&c->dLx, sizeof(double) * (nnz ? nnz : 1));
+    const hipError_t a3 = hipMalloc(&c->dx, sizeof(double) * n);
+    if (a0 != hipSuccess || a1 != hipSuccess || a2 != hipSuccess || a3 != hipSuccess) {
+      release(c);
+      throw std::runtime_error("sympiler_amd: device allocation for a drop-in call failed");
+    }
+    int rc = hipMemcpy(c->dLp, Lp, sizeof(int) * (n + 1), hipMemcpyHostToDevice) == hipSuccess &&
+                     hipMemcpy(c->dLi, Li, sizeof(int) * nnz, hipMemcpyHostToDevice) ==
+                         hipSuccess &&
+                     hipMemcpy(c->dLx, Lx, sizeof(double) * nnz, hipMemcpyHostToDevice) ==
+                         hipSuccess
+                 ? SYMAMD_OK
+                 : SYMAMD_ERR_HIP;
+    if (rc == SYMAMD_OK) rc = symamd_analyze(h, n, c->dLp, c->dLi, c->dLx, SYMAMD_F64, &c->plan);
+    if (rc == SYMAMD_OK) rc = symamd_plan_set_lanes(c->plan, 1);
+    if (rc != SYMAMD_OK) {
+      const std::string msg = rc == SYMAMD_ERR_HIP && !c->plan ? "upload failed"
+                                                               : symamd_last_error();
+      release(c);
+      throw std::runtime_error("sympiler_amd analyze: " + msg);
+    }
+    ks.push_back(c);
+    e = c;
+  }
+  e->used = ++clock;
+  if (x) hcheck(hipMemcpy(e->dx, x, sizeof(double) * n, hipMemcpyHostToDevice), "upload");
+  return e;
+}
+
 // CSC (diagonal first) -> CSR of the same lower triangle (diagonal last).
 void csc_to_csr(int n, const int *Cp, const int *Ci, const double *Cx,
                 std::vector<int> &Rp, std::vector<int> &Ri,
@@ -84,15 +178,10 @@ void solve_csr_host(int n, const int *Lp, const int *Li, const double *Lx,
                     double *x) {
   if (n <= 0) return;
   symamd_handle h = handle();
-  const int nnz = Lp[n];
-  DevBuf<int> dLp(n + 1, Lp), dLi(nnz, Li);
-  DevBuf<double> dLx(nnz, Lx), dx(n, x);
-  Plan pl;
-  check(symamd_analyze(h, n, dLp.p, dLi.p, dLx.p, SYMAMD_F64, &pl.p), "analyze");
-  check(symamd_plan_set_lanes(pl.p, 1), "set_lanes");
-  check(symamd_solve(h, pl.p, dx.p, 1, n, SYMAMD_COL_MAJOR, SYMAMD_EXEC_SYNCFREE), "solve");
-  check(symamd_plan_check(h, pl.p), "solve");
-  dx.download(x, n);
+  Kept *e = analysis(h, n, Lp, Li, Lx, x);
+  check(symamd_solve(h, e->plan, e->dx, 1, n, SYMAMD_COL_MAJOR, SYMAMD_EXEC_SYNCFREE), "solve");
+  check(symamd_plan_check(h, e->plan), "solve");
+  hcheck(hipMemcpy(x, e->dx, sizeof(double) * n, hipMemcpyDeviceToHost), "download");
 }
 
 // Checks a caller level set: every row once, dependencies in earlier levels.
@@ -402,17 +491,13 @@ int ltsolve(int n, int *Lp, int *Li, double *Lx, double *x) {
   std::vector<double> Rx;
   csc_to_csr(n, Lp, Li, Lx, Rp, Ri, Rx);
   symamd_handle h = handle();
-  const int nnz = Rp[n];
-  DevBuf<int> dLp(n + 1, Rp.data()), dLi(nnz, Ri.data());
-  DevBuf<double> dLx(nnz, Rx.data()), dx(n, x), dzero(n);
-  Plan pl;
-  check(symamd_analyze(h, n, dLp.p, dLi.p, dLx.p, SYMAMD_F64, &pl.p), "analyze");
-  check(symamd_plan_set_lanes(pl.p, 1), "set_lanes");
-  // gb = L^-T gx with gx = b, in place; x is not used when gLx is NULL
-  check(symamd_adjoint(h, pl.p, dzero.p, dx.p, dx.p, nullptr, SYMAMD_EXEC_SYNCFREE),
+  Kept *e = analysis(h, n, Rp.data(), Ri.data(), Rx.data(), x);
+  // gb = L^-T gx with gx = b, in place; the forward solution argument is
+  // not read when no grad-L is asked for
+  check(symamd_adjoint(h, e->plan, e->dx, e->dx, e->dx, nullptr, SYMAMD_EXEC_SYNCFREE),
         "ltsolve");
-  check(symamd_plan_check(h, pl.p), "ltsolve");
-  dx.download(x, n);
+  check(symamd_plan_check(h, e->plan), "ltsolve");
+  hcheck(hipMemcpy(x, e->dx, sizeof(double) * n, hipMemcpyDeviceToHost), "download");
   return 1;
 }
 
@@ -436,14 +521,12 @@ int ic0_csc(int n, const int *Ap, const int *Ai, const double *Ax, symamd_ic0_va
   for (int c = 0; c < n; ++c)
     for (int p = Ap[c]; p < Ap[c + 1]; ++p) pos[p] = fill[Ai[p]]++;
   symamd_handle h = handle();
-  DevBuf<int> dLp(n + 1, Rp.data()), dLi(nnz, Ri.data());
-  DevBuf<double> dA(nnz, Rx.data());
-  Plan pl;
-  check(symamd_analyze(h, n, dLp.p, dLi.p, dA.p, SYMAMD_F64, &pl.p), "analyze");
+  Kept *e = analysis(h, n, Rp.data(), Ri.data(), Rx.data(), nullptr);
   int bad = -1;
-  const int rc = symamd_ic0(h, pl.p, variant, dA.p, dA.p, &bad);
+  // the factor overwrites the kept value buffer (every call uploads anew)
+  const int rc = symamd_ic0(h, e->plan, variant, e->dLx, e->dLx, &bad);
   if (rc != SYMAMD_ERR_NOT_SPD) check(rc, "ic0");
-  dA.download(Rx.data(), nnz);
+  hcheck(hipMemcpy(Rx.data(), e->dLx, sizeof(double) * nnz, hipMemcpyDeviceToHost), "download");
   out.resize(nnz);
   for (int p = 0; p < nnz; ++p) out[p] = Rx[pos[p]];
   return bad;

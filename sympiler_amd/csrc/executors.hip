@@ -451,18 +451,19 @@ __global__ __launch_bounds__(256) void k_level(SolveArgs<T> a, int q0, int q1) {
 // One launch per level, one lane per row and column (forward solves): a
 // wave solves 64/KT consecutive rows of [q0, q1) for column tile
 // blockIdx.y, so the tiles of a row run side by side instead of one after
-// another.  Each lane keeps LC_DEPS dependency values in flight and has the
+// another.  Each lane keeps DC dependency values in flight and has the
 // column ids of the next chunk loading meanwhile; the products are
 // subtracted in ascending entry order, the serial solve's order, so the
 // result is bit-identical.  With KT == 64 a wave holds one row and every
-// row-structure load is wave-uniform.
-constexpr int LC_DEPS = 16;
+// row-structure load is wave-uniform.  DC as for the unit executors (4, or
+// 8 for tiles of 32 or more columns): deeper chunks only add dead loads on
+// short rows, whose scattered addresses cost throughput.
 
 // Row q (level-ordered position) for one column: x = (b - sum L*x) / diag
-// with the products subtracted in entry order.  Rows with more than
-// `max_off` off-diagonal entries are left alone (another workgroup solves
-// them).
-template <typename T>
+// with the products subtracted in entry order, DC loads in flight.  Rows
+// with more than `max_off` off-diagonal entries are left alone (another
+// workgroup solves them).
+template <typename T, int DC>
 __device__ __forceinline__ void serial_row(const SolveArgs<T> &a, int q, bool rowok,
                                            bool colok, int64_t coff, int max_off) {
   int i = 0, jb = 0, je = 0;
@@ -479,25 +480,25 @@ __device__ __forceinline__ void serial_row(const SolveArgs<T> &a, int q, bool ro
     acc = a.B[(int64_t)i * a.rs + coff];
   }
   const int64_t own = (int64_t)i * a.rs + coff;  // a valid address for dead slots
-  int d[LC_DEPS];
+  int d[DC];
 #pragma unroll
-  for (int t = 0; t < LC_DEPS; ++t) d[t] = a.ci[jb + t < je ? jb + t : je];
-  for (int e0 = jb; e0 < je; e0 += LC_DEPS) {
-    T xv[LC_DEPS], vv[LC_DEPS];
+  for (int t = 0; t < DC; ++t) d[t] = a.ci[jb + t < je ? jb + t : je];
+  for (int e0 = jb; e0 < je; e0 += DC) {
+    T xv[DC], vv[DC];
 #pragma unroll
-    for (int t = 0; t < LC_DEPS; ++t) {
+    for (int t = 0; t < DC; ++t) {
       const bool live = e0 + t < je;
       xv[t] = a.X[live ? (int64_t)d[t] * a.rs + coff : own];
       vv[t] = a.val[live ? e0 + t : je];
     }
     // the next chunk's column ids, loading while this chunk is summed
 #pragma unroll
-    for (int t = 0; t < LC_DEPS; ++t) {
-      const int e = e0 + LC_DEPS + t;
+    for (int t = 0; t < DC; ++t) {
+      const int e = e0 + DC + t;
       d[t] = a.ci[e < je ? e : je];
     }
 #pragma unroll
-    for (int t = 0; t < LC_DEPS; ++t) {
+    for (int t = 0; t < DC; ++t) {
       const T prod = vv[t] * xv[t];
       acc = e0 + t < je ? acc - prod : acc;
     }
@@ -515,7 +516,8 @@ __global__ __launch_bounds__(256) void k_level_cols(SolveArgs<T> a, int q0, int 
   if constexpr (RPW == 1) q = __builtin_amdgcn_readfirstlane(q);
   const int col = (int)blockIdx.y * KT + lane % KT;
   const bool colok = col < a.k;
-  serial_row<T>(a, q, q < q1, colok, (int64_t)(colok ? col : 0) * a.cs, 0x7fffffff);
+  serial_row<T, dep_chunk<1, KT>()>(a, q, q < q1, colok, (int64_t)(colok ? col : 0) * a.cs,
+                                    0x7fffffff);
 }
 
 // Split level executor: one launch per level.  Blocks [0, nlong) each solve
@@ -529,6 +531,7 @@ __global__ __launch_bounds__(256) void k_level_cols(SolveArgs<T> a, int q0, int 
 // blockIdx.y.  (Eight wavefronts per group measured slower at k = 32 and
 // 128: fewer slices per long row.)
 constexpr int SPLIT_W = 16;
+constexpr int LC_DEPS = 16;
 
 template <typename T, int KT>
 __device__ __forceinline__ void split_rows(const SolveArgs<T> &a, int q0, int q1,
@@ -545,7 +548,7 @@ __device__ __forceinline__ void split_rows(const SolveArgs<T> &a, int q0, int q1
     if (qw >= q1) return;  // whole wave past the level
     int q = qw + lane / KT;
     if constexpr (SPW == 1) q = __builtin_amdgcn_readfirstlane(q);
-    serial_row<T>(a, q, q < q1, colok, coff, max_off);
+    serial_row<T, LC_DEPS>(a, q, q < q1, colok, coff, max_off);
     return;
   }
   constexpr int S = SPLIT_W * SPW;  // entry slices of the row

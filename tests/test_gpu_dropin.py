@@ -85,6 +85,49 @@ def test_ltsolve_and_csc_levelset(L):
     assert bitwise_equal(x, oracle.sptrsv_csc(Cp, Ci, Cx, b))
 
 
+def test_dropins_reuse_an_analysis_only_for_the_same_pattern(L):
+    """Drop-in calls keep the analysis of the most recently used patterns:
+    a repeated pattern with new values or a new right-hand side, a pattern
+    changed in place in the caller's arrays (same n and nnz), more patterns
+    than are kept (least recently used evicted), and the transposed solve
+    on a kept pattern all match the serial solve bit for bit."""
+    fwd = L["_ZN7sym_lib10sptrsv_csrEiPiS0_PdS1_"]
+    n = 2500
+    Lp, Li, Lx = W.random_lower(n, 0.003, 1, seed=31)
+    for rep in range(3):
+        Lx_r = Lx * (1.0 + 0.25 * rep)          # same pattern, new values
+        b = W.uniform(40 + rep, 0, n)
+        x = b.copy()
+        fwd(c_int(n), P(Lp), P(Li), P(Lx_r), P(x))
+        assert bitwise_equal(x, oracle.sptrsv_csr(Lp, Li, Lx_r, b)), rep
+    # move one dependency of some row to another column, in place
+    Li2 = Li
+    i = next(r for r in range(n) if Lp[r + 1] - Lp[r] == 2 and Li[Lp[r]] > 0)
+    old = Li2[Lp[i]]
+    Li2[Lp[i]] = old - 1
+    b = W.uniform(50, 0, n)
+    x = b.copy()
+    fwd(c_int(n), P(Lp), P(Li2), P(Lx), P(x))
+    assert bitwise_equal(x, oracle.sptrsv_csr(Lp, Li2, Lx, b))
+    Li2[Lp[i]] = old
+    # more patterns than are kept, then the first again
+    mats = [W.random_lower(n + 7 * t, 0.003, 1, seed=60 + t) for t in range(6)]
+    for t, (Lq, Lj, Lv) in enumerate(mats + mats[:1]):
+        m = len(Lq) - 1
+        b = W.uniform(70 + t, 0, m)
+        x = b.copy()
+        fwd(c_int(m), P(Lq), P(Lj), P(Lv), P(x))
+        assert bitwise_equal(x, oracle.sptrsv_csr(Lq, Lj, Lv, b)), t
+    # the transposed solve on a kept pattern with new values
+    Cp, Ci, Cx, _ = oracle.csr_to_csc(Lp, Li, Lx)
+    for rep in range(2):
+        Cx_r = Cx * (2.0 - 0.5 * rep)
+        b = W.uniform(80 + rep, 0, n)
+        x = b.copy()
+        assert L["_ZN7sym_lib7ltsolveEiPiS0_PdS1_"](c_int(n), P(Cp), P(Ci), P(Cx_r), P(x)) == 1
+        assert bitwise_equal(x, oracle.ltsolve(Cp, Ci, Cx_r, b)), rep
+
+
 def test_spmv_known_answers(L):
     fn = L["_ZN7sym_lib8spmv_csrEiPKiS1_PKdS3_Pd"]
     Ap = np.array([0, 2, 3, 4, 6], np.int32)
