@@ -106,9 +106,13 @@ def run_leg(args):
         plan = sa.Plan(up(w3.Lp), up(w3.Li), up(w3.Lx))
         B0 = up(W.uniform(2, 0, w3.n * k).reshape(k, w3.n))
         X = torch.empty_like(B0)
-        out.update(workload="zipf_n1e6", k=k,
-                   tuned=plan.tune(k=k, layout="col").strip().splitlines()[-1])
-        fn, reset = (lambda: plan.solve(X, layout="col")), (lambda: X.copy_(B0))
+        out.update(workload="zipf_n1e6", k=k)
+        if args.executor == "auto":
+            out["tuned"] = plan.tune(k=k, layout="col").strip().splitlines()[-1]
+        else:
+            out["executor"] = args.executor
+        fn = lambda: plan.solve(X, layout="col", executor=args.executor)
+        reset = lambda: X.copy_(B0)
         nbytes = W.algorithmic_bytes_forward(w3.n, w3.nnz, k)
     else:
         w4 = W.config4()
@@ -701,7 +705,7 @@ def cpu_baseline(Lp, Li, Lx, Bcols, Xgpu, args, k):
     Xh = Xgpu.cpu().numpy()
     Xh = Xh.T if args.layout == "row" else Xh
     t_total, done, worst = 0.0, 0, 0.0
-    while t_total < args.cpu_seconds and done < 4 * k:
+    while t_total < args.cpu_seconds and done < 64 * k:
         c = done % k
         b = Bcols[c]
         t0 = time.perf_counter()
