@@ -238,6 +238,55 @@ def test_split_level_executor(dev, k, layout):
                 assert bitwise_equal(X, X_ref), (name, k, layout, rep)
 
 
+def _crowded_level(roots=40, wide=400, tail=30, seed=5):
+    """A lower triangle whose level 1 holds `wide` rows that each read all
+    `roots` rows of level 0 (more long rows in one level than a launch's
+    arguments carry), followed by `tail` rows reading a few of them."""
+    rng = np.random.default_rng(seed)
+    rows = []
+    for i in range(roots):
+        rows.append(([], 8.0))
+    for i in range(roots, roots + wide):
+        rows.append((list(range(roots)), 60.0))
+    for i in range(roots + wide, roots + wide + tail):
+        rows.append((sorted(rng.choice(np.arange(roots, roots + wide), 20, replace=False).tolist()), 30.0))
+    Lp, Li, Lx = [0], [], []
+    for i, (cols, dg) in enumerate(rows):
+        Li.extend(cols)
+        Lx.extend(rng.uniform(-1.0, 1.0, len(cols)).tolist())
+        Li.append(i)
+        Lx.append(dg)
+        Lp.append(len(Li))
+    return (np.array(Lp, np.int32), np.array(Li, np.int32), np.array(Lx, np.float64))
+
+
+@pytest.mark.parametrize("layout", ["col", "row"])
+def test_split_level_executor_crowded_level_and_value_update(dev, layout):
+    """More long rows in one level than the launch arguments carry (the rest
+    read from the plan's list), both thresholds, then new values on the same
+    plan (the padded short rows are refreshed)."""
+    Lp, Li, Lx = _crowded_level()
+    n = len(Lp) - 1
+    k = 24
+    B = W.uniform(91, 0, n * k).reshape(k, n)
+    Bl = B if layout == "col" else np.ascontiguousarray(B.T)
+    h = sa.Handle()
+    plan = sa.Plan(*to_dev(Lp, Li, Lx, device=dev), handle=h)
+    for thr in (16, 32):
+        h.set_split_long(thr)
+        for vals in (Lx, Lx * 1.25):
+            plan.update_values(to_dev(vals, device=dev))
+            X_ref = oracle.sptrsm_csr(Lp, Li, vals, B)
+            dB = to_dev(Bl.copy(), device=dev)
+            print(f"[split-crowded] thr={thr} layout={layout} scale={vals[-1] / Lx[-1]}",
+                  flush=True)
+            plan.solve(dB, layout=layout, executor="split")
+            plan.check()
+            X = dB.cpu().numpy()
+            assert rel_err(X.T if layout == "row" else X, X_ref) <= RTOL64, (thr, layout)
+    h.set_split_long(0)
+
+
 def test_tuner_considers_the_split_level_executor(dev):
     Lp, Li, Lx = W.zipf_lower(6000, 1.8, 256, 3000, 9)
     plan = _plan(Lp, Li, Lx, dev)
