@@ -388,7 +388,7 @@ struct symamd_ilu0_s {
 // arrays on the device, the temporaries' offsets, the per-row lists of the
 // temporaries each row subtracts, the dependency lists, tags and state.
 constexpr int SN_WMAX = 1024;  // widest supernode (LDS of one wave: 8 KiB)
-constexpr int SN_CHUNK = 256;  // rows below a top per task
+constexpr int SN_CHUNK = 128;  // rows below a top per task
 struct symamd_snplan_s {
   int n = 0, m = 0, ntasks = 0, kcap = 1;  // kcap: columns d_T and d_flags hold
   int *d_task = nullptr;       // (supernode, chunk) per task

@@ -72,7 +72,7 @@ __device__ __forceinline__ int matvec_block(int col, int w) {
 
 // Rows per lane processed together in the top's updates and the product
 // below (their loads in flight at once).
-constexpr int SN_ROWS = 4;
+constexpr int SN_ROWS = 2;
 static_assert(SN_CHUNK == 64 * SN_ROWS, "a chunk of rows below is one pass of a wave");
 
 __global__ __launch_bounds__(256) void k_supernodal(SnArgs a) {
