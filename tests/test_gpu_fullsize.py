@@ -4,8 +4,10 @@
     reference's multi-RHS convention, x[idx + j*n] in
     sparse_blas/parsy/triangularSolve/Triangular_BCSC.cpp:114): one lane per
     row bit-identical to the oracle's column-by-column sptrsv_csr
-    (sparse_blas/sptrsv.cpp:17-25); the tuned choice within 1e-12; columns 0
-    and 1 against the digests written from the reference build;
+    (sparse_blas/sptrsv.cpp:17-25) on the sync-free executor and on the
+    one-lane level path replayed from a graph; the split level executor and
+    the tuned choice within 1e-12; columns 0 and 1 against the digests
+    written from the reference build;
   * the 512-system uneven batch on its timed path: the tuner's choice
     captured as a hipGraph and replayed, against the reference-build digest;
   * the 128^3 IC(0) factor on the plain level-set executor (one launch per
@@ -61,6 +63,16 @@ def test_c3_sptrsm_full_size(dev, zipf, k):
     plan.check()
     Xh = X.cpu().numpy()
     assert bitwise_equal(Xh, X_ref), k
+    # the one-lane level path replayed from a graph: bit for bit as well
+    X.copy_(dB)
+    plan.solve(X, layout="col", executor="graph")
+    plan.check()
+    assert bitwise_equal(X.cpu().numpy(), X_ref), ("graph", k)
+    # the split level executor (long rows re-associated): the 1e-12 bar
+    X.copy_(dB)
+    plan.solve(X, layout="col", executor="split")
+    plan.check()
+    assert rel_err(X.cpu().numpy(), X_ref) <= RTOL64, ("split", k)
     # the tuned choice (what the bench times), within the 1e-12 bar
     plan.set_lanes(0)
     summary = plan.tune(k=k, layout="col")
