@@ -238,6 +238,31 @@ def test_split_level_executor(dev, k, layout):
                 assert bitwise_equal(X, X_ref), (name, k, layout, rep)
 
 
+@pytest.mark.parametrize("k", [1, 8, 40])
+def test_split_level_executor_fp32(dev, k):
+    """FP32: without long rows the split executor performs the one-lane
+    level path's float operations in the same order (bit for bit); with
+    long rows it stays within FP32 rounding of the FP64 oracle."""
+    import torch
+    for name, (Lp, Li, Lx) in (("grid", W.ic0_lap3d(12, 0.01)),
+                               ("zipf", W.zipf_lower(6000, 1.8, 256, 3000, 9))):
+        n = len(Lp) - 1
+        B = W.uniform(23 + k, 0, n * k).reshape(k, n)
+        plan = _plan(Lp, Li, Lx, dev, dtype=torch.float32)
+        plan.set_lanes(1)
+        out = {}
+        for ex in ("split", "levelset"):
+            dB = to_dev(B.copy(), device=dev).to(torch.float32)
+            print(f"[split-fp32] {name} k={k} {ex}", flush=True)
+            plan.solve(dB, layout="col", executor=ex)
+            plan.check()
+            out[ex] = dB
+        if name == "grid":
+            assert torch.equal(out["split"].view(torch.int32), out["levelset"].view(torch.int32))
+        X_ref = oracle.sptrsm_csr(Lp, Li, Lx, B)
+        assert rel_err(out["split"].double().cpu().numpy(), X_ref) <= 1e-4, name
+
+
 def _crowded_level(roots=40, wide=400, tail=30, seed=5):
     """A lower triangle whose level 1 holds `wide` rows that each read all
     `roots` rows of level 0 (more long rows in one level than a launch's
