@@ -22,6 +22,7 @@
 #include <cstdint>
 #include <cstring>
 #include <numeric>
+#include <thread>
 #include <vector>
 
 #include "common.h"
@@ -146,6 +147,24 @@ int cluster_regions(int n, const int *Lp, const int *Li, int max_rows,
 }
 
 namespace {
+
+// f(part, lo, hi) over contiguous ranges of [0, count) on up to 16 host
+// threads (one when count is small); parts are numbered in range order.
+template <typename F>
+int parallel_ranges(int count, F f) {
+  int T = (int)std::thread::hardware_concurrency();
+  T = std::max(1, std::min({T, 16, count / 64}));
+  if (T == 1) {
+    f(0, 0, count);
+    return 1;
+  }
+  std::vector<std::thread> th;
+  th.reserve(T);
+  for (int t = 0; t < T; ++t)
+    th.emplace_back(f, t, (int)((int64_t)count * t / T), (int)((int64_t)count * (t + 1) / T));
+  for (auto &x : th) x.join();
+  return T;
+}
 
 // Stable counting sort of `items` by key(item) in [0, nkeys).
 template <typename Key>
@@ -563,7 +582,9 @@ bool build_region_schedule(int n, int64_t nnz, int levels, const int *lvl_ptr,
   }
   // halos (ordered by the band that first reads them, then row), entries,
   // and the waits of each band: (rank of a region read, bands of it that
-  // must be published), emitted where a region's requirement first rises
+  // must be published), emitted where a region's requirement first rises.
+  // Regions are independent: host threads take contiguous ranges of ranks,
+  // each with its own output lists, joined in rank order afterwards.
   std::vector<int> hptr(R + 1, 0), halo, hband, dptr(R + 1, 0), dlist, wptr;
   // entries: the first RegionPlan::PACK of every row packed per row (pcnt,
   // pcode, psrc; padding slots read the all-zero LDS slot smax + hmax, and
@@ -571,85 +592,125 @@ bool build_region_schedule(int n, int64_t nnz, int levels, const int *lvl_ptr,
   constexpr int PK = RegionPlan::PACK;
   std::vector<int> eptr(n + 1, 0), ecode, esrc, dsrc(n);
   std::vector<int> pcnt(n), pcode((size_t)PK * n, 0), psrc((size_t)PK * n, 0);
-  std::vector<int> hslot(n, -1), hstamp(n, -1), first_use(n, 0), need_max(R, 0),
-      need_stamp(R, -1);
+  struct Part {
+    std::vector<int> halo, hband, wptr, dlist, ecode, esrc;
+    int hmax = 0, emax = 0;
+  };
+  std::vector<Part> parts(16);
+  const int T = parallel_ranges(R, [&](int part, int r_lo, int r_hi) {
+    Part &P = parts[part];
+    std::vector<int> need_max(R, 0), need_stamp(R, -1), touched;
+    std::vector<std::pair<int, int>> uses;   // (column, band of a reading row)
+    std::vector<std::pair<int, int>> horder;  // (first band, column): halo order
+    std::vector<std::pair<int, int>> slot;   // (column, halo slot), by column
+    for (int r = r_lo; r < r_hi; ++r) {
+      const int g = order[r];  // cluster id of rank r
+      const int a = rptr[r], b = rptr[r + 1];
+      const int nb = bptr[r + 1] - bptr[r] - 1;
+      uses.clear();
+      for (int t = a; t < b; ++t) {
+        const int i = rows[t];
+        for (int j = Lp[i]; j < Lp[i + 1] - 1; ++j)
+          if (reg[Li[j]] != g) uses.emplace_back(Li[j], band_of[i]);
+      }
+      std::sort(uses.begin(), uses.end());
+      horder.clear();
+      for (size_t q = 0; q < uses.size(); ++q)
+        if (q == 0 || uses[q].first != uses[q - 1].first)
+          horder.emplace_back(uses[q].second, uses[q].first);  // smallest band first
+      std::sort(horder.begin(), horder.end());
+      slot.resize(horder.size());
+      for (size_t q = 0; q < horder.size(); ++q) slot[q] = {horder[q].second, (int)q};
+      std::sort(slot.begin(), slot.end());
+      for (const auto &o2 : horder) P.halo.push_back(o2.second);
+      hptr[r + 1] = (int)horder.size();  // per region; made cumulative below
+      P.hmax = std::max(P.hmax, (int)horder.size());
+      // halo bounds and waits per band
+      size_t k = 0;
+      const int dl0 = (int)P.dlist.size();
+      for (int bb = 0; bb <= nb; ++bb) {
+        P.hband.push_back((int)k);
+        P.wptr.push_back((int)P.dlist.size() - dl0);
+        if (bb == nb) break;
+        const size_t k0 = k;
+        while (k < horder.size() && horder[k].first == bb) ++k;
+        // the largest requirement of this band per region read, in halo order
+        touched.clear();
+        for (size_t q = k0; q < k; ++q) {
+          const int c = horder[q].second;
+          const int u = rank[reg[c]];
+          const int need = band_of[c] + 1;
+          if (need_stamp[u] != r) {
+            need_stamp[u] = r;
+            need_max[u] = 0;
+          }
+          if (need > need_max[u]) {
+            if (std::find(touched.begin(), touched.end(), u) == touched.end()) touched.push_back(u);
+            need_max[u] = need;
+          }
+        }
+        std::sort(touched.begin(), touched.end());
+        for (int u : touched) P.dlist.push_back(u | (need_max[u] << REGION_RANK_BITS));
+      }
+      dptr[r + 1] = (int)P.dlist.size() - dl0;  // per region; made cumulative below
+      const int e0 = (int)P.ecode.size();
+      for (int t = a; t < b; ++t) {
+        const int i = rows[t];
+        const int q = ipos[i];
+        dsrc[t] = crp[q + 1] - 1;
+        pcnt[t] = Lp[i + 1] - 1 - Lp[i];
+        for (int u = 0; u < PK; ++u) psrc[(size_t)PK * t + u] = dsrc[t];
+        for (int j = Lp[i]; j < Lp[i + 1] - 1; ++j) {
+          const int c = Li[j];
+          const int u = j - Lp[i];
+          int code;
+          if (reg[c] == g) {
+            code = lidx[c];
+          } else {
+            const auto it = std::lower_bound(slot.begin(), slot.end(), std::make_pair(c, INT_MIN));
+            code = smax + it->second;
+          }
+          const int src = crp[q] + u;
+          if (u < PK) {
+            pcode[(size_t)PK * t + u] = code;
+            psrc[(size_t)PK * t + u] = src;
+          } else {
+            P.ecode.push_back(code);
+            P.esrc.push_back(src);
+          }
+        }
+        eptr[t + 1] = (int)P.ecode.size();  // per part; made global below
+      }
+      P.emax = std::max(P.emax, (int)P.ecode.size() - e0);
+    }
+  });
   int hmax = 0, emax = 0;
-  for (int r = 0; r < R; ++r) {
-    const int g = order[r];  // cluster id of rank r
-    const int a = rptr[r], b = rptr[r + 1];
-    const int nb = bptr[r + 1] - bptr[r] - 1;
-    const size_t h0 = halo.size();
-    for (int t = a; t < b; ++t) {
-      const int i = rows[t];
-      for (int j = Lp[i]; j < Lp[i + 1] - 1; ++j) {
-        const int c = Li[j];
-        if (reg[c] == g) continue;
-        if (hstamp[c] != r) {
-          hstamp[c] = r;
-          halo.push_back(c);
-          first_use[c] = band_of[i];
-        } else {
-          first_use[c] = std::min(first_use[c], band_of[i]);
-        }
+  {
+    // join the parts in rank order: list offsets become global
+    std::vector<int> rbound(T + 1);
+    for (int t = 0; t <= T; ++t) rbound[t] = (int)((int64_t)R * t / T);
+    int64_t hbase = 0, dbase = 0, ebase = 0;
+    for (int t = 0; t < T; ++t) {
+      Part &P = parts[t];
+      for (int r = rbound[t]; r < rbound[t + 1]; ++r) {
+        hptr[r + 1] += hptr[r];
+        dptr[r + 1] += dptr[r];
+        for (int q = rptr[r]; q < rptr[r + 1]; ++q) eptr[q + 1] += (int)ebase;
       }
+      halo.insert(halo.end(), P.halo.begin(), P.halo.end());
+      hband.insert(hband.end(), P.hband.begin(), P.hband.end());
+      wptr.insert(wptr.end(), P.wptr.begin(), P.wptr.end());
+      dlist.insert(dlist.end(), P.dlist.begin(), P.dlist.end());
+      ecode.insert(ecode.end(), P.ecode.begin(), P.ecode.end());
+      esrc.insert(esrc.end(), P.esrc.begin(), P.esrc.end());
+      hbase += (int64_t)P.halo.size();
+      dbase += (int64_t)P.dlist.size();
+      ebase += (int64_t)P.ecode.size();
+      hmax = std::max(hmax, P.hmax);
+      emax = std::max(emax, P.emax);
     }
-    std::sort(halo.begin() + h0, halo.end(), [&](int x, int y) {
-      return first_use[x] != first_use[y] ? first_use[x] < first_use[y] : x < y;
-    });
-    for (size_t k = h0; k < halo.size(); ++k) hslot[halo[k]] = (int)(k - h0);
-    hptr[r + 1] = (int)halo.size();
-    hmax = std::max(hmax, hptr[r + 1] - hptr[r]);
-    // halo bounds and waits per band
-    size_t k = h0;
-    const int dl0 = (int)dlist.size();
-    for (int bb = 0; bb <= nb; ++bb) {
-      hband.push_back((int)(k - h0));
-      wptr.push_back((int)dlist.size() - dl0);
-      if (bb == nb) break;
-      const size_t k0 = k;
-      while (k < halo.size() && first_use[halo[k]] == bb) ++k;
-      // the largest requirement of this band per region read, in halo order
-      std::vector<int> touched;
-      for (size_t q = k0; q < k; ++q) {
-        const int c = halo[q];
-        const int u = rank[reg[c]];
-        const int need = band_of[c] + 1;
-        if (need_stamp[u] != r) {
-          need_stamp[u] = r;
-          need_max[u] = 0;
-        }
-        if (need > need_max[u]) {
-          if (std::find(touched.begin(), touched.end(), u) == touched.end()) touched.push_back(u);
-          need_max[u] = need;
-        }
-      }
-      std::sort(touched.begin(), touched.end());
-      for (int u : touched) dlist.push_back(u | (need_max[u] << REGION_RANK_BITS));
-    }
-    dptr[r + 1] = (int)dlist.size();
-    const int e0 = (int)ecode.size();
-    for (int t = a; t < b; ++t) {
-      const int i = rows[t];
-      const int q = ipos[i];
-      dsrc[t] = crp[q + 1] - 1;
-      pcnt[t] = Lp[i + 1] - 1 - Lp[i];
-      for (int u = 0; u < PK; ++u) psrc[(size_t)PK * t + u] = dsrc[t];
-      for (int j = Lp[i]; j < Lp[i + 1] - 1; ++j) {
-        const int c = Li[j];
-        const int u = j - Lp[i];
-        const int code = reg[c] == g ? lidx[c] : smax + hslot[c];
-        const int src = crp[q] + u;
-        if (u < PK) {
-          pcode[(size_t)PK * t + u] = code;
-          psrc[(size_t)PK * t + u] = src;
-        } else {
-          ecode.push_back(code);
-          esrc.push_back(src);
-        }
-      }
-      eptr[t + 1] = (int)ecode.size();
-    }
-    emax = std::max(emax, (int)ecode.size() - e0);
+    (void)hbase;
+    (void)dbase;
   }
   if (R >= (1 << REGION_RANK_BITS)) return false;  // ranks do not fit the wait entries
   for (int t = 0; t < n; ++t)
@@ -689,6 +750,9 @@ bool pack_region_records(const HostRegions &o, int vs, HostRecords &out,
   out.rhead.assign(8 * (size_t)R, 0);
   out.recmax = 0;
   int64_t units = 0;
+  // value slots of each region (packed entries, diagonals, overflow), so
+  // that regions can be written in parallel at known offsets
+  std::vector<int64_t> voff(R + 1, 0);
   for (int r = 0; r < R; ++r) {
     const int a = o.rptr[r], nr = o.rptr[r + 1] - a;
     const int nh = o.hptr[r + 1] - o.hptr[r];
@@ -708,13 +772,18 @@ bool pack_region_records(const HostRegions &o, int vs, HostRecords &out,
     hd[7] = nb;
     out.recmax = std::max(out.recmax, L.bytes);
     units += L.bytes / 16;
+    int64_t vc = ne;
+    for (int t = a; t < a + nr; ++t) vc += std::min(o.pcnt[t], PK) + 1;
+    voff[r + 1] = voff[r] + vc;
     // value positions are counted in values of vs bytes
     if (units > INT_MAX || units * 16 / vs > INT_MAX) return false;
   }
   out.bytes.assign((size_t)units * 16, 0);
-  out.vsrc.clear();
-  out.vdst.clear();
-  for (int r = 0; r < R; ++r) {
+  out.vsrc.assign((size_t)voff[R], 0);
+  out.vdst.assign((size_t)voff[R], 0);
+  parallel_ranges(R, [&](int, int r_lo, int r_hi) {
+  for (int r = r_lo; r < r_hi; ++r) {
+    size_t vq = (size_t)voff[r];
     const int a = o.rptr[r], nr = o.rptr[r + 1] - a;
     const int h0 = o.hptr[r], nh = o.hptr[r + 1] - h0;
     const int e0 = o.eptr[a], ne = o.eptr[a + nr] - e0;
@@ -734,11 +803,11 @@ bool pack_region_records(const HostRegions &o, int vs, HostRecords &out,
       for (int u = 0; u < PK; ++u) {
         put16(rec + L.code + 2 * (PK * l + u), o.pcode[(size_t)PK * t + u]);
         if (u >= o.pcnt[t]) continue;  // padding: value stays +0.0
-        out.vsrc.push_back(o.psrc[(size_t)PK * t + u]);
-        out.vdst.push_back((int)(vbase + (L.pval / vs) + PK * l + u));
+        out.vsrc[vq] = o.psrc[(size_t)PK * t + u];
+        out.vdst[vq++] = (int)(vbase + (L.pval / vs) + PK * l + u);
       }
-      out.vsrc.push_back(o.dsrc[t]);
-      out.vdst.push_back((int)(vbase + L.diag / vs + l));
+      out.vsrc[vq] = o.dsrc[t];
+      out.vdst[vq++] = (int)(vbase + L.diag / vs + l);
       rec[L.cnt + l] = (uint8_t)std::min(o.pcnt[t], PK + 1);
       put32(rec + L.rows + 4 * l, o.rows[t]);
       if (ne > 0) put32(rec + L.lep + 4 * l, o.eptr[t] - e0);
@@ -746,8 +815,8 @@ bool pack_region_records(const HostRegions &o, int vs, HostRecords &out,
     if (ne > 0) put32(rec + L.lep + 4 * nr, ne);
     for (int e = 0; e < ne; ++e) {
       put16(rec + L.lcode + 2 * e, o.ecode[e0 + e]);
-      out.vsrc.push_back(o.esrc[e0 + e]);
-      out.vdst.push_back((int)(vbase + L.lval / vs + e));
+      out.vsrc[vq] = o.esrc[e0 + e];
+      out.vdst[vq++] = (int)(vbase + L.lval / vs + e);
     }
     for (int q = 0; q < nh; ++q) put32(rec + L.halo + 4 * q, o.halo[h0 + q]);
     for (int s = 0; s <= ns; ++s) put32(rec + L.slices + 4 * s, o.slices[s0 + s]);
@@ -769,6 +838,7 @@ bool pack_region_records(const HostRegions &o, int vs, HostRecords &out,
       for (int e = 0; e < ne; ++e) put32(rec + L.gpos + 4 * ((PK + 1) * nr + e), gm[o.esrc[e0 + e]]);
     }
   }
+  });
   return true;
 }
 
