@@ -279,12 +279,15 @@ __device__ __forceinline__ void read_deps(const Pairs<T, CPL> &m, const T *xs,
 // The row arithmetic, per column: the four packed products in entry order
 // (padding products are +0.0 * +0.0 from the zero slot, which leave the
 // value unchanged, so no test is needed), then the row's overflow entries in
-// order (bounds clamped to [0, ne]), then the division; results go to the
-// row's slots of the columns that exist.
+// order (bounds clamped to [0, ne]), then the divisions, one per column,
+// computed together and stored in one access.  A pair's second column may
+// lie beyond kc: its slot is padding that only the same padding column of
+// later rows reads and that no store to X takes, so it is written without a
+// test (and the two divisions share one branch-free block).
 template <typename T, int KT, int CPL>
 __device__ __forceinline__ void finish_pairs(Pairs<T, CPL> &m, const T (&xv)[PK][CPL], T *xs,
                                              const int *lep, const T *lval,
-                                             const uint16_t *lcode, int ne, int kc) {
+                                             const uint16_t *lcode, int ne) {
   const int cc = m.t & (KT - 1);
 #pragma unroll
   for (int u = 0; u < PK; ++u)
@@ -298,9 +301,16 @@ __device__ __forceinline__ void finish_pairs(Pairs<T, CPL> &m, const T (&xv)[PK]
       for (int j = 0; j < CPL; ++j) m.acc[j] -= lval[e] * xe[j];
     }
   if (m.ok) {
+    T q[CPL];
 #pragma unroll
-    for (int j = 0; j < CPL; ++j)
-      if (cc + j < kc) xs[m.t + j] = m.acc[j] / m.dg;
+    for (int j = 0; j < CPL; ++j) q[j] = m.acc[j] / m.dg;
+    if constexpr (CPL == 1) {
+      xs[m.t] = q[0];
+    } else if constexpr (sizeof(T) == 8) {
+      *reinterpret_cast<double2 *>(xs + m.t) = make_double2(q[0], q[1]);
+    } else {
+      *reinterpret_cast<float2 *>(xs + m.t) = make_float2(q[0], q[1]);
+    }
   }
 }
 
@@ -308,21 +318,24 @@ __device__ __forceinline__ void finish_pairs(Pairs<T, CPL> &m, const T (&xv)[PK]
 // slice never read each other), then the arithmetic.
 template <typename T, int KT, int CPL, int P>
 __device__ __forceinline__ void solve_pairs(Pairs<T, CPL> (&m)[P], T *xs, const int *lep,
-                                            const T *lval, const uint16_t *lcode, int ne,
-                                            int kc) {
+                                            const T *lval, const uint16_t *lcode, int ne) {
   T xv[P][PK][CPL];
 #pragma unroll
   for (int p = 0; p < P; ++p) read_deps<T, KT, CPL>(m[p], xs, xv[p]);
 #pragma unroll
-  for (int p = 0; p < P; ++p) finish_pairs<T, KT, CPL>(m[p], xv[p], xs, lep, lval, lcode, ne, kc);
+  for (int p = 0; p < P; ++p) finish_pairs<T, KT, CPL>(m[p], xv[p], xs, lep, lval, lcode, ne);
 }
 
 // The internal levels of a task, slice after slice.  Each lane solves CPL
 // columns of one row at a time, P such groups of the next slice loaded while
 // this slice is solved, so only the dependency values sit on the chain (a
 // slice with more groups than one pass of the lanes loads the rest as it
-// goes).  One wave's LDS operations complete in issue order, so a wave's
-// next slice sees its own writes of this slice without a wait.  Slice bounds: lane j holds
+// goes).  Within a slice the dependency reads are issued before the next
+// slice's loads, and the loop is unrolled by two with the two sets of pairs
+// trading roles, so no register copies (and no wait for every LDS operation
+// they would need) close a slice.  One wave's LDS operations complete in
+// issue order, so a wave's next slice sees its own writes of this slice
+// without a wait.  Slice bounds: lane j holds
 // lsl[j]; with SHORT (ns <= 62) every bound is read from that register,
 // otherwise those beyond lane 63 from LDS.  Bounds are clamped to [0, nr].
 template <typename T, int KT, int CPL, int P, bool SHORT, int NW>
@@ -362,24 +375,32 @@ __device__ __forceinline__ void solve_slices(T *xs, const uint8_t *rb, const Rec
       load_pairs<T, KT, CPL>(cur[p], b0 * KT + off + p * STEP, b1 * KT, kc, xs, cnt8, code,
                              pval, ldiag);
   }
-  for (int s = sb; s < se; ++s) {
+  // slice s: solve c (rows [b0, b1)) while nx takes slice s + 1
+  auto slice = [&](Pairs<T, CPL>(&c)[P], Pairs<T, CPL>(&nx)[P], int s) {
     // after the last slice b2 == b1: the loads are harmless and unused
     const int b2 = s + 1 < se ? bound(s + 2) : b1;
+    T xv[P][PK][CPL];
+#pragma unroll
+    for (int p = 0; p < P; ++p) read_deps<T, KT, CPL>(c[p], xs, xv[p]);
 #pragma unroll
     for (int p = 0; p < P; ++p)
-      load_pairs<T, KT, CPL>(nxt[p], b1 * KT + off + p * STEP, b2 * KT, kc, xs, cnt8, code,
+      load_pairs<T, KT, CPL>(nx[p], b1 * KT + off + p * STEP, b2 * KT, kc, xs, cnt8, code,
                              pval, ldiag);
-    solve_pairs<T, KT, CPL, P>(cur, xs, lep, lval, lcode, ne, kc);
+#pragma unroll
+    for (int p = 0; p < P; ++p) finish_pairs<T, KT, CPL>(c[p], xv[p], xs, lep, lval, lcode, ne);
     for (int t = b0 * KT + off + P * STEP; t < b1 * KT; t += STEP) {
       Pairs<T, CPL> more[1];
       load_pairs<T, KT, CPL>(more[0], t, b1 * KT, kc, xs, cnt8, code, pval, ldiag);
-      solve_pairs<T, KT, CPL, 1>(more, xs, lep, lval, lcode, ne, kc);
+      solve_pairs<T, KT, CPL, 1>(more, xs, lep, lval, lcode, ne);
     }
     task_sync<SPLIT ? 1 : NW>();
-#pragma unroll
-    for (int p = 0; p < P; ++p) cur[p] = nxt[p];
     b0 = b1;
     b1 = b2;
+  };
+  for (int s = sb; s < se; s += 2) {
+    slice(cur, nxt, s);
+    if (s + 1 >= se) break;
+    slice(nxt, cur, s + 1);
   }
 }
 
