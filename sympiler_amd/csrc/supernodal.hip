@@ -74,7 +74,74 @@ __device__ __forceinline__ int matvec_block(int col, int w) {
 // below (their loads in flight at once).
 constexpr int SN_ROWS = 2;
 static_assert(SN_CHUNK == 64 * SN_ROWS, "a chunk of rows below is one pass of a wave");
+// TOP_ROWS: rows per lane per pass of the top's update after each column
+// block.  Each pass is a memory round trip on the chain from one block to
+// the next; four rows per pass take more registers (two waves per SIMD
+// instead of three), which pays for one right-hand side, where the chain of
+// the widest supernodes sets the time, and not for blocks, where the (task,
+// column) tickets keep every wave busy.
 
+// A supernode's own rows: xs[r] = x[s+r] minus the temporaries listed for
+// row s+r, in list (ascending supernode) order, for rows r0 <= r < w.  Lane l
+// takes rows r0 + l + 64q, q < NQ, and its rows advance together, GB list entries each
+// per round: a round's temporaries are loaded for all of its rows at once
+// (their indices were loaded in the round before), then subtracted row by
+// row in list order.  So a lane pays one memory round trip per GB entries of
+// its longest row, not per eight entries of each row in turn.
+template <int NQ>
+__device__ __forceinline__ void gather_rows(double *xs, const double *x, const double *T,
+                                            const int *cptr, const int *cidx, int s, int w,
+                                            int r0, int lane) {
+  static_assert(NQ >= 1 && NQ <= 8, "at most 512 rows per call");
+  constexpr int GB = 16 / NQ;
+  double v[NQ];
+  int cp[NQ], ce[NQ], idx[NQ][GB];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    const int r = r0 + lane + 64 * q;
+    const bool in = r < w;
+    v[q] = in ? x[s + r] : 0.0;
+    cp[q] = in ? cptr[s + r] : 0;
+    ce[q] = in ? cptr[s + r + 1] : 0;
+  }
+  bool more = false;
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+#pragma unroll
+    for (int u = 0; u < GB; ++u) idx[q][u] = cp[q] + u < ce[q] ? cidx[cp[q] + u] : -1;
+    more = more || cp[q] < ce[q];
+  }
+  while (more) {
+    double tv[NQ][GB];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+#pragma unroll
+      for (int u = 0; u < GB; ++u) tv[q][u] = idx[q][u] >= 0 ? ld_dev(&T[idx[q][u]]) : 0.0;
+    int nidx[NQ][GB];
+    more = false;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int c1 = cp[q] + GB;
+#pragma unroll
+      for (int u = 0; u < GB; ++u) nidx[q][u] = c1 + u < ce[q] ? cidx[c1 + u] : -1;
+      more = more || c1 < ce[q];
+    }
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+#pragma unroll
+      for (int u = 0; u < GB; ++u)
+        if (idx[q][u] >= 0) v[q] -= tv[q][u];
+      cp[q] += GB;
+#pragma unroll
+      for (int u = 0; u < GB; ++u) idx[q][u] = nidx[q][u];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < NQ; ++q)
+    if (r0 + lane + 64 * q < w) xs[r0 + lane + 64 * q] = v[q];
+}
+
+template <int TOP_ROWS>
 __global__ __launch_bounds__(256) void k_supernodal(SnArgs a) {
   __shared__ double slab[4][SN_WMAX];
   double *xs = slab[threadIdx.x >> 6];
@@ -106,23 +173,15 @@ __global__ __launch_bounds__(256) void k_supernodal(SnArgs a) {
       wave_lds_sync();
     } else {
       // 2. own rows: b minus the temporaries, in ascending supernode order
-      // Eight temporaries are loaded before any is subtracted (the index of
-      // a slot past the row's list is clamped to its last entry and the
-      // value discarded), so a long list costs one memory round trip per
-      // eight entries; the subtractions keep the list's order.
-      for (int r = lane; r < w; r += 64) {
-        double v = x[s + r];
-        const int ce = a.cptr[s + r + 1];
-        for (int c0 = a.cptr[s + r]; c0 < ce; c0 += 8) {
-          double t[8];
-#pragma unroll
-          for (int u = 0; u < 8; ++u) t[u] = ld_dev(&T[a.cidx[min(c0 + u, ce - 1)]]);
-#pragma unroll
-          for (int u = 0; u < 8; ++u)
-            if (c0 + u < ce) v -= t[u];
-        }
-        xs[r] = v;
-      }
+      if (w <= 64)
+        gather_rows<1>(xs, x, T, a.cptr, a.cidx, s, w, 0, lane);
+      else if (w <= 128)
+        gather_rows<2>(xs, x, T, a.cptr, a.cidx, s, w, 0, lane);
+      else if (w <= 256)
+        gather_rows<4>(xs, x, T, a.cptr, a.cidx, s, w, 0, lane);
+      else
+        for (int r0 = 0; r0 < w; r0 += 512)
+          gather_rows<8>(xs, x, T, a.cptr, a.cidx, s, w, r0, lane);
       wave_lds_sync();
       // 3. the dense top, column block by column block
       for (int col = 0; col < w;) {
@@ -132,12 +191,12 @@ __global__ __launch_bounds__(256) void k_supernodal(SnArgs a) {
 #pragma unroll
         for (int c = 0; c < 8; ++c)
           mrow[c] = lane < B && c <= lane ? M[(col + lane) + (int64_t)(col + c) * nr] : 1.0;
-        // the first SN_ROWS rows per lane of the update, loaded with the block
+        // the first TOP_ROWS rows per lane of the update, loaded with the block
         // (they do not depend on its values)
-        double mk[SN_ROWS][8];
+        double mk[TOP_ROWS][8];
         const int k0 = col + B + lane;
 #pragma unroll
-        for (int u = 0; u < SN_ROWS; ++u) {
+        for (int u = 0; u < TOP_ROWS; ++u) {
           const int k = k0 + 64 * u;
 #pragma unroll
           for (int c = 0; c < 8; ++c)
@@ -160,11 +219,11 @@ __global__ __launch_bounds__(256) void k_supernodal(SnArgs a) {
         if (lane < B) xs[col + lane] = lane == 0 ? xv[0] : lane == 1 ? xv[1] : lane == 2 ? xv[2]
                                      : lane == 3 ? xv[3] : lane == 4 ? xv[4] : lane == 5 ? xv[5]
                                      : lane == 6 ? xv[6] : xv[7];
-        // SN_ROWS rows per lane at a time: all their entries in flight at once
-        for (int kc = k0; kc < w; kc += 64 * SN_ROWS) {
+        // TOP_ROWS rows per lane at a time: all their entries in flight at once
+        for (int kc = k0; kc < w; kc += 64 * TOP_ROWS) {
           if (kc != k0) {
 #pragma unroll
-            for (int u = 0; u < SN_ROWS; ++u) {
+            for (int u = 0; u < TOP_ROWS; ++u) {
               const int k = kc + 64 * u;
 #pragma unroll
               for (int c = 0; c < 8; ++c)
@@ -172,7 +231,7 @@ __global__ __launch_bounds__(256) void k_supernodal(SnArgs a) {
             }
           }
 #pragma unroll
-          for (int u = 0; u < SN_ROWS; ++u) {
+          for (int u = 0; u < TOP_ROWS; ++u) {
             const int k = kc + 64 * u;
             if (k < w) {
               double t = xs[k];
@@ -535,7 +594,9 @@ int sn_solve(symamd_handle h, symamd_snplan_s *q, const double *d_Lx, double *d_
   a.ldx = ldx;
   a.tstride = q->tcount > 0 ? q->tcount : 1;
   a.timeout = h->timeout_ticks();
-  const void *kfn = transposed ? (const void *)&k_supernodal_t : (const void *)&k_supernodal;
+  const void *kfn = transposed ? (const void *)&k_supernodal_t
+                    : k == 1   ? (const void *)&k_supernodal<4>
+                               : (const void *)&k_supernodal<2>;
   int per_cu = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, 256, 0) != hipSuccess ||
       per_cu < 1)
@@ -548,8 +609,10 @@ int sn_solve(symamd_handle h, symamd_snplan_s *q, const double *d_Lx, double *d_
   // tickets are dealt in task order, so any grid size is safe
   if (transposed)
     k_supernodal_t<<<(unsigned)grid, 256, 0, h->stream>>>(a);
+  else if (k == 1)
+    k_supernodal<4><<<(unsigned)grid, 256, 0, h->stream>>>(a);
   else
-    k_supernodal<<<(unsigned)grid, 256, 0, h->stream>>>(a);
+    k_supernodal<2><<<(unsigned)grid, 256, 0, h->stream>>>(a);
   SYMAMD_HIP(hipGetLastError());
   return SYMAMD_OK;
 }
