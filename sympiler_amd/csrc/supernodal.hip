@@ -83,32 +83,42 @@ static_assert(SN_CHUNK == 64 * SN_ROWS, "a chunk of rows below is one pass of a 
 
 // A supernode's own rows: xs[r] = x[s+r] minus the temporaries listed for
 // row s+r, in list (ascending supernode) order, for rows r0 <= r < w.  Lane l
-// takes rows r0 + l + 64q, q < NQ, and its rows advance together, GB list entries each
-// per round: a round's temporaries are loaded for all of its rows at once
-// (their indices were loaded in the round before), then subtracted row by
-// row in list order.  So a lane pays one memory round trip per GB entries of
-// its longest row, not per eight entries of each row in turn.
+// takes rows r0 + l + 64q, q < NQ, and its rows advance together, GB list
+// entries each per round: a round's temporaries are loaded for all of its
+// rows at once (their indices were loaded in the round before), then
+// subtracted row by row in list order.  So a lane pays one memory round
+// trip per GB entries of its longest row, not per eight entries of each row
+// in turn.  No load is behind a branch: list positions past a row's end are
+// clamped into the index array (which holds at least one entry), and their
+// temporaries are replaced by the row's own x (rows past w by row w-1's),
+// loaded and not used; whether a slot is used depends only on the list
+// bounds, so no value is waited for before its use.
 template <int NQ>
 __device__ __forceinline__ void gather_rows(double *xs, const double *x, const double *T,
                                             const int *cptr, const int *cidx, int s, int w,
                                             int r0, int lane) {
   static_assert(NQ >= 1 && NQ <= 8, "at most 512 rows per call");
   constexpr int GB = 16 / NQ;
+  if (w <= r0) return;
   double v[NQ];
   int cp[NQ], ce[NQ], idx[NQ][GB];
+  auto row = [&](int q) { return min(r0 + lane + 64 * q, w - 1); };
 #pragma unroll
   for (int q = 0; q < NQ; ++q) {
-    const int r = r0 + lane + 64 * q;
-    const bool in = r < w;
-    v[q] = in ? x[s + r] : 0.0;
-    cp[q] = in ? cptr[s + r] : 0;
-    ce[q] = in ? cptr[s + r + 1] : 0;
+    const bool in = r0 + lane + 64 * q < w;
+    v[q] = x[s + row(q)];
+    const int c0 = cptr[s + row(q)], c1 = cptr[s + row(q) + 1];
+    cp[q] = in ? c0 : 0;
+    ce[q] = in ? c1 : 0;
   }
+  auto load_idx = [&](int q, int c, int (&d)[GB]) {
+#pragma unroll
+    for (int u = 0; u < GB; ++u) d[u] = cidx[max(min(c + u, ce[q] - 1), 0)];
+  };
   bool more = false;
 #pragma unroll
   for (int q = 0; q < NQ; ++q) {
-#pragma unroll
-    for (int u = 0; u < GB; ++u) idx[q][u] = cp[q] + u < ce[q] ? cidx[cp[q] + u] : -1;
+    load_idx(q, cp[q], idx[q]);
     more = more || cp[q] < ce[q];
   }
   while (more) {
@@ -116,21 +126,20 @@ __device__ __forceinline__ void gather_rows(double *xs, const double *x, const d
 #pragma unroll
     for (int q = 0; q < NQ; ++q)
 #pragma unroll
-      for (int u = 0; u < GB; ++u) tv[q][u] = idx[q][u] >= 0 ? ld_dev(&T[idx[q][u]]) : 0.0;
+      for (int u = 0; u < GB; ++u)
+        tv[q][u] = ld_dev(cp[q] + u < ce[q] ? &T[idx[q][u]] : &x[s + row(q)]);
     int nidx[NQ][GB];
     more = false;
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
-      const int c1 = cp[q] + GB;
-#pragma unroll
-      for (int u = 0; u < GB; ++u) nidx[q][u] = c1 + u < ce[q] ? cidx[c1 + u] : -1;
-      more = more || c1 < ce[q];
+      load_idx(q, cp[q] + GB, nidx[q]);
+      more = more || cp[q] + GB < ce[q];
     }
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
 #pragma unroll
       for (int u = 0; u < GB; ++u)
-        if (idx[q][u] >= 0) v[q] -= tv[q][u];
+        if (cp[q] + u < ce[q]) v[q] -= tv[q][u];
       cp[q] += GB;
 #pragma unroll
       for (int u = 0; u < GB; ++u) idx[q][u] = nidx[q][u];
@@ -141,8 +150,16 @@ __device__ __forceinline__ void gather_rows(double *xs, const double *x, const d
     if (r0 + lane + 64 * q < w) xs[r0 + lane + 64 * q] = v[q];
 }
 
+// v on lane j (j uniform), for every lane
+__device__ __forceinline__ double lane_value(double v, int j) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = __builtin_amdgcn_readlane((unsigned)u, j);
+  const unsigned hi = __builtin_amdgcn_readlane((unsigned)(u >> 32), j);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 template <int TOP_ROWS>
-__global__ __launch_bounds__(256) void k_supernodal(SnArgs a) {
+__global__ __launch_bounds__(256, TOP_ROWS >= 4 ? 2 : 3) void k_supernodal(SnArgs a) {
   __shared__ double slab[4][SN_WMAX];
   double *xs = slab[threadIdx.x >> 6];
   const int lane = threadIdx.x & 63;
@@ -183,42 +200,61 @@ __global__ __launch_bounds__(256) void k_supernodal(SnArgs a) {
         for (int r0 = 0; r0 < w; r0 += 512)
           gather_rows<8>(xs, x, T, a.cptr, a.cidx, s, w, r0, lane);
       wave_lds_sync();
-      // 3. the dense top, column block by column block
-      for (int col = 0; col < w;) {
-        const int B = lsolve_block(col, w);
-        // lane j < B holds row col+j of the block's columns (and its diagonal)
-        double mrow[8];
+      // 3. the dense top, column block by column block.  Block (col, B):
+      // lane j < B holds row col+j of the block's columns (and its diagonal)
+      // in mr, and mk the first TOP_ROWS rows per lane of the update after
+      // the block (B = 0: past the last block, nothing loaded).  The entries
+      // do not depend on the solve: with PF the next block's are loaded
+      // while this block is solved.
+      constexpr bool PF = TOP_ROWS >= 4;
+      // Addresses outside the block are clamped into the top, so the loads
+      // need no branches; the values so loaded are never used (a lane's
+      // triangle reads its own row up to its diagonal, the update skips
+      // columns past B and rows past w).  B = 0 loads the top's last entry.
+      auto load_block = [&](int col, int B, double (&mr)[8], double (&mk)[TOP_ROWS][8]) {
+        const int rl = min(lane, B - 1);
 #pragma unroll
-        for (int c = 0; c < 8; ++c)
-          mrow[c] = lane < B && c <= lane ? M[(col + lane) + (int64_t)(col + c) * nr] : 1.0;
-        // the first TOP_ROWS rows per lane of the update, loaded with the block
-        // (they do not depend on its values)
-        double mk[TOP_ROWS][8];
+        for (int c = 0; c < 8; ++c) mr[c] = M[(col + rl) + (int64_t)(col + min(c, rl)) * nr];
         const int k0 = col + B + lane;
 #pragma unroll
         for (int u = 0; u < TOP_ROWS; ++u) {
           const int k = k0 + 64 * u;
 #pragma unroll
           for (int c = 0; c < 8; ++c)
-            mk[u][c] = k < w && c < B ? M[k + (int64_t)(col + c) * nr] : 0.0;
+            mk[u][c] = M[min(k, w - 1) + (int64_t)(col + min(c, B - 1)) * nr];
         }
+      };
+      double mrow[8], mk[TOP_ROWS][8];
+      int B = w > 0 ? lsolve_block(0, w) : 0;
+      if constexpr (PF) load_block(0, B, mrow, mk);
+      for (int col = 0; col < w;) {
+        const int cn = col + B;
+        const int Bn = cn < w ? lsolve_block(cn, w) : 0;
+        double nrow[8], nk[TOP_ROWS][8];
+        if constexpr (PF)
+          load_block(cn, Bn, nrow, nk);
+        else
+          load_block(col, B, mrow, mk);
+        const int k0 = col + B + lane;
+        // the block's triangle: lane l < B keeps its row's running value;
+        // at step j lane j divides, the quotient is broadcast, and every
+        // lane subtracts its product (each row loses its products in column
+        // order, as the reference computes them)
+        double acc = lane < B ? xs[col + lane] : 0.0, mine = 0.0;
         double xv[8];
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
           if (j < B) {
-            double t = xs[col + j];
-#pragma unroll
-            for (int c = 0; c < j; ++c) t = t - xv[c] * mrow[c];
-            const double q = t / mrow[j];
-            xv[j] = __shfl(q, j);
+            const double q = acc / mrow[j];
+            mine = lane == j ? q : mine;
+            xv[j] = lane_value(q, j);
+            acc = acc - xv[j] * mrow[j];
           } else {
             xv[j] = 0.0;
           }
         }
         wave_lds_sync();
-        if (lane < B) xs[col + lane] = lane == 0 ? xv[0] : lane == 1 ? xv[1] : lane == 2 ? xv[2]
-                                     : lane == 3 ? xv[3] : lane == 4 ? xv[4] : lane == 5 ? xv[5]
-                                     : lane == 6 ? xv[6] : xv[7];
+        if (lane < B) xs[col + lane] = mine;
         // TOP_ROWS rows per lane at a time: all their entries in flight at once
         for (int kc = k0; kc < w; kc += 64 * TOP_ROWS) {
           if (kc != k0) {
@@ -227,7 +263,7 @@ __global__ __launch_bounds__(256) void k_supernodal(SnArgs a) {
               const int k = kc + 64 * u;
 #pragma unroll
               for (int c = 0; c < 8; ++c)
-                mk[u][c] = k < w && c < B ? M[k + (int64_t)(col + c) * nr] : 0.0;
+                mk[u][c] = M[min(k, w - 1) + (int64_t)(col + min(c, B - 1)) * nr];
             }
           }
 #pragma unroll
@@ -243,13 +279,24 @@ __global__ __launch_bounds__(256) void k_supernodal(SnArgs a) {
           }
         }
         wave_lds_sync();
-        col += B;
+        if constexpr (PF) {
+#pragma unroll
+          for (int c = 0; c < 8; ++c) mrow[c] = nrow[c];
+#pragma unroll
+          for (int u = 0; u < TOP_ROWS; ++u)
+#pragma unroll
+            for (int c = 0; c < 8; ++c) mk[u][c] = nk[u][c];
+        }
+        col = cn;
+        B = Bn;
       }
       for (int r = lane; r < w; r += 64) st_dev(&x[s + r], xs[r]);
     }
     // 4. this task's chunk of the rows below: products with the solved top
     // into T; SN_ROWS rows per lane, each column block's entries of those
-    // rows loaded together, each row's sum kept in the reference's order
+    // rows loaded together, each row's sum kept in the reference's order.
+    // Lanes past the last row load row nr-1 (no branch per load) and store
+    // nothing; the tail skips the columns past its block.
     const int tb = a.tb[i];
     {
       const int k0 = w + chunk * SN_CHUNK + lane;
@@ -268,7 +315,7 @@ __global__ __launch_bounds__(256) void k_supernodal(SnArgs a) {
               const int k = k0 + 64 * u;
 #pragma unroll
               for (int c = 0; c < 8; ++c)
-                mk2[h][u][c] = k < nr ? M[k + (int64_t)(col + 8 * h + c) * nr] : 0.0;
+                mk2[h][u][c] = M[min(k, nr - 1) + (int64_t)(col + 8 * h + c) * nr];
             }
 #pragma unroll
           for (int h = 0; h < 2; ++h)
@@ -289,7 +336,7 @@ __global__ __launch_bounds__(256) void k_supernodal(SnArgs a) {
           const int k = k0 + 64 * u;
 #pragma unroll
           for (int c = 0; c < 8; ++c)
-            mk[u][c] = k < nr && c < B ? M[k + (int64_t)(col + c) * nr] : 0.0;
+            mk[u][c] = M[min(k, nr - 1) + (int64_t)(col + min(c, B - 1)) * nr];
         }
 #pragma unroll
         for (int u = 0; u < SN_ROWS; ++u) {
