@@ -83,8 +83,9 @@ static_assert(SN_CHUNK == 64 * SN_ROWS, "a chunk of rows below is one pass of a 
 
 // A supernode's own rows: xs[r] = x[s+r] minus the temporaries listed for
 // row s+r, in list (ascending supernode) order, for rows r0 <= r < w.  Lane l
-// takes rows r0 + l + 64q, q < NQ, and its rows advance together, GB list
-// entries each per round: a round's temporaries are loaded for all of its
+// takes rows r0 + l + 64q, q < NQ, and its rows advance together, GB = 16/NQ
+// list entries each per round (32 slots per lane measured slower: most of the
+// extra slots are stand-ins): a round's temporaries are loaded for all of its
 // rows at once (their indices were loaded in the round before), then
 // subtracted row by row in list order.  So a lane pays one memory round
 // trip per GB entries of its longest row, not per eight entries of each row
@@ -497,20 +498,37 @@ int sn_analyze(symamd_handle h, int n, int m, const int *h_sup, const int *h_nro
   for (int r = 0; r < n; ++r) cnt[r + 1] += cnt[r];
   // contributions per row in ascending supernode order; dependency lists
   std::vector<int> cidx(cnt[n] > 0 ? cnt[n] : 1), fill(cnt.begin(), cnt.end() - 1);
-  // tasks: supernode i's top (with the first SN_CHUNK rows below), then one
-  // task per further SN_CHUNK rows below
-  std::vector<int> first(m + 1, 0);
-  for (int i = 0; i < m; ++i) {
-    const int below = h_nrows[i] - (h_sup[i + 1] - h_sup[i]);
-    first[i + 1] = first[i] + std::max(1, (below + SN_CHUNK - 1) / SN_CHUNK);
-  }
-  const int ntasks = first[m];
-  std::vector<int2> task(ntasks > 0 ? ntasks : 1);
-  for (int i = 0; i < m; ++i)
-    for (int t = first[i]; t < first[i + 1]; ++t) task[t] = make_int2(i, t - first[i]);
   std::vector<int> owner(n);
   for (int i = 0; i < m; ++i)
     for (int c = h_sup[i]; c < h_sup[i + 1]; ++c) owner[c] = i;
+  // tasks: supernode i's top (with the first SN_CHUNK rows below), then one
+  // task per further SN_CHUNK rows below.  Tickets are dealt level by level
+  // (a supernode's level is one above the highest of the supernodes writing
+  // to its rows), so every leaf is taken before any supernode that waits,
+  // and the chain of the last subtree does not start behind the others;
+  // every task still waits only for tasks dealt before it.
+  std::vector<int> lev(m, 0);
+  for (int i = 0; i < m; ++i) {
+    const int w = h_sup[i + 1] - h_sup[i];
+    for (int r = w; r < h_nrows[i]; ++r) {
+      const int o = owner[h_Li[h_Lp[i] + r]];
+      lev[o] = std::max(lev[o], lev[i] + 1);
+    }
+  }
+  std::vector<int> order(m);
+  for (int i = 0; i < m; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return lev[x] < lev[y]; });
+  std::vector<int> first(m + 1, 0), ntask(m);  // first[i]: ticket of i's top
+  int ntasks = 0;
+  for (int i : order) {
+    const int below = h_nrows[i] - (h_sup[i + 1] - h_sup[i]);
+    ntask[i] = std::max(1, (below + SN_CHUNK - 1) / SN_CHUNK);
+    first[i] = ntasks;
+    ntasks += ntask[i];
+  }
+  std::vector<int2> task(ntasks > 0 ? ntasks : 1);
+  for (int i = 0; i < m; ++i)
+    for (int c = 0; c < ntask[i]; ++c) task[first[i] + c] = make_int2(i, c);
   // a top task waits for the tasks writing its rows' temporaries, a later
   // chunk for its supernode's top
   std::vector<std::vector<int>> deps(ntasks);
@@ -523,7 +541,7 @@ int sn_analyze(symamd_handle h, int n, int m, const int *h_sup, const int *h_nro
       const int producer = first[i] + k / SN_CHUNK;
       if (d.empty() || d.back() != producer) d.push_back(producer);
     }
-    for (int t = first[i] + 1; t < first[i + 1]; ++t) deps[t].push_back(first[i]);
+    for (int t = first[i] + 1; t < first[i] + ntask[i]; ++t) deps[t].push_back(first[i]);
   }
   // transposed solve: the rows below of each supernode (in temporaries
   // order) and the supernodes owning them, which it waits for
