@@ -151,6 +151,32 @@ __device__ __forceinline__ void gather_rows(double *xs, const double *x, const d
     if (r0 + lane + 64 * q < w) xs[r0 + lane + 64 * q] = v[q];
 }
 
+// NB full column blocks of eight from col: each block's products with the
+// solved top summed left to right and added to the row's temporary, for the
+// SN_ROWS rows k0 + 64u (rows past nr read row nr-1; their sums are unused)
+template <int NB>
+__device__ __forceinline__ void product_blocks(double (&t)[SN_ROWS], const double *xs,
+                                               const double *M, int k0, int nr, int col) {
+  double mb[NB][SN_ROWS][8];
+#pragma unroll
+  for (int h = 0; h < NB; ++h)
+#pragma unroll
+    for (int u = 0; u < SN_ROWS; ++u) {
+      const int k = k0 + 64 * u;
+#pragma unroll
+      for (int c = 0; c < 8; ++c) mb[h][u][c] = M[min(k, nr - 1) + (int64_t)(col + 8 * h + c) * nr];
+    }
+#pragma unroll
+  for (int h = 0; h < NB; ++h)
+#pragma unroll
+    for (int u = 0; u < SN_ROWS; ++u) {
+      double sum = xs[col + 8 * h] * mb[h][u][0];
+#pragma unroll
+      for (int c = 1; c < 8; ++c) sum = sum + xs[col + 8 * h + c] * mb[h][u][c];
+      t[u] = t[u] + sum;
+    }
+}
+
 // v on lane j (j uniform), for every lane
 __device__ __forceinline__ double lane_value(double v, int j) {
   const unsigned long long u = (unsigned long long)__double_as_longlong(v);
@@ -304,32 +330,16 @@ __global__ __launch_bounds__(256, TOP_ROWS >= 4 ? 2 : 3) void k_supernodal(SnArg
       double t[SN_ROWS];
 #pragma unroll
       for (int u = 0; u < SN_ROWS; ++u) t[u] = 0.0;
-      for (int col = 0; col < w;) {
-        if (w - col >= 16) {
-          // two full blocks of eight: all their entries in flight together,
-          // each block's sum formed and added as the reference orders it
-          double mk2[2][SN_ROWS][8];
-#pragma unroll
-          for (int h = 0; h < 2; ++h)
-#pragma unroll
-            for (int u = 0; u < SN_ROWS; ++u) {
-              const int k = k0 + 64 * u;
-#pragma unroll
-              for (int c = 0; c < 8; ++c)
-                mk2[h][u][c] = M[min(k, nr - 1) + (int64_t)(col + 8 * h + c) * nr];
-            }
-#pragma unroll
-          for (int h = 0; h < 2; ++h)
-#pragma unroll
-            for (int u = 0; u < SN_ROWS; ++u) {
-              double sum = xs[col + 8 * h] * mk2[h][u][0];
-#pragma unroll
-              for (int c = 1; c < 8; ++c) sum = sum + xs[col + 8 * h + c] * mk2[h][u][c];
-              t[u] = t[u] + sum;
-            }
-          col += 16;
-          continue;
-        }
+      // full blocks of eight, WIDE (one right-hand side: 4, where the
+      // registers allow; else 2) and then 2 at a time: all their entries in
+      // flight together, each block's sum formed and added as the reference
+      // orders it
+      constexpr int WIDE = TOP_ROWS >= 4 ? 4 : 2;
+      int col = 0;
+      for (; w - col >= 8 * WIDE; col += 8 * WIDE) product_blocks<WIDE>(t, xs, M, k0, nr, col);
+      for (; w - col >= 16; col += 16) product_blocks<2>(t, xs, M, k0, nr, col);
+      // the last columns in blocks of 8, 4 and 1
+      while (col < w) {
         const int B = matvec_block(col, w);
         double mk[SN_ROWS][8];
 #pragma unroll
