@@ -381,11 +381,100 @@ __global__ __launch_bounds__(256, TOP_ROWS >= 4 ? 2 : 3) void k_supernodal(SnArg
 // column c (lanes), t = sum over the rows below, ascending, of M[r, c] times
 // x at that row (the rows' x staged in LDS chunk by chunk), x[s+c] -= t;
 // then the top's transposed solve in LDS from the last column: x_c /= M[c, c]
-// and every x_r, r < c, loses M[c, r] * x_c (the next column's entries are
-// loaded while this one is finished).  The order is the oracle's
+// and every x_r, r < c, loses M[c, r] * x_c.  The order is the oracle's
 // (orc_sptrsv_csc_block_t).
-constexpr int SN_TCHUNK = 256;   // rows below staged per pass
-constexpr int SN_TCOLS = SN_WMAX / 64;  // top columns per lane
+//
+// NT column tiles of 64 per lane (NT * 64 >= w).  The matrix entries do
+// not depend on the solve, so their loads are batched: RB rows below per
+// round (RB * NT loads in flight per lane), and the top's columns in groups
+// of D whose entries load while the group before is finished.  Addresses
+// are clamped into the block (no branch per load); the values so loaded are
+// not used.
+constexpr int SN_TCHUNK = 256;  // rows below staged per pass
+
+template <int NT>
+__device__ __forceinline__ void sn_solve_t_top(double *xs, double *xb, const double *x,
+                                               const double *M, const int *rows, int s,
+                                               int w, int nr, int lane) {
+  constexpr int RB = 32 / NT;           // rows below per round
+  constexpr int D = NT >= 8 ? 32 / NT : 8;  // top columns per group
+  int cl[NT];
+#pragma unroll
+  for (int j = 0; j < NT; ++j) cl[j] = min(lane + 64 * j, w - 1);
+  // 1. the rows below: one running sum per top column, rows ascending
+  double t[NT];
+#pragma unroll
+  for (int j = 0; j < NT; ++j) t[j] = 0.0;
+  for (int r0 = w; r0 < nr; r0 += SN_TCHUNK) {
+    const int cnt = min(SN_TCHUNK, nr - r0);
+    for (int r = lane; r < cnt; r += 64) xb[r] = ld_dev(&x[rows[r0 - w + r]]);
+    wave_lds_sync();
+    for (int rb = 0; rb < cnt; rb += RB) {
+      double mv[RB][NT];
+#pragma unroll
+      for (int u = 0; u < RB; ++u) {
+        const int row = r0 + min(rb + u, cnt - 1);
+#pragma unroll
+        for (int j = 0; j < NT; ++j) mv[u][j] = M[row + (int64_t)cl[j] * nr];
+      }
+#pragma unroll
+      for (int u = 0; u < RB; ++u) {
+        if (rb + u < cnt) {
+          const double v = xb[rb + u];
+          const bool first = r0 + rb + u == w;
+#pragma unroll
+          for (int j = 0; j < NT; ++j) {
+            const double p = mv[u][j] * v;
+            t[j] = first ? p : t[j] + p;
+          }
+        }
+      }
+    }
+    wave_lds_sync();
+  }
+#pragma unroll
+  for (int j = 0; j < NT; ++j) {
+    const int c = lane + 64 * j;
+    if (c < w) xs[c] = nr > w ? x[s + c] - t[j] : x[s + c];
+  }
+  wave_lds_sync();
+  // 2. the top, transposed, from the last column, D columns per group
+  auto load_group = [&](int cb, double (&m)[D][NT], double (&dg)[D]) {
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      const int c = max(cb - k, 0);
+#pragma unroll
+      for (int j = 0; j < NT; ++j) m[k][j] = M[c + (int64_t)cl[j] * nr];
+      dg[k] = M[c + (int64_t)c * nr];
+    }
+  };
+  double mc[D][NT], dc[D];
+  load_group(w - 1, mc, dc);
+  for (int cb = w - 1; cb >= 0; cb -= D) {
+    double mn[D][NT], dn[D];
+    load_group(cb - D, mn, dn);
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      const int c = cb - k;
+      if (c >= 0) {
+        const double y = xs[c] / dc[k];
+#pragma unroll
+        for (int j = 0; j < NT; ++j) {
+          const int r = lane + 64 * j;
+          if (r < c) xs[r] = xs[r] - mc[k][j] * y;
+        }
+        if (lane == (c & 63)) xs[c] = y;
+        wave_lds_sync();
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      dc[k] = dn[k];
+#pragma unroll
+      for (int j = 0; j < NT; ++j) mc[k][j] = mn[k][j];
+    }
+  }
+}
 
 __global__ __launch_bounds__(256) void k_supernodal_t(SnArgs a) {
   __shared__ double slab[4][SN_WMAX];
@@ -407,66 +496,16 @@ __global__ __launch_bounds__(256) void k_supernodal_t(SnArgs a) {
     const int s = a.sup[i], w = a.sup[i + 1] - s, nr = a.nrows[i];
     const double *M = a.Lx + a.Lp[i];
     const int *rows = a.rowb + a.tb[i];
-    // 1. the rows below: one running sum per top column, rows ascending
-    double t[SN_TCOLS];
-#pragma unroll
-    for (int j = 0; j < SN_TCOLS; ++j) t[j] = 0.0;
-    for (int r0 = w; r0 < nr; r0 += SN_TCHUNK) {
-      const int cnt = min(SN_TCHUNK, nr - r0);
-      for (int r = lane; r < cnt; r += 64) xb[r] = ld_dev(&x[rows[r0 - w + r]]);
-      wave_lds_sync();
-      for (int r = 0; r < cnt; ++r) {
-        const double v = xb[r];
-        const int row = r0 + r;
-#pragma unroll
-        for (int j = 0; j < SN_TCOLS; ++j) {
-          const int c = lane + 64 * j;
-          if (c < w) {
-            const double p = M[row + (int64_t)c * nr] * v;
-            t[j] = row == w ? p : t[j] + p;
-          }
-        }
-      }
-      wave_lds_sync();
-    }
-#pragma unroll
-    for (int j = 0; j < SN_TCOLS; ++j) {
-      const int c = lane + 64 * j;
-      if (c < w) xs[c] = nr > w ? x[s + c] - t[j] : x[s + c];
-    }
-    wave_lds_sync();
-    // 2. the top, transposed, from the last column
-    double mc[SN_TCOLS];
-    const int c1 = w - 1;
-#pragma unroll
-    for (int j = 0; j < SN_TCOLS; ++j) {
-      const int r = lane + 64 * j;
-      mc[j] = r < c1 ? M[c1 + (int64_t)r * nr] : 0.0;
-    }
-    double dg = M[c1 + (int64_t)c1 * nr];
-    for (int c = w - 1; c >= 0; --c) {
-      // the next column's entries (row c-1 of the top) and diagonal, in
-      // flight while this column is finished
-      double mn[SN_TCOLS];
-      const int cn = c - 1;
-#pragma unroll
-      for (int j = 0; j < SN_TCOLS; ++j) {
-        const int r = lane + 64 * j;
-        mn[j] = cn >= 0 && r < cn ? M[cn + (int64_t)r * nr] : 0.0;
-      }
-      const double dn = cn >= 0 ? M[cn + (int64_t)cn * nr] : 1.0;
-      const double y = xs[c] / dg;
-#pragma unroll
-      for (int j = 0; j < SN_TCOLS; ++j) {
-        const int r = lane + 64 * j;
-        if (r < c) xs[r] = xs[r] - mc[j] * y;
-      }
-      if (lane == (c & 63)) xs[c] = y;
-      wave_lds_sync();
-#pragma unroll
-      for (int j = 0; j < SN_TCOLS; ++j) mc[j] = mn[j];
-      dg = dn;
-    }
+    if (w <= 64)
+      sn_solve_t_top<1>(xs, xb, x, M, rows, s, w, nr, lane);
+    else if (w <= 128)
+      sn_solve_t_top<2>(xs, xb, x, M, rows, s, w, nr, lane);
+    else if (w <= 256)
+      sn_solve_t_top<4>(xs, xb, x, M, rows, s, w, nr, lane);
+    else if (w <= 512)
+      sn_solve_t_top<8>(xs, xb, x, M, rows, s, w, nr, lane);
+    else
+      sn_solve_t_top<16>(xs, xb, x, M, rows, s, w, nr, lane);
     for (int r = lane; r < w; r += 64) st_dev(&x[s + r], xs[r]);
     publish_fence();
     if (lane == 0) st_dev(&flags[i], target);
