@@ -284,8 +284,15 @@ def supernodal_config(torch, sa, W, dev, stream):
     mk = float(np.mean(ek))
     out["k32"] = {"kernel_ms": round(mk, 4), "rhs_solves_per_s": round(k / (mk * 1e-3), 1),
                   "vs_k1_rhs_per_s": round(k * ms / mk, 2)}
+    # the transposed solve L^T x = b on the same plan (blockedLTsolve), one column
+    xt = db.clone()
+    _, tt = timed_steps(lambda: plan.solve_transposed(dLx, xt), lambda: xt.copy_(db), 5, 1,
+                        stream, torch)
+    plan.check()
+    mt = float(np.mean(tt))
+    out["transposed"] = {"kernel_ms": round(mt, 4), "gbs": round(byts / (mt * 1e-3) / 1e9, 1)}
     log(f"supernodal tree13: {out}")
-    del plan, dLx, db, x, B0, X
+    del plan, dLx, db, x, xt, B0, X
     return out
 
 
