@@ -400,6 +400,7 @@ struct symamd_snplan_s {
   // transposed solve: rows below (temporaries order), the supernodes owning
   // them per supernode
   int *d_rowb = nullptr, *d_tdptr = nullptr, *d_tdlist = nullptr;
+  int *d_torder = nullptr;  // transposed solve: supernode of each ticket
 };
 
 struct symamd_graph_s {

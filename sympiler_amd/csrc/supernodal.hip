@@ -55,6 +55,7 @@ struct SnArgs {
   // transposed solve: rows below each supernode (at its temporaries'
   // offsets tb), and the supernodes owning them that it waits for
   const int *rowb, *tdptr, *tdlist;
+  const int *torder;            // supernode of each transposed ticket
   int m;
   int k;                        // right-hand sides: column c at x + c*ldx
   int64_t ldx, tstride;         // its temporaries at T + c*tstride
@@ -375,8 +376,8 @@ __global__ __launch_bounds__(256, TOP_ROWS >= 4 ? 2 : 3) void k_supernodal(SnArg
 // ---------------------------------------------------- transposed solve
 // L^T x = b on the same storage (the semantics of parsy's blockedLTsolve,
 // sparse_blas/parsy/triangularSolve/Triangular_BCSC.cpp:125-186), supernodes
-// from the last to the first; one wave per (supernode, column) ticket,
-// supernode-major, so every wait is for an earlier ticket.  Per supernode:
+// level by level from the root (torder); one wave per (supernode, column)
+// ticket, supernode-major, so every wait is for an earlier ticket.  Per supernode:
 // wait for the supernodes owning its rows below (solved earlier); per top
 // column c (lanes), t = sum over the rows below, ascending, of M[r, c] times
 // x at that row (the rows' x staged in LDS chunk by chunk), x[s+c] -= t;
@@ -489,7 +490,7 @@ __global__ __launch_bounds__(256) void k_supernodal_t(SnArgs a) {
     tt = __shfl(tt, 0);
     if (tt >= a.m * a.k) break;
     const int ti = tt / a.k, colk = tt - ti * a.k;
-    const int i = a.m - 1 - ti;
+    const int i = a.torder[ti];
     double *const x = a.x + colk * a.ldx;
     int *const flags = a.flags + (int64_t)colk * a.ntasks;
     wait_units(a.tdptr, a.tdlist, i, flags, target, a.state, a.timeout);
@@ -608,6 +609,16 @@ int sn_analyze(symamd_handle h, int n, int m, const int *h_sup, const int *h_nro
     tdlist.insert(tdlist.end(), own.begin(), own.end());
     tdptr[i + 1] = (int)tdlist.size();
   }
+  // transposed tickets level by level from the root: a supernode's level is
+  // one below the deepest of the supernodes owning its rows below (all of
+  // higher index), so each waits only for tickets dealt before it, and no
+  // subtree's waiting supernodes hold the waves another chain needs
+  std::vector<int> tlev(m, 0), torder(m > 0 ? m : 1);
+  for (int i = m - 1; i >= 0; --i)
+    for (int e = tdptr[i]; e < tdptr[i + 1]; ++e) tlev[i] = std::max(tlev[i], tlev[tdlist[e]] + 1);
+  for (int i = 0; i < m; ++i) torder[i] = m - 1 - i;
+  std::stable_sort(torder.begin(), torder.begin() + m,
+                   [&](int x, int y) { return tlev[x] < tlev[y]; });
   std::vector<int> dptr(ntasks + 1, 0), dlist;
   for (int j = 0; j < ntasks; ++j) {
     std::sort(deps[j].begin(), deps[j].end());
@@ -639,6 +650,7 @@ int sn_analyze(symamd_handle h, int n, int m, const int *h_sup, const int *h_nro
   up(&q->d_rowb, rowb.data(), tb[m]);
   up(&q->d_tdptr, tdptr.data(), m + 1);
   up(&q->d_tdlist, tdlist.data(), tdlist.size());
+  up(&q->d_torder, torder.data(), m);
   if (rc == SYMAMD_OK &&
       (hipMalloc(&q->d_T, sizeof(double) * (q->tcount > 0 ? q->tcount : 1)) != hipSuccess ||
        hipMalloc(&q->d_flags, sizeof(int) * (ntasks > 0 ? ntasks : 1)) != hipSuccess ||
@@ -700,6 +712,7 @@ int sn_solve(symamd_handle h, symamd_snplan_s *q, const double *d_Lx, double *d_
   a.rowb = q->d_rowb;
   a.tdptr = q->d_tdptr;
   a.tdlist = q->d_tdlist;
+  a.torder = q->d_torder;
   a.m = q->m;
   a.flags = q->d_flags;
   a.state = q->d_state;
@@ -743,7 +756,8 @@ void sn_destroy(symamd_snplan_s *q) {
   if (!q) return;
   void *ptrs[] = {q->d_sup,   q->d_nrows, q->d_Lp,    q->d_tb,     q->d_cptr,
                   q->d_cidx,  q->d_dptr,  q->d_dlist, q->d_T,      q->d_flags,
-                  q->d_state, q->d_task,  q->d_rowb,  q->d_tdptr, q->d_tdlist};
+                  q->d_state, q->d_task,  q->d_rowb,  q->d_tdptr, q->d_tdlist,
+                  q->d_torder};
   for (void *x : ptrs)
     if (x) (void)hipFree(x);
   delete q;
