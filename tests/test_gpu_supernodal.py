@@ -35,6 +35,31 @@ def test_supernodal_bitwise(dev, case):
         assert bitwise_equal(x.cpu().numpy(), want)
 
 
+@pytest.mark.parametrize("k", [1, 3])
+def test_supernodal_wide_tops_bitwise(dev, k):
+    """Tops from a few columns to 823 wide (five past 512, three in (256, 512]),
+    up to 682 rows below: the gather in two 512-row halves, the top's update
+    in several passes, four column blocks per product batch (k = 1 kernel) or
+    two (block kernel), and the transposed solve with 16 column tiles per
+    lane; forward and transposed, every column bit-identical to the oracle."""
+    n = 5000
+    sup, nrows, Lp, Li, Lx = W.bcsc(n, 1000, 700, 4000, 9)
+    widths = np.diff(sup)
+    assert widths.max() > 512 and ((widths > 256) & (widths <= 512)).any()
+    print(f"[sn] wide tops k={k} supernodes={len(sup) - 1} widest={widths.max()}", flush=True)
+    B = W.uniform(21, 0, k * n).reshape(k, n)
+    plan = sa.SupernodalPlan(sup, nrows, Lp, Li)
+    dLx = to_dev(Lx)
+    for solve, ref in ((plan.solve, oracle.sptrsv_csc_block),
+                       (plan.solve_transposed, oracle.sptrsv_csc_block_t)):
+        X = to_dev(B.copy())
+        solve(dLx, X if k > 1 else X[0])
+        plan.check()
+        got = X.cpu().numpy()
+        for c in range(k):
+            assert bitwise_equal(got[c], ref(sup, nrows, Lp, Li, Lx, B[c])), (solve, c)
+
+
 @pytest.mark.parametrize("k", [3, 32])
 def test_supernodal_block_columns_bitwise(dev, k):
     n = 3000
